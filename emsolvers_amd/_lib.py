@@ -1,0 +1,96 @@
+"""ctypes loader of libems.so (the HIP kernels behind the C ABI of include/ems/ems.h).
+
+There is no CPU fallback: if the library is missing or no GPU is visible the
+calls fail loudly.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libems.so")
+
+EMS_OK = 0
+STATUS_NAMES = {
+    0: "EMS_OK", 1: "EMS_ERR_INVALID_ARG", 2: "EMS_ERR_CFL", 3: "EMS_ERR_IO",
+    4: "EMS_ERR_HIP", 5: "EMS_ERR_NOT_SUPPORTED", 6: "EMS_ERR_NO_MEMORY", 7: "EMS_ERR_STATE",
+}
+
+# Every exported C symbol with its (restype, argtypes).
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+
+class FdtdDesc(ctypes.Structure):
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("h", _D), ("dt", _D), ("eps", _D),
+                ("mu", _D), ("tf", _I), ("precision", _I)]
+
+
+SIGNATURES = {
+    "ems_last_error": (ctypes.c_char_p, []),
+    "ems_version": (ctypes.c_char_p, []),
+    "ems_device_count": (_I, []),
+    # FDTD
+    "ems_fdtd_create": (_I, [ctypes.POINTER(FdtdDesc), _PP]),
+    "ems_fdtd_destroy": (None, [_P]),
+    "ems_fdtd_add_source": (_I, [_P, _I, _I, _I, _I]),
+    "ems_fdtd_add_observation": (_I, [_P, _IP, _IP]),
+    "ems_fdtd_set_material": (_I, [_P, _DP, _DP]),
+    "ems_fdtd_set_cpml": (_I, [_P, _I, _D, _D, _D, _D]),
+    "ems_fdtd_set_wavelet": (_I, [_P, _I, _D, _D, _D]),
+    "ems_fdtd_set_kernel": (_I, [_P, _I]),
+    "ems_fdtd_run": (_I, [_P, _I]),
+    "ems_fdtd_run_timed": (_I, [_P, _I, _DP, _DP]),
+    "ems_fdtd_trace_shape": (_I, [_P, _IP, _IP]),
+    "ems_fdtd_get_traces": (_I, [_P, _DP, _SZ]),
+    "ems_fdtd_field_size": (_I, [_P, _I, ctypes.POINTER(_SZ)]),
+    "ems_fdtd_get_field": (_I, [_P, _I, _DP, _SZ]),
+    "ems_fdtd_set_field": (_I, [_P, _I, _DP, _SZ]),
+    "ems_fdtd_get_time": (_I, [_P, _DP]),
+    "ems_fdtd_save_results": (_I, [_P, _I, _D, ctypes.c_char_p]),
+    "ems_fdtd_device_bytes": (_I, [_P, ctypes.POINTER(_SZ)]),
+    "ems_fdtd_check_params": (_I, [_DP, _IP, _D, _D, _D, _D]),
+}
+
+_lib = None
+
+
+class EmsError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def load():
+    """Load libems.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libems.so is not built (run python -m emsolvers_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != EMS_OK:
+        msg = load().ems_last_error().decode(errors="replace")
+        raise EmsError(status, msg)
+    return status
+
+
+def dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+def iptr(a):
+    return a.ctypes.data_as(_IP)

@@ -1,0 +1,729 @@
+// Yee FDTD solver: C ABI over the gfx950 kernels of fdtd_kernels.hpp.
+//
+// Drop-in for the reference class FemSimulation of src/fdtd/
+// (fdtd_simulation.hpp:9-80): same constructor arguments, sources,
+// observation lists, run() semantics and trace layout.  The whole domain is
+// device resident; traces stay on the device until they are read.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "fdtd_kernels.hpp"
+
+using namespace ems;
+using namespace ems::fdtd;
+
+namespace {
+
+constexpr long kPitchAlign = 16;  // elements: 64 B rows (FP32), 128 B rows (FP64)
+constexpr long kFrontPad = 64;    // keeps (I-1) reads of row 0 inside the allocation
+
+struct Src {
+    int comp, i, j, k;
+    long u;         // unified index
+    double eps;     // permittivity at the source cell
+    double scale;   // 1 + sigma*dt/(2 eps) at the source cell
+};
+
+using ObsLists = std::array<std::vector<std::array<int, 3>>, 3>;
+
+}  // namespace
+
+struct ems_fdtd {
+    ems_fdtd_desc d{};
+    GridView g{};
+    size_t esz = 8;
+    size_t nalloc = 0;  // elements per unified array
+    DevBuf f[6];
+    DevBuf ca, cb;
+    int mat_mode = 0;  // 0 uniform, 1 Cb, 2 Ca + Cb
+    std::vector<double> eps_r, sigma;  // material grid copies (empty = default)
+    // CPML
+    int npml = 0;
+    double pml_order = 3.0, pml_r0 = 1e-6, pml_kmax = 1.0, pml_amax = 0.0;
+    DevBuf prof;    // 18 profile tables
+    DevBuf psi[12];
+    // sources / receivers
+    std::vector<Src> src;
+    std::vector<ObsLists> obs;
+    int wavelet = EMS_WAVELET_HERTZ;
+    double f0 = 1.0e9, t0 = -1.0, amp = 1.0;
+    // traces (host, reference semantics) and device staging
+    std::vector<double> traces;  // [nobs][nslot][3]
+    int nslot = 0;
+    double t_now = 0.0;
+    int kernel = EMS_KERNEL_AUTO;
+    hipStream_t stream = nullptr;
+    DevBuf d_src_comp, d_src_u, d_src_val, d_pt_comp, d_pt_u, d_rec;
+    bool dirty_pml = true;
+};
+
+namespace {
+
+GridView make_grid(int nx, int ny, int nz)
+{
+    GridView g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.pitch = ((long)nx + 2 + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
+    g.slab = g.pitch * (ny + 2);
+    g.off = kFrontPad;
+    return g;
+}
+
+inline long uidx(const GridView &g, int I, int J, int K) { return g.off + I + g.pitch * J + g.slab * K; }
+
+// Reference-layout dims and unified shift of component w.
+void ref_dims(const GridView &g, int w, int dims[3], int shift[3])
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int D[6][3] = {{nx + 1, ny + 2, nz + 2}, {nx + 2, ny + 1, nz + 2}, {nx + 2, ny + 2, nz + 1},
+                         {nx, ny + 1, nz + 1},     {nx + 1, ny, nz + 1},     {nx + 1, ny + 1, nz}};
+    const int S[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int a = 0; a < 3; ++a) {
+        dims[a] = D[w][a];
+        shift[a] = S[w][a];
+    }
+}
+
+size_t ref_size(const GridView &g, int w)
+{
+    int d[3], s[3];
+    ref_dims(g, w, d, s);
+    return (size_t)d[0] * d[1] * d[2];
+}
+
+long ref_to_unified(const GridView &g, int comp, int i, int j, int k)
+{
+    int d[3], s[3];
+    ref_dims(g, comp, d, s);
+    return uidx(g, i + s[0], j + s[1], k + s[2]);
+}
+
+bool ref_in_range(const GridView &g, int comp, int i, int j, int k)
+{
+    int d[3], s[3];
+    ref_dims(g, comp, d, s);
+    return i >= 0 && j >= 0 && k >= 0 && i < d[0] && j < d[1] && k < d[2];
+}
+
+// src/fdtd/fdtd_simulation.cpp:24-36 with the same operation order.
+double hertz(double t, double domain_size, double f0)
+{
+    return -4.0 * M_PI * M_PI * f0 * f0 * (t - 1.0 / f0) *
+           std::exp(-2.0 * M_PI * M_PI * f0 * f0 * (t - 1.0 / f0) * (t - 1.0 / f0)) / domain_size /
+           domain_size / domain_size * 0.001;
+}
+
+double wavelet_value(const ems_fdtd *s, double t)
+{
+    const double t0 = s->t0 < 0 ? std::sqrt(2.0) / s->f0 : s->t0;
+    if (s->wavelet == EMS_WAVELET_RICKER) {
+        double a = M_PI * M_PI * s->f0 * s->f0 * (t - t0) * (t - t0);
+        return s->amp * ((1.0 - 2.0 * a) * std::exp(-a));
+    }
+    if (s->wavelet == EMS_WAVELET_GAUSS) {
+        double a = M_PI * s->f0 * (t - t0);
+        return s->amp * std::exp(-a * a);
+    }
+    return hertz(t, s->d.h, s->f0);
+}
+
+// CPML grading at physical coordinate x on an axis of n cells.
+void cpml_profile(const ems_fdtd *s, int n, double x, double &kinv, double &b, double &a)
+{
+    const int P = s->npml;
+    const double h = s->d.h, L = P * h;
+    double d = 0.0;
+    if (x < L) d = L - x;
+    if (x > n * h - L) d = std::max(d, x - (n * h - L));
+    if (P <= 0 || d <= 0.0) {
+        kinv = 1.0;
+        b = 1.0;
+        a = 0.0;
+        return;
+    }
+    const double m = s->pml_order;
+    const double eta = std::sqrt(s->d.mu / s->d.eps);
+    const double smax = -(m + 1.0) * std::log(s->pml_r0) / (2.0 * eta * L);
+    const double r = d / L;
+    const double gr = std::pow(r, m);
+    const double sig = smax * gr;
+    const double kap = 1.0 + (s->pml_kmax - 1.0) * gr;
+    const double alp = s->pml_amax * (1.0 - r);
+    kinv = 1.0 / kap;
+    b = std::exp(-(sig / kap + alp) * s->d.dt / s->d.eps);
+    a = (sig > 0.0) ? sig / (sig * kap + kap * kap * alp) * (b - 1.0) : 0.0;
+}
+
+template <typename T> ems_status upload_as(DevBuf &dst, const std::vector<double> &v)
+{
+    std::vector<T> tmp(v.begin(), v.end());
+    EMS_TRY(dst.reserve(tmp.size() * sizeof(T)));
+    EMS_HIP(hipMemcpy(dst.p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+ems_status setup_pml(ems_fdtd *s)
+{
+    if (!s->dirty_pml) return EMS_OK;
+    s->dirty_pml = false;
+    for (auto &p : s->psi) p.release();
+    s->prof.release();
+    if (s->npml <= 0) return EMS_OK;
+    const GridView &g = s->g;
+    const int nn[3] = {g.nx, g.ny, g.nz};
+    const int P = s->npml, W = 2 * P + 1;
+    // 18 tables (kh bh ah ke be ae) x 3 axes, each of length maxn+2
+    const int len = std::max({g.nx, g.ny, g.nz}) + 2;
+    std::vector<double> tab((size_t)18 * len, 0.0);
+    for (int ax = 0; ax < 3; ++ax)
+        for (int I = 0; I < nn[ax] + 2; ++I) {
+            double kh, bh, ah, ke, be, ae;
+            cpml_profile(s, nn[ax], I * s->d.h, kh, bh, ah);
+            cpml_profile(s, nn[ax], (I - 0.5) * s->d.h, ke, be, ae);
+            double v[6] = {kh, bh, ah, ke, be, ae};
+            for (int t = 0; t < 6; ++t) tab[(size_t)(t * 3 + ax) * len + I] = v[t];
+        }
+    if (s->d.precision == EMS_F32) EMS_TRY(upload_as<float>(s->prof, tab));
+    else EMS_TRY(upload_as<double>(s->prof, tab));
+    const size_t nslab[3] = {(size_t)W * (g.ny + 2) * (g.nz + 2), (size_t)(g.nx + 2) * W * (g.nz + 2),
+                             (size_t)(g.nx + 2) * (g.ny + 2) * W};
+    for (int ax = 0; ax < 3; ++ax)
+        for (int q = 0; q < 4; ++q) {
+            DevBuf &b = s->psi[ax * 4 + q];
+            EMS_TRY(b.reserve(nslab[ax] * s->esz));
+            EMS_HIP(hipMemset(b.p, 0, nslab[ax] * s->esz));
+        }
+    return EMS_OK;
+}
+
+template <typename T> PmlView<T> pml_view(ems_fdtd *s)
+{
+    PmlView<T> v{};
+    v.P = s->npml > 0 ? s->npml : 0;
+    v.W = 2 * v.P + 1;
+    if (s->npml <= 0) return v;
+    const int len = std::max({s->g.nx, s->g.ny, s->g.nz}) + 2;
+    const T *t = s->prof.as<T>();
+    for (int ax = 0; ax < 3; ++ax) {
+        v.kh[ax] = t + (size_t)(0 * 3 + ax) * len;
+        v.bh[ax] = t + (size_t)(1 * 3 + ax) * len;
+        v.ah[ax] = t + (size_t)(2 * 3 + ax) * len;
+        v.ke[ax] = t + (size_t)(3 * 3 + ax) * len;
+        v.be[ax] = t + (size_t)(4 * 3 + ax) * len;
+        v.ae[ax] = t + (size_t)(5 * 3 + ax) * len;
+        v.psi_h[ax][0] = s->psi[ax * 4 + 0].as<T>();
+        v.psi_h[ax][1] = s->psi[ax * 4 + 1].as<T>();
+        v.psi_e[ax][0] = s->psi[ax * 4 + 2].as<T>();
+        v.psi_e[ax][1] = s->psi[ax * 4 + 3].as<T>();
+    }
+    return v;
+}
+
+template <typename T> Fields<T> fields(ems_fdtd *s)
+{
+    return Fields<T>{s->f[0].as<T>(), s->f[1].as<T>(), s->f[2].as<T>(),
+                     s->f[3].as<T>(), s->f[4].as<T>(), s->f[5].as<T>()};
+}
+
+template <typename T, int MAT, bool PML>
+void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    const GridView &g = s->g;
+    dim3 blk(64, 4, 1);
+    dim3 grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    Fields<T> f = fields<T>(s);
+    k_h_split<T, PML><<<grd, blk, 0, s->stream>>>(g, ch, f, pm);
+    k_e_split<T, MAT, PML><<<grd, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm);
+}
+
+template <typename T> void launch_step(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    const bool pml = s->npml > 0;
+    switch (s->mat_mode * 2 + (pml ? 1 : 0)) {
+    case 0: launch_split<T, 0, false>(s, ch, ce, pm); break;
+    case 1: launch_split<T, 0, true>(s, ch, ce, pm); break;
+    case 2: launch_split<T, 1, false>(s, ch, ce, pm); break;
+    case 3: launch_split<T, 1, true>(s, ch, ce, pm); break;
+    case 4: launch_split<T, 2, false>(s, ch, ce, pm); break;
+    default: launch_split<T, 2, true>(s, ch, ce, pm); break;
+    }
+}
+
+// Runs num_steps steps; when ev != nullptr, records HIP events around the
+// Yee kernels of every step (ev[2*step], ev[2*step+1]).
+template <typename T>
+ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    const int tf = s->d.tf;
+    // trace slots: saved_electric_field_ resize semantics (old values kept)
+    const int nslot_new = num_steps / tf + 1;
+    const int nobs = (int)s->obs.size();
+    if (nslot_new > s->nslot) {
+        std::vector<double> t((size_t)nobs * nslot_new * 3, 0.0);
+        for (int o = 0; o < nobs; ++o)
+            for (int q = 0; q < s->nslot; ++q)
+                for (int c = 0; c < 3; ++c)
+                    t[((size_t)o * nslot_new + q) * 3 + c] = s->traces[((size_t)o * s->nslot + q) * 3 + c];
+        s->traces.swap(t);
+        s->nslot = nslot_new;
+    }
+    if (num_steps <= 0) return EMS_OK;
+    EMS_TRY(setup_pml(s));
+
+    // sources sorted by component (x list, then y, then z; :222-239)
+    std::vector<Src> src = s->src;
+    std::stable_sort(src.begin(), src.end(), [](const Src &a, const Src &b) { return a.comp < b.comp; });
+    const int nsrc = (int)src.size();
+    std::vector<int> sc(nsrc);
+    std::vector<long> su(nsrc);
+    for (int q = 0; q < nsrc; ++q) {
+        sc[q] = src[q].comp;
+        su[q] = src[q].u;
+    }
+    // per-step injected values (s(t)*dt)/eps [/scale], t accumulated as current_time_
+    std::vector<double> sval((size_t)num_steps * std::max(nsrc, 1), 0.0);
+    double t = s->t_now;
+    for (int st = 0; st < num_steps; ++st) {
+        double w = wavelet_value(s, t);
+        for (int q = 0; q < nsrc; ++q) {
+            double v = w * s->d.dt / src[q].eps;
+            if (src[q].scale != 1.0) v = v / src[q].scale;
+            sval[(size_t)st * nsrc + q] = v;
+        }
+        t += s->d.dt;
+    }
+    // receiver points
+    std::vector<int> pc;
+    std::vector<long> pu;
+    for (auto &o : s->obs)
+        for (int c = 0; c < 3; ++c)
+            for (auto &p : o[c]) {
+                pc.push_back(c);
+                pu.push_back(ref_to_unified(s->g, c, p[0], p[1], p[2]));
+            }
+    const int npts = (int)pc.size();
+    const int nrec = num_steps / tf;
+    EMS_TRY(s->d_src_comp.reserve(sizeof(int) * std::max(nsrc, 1)));
+    EMS_TRY(s->d_src_u.reserve(sizeof(long) * std::max(nsrc, 1)));
+    EMS_TRY(s->d_src_val.reserve(sizeof(double) * sval.size()));
+    EMS_TRY(s->d_pt_comp.reserve(sizeof(int) * std::max(npts, 1)));
+    EMS_TRY(s->d_pt_u.reserve(sizeof(long) * std::max(npts, 1)));
+    EMS_TRY(s->d_rec.reserve(sizeof(double) * std::max((size_t)nrec * npts, (size_t)1)));
+    if (nsrc) {
+        EMS_HIP(hipMemcpy(s->d_src_comp.p, sc.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_src_u.p, su.data(), sizeof(long) * nsrc, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_src_val.p, sval.data(), sizeof(double) * sval.size(), hipMemcpyHostToDevice));
+    }
+    if (npts) {
+        EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
+    }
+
+    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    const PmlView<T> pm = pml_view<T>(s);
+    Fields<T> f = fields<T>(s);
+
+    // timed mode: one event pair around the Yee kernels of every step plus
+    // one pair around the whole sequence; read back after the last step
+    const bool timed = total_ms != nullptr;
+    std::vector<hipEvent_t> ev;
+    if (timed) {
+        ev.resize(2 * (size_t)num_steps + 2, nullptr);
+        for (auto &e : ev) EMS_HIP(hipEventCreate(&e));
+        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps], s->stream));
+    }
+    for (int step = 1; step <= num_steps; ++step) {
+        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1)], s->stream));
+        launch_step<T>(s, ch, ce, pm);
+        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1) + 1], s->stream));
+        EMS_HIP(hipGetLastError());
+        double *rec = nullptr;
+        if (npts && step % tf == 0) rec = s->d_rec.as<double>() + (size_t)(step / tf - 1) * npts;
+        if (nsrc || rec) {
+            k_post<T><<<1, 256, 0, s->stream>>>(f, nsrc, s->d_src_comp.as<int>(), s->d_src_u.as<long>(),
+                                                s->d_src_val.as<double>() + (size_t)(step - 1) * nsrc, npts,
+                                                s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(), rec);
+            EMS_HIP(hipGetLastError());
+        }
+    }
+    if (timed) {
+        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps + 1], s->stream));
+        EMS_HIP(hipEventSynchronize(ev[2 * (size_t)num_steps + 1]));
+        double ksum = 0.0;
+        for (int st = 0; st < num_steps; ++st) {
+            float ms = 0.f;
+            EMS_HIP(hipEventElapsedTime(&ms, ev[2 * (size_t)st], ev[2 * (size_t)st + 1]));
+            ksum += ms;
+        }
+        float ms = 0.f;
+        EMS_HIP(hipEventElapsedTime(&ms, ev[2 * (size_t)num_steps], ev[2 * (size_t)num_steps + 1]));
+        *total_ms = ms;
+        *kernel_ms = ksum;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    s->t_now = t;
+
+    // combine raw samples into the traces exactly as the reference:
+    // saved += p for each point in order, then /= count
+    if (nrec > 0 && nobs > 0) {
+        std::vector<double> raw((size_t)nrec * npts);
+        if (npts)
+            EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrec; ++r) {
+            const int slot = r + 1;
+            int p = 0;
+            for (int o = 0; o < nobs; ++o)
+                for (int c = 0; c < 3; ++c) {
+                    double &dst = s->traces[((size_t)o * s->nslot + slot) * 3 + c];
+                    const size_t cnt = s->obs[o][c].size();
+                    for (size_t q = 0; q < cnt; ++q, ++p) dst += raw[(size_t)r * npts + p];
+                    dst /= (double)cnt;
+                }
+        }
+    }
+    return EMS_OK;
+}
+
+ems_status set_material_impl(ems_fdtd *s)
+{
+    const GridView &g = s->g;
+    const bool has_eps = !s->eps_r.empty(), has_sig = !s->sigma.empty();
+    // source cells see the local material
+    for (auto &q : s->src) {
+        size_t m = (size_t)q.i + (size_t)(g.nx + 1) * (q.j + (size_t)(g.ny + 1) * q.k);
+        bool in = q.i <= g.nx && q.j <= g.ny && q.k <= g.nz;
+        double er = (has_eps && in) ? s->eps_r[m] : 1.0;
+        double sg = (has_sig && in) ? s->sigma[m] : 0.0;
+        q.eps = s->d.eps * er;
+        q.scale = 1.0 + sg * s->d.dt / (2.0 * q.eps);
+    }
+    if (!has_eps && !has_sig) {
+        s->mat_mode = 0;
+        s->ca.release();
+        s->cb.release();
+        return EMS_OK;
+    }
+    std::vector<double> ca(s->nalloc, 1.0), cb(s->nalloc, 0.0);
+    for (int K = 0; K <= g.nz; ++K)
+        for (int J = 0; J <= g.ny; ++J)
+            for (int I = 0; I <= g.nx; ++I) {
+                size_t m = (size_t)I + (size_t)(g.nx + 1) * (J + (size_t)(g.ny + 1) * K);
+                double er = has_eps ? s->eps_r[m] : 1.0;
+                double sg = has_sig ? s->sigma[m] : 0.0;
+                double e = s->d.eps * er;
+                double hs = sg * s->d.dt / (2.0 * e);
+                long u = uidx(g, I, J, K);
+                ca[u] = (1.0 - hs) / (1.0 + hs);
+                cb[u] = s->d.dt / e / s->d.h / (1.0 + hs);
+            }
+    s->mat_mode = has_sig ? 2 : 1;
+    if (s->d.precision == EMS_F32) {
+        EMS_TRY(upload_as<float>(s->cb, cb));
+        if (has_sig) EMS_TRY(upload_as<float>(s->ca, ca));
+    } else {
+        EMS_TRY(upload_as<double>(s->cb, cb));
+        if (has_sig) EMS_TRY(upload_as<double>(s->ca, ca));
+    }
+    if (!has_sig) s->ca.release();
+    return EMS_OK;
+}
+
+template <typename T> ems_status get_field_t(ems_fdtd *s, int w, double *out)
+{
+    std::vector<T> host(s->nalloc);
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    EMS_HIP(hipMemcpy(host.data(), s->f[w].p, s->nalloc * sizeof(T), hipMemcpyDeviceToHost));
+    int d[3], sh[3];
+    ref_dims(s->g, w, d, sh);
+    size_t n = 0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) out[n++] = (double)host[uidx(s->g, i + sh[0], j + sh[1], k + sh[2])];
+    return EMS_OK;
+}
+
+template <typename T> ems_status set_field_t(ems_fdtd *s, int w, const double *in)
+{
+    std::vector<T> host(s->nalloc);
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    EMS_HIP(hipMemcpy(host.data(), s->f[w].p, s->nalloc * sizeof(T), hipMemcpyDeviceToHost));
+    int d[3], sh[3];
+    ref_dims(s->g, w, d, sh);
+    size_t n = 0;
+    for (int k = 0; k < d[2]; ++k)
+        for (int j = 0; j < d[1]; ++j)
+            for (int i = 0; i < d[0]; ++i) host[uidx(s->g, i + sh[0], j + sh[1], k + sh[2])] = (T)in[n++];
+    EMS_HIP(hipMemcpy(s->f[w].p, host.data(), s->nalloc * sizeof(T), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// C ABI
+// =========================================================================
+extern "C" {
+
+ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out)
+{
+    clear_error();
+    if (!desc || !out) return invalid("ems_fdtd_create: null argument");
+    *out = nullptr;
+    if (desc->nx <= 0 || desc->ny <= 0 || desc->nz <= 0) return invalid("grid sizes must be positive");
+    if (desc->tf <= 0) return invalid("time_frequency must be >= 1");
+    if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
+        return invalid("h, dt, eps and mu must be positive");
+    if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
+    if (ems_device_count() <= 0) {
+        set_error("no HIP device available");
+        return EMS_ERR_HIP;
+    }
+    ems_fdtd *s = new ems_fdtd();
+    s->d = *desc;
+    s->esz = desc->precision == EMS_F32 ? 4 : 8;
+    s->g = make_grid(desc->nx, desc->ny, desc->nz);
+    s->nalloc = (size_t)(s->g.off + s->g.slab * (desc->nz + 2) + kFrontPad);
+    for (int w = 0; w < 6; ++w) {
+        ems_status st = s->f[w].reserve(s->nalloc * s->esz);
+        if (st != EMS_OK) {
+            delete s;
+            return st;
+        }
+        hipError_t e = hipMemset(s->f[w].p, 0, s->nalloc * s->esz);
+        if (e != hipSuccess) {
+            delete s;
+            return hip_fail(e, "hipMemset", __FILE__, __LINE__);
+        }
+    }
+    *out = s;
+    return EMS_OK;
+}
+
+void ems_fdtd_destroy(ems_fdtd *s) { delete s; }
+
+ems_status ems_fdtd_add_source(ems_fdtd *s, int comp, int i, int j, int k)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (comp < 0 || comp > 2) return invalid("source component must be 0, 1 or 2");
+    if (!ref_in_range(s->g, comp, i, j, k)) return invalid("source index outside the field array");
+    Src q{comp, i, j, k, ref_to_unified(s->g, comp, i, j, k), s->d.eps, 1.0};
+    s->src.push_back(q);
+    if (!s->eps_r.empty() || !s->sigma.empty()) return set_material_impl(s);
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_add_observation(ems_fdtd *s, const int counts[3], const int *pts)
+{
+    clear_error();
+    if (!s || !counts) return invalid("null argument");
+    ObsLists o;
+    int p = 0;
+    for (int c = 0; c < 3; ++c) {
+        if (counts[c] < 0) return invalid("negative point count");
+        for (int q = 0; q < counts[c]; ++q, ++p) {
+            std::array<int, 3> v{pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+            if (!ref_in_range(s->g, c, v[0], v[1], v[2])) return invalid("observation point outside the field array");
+            o[c].push_back(v);
+        }
+    }
+    s->obs.push_back(o);
+    // a new observation set starts with an empty trace (reference push_back)
+    const int nobs = (int)s->obs.size();
+    std::vector<double> t((size_t)nobs * s->nslot * 3, 0.0);
+    std::copy(s->traces.begin(), s->traces.end(), t.begin());
+    s->traces.swap(t);
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_material(ems_fdtd *s, const double *eps_r, const double *sigma)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    const size_t n = (size_t)(s->g.nx + 1) * (s->g.ny + 1) * (s->g.nz + 1);
+    for (size_t q = 0; eps_r && q < n; ++q)
+        if (!(eps_r[q] > 0)) return invalid("eps_r must be positive");
+    for (size_t q = 0; sigma && q < n; ++q)
+        if (!(sigma[q] >= 0)) return invalid("sigma must be non-negative");
+    if (eps_r) s->eps_r.assign(eps_r, eps_r + n);
+    else s->eps_r.clear();
+    if (sigma) s->sigma.assign(sigma, sigma + n);
+    else s->sigma.clear();
+    return set_material_impl(s);
+}
+
+ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0, double kappa_max,
+                             double alpha_max)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (ncells < 0) return invalid("CPML thickness must be >= 0");
+    const int nmin = std::min({s->g.nx, s->g.ny, s->g.nz});
+    if (ncells > 0 && 2 * ncells + 1 > nmin) return invalid("CPML thicker than half the grid");
+    if (ncells > 0 && (!(r0 > 0 && r0 < 1) || !(order >= 0) || !(kappa_max >= 1) || !(alpha_max >= 0)))
+        return invalid("invalid CPML parameters");
+    s->npml = ncells;
+    s->pml_order = order;
+    s->pml_r0 = r0;
+    s->pml_kmax = kappa_max;
+    s->pml_amax = alpha_max;
+    s->dirty_pml = true;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, double amp)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (kind < 0 || kind > 2) return invalid("unknown wavelet");
+    if (!(f0 > 0)) return invalid("f0 must be positive");
+    s->wavelet = kind;
+    s->f0 = f0;
+    s->t0 = t0;
+    s->amp = amp;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_FUSED) return invalid("unknown kernel variant");
+    s->kernel = variant;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_run(ems_fdtd *s, int num_steps)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->d.precision == EMS_F32 ? run_impl<float>(s, num_steps, nullptr, nullptr)
+                                     : run_impl<double>(s, num_steps, nullptr, nullptr);
+}
+
+ems_status ems_fdtd_run_timed(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    clear_error();
+    if (!s || !total_ms || !kernel_ms) return invalid("null argument");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->d.precision == EMS_F32 ? run_impl<float>(s, num_steps, total_ms, kernel_ms)
+                                     : run_impl<double>(s, num_steps, total_ms, kernel_ms);
+}
+
+ems_status ems_fdtd_trace_shape(const ems_fdtd *s, int *nobs, int *nslot)
+{
+    clear_error();
+    if (!s || !nobs || !nslot) return invalid("null argument");
+    *nobs = (int)s->obs.size();
+    *nslot = s->nslot;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_get_traces(ems_fdtd *s, double *out, size_t count)
+{
+    clear_error();
+    if (!s || (!out && count)) return invalid("null argument");
+    if (count != s->traces.size()) return invalid("trace buffer has the wrong size");
+    std::copy(s->traces.begin(), s->traces.end(), out);
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_field_size(const ems_fdtd *s, int which, size_t *n)
+{
+    clear_error();
+    if (!s || !n) return invalid("null argument");
+    if (which < 0 || which > 5) return invalid("field selector must be 0..5");
+    *n = ref_size(s->g, which);
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_get_field(ems_fdtd *s, int which, double *out, size_t count)
+{
+    clear_error();
+    if (!s || !out) return invalid("null argument");
+    if (which < 0 || which > 5) return invalid("field selector must be 0..5");
+    if (count != ref_size(s->g, which)) return invalid("field buffer has the wrong size");
+    return s->d.precision == EMS_F32 ? get_field_t<float>(s, which, out) : get_field_t<double>(s, which, out);
+}
+
+ems_status ems_fdtd_set_field(ems_fdtd *s, int which, const double *in, size_t count)
+{
+    clear_error();
+    if (!s || !in) return invalid("null argument");
+    if (which < 0 || which > 5) return invalid("field selector must be 0..5");
+    if (count != ref_size(s->g, which)) return invalid("field buffer has the wrong size");
+    return s->d.precision == EMS_F32 ? set_field_t<float>(s, which, in) : set_field_t<double>(s, which, in);
+}
+
+ems_status ems_fdtd_get_time(const ems_fdtd *s, double *t)
+{
+    clear_error();
+    if (!s || !t) return invalid("null argument");
+    *t = s->t_now;
+    return EMS_OK;
+}
+
+// src/fdtd/fdtd_simulation.cpp:308-331 (file name, header and default
+// 6-significant-digit stream formatting)
+ems_status ems_fdtd_save_results(ems_fdtd *s, int num_steps, double c, const char *dir)
+{
+    (void)c;
+    clear_error();
+    if (!s) return invalid("null handle");
+    const int nobs = (int)s->obs.size();
+    for (int i = 0; i < nobs; ++i) {
+        std::string name = "results_" + std::to_string(i) + "_" + std::to_string(s->d.dt) + "_" +
+                           std::to_string(s->d.h) + "_" + std::to_string(num_steps) + ".csv";
+        if (dir && dir[0]) name = std::string(dir) + "/" + name;
+        std::ofstream file(name);
+        if (!file.is_open()) {
+            set_error("Error: Could not open file " + name);
+            return EMS_ERR_IO;
+        }
+        file << "time,Ex,Ey,Ez" << std::endl;
+        for (int j = 0; j < s->nslot; ++j) {
+            const double *v = &s->traces[((size_t)i * s->nslot + j) * 3];
+            file << j * s->d.tf * s->d.dt << "," << v[0] << "," << v[1] << "," << v[2] << std::endl;
+        }
+    }
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_device_bytes(const ems_fdtd *s, size_t *bytes)
+{
+    clear_error();
+    if (!s || !bytes) return invalid("null argument");
+    size_t b = 0;
+    for (auto &f : s->f) b += f.bytes;
+    b += s->ca.bytes + s->cb.bytes + s->prof.bytes;
+    for (auto &p : s->psi) b += p.bytes;
+    *bytes = b;
+    return EMS_OK;
+}
+
+int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], double duration,
+                          double time_step, double domain_size, double c)
+{
+    if (domain_sizes[0] <= 0 || domain_sizes[1] <= 0 || domain_sizes[2] <= 0) return 0;
+    if (grid_num[0] <= 0 || grid_num[1] <= 0 || grid_num[2] <= 0) return 0;
+    if (duration <= 0) return 0;
+    if (time_step <= 0) return 0;
+    if (std::round(domain_sizes[0] / grid_num[0]) != std::round(domain_sizes[1] / grid_num[1]) ||
+        std::round(domain_sizes[0] / grid_num[0]) != std::round(domain_sizes[2] / grid_num[2]))
+        return 0;
+    if (c * time_step >= domain_size) return 0;
+    return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""Python mirror of the reference FDTD class over the C ABI.
+
+``FemSimulation`` keeps the names, argument meanings and error behaviour of
+the reference class ``FemSimulation`` in src/fdtd/fdtd_simulation.hpp:9-80
+(yes, the reference names its FDTD class that way).  All work runs in
+libems.so on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+F64, F32 = 0, 1
+WAVELET_HERTZ, WAVELET_RICKER, WAVELET_GAUSS = 0, 1, 2
+KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+FIELD_NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+# Reference physical constants (src/fdtd/main.cpp:24-27)
+EPS0 = 8.854187817e-12
+MU0 = 1.25663706212e-6
+C0 = 1.0 / (EPS0 * MU0) ** 0.5
+
+
+def _prec(p):
+    if p in (F64, "f64", "fp64", "float64", np.float64):
+        return F64
+    if p in (F32, "f32", "fp32", "float32", np.float32):
+        return F32
+    raise ValueError("precision must be f64 or f32")
+
+
+class FemSimulation:
+    """Yee FDTD on one GPU (reference: src/fdtd/fdtd_simulation.hpp:13-28)."""
+
+    def __init__(self, grid_size, domain_size, time_step, permittivity, permeability,
+                 time_frequency, precision="f64"):
+        L = _lib.load()
+        nx, ny, nz = (int(g) for g in grid_size)
+        self.grid = (nx, ny, nz)
+        self.domain_size = float(domain_size)
+        self.time_step = float(time_step)
+        self.time_frequency = int(time_frequency)
+        self.precision = _prec(precision)
+        d = _lib.FdtdDesc(nx, ny, nz, float(domain_size), float(time_step), float(permittivity),
+                          float(permeability), int(time_frequency), self.precision)
+        h = ctypes.c_void_p()
+        _lib.check(L.ems_fdtd_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().ems_fdtd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- reference API -----------------------------------------------------
+    def setSource_x(self, position):
+        self._source(0, position)
+
+    def setSource_y(self, position):
+        self._source(1, position)
+
+    def setSource_z(self, position):
+        self._source(2, position)
+
+    def _source(self, comp, p):
+        _lib.check(_lib.load().ems_fdtd_add_source(self._h, comp, int(p[0]), int(p[1]), int(p[2])))
+
+    def setObservationPoint(self, position):
+        """position: three lists of (i,j,k) points for Ex, Ey and Ez."""
+        counts = np.array([len(l) for l in position], dtype=np.int32)
+        pts = np.array([tuple(p) for l in position for p in l] or [(0, 0, 0)], dtype=np.int32)
+        pts = np.ascontiguousarray(pts.reshape(-1, 3))
+        _lib.check(_lib.load().ems_fdtd_add_observation(self._h, _lib.iptr(counts), _lib.iptr(pts)))
+
+    def run(self, num_steps):
+        _lib.check(_lib.load().ems_fdtd_run(self._h, int(num_steps)))
+
+    def saveResults(self, num_steps, c, directory=""):
+        _lib.check(_lib.load().ems_fdtd_save_results(self._h, int(num_steps), float(c),
+                                                     directory.encode()))
+
+    # --- extensions ----------------------------------------------------------
+    def set_material(self, eps_r=None, sigma=None):
+        """Per-cell eps_r / sigma on the (nx+1)(ny+1)(nz+1) material grid."""
+        nx, ny, nz = self.grid
+        n = (nx + 1) * (ny + 1) * (nz + 1)
+        er = None if eps_r is None else np.ascontiguousarray(eps_r, dtype=np.float64).reshape(n)
+        sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64).reshape(n)
+        _lib.check(_lib.load().ems_fdtd_set_material(
+            self._h, None if er is None else _lib.dptr(er), None if sg is None else _lib.dptr(sg)))
+
+    def set_cpml(self, ncells, order=3.0, r0=1e-6, kappa_max=1.0, alpha_max=0.0):
+        _lib.check(_lib.load().ems_fdtd_set_cpml(self._h, int(ncells), float(order), float(r0),
+                                                 float(kappa_max), float(alpha_max)))
+
+    def set_wavelet(self, kind, f0, t0=-1.0, amp=1.0):
+        _lib.check(_lib.load().ems_fdtd_set_wavelet(self._h, int(kind), float(f0), float(t0),
+                                                    float(amp)))
+
+    def set_kernel(self, variant):
+        _lib.check(_lib.load().ems_fdtd_set_kernel(self._h, int(variant)))
+
+    def run_timed(self, num_steps):
+        """Returns (total_ms, kernel_ms) measured with HIP events."""
+        tot, ker = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.load().ems_fdtd_run_timed(self._h, int(num_steps), ctypes.byref(tot),
+                                                  ctypes.byref(ker)))
+        return tot.value, ker.value
+
+    def traces(self):
+        """saved_electric_field_ as an array [nobs][nslot][3]."""
+        nobs, nslot = ctypes.c_int(), ctypes.c_int()
+        L = _lib.load()
+        _lib.check(L.ems_fdtd_trace_shape(self._h, ctypes.byref(nobs), ctypes.byref(nslot)))
+        out = np.zeros((nobs.value, nslot.value, 3), dtype=np.float64)
+        _lib.check(L.ems_fdtd_get_traces(self._h, _lib.dptr(out), out.size))
+        return out
+
+    def field_size(self, which):
+        n = ctypes.c_size_t()
+        _lib.check(_lib.load().ems_fdtd_field_size(self._h, self._which(which), ctypes.byref(n)))
+        return n.value
+
+    def field(self, which):
+        w = self._which(which)
+        out = np.empty(self.field_size(w), dtype=np.float64)
+        _lib.check(_lib.load().ems_fdtd_get_field(self._h, w, _lib.dptr(out), out.size))
+        return out
+
+    def set_field(self, which, values):
+        w = self._which(which)
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _lib.check(_lib.load().ems_fdtd_set_field(self._h, w, _lib.dptr(v), v.size))
+
+    def fields(self):
+        return [self.field(w) for w in range(6)]
+
+    @property
+    def current_time(self):
+        t = ctypes.c_double()
+        _lib.check(_lib.load().ems_fdtd_get_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        _lib.check(_lib.load().ems_fdtd_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    @staticmethod
+    def _which(w):
+        return FIELD_NAMES.index(w) if isinstance(w, str) else int(w)
+
+
+def check_params(domain_sizes, grid_num, duration, time_step, domain_size, c):
+    """Reference check_params (src/fdtd/fdtd_simulation.cpp:333-366)."""
+    ds = np.ascontiguousarray(domain_sizes, dtype=np.float64)
+    gn = np.ascontiguousarray(grid_num, dtype=np.int32)
+    return bool(_lib.load().ems_fdtd_check_params(_lib.dptr(ds), _lib.iptr(gn), float(duration),
+                                                  float(time_step), float(domain_size), float(c)))

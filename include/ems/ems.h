@@ -1,0 +1,148 @@
+/*
+ * EMSolvers for MI355X — C ABI of the explicit time-domain Maxwell hot path.
+ *
+ * One opaque handle per solver instance; the handle owns its device buffers
+ * (no process globals, several handles may live in one process).  Every call
+ * returns an ems_status; ems_last_error() gives the message of the last
+ * failure on the calling thread.  A handle is externally synchronised: do not
+ * call into one handle from two threads at once.
+ *
+ * Field arrays crossing this boundary are FP64 in the reference layouts
+ * (x fastest):
+ *   Ex[k][j][i]  (nx+1)(ny+2)(nz+2)     Hx[k][j][i]  nx(ny+1)(nz+1)
+ *   Ey[k][j][i]  (nx+2)(ny+1)(nz+2)     Hy[k][j][i]  (nx+1)ny(nz+1)
+ *   Ez[k][j][i]  (nx+2)(ny+2)(nz+1)     Hz[k][j][i]  (nx+1)(ny+1)nz
+ * as allocated by src/fdtd/fdtd_simulation.cpp:55-88.
+ */
+#ifndef EMS_EMS_H
+#define EMS_EMS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ems_status;
+enum {
+    EMS_OK = 0,
+    EMS_ERR_INVALID_ARG = 1,
+    EMS_ERR_CFL = 2,
+    EMS_ERR_IO = 3,
+    EMS_ERR_HIP = 4,
+    EMS_ERR_NOT_SUPPORTED = 5,
+    EMS_ERR_NO_MEMORY = 6,
+    EMS_ERR_STATE = 7
+};
+
+enum { EMS_F64 = 0, EMS_F32 = 1 };
+
+/* field selectors for get/set_field */
+enum { EMS_EX = 0, EMS_EY = 1, EMS_EZ = 2, EMS_HX = 3, EMS_HY = 4, EMS_HZ = 5 };
+
+/* source wavelets */
+enum {
+    EMS_WAVELET_HERTZ = 0,  /* src/fdtd/fdtd_simulation.cpp:24-36 (default)          */
+    EMS_WAVELET_RICKER = 1, /* (1 - 2 a) exp(-a), a = (pi f0 (t - t0))^2, times amp  */
+    EMS_WAVELET_GAUSS = 2   /* exp(-(pi f0 (t - t0))^2), times amp                   */
+};
+
+/* kernel variants for ems_fdtd_set_kernel */
+enum { EMS_KERNEL_AUTO = 0, EMS_KERNEL_SPLIT = 1, EMS_KERNEL_FUSED = 2 };
+
+/* Last error message of the calling thread ("" if none). */
+const char *ems_last_error(void);
+
+/* Library version string. */
+const char *ems_version(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int ems_device_count(void);
+
+/* ------------------------------------------------------------------ FDTD */
+
+typedef struct ems_fdtd ems_fdtd;
+
+/* Constructor arguments of FemSimulation (src/fdtd/fdtd_simulation.hpp:13-14)
+ * plus the arithmetic precision of the device fields. */
+typedef struct {
+    int nx, ny, nz;   /* grid_size: cells per axis                         */
+    double h;         /* domain_size: cell edge [m]                        */
+    double dt;        /* time_step [s]                                     */
+    double eps;       /* permittivity [F/m] (background)                   */
+    double mu;        /* permeability [H/m]                                */
+    int tf;           /* time_frequency: record every tf steps (>= 1)      */
+    int precision;    /* EMS_F64 or EMS_F32                                */
+} ems_fdtd_desc;
+
+/* FemSimulation::FemSimulation (src/fdtd/fdtd_simulation.cpp:38-53). */
+ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out);
+void ems_fdtd_destroy(ems_fdtd *s);
+
+/* setSource_x / _y / _z (src/fdtd/fdtd_simulation.cpp:90-106):
+ * comp 0/1/2 selects the E component, (i,j,k) indexes its reference array. */
+ems_status ems_fdtd_add_source(ems_fdtd *s, int comp, int i, int j, int k);
+
+/* setObservationPoint (src/fdtd/fdtd_simulation.cpp:108-112): counts[c]
+ * points for component c, pts = the concatenated [n][3] (i,j,k) triples. */
+ems_status ems_fdtd_add_observation(ems_fdtd *s, const int counts[3], const int *pts);
+
+/* Per-cell relative permittivity and conductivity on the (nx+1)(ny+1)(nz+1)
+ * material grid (x fastest); value (i,j,k) applies to Ex/Ey/Ez[k][j][i].
+ * Either pointer may be NULL (eps_r = 1, sigma = 0).  No reference
+ * counterpart: extends the scalar coefficient of fdtd_simulation.cpp:170-220. */
+ems_status ems_fdtd_set_material(ems_fdtd *s, const double *eps_r, const double *sigma);
+
+/* CPML on all six faces, ncells thick (0 disables).  No reference counterpart. */
+ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0,
+                             double kappa_max, double alpha_max);
+
+/* Source wavelet (default EMS_WAVELET_HERTZ with f0 = 1 GHz, as the
+ * reference).  t0 < 0 selects sqrt(2)/f0. */
+ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, double amp);
+
+/* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT or EMS_KERNEL_FUSED. */
+ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
+
+/* FemSimulation::run (src/fdtd/fdtd_simulation.cpp:242-306): advances
+ * num_steps steps on the device and records the receivers every tf steps
+ * into device-resident traces; synchronous. */
+ems_status ems_fdtd_run(ems_fdtd *s, int num_steps);
+
+/* Runs num_steps steps and measures them with HIP events on the handle's
+ * stream: *total_ms = whole step sequence, *kernel_ms = summed duration of
+ * the Yee update kernel(s) alone.  Same results as ems_fdtd_run. */
+ems_status ems_fdtd_run_timed(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms);
+
+/* Trace shape: nobs observation sets, nslot = num_steps/tf + 1 slots. */
+ems_status ems_fdtd_trace_shape(const ems_fdtd *s, int *nobs, int *nslot);
+
+/* saved_electric_field_ as [nobs][nslot][3] FP64 (count = nobs*nslot*3). */
+ems_status ems_fdtd_get_traces(ems_fdtd *s, double *out, size_t count);
+
+/* Number of entries of field `which` in its reference layout. */
+ems_status ems_fdtd_field_size(const ems_fdtd *s, int which, size_t *n);
+
+/* Copy a field out of / into the device, reference layout, FP64. */
+ems_status ems_fdtd_get_field(ems_fdtd *s, int which, double *out, size_t count);
+ems_status ems_fdtd_set_field(ems_fdtd *s, int which, const double *in, size_t count);
+
+/* current_time_ of the reference (sum of dt over the steps taken). */
+ems_status ems_fdtd_get_time(const ems_fdtd *s, double *t);
+
+/* FemSimulation::saveResults (src/fdtd/fdtd_simulation.cpp:308-331): writes
+ * results_<i>_<dt>_<h>_<num_steps>.csv into dir ("" = cwd). */
+ems_status ems_fdtd_save_results(ems_fdtd *s, int num_steps, double c, const char *dir);
+
+/* Device bytes held by the handle. */
+ems_status ems_fdtd_device_bytes(const ems_fdtd *s, size_t *bytes);
+
+/* check_params (src/fdtd/fdtd_simulation.cpp:333-366): 1 valid, 0 invalid. */
+int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], double duration,
+                          double time_step, double domain_size, double c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMS_EMS_H */

@@ -1,0 +1,83 @@
+"""C ABI checks that need no GPU: libems.so loads, exports every function
+declared in include/ems/*.h, and host-only entry points behave like the
+reference.  No compute call is made here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER_DIR = os.path.join(ROOT, "include", "ems")
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(HEADER_DIR):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(HEADER_DIR, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(ems_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "ems_fdtd_create" in names and "ems_fdtd_run" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from emsolvers_amd import _lib
+    L = _lib.load()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    # the ctypes signature table covers them too
+    assert not [n for n in declared_functions() if n not in _lib.SIGNATURES]
+
+
+def test_version_and_device_count():
+    from emsolvers_amd import _lib
+    L = _lib.load()
+    assert b"gfx950" in L.ems_version()
+    assert L.ems_device_count() >= 0
+
+
+def test_create_without_device_fails_loudly():
+    from emsolvers_amd import _lib
+    from emsolvers_amd.fdtd import FemSimulation
+    if _lib.load().ems_device_count() > 0:
+        pytest.skip("a GPU is visible: covered by the gpu tests")
+    with pytest.raises(_lib.EmsError) as ei:
+        FemSimulation((8, 8, 8), 1e-3, 1e-12, 8.854187817e-12, 1.25663706212e-6, 1)
+    assert "no HIP device" in str(ei.value)
+
+
+def test_invalid_descriptor_rejected_before_device_use():
+    from emsolvers_amd import _lib
+    from emsolvers_amd.fdtd import FemSimulation
+    with pytest.raises(_lib.EmsError) as ei:
+        FemSimulation((0, 8, 8), 1e-3, 1e-12, 8.854187817e-12, 1.25663706212e-6, 1)
+    assert ei.value.status == 1
+    with pytest.raises(_lib.EmsError):
+        FemSimulation((8, 8, 8), 1e-3, 1e-12, 8.854187817e-12, 1.25663706212e-6, 0)
+
+
+def test_check_params_matches_oracle():
+    from emsolvers_amd.fdtd import check_params
+    from oracle import fdtd as of
+    c = of.C0
+    rng = np.random.default_rng(0)
+    cases = [([1, 1, 1], [500, 500, 500], 3e-9, 3e-9 / 780, 1 / 500, c),
+             ([1, 1, 1], [500, 500, 500], 3e-9, 1e-9, 1 / 500, c),
+             ([1, 2, 1], [500, 500, 500], 3e-9, 1e-12, 1 / 500, c)]
+    for _ in range(50):
+        ds = list(rng.uniform(-0.5, 3, 3))
+        gn = list(rng.integers(-2, 50, 3))
+        cases.append((ds, gn, rng.uniform(-1, 1), rng.uniform(-1e-11, 1e-11), rng.uniform(1e-4, 1e-2), c))
+    for args in cases:
+        assert check_params(*args) == of.check_params(*args), args
