@@ -48,6 +48,12 @@ struct ems_fdtd {
     double pml_order = 3.0, pml_r0 = 1e-6, pml_kmax = 1.0, pml_amax = 0.0;
     DevBuf prof;    // 18 profile tables
     DevBuf psi[12];
+    // fused kernel: second field set and second H-psi set (ping-pong)
+    DevBuf f2[6];
+    DevBuf psi2[6];
+    bool alt_ready = false;
+    int tile_by = 8;   // workgroup rows of the fused kernel (4, 8 or 16)
+    int tile_lz = 32;  // planes per workgroup of the fused kernel
     // sources / receivers
     std::vector<Src> src;
     std::vector<ObsLists> obs;
@@ -175,6 +181,7 @@ ems_status setup_pml(ems_fdtd *s)
     if (!s->dirty_pml) return EMS_OK;
     s->dirty_pml = false;
     for (auto &p : s->psi) p.release();
+    for (auto &p : s->psi2) p.release();
     s->prof.release();
     if (s->npml <= 0) return EMS_OK;
     const GridView &g = s->g;
@@ -201,6 +208,31 @@ ems_status setup_pml(ems_fdtd *s)
             EMS_TRY(b.reserve(nslab[ax] * s->esz));
             EMS_HIP(hipMemset(b.p, 0, nslab[ax] * s->esz));
         }
+    for (int ax = 0; ax < 3; ++ax)
+        for (int q = 0; q < 2; ++q) {
+            DevBuf &b = s->psi2[ax * 2 + q];
+            EMS_TRY(b.reserve(nslab[ax] * s->esz));
+            EMS_HIP(hipMemset(b.p, 0, nslab[ax] * s->esz));
+        }
+    return EMS_OK;
+}
+
+void swap_buf(DevBuf &a, DevBuf &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.bytes, b.bytes);
+}
+
+// Second field set for the fused kernel, initialised as a copy of the
+// current one so that entries no kernel writes (ghosts) agree in both.
+ems_status ensure_alt(ems_fdtd *s)
+{
+    if (s->alt_ready) return EMS_OK;
+    for (int w = 0; w < 6; ++w) {
+        EMS_TRY(s->f2[w].reserve(s->nalloc * s->esz));
+        EMS_HIP(hipMemcpy(s->f2[w].p, s->f[w].p, s->nalloc * s->esz, hipMemcpyDeviceToDevice));
+    }
+    s->alt_ready = true;
     return EMS_OK;
 }
 
@@ -244,16 +276,60 @@ void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
     k_e_split<T, MAT, PML><<<grd, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm);
 }
 
+template <typename T, int BY, int MAT, bool PML>
+void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    const GridView &g = s->g;
+    const int lz = s->tile_lz;
+    const int ntx = (g.nx + 1 + 62) / 63;
+    const int nty = (g.ny + 1 + BY - 2) / (BY - 1);
+    const int nkc = (g.nz + 1 + lz - 1) / lz;
+    const int nblk = ntx * nty * nkc;
+    const int grid = (nblk + 7) / 8 * 8;
+    Fields<T> o = fields<T>(s);
+    Fields<T> nw{s->f2[0].as<T>(), s->f2[1].as<T>(), s->f2[2].as<T>(),
+                 s->f2[3].as<T>(), s->f2[4].as<T>(), s->f2[5].as<T>()};
+    PsiH<T> ph{};
+    if (PML)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q) {
+                ph.old_[ax][q] = s->psi[ax * 4 + q].as<T>();
+                ph.new_[ax][q] = s->psi2[ax * 2 + q].as<T>();
+            }
+    k_fused<T, BY, MAT, PML><<<grid, dim3(64, BY, 1), 0, s->stream>>>(
+        g, ch, ce, o, nw, s->ca.as<T>(), s->cb.as<T>(), pm, ph, lz, ntx, nty, nblk);
+    // the new buffers become current
+    for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
+    if (PML)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 2 + q]);
+}
+
+template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    if (s->tile_by == 4) launch_fused<T, 4, MAT, PML>(s, ch, ce, pm);
+    else if (s->tile_by == 16) launch_fused<T, 16, MAT, PML>(s, ch, ce, pm);
+    else launch_fused<T, 8, MAT, PML>(s, ch, ce, pm);
+}
+
+template <typename T, int MAT, bool PML> void launch_variant(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    if (s->kernel == EMS_KERNEL_SPLIT) launch_split<T, MAT, PML>(s, ch, ce, pm);
+    else launch_fused_by<T, MAT, PML>(s, ch, ce, pm);
+}
+
+// One Yee step; the psi pointers of pm are refreshed by the caller after a
+// fused step (the H-psi buffers swap).
 template <typename T> void launch_step(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     const bool pml = s->npml > 0;
     switch (s->mat_mode * 2 + (pml ? 1 : 0)) {
-    case 0: launch_split<T, 0, false>(s, ch, ce, pm); break;
-    case 1: launch_split<T, 0, true>(s, ch, ce, pm); break;
-    case 2: launch_split<T, 1, false>(s, ch, ce, pm); break;
-    case 3: launch_split<T, 1, true>(s, ch, ce, pm); break;
-    case 4: launch_split<T, 2, false>(s, ch, ce, pm); break;
-    default: launch_split<T, 2, true>(s, ch, ce, pm); break;
+    case 0: launch_variant<T, 0, false>(s, ch, ce, pm); break;
+    case 1: launch_variant<T, 0, true>(s, ch, ce, pm); break;
+    case 2: launch_variant<T, 1, false>(s, ch, ce, pm); break;
+    case 3: launch_variant<T, 1, true>(s, ch, ce, pm); break;
+    case 4: launch_variant<T, 2, false>(s, ch, ce, pm); break;
+    default: launch_variant<T, 2, true>(s, ch, ce, pm); break;
     }
 }
 
@@ -329,8 +405,7 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
 
     const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    const PmlView<T> pm = pml_view<T>(s);
-    Fields<T> f = fields<T>(s);
+    if (s->kernel != EMS_KERNEL_SPLIT) EMS_TRY(ensure_alt(s));
 
     // timed mode: one event pair around the Yee kernels of every step plus
     // one pair around the whole sequence; read back after the last step
@@ -343,7 +418,9 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
     }
     for (int step = 1; step <= num_steps; ++step) {
         if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1)], s->stream));
+        const PmlView<T> pm = pml_view<T>(s);
         launch_step<T>(s, ch, ce, pm);
+        const Fields<T> f = fields<T>(s);
         if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1) + 1], s->stream));
         EMS_HIP(hipGetLastError());
         double *rec = nullptr;
@@ -464,6 +541,10 @@ template <typename T> ems_status set_field_t(ems_fdtd *s, int w, const double *i
         for (int j = 0; j < d[1]; ++j)
             for (int i = 0; i < d[0]; ++i) host[uidx(s->g, i + sh[0], j + sh[1], k + sh[2])] = (T)in[n++];
     EMS_HIP(hipMemcpy(s->f[w].p, host.data(), s->nalloc * sizeof(T), hipMemcpyHostToDevice));
+    // keep the ping-pong partner identical: entries no kernel writes (the
+    // ghost ring) must hold the same values in both buffers
+    if (s->alt_ready)
+        EMS_HIP(hipMemcpy(s->f2[w].p, host.data(), s->nalloc * sizeof(T), hipMemcpyHostToDevice));
     return EMS_OK;
 }
 
@@ -600,6 +681,17 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant)
     if (!s) return invalid("null handle");
     if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_FUSED) return invalid("unknown kernel variant");
     s->kernel = variant;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (rows != 4 && rows != 8 && rows != 16) return invalid("tile rows must be 4, 8 or 16");
+    if (planes < 1) return invalid("tile planes must be >= 1");
+    s->tile_by = rows;
+    s->tile_lz = planes;
     return EMS_OK;
 }
 

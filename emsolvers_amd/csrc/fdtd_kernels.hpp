@@ -226,3 +226,190 @@ __global__ __launch_bounds__(256) void k_post(Fields<T> f, int nsrc, const int *
 
 }  // namespace fdtd
 }  // namespace ems
+
+namespace ems {
+namespace fdtd {
+
+// ---------------------------------------------------------------------------
+// Fused single-sweep kernel.
+//
+// One launch per time step reads E^n, H^{n-1/2} from the "old" buffers and
+// writes H^{n+1/2}, E^{n+1} into the "new" buffers (ping-pong), so every
+// field word is read once and written once per step.  A workgroup of
+// 64 x BY threads owns a 63 x (BY-1) tile of (I, J) columns and marches up a
+// chunk of lz planes in K.  E(K) needs H^{n+1/2} at I-1, J-1 and K-1, so the
+// workgroup also computes H on a one-cell halo on the low side: lane 0 (I-1),
+// row 0 (J-1) and the plane below the chunk (K0-1); these are recomputed from
+// the old buffers and never stored.  H at I-1 / J-1 is exchanged through LDS
+// (double-buffered by plane parity: one barrier per plane), H at K-1 and
+// E^n at K+1 are carried in registers.  CPML psi for H is ping-ponged as
+// well (halo threads read the old values); psi for E is updated in place by
+// the owning thread.  Arithmetic per output is identical to the split
+// kernels, so both give the same bits.
+// ---------------------------------------------------------------------------
+template <typename T> struct PsiH {
+    const T *old_[3][2];
+    T *new_[3][2];
+};
+
+template <typename T, int BY, int MAT, bool PML>
+__global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Fields<T> o, Fields<T> nw,
+                                                    const T *__restrict__ ca, const T *__restrict__ cb,
+                                                    PmlView<T> pm, PsiH<T> ph, int lz, int ntx, int nty,
+                                                    int nblk)
+{
+    __shared__ T sH[2][3][BY][64];
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
+    // give each XCD a contiguous range of tiles so halos are shared in its L2.
+    const int per = gridDim.x >> 3;
+    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (lb >= nblk) return;  // uniform for the whole workgroup
+    const int ti = lb % ntx;
+    const int tj = (lb / ntx) % nty;
+    const int kc = lb / (ntx * nty);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int I = ti * 63 - 1 + tx;
+    const int J = tj * (BY - 1) - 1 + ty;
+    const int K0 = kc * lz;
+    const int K1 = min(K0 + lz, g.nz + 1);
+    const bool act = I >= 0 && J >= 0 && I <= g.nx && J <= g.ny;
+    const bool own = act && tx >= 1 && ty >= 1;
+    const int Kst = K0 > 0 ? K0 - 1 : 0;
+    long u = g.off + (long)(act ? I : 0) + g.pitch * (act ? J : 0) + g.slab * Kst;
+    bool px = false, py = false;
+    if (PML) {
+        px = in_pml(pm.P, g.nx, I);
+        py = in_pml(pm.P, g.ny, J);
+    }
+    T ex_k = 0, ey_k = 0, hx_km1 = 0, hy_km1 = 0;
+    if (act) {
+        ex_k = o.ex[u];
+        ey_k = o.ey[u];
+    }
+    for (int K = Kst; K < K1; ++K) {
+        const int par = K & 1;
+        const bool wr = own && K >= K0;
+        T hxn = 0, hyn = 0, hzn = 0, ez_k = 0, ex_k1 = 0, ey_k1 = 0;
+        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        if (act) {
+            ex_k1 = o.ex[u + g.slab];
+            ey_k1 = o.ey[u + g.slab];
+            ez_k = o.ez[u];
+            const T ez_i1 = o.ez[u + 1], ey_i1 = o.ey[u + 1];
+            const T ez_j1 = o.ez[u + g.pitch], ex_j1 = o.ex[u + g.pitch];
+            const T hx = o.hx[u], hy = o.hy[u], hz = o.hz[u];
+            // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
+            if (PML && (py || pz)) {
+                T t1 = ez_j1 - ez_k, t2 = ey_k1 - ey_k;
+                if (py) {
+                    long q = slab_y(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[1][J] * ph.old_[1][0][q] + pm.ah[1][J] * t1;
+                    if (wr) ph.new_[1][0][q] = p;
+                    t1 = t1 * pm.kh[1][J] + p;
+                }
+                if (pz) {
+                    long q = slab_z(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[2][K] * ph.old_[2][0][q] + pm.ah[2][K] * t2;
+                    if (wr) ph.new_[2][0][q] = p;
+                    t2 = t2 * pm.kh[2][K] + p;
+                }
+                hxn = hx - ch * (t1 - t2);
+            } else {
+                hxn = hx - ch * (ez_j1 - ez_k - ey_k1 + ey_k);
+            }
+            // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
+            if (PML && (pz || px)) {
+                T t1 = ex_k1 - ex_k, t2 = ez_i1 - ez_k;
+                if (pz) {
+                    long q = slab_z(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[2][K] * ph.old_[2][1][q] + pm.ah[2][K] * t1;
+                    if (wr) ph.new_[2][1][q] = p;
+                    t1 = t1 * pm.kh[2][K] + p;
+                }
+                if (px) {
+                    long q = slab_x(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[0][I] * ph.old_[0][0][q] + pm.ah[0][I] * t2;
+                    if (wr) ph.new_[0][0][q] = p;
+                    t2 = t2 * pm.kh[0][I] + p;
+                }
+                hyn = hy - ch * (t1 - t2);
+            } else {
+                hyn = hy - ch * (ex_k1 - ex_k - ez_i1 + ez_k);
+            }
+            // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
+            if (PML && (px || py)) {
+                T t1 = ey_i1 - ey_k, t2 = ex_j1 - ex_k;
+                if (px) {
+                    long q = slab_x(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[0][I] * ph.old_[0][1][q] + pm.ah[0][I] * t1;
+                    if (wr) ph.new_[0][1][q] = p;
+                    t1 = t1 * pm.kh[0][I] + p;
+                }
+                if (py) {
+                    long q = slab_y(g, pm.P, pm.W, I, J, K);
+                    T p = pm.bh[1][J] * ph.old_[1][1][q] + pm.ah[1][J] * t2;
+                    if (wr) ph.new_[1][1][q] = p;
+                    t2 = t2 * pm.kh[1][J] + p;
+                }
+                hzn = hz - ch * (t1 - t2);
+            } else {
+                hzn = hz - ch * (ey_i1 - ey_k - ex_j1 + ex_k);
+            }
+        }
+        sH[par][0][ty][tx] = hxn;
+        sH[par][1][ty][tx] = hyn;
+        sH[par][2][ty][tx] = hzn;
+        __syncthreads();
+        if (wr) {
+            if (I >= 1) nw.hx[u] = hxn;
+            if (J >= 1) nw.hy[u] = hyn;
+            if (K >= 1) nw.hz[u] = hzn;
+            const T hz_im1 = sH[par][2][ty][tx - 1], hy_im1 = sH[par][1][ty][tx - 1];
+            const T hz_jm1 = sH[par][2][ty - 1][tx], hx_jm1 = sH[par][0][ty - 1][tx];
+            if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
+                T br;
+                if (PML && (py || pz)) {
+                    T t1 = hzn - hz_jm1, t2 = hyn - hy_km1;
+                    if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t1);
+                    if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t2);
+                    br = t1 - t2;
+                } else {
+                    br = hzn - hz_jm1 - hyn + hy_km1;
+                }
+                nw.ex[u] = e_update<T, MAT>(ex_k, br, ce, ca, cb, u);
+            }
+            if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
+                T br;
+                if (PML && (pz || px)) {
+                    T t1 = hxn - hx_km1, t2 = hzn - hz_im1;
+                    if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t1);
+                    if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t2);
+                    br = t1 - t2;
+                } else {
+                    br = hxn - hx_km1 - hzn + hz_im1;
+                }
+                nw.ey[u] = e_update<T, MAT>(ey_k, br, ce, ca, cb, u);
+            }
+            if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
+                T br;
+                if (PML && (px || py)) {
+                    T t1 = hyn - hy_im1, t2 = hxn - hx_jm1;
+                    if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t1);
+                    if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t2);
+                    br = t1 - t2;
+                } else {
+                    br = hyn - hy_im1 - hxn + hx_jm1;
+                }
+                nw.ez[u] = e_update<T, MAT>(ez_k, br, ce, ca, cb, u);
+            }
+        }
+        ex_k = ex_k1;
+        ey_k = ey_k1;
+        hx_km1 = hxn;
+        hy_km1 = hyn;
+        u += g.slab;
+    }
+}
+
+}  // namespace fdtd
+}  // namespace ems

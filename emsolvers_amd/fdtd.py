@@ -113,6 +113,10 @@ class FemSimulation:
     def set_kernel(self, variant):
         _lib.check(_lib.load().ems_fdtd_set_kernel(self._h, int(variant)))
 
+    def set_tiling(self, rows, planes):
+        """Fused-kernel workgroup rows (4, 8, 16) and planes per workgroup."""
+        _lib.check(_lib.load().ems_fdtd_set_tiling(self._h, int(rows), int(planes)))
+
     def run_timed(self, num_steps):
         """Returns (total_ms, kernel_ms) measured with HIP events."""
         tot, ker = ctypes.c_double(), ctypes.c_double()

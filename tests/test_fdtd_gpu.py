@@ -27,7 +27,7 @@ def gpu_sim(grid, h, dt, eps, mu, tf, precision="f64", kernel=None):
     return s
 
 
-KERNELS = [1]  # split (fused variant added with its kernel)
+KERNELS = [1, 2]  # split, fused
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -101,7 +101,6 @@ def test_fp32_error_against_fp64_oracle(oracle_runs, name, kernel):
         assert rel_l2(s.field(w), o.field(w)) <= 1e-4, NAMES[w]
 
 
-@pytest.mark.skip(reason="fused kernel not built yet")
 def test_split_and_fused_agree_bitwise_fp32():
     case = fc.PARITY_CASES["lossy_cpml_rect"]
     a = build_gpu(case, "f32", 1)
@@ -110,6 +109,21 @@ def test_split_and_fused_agree_bitwise_fp32():
     b.run(case["nsteps"])
     for w in range(6):
         assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+
+
+@pytest.mark.parametrize("rows,planes", [(4, 1), (4, 7), (8, 32), (16, 3), (16, 1000)])
+def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes):
+    """Tile shape and K-chunking change only the halo recomputation, never the
+    arithmetic: every tiling must give the oracle's FP64 bits."""
+    case = fc.PARITY_CASES["lossy_cpml_rect"]
+    print("tiling", rows, planes, flush=True)
+    o = oracle_runs("lossy_cpml_rect")
+    s = build_gpu(case, "f64", 2)
+    s.set_tiling(rows, planes)
+    s.run(case["nsteps"])
+    assert np.array_equal(s.traces(), o.traces())
+    for w in range(6):
+        assert np.array_equal(s.field(w), o.field(w)), NAMES[w]
 
 
 def test_run_twice_accumulates_like_reference(golden):
