@@ -31,6 +31,11 @@ class FdtdDesc(ctypes.Structure):
                 ("mu", _D), ("tf", _I), ("precision", _I)]
 
 
+class FemDesc(ctypes.Structure):
+    _fields_ = [("order", _I), ("nx", _I), ("ny", _I), ("nz", _I), ("h", _D), ("dt", _D), ("eps", _D),
+                ("mu", _D), ("tf", _I), ("stepper", _I), ("precision", _I)]
+
+
 SIGNATURES = {
     "ems_last_error": (ctypes.c_char_p, []),
     "ems_version": (ctypes.c_char_p, []),
@@ -56,6 +61,22 @@ SIGNATURES = {
     "ems_fdtd_save_results": (_I, [_P, _I, _D, ctypes.c_char_p]),
     "ems_fdtd_device_bytes": (_I, [_P, ctypes.POINTER(_SZ)]),
     "ems_fdtd_check_params": (_I, [_DP, _IP, _D, _D, _D, _D]),
+    # structured-hex FEM
+    "ems_fem_load_kmat": (_I, [ctypes.c_char_p, _DP, _SZ, _IP]),
+    "ems_fem_create": (_I, [ctypes.POINTER(FemDesc), _DP, _PP]),
+    "ems_fem_destroy": (None, [_P]),
+    "ems_fem_add_source": (_I, [_P, _I, _I, _I, _I]),
+    "ems_fem_add_observation": (_I, [_P, _I, _I, _I]),
+    "ems_fem_run": (_I, [_P, _I]),
+    "ems_fem_run_timed": (_I, [_P, _I, _DP, _DP]),
+    "ems_fem_trace_shape": (_I, [_P, _IP, _IP]),
+    "ems_fem_get_traces": (_I, [_P, _DP, _SZ]),
+    "ems_fem_field_size": (_I, [_P, _I, ctypes.POINTER(_SZ)]),
+    "ems_fem_get_field": (_I, [_P, _I, _I, _DP, _SZ]),
+    "ems_fem_set_field": (_I, [_P, _I, _I, _DP, _SZ]),
+    "ems_fem_get_time": (_I, [_P, _DP]),
+    "ems_fem_save_results": (_I, [_P, _I, ctypes.c_char_p]),
+    "ems_fem_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
 }
 
 _lib = None

@@ -1,0 +1,555 @@
+// Structured-hex edge FEM (fem_v2, symplectic_2, symplectic_4) behind the C ABI.
+//
+// Drop-in for the reference class FemSimulation of src/fem_v2 and
+// src/symplectic_{2,4} with one GPU owning the whole domain: the MPI
+// interface sums of the reference (fem_simulation.cpp:657-1113) vanish.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "fem_kernels.hpp"
+
+using namespace ems;
+using namespace ems::fem;
+
+struct ems_fem {
+    ems_fem_desc d{};
+    FemGrid g{};
+    size_t esz = 8;
+    std::vector<double> kscaled;  // n x n, FP64
+    DevBuf K, off;
+    // leapfrog: slot[s][comp]; symplectic: U = slot 0, P = slot 1, and the
+    // ping-pong partners slot 2 (U), slot 3 (P)
+    DevBuf f[4][3];
+    int t0 = 0, t1 = 1, t2 = 2;  // leapfrog rotation
+    std::vector<long> src_z;     // z sources (flat Ez index)
+    int nsrc_xy = 0;             // x/y sources: stored by the reference, never injected
+    std::vector<std::array<long, 9>> obs;
+    std::vector<double> traces;  // [nobs][nslot][4]
+    int nslot = 0;
+    double t_now = 0.0;
+    hipStream_t stream = nullptr;
+    DevBuf d_src, d_sval, d_obs, d_rec;
+};
+
+namespace {
+
+// src/fem_v2/fem_simulation.cpp:77-87
+double fem_source(double t)
+{
+    const double freq = 1.0e9;
+    const double chi = 1.0 / freq;
+    double delay = t - chi;
+    double zeta = 2.0 * M_PI * M_PI * freq * freq;
+    return 2.0 * zeta * (2.0 * zeta * delay * delay - 1.0) * std::exp(0.0 - zeta * delay * delay) * 0.001;
+}
+
+size_t comp_size(const ems_fem *s, int c) { return c == 0 ? s->g.NX : c == 1 ? s->g.NY : s->g.NZ; }
+
+template <typename T> CComp3<T> cc3(ems_fem *s, int slot)
+{
+    return CComp3<T>{s->f[slot][0].as<T>(), s->f[slot][1].as<T>(), s->f[slot][2].as<T>()};
+}
+template <typename T> Comp3<T> c3(ems_fem *s, int slot)
+{
+    return Comp3<T>{s->f[slot][0].as<T>(), s->f[slot][1].as<T>(), s->f[slot][2].as<T>()};
+}
+
+void swap_slots(ems_fem *s, int a, int b)
+{
+    for (int c = 0; c < 3; ++c) {
+        std::swap(s->f[a][c].p, s->f[b][c].p);
+        std::swap(s->f[a][c].bytes, s->f[b][c].bytes);
+    }
+}
+
+template <typename T, int P>
+void launch_leapfrog(ems_fem *s, const double *sval)
+{
+    const long ndof = s->g.NX + s->g.NY + s->g.NZ;
+    const int blk = 256;
+    const long grd = (ndof + blk - 1) / blk;
+    k_fem_leapfrog<T, P><<<grd, blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), cc3<T>(s, s->t0),
+                                                    cc3<T>(s, s->t1), c3<T>(s, s->t2), (int)s->src_z.size(),
+                                                    s->d_src.as<long>(), sval);
+}
+
+template <typename T, int P>
+void launch_stage(ems_fem *s, double deltat, const double *sval)
+{
+    const long ndof = s->g.NX + s->g.NY + s->g.NZ;
+    const int blk = 256;
+    const long grd = (ndof + blk - 1) / blk;
+    const double mm = 8.0 / s->g.p / s->g.p / s->g.p;
+    k_fem_symplectic<T, P><<<grd, blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), cc3<T>(s, 0),
+                                                      cc3<T>(s, 1), c3<T>(s, 2), c3<T>(s, 3), (T)deltat,
+                                                      (T)(0.5 * deltat), (T)mm, (int)s->src_z.size(),
+                                                      s->d_src.as<long>(), sval);
+    swap_slots(s, 0, 2);
+    swap_slots(s, 1, 3);
+}
+
+template <typename T> void step_leapfrog(ems_fem *s, const double *sval)
+{
+    switch (s->g.p) {
+    case 1: launch_leapfrog<T, 1>(s, sval); break;
+    case 2: launch_leapfrog<T, 2>(s, sval); break;
+    default: launch_leapfrog<T, 3>(s, sval); break;
+    }
+}
+
+template <typename T> void step_stage(ems_fem *s, double deltat, const double *sval)
+{
+    switch (s->g.p) {
+    case 1: launch_stage<T, 1>(s, deltat, sval); break;
+    case 2: launch_stage<T, 2>(s, deltat, sval); break;
+    default: launch_stage<T, 3>(s, deltat, sval); break;
+    }
+}
+
+template <typename T> ems_status record(ems_fem *s, int slot, size_t row)
+{
+    const int nobs = (int)s->obs.size();
+    if (!nobs) return EMS_OK;
+    k_fem_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(cc3<T>(s, slot), nobs, s->d_obs.as<long>(),
+                                                            s->d_rec.as<double>() + row * 9 * nobs);
+    EMS_HIP(hipGetLastError());
+    return EMS_OK;
+}
+
+template <typename T> ems_status run_impl(ems_fem *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    const int tf = s->d.tf;
+    // saved_electric_field_ resize (src/fem_v2/fem_simulation.cpp:1251-1255)
+    const int nslot = (num_steps - 1) / tf + 1;
+    const int nobs = (int)s->obs.size();
+    if (nslot != s->nslot) {
+        std::vector<double> t((size_t)nobs * std::max(nslot, 0) * 4, 0.0);
+        for (int o = 0; o < nobs; ++o)
+            for (int q = 0; q < std::min(nslot, s->nslot); ++q)
+                for (int c = 0; c < 4; ++c)
+                    t[((size_t)o * nslot + q) * 4 + c] = s->traces[((size_t)o * s->nslot + q) * 4 + c];
+        s->traces.swap(t);
+        s->nslot = std::max(nslot, 0);
+    }
+    if (num_steps <= 0) return EMS_OK;
+    const bool leap = s->d.stepper == EMS_FEM_LEAPFROG;
+    const int nstage = leap ? 1 : s->d.stepper == EMS_FEM_SYMPLECTIC2 ? 1 : 3;
+    const double x1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0));
+    const double x2 = 1.0 - 2.0 * x1;
+    const double dt = s->d.dt, h = s->d.h, hp = s->d.h * s->g.p;
+    // per-(step, stage) source value and the host time stamps
+    std::vector<double> sval((size_t)num_steps * nstage);
+    std::vector<double> stamp(num_steps);
+    double t = s->t_now;
+    for (int st = 0; st < num_steps; ++st) {
+        stamp[st] = t;
+        if (leap) {
+            sval[st] = fem_source(t) * dt * dt / s->d.eps / h / h / h;
+        } else if (nstage == 1) {
+            sval[st] = fem_source(t + 0.0 + 0.5 * dt) * 8.0 / s->d.eps / hp / hp / hp;
+        } else {
+            const double D[3] = {x1 * dt, x2 * dt, x1 * dt};
+            const double O[3] = {0.0, x1 * dt, (x1 + x2) * dt};
+            for (int q = 0; q < 3; ++q)
+                sval[(size_t)st * 3 + q] = fem_source(t + O[q] + 0.5 * D[q]) * 8.0 / s->d.eps / hp / hp / hp;
+        }
+        t += dt;
+    }
+    const int nrec = (num_steps - 1) / tf + 1;
+    EMS_TRY(s->d_sval.reserve(sizeof(double) * sval.size()));
+    EMS_HIP(hipMemcpy(s->d_sval.p, sval.data(), sizeof(double) * sval.size(), hipMemcpyHostToDevice));
+    EMS_TRY(s->d_rec.reserve(sizeof(double) * std::max((size_t)nrec * nobs * 9, (size_t)1)));
+
+    const bool timed = total_ms != nullptr;
+    std::vector<hipEvent_t> ev;
+    if (timed) {
+        ev.resize(2 * (size_t)num_steps + 2, nullptr);
+        for (auto &e : ev) EMS_HIP(hipEventCreate(&e));
+        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps], s->stream));
+    }
+    const double *dsv = s->d_sval.as<double>();
+    for (int st = 0; st < num_steps; ++st) {
+        if (leap) {
+            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)st], s->stream));
+            step_leapfrog<T>(s, dsv + st);
+            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)st + 1], s->stream));
+            EMS_HIP(hipGetLastError());
+            // the reference records E^n (slot time_1_) after the update
+            if (st % tf == 0) EMS_TRY(record<T>(s, s->t1, st / tf));
+            const int o0 = s->t0;
+            s->t0 = s->t1;
+            s->t1 = s->t2;
+            s->t2 = o0;
+        } else {
+            // the symplectic family records U before the step
+            if (st % tf == 0) EMS_TRY(record<T>(s, 0, st / tf));
+            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)st], s->stream));
+            if (nstage == 1) {
+                step_stage<T>(s, dt, dsv + st);
+            } else {
+                step_stage<T>(s, x1 * dt, dsv + (size_t)st * 3);
+                step_stage<T>(s, x2 * dt, dsv + (size_t)st * 3 + 1);
+                step_stage<T>(s, x1 * dt, dsv + (size_t)st * 3 + 2);
+            }
+            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)st + 1], s->stream));
+            EMS_HIP(hipGetLastError());
+        }
+    }
+    if (timed) {
+        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps + 1], s->stream));
+        EMS_HIP(hipEventSynchronize(ev[2 * (size_t)num_steps + 1]));
+        double ksum = 0;
+        for (int st = 0; st < num_steps; ++st) {
+            float ms = 0;
+            EMS_HIP(hipEventElapsedTime(&ms, ev[2 * (size_t)st], ev[2 * (size_t)st + 1]));
+            ksum += ms;
+        }
+        float ms = 0;
+        EMS_HIP(hipEventElapsedTime(&ms, ev[2 * (size_t)num_steps], ev[2 * (size_t)num_steps + 1]));
+        *total_ms = ms;
+        *kernel_ms = ksum;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    s->t_now = t;
+    if (nobs) {
+        std::vector<double> raw((size_t)nrec * nobs * 9);
+        EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrec && r < s->nslot; ++r)
+            for (int o = 0; o < nobs; ++o) {
+                const double *v = &raw[((size_t)r * nobs + o) * 9];
+                double *dst = &s->traces[((size_t)o * s->nslot + r) * 4];
+                dst[0] = stamp[(size_t)r * tf];
+                dst[1] = (v[0] + v[1] + v[2] + v[3]) / 4.0;
+                dst[2] = (v[4] + v[5] + v[6] + v[7]) / 4.0;
+                dst[3] = v[8];
+            }
+    }
+    return EMS_OK;
+}
+
+template <typename T> ems_status upload_kmat(ems_fem *s)
+{
+    std::vector<T> k(s->kscaled.begin(), s->kscaled.end());
+    EMS_TRY(s->K.reserve(sizeof(T) * k.size()));
+    EMS_HIP(hipMemcpy(s->K.p, k.data(), sizeof(T) * k.size(), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+int logical_slot(const ems_fem *s, int logical)
+{
+    if (s->d.stepper != EMS_FEM_LEAPFROG) return logical;  // 0 = U, 1 = P
+    return logical == 0 ? s->t0 : logical == 1 ? s->t1 : -1;
+}
+
+// src/fem_v2/fem_simulation.cpp:1477-1493
+double parse_value(const std::string &str)
+{
+    size_t start = str.find_first_not_of(" \t\r\n\"");
+    size_t end = str.find_last_not_of(" \t\r\n\"");
+    if (start == std::string::npos) return 0.0;
+    std::string t = str.substr(start, end - start + 1);
+    size_t slash = t.find('/');
+    if (slash != std::string::npos) return std::stod(t.substr(0, slash)) / std::stod(t.substr(slash + 1));
+    return std::stod(t);
+}
+
+}  // namespace
+
+extern "C" {
+
+ems_status ems_fem_load_kmat(const char *path, double *out, size_t count, int *rows)
+{
+    clear_error();
+    if (!path) return invalid("null path");
+    std::ifstream file(path);
+    if (!file.is_open()) {
+        set_error(std::string("Cannot open file: ") + path);
+        return EMS_ERR_IO;
+    }
+    std::vector<std::vector<double>> m;
+    std::string line;
+    try {
+        while (std::getline(file, line)) {
+            std::vector<double> row;
+            std::stringstream ss(line);
+            std::string cell;
+            while (std::getline(ss, cell, ',')) row.push_back(parse_value(cell));
+            if (!row.empty()) m.push_back(row);
+        }
+    } catch (const std::exception &e) {
+        set_error(std::string("malformed kmat file ") + path + ": " + e.what());
+        return EMS_ERR_IO;
+    }
+    const size_t n = m.size();
+    for (auto &r : m)
+        if (r.size() != n) {
+            set_error(std::string("kmat file is not square: ") + path);
+            return EMS_ERR_IO;
+        }
+    if (rows) *rows = (int)n;
+    if (!out) return EMS_OK;
+    if (count != n * n) return invalid("kmat buffer has the wrong size");
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < n; ++j) out[i * n + j] = m[i][j];
+    return EMS_OK;
+}
+
+ems_status ems_fem_create(const ems_fem_desc *desc, const double *kmat, ems_fem **out)
+{
+    clear_error();
+    if (!desc || !kmat || !out) return invalid("ems_fem_create: null argument");
+    *out = nullptr;
+    const int p = desc->order;
+    if (p < 1 || p > 3) return invalid("order must be 1, 2 or 3");
+    if (desc->nx <= 0 || desc->ny <= 0 || desc->nz <= 0) return invalid("grid sizes must be positive");
+    if (desc->nx % p || desc->ny % p || desc->nz % p) return invalid("grid sizes must be multiples of the order");
+    if (desc->nx / p < 2 || desc->ny / p < 2 || desc->nz / p < 2)
+        return invalid("at least two elements per axis are required");
+    if (desc->tf <= 0) return invalid("time_frequency must be >= 1");
+    if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
+        return invalid("h, dt, eps and mu must be positive");
+    if (desc->stepper < EMS_FEM_LEAPFROG || desc->stepper > EMS_FEM_SYMPLECTIC4) return invalid("unknown stepper");
+    if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
+    if (ems_device_count() <= 0) {
+        set_error("no HIP device available");
+        return EMS_ERR_HIP;
+    }
+    ems_fem *s = new ems_fem();
+    s->d = *desc;
+    s->esz = desc->precision == EMS_F32 ? 4 : 8;
+    FemGrid &g = s->g;
+    g.p = p;
+    g.Nx = desc->nx;
+    g.Ny = desc->ny;
+    g.Nz = desc->nz;
+    g.nex = desc->nx / p;
+    g.ney = desc->ny / p;
+    g.nez = desc->nz / p;
+    const long Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+    g.NX = Nx * (Ny + 1) * (Nz + 1);
+    g.NY = (Nx + 1) * Ny * (Nz + 1);
+    g.NZ = (Nx + 1) * (Ny + 1) * Nz;
+    const long a = g.nex, b = g.ney, c = g.nez;
+    g.OE = 2 * (a * b + b * c + c * a) - 4 * (a + b + c) + 8;
+    g.D = p * (p + 1) * (p + 1);
+    g.n = 3 * g.D;
+    const int n = g.n;
+    // element operator scale (fem_v2: fem_simulation.cpp:476; symplectic: :473)
+    const double h = desc->h, dt = desc->dt;
+    double sc;
+    if (desc->stepper == EMS_FEM_LEAPFROG)
+        sc = 4.0 * dt * dt / (h * p) / (h * p) / desc->mu / desc->eps / (8.0 / p / p / p);
+    else
+        sc = 4.0 / (h * p) / (h * p) / desc->mu / desc->eps;
+    s->kscaled.resize((size_t)n * n);
+    for (int q = 0; q < n * n; ++q) s->kscaled[q] = sc * kmat[q];
+    // local DOF -> offset from the element base
+    std::vector<int> off(n);
+    for (int nn = 0; nn < p + 1; ++nn)
+        for (int m = 0; m < p + 1; ++m)
+            for (int l = 0; l < p; ++l) off[l + m * p + nn * p * (p + 1)] = (int)(l + m * Nx + nn * Nx * (Ny + 1));
+    for (int nn = 0; nn < p + 1; ++nn)
+        for (int m = 0; m < p; ++m)
+            for (int l = 0; l < p + 1; ++l)
+                off[g.D + l + m * (p + 1) + nn * p * (p + 1)] = (int)(l + m * (Nx + 1) + nn * (Nx + 1) * Ny);
+    for (int nn = 0; nn < p; ++nn)
+        for (int m = 0; m < p + 1; ++m)
+            for (int l = 0; l < p + 1; ++l)
+                off[2 * g.D + l + m * (p + 1) + nn * (p + 1) * (p + 1)] =
+                    (int)(l + m * (Nx + 1) + nn * (Nx + 1) * (Ny + 1));
+    ems_status st = s->off.reserve(sizeof(int) * n);
+    if (st == EMS_OK && hipMemcpy(s->off.p, off.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
+        st = hip_fail(hipGetLastError(), "hipMemcpy", __FILE__, __LINE__);
+    if (st == EMS_OK) st = desc->precision == EMS_F32 ? upload_kmat<float>(s) : upload_kmat<double>(s);
+    const int nslots = desc->stepper == EMS_FEM_LEAPFROG ? 3 : 4;
+    for (int sl = 0; sl < nslots && st == EMS_OK; ++sl)
+        for (int cc = 0; cc < 3 && st == EMS_OK; ++cc) {
+            st = s->f[sl][cc].reserve(comp_size(s, cc) * s->esz);
+            if (st == EMS_OK && hipMemset(s->f[sl][cc].p, 0, comp_size(s, cc) * s->esz) != hipSuccess)
+                st = hip_fail(hipGetLastError(), "hipMemset", __FILE__, __LINE__);
+        }
+    if (st != EMS_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return EMS_OK;
+}
+
+void ems_fem_destroy(ems_fem *s) { delete s; }
+
+ems_status ems_fem_add_source(ems_fem *s, int comp, int i, int j, int k)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (comp < 0 || comp > 2) return invalid("source component must be 0, 1 or 2");
+    const long Nx = s->g.Nx, Ny = s->g.Ny, Nz = s->g.Nz;
+    if (comp != 2) {
+        // setSource_x / _y: stored by the reference but never injected
+        s->nsrc_xy++;
+        return EMS_OK;
+    }
+    if (i < 0 || j < 0 || k < 0 || i > Nx || j > Ny || k >= Nz) return invalid("source outside the Ez array");
+    s->src_z.push_back(i + j * (Nx + 1) + k * (Nx + 1) * (Ny + 1));
+    EMS_TRY(s->d_src.reserve(sizeof(long) * s->src_z.size()));
+    EMS_HIP(hipMemcpy(s->d_src.p, s->src_z.data(), sizeof(long) * s->src_z.size(), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+ems_status ems_fem_add_observation(ems_fem *s, int i, int j, int k)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    const long Nx = s->g.Nx, Ny = s->g.Ny;
+    const long o0 = i + j * Nx + k * Nx * (Ny + 1);
+    const long o1 = i + j * (Nx + 1) + k * (Nx + 1) * Ny;
+    const long o2 = i + j * (Nx + 1) + k * (Nx + 1) * (Ny + 1);
+    std::array<long, 9> q = {o0 - 1, o0, o0 + Nx * (Ny + 1) - 1, o0 + Nx * (Ny + 1),
+                             o1 - (Nx + 1), o1, o1 + (Nx + 1) * (Ny - 1), o1 + (Nx + 1) * Ny, o2};
+    for (int a = 0; a < 9; ++a) {
+        const long lim = a < 4 ? s->g.NX : a < 8 ? s->g.NY : s->g.NZ;
+        if (i < 0 || j < 0 || k < 0 || q[a] < 0 || q[a] >= lim)
+            return invalid("observation point reads outside the field arrays");
+    }
+    s->obs.push_back(q);
+    std::vector<long> flat;
+    for (auto &o : s->obs) flat.insert(flat.end(), o.begin(), o.end());
+    EMS_TRY(s->d_obs.reserve(sizeof(long) * flat.size()));
+    EMS_HIP(hipMemcpy(s->d_obs.p, flat.data(), sizeof(long) * flat.size(), hipMemcpyHostToDevice));
+    std::vector<double> t(s->obs.size() * (size_t)s->nslot * 4, 0.0);
+    std::copy(s->traces.begin(), s->traces.end(), t.begin());
+    s->traces.swap(t);
+    return EMS_OK;
+}
+
+ems_status ems_fem_run(ems_fem *s, int num_steps)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->d.precision == EMS_F32 ? run_impl<float>(s, num_steps, nullptr, nullptr)
+                                     : run_impl<double>(s, num_steps, nullptr, nullptr);
+}
+
+ems_status ems_fem_run_timed(ems_fem *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    clear_error();
+    if (!s || !total_ms || !kernel_ms) return invalid("null argument");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->d.precision == EMS_F32 ? run_impl<float>(s, num_steps, total_ms, kernel_ms)
+                                     : run_impl<double>(s, num_steps, total_ms, kernel_ms);
+}
+
+ems_status ems_fem_trace_shape(const ems_fem *s, int *nobs, int *nslot)
+{
+    clear_error();
+    if (!s || !nobs || !nslot) return invalid("null argument");
+    *nobs = (int)s->obs.size();
+    *nslot = s->nslot;
+    return EMS_OK;
+}
+
+ems_status ems_fem_get_traces(ems_fem *s, double *out, size_t count)
+{
+    clear_error();
+    if (!s || (!out && count)) return invalid("null argument");
+    if (count != s->traces.size()) return invalid("trace buffer has the wrong size");
+    std::copy(s->traces.begin(), s->traces.end(), out);
+    return EMS_OK;
+}
+
+ems_status ems_fem_field_size(const ems_fem *s, int comp, size_t *n)
+{
+    clear_error();
+    if (!s || !n) return invalid("null argument");
+    if (comp < 0 || comp > 2) return invalid("component must be 0, 1 or 2");
+    *n = comp_size(s, comp);
+    return EMS_OK;
+}
+
+ems_status ems_fem_get_field(ems_fem *s, int comp, int slot, double *out, size_t count)
+{
+    clear_error();
+    if (!s || !out) return invalid("null argument");
+    if (comp < 0 || comp > 2) return invalid("component must be 0, 1 or 2");
+    const int sl = logical_slot(s, slot);
+    if (slot < 0 || slot > 1 || sl < 0) return invalid("slot must be 0 or 1");
+    if (count != comp_size(s, comp)) return invalid("field buffer has the wrong size");
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    if (s->esz == 8) {
+        EMS_HIP(hipMemcpy(out, s->f[sl][comp].p, count * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(count);
+        EMS_HIP(hipMemcpy(t.data(), s->f[sl][comp].p, count * 4, hipMemcpyDeviceToHost));
+        std::copy(t.begin(), t.end(), out);
+    }
+    return EMS_OK;
+}
+
+ems_status ems_fem_set_field(ems_fem *s, int comp, int slot, const double *in, size_t count)
+{
+    clear_error();
+    if (!s || !in) return invalid("null argument");
+    if (comp < 0 || comp > 2) return invalid("component must be 0, 1 or 2");
+    const int sl = logical_slot(s, slot);
+    if (slot < 0 || slot > 1 || sl < 0) return invalid("slot must be 0 or 1");
+    if (count != comp_size(s, comp)) return invalid("field buffer has the wrong size");
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    if (s->esz == 8) {
+        EMS_HIP(hipMemcpy(s->f[sl][comp].p, in, count * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(in, in + count);
+        EMS_HIP(hipMemcpy(s->f[sl][comp].p, t.data(), count * 4, hipMemcpyHostToDevice));
+    }
+    return EMS_OK;
+}
+
+ems_status ems_fem_get_time(const ems_fem *s, double *t)
+{
+    clear_error();
+    if (!s || !t) return invalid("null argument");
+    *t = s->t_now;
+    return EMS_OK;
+}
+
+// FemSimulation::saveResults (src/fem_v2/fem_simulation.cpp:1324-1349):
+// every observation writes the same file name (a reference quirk: the last
+// one wins); values in scientific notation with 15 digits.
+ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename)
+{
+    (void)num_steps;
+    clear_error();
+    if (!s || !filename) return invalid("null argument");
+    for (size_t i = 0; i < s->obs.size(); ++i) {
+        std::ofstream file(filename);
+        if (!file.is_open()) {
+            set_error(std::string("Error: Could not open file ") + filename);
+            return EMS_ERR_IO;
+        }
+        file << std::setprecision(15) << std::scientific;
+        file << "time,Ex,Ey,Ez" << std::endl;
+        for (int j = 0; j < s->nslot; ++j) {
+            const double *v = &s->traces[(i * s->nslot + j) * 4];
+            file << v[0] << "," << v[1] << "," << v[2] << "," << v[3] << std::endl;
+        }
+    }
+    return EMS_OK;
+}
+
+ems_status ems_fem_dof_count(const ems_fem *s, size_t *n)
+{
+    clear_error();
+    if (!s || !n) return invalid("null argument");
+    *n = (size_t)(s->g.NX + s->g.NY + s->g.NZ);
+    return EMS_OK;
+}
+
+}  // extern "C"

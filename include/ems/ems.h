@@ -145,6 +145,71 @@ ems_status ems_fdtd_device_bytes(const ems_fdtd *s, size_t *bytes);
 int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], double duration,
                           double time_step, double domain_size, double c);
 
+/* --------------------------------------------------------- structured-hex FEM
+ * The fem_v2 / symplectic_2 / symplectic_4 family: order-p edge elements on a
+ * structured hex grid (grid_size = fine sub-edges per axis, multiple of p),
+ * lumped scalar mass, PEC outer boundary, one GPU owning the whole domain.
+ * Fields are FP64 in the reference layouts (src/fem_v2/fem_simulation.cpp:
+ * 169-171): Ex Nx(Ny+1)(Nz+1), Ey (Nx+1)Ny(Nz+1), Ez (Nx+1)(Ny+1)Nz, x fastest. */
+
+enum { EMS_FEM_LEAPFROG = 0, EMS_FEM_SYMPLECTIC2 = 1, EMS_FEM_SYMPLECTIC4 = 2 };
+
+typedef struct ems_fem ems_fem;
+
+/* FemSimulation constructor arguments (src/fem_v2/fem_simulation.hpp:14-24)
+ * without the MPI topology (one GPU owns the whole domain). */
+typedef struct {
+    int order;        /* 1, 2 or 3                                            */
+    int nx, ny, nz;   /* grid_size (fine sub-edges per axis)                  */
+    double h;         /* domain_size: fine spacing [m]                        */
+    double dt;        /* time_step [s]                                        */
+    double eps, mu;   /* permittivity, permeability                           */
+    int tf;           /* time_frequency                                       */
+    int stepper;      /* EMS_FEM_LEAPFROG (fem_v2) / _SYMPLECTIC2 / _SYMPLECTIC4 */
+    int precision;    /* EMS_F64 or EMS_F32                                   */
+} ems_fem_desc;
+
+/* loadMatrixFromCSV (src/fem_v2/fem_simulation.cpp:1477-1524): quoted
+ * rationals "a/b".  out == NULL only reports the row count in *rows. */
+ems_status ems_fem_load_kmat(const char *path, double *out, size_t count, int *rows);
+
+/* FemSimulation::FemSimulation + calculateElementStiffnessMatrix
+ * (src/fem_v2/fem_simulation.cpp:89-160, :458-490): kmat is the raw n x n
+ * element matrix (n = 3 p (p+1)^2), scaled here as the reference scales it. */
+ems_status ems_fem_create(const ems_fem_desc *desc, const double *kmat, ems_fem **out);
+void ems_fem_destroy(ems_fem *s);
+
+/* setSource_x/_y/_z (src/fem_v2/fem_simulation.cpp:492-538); like the
+ * reference only z sources are injected. */
+ems_status ems_fem_add_source(ems_fem *s, int comp, int i, int j, int k);
+
+/* setObservationPoint (src/fem_v2/fem_simulation.cpp:540-558). */
+ems_status ems_fem_add_observation(ems_fem *s, int i, int j, int k);
+
+/* run (src/fem_v2/fem_simulation.cpp:1249-1322; symplectic_2 :1278-1351;
+ * symplectic_4 :1278-1359). */
+ems_status ems_fem_run(ems_fem *s, int num_steps);
+
+/* As ems_fem_run, with HIP-event timing of the step kernels (see FDTD). */
+ems_status ems_fem_run_timed(ems_fem *s, int num_steps, double *total_ms, double *kernel_ms);
+
+/* Traces [nobs][nslot][4] = (time, Ex, Ey, Ez) as saved_electric_field_. */
+ems_status ems_fem_trace_shape(const ems_fem *s, int *nobs, int *nslot);
+ems_status ems_fem_get_traces(ems_fem *s, double *out, size_t count);
+
+/* Field access: comp 0/1/2; slot 0/1 = E^{n-1}/E^n (leapfrog) or U/P
+ * (symplectic). */
+ems_status ems_fem_field_size(const ems_fem *s, int comp, size_t *n);
+ems_status ems_fem_get_field(ems_fem *s, int comp, int slot, double *out, size_t count);
+ems_status ems_fem_set_field(ems_fem *s, int comp, int slot, const double *in, size_t count);
+ems_status ems_fem_get_time(const ems_fem *s, double *t);
+
+/* saveResults (src/fem_v2/fem_simulation.cpp:1324-1349). */
+ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename);
+
+/* Total edge DOFs (NX + NY + NZ). */
+ems_status ems_fem_dof_count(const ems_fem *s, size_t *n);
+
 #ifdef __cplusplus
 }
 #endif
