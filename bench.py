@@ -51,6 +51,47 @@ def gpu_fdtd(n, steps, warmup, precision, kernel, npml=10, seed=11):
                 cells=n ** 3, device_bytes=sim.device_bytes())
 
 
+def gpu_fem(order, stepper, n, steps, precision="f64"):
+    """Structured-hex FEM (kmat operators) on an n^3 fine grid; returns
+    DOF-updates/s of the step kernels (HIP events)."""
+    from emsolvers_amd.fem import FemSimulation
+    from emsolvers_amd.workloads import EPS0, MU0, C0
+    kmat = os.path.join(ROOT, "tests", "golden", "kmat")
+    h = 1e-3
+    dt = h / (C0 * 3 ** 0.5) / 2.0
+    sim = FemSimulation(order, (n, n, n), h, dt, EPS0, MU0, 1, 1, stepper=stepper, precision=precision,
+                        kmat_dir=kmat)
+    sim.setSource_z((n // 2, n // 2, n // 2))
+    sim.setObservationPoint((n // 2 + order, n // 2, n // 2))
+    sim.run(2)
+    total_ms, kernel_ms = sim.run_timed(steps)
+    ndof = sim.ndof()
+    return dict(value=round(ndof * steps / (kernel_ms / 1e3) / 1e6, 1), unit="MDOF-updates/s",
+                dofs=ndof, ms_per_step=round(kernel_ms / steps, 4), order=order, stepper=stepper,
+                grid=[n] * 3, dtype="fp64" if precision == "f64" else "fp32")
+
+
+def pmc_traffic(path):
+    """HBM bytes per step from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a device copy of
+    known size measured in the same runs)."""
+    if not os.path.exists(path):
+        return None
+    kv = {}
+    kern = []
+    for line in open(path):
+        t = line.split()
+        if not t or t[0].startswith("#"):
+            continue
+        if t[0] == "kernel":
+            kern.append((float(t[3]), float(t[5])))
+        else:
+            kv[t[0]] = float(t[1])
+    fs = kv["copy_bytes"] / (kv["copy_fetch_kib"] * 1024.0)
+    ws = kv["copy_bytes"] / (kv["copy_write_kib"] * 1024.0)
+    return sum(f * 1024.0 * fs + w * 1024.0 * ws for f, w in kern)
+
+
 def cpu_reference_sample(n=256, steps=None, budget_s=12.0):
     """Reference FDTD (src/fdtd, compiled serially from its sources into
     oracle/_ref) on an n^3 vacuum/PEC grid: the closest workload it supports
@@ -102,6 +143,9 @@ def main():
     ap.add_argument("--precision", default="f32")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 split, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-fem", action="store_true", help="skip the FEM secondary figures")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="first run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -111,6 +155,13 @@ def main():
         print("bench.py: multi-rank launch is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibrate:
+        import torch
+        x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     r = gpu_fdtd(args.n, args.steps, args.warmup, args.precision, args.kernel)
     cells = r["cells"]
     ms_per_step = r["total_ms"] / args.steps
@@ -135,7 +186,7 @@ def main():
             "grid": [args.n] * 3,
             "cpml_cells": 10,
             "material_words": 1,
-            "kernel": {0: "auto", 1: "split", 2: "fused"}[args.kernel],
+            "kernel": {0: "auto (split H/E sweeps when a CPML is present, fused otherwise)", 1: "split", 2: "fused"}[args.kernel],
             "parallelism": "single-gpu",
         },
         "roofline": {
@@ -144,13 +195,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": pmc_traffic(os.path.join(ROOT, "profiles", "fdtd_yee_256_f32_cpml10_pmc.txt"))
+            if (args.n == 256 and args.precision == "f32" and args.kernel in (0, 1)) else None,
             "kernel_avg_ms": round(k_avg_ms, 5),
             "algorithmic_bytes_per_step": int(r["step_bytes"]),
             "bytes_per_cell": round(r["per_cell"], 3),
         },
         "device_bytes": r["device_bytes"],
     }
+    if not args.no_fem:
+        out["fem"] = [gpu_fem(2, "symplectic_4", 128, 20), gpu_fem(1, "fem_v2", 128, 50),
+                      gpu_fem(1, "symplectic_2", 128, 50, "f32")]
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_reference_sample(args.n)

@@ -46,14 +46,22 @@ struct ems_fdtd {
     // CPML
     int npml = 0;
     double pml_order = 3.0, pml_r0 = 1e-6, pml_kmax = 1.0, pml_amax = 0.0;
-    DevBuf prof;    // 18 profile tables
+    DevBuf prof;    // packed CPML grading tables
     DevBuf psi[12];
     // fused kernel: second field set and second H-psi set (ping-pong)
     DevBuf f2[6];
     DevBuf psi2[6];
     bool alt_ready = false;
     int tile_by = 8;   // workgroup rows of the fused kernel (4, 8 or 16)
-    int tile_lz = 32;  // planes per workgroup of the fused kernel
+    int tile_lz = 4;   // planes per workgroup of the fused kernel
+    int tile_flags = 1;  // bit 0: prefetch the next plane, bit 1: non-temporal stores
+    int split_by = 8;    // split kernels: workgroup rows
+    int split_lz = 2;    // split kernels: planes per thread
+    // fused-kernel tile lists (interior / CPML-touching)
+    DevBuf d_tiles_inner, d_tiles_pml;
+    int n_inner = 0, n_pml = 0, ntx = 0, nty = 0;
+    int tiles_key[3] = {0, 0, 0};
+    bool tiles_ok = false;
     // sources / receivers
     std::vector<Src> src;
     std::vector<ObsLists> obs;
@@ -187,16 +195,18 @@ ems_status setup_pml(ems_fdtd *s)
     const GridView &g = s->g;
     const int nn[3] = {g.nx, g.ny, g.nz};
     const int P = s->npml, W = 2 * P + 1;
-    // 18 tables (kh bh ah ke be ae) x 3 axes, each of length maxn+2
+    // packed grading tables: [type H/E][axis][position], each (b, a, 1/kappa, 0)
     const int len = std::max({g.nx, g.ny, g.nz}) + 2;
-    std::vector<double> tab((size_t)18 * len, 0.0);
+    std::vector<double> tab((size_t)2 * 3 * len * 4, 0.0);
     for (int ax = 0; ax < 3; ++ax)
         for (int I = 0; I < nn[ax] + 2; ++I) {
             double kh, bh, ah, ke, be, ae;
             cpml_profile(s, nn[ax], I * s->d.h, kh, bh, ah);
             cpml_profile(s, nn[ax], (I - 0.5) * s->d.h, ke, be, ae);
-            double v[6] = {kh, bh, ah, ke, be, ae};
-            for (int t = 0; t < 6; ++t) tab[(size_t)(t * 3 + ax) * len + I] = v[t];
+            double *th = &tab[(((size_t)0 * 3 + ax) * len + I) * 4];
+            double *te = &tab[(((size_t)1 * 3 + ax) * len + I) * 4];
+            th[0] = bh; th[1] = ah; th[2] = kh;
+            te[0] = be; te[1] = ae; te[2] = ke;
         }
     if (s->d.precision == EMS_F32) EMS_TRY(upload_as<float>(s->prof, tab));
     else EMS_TRY(upload_as<double>(s->prof, tab));
@@ -243,14 +253,10 @@ template <typename T> PmlView<T> pml_view(ems_fdtd *s)
     v.W = 2 * v.P + 1;
     if (s->npml <= 0) return v;
     const int len = std::max({s->g.nx, s->g.ny, s->g.nz}) + 2;
-    const T *t = s->prof.as<T>();
+    const Prof<T> *t = reinterpret_cast<const Prof<T> *>(s->prof.p);
     for (int ax = 0; ax < 3; ++ax) {
-        v.kh[ax] = t + (size_t)(0 * 3 + ax) * len;
-        v.bh[ax] = t + (size_t)(1 * 3 + ax) * len;
-        v.ah[ax] = t + (size_t)(2 * 3 + ax) * len;
-        v.ke[ax] = t + (size_t)(3 * 3 + ax) * len;
-        v.be[ax] = t + (size_t)(4 * 3 + ax) * len;
-        v.ae[ax] = t + (size_t)(5 * 3 + ax) * len;
+        v.hp[ax] = t + (size_t)(0 * 3 + ax) * len;
+        v.ep[ax] = t + (size_t)(1 * 3 + ax) * len;
         v.psi_h[ax][0] = s->psi[ax * 4 + 0].as<T>();
         v.psi_h[ax][1] = s->psi[ax * 4 + 1].as<T>();
         v.psi_e[ax][0] = s->psi[ax * 4 + 2].as<T>();
@@ -269,26 +275,76 @@ template <typename T, int MAT, bool PML>
 void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     const GridView &g = s->g;
-    dim3 blk(64, 4, 1);
-    dim3 grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    const int by = s->split_by, lz = s->split_lz;
+    // CPML: the x-slab columns get wavefronts of their own
+    const int P = PML ? s->npml : 0;
+    const int npw = PML ? (2 * P + 1 + 63) / 64 : 0;
+    const int nbx = PML ? npw + (g.nx - 2 * P + 63) / 64 : (g.nx + 1 + 63) / 64;
+    dim3 blk(64, by, 1);
+    dim3 grd(nbx, (g.ny + 1 + by - 1) / by, (g.nz + 1 + lz - 1) / lz);
     Fields<T> f = fields<T>(s);
-    k_h_split<T, PML><<<grd, blk, 0, s->stream>>>(g, ch, f, pm);
-    k_e_split<T, MAT, PML><<<grd, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm);
+    k_h_split<T, PML><<<grd, blk, 0, s->stream>>>(g, ch, f, pm, lz, npw);
+    k_e_split<T, MAT, PML><<<grd, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm, lz, npw);
 }
 
-template <typename T, int BY, int MAT, bool PML>
-void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+// Tile lists of the fused kernel: tiles whose cells (halos included) touch
+// the CPML, and the others.  Rebuilt when the tiling or the CPML changes.
+ems_status build_tiles(ems_fdtd *s)
 {
     const GridView &g = s->g;
-    const int lz = s->tile_lz;
+    const int R = s->tile_by, lz = s->tile_lz, P = s->npml;
+    if (s->tiles_ok && s->tiles_key[0] == R && s->tiles_key[1] == lz && s->tiles_key[2] == P) return EMS_OK;
     const int ntx = (g.nx + 1 + 62) / 63;
-    const int nty = (g.ny + 1 + BY - 2) / (BY - 1);
+    const int rows = R - 1;  // owned rows per workgroup (row 0 is the halo)
+    const int nty = (g.ny + 1 + rows - 1) / rows;
     const int nkc = (g.nz + 1 + lz - 1) / lz;
-    const int nblk = ntx * nty * nkc;
-    const int grid = (nblk + 7) / 8 * 8;
+    std::vector<int> inner, pml;
+    auto touches = [&](int a, int b, int n) { return P > 0 && (a <= P || b >= n - P + 1); };
+    for (int kc = 0; kc < nkc; ++kc)
+        for (int tj = 0; tj < nty; ++tj)
+            for (int ti = 0; ti < ntx; ++ti) {
+                const int i0 = std::max(ti * 63 - 1, 0), i1 = std::min(ti * 63 + 62, g.nx);
+                const int j0 = std::max(tj * rows - 1, 0), j1 = std::min(tj * rows + rows - 1, g.ny);
+                const int k0 = std::max(kc * lz - 1, 0), k1 = std::min(kc * lz + lz - 1, g.nz);
+                const int id = ti + ntx * (tj + nty * kc);
+                if (touches(i0, i1, g.nx) || touches(j0, j1, g.ny) || touches(k0, k1, g.nz)) pml.push_back(id);
+                else inner.push_back(id);
+            }
+    s->ntx = ntx;
+    s->nty = nty;
+    s->n_inner = (int)inner.size();
+    s->n_pml = (int)pml.size();
+    if (!inner.empty()) {
+        EMS_TRY(s->d_tiles_inner.reserve(sizeof(int) * inner.size()));
+        EMS_HIP(hipMemcpy(s->d_tiles_inner.p, inner.data(), sizeof(int) * inner.size(), hipMemcpyHostToDevice));
+    }
+    if (!pml.empty()) {
+        EMS_TRY(s->d_tiles_pml.reserve(sizeof(int) * pml.size()));
+        EMS_HIP(hipMemcpy(s->d_tiles_pml.p, pml.data(), sizeof(int) * pml.size(), hipMemcpyHostToDevice));
+    }
+    s->tiles_key[0] = R;
+    s->tiles_key[1] = lz;
+    s->tiles_key[2] = P;
+    s->tiles_ok = true;
+    return EMS_OK;
+}
+
+template <typename T, int BY, int MAT, bool PML, bool PF, bool NT>
+void launch_tiles(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const PsiH<T> &ph, const int *list, int count)
+{
+    if (count <= 0) return;
+    TileSet ts{list, count, s->ntx, s->nty, s->tile_lz};
+    const int grid = (count + 7) / 8 * 8;
     Fields<T> o = fields<T>(s);
     Fields<T> nw{s->f2[0].as<T>(), s->f2[1].as<T>(), s->f2[2].as<T>(),
                  s->f2[3].as<T>(), s->f2[4].as<T>(), s->f2[5].as<T>()};
+    k_fused<T, BY, MAT, PML, PF, NT><<<grid, dim3(64, BY, 1), 0, s->stream>>>(s->g, ch, ce, o, nw, s->ca.as<T>(),
+                                                                      s->cb.as<T>(), pm, ph, ts);
+}
+
+template <typename T, int R, int MAT, bool PML>
+void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
     PsiH<T> ph{};
     if (PML)
         for (int ax = 0; ax < 3; ++ax)
@@ -296,8 +352,26 @@ void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
                 ph.old_[ax][q] = s->psi[ax * 4 + q].as<T>();
                 ph.new_[ax][q] = s->psi2[ax * 2 + q].as<T>();
             }
-    k_fused<T, BY, MAT, PML><<<grid, dim3(64, BY, 1), 0, s->stream>>>(
-        g, ch, ce, o, nw, s->ca.as<T>(), s->cb.as<T>(), pm, ph, lz, ntx, nty, nblk);
+    // interior tiles without CPML code, then the CPML tiles; both read the
+    // old buffers and write disjoint cells of the new ones
+    switch (s->tile_flags & 3) {
+    case 0:
+        launch_tiles<T, R, MAT, false, false, false>(s, ch, ce, pm, ph, s->d_tiles_inner.as<int>(), s->n_inner);
+        if (PML) launch_tiles<T, R, MAT, true, false, false>(s, ch, ce, pm, ph, s->d_tiles_pml.as<int>(), s->n_pml);
+        break;
+    case 1:
+        launch_tiles<T, R, MAT, false, true, false>(s, ch, ce, pm, ph, s->d_tiles_inner.as<int>(), s->n_inner);
+        if (PML) launch_tiles<T, R, MAT, true, true, false>(s, ch, ce, pm, ph, s->d_tiles_pml.as<int>(), s->n_pml);
+        break;
+    case 2:
+        launch_tiles<T, R, MAT, false, false, true>(s, ch, ce, pm, ph, s->d_tiles_inner.as<int>(), s->n_inner);
+        if (PML) launch_tiles<T, R, MAT, true, false, true>(s, ch, ce, pm, ph, s->d_tiles_pml.as<int>(), s->n_pml);
+        break;
+    default:
+        launch_tiles<T, R, MAT, false, true, true>(s, ch, ce, pm, ph, s->d_tiles_inner.as<int>(), s->n_inner);
+        if (PML) launch_tiles<T, R, MAT, true, true, true>(s, ch, ce, pm, ph, s->d_tiles_pml.as<int>(), s->n_pml);
+        break;
+    }
     // the new buffers become current
     for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
     if (PML)
@@ -307,14 +381,23 @@ void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 
 template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
-    if (s->tile_by == 4) launch_fused<T, 4, MAT, PML>(s, ch, ce, pm);
-    else if (s->tile_by == 16) launch_fused<T, 16, MAT, PML>(s, ch, ce, pm);
+    if (s->tile_by == 16) launch_fused<T, 16, MAT, PML>(s, ch, ce, pm);
     else launch_fused<T, 8, MAT, PML>(s, ch, ce, pm);
+}
+
+// EMS_KERNEL_AUTO: measured on MI355X (256^3), the split sweeps with the CPML
+// columns on wavefronts of their own are fastest with a CPML, the fused
+// sweep without one.
+bool use_fused(const ems_fdtd *s)
+{
+    if (s->kernel == EMS_KERNEL_SPLIT) return false;
+    if (s->kernel == EMS_KERNEL_FUSED) return true;
+    return s->npml <= 0;
 }
 
 template <typename T, int MAT, bool PML> void launch_variant(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
-    if (s->kernel == EMS_KERNEL_SPLIT) launch_split<T, MAT, PML>(s, ch, ce, pm);
+    if (!use_fused(s)) launch_split<T, MAT, PML>(s, ch, ce, pm);
     else launch_fused_by<T, MAT, PML>(s, ch, ce, pm);
 }
 
@@ -405,7 +488,10 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
 
     const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    if (s->kernel != EMS_KERNEL_SPLIT) EMS_TRY(ensure_alt(s));
+    if (use_fused(s)) {
+        EMS_TRY(ensure_alt(s));
+        EMS_TRY(build_tiles(s));
+    }
 
     // timed mode: one event pair around the Yee kernels of every step plus
     // one pair around the whole sequence; read back after the last step
@@ -684,14 +770,18 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant)
     return EMS_OK;
 }
 
-ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes)
+ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes, int flags)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (rows != 4 && rows != 8 && rows != 16) return invalid("tile rows must be 4, 8 or 16");
+    if (rows != 8 && rows != 16) return invalid("tile rows must be 8 or 16");
+    if (flags < 0 || flags > 3) return invalid("tile flags must be 0..3");
+    s->tile_flags = flags;
     if (planes < 1) return invalid("tile planes must be >= 1");
     s->tile_by = rows;
     s->tile_lz = planes;
+    s->split_lz = planes;
+    s->tiles_ok = false;
     return EMS_OK;
 }
 

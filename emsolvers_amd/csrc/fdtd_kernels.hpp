@@ -38,9 +38,14 @@ template <typename T> struct Fields {
 // CPML: per-axis profile tables indexed by I, J or K (H-type positions at
 // I*h, E-type positions at (I - 1/2)*h) and compact psi slabs, 2P+1 planes
 // thick per axis: planes I <= P and I >= n - P + 1.
+// CPML grading at one position: psi <- b psi + a t, term = t k + psi (k = 1/kappa)
+template <typename T> struct alignas(4 * sizeof(T)) Prof {
+    T b, a, k, pad;
+};
+
 template <typename T> struct PmlView {
     int P, W;
-    const T *kh[3], *bh[3], *ah[3], *ke[3], *be[3], *ae[3];
+    const Prof<T> *hp[3], *ep[3];  // H-type (I*h) and E-type ((I-1/2)*h) positions
     // psi_h[axis][0/1], psi_e[axis][0/1]:
     //   x: hy_x hz_x | ey_x ez_x    y: hx_y hz_y | ex_y ez_y    z: hx_z hy_z | ex_z ey_z
     T *psi_h[3][2];
@@ -66,11 +71,11 @@ __device__ __forceinline__ long slab_z(const GridView &g, int P, int W, int I, i
 }
 
 // psi <- b*psi + a*t ; returns t/kappa + psi  (Roden & Gedney recursion)
-template <typename T> __device__ __forceinline__ T cpml_term(T *psi, long q, T b, T a, T kinv, T t)
+template <typename T> __device__ __forceinline__ T cpml_term(T *psi, long q, const Prof<T> pr, T t)
 {
-    T p = b * psi[q] + a * t;
+    T p = pr.b * psi[q] + pr.a * t;
     psi[q] = p;
-    return t * kinv + p;
+    return t * pr.k + p;
 }
 
 // E <- update(E, bracket) for the three material modes
@@ -84,118 +89,229 @@ __device__ __forceinline__ T e_update(T e, T br, T ce, const T *__restrict__ ca,
 }
 
 // ---------------------------------------------------------------------------
-// Split kernels: one thread per (I, J, K), H sweep then E sweep, in place.
-// Block 64 x 4, grid (ceil((nx+1)/64), ceil((ny+1)/4), nz+1).
+// Split kernels: an H sweep then an E sweep, in place.  Each thread owns one
+// (I, J) column and marches up a chunk of lz planes in K, carrying the
+// K-neighbour values in registers (each field word is loaded from HBM once
+// per sweep); I and J neighbours are served by L1 (same wavefront / same
+// workgroup).  Workgroup 64 x BY, grid (ceil((nx+1)/64), ceil((ny+1)/BY),
+// ceil((nz+1)/lz)).
 // ---------------------------------------------------------------------------
-template <typename T, bool PML>
-__global__ __launch_bounds__(256) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm)
+// Column of a split-kernel thread.  With a CPML the 2P+1 columns of the two
+// x-slabs (I <= P, I >= nx-P+1) are packed into the first npw wavefronts of
+// every row and the interior columns follow, so that no wavefront mixes CPML
+// and interior lanes (the CPML branch would otherwise run for most rows).
+// Returns -1 for idle lanes.
+__device__ __forceinline__ int split_column(const GridView &g, int P, int npw, int bx, int tx)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    bool px = false, py = false, pz = false;
+    if (npw == 0) return bx * 64 + tx;
+    if (bx < npw) {
+        const int L = bx * 64 + tx;
+        if (L > 2 * P) return -1;
+        return L <= P ? L : g.nx - 2 * P + L;
+    }
+    const int I = P + 1 + (bx - npw) * 64 + tx;
+    return I <= g.nx - P ? I : -1;
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw)
+{
+    const int I = split_column(g, pm.P, npw, blockIdx.x, threadIdx.x);
+    // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
+    const int K0 = blockIdx.z * lz;
+    const int K1 = min(K0 + lz, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    const T *__restrict__ ex = f.ex;
+    const T *__restrict__ ey = f.ey;
+    const T *__restrict__ ez = f.ez;
+    T *__restrict__ hx = f.hx;
+    T *__restrict__ hy = f.hy;
+    T *__restrict__ hz = f.hz;
+    const long P = g.pitch, S = g.slab;
+    long u = g.off + I + P * J + S * K0;
+    bool px = false, py = false;
     if (PML) {
         px = in_pml(pm.P, g.nx, I);
         py = in_pml(pm.P, g.ny, J);
-        pz = in_pml(pm.P, g.nz, K);
     }
-    if (I >= 1) {  // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
-        T a = f.ez[u + g.pitch], b = f.ez[u], c = f.ey[u + g.slab], d = f.ey[u];
-        T br;
-        if (PML && (py || pz)) {
-            T t1 = a - b, t2 = c - d;
-            if (py) t1 = cpml_term(pm.psi_h[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.bh[1][J], pm.ah[1][J], pm.kh[1][J], t1);
-            if (pz) t2 = cpml_term(pm.psi_h[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.bh[2][K], pm.ah[2][K], pm.kh[2][K], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+    T ex_k = ex[u], ey_k = ey[u];
+    // values of the current plane; the next plane is loaded before the stores
+    T ex_k1 = ex[u + S], ey_k1 = ey[u + S], ez_k = ez[u], ez_j1 = ez[u + P], ez_i1 = ez[u + 1];
+    T ey_i1 = ey[u + 1], ex_j1 = ex[u + P], hxo = hx[u], hyo = hy[u], hzo = hz[u];
+    for (int K = K0; K < K1; ++K, u += S) {
+        T n_ex_k1 = 0, n_ey_k1 = 0, n_ez_k = 0, n_ez_j1 = 0, n_ez_i1 = 0, n_ey_i1 = 0, n_ex_j1 = 0;
+        T n_hx = 0, n_hy = 0, n_hz = 0;
+        if (K + 1 < K1) {
+            const long v = u + S;
+            n_ex_k1 = ex[v + S];
+            n_ey_k1 = ey[v + S];
+            n_ez_k = ez[v];
+            n_ez_j1 = ez[v + P];
+            n_ez_i1 = ez[v + 1];
+            n_ey_i1 = ey[v + 1];
+            n_ex_j1 = ex[v + P];
+            n_hx = hx[v];
+            n_hy = hy[v];
+            n_hz = hz[v];
         }
-        f.hx[u] -= ch * br;
-    }
-    if (J >= 1) {  // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
-        T a = f.ex[u + g.slab], b = f.ex[u], c = f.ez[u + 1], d = f.ez[u];
-        T br;
-        if (PML && (pz || px)) {
-            T t1 = a - b, t2 = c - d;
-            if (pz) t1 = cpml_term(pm.psi_h[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.bh[2][K], pm.ah[2][K], pm.kh[2][K], t1);
-            if (px) t2 = cpml_term(pm.psi_h[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.bh[0][I], pm.ah[0][I], pm.kh[0][I], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        if (I >= 1) {  // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
+            const T a = ez_j1, b = ez_k, c = ey_k1, d = ey_k;
+            T br;
+            if (PML && (py || pz)) {
+                T t1 = a - b, t2 = c - d;
+                if (py) t1 = cpml_term(pm.psi_h[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.hp[1][J], t1);
+                if (pz) t2 = cpml_term(pm.psi_h[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.hp[2][K], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            hx[u] = hxo - ch * br;
         }
-        f.hy[u] -= ch * br;
-    }
-    if (K >= 1) {  // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
-        T a = f.ey[u + 1], b = f.ey[u], c = f.ex[u + g.pitch], d = f.ex[u];
-        T br;
-        if (PML && (px || py)) {
-            T t1 = a - b, t2 = c - d;
-            if (px) t1 = cpml_term(pm.psi_h[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.bh[0][I], pm.ah[0][I], pm.kh[0][I], t1);
-            if (py) t2 = cpml_term(pm.psi_h[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.bh[1][J], pm.ah[1][J], pm.kh[1][J], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+        if (J >= 1) {  // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
+            const T a = ex_k1, b = ex_k, c = ez_i1, d = ez_k;
+            T br;
+            if (PML && (pz || px)) {
+                T t1 = a - b, t2 = c - d;
+                if (pz) t1 = cpml_term(pm.psi_h[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.hp[2][K], t1);
+                if (px) t2 = cpml_term(pm.psi_h[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.hp[0][I], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            hy[u] = hyo - ch * br;
         }
-        f.hz[u] -= ch * br;
+        if (K >= 1) {  // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
+            const T a = ey_i1, b = ey_k, c = ex_j1, d = ex_k;
+            T br;
+            if (PML && (px || py)) {
+                T t1 = a - b, t2 = c - d;
+                if (px) t1 = cpml_term(pm.psi_h[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.hp[0][I], t1);
+                if (py) t2 = cpml_term(pm.psi_h[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.hp[1][J], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            hz[u] = hzo - ch * br;
+        }
+        ex_k = ex_k1;
+        ey_k = ey_k1;
+        ex_k1 = n_ex_k1;
+        ey_k1 = n_ey_k1;
+        ez_k = n_ez_k;
+        ez_j1 = n_ez_j1;
+        ez_i1 = n_ez_i1;
+        ey_i1 = n_ey_i1;
+        ex_j1 = n_ex_j1;
+        hxo = n_hx;
+        hyo = n_hy;
+        hzo = n_hz;
     }
 }
 
 template <typename T, int MAT, bool PML>
-__global__ __launch_bounds__(256) void k_e_split(GridView g, T ce, Fields<T> f,
-                                                 const T *__restrict__ ca, const T *__restrict__ cb,
-                                                 PmlView<T> pm)
+__global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
+                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    bool px = false, py = false, pz = false;
+    const int I = split_column(g, pm.P, npw, blockIdx.x, threadIdx.x);
+    // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
+    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
+    const int K0 = blockIdx.z * lz;
+    const int K1 = min(K0 + lz, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    T *__restrict__ ex = f.ex;
+    T *__restrict__ ey = f.ey;
+    T *__restrict__ ez = f.ez;
+    const T *__restrict__ hx = f.hx;
+    const T *__restrict__ hy = f.hy;
+    const T *__restrict__ hz = f.hz;
+    const long P = g.pitch, S = g.slab;
+    long u = g.off + I + P * J + S * K0;
+    bool px = false, py = false;
     if (PML) {
         px = in_pml(pm.P, g.nx, I);
         py = in_pml(pm.P, g.ny, J);
-        pz = in_pml(pm.P, g.nz, K);
     }
-    if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
-        T a = f.hz[u], b = f.hz[u - g.pitch], c = f.hy[u], d = f.hy[u - g.slab];
-        T br;
-        if (PML && (py || pz)) {
-            T t1 = a - b, t2 = c - d;
-            if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t1);
-            if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+    // H at K-1 (zero below the grid: never used there)
+    T hx_km1 = K0 >= 1 ? hx[u - S] : (T)0, hy_km1 = K0 >= 1 ? hy[u - S] : (T)0;
+    const long um = u - (I >= 1 ? 1 : 0), up = u - (J >= 1 ? P : 0);  // clamped neighbour bases
+    long dm = um - u, dp = up - u;
+    T hx_k = hx[u], hy_k = hy[u], hz_k = hz[u], hz_jm = hz[u + dp], hy_im = hy[u + dm], hz_im = hz[u + dm];
+    T hx_jm = hx[u + dp], exo = ex[u], eyo = ey[u], ezo = ez[u];
+    T cbv = MAT >= 1 ? cb[u] : (T)0, cav = MAT == 2 ? ca[u] : (T)0;
+    for (int K = K0; K < K1; ++K, u += S) {
+        T n_hx = 0, n_hy = 0, n_hz = 0, n_hz_jm = 0, n_hy_im = 0, n_hz_im = 0, n_hx_jm = 0;
+        T n_ex = 0, n_ey = 0, n_ez = 0, n_cb = 0, n_ca = 0;
+        if (K + 1 < K1) {
+            const long v = u + S;
+            n_hx = hx[v];
+            n_hy = hy[v];
+            n_hz = hz[v];
+            n_hz_jm = hz[v + dp];
+            n_hy_im = hy[v + dm];
+            n_hz_im = hz[v + dm];
+            n_hx_jm = hx[v + dp];
+            n_ex = ex[v];
+            n_ey = ey[v];
+            n_ez = ez[v];
+            if (MAT >= 1) n_cb = cb[v];
+            if (MAT == 2) n_ca = ca[v];
         }
-        f.ex[u] = e_update<T, MAT>(f.ex[u], br, ce, ca, cb, u);
-    }
-    if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
-        T a = f.hx[u], b = f.hx[u - g.slab], c = f.hz[u], d = f.hz[u - 1];
-        T br;
-        if (PML && (pz || px)) {
-            T t1 = a - b, t2 = c - d;
-            if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t1);
-            if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
+            const T a = hz_k, b = hz_jm, c = hy_k, d = hy_km1;
+            T br;
+            if (PML && (py || pz)) {
+                T t1 = a - b, t2 = c - d;
+                if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t1);
+                if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            ex[u] = MAT == 0 ? exo + ce * br : MAT == 1 ? exo + cbv * br : cav * exo + cbv * br;
         }
-        f.ey[u] = e_update<T, MAT>(f.ey[u], br, ce, ca, cb, u);
-    }
-    if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
-        T a = f.hy[u], b = f.hy[u - 1], c = f.hx[u], d = f.hx[u - g.pitch];
-        T br;
-        if (PML && (px || py)) {
-            T t1 = a - b, t2 = c - d;
-            if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t1);
-            if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t2);
-            br = t1 - t2;
-        } else {
-            br = a - b - c + d;
+        if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
+            const T a = hx_k, b = hx_km1, c = hz_k, d = hz_im;
+            T br;
+            if (PML && (pz || px)) {
+                T t1 = a - b, t2 = c - d;
+                if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t1);
+                if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            ey[u] = MAT == 0 ? eyo + ce * br : MAT == 1 ? eyo + cbv * br : cav * eyo + cbv * br;
         }
-        f.ez[u] = e_update<T, MAT>(f.ez[u], br, ce, ca, cb, u);
+        if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
+            const T a = hy_k, b = hy_im, c = hx_k, d = hx_jm;
+            T br;
+            if (PML && (px || py)) {
+                T t1 = a - b, t2 = c - d;
+                if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t1);
+                if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t2);
+                br = t1 - t2;
+            } else {
+                br = a - b - c + d;
+            }
+            ez[u] = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
+        }
+        hx_km1 = hx_k;
+        hy_km1 = hy_k;
+        hx_k = n_hx;
+        hy_k = n_hy;
+        hz_k = n_hz;
+        hz_jm = n_hz_jm;
+        hy_im = n_hy_im;
+        hz_im = n_hz_im;
+        hx_jm = n_hx_jm;
+        exo = n_ex;
+        eyo = n_ey;
+        ezo = n_ez;
+        cbv = n_cb;
+        cav = n_ca;
     }
 }
 
@@ -231,50 +347,96 @@ namespace ems {
 namespace fdtd {
 
 // ---------------------------------------------------------------------------
-// Fused single-sweep kernel.
+// Fused single-sweep kernel, one independent wavefront per tile.
 //
 // One launch per time step reads E^n, H^{n-1/2} from the "old" buffers and
 // writes H^{n+1/2}, E^{n+1} into the "new" buffers (ping-pong), so every
-// field word is read once and written once per step.  A workgroup of
-// 64 x BY threads owns a 63 x (BY-1) tile of (I, J) columns and marches up a
-// chunk of lz planes in K.  E(K) needs H^{n+1/2} at I-1, J-1 and K-1, so the
-// workgroup also computes H on a one-cell halo on the low side: lane 0 (I-1),
-// row 0 (J-1) and the plane below the chunk (K0-1); these are recomputed from
-// the old buffers and never stored.  H at I-1 / J-1 is exchanged through LDS
-// (double-buffered by plane parity: one barrier per plane), H at K-1 and
-// E^n at K+1 are carried in registers.  CPML psi for H is ping-ponged as
-// well (halo threads read the old values); psi for E is updated in place by
-// the owning thread.  Arithmetic per output is identical to the split
-// kernels, so both give the same bits.
+// field word crosses HBM once in each direction per step.  A 64-lane wave
+// owns 63 columns (I), R rows (J) and a chunk of lz planes (K) and marches
+// J inside each plane and K across planes.  E(K) needs H^{n+1/2} at I-1, J-1
+// and K-1:
+//   I-1 : from the neighbouring lane (__shfl_up), lane 0 is a halo column,
+//   J-1 : from the previous row in registers, row 0 is a halo row,
+//   K-1 : from the previous plane in registers, the plane below the chunk is
+//         a halo plane;
+// halo H is recomputed from the old buffers and never stored.  No LDS and no
+// barriers: waves are independent, so load latency is hidden by occupancy.
+// CPML psi for H is ping-ponged too (halo recomputation reads old values);
+// psi for E is updated in place by the owning lane.  Tiles that touch the
+// CPML run a PML=true instantiation, the others one without CPML code.  The
+// arithmetic per output is that of the split kernels: same bits.
 // ---------------------------------------------------------------------------
 template <typename T> struct PsiH {
     const T *old_[3][2];
     T *new_[3][2];
 };
 
-template <typename T, int BY, int MAT, bool PML>
+struct TileSet {
+    const int *list;  // tile ids (nullptr: identity)
+    int count;        // tiles in this launch
+    int ntx, nty;     // tiles along I and J
+    int lz;           // planes per tile
+};
+
+// Loads of one plane for the fused kernel (own cell, +I, +J neighbours).
+template <typename T> struct PlaneVals {
+    T ex_k1, ey_k1, ez_k, ez_i1, ey_i1, ez_j1, ex_j1, hx, hy, hz;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_plane(const Fields<T> &o, const GridView &g, long u, PlaneVals<T> &v)
+{
+    v.ex_k1 = o.ex[u + g.slab];
+    v.ey_k1 = o.ey[u + g.slab];
+    v.ez_k = o.ez[u];
+    v.ez_i1 = o.ez[u + 1];
+    v.ey_i1 = o.ey[u + 1];
+    v.ez_j1 = o.ez[u + g.pitch];
+    v.ex_j1 = o.ex[u + g.pitch];
+    v.hx = o.hx[u];
+    v.hy = o.hy[u];
+    v.hz = o.hz[u];
+}
+
+template <bool NT, typename T> __device__ __forceinline__ void st_f(T *p, T v)
+{
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// A workgroup of 64 x BY threads owns a 63 x (BY-1) tile of (I, J) columns
+// over a chunk of lz planes; lane 0 and row 0 recompute the I-1 / J-1 halo
+// of H, the first iteration the K-1 halo plane.  H^{n+1/2} at I-1 / J-1 is
+// exchanged through LDS (double-buffered by plane parity: one barrier per
+// plane); the next plane's loads are issued before the barrier.
+template <typename T, int BY, int MAT, bool PML, bool PF, bool NT>
 __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Fields<T> o, Fields<T> nw,
                                                     const T *__restrict__ ca, const T *__restrict__ cb,
-                                                    PmlView<T> pm, PsiH<T> ph, int lz, int ntx, int nty,
-                                                    int nblk)
+                                                    PmlView<T> pm, PsiH<T> ph, TileSet ts)
 {
     __shared__ T sH[2][3][BY][64];
-    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs;
-    // give each XCD a contiguous range of tiles so halos are shared in its L2.
     const int per = gridDim.x >> 3;
     const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (lb >= nblk) return;  // uniform for the whole workgroup
-    const int ti = lb % ntx;
-    const int tj = (lb / ntx) % nty;
-    const int kc = lb / (ntx * nty);
+    if (lb >= ts.count) return;  // uniform for the whole workgroup
+    const int tile = ts.list ? ts.list[lb] : lb;
+    const int ti = tile % ts.ntx;
+    const int tj = (tile / ts.ntx) % ts.nty;
+    const int kc = tile / (ts.ntx * ts.nty);
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int I = ti * 63 - 1 + tx;
-    const int J = tj * (BY - 1) - 1 + ty;
-    const int K0 = kc * lz;
-    const int K1 = min(K0 + lz, g.nz + 1);
+    const int J = __builtin_amdgcn_readfirstlane(tj * (BY - 1) - 1 + ty);  // wave-uniform
+    const int K0 = kc * ts.lz;
+    const int K1 = min(K0 + ts.lz, g.nz + 1);
     const bool act = I >= 0 && J >= 0 && I <= g.nx && J <= g.ny;
     const bool own = act && tx >= 1 && ty >= 1;
     const int Kst = K0 > 0 ? K0 - 1 : 0;
+    // old and new buffers never alias: let the compiler move loads past stores
+    T *__restrict__ nex = nw.ex;
+    T *__restrict__ ney = nw.ey;
+    T *__restrict__ nez = nw.ez;
+    T *__restrict__ nhx = nw.hx;
+    T *__restrict__ nhy = nw.hy;
+    T *__restrict__ nhz = nw.hz;
     long u = g.off + (long)(act ? I : 0) + g.pitch * (act ? J : 0) + g.slab * Kst;
     bool px = false, py = false;
     if (PML) {
@@ -282,78 +444,85 @@ __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Field
         py = in_pml(pm.P, g.ny, J);
     }
     T ex_k = 0, ey_k = 0, hx_km1 = 0, hy_km1 = 0;
+    PlaneVals<T> cur{}, nxt{};
     if (act) {
         ex_k = o.ex[u];
         ey_k = o.ey[u];
+        load_plane(o, g, u, cur);
     }
     for (int K = Kst; K < K1; ++K) {
         const int par = K & 1;
         const bool wr = own && K >= K0;
-        T hxn = 0, hyn = 0, hzn = 0, ez_k = 0, ex_k1 = 0, ey_k1 = 0;
+        // issue the next plane's loads early: they overlap the barrier and E
+        if (PF && act && K + 1 < K1) load_plane(o, g, u + g.slab, nxt);
+        T cbu = 0, cau = 0;
+        if (MAT >= 1 && wr) cbu = cb[u];
+        if (MAT == 2 && wr) cau = ca[u];
+        T hxn = 0, hyn = 0, hzn = 0;
         const bool pz = PML && in_pml(pm.P, g.nz, K);
         if (act) {
-            ex_k1 = o.ex[u + g.slab];
-            ey_k1 = o.ey[u + g.slab];
-            ez_k = o.ez[u];
-            const T ez_i1 = o.ez[u + 1], ey_i1 = o.ey[u + 1];
-            const T ez_j1 = o.ez[u + g.pitch], ex_j1 = o.ex[u + g.pitch];
-            const T hx = o.hx[u], hy = o.hy[u], hz = o.hz[u];
             // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
             if (PML && (py || pz)) {
-                T t1 = ez_j1 - ez_k, t2 = ey_k1 - ey_k;
+                T t1 = cur.ez_j1 - cur.ez_k, t2 = cur.ey_k1 - ey_k;
                 if (py) {
                     long q = slab_y(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[1][J] * ph.old_[1][0][q] + pm.ah[1][J] * t1;
+                    const Prof<T> pr = pm.hp[1][J];
+                    T p = pr.b * ph.old_[1][0][q] + pr.a * t1;
                     if (wr) ph.new_[1][0][q] = p;
-                    t1 = t1 * pm.kh[1][J] + p;
+                    t1 = t1 * pr.k + p;
                 }
                 if (pz) {
                     long q = slab_z(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[2][K] * ph.old_[2][0][q] + pm.ah[2][K] * t2;
+                    const Prof<T> pr = pm.hp[2][K];
+                    T p = pr.b * ph.old_[2][0][q] + pr.a * t2;
                     if (wr) ph.new_[2][0][q] = p;
-                    t2 = t2 * pm.kh[2][K] + p;
+                    t2 = t2 * pr.k + p;
                 }
-                hxn = hx - ch * (t1 - t2);
+                hxn = cur.hx - ch * (t1 - t2);
             } else {
-                hxn = hx - ch * (ez_j1 - ez_k - ey_k1 + ey_k);
+                hxn = cur.hx - ch * (cur.ez_j1 - cur.ez_k - cur.ey_k1 + ey_k);
             }
             // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
             if (PML && (pz || px)) {
-                T t1 = ex_k1 - ex_k, t2 = ez_i1 - ez_k;
+                T t1 = cur.ex_k1 - ex_k, t2 = cur.ez_i1 - cur.ez_k;
                 if (pz) {
                     long q = slab_z(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[2][K] * ph.old_[2][1][q] + pm.ah[2][K] * t1;
+                    const Prof<T> pr = pm.hp[2][K];
+                    T p = pr.b * ph.old_[2][1][q] + pr.a * t1;
                     if (wr) ph.new_[2][1][q] = p;
-                    t1 = t1 * pm.kh[2][K] + p;
+                    t1 = t1 * pr.k + p;
                 }
                 if (px) {
                     long q = slab_x(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[0][I] * ph.old_[0][0][q] + pm.ah[0][I] * t2;
+                    const Prof<T> pr = pm.hp[0][I];
+                    T p = pr.b * ph.old_[0][0][q] + pr.a * t2;
                     if (wr) ph.new_[0][0][q] = p;
-                    t2 = t2 * pm.kh[0][I] + p;
+                    t2 = t2 * pr.k + p;
                 }
-                hyn = hy - ch * (t1 - t2);
+                hyn = cur.hy - ch * (t1 - t2);
             } else {
-                hyn = hy - ch * (ex_k1 - ex_k - ez_i1 + ez_k);
+                hyn = cur.hy - ch * (cur.ex_k1 - ex_k - cur.ez_i1 + cur.ez_k);
             }
             // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
             if (PML && (px || py)) {
-                T t1 = ey_i1 - ey_k, t2 = ex_j1 - ex_k;
+                T t1 = cur.ey_i1 - ey_k, t2 = cur.ex_j1 - ex_k;
                 if (px) {
                     long q = slab_x(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[0][I] * ph.old_[0][1][q] + pm.ah[0][I] * t1;
+                    const Prof<T> pr = pm.hp[0][I];
+                    T p = pr.b * ph.old_[0][1][q] + pr.a * t1;
                     if (wr) ph.new_[0][1][q] = p;
-                    t1 = t1 * pm.kh[0][I] + p;
+                    t1 = t1 * pr.k + p;
                 }
                 if (py) {
                     long q = slab_y(g, pm.P, pm.W, I, J, K);
-                    T p = pm.bh[1][J] * ph.old_[1][1][q] + pm.ah[1][J] * t2;
+                    const Prof<T> pr = pm.hp[1][J];
+                    T p = pr.b * ph.old_[1][1][q] + pr.a * t2;
                     if (wr) ph.new_[1][1][q] = p;
-                    t2 = t2 * pm.kh[1][J] + p;
+                    t2 = t2 * pr.k + p;
                 }
-                hzn = hz - ch * (t1 - t2);
+                hzn = cur.hz - ch * (t1 - t2);
             } else {
-                hzn = hz - ch * (ey_i1 - ey_k - ex_j1 + ex_k);
+                hzn = cur.hz - ch * (cur.ey_i1 - ey_k - cur.ex_j1 + ex_k);
             }
         }
         sH[par][0][ty][tx] = hxn;
@@ -361,53 +530,55 @@ __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Field
         sH[par][2][ty][tx] = hzn;
         __syncthreads();
         if (wr) {
-            if (I >= 1) nw.hx[u] = hxn;
-            if (J >= 1) nw.hy[u] = hyn;
-            if (K >= 1) nw.hz[u] = hzn;
+            if (I >= 1) st_f<NT>(&nhx[u], hxn);
+            if (J >= 1) st_f<NT>(&nhy[u], hyn);
+            if (K >= 1) st_f<NT>(&nhz[u], hzn);
             const T hz_im1 = sH[par][2][ty][tx - 1], hy_im1 = sH[par][1][ty][tx - 1];
             const T hz_jm1 = sH[par][2][ty - 1][tx], hx_jm1 = sH[par][0][ty - 1][tx];
             if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
                 T br;
                 if (PML && (py || pz)) {
                     T t1 = hzn - hz_jm1, t2 = hyn - hy_km1;
-                    if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t1);
-                    if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t2);
+                    if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t1);
+                    if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t2);
                     br = t1 - t2;
                 } else {
                     br = hzn - hz_jm1 - hyn + hy_km1;
                 }
-                nw.ex[u] = e_update<T, MAT>(ex_k, br, ce, ca, cb, u);
+                st_f<NT>(&nex[u], MAT == 0 ? ex_k + ce * br : MAT == 1 ? ex_k + cbu * br : cau * ex_k + cbu * br);
             }
             if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
                 T br;
                 if (PML && (pz || px)) {
                     T t1 = hxn - hx_km1, t2 = hzn - hz_im1;
-                    if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.be[2][K], pm.ae[2][K], pm.ke[2][K], t1);
-                    if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t2);
+                    if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t1);
+                    if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t2);
                     br = t1 - t2;
                 } else {
                     br = hxn - hx_km1 - hzn + hz_im1;
                 }
-                nw.ey[u] = e_update<T, MAT>(ey_k, br, ce, ca, cb, u);
+                st_f<NT>(&ney[u], MAT == 0 ? ey_k + ce * br : MAT == 1 ? ey_k + cbu * br : cau * ey_k + cbu * br);
             }
             if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
                 T br;
                 if (PML && (px || py)) {
                     T t1 = hyn - hy_im1, t2 = hxn - hx_jm1;
-                    if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.be[0][I], pm.ae[0][I], pm.ke[0][I], t1);
-                    if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.be[1][J], pm.ae[1][J], pm.ke[1][J], t2);
+                    if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t1);
+                    if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t2);
                     br = t1 - t2;
                 } else {
                     br = hyn - hy_im1 - hxn + hx_jm1;
                 }
-                nw.ez[u] = e_update<T, MAT>(ez_k, br, ce, ca, cb, u);
+                st_f<NT>(&nez[u], MAT == 0 ? cur.ez_k + ce * br : MAT == 1 ? cur.ez_k + cbu * br : cau * cur.ez_k + cbu * br);
             }
         }
-        ex_k = ex_k1;
-        ey_k = ey_k1;
+        ex_k = cur.ex_k1;
+        ey_k = cur.ey_k1;
         hx_km1 = hxn;
         hy_km1 = hyn;
         u += g.slab;
+        if (PF) cur = nxt;
+        else if (act && K + 1 < K1) load_plane(o, g, u, cur);
     }
 }
 

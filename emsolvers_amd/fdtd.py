@@ -113,9 +113,10 @@ class FemSimulation:
     def set_kernel(self, variant):
         _lib.check(_lib.load().ems_fdtd_set_kernel(self._h, int(variant)))
 
-    def set_tiling(self, rows, planes):
-        """Fused-kernel workgroup rows (4, 8, 16) and planes per workgroup."""
-        _lib.check(_lib.load().ems_fdtd_set_tiling(self._h, int(rows), int(planes)))
+    def set_tiling(self, rows, planes, flags=0):
+        """Fused-kernel workgroup rows (8, 16), planes per workgroup and flags
+        (bit 0 prefetch, bit 1 non-temporal stores)."""
+        _lib.check(_lib.load().ems_fdtd_set_tiling(self._h, int(rows), int(planes), int(flags)))
 
     def run_timed(self, num_steps):
         """Returns (total_ms, kernel_ms) measured with HIP events."""

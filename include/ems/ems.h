@@ -104,9 +104,11 @@ ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, dou
 /* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT or EMS_KERNEL_FUSED. */
 ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
 
-/* Fused-kernel tiling: workgroup rows (4, 8 or 16; 63 x (rows-1) cells per
- * plane) and planes per workgroup (>= 1).  Results do not depend on it. */
-ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes);
+/* Kernel tiling: fused-kernel workgroup rows (8 or 16; 63 x (rows-1) cells
+ * per plane), planes per workgroup or per split-kernel thread (>= 1) and
+ * flags (bit 0: prefetch the next plane into registers, bit 1: non-temporal
+ * stores).  Results do not depend on it. */
+ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes, int flags);
 
 /* FemSimulation::run (src/fdtd/fdtd_simulation.cpp:242-306): advances
  * num_steps steps on the device and records the receivers every tf steps

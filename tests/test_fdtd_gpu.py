@@ -111,15 +111,15 @@ def test_split_and_fused_agree_bitwise_fp32():
         assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
 
 
-@pytest.mark.parametrize("rows,planes", [(4, 1), (4, 7), (8, 32), (16, 3), (16, 1000)])
-def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes):
+@pytest.mark.parametrize("rows,planes,flags", [(8, 1, 0), (8, 7, 1), (8, 8, 2), (16, 3, 3), (16, 1000, 1)])
+def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes, flags):
     """Tile shape and K-chunking change only the halo recomputation, never the
     arithmetic: every tiling must give the oracle's FP64 bits."""
     case = fc.PARITY_CASES["lossy_cpml_rect"]
-    print("tiling", rows, planes, flush=True)
+    print("tiling", rows, planes, flags, flush=True)
     o = oracle_runs("lossy_cpml_rect")
     s = build_gpu(case, "f64", 2)
-    s.set_tiling(rows, planes)
+    s.set_tiling(rows, planes, flags)
     s.run(case["nsteps"])
     assert np.array_equal(s.traces(), o.traces())
     for w in range(6):
