@@ -69,28 +69,45 @@ void swap_slots(ems_fem *s, int a, int b)
     }
 }
 
+// 3-D launch over component COMP's array: workgroup 64 x 4
+template <int COMP> dim3 comp_grid(const ems_fem *s)
+{
+    const int dx = s->g.Nx + (COMP != 0), dy = s->g.Ny + (COMP != 1), dz = s->g.Nz + (COMP != 2);
+    return dim3((dx + 63) / 64, (dy + 3) / 4, dz);
+}
+
 template <typename T, int P>
 void launch_leapfrog(ems_fem *s, const double *sval)
 {
-    const long ndof = s->g.NX + s->g.NY + s->g.NZ;
-    const int blk = 256;
-    const long grd = (ndof + blk - 1) / blk;
-    k_fem_leapfrog<T, P><<<grd, blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), cc3<T>(s, s->t0),
-                                                    cc3<T>(s, s->t1), c3<T>(s, s->t2), (int)s->src_z.size(),
-                                                    s->d_src.as<long>(), sval);
+    const dim3 blk(64, 4, 1);
+    const CComp3<T> e0 = cc3<T>(s, s->t0), e1 = cc3<T>(s, s->t1);
+    const Comp3<T> e2 = c3<T>(s, s->t2);
+    const int ns = (int)s->src_z.size();
+    k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
+                                                                     e2, ns, s->d_src.as<long>(), sval);
+    k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
+                                                                     e2, ns, s->d_src.as<long>(), sval);
+    k_fem_leapfrog<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
+                                                                     e2, ns, s->d_src.as<long>(), sval);
 }
 
 template <typename T, int P>
 void launch_stage(ems_fem *s, double deltat, const double *sval)
 {
-    const long ndof = s->g.NX + s->g.NY + s->g.NZ;
-    const int blk = 256;
-    const long grd = (ndof + blk - 1) / blk;
     const double mm = 8.0 / s->g.p / s->g.p / s->g.p;
-    k_fem_symplectic<T, P><<<grd, blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), cc3<T>(s, 0),
-                                                      cc3<T>(s, 1), c3<T>(s, 2), c3<T>(s, 3), (T)deltat,
-                                                      (T)(0.5 * deltat), (T)mm, (int)s->src_z.size(),
-                                                      s->d_src.as<long>(), sval);
+    const dim3 blk(64, 4, 1);
+    const CComp3<T> u = cc3<T>(s, 0), pm = cc3<T>(s, 1);
+    const Comp3<T> uo = c3<T>(s, 2), po = c3<T>(s, 3);
+    const int ns = (int)s->src_z.size();
+    k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
+                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+                                                                       s->d_src.as<long>(), sval);
+    k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
+                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+                                                                       s->d_src.as<long>(), sval);
+    k_fem_symplectic<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
+                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+                                                                       s->d_src.as<long>(), sval);
     swap_slots(s, 0, 2);
     swap_slots(s, 1, 3);
 }

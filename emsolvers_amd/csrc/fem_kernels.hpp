@@ -51,9 +51,10 @@ struct Cand {
 };
 
 // Elements containing DOF (comp, X, Y, Z), sorted by rank.  Returns count.
+template <int P>
 __device__ __forceinline__ int dof_elements(const FemGrid &g, int comp, int X, int Y, int Z, Cand *out)
 {
-    const int p = g.p;
+    constexpr int p = P;
     // per axis: up to two (element index, sub-index) choices
     int ei[3][2], li[3][2], cnt[3];
     const int C[3] = {X, Y, Z};
@@ -117,31 +118,14 @@ __device__ __forceinline__ bool pec_dof(const FemGrid &g, int comp, int X, int Y
     return X == 0 || X == g.Nx || Y == 0 || Y == g.Ny;
 }
 
-// flat DOF index t over [Ex | Ey | Ez] -> component and fine coordinates
-__device__ __forceinline__ void dof_coords(const FemGrid &g, long t, int &comp, long &idx, int &X, int &Y, int &Z)
+// Extent of component COMP's array (x, y, z) and its coordinates from a 3-D
+// launch: workgroup 64 x 4, grid (ceil(nx/64), ceil(ny/4), nz).  No integer
+// division on the device.
+template <int COMP> __device__ __forceinline__ void comp_dims(const FemGrid &g, int &dx, int &dy, int &dz)
 {
-    if (t < g.NX) {
-        comp = 0;
-        idx = t;
-        X = (int)(t % g.Nx);
-        long r = t / g.Nx;
-        Y = (int)(r % (g.Ny + 1));
-        Z = (int)(r / (g.Ny + 1));
-    } else if (t < g.NX + g.NY) {
-        comp = 1;
-        idx = t - g.NX;
-        X = (int)(idx % (g.Nx + 1));
-        long r = idx / (g.Nx + 1);
-        Y = (int)(r % g.Ny);
-        Z = (int)(r / g.Ny);
-    } else {
-        comp = 2;
-        idx = t - g.NX - g.NY;
-        X = (int)(idx % (g.Nx + 1));
-        long r = idx / (g.Nx + 1);
-        Y = (int)(r % (g.Ny + 1));
-        Z = (int)(r / (g.Ny + 1));
-    }
+    dx = g.Nx + (COMP != 0);
+    dy = g.Ny + (COMP != 1);
+    dz = g.Nz + (COMP != 2);
 }
 
 template <typename T> struct Comp3 {
@@ -183,30 +167,32 @@ __device__ __forceinline__ T row_dot(const FemGrid &g, const T *__restrict__ K, 
 // fem_v2 leapfrog step (src/fem_v2/fem_simulation.cpp:560-655 + BC):
 //   E2 = ((0 - sum_e K_e E1) + (2 E1 - E0)) - src,  PEC-tangential DOFs -> 0
 // ---------------------------------------------------------------------------
-template <typename T, int P>
+template <typename T, int P, int COMP>
 __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__restrict__ K, const int *__restrict__ off,
                                                       CComp3<T> e0, CComp3<T> e1, Comp3<T> e2, int nsrc,
                                                       const long *__restrict__ src_idx, const double *__restrict__ src_val)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.NX + g.NY + g.NZ) return;
-    int comp, X, Y, Z;
-    long idx;
-    dof_coords(g, t, comp, idx, X, Y, Z);
-    T *out = comp == 0 ? e2.x : comp == 1 ? e2.y : e2.z;
-    if (pec_dof(g, comp, X, Y, Z)) {
+    int dx, dy, dz;
+    comp_dims<COMP>(g, dx, dy, dz);
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    if (X >= dx || Y >= dy) return;
+    const long idx = X + (long)dx * (Y + (long)dy * Z);
+    T *out = COMP == 0 ? e2.x : COMP == 1 ? e2.y : e2.z;
+    if (pec_dof(g, COMP, X, Y, Z)) {
         out[idx] = (T)0;
         return;
     }
     Cand c[4];
-    const int nc = dof_elements(g, comp, X, Y, Z, c);
+    const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
     T acc = 0;
     for (int q = 0; q < nc; ++q)
         acc -= row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, e1, e1, (T)0, (T)1);
-    const T *a1 = comp == 0 ? e1.x : comp == 1 ? e1.y : e1.z;
-    const T *a0 = comp == 0 ? e0.x : comp == 1 ? e0.y : e0.z;
+    const T *a1 = COMP == 0 ? e1.x : COMP == 1 ? e1.y : e1.z;
+    const T *a0 = COMP == 0 ? e0.x : COMP == 1 ? e0.y : e0.z;
     T v = acc + ((T)2 * a1[idx] - a0[idx]);
-    if (comp == 2)
+    if (COMP == 2)
         for (int s = 0; s < nsrc; ++s)
             if (src_idx[s] == idx) v -= (T)src_val[0];
     out[idx] = v;
@@ -220,29 +206,31 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__rest
 // Reads (U, P) from one buffer pair and writes the other; neighbours' drift
 // is recomputed in the gather, so one pass per stage.
 // ---------------------------------------------------------------------------
-template <typename T, int P>
+template <typename T, int P, int COMP>
 __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__restrict__ K,
                                                         const int *__restrict__ off, CComp3<T> u, CComp3<T> pm,
                                                         Comp3<T> u_out, Comp3<T> p_out, T deltat, T hd, T mm,
                                                         int nsrc, const long *__restrict__ src_idx,
                                                         const double *__restrict__ src_val)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.NX + g.NY + g.NZ) return;
-    int comp, X, Y, Z;
-    long idx;
-    dof_coords(g, t, comp, idx, X, Y, Z);
-    T *uo = comp == 0 ? u_out.x : comp == 1 ? u_out.y : u_out.z;
-    T *po = comp == 0 ? p_out.x : comp == 1 ? p_out.y : p_out.z;
-    if (pec_dof(g, comp, X, Y, Z)) {
+    int dx, dy, dz;
+    comp_dims<COMP>(g, dx, dy, dz);
+    const int X = blockIdx.x * 64 + threadIdx.x;
+    const int Y = blockIdx.y * 4 + threadIdx.y;
+    const int Z = blockIdx.z;
+    if (X >= dx || Y >= dy) return;
+    const long idx = X + (long)dx * (Y + (long)dy * Z);
+    T *uo = COMP == 0 ? u_out.x : COMP == 1 ? u_out.y : u_out.z;
+    T *po = COMP == 0 ? p_out.x : COMP == 1 ? p_out.y : p_out.z;
+    if (pec_dof(g, COMP, X, Y, Z)) {
         uo[idx] = (T)0;
         po[idx] = (T)0;
         return;
     }
     Cand c[4];
-    const int nc = dof_elements(g, comp, X, Y, Z, c);
+    const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
     int nsrc_here = 0;
-    if (comp == 2)
+    if (COMP == 2)
         for (int s = 0; s < nsrc; ++s) nsrc_here += src_idx[s] == idx;
     const T sv = (T)src_val[0];
     T tmp = 0;
@@ -256,8 +244,8 @@ __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__re
     }
     if (!src_done)
         for (int s = 0; s < nsrc_here; ++s) tmp += sv;
-    const T *ua = comp == 0 ? u.x : comp == 1 ? u.y : u.z;
-    const T *pa = comp == 0 ? pm.x : comp == 1 ? pm.y : pm.z;
+    const T *ua = COMP == 0 ? u.x : COMP == 1 ? u.y : u.z;
+    const T *pa = COMP == 0 ? pm.x : COMP == 1 ? pm.y : pm.z;
     const T u1 = ua[idx] + hd * pa[idx] / mm;
     const T p1 = pa[idx] - deltat * tmp;
     uo[idx] = u1 + hd * p1 / mm;
