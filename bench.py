@@ -149,7 +149,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
-              "(the multi-GPU halo design is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(multi-GPU runs would shard the grid into slabs with halo exchange, not built)"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: multi-rank launch is not supported", file=sys.stderr)
