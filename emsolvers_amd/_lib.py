@@ -36,6 +36,11 @@ class FemDesc(ctypes.Structure):
                 ("mu", _D), ("tf", _I), ("stepper", _I), ("precision", _I)]
 
 
+class Fem1Desc(ctypes.Structure):
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("h", _D), ("dt", _D), ("eps", _D), ("mu", _D),
+                ("tf", _I), ("stepper", _I), ("stab", _D), ("use_ofem", _I), ("precision", _I)]
+
+
 SIGNATURES = {
     "ems_last_error": (ctypes.c_char_p, []),
     "ems_version": (ctypes.c_char_p, []),
@@ -77,6 +82,24 @@ SIGNATURES = {
     "ems_fem_get_time": (_I, [_P, _DP]),
     "ems_fem_save_results": (_I, [_P, _I, ctypes.c_char_p]),
     "ems_fem_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
+    # first-order stabilized hex FEM (fem, fem_abm)
+    "ems_fem1_element_matrix": (_I, [ctypes.POINTER(Fem1Desc), _DP]),
+    "ems_fem1_create": (_I, [ctypes.POINTER(Fem1Desc), _PP]),
+    "ems_fem1_destroy": (None, [_P]),
+    "ems_fem1_add_source": (_I, [_P, _I, _I, _I, _I]),
+    "ems_fem1_add_dipole_x": (_I, [_P, _I, _I, _I]),
+    "ems_fem1_add_observation": (_I, [_P, _IP, _IP]),
+    "ems_fem1_run": (_I, [_P, _I]),
+    "ems_fem1_run_timed": (_I, [_P, _I, _DP, _DP]),
+    "ems_fem1_trace_shape": (_I, [_P, _IP, _IP]),
+    "ems_fem1_get_traces": (_I, [_P, _DP, _SZ]),
+    "ems_fem1_field_size": (_I, [_P, _I, ctypes.POINTER(_SZ)]),
+    "ems_fem1_get_field": (_I, [_P, _I, _I, _DP, _SZ]),
+    "ems_fem1_set_field": (_I, [_P, _I, _I, _DP, _SZ]),
+    "ems_fem1_get_time": (_I, [_P, _DP]),
+    "ems_fem1_matrix": (_I, [_P, _DP]),
+    "ems_fem1_save_results": (_I, [_P, _I, ctypes.c_char_p]),
+    "ems_fem1_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
 }
 
 _lib = None

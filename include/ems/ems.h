@@ -212,6 +212,78 @@ ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename)
 /* Total edge DOFs (NX + NY + NZ). */
 ems_status ems_fem_dof_count(const ems_fem *s, size_t *n);
 
+/* ------------------------------------------ first-order stabilized hex FEM
+ * src/fem (leapfrog, element matrix 1/2 dt^2/(mu eps h^2) C_s B_s^-1 A_s with
+ * stabilization factor s, Hertz-dipole source) and src/fem_abm
+ * (Adams-Bashforth-2 predictor / Adams-Moulton-3 corrector on co-located E
+ * and H, element matrix A_s).  Layouts as the hex FEM above with nx, ny, nz
+ * cells; fields 0..2 = Ex, Ey, Ez and (fem_abm only) 3..5 = Hx, Hy, Hz. */
+
+enum { EMS_FEM1_LEAPFROG = 0, EMS_FEM1_ABM = 1 };
+
+typedef struct ems_fem1 ems_fem1;
+
+/* Constructor arguments of src/fem/fem_simulation.hpp:13-15 (stab, use_ofem)
+ * and src/fem_abm/fem_simulation.hpp:13-14 (stab is the file-scope
+ * stabilization_factor = 0.05 of src/fem_abm/fem_simulation.cpp:29). */
+typedef struct {
+    int nx, ny, nz;   /* grid_size: cells per axis                     */
+    double h;         /* domain_size [m]                               */
+    double dt;        /* time_step [s]                                 */
+    double eps, mu;   /* permittivity, permeability                    */
+    int tf;           /* time_frequency                                */
+    int stepper;      /* EMS_FEM1_LEAPFROG (src/fem) or EMS_FEM1_ABM   */
+    double stab;      /* stabilization factor                          */
+    int use_ofem;     /* src/fem: 0 conventional, 1 orthogonal basis   */
+    int precision;    /* EMS_F64 or EMS_F32                            */
+} ems_fem1_desc;
+
+/* calculateElementStiffnessMatrix (src/fem/fem_simulation.cpp:125-434,
+ * src/fem_abm/fem_simulation.cpp:143-246): the 12 x 12 row-major element
+ * matrix the step uses.  Host only (no device needed). */
+ems_status ems_fem1_element_matrix(const ems_fem1_desc *desc, double *out144);
+
+/* FemSimulation::FemSimulation (src/fem/fem_simulation.cpp:41-62,
+ * src/fem_abm/fem_simulation.cpp:41-59). */
+ems_status ems_fem1_create(const ems_fem1_desc *desc, ems_fem1 **out);
+void ems_fem1_destroy(ems_fem1 *s);
+
+/* setSource_x/_y/_z (src/fem/fem_simulation.cpp:436-452). */
+ems_status ems_fem1_add_source(ems_fem1 *s, int comp, int i, int j, int k);
+
+/* setSourceDipole_x (src/fem/fem_simulation.cpp:454-457). */
+ems_status ems_fem1_add_dipole_x(ems_fem1 *s, int i, int j, int k);
+
+/* setObservationPoint (src/fem/fem_simulation.cpp:459-464): counts[c]
+ * points of component c, pts the concatenated [n][3] triples. */
+ems_status ems_fem1_add_observation(ems_fem1 *s, const int counts[3], const int *pts);
+
+/* run (src/fem/fem_simulation.cpp:835-898, src/fem_abm/fem_simulation.cpp:944-1003). */
+ems_status ems_fem1_run(ems_fem1 *s, int num_steps);
+ems_status ems_fem1_run_timed(ems_fem1 *s, int num_steps, double *total_ms, double *kernel_ms);
+
+/* saved_electric_field_ as [nobs][nslot][3], nslot = num_steps/tf + 1. */
+ems_status ems_fem1_trace_shape(const ems_fem1 *s, int *nobs, int *nslot);
+ems_status ems_fem1_get_traces(ems_fem1 *s, double *out, size_t count);
+
+/* Field access, slot 0/1 = time_0_/time_1_ (the previous / latest step). */
+ems_status ems_fem1_field_size(const ems_fem1 *s, int field, size_t *n);
+ems_status ems_fem1_get_field(ems_fem1 *s, int field, int slot, double *out, size_t count);
+ems_status ems_fem1_set_field(ems_fem1 *s, int field, int slot, const double *in, size_t count);
+ems_status ems_fem1_get_time(const ems_fem1 *s, double *t);
+
+/* The element matrix of this handle (row-major 144 doubles). */
+ems_status ems_fem1_matrix(const ems_fem1 *s, double *out144);
+
+/* saveResults: src/fem writes every observation to `path` (the file name;
+ * the last one wins, src/fem/fem_simulation.cpp:900-924); src/fem_abm writes
+ * results_<i>_<dt>_<h>_<num_steps>.csv into directory `path` ("" = cwd,
+ * src/fem_abm/fem_simulation.cpp:1005-1030). */
+ems_status ems_fem1_save_results(ems_fem1 *s, int num_steps, const char *path);
+
+/* Total edge DOFs per field set. */
+ems_status ems_fem1_dof_count(const ems_fem1 *s, size_t *n);
+
 #ifdef __cplusplus
 }
 #endif

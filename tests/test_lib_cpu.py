@@ -81,3 +81,28 @@ def test_check_params_matches_oracle():
         cases.append((ds, gn, rng.uniform(-1, 1), rng.uniform(-1e-11, 1e-11), rng.uniform(1e-4, 1e-2), c))
     for args in cases:
         assert check_params(*args) == of.check_params(*args), args
+
+
+@pytest.mark.parametrize("stepper,ofem,stab", [(0, 0, 0.3), (0, 1, 0.3), (0, 0, 0.0), (0, 0, 1.25), (1, 0, 0.05)])
+def test_fem1_element_matrix_equals_exact_rounding(stepper, ofem, stab):
+    """The library's host-side element matrix of src/fem / src/fem_abm equals
+    the exact rational product C_s B_s^-1 A_s (scaled) rounded once."""
+    from emsolvers_amd import fem1 as F
+    from oracle import fem1 as O
+    EPS0, MU0 = 8.854187817e-12, 1.25663706212e-6
+    for h in (0.02, 1.7e-3):
+        dt = 0.3 * h * np.sqrt(EPS0 * MU0)
+        a = F.element_matrix(stepper, (4, 4, 4), h, dt, EPS0, MU0, stab, ofem)
+        b = O.element_matrix(stepper, dt, h, EPS0, MU0, stab, ofem)
+        assert np.array_equal(a, b)
+
+
+def test_fem1_invalid_descriptor_rejected():
+    from emsolvers_amd import _lib
+    from emsolvers_amd import fem1 as F
+    with pytest.raises(_lib.EmsError):
+        F.element_matrix(2, (4, 4, 4), 0.02, 1e-11, 8.854187817e-12, 1.25663706212e-6, 0.3)
+    with pytest.raises(_lib.EmsError):
+        F.element_matrix(1, (4, 4, 4), 0.02, 1e-11, 8.854187817e-12, 1.25663706212e-6, 0.05, use_ofem=1)
+    with pytest.raises(_lib.EmsError):
+        F.FemSimulation((4, -1, 4), 0.02, 1e-11, 8.854187817e-12, 1.25663706212e-6, 1, 0.3, 0)
