@@ -66,6 +66,8 @@ SIGNATURES = {
     "ems_fdtd_save_results": (_I, [_P, _I, _D, ctypes.c_char_p]),
     "ems_fdtd_device_bytes": (_I, [_P, ctypes.POINTER(_SZ)]),
     "ems_fdtd_check_params": (_I, [_DP, _IP, _D, _D, _D, _D]),
+    "ems_fdtd_gradient": (_I, [_P, _I, _DP, _SZ, _DP, _DP, _DP]),
+    "ems_fdtd_set_checkpoint_interval": (_I, [_P, _I]),
     # structured-hex FEM
     "ems_fem_load_kmat": (_I, [ctypes.c_char_p, _DP, _SZ, _IP]),
     "ems_fem_create": (_I, [ctypes.POINTER(FemDesc), _DP, _PP]),

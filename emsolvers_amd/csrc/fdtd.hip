@@ -75,6 +75,10 @@ struct ems_fdtd {
     hipStream_t stream = nullptr;
     DevBuf d_src_comp, d_src_u, d_src_val, d_pt_comp, d_pt_u, d_rec;
     bool dirty_pml = true;
+    // misfit gradient: checkpoint interval (0 = ceil(sqrt(steps))) and buffers
+    int ckpt_interval = 0;
+    struct Adj *adj = nullptr;
+    ~ems_fdtd();
 };
 
 namespace {
@@ -637,6 +641,343 @@ template <typename T> ems_status set_field_t(ems_fdtd *s, int w, const double *i
 }  // namespace
 
 // =========================================================================
+// Misfit gradient by the discrete adjoint with sqrt(N) checkpointing
+// (no reference counterpart).
+// =========================================================================
+struct Adj {
+    std::vector<DevBuf> ckpt;   // per checkpoint: 6 fields + 12 psi, packed
+    DevBuf fin;                 // the final forward state
+    DevBuf seg;                 // E^n of one segment: [K+1][3][nalloc]
+    DevBuf e[2][3], h[3];       // adjoint fields (e ping-pong)
+    DevBuf pe[2][6], ph[2][6];  // psi adjoints (ping-pong), compact slabs
+    DevBuf fac, grad;           // 4 x nmat derivative factors, 2 x nmat gradient (FP64)
+    DevBuf w;                   // residual weights [rows][npts]
+};
+
+ems_fdtd::~ems_fdtd() { delete adj; }
+
+namespace {
+
+size_t state_bytes(const ems_fdtd *s)
+{
+    size_t b = 6 * s->nalloc * s->esz;
+    for (int p = 0; p < 12; ++p) b += s->psi[p].bytes;
+    return b;
+}
+
+ems_status copy_state(ems_fdtd *s, DevBuf &dst, bool save)
+{
+    char *q = static_cast<char *>(dst.p);
+    const size_t fb = s->nalloc * s->esz;
+    for (int w = 0; w < 6; ++w, q += fb)
+        EMS_HIP(hipMemcpyAsync(save ? q : s->f[w].p, save ? s->f[w].p : q, fb, hipMemcpyDeviceToDevice, s->stream));
+    for (int p = 0; p < 12; ++p) {
+        if (!s->psi[p].bytes) continue;
+        EMS_HIP(hipMemcpyAsync(save ? q : s->psi[p].p, save ? s->psi[p].p : q, s->psi[p].bytes,
+                               hipMemcpyDeviceToDevice, s->stream));
+        q += s->psi[p].bytes;
+    }
+    return EMS_OK;
+}
+
+ems_status snap_e(ems_fdtd *s, int slot)
+{
+    const size_t fb = s->nalloc * s->esz;
+    char *q = static_cast<char *>(s->adj->seg.p) + (size_t)slot * 3 * fb;
+    for (int c = 0; c < 3; ++c)
+        EMS_HIP(hipMemcpyAsync(q + c * fb, s->f[c].p, fb, hipMemcpyDeviceToDevice, s->stream));
+    return EMS_OK;
+}
+
+template <typename T> Adj3<T> seg_e(ems_fdtd *s, int slot)
+{
+    T *q = s->adj->seg.as<T>() + (size_t)slot * 3 * s->nalloc;
+    return Adj3<T>{{q, q + s->nalloc, q + 2 * s->nalloc}};
+}
+
+template <typename T> AdjPml<T> adj_pml(ems_fdtd *s, int cur)
+{
+    AdjPml<T> a{};
+    a.P = s->npml > 0 ? s->npml : 0;
+    a.W = 2 * a.P + 1;
+    if (s->npml <= 0) return a;
+    const PmlView<T> v = pml_view<T>(s);
+    Adj *A = s->adj;
+    for (int ax = 0; ax < 3; ++ax) {
+        a.hp[ax] = v.hp[ax];
+        a.ep[ax] = v.ep[ax];
+        for (int q = 0; q < 2; ++q) {
+            a.pe_old[ax][q] = A->pe[cur][ax * 2 + q].as<T>();
+            a.pe_new[ax][q] = A->pe[1 - cur][ax * 2 + q].as<T>();
+            a.ph_old[ax][q] = A->ph[cur][ax * 2 + q].as<T>();
+            a.ph_new[ax][q] = A->ph[1 - cur][ax * 2 + q].as<T>();
+        }
+    }
+    return a;
+}
+
+template <typename T, int MAT, bool PML>
+void launch_adjoint(ems_fdtd *s, int cur, Adj3<T> en, Adj3<T> e1, GradView gv)
+{
+    const GridView &g = s->g;
+    Adj *A = s->adj;
+    const dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    const T nch = -(T)(s->d.dt / s->d.mu / s->d.h);
+    const AdjPml<T> a = adj_pml<T>(s, cur);
+    Adj3<T> ein{{A->e[0][0].as<T>(), A->e[0][1].as<T>(), A->e[0][2].as<T>()}};
+    Adj3<T> eout{{A->e[1][0].as<T>(), A->e[1][1].as<T>(), A->e[1][2].as<T>()}};
+    Adj3<T> hb{{A->h[0].as<T>(), A->h[1].as<T>(), A->h[2].as<T>()}};
+    k_adj_a<T, MAT, PML, true><<<grd, blk, 0, s->stream>>>(g, ce, ein, eout, hb, s->ca.as<T>(), s->cb.as<T>(), a, en,
+                                                          e1, gv);
+    k_adj_b<T, PML><<<grd, blk, 0, s->stream>>>(g, nch, eout, hb, a);
+}
+
+template <typename T> void adjoint_step(ems_fdtd *s, int cur, Adj3<T> en, Adj3<T> e1, GradView gv)
+{
+    const bool pml = s->npml > 0;
+    switch (s->mat_mode * 2 + (pml ? 1 : 0)) {
+    case 0: launch_adjoint<T, 0, false>(s, cur, en, e1, gv); break;
+    case 1: launch_adjoint<T, 0, true>(s, cur, en, e1, gv); break;
+    case 2: launch_adjoint<T, 1, false>(s, cur, en, e1, gv); break;
+    case 3: launch_adjoint<T, 1, true>(s, cur, en, e1, gv); break;
+    case 4: launch_adjoint<T, 2, false>(s, cur, en, e1, gv); break;
+    default: launch_adjoint<T, 2, true>(s, cur, en, e1, gv); break;
+    }
+}
+
+ems_status zero(DevBuf &b, size_t bytes, hipStream_t st)
+{
+    EMS_TRY(b.reserve(bytes));
+    EMS_HIP(hipMemsetAsync(b.p, 0, bytes, st));
+    return EMS_OK;
+}
+
+// Material derivative factors (FP64, material grid):
+//   s = sigma dt/(2 eps), dCa/deps_r = 2 s/(eps_r (1+s)^2), (dCb/deps_r)/Cb = -1/(eps_r (1+s)),
+//   dCa/dsigma = -dt/(eps (1+s)^2), (dCb/dsigma)/Cb = -dt/(2 eps (1+s)).
+// The source amplitude dt/(eps_src (1+s_src)) equals h Cb at the source
+// cell, so with E^{n+1} - Ca E^n = Cb curl H - src its derivative is carried
+// by the (dCb/Cb)(E^{n+1} - Ca E^n) term.
+std::vector<double> material_factors(const ems_fdtd *s)
+{
+    const GridView &g = s->g;
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    std::vector<double> f(4 * nm);
+    for (size_t m = 0; m < nm; ++m) {
+        const double er = s->eps_r.empty() ? 1.0 : s->eps_r[m];
+        const double sg = s->sigma.empty() ? 0.0 : s->sigma[m];
+        const double e = s->d.eps * er;
+        const double hs = sg * s->d.dt / (2.0 * e);
+        const double op = 1.0 + hs;
+        f[m] = 2.0 * hs / (er * op * op);
+        f[nm + m] = -1.0 / (er * op);
+        f[2 * nm + m] = -s->d.dt / (e * op * op);
+        f[3 * nm + m] = -s->d.dt / (2.0 * e * op);
+    }
+    return f;
+}
+
+template <typename T>
+ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit, double *g_eps, double *g_sig)
+{
+    const GridView &g = s->g;
+    const int tf = s->d.tf;
+    EMS_TRY(setup_pml(s));
+    if (!s->adj) s->adj = new Adj();
+    Adj *A = s->adj;
+    hipStream_t st = s->stream;
+    // zero forward state at t = 0
+    for (int w = 0; w < 6; ++w) EMS_HIP(hipMemsetAsync(s->f[w].p, 0, s->nalloc * s->esz, st));
+    for (int p = 0; p < 12; ++p)
+        if (s->psi[p].bytes) EMS_HIP(hipMemsetAsync(s->psi[p].p, 0, s->psi[p].bytes, st));
+    if (s->alt_ready)
+        for (int w = 0; w < 6; ++w) EMS_HIP(hipMemsetAsync(s->f2[w].p, 0, s->nalloc * s->esz, st));
+    // sources (x, then y, then z) and their per-step values
+    std::vector<Src> src = s->src;
+    std::stable_sort(src.begin(), src.end(), [](const Src &a, const Src &b) { return a.comp < b.comp; });
+    const int nsrc = (int)src.size();
+    std::vector<int> sc(nsrc);
+    std::vector<long> su(nsrc);
+    for (int q = 0; q < nsrc; ++q) {
+        sc[q] = src[q].comp;
+        su[q] = src[q].u;
+    }
+    std::vector<double> sval((size_t)N * std::max(nsrc, 1), 0.0);
+    double t = 0.0;
+    for (int n = 0; n < N; ++n) {
+        const double wv = wavelet_value(s, t);
+        for (int q = 0; q < nsrc; ++q) {
+            double v = wv * s->d.dt / src[q].eps;
+            if (src[q].scale != 1.0) v = v / src[q].scale;
+            sval[(size_t)n * nsrc + q] = v;
+        }
+        t += s->d.dt;
+    }
+    std::vector<int> pc;
+    std::vector<long> pu;
+    for (auto &o : s->obs)
+        for (int c = 0; c < 3; ++c)
+            for (auto &p : o[c]) {
+                pc.push_back(c);
+                pu.push_back(ref_to_unified(g, c, p[0], p[1], p[2]));
+            }
+    const int npts = (int)pc.size();
+    const int nrec = N / tf;
+    EMS_TRY(s->d_src_comp.reserve(sizeof(int) * std::max(nsrc, 1)));
+    EMS_TRY(s->d_src_u.reserve(sizeof(long) * std::max(nsrc, 1)));
+    EMS_TRY(s->d_src_val.reserve(sizeof(double) * sval.size()));
+    EMS_TRY(s->d_pt_comp.reserve(sizeof(int) * std::max(npts, 1)));
+    EMS_TRY(s->d_pt_u.reserve(sizeof(long) * std::max(npts, 1)));
+    EMS_TRY(s->d_rec.reserve(sizeof(double) * std::max((size_t)nrec * npts, (size_t)1)));
+    if (nsrc) {
+        EMS_HIP(hipMemcpy(s->d_src_comp.p, sc.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_src_u.p, su.data(), sizeof(long) * nsrc, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_src_val.p, sval.data(), sizeof(double) * sval.size(), hipMemcpyHostToDevice));
+    }
+    if (npts) {
+        EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
+    }
+    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    if (use_fused(s)) {
+        EMS_TRY(ensure_alt(s));
+        EMS_TRY(build_tiles(s));
+    }
+    // one forward step n -> n+1 (sources; receivers when `record`)
+    auto fstep = [&](int n, bool record) -> ems_status {
+        const PmlView<T> pm = pml_view<T>(s);
+        launch_step<T>(s, ch, ce, pm);
+        const Fields<T> f = fields<T>(s);
+        double *rec = nullptr;
+        if (record && npts && (n + 1) % tf == 0) rec = s->d_rec.as<double>() + (size_t)((n + 1) / tf - 1) * npts;
+        if (nsrc || rec)
+            k_post<T><<<1, 256, 0, st>>>(f, nsrc, s->d_src_comp.as<int>(), s->d_src_u.as<long>(),
+                                         s->d_src_val.as<double>() + (size_t)n * nsrc, npts, s->d_pt_comp.as<int>(),
+                                         s->d_pt_u.as<long>(), rec);
+        EMS_HIP(hipGetLastError());
+        return EMS_OK;
+    };
+    // checkpoints
+    const int K = s->ckpt_interval > 0 ? std::min(s->ckpt_interval, N)
+                                       : std::max(1, (int)std::ceil(std::sqrt((double)N)));
+    const int nseg = (N + K - 1) / K;
+    const size_t sb = state_bytes(s);
+    if ((int)A->ckpt.size() < nseg) A->ckpt.resize(nseg);
+    for (int q = 0; q < nseg; ++q) EMS_TRY(A->ckpt[q].reserve(sb));
+    EMS_TRY(A->fin.reserve(sb));
+    EMS_TRY(A->seg.reserve((size_t)(K + 1) * 3 * s->nalloc * s->esz));
+    // forward: checkpoints at segment starts; the last segment keeps its E
+    for (int q = 0; q < nseg; ++q) {
+        const int s0 = q * K, s1 = std::min(N, s0 + K);
+        EMS_TRY(copy_state(s, A->ckpt[q], true));
+        const bool last = q == nseg - 1;
+        if (last) EMS_TRY(snap_e(s, 0));
+        for (int n = s0; n < s1; ++n) {
+            EMS_TRY(fstep(n, true));
+            if (last) EMS_TRY(snap_e(s, n + 1 - s0));
+        }
+    }
+    EMS_TRY(copy_state(s, A->fin, true));
+    EMS_HIP(hipStreamSynchronize(st));
+    // traces (the reference's averaging) and the misfit
+    const int nobs = (int)s->obs.size();
+    const int nslot = N / tf + 1;
+    std::vector<double> raw((size_t)nrec * npts);
+    if (!raw.empty()) EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+    std::vector<double> tr((size_t)nobs * nslot * 3, 0.0);
+    for (int r = 0; r < nrec; ++r) {
+        int p = 0;
+        for (int o = 0; o < nobs; ++o)
+            for (int c = 0; c < 3; ++c) {
+                double v = 0.0;
+                const size_t cnt = s->obs[o][c].size();
+                for (size_t q = 0; q < cnt; ++q, ++p) v += raw[(size_t)r * npts + p];
+                tr[((size_t)o * nslot + r + 1) * 3 + c] = v / (double)cnt;
+            }
+    }
+    double J = 0.0;
+    for (int o = 0; o < nobs; ++o)
+        for (int sl = 1; sl < nslot; ++sl)
+            for (int c = 0; c < 3; ++c) {
+                if (s->obs[o][c].empty()) continue;
+                const double r = tr[((size_t)o * nslot + sl) * 3 + c] - d_obs[((size_t)o * nslot + sl) * 3 + c];
+                J += 0.5 * r * r;
+            }
+    // residual weights per recorded row and point: r / count
+    std::vector<double> w((size_t)std::max(nrec, 1) * std::max(npts, 1), 0.0);
+    for (int r = 0; r < nrec; ++r) {
+        int p = 0;
+        for (int o = 0; o < nobs; ++o)
+            for (int c = 0; c < 3; ++c) {
+                const size_t cnt = s->obs[o][c].size();
+                const double res =
+                    cnt ? tr[((size_t)o * nslot + r + 1) * 3 + c] - d_obs[((size_t)o * nslot + r + 1) * 3 + c] : 0.0;
+                for (size_t q = 0; q < cnt; ++q, ++p) w[(size_t)r * npts + p] = res / (double)cnt;
+            }
+    }
+    EMS_TRY(A->w.reserve(sizeof(double) * w.size()));
+    EMS_HIP(hipMemcpy(A->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    const std::vector<double> fac = material_factors(s);
+    EMS_TRY(A->fac.reserve(sizeof(double) * fac.size()));
+    EMS_HIP(hipMemcpy(A->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+    EMS_TRY(zero(A->grad, sizeof(double) * 2 * nm, st));
+    // adjoint state
+    const size_t fb = s->nalloc * s->esz;
+    for (int c = 0; c < 3; ++c) {
+        EMS_TRY(zero(A->e[0][c], fb, st));
+        EMS_TRY(zero(A->e[1][c], fb, st));
+        EMS_TRY(zero(A->h[c], fb, st));
+    }
+    if (s->npml > 0)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q)
+                for (int b = 0; b < 2; ++b) {
+                    EMS_TRY(zero(A->pe[b][ax * 2 + q], s->psi[ax * 4 + 2 + q].bytes, st));
+                    EMS_TRY(zero(A->ph[b][ax * 2 + q], s->psi[ax * 4 + q].bytes, st));
+                }
+    GradView gv{A->fac.as<double>(), A->fac.as<double>() + nm, A->fac.as<double>() + 2 * nm,
+                A->fac.as<double>() + 3 * nm, A->grad.as<double>(), A->grad.as<double>() + nm};
+    int cur = 0;  // psi adjoint ping-pong
+    for (int q = nseg - 1; q >= 0; --q) {
+        const int s0 = q * K, s1 = std::min(N, s0 + K);
+        if (q != nseg - 1) {
+            EMS_TRY(copy_state(s, A->ckpt[q], false));
+            EMS_TRY(snap_e(s, 0));
+            for (int n = s0; n < s1; ++n) {
+                EMS_TRY(fstep(n, false));
+                EMS_TRY(snap_e(s, n + 1 - s0));
+            }
+        }
+        for (int n = s1 - 1; n >= s0; --n) {
+            Adj3<T> e_cur{{A->e[0][0].as<T>(), A->e[0][1].as<T>(), A->e[0][2].as<T>()}};
+            if (npts && (n + 1) % tf == 0)
+                k_adj_inject<T><<<1, 64, 0, st>>>(e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
+                                                  A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts);
+            adjoint_step<T>(s, cur, seg_e<T>(s, n - s0), seg_e<T>(s, n + 1 - s0), gv);
+            EMS_HIP(hipGetLastError());
+            for (int c = 0; c < 3; ++c) swap_buf(A->e[0][c], A->e[1][c]);
+            cur = 1 - cur;
+        }
+    }
+    EMS_TRY(copy_state(s, A->fin, false));
+    EMS_HIP(hipStreamSynchronize(st));
+    std::vector<double> gr(2 * nm);
+    EMS_HIP(hipMemcpy(gr.data(), A->grad.p, sizeof(double) * gr.size(), hipMemcpyDeviceToHost));
+    if (g_eps) std::copy(gr.begin(), gr.begin() + nm, g_eps);
+    if (g_sig) std::copy(gr.begin() + nm, gr.end(), g_sig);
+    if (misfit) *misfit = J;
+    s->traces.swap(tr);
+    s->nslot = nslot;
+    s->t_now = t;
+    return EMS_OK;
+}
+
+}  // namespace
+
+// =========================================================================
 // C ABI
 // =========================================================================
 extern "C" {
@@ -906,6 +1247,27 @@ int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], d
         return 0;
     if (c * time_step >= domain_size) return 0;
     return 1;
+}
+
+ems_status ems_fdtd_set_checkpoint_interval(ems_fdtd *s, int interval)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (interval < 0) return invalid("checkpoint interval must be >= 0");
+    s->ckpt_interval = interval;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_gradient(ems_fdtd *s, int num_steps, const double *d_obs, size_t count, double *misfit,
+                             double *grad_eps_r, double *grad_sigma)
+{
+    clear_error();
+    if (!s || !d_obs) return invalid("null argument");
+    if (num_steps <= 0) return invalid("num_steps must be >= 1");
+    const size_t need = s->obs.size() * (size_t)(num_steps / s->d.tf + 1) * 3;
+    if (count != need) return invalid("d_obs must hold nobs * (num_steps/tf + 1) * 3 values");
+    return s->esz == 4 ? gradient_impl<float>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma)
+                       : gradient_impl<double>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma);
 }
 
 }  // extern "C"

@@ -584,3 +584,237 @@ __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Field
 
 }  // namespace fdtd
 }  // namespace ems
+
+namespace ems {
+namespace fdtd {
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the Yee step (no reference counterpart): the exact
+// transpose of one step (H update, then E update, CPML included), applied
+// backwards in time, for the gradient of the receiver misfit with respect to
+// the per-cell eps_r and sigma.  Over the unified grid the forward E update
+// reads backward differences of H and the H update forward differences of E,
+// so the transpose is a gather of forward (pass A) and backward (pass B)
+// differences:
+//   pass A (E update transposed):  mu = Cb e'; per derivative u = +-mu,
+//     in the CPML of its axis  p = psiE' + u, tbar = u k + a p, psiE = b p,
+//     else tbar = u;  hbar += (tbar(own) - tbar(+1)) + (tbar(own) - tbar(+1));
+//     e = Ca e'; gradient += e' (dCa E^n + (dCb/Cb)(E^{n+1} - Ca E^n)).
+//   pass B (H update transposed):  nu = -ch hbar, the same CPML transpose with
+//     the H-type profiles, e += (sbar(-1) - sbar(own)) + (sbar(-1) - sbar(own)).
+// psi adjoints and e are ping-ponged between old and new buffers; every term
+// of a cell outside a component's update range is 0.  One thread per unified
+// point, workgroup 64 x 4, grid (ceil((nx+1)/64), ceil((ny+1)/4), nz+1).
+// ---------------------------------------------------------------------------
+template <typename T> struct Adj3 {
+    T *c[3];
+};
+
+template <typename T> struct AdjPml {
+    int P, W;
+    const Prof<T> *hp[3], *ep[3];
+    const T *pe_old[3][2];
+    T *pe_new[3][2];
+    const T *ph_old[3][2];
+    T *ph_new[3][2];
+};
+
+// derivative d of component c: axis and slot within the axis' psi pair
+__device__ __forceinline__ int adj_axis(int c, int d) { return c == 0 ? 1 + d : c == 1 ? (d == 0 ? 2 : 0) : (d == 0 ? 0 : 1); }
+__device__ __forceinline__ int adj_slot(int c, int d) { return c == 0 ? 0 : c == 1 ? (d == 0 ? 1 : 0) : 1; }
+
+__device__ __forceinline__ bool e_valid(const GridView &g, int c, int I, int J, int K)
+{
+    if (c == 0) return I <= g.nx && J >= 1 && J <= g.ny && K >= 1 && K <= g.nz;
+    if (c == 1) return I >= 1 && I <= g.nx && J <= g.ny && K >= 1 && K <= g.nz;
+    return I >= 1 && I <= g.nx && J >= 1 && J <= g.ny && K <= g.nz;
+}
+__device__ __forceinline__ bool h_valid(const GridView &g, int c, int I, int J, int K)
+{
+    if (I < 0 || J < 0 || K < 0) return false;
+    if (c == 0) return I >= 1 && I <= g.nx && J <= g.ny && K <= g.nz;
+    if (c == 1) return I <= g.nx && J >= 1 && J <= g.ny && K <= g.nz;
+    return I <= g.nx && J <= g.ny && K >= 1 && K <= g.nz;
+}
+
+__device__ __forceinline__ long slab_of(const GridView &g, int P, int W, int ax, int I, int J, int K)
+{
+    return ax == 0 ? slab_x(g, P, W, I, J, K) : ax == 1 ? slab_y(g, P, W, I, J, K) : slab_z(g, P, W, I, J, K);
+}
+
+template <typename T, int MAT, bool PML>
+__device__ __forceinline__ T adj_tbar(const GridView &g, const AdjPml<T> &a, const T *__restrict__ eb, T ce,
+                                      const T *__restrict__ cb, int c, int d, int I, int J, int K)
+{
+    if (!e_valid(g, c, I, J, K)) return (T)0;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    const T mu = (MAT == 0 ? ce : cb[u]) * eb[u];
+    const T v = d == 0 ? mu : -mu;
+    if (PML) {
+        const int ax = adj_axis(c, d);
+        const int pos = ax == 0 ? I : ax == 1 ? J : K;
+        const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+        if (in_pml(a.P, n, pos)) {
+            const T p = a.pe_old[ax][adj_slot(c, d)][slab_of(g, a.P, a.W, ax, I, J, K)] + v;
+            const Prof<T> pr = a.ep[ax][pos];
+            return v * pr.k + pr.a * p;
+        }
+    }
+    return v;
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ T adj_sbar(const GridView &g, const AdjPml<T> &a, const T *__restrict__ hb, T nch, int c,
+                                      int d, int I, int J, int K)
+{
+    if (!h_valid(g, c, I, J, K)) return (T)0;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    const T nu = nch * hb[u];
+    const T v = d == 0 ? nu : -nu;
+    if (PML) {
+        const int ax = adj_axis(c, d);
+        const int pos = ax == 0 ? I : ax == 1 ? J : K;
+        const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+        if (in_pml(a.P, n, pos)) {
+            const T p = a.ph_old[ax][adj_slot(c, d)][slab_of(g, a.P, a.W, ax, I, J, K)] + v;
+            const Prof<T> pr = a.hp[ax][pos];
+            return v * pr.k + pr.a * p;
+        }
+    }
+    return v;
+}
+
+// Material derivative factors on the (nx+1)(ny+1)(nz+1) material grid.
+struct GradView {
+    const double *dca_e, *dcbn_e, *dca_s, *dcbn_s;
+    double *g_eps, *g_sig;
+};
+
+template <typename T, int MAT, bool PML, bool GRAD>
+__global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Adj3<T> eout, Adj3<T> hb,
+                                              const T *__restrict__ ca, const T *__restrict__ cb, AdjPml<T> a,
+                                              Adj3<T> en, Adj3<T> e1, GradView gv)
+{
+    const int I = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I > g.nx || J > g.ny || K > g.nz) return;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    const T *ex = ein.c[0], *ey = ein.c[1], *ez = ein.c[2];
+    if (h_valid(g, 0, I, J, K)) {
+        const T A = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 0, I, J, K);
+        const T B = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 0, I, J, K + 1);
+        const T C = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 1, I, J, K);
+        const T D = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 1, I, J + 1, K);
+        hb.c[0][u] = hb.c[0][u] + ((A - B) + (C - D));
+    }
+    if (h_valid(g, 1, I, J, K)) {
+        const T A = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 0, I, J, K);
+        const T B = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 0, I + 1, J, K);
+        const T C = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 1, I, J, K);
+        const T D = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 1, I, J, K + 1);
+        hb.c[1][u] = hb.c[1][u] + ((A - B) + (C - D));
+    }
+    if (h_valid(g, 2, I, J, K)) {
+        const T A = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 0, I, J, K);
+        const T B = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 0, I, J + 1, K);
+        const T C = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 1, I, J, K);
+        const T D = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 1, I + 1, J, K);
+        hb.c[2][u] = hb.c[2][u] + ((A - B) + (C - D));
+    }
+    double tse = 0.0, tss = 0.0;
+    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
+    for (int c = 0; c < 3; ++c) {
+        if (!e_valid(g, c, I, J, K)) continue;
+        const T lam = ein.c[c][u];
+        if (PML) {
+            const T mu = (MAT == 0 ? ce : cb[u]) * lam;
+            for (int d = 0; d < 2; ++d) {
+                const int ax = adj_axis(c, d);
+                const int pos = ax == 0 ? I : ax == 1 ? J : K;
+                const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+                if (!in_pml(a.P, n, pos)) continue;
+                const long q = slab_of(g, a.P, a.W, ax, I, J, K);
+                const int sl = adj_slot(c, d);
+                const T v = d == 0 ? mu : -mu;
+                a.pe_new[ax][sl][q] = a.ep[ax][pos].b * (a.pe_old[ax][sl][q] + v);
+            }
+        }
+        const T cav = MAT == 2 ? ca[u] : (T)1;
+        eout.c[c][u] = cav * lam;
+        if (GRAD) {
+            const double l = (double)lam, e0 = (double)en.c[c][u], ep = (double)e1.c[c][u];
+            const double de = ep - (double)cav * e0;
+            tse = tse + l * (gv.dca_e[m] * e0 + gv.dcbn_e[m] * de);
+            tss = tss + l * (gv.dca_s[m] * e0 + gv.dcbn_s[m] * de);
+        }
+    }
+    if (GRAD) {
+        gv.g_eps[m] += tse;
+        gv.g_sig[m] += tss;
+    }
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Adj3<T> hb, AdjPml<T> a)
+{
+    const int I = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I > g.nx || J > g.ny || K > g.nz) return;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    const T *hx = hb.c[0], *hy = hb.c[1], *hz = hb.c[2];
+    if (e_valid(g, 0, I, J, K)) {
+        const T A = adj_sbar<T, PML>(g, a, hy, nch, 1, 0, I, J, K - 1);
+        const T B = adj_sbar<T, PML>(g, a, hy, nch, 1, 0, I, J, K);
+        const T C = adj_sbar<T, PML>(g, a, hz, nch, 2, 1, I, J - 1, K);
+        const T D = adj_sbar<T, PML>(g, a, hz, nch, 2, 1, I, J, K);
+        eb.c[0][u] = eb.c[0][u] + ((A - B) + (C - D));
+    }
+    if (e_valid(g, 1, I, J, K)) {
+        const T A = adj_sbar<T, PML>(g, a, hx, nch, 0, 1, I, J, K - 1);
+        const T B = adj_sbar<T, PML>(g, a, hx, nch, 0, 1, I, J, K);
+        const T C = adj_sbar<T, PML>(g, a, hz, nch, 2, 0, I - 1, J, K);
+        const T D = adj_sbar<T, PML>(g, a, hz, nch, 2, 0, I, J, K);
+        eb.c[1][u] = eb.c[1][u] + ((A - B) + (C - D));
+    }
+    if (e_valid(g, 2, I, J, K)) {
+        const T A = adj_sbar<T, PML>(g, a, hx, nch, 0, 0, I, J - 1, K);
+        const T B = adj_sbar<T, PML>(g, a, hx, nch, 0, 0, I, J, K);
+        const T C = adj_sbar<T, PML>(g, a, hy, nch, 1, 1, I - 1, J, K);
+        const T D = adj_sbar<T, PML>(g, a, hy, nch, 1, 1, I, J, K);
+        eb.c[2][u] = eb.c[2][u] + ((A - B) + (C - D));
+    }
+    if (PML) {
+        for (int c = 0; c < 3; ++c) {
+            if (!h_valid(g, c, I, J, K)) continue;
+            const T nu = nch * hb.c[c][u];
+            for (int d = 0; d < 2; ++d) {
+                const int ax = adj_axis(c, d);
+                const int pos = ax == 0 ? I : ax == 1 ? J : K;
+                const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
+                if (!in_pml(a.P, n, pos)) continue;
+                const long q = slab_of(g, a.P, a.W, ax, I, J, K);
+                const int sl = adj_slot(c, d);
+                const T v = d == 0 ? nu : -nu;
+                a.ph_new[ax][sl][q] = a.hp[ax][pos].b * (a.ph_old[ax][sl][q] + v);
+            }
+        }
+    }
+}
+
+// Residual injection at a recorded step (one thread, the misfit's order):
+// ebar[point] += w[point], w = (trace - d_obs) / count.
+template <typename T>
+__global__ void k_adj_inject(Adj3<T> eb, int npts, const int *__restrict__ pt_comp, const long *__restrict__ pt_u,
+                             const double *__restrict__ w)
+{
+    if (threadIdx.x != 0) return;
+    for (int p = 0; p < npts; ++p) {
+        T *q = eb.c[pt_comp[p]] + pt_u[p];
+        *q = *q + (T)w[p];
+    }
+}
+
+}  // namespace fdtd
+}  // namespace ems

@@ -159,6 +159,24 @@ class FemSimulation:
         _lib.check(_lib.load().ems_fdtd_get_time(self._h, ctypes.byref(t)))
         return t.value
 
+    def set_checkpoint_interval(self, interval):
+        """Checkpoint interval of gradient() (0 = ceil(sqrt(num_steps)))."""
+        _lib.check(_lib.load().ems_fdtd_set_checkpoint_interval(self._h, int(interval)))
+
+    def gradient(self, num_steps, d_obs):
+        """Misfit J = 1/2 sum (trace - d_obs)^2 of a run of num_steps from a
+        zero state, and dJ/deps_r, dJ/dsigma on the material grid (FP64).
+        d_obs: [nobs][num_steps/tf + 1][3]."""
+        nx, ny, nz = self.grid
+        nm = (nx + 1) * (ny + 1) * (nz + 1)
+        d = np.ascontiguousarray(d_obs, dtype=np.float64)
+        ge, gs = np.zeros(nm), np.zeros(nm)
+        J = ctypes.c_double()
+        _lib.check(_lib.load().ems_fdtd_gradient(self._h, int(num_steps), _lib.dptr(d), d.size,
+                                                 ctypes.cast(ctypes.byref(J), _lib._DP), _lib.dptr(ge),
+                                                 _lib.dptr(gs)))
+        return J.value, ge, gs
+
     def device_bytes(self):
         b = ctypes.c_size_t()
         _lib.check(_lib.load().ems_fdtd_device_bytes(self._h, ctypes.byref(b)))

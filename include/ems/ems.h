@@ -147,6 +147,24 @@ ems_status ems_fdtd_device_bytes(const ems_fdtd *s, size_t *bytes);
 int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], double duration,
                           double time_step, double domain_size, double c);
 
+/* Misfit gradient (no reference counterpart; extends run() of
+ * src/fdtd/fdtd_simulation.cpp:242-306 with its discrete adjoint).
+ * Runs num_steps steps from a zero state at t = 0 with the handle's sources,
+ * receivers, material and CPML, and returns
+ *   J = 1/2 sum over observations o, recorded slots 1..num_steps/tf and
+ *       components c with a non-empty point list of (trace - d_obs)^2
+ * and dJ/deps_r, dJ/dsigma on the (nx+1)(ny+1)(nz+1) material grid (either
+ * output may be NULL), accumulated in FP64.  d_obs is [nobs][num_steps/tf+1][3]
+ * (slot 0 unused).  Forward states are recovered by checkpointing every
+ * `interval` steps and recomputing one segment at a time.  Afterwards the
+ * handle holds the final forward state, time num_steps*dt and the run's
+ * traces (not accumulated onto earlier ones). */
+ems_status ems_fdtd_gradient(ems_fdtd *s, int num_steps, const double *d_obs, size_t count,
+                             double *misfit, double *grad_eps_r, double *grad_sigma);
+
+/* Checkpoint interval of ems_fdtd_gradient (0 = ceil(sqrt(num_steps))). */
+ems_status ems_fdtd_set_checkpoint_interval(ems_fdtd *s, int interval);
+
 /* --------------------------------------------------------- structured-hex FEM
  * The fem_v2 / symplectic_2 / symplectic_4 family: order-p edge elements on a
  * structured hex grid (grid_size = fine sub-edges per axis, multiple of p),

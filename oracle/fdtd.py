@@ -68,6 +68,14 @@ def lib():
         L.ems_oracle_ricker.argtypes = [ctypes.c_double] * 3
         L.ems_oracle_gauss.restype = ctypes.c_double
         L.ems_oracle_gauss.argtypes = [ctypes.c_double] * 3
+        L.ems_oracle_fdtd_adjoint_bare_step.argtypes = [P]
+        L.ems_oracle_fdtd_get_adjoint.argtypes = [P, ctypes.c_int, dp]
+        L.ems_oracle_fdtd_set_adjoint.argtypes = [P, ctypes.c_int, dp]
+        L.ems_oracle_fdtd_get_psi.argtypes = [P, ctypes.c_int, dp]
+        L.ems_oracle_fdtd_set_psi.argtypes = [P, ctypes.c_int, dp]
+        L.ems_oracle_fdtd_uni_size.argtypes = [P]
+        L.ems_oracle_fdtd_uni_size.restype = ctypes.c_size_t
+        L.ems_oracle_fdtd_gradient.argtypes = [P, ctypes.c_int, dp, dp, dp, dp, dp, dp]
         L.ems_oracle_check_params.restype = ctypes.c_int
         L.ems_oracle_check_params.argtypes = [dp, ip, ctypes.c_double, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_double]
@@ -181,6 +189,48 @@ class OracleFDTD:
 
     def fields(self):
         return [self.field(w) for w in range(6)]
+
+    # ---- discrete adjoint (no reference counterpart)
+    def uni_size(self):
+        return lib().ems_oracle_fdtd_uni_size(self._h)
+
+    def adjoint_bare_step(self):
+        lib().ems_oracle_fdtd_adjoint_bare_step(self._h)
+
+    def adjoint(self, which):
+        n = self.uni_size() if which >= 6 else lib().ems_oracle_fdtd_field_size(self._h, which)
+        out = np.empty(n, dtype=np.float64)
+        lib().ems_oracle_fdtd_get_adjoint(self._h, which, _dp(out))
+        return out
+
+    def set_adjoint(self, which, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        lib().ems_oracle_fdtd_set_adjoint(self._h, which, _dp(v))
+
+    def psi(self, slot):
+        out = np.empty(self.uni_size(), dtype=np.float64)
+        if lib().ems_oracle_fdtd_get_psi(self._h, slot, _dp(out)) != 0:
+            raise ValueError("no CPML")
+        return out
+
+    def set_psi(self, slot, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        lib().ems_oracle_fdtd_set_psi(self._h, slot, _dp(v))
+
+    def gradient(self, nsteps, d_obs):
+        """Misfit J and its gradient with respect to eps_r and sigma (material
+        grid) of one run of nsteps from a zero state; d_obs [nobs][nsteps/tf+1][3]."""
+        nx, ny, nz = self.grid
+        nm = (nx + 1) * (ny + 1) * (nz + 1)
+        er, sg = self._mat
+        d = np.ascontiguousarray(d_obs, dtype=np.float64)
+        ge, gs = np.zeros(nm), np.zeros(nm)
+        J = ctypes.c_double()
+        lib().ems_oracle_fdtd_gradient(self._h, int(nsteps), None if er is None else _dp(er),
+                                       None if sg is None else _dp(sg), _dp(d),
+                                       ctypes.cast(ctypes.byref(J), ctypes.POINTER(ctypes.c_double)),
+                                       _dp(ge), _dp(gs))
+        return J.value, ge, gs
 
 
 def hertz(t, h, f0=1.0e9):

@@ -250,9 +250,12 @@ fail:
     return NULL;
 }
 
+void ems_oracle_fdtd_adjoint_release(ems_oracle_fdtd *s);
+
 void ems_oracle_fdtd_destroy(ems_oracle_fdtd *s)
 {
     if (!s) return;
+    ems_oracle_fdtd_adjoint_release(s);
     free(s->ex);
     free(s->ey);
     free(s->ez);
@@ -638,5 +641,447 @@ int ems_oracle_fdtd_bare_step(ems_oracle_fdtd *s)
 {
     update_h(s);
     update_e(s);
+    return 0;
+}
+
+/* ======================================================================
+ * Discrete adjoint (no reference counterpart).
+ *
+ * The forward step maps the state (E, H, psi_H, psi_E) linearly; its exact
+ * transpose, applied backwards in time, gives the gradient of the receiver
+ * misfit J = 1/2 sum (trace - d_obs)^2 with respect to the per-cell eps_r and
+ * sigma.  Everything is written over the unified grid (I, J, K), in which
+ * E*(I,J,K) = E*[K][J][I] and Hx/Hy/Hz(I,J,K) = H*[K][J][I-1] / [K][J-1][I] /
+ * [K-1][J][I]; the forward E update then reads backward differences of H and
+ * the H update forward differences of E.  Adjoint step (operation order is
+ * part of the definition; the GPU follows it):
+ *  pass A (transpose of the E update), for every updated E cell:
+ *      mu = Cb e'                 u1 = mu, u2 = -mu
+ *      per derivative: in the CPML along its axis
+ *          p = psiE' + u,  tbar = u * kinv + a * p,  psiE = b * p
+ *      else tbar = u
+ *      e = Ca e'
+ *    and for every H cell, e.g. hbar_z += (tb1_Ex(I,J,K) - tb1_Ex(I,J+1,K))
+ *                                      + (tb2_Ey(I,J,K) - tb2_Ey(I+1,J,K))
+ *  pass B (transpose of the H update), for every H cell:
+ *      nu = -ch hbar,  v1 = nu, v2 = -nu, the same CPML transpose with the
+ *      H-type profiles and psiH, and for every updated E cell, e.g.
+ *      ebar_z += (sb1_Hx(I,J-1,K) - sb1_Hx(I,J,K)) + (sb2_Hy(I-1,J,K) - sb2_Hy(I,J,K))
+ *  Terms of cells outside the update ranges are 0.
+ * ====================================================================== */
+
+typedef struct {
+    double *e[3], *h[3], *psi[12];
+} adj_state;
+
+static adj_state *adj_of(ems_oracle_fdtd *s);
+
+/* per-oracle adjoint storage, kept in a side table keyed by the handle */
+#define MAX_ADJ 64
+static struct {
+    ems_oracle_fdtd *owner;
+    adj_state st;
+} g_adj[MAX_ADJ];
+
+static adj_state *adj_of(ems_oracle_fdtd *s)
+{
+    int free_slot = -1;
+    for (int q = 0; q < MAX_ADJ; ++q) {
+        if (g_adj[q].owner == s) return &g_adj[q].st;
+        if (!g_adj[q].owner && free_slot < 0) free_slot = q;
+    }
+    if (free_slot < 0) return NULL;
+    adj_state *a = &g_adj[free_slot].st;
+    memset(a, 0, sizeof(*a));
+    const size_t n[6] = {s->nex, s->ney, s->nez, s->nhx, s->nhy, s->nhz};
+    for (int c = 0; c < 3; ++c) {
+        a->e[c] = calloc(n[c], sizeof(double));
+        a->h[c] = calloc(n[3 + c], sizeof(double));
+    }
+    for (int p = 0; p < 12; ++p) a->psi[p] = calloc(s->nuni, sizeof(double));
+    g_adj[free_slot].owner = s;
+    return a;
+}
+
+void ems_oracle_fdtd_adjoint_release(ems_oracle_fdtd *s)
+{
+    for (int q = 0; q < MAX_ADJ; ++q)
+        if (g_adj[q].owner == s) {
+            adj_state *a = &g_adj[q].st;
+            for (int c = 0; c < 3; ++c) {
+                free(a->e[c]);
+                free(a->h[c]);
+            }
+            for (int p = 0; p < 12; ++p) free(a->psi[p]);
+            g_adj[q].owner = NULL;
+        }
+}
+
+/* update ranges over the unified grid */
+static int e_in(const ems_oracle_fdtd_cfg *c, int comp, int I, int J, int K)
+{
+    if (I < 0 || J < 0 || K < 0 || I > c->nx + 1 || J > c->ny + 1 || K > c->nz + 1) return 0;
+    if (comp == 0) return I <= c->nx && J >= 1 && J <= c->ny && K >= 1 && K <= c->nz;
+    if (comp == 1) return I >= 1 && I <= c->nx && J <= c->ny && K >= 1 && K <= c->nz;
+    return I >= 1 && I <= c->nx && J >= 1 && J <= c->ny && K <= c->nz;
+}
+static int h_in(const ems_oracle_fdtd_cfg *c, int comp, int I, int J, int K)
+{
+    if (I < 0 || J < 0 || K < 0) return 0;
+    if (comp == 0) return I >= 1 && I <= c->nx && J <= c->ny && K <= c->nz;
+    if (comp == 1) return I <= c->nx && J >= 1 && J <= c->ny && K <= c->nz;
+    return I <= c->nx && J <= c->ny && K >= 1 && K <= c->nz;
+}
+static size_t e_ref(const ems_oracle_fdtd_cfg *c, int comp, int I, int J, int K)
+{
+    const size_t nx = c->nx, ny = c->ny;
+    return comp == 0 ? EX(I, J, K) : comp == 1 ? EY(I, J, K) : EZ(I, J, K);
+}
+static size_t h_ref(const ems_oracle_fdtd_cfg *c, int comp, int I, int J, int K)
+{
+    const size_t nx = c->nx, ny = c->ny;
+    return comp == 0 ? HX(I - 1, J, K) : comp == 1 ? HY(I, J - 1, K) : HZ(I, J, K - 1);
+}
+
+/* derivative axes: E comp c, derivative d (0: t1, 1: t2) -> axis; psi slot */
+static const int E_AXIS[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+static const int H_AXIS[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+
+static double cb_at(const ems_oracle_fdtd *s, int I, int J, int K)
+{
+    const size_t nx = s->c.nx, ny = s->c.ny;
+    return s->cb ? s->cb[MAT(I, J, K)] : s->c.dt / s->c.eps / s->c.h;
+}
+static double ca_at(const ems_oracle_fdtd *s, int I, int J, int K)
+{
+    const size_t nx = s->c.nx, ny = s->c.ny;
+    return (s->cb && s->has_ca) ? s->ca[MAT(I, J, K)] : 1.0;
+}
+
+/* tbar of E cell (comp, I, J, K), derivative d, from the adjoint e' and the
+ * old psiE; 0 outside the update range */
+static double tbar_e(ems_oracle_fdtd *s, adj_state *a, int comp, int d, int I, int J, int K)
+{
+    const ems_oracle_fdtd_cfg *c = &s->c;
+    const size_t nx = c->nx, ny = c->ny;
+    if (!e_in(c, comp, I, J, K)) return 0.0;
+    const double mu = cb_at(s, I, J, K) * a->e[comp][e_ref(c, comp, I, J, K)];
+    const double u = d == 0 ? mu : -mu;
+    const int ax = E_AXIS[comp][d];
+    const int n[3] = {c->nx, c->ny, c->nz};
+    const int pos[3] = {I, J, K};
+    if (!in_pml(c->npml, n[ax], pos[ax])) return u;
+    const double p = a->psi[6 + 2 * comp + d][UNI(I, J, K)] + u;
+    return u * s->kinve[ax][pos[ax]] + s->ae[ax][pos[ax]] * p;
+}
+
+static double sbar_h(ems_oracle_fdtd *s, adj_state *a, int comp, int d, int I, int J, int K)
+{
+    const ems_oracle_fdtd_cfg *c = &s->c;
+    const size_t nx = c->nx, ny = c->ny;
+    if (!h_in(c, comp, I, J, K)) return 0.0;
+    const double nu = -(c->dt / c->mu / c->h) * a->h[comp][h_ref(c, comp, I, J, K)];
+    const double v = d == 0 ? nu : -nu;
+    const int ax = H_AXIS[comp][d];
+    const int n[3] = {c->nx, c->ny, c->nz};
+    const int pos[3] = {I, J, K};
+    if (!in_pml(c->npml, n[ax], pos[ax])) return v;
+    const double p = a->psi[2 * comp + d][UNI(I, J, K)] + v;
+    return v * s->kinvh[ax][pos[ax]] + s->ah[ax][pos[ax]] * p;
+}
+
+/* the transpose of one bare step (update_h then update_e) applied to the
+ * adjoint state; `grad` (may be NULL): accumulate the material gradient from
+ * E^n = en[3], E^{n+1} = e1[3] (reference layouts) */
+static void adjoint_step(ems_oracle_fdtd *s, adj_state *a, double *const *en, double *const *e1,
+                         const double *dca_e, const double *dcbn_e, const double *dca_s,
+                         const double *dcbn_s, double *geps, double *gsig)
+{
+    const ems_oracle_fdtd_cfg *c = &s->c;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const size_t nxs = nx, nys = ny;
+    const int n3[3] = {nx, ny, nz};
+    /* ---- pass A: tbar with the old psiE, then hbar gathers, psiE, e = Ca e' */
+    double *tb[3][2];
+    for (int q = 0; q < 3; ++q)
+        for (int d = 0; d < 2; ++d) tb[q][d] = calloc(s->nuni, sizeof(double));
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I)
+                for (int q = 0; q < 3; ++q)
+                    for (int d = 0; d < 2; ++d) tb[q][d][UNI(I, J, K)] = tbar_e(s, a, q, d, I, J, K);
+    /* gradient (uses e' = the adjoint of E^{n+1}) */
+    if (geps && en && e1) {
+        for (int K = 0; K <= nz; ++K)
+            for (int J = 0; J <= ny; ++J)
+                for (int I = 0; I <= nx; ++I) {
+                    double tse = 0.0, tss = 0.0;
+                    size_t m = I + J * (nxs + 1) + (size_t)K * (nxs + 1) * (nys + 1);
+                    for (int q = 0; q < 3; ++q) {
+                        if (!e_in(c, q, I, J, K)) continue;
+                        size_t r = e_ref(c, q, I, J, K);
+                        double lam = a->e[q][r];
+                        double ca = ca_at(s, I, J, K);
+                        double de = e1[q][r] - ca * en[q][r];
+                        tse = tse + lam * (dca_e[m] * en[q][r] + dcbn_e[m] * de);
+                        tss = tss + lam * (dca_s[m] * en[q][r] + dcbn_s[m] * de);
+                    }
+                    geps[m] += tse;
+                    gsig[m] += tss;
+                }
+    }
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I) {
+                size_t u = UNI(I, J, K);
+                /* hbar gathers (forward differences of tbar) */
+                if (h_in(c, 0, I, J, K)) {
+                    double *hb = &a->h[0][h_ref(c, 0, I, J, K)];
+                    double A = tb[1][0][u], B = K <= nz ? tb[1][0][UNI(I, J, K + 1)] : 0.0;
+                    double C = tb[2][1][u], D = J <= ny ? tb[2][1][UNI(I, J + 1, K)] : 0.0;
+                    *hb = *hb + ((A - B) + (C - D));
+                }
+                if (h_in(c, 1, I, J, K)) {
+                    double *hb = &a->h[1][h_ref(c, 1, I, J, K)];
+                    double A = tb[2][0][u], B = I <= nx ? tb[2][0][UNI(I + 1, J, K)] : 0.0;
+                    double C = tb[0][1][u], D = K <= nz ? tb[0][1][UNI(I, J, K + 1)] : 0.0;
+                    *hb = *hb + ((A - B) + (C - D));
+                }
+                if (h_in(c, 2, I, J, K)) {
+                    double *hb = &a->h[2][h_ref(c, 2, I, J, K)];
+                    double A = tb[0][0][u], B = J <= ny ? tb[0][0][UNI(I, J + 1, K)] : 0.0;
+                    double C = tb[1][1][u], D = I <= nx ? tb[1][1][UNI(I + 1, J, K)] : 0.0;
+                    *hb = *hb + ((A - B) + (C - D));
+                }
+            }
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I)
+                for (int q = 0; q < 3; ++q) {
+                    if (!e_in(c, q, I, J, K)) continue;
+                    size_t r = e_ref(c, q, I, J, K);
+                    double mu = cb_at(s, I, J, K) * a->e[q][r];
+                    const int pos[3] = {I, J, K};
+                    for (int d = 0; d < 2; ++d) {
+                        int ax = E_AXIS[q][d];
+                        if (!in_pml(c->npml, n3[ax], pos[ax])) continue;
+                        double uu = d == 0 ? mu : -mu;
+                        double *ps = &a->psi[6 + 2 * q + d][UNI(I, J, K)];
+                        *ps = s->be[ax][pos[ax]] * (*ps + uu);
+                    }
+                    a->e[q][r] = ca_at(s, I, J, K) * a->e[q][r];
+                }
+    for (int q = 0; q < 3; ++q)
+        for (int d = 0; d < 2; ++d) free(tb[q][d]);
+    /* ---- pass B: sbar with the old psiH, then ebar gathers, psiH */
+    double *sb[3][2];
+    for (int q = 0; q < 3; ++q)
+        for (int d = 0; d < 2; ++d) sb[q][d] = calloc(s->nuni, sizeof(double));
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I)
+                for (int q = 0; q < 3; ++q)
+                    for (int d = 0; d < 2; ++d) sb[q][d][UNI(I, J, K)] = sbar_h(s, a, q, d, I, J, K);
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I) {
+                size_t u = UNI(I, J, K);
+                if (e_in(c, 0, I, J, K)) {
+                    double *eb = &a->e[0][e_ref(c, 0, I, J, K)];
+                    double A = sb[1][0][UNI(I, J, K - 1)], B = sb[1][0][u];
+                    double C = sb[2][1][UNI(I, J - 1, K)], D = sb[2][1][u];
+                    *eb = *eb + ((A - B) + (C - D));
+                }
+                if (e_in(c, 1, I, J, K)) {
+                    double *eb = &a->e[1][e_ref(c, 1, I, J, K)];
+                    double A = sb[0][1][UNI(I, J, K - 1)], B = sb[0][1][u];
+                    double C = sb[2][0][UNI(I - 1, J, K)], D = sb[2][0][u];
+                    *eb = *eb + ((A - B) + (C - D));
+                }
+                if (e_in(c, 2, I, J, K)) {
+                    double *eb = &a->e[2][e_ref(c, 2, I, J, K)];
+                    double A = sb[0][0][UNI(I, J - 1, K)], B = sb[0][0][u];
+                    double C = sb[1][1][UNI(I - 1, J, K)], D = sb[1][1][u];
+                    *eb = *eb + ((A - B) + (C - D));
+                }
+            }
+    for (int K = 0; K <= nz + 1; ++K)
+        for (int J = 0; J <= ny + 1; ++J)
+            for (int I = 0; I <= nx + 1; ++I)
+                for (int q = 0; q < 3; ++q) {
+                    if (!h_in(c, q, I, J, K)) continue;
+                    double nu = -(c->dt / c->mu / c->h) * a->h[q][h_ref(c, q, I, J, K)];
+                    const int pos[3] = {I, J, K};
+                    for (int d = 0; d < 2; ++d) {
+                        int ax = H_AXIS[q][d];
+                        if (!in_pml(c->npml, n3[ax], pos[ax])) continue;
+                        double vv = d == 0 ? nu : -nu;
+                        double *ps = &a->psi[2 * q + d][UNI(I, J, K)];
+                        *ps = s->bh[ax][pos[ax]] * (*ps + vv);
+                    }
+                }
+    for (int q = 0; q < 3; ++q)
+        for (int d = 0; d < 2; ++d) free(sb[q][d]);
+}
+
+int ems_oracle_fdtd_adjoint_bare_step(ems_oracle_fdtd *s)
+{
+    adj_state *a = adj_of(s);
+    if (!a) return -1;
+    adjoint_step(s, a, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
+    return 0;
+}
+
+/* adjoint arrays: which 0..5 = ebar/hbar (reference layouts), 6..17 = psi
+ * adjoint slot which-6 (unified grid) */
+int ems_oracle_fdtd_get_adjoint(ems_oracle_fdtd *s, int which, double *out)
+{
+    adj_state *a = adj_of(s);
+    if (!a) return -1;
+    if (which < 3) memcpy(out, a->e[which], sizeof(double) * ems_oracle_fdtd_field_size(s, which));
+    else if (which < 6) memcpy(out, a->h[which - 3], sizeof(double) * ems_oracle_fdtd_field_size(s, which));
+    else memcpy(out, a->psi[which - 6], sizeof(double) * s->nuni);
+    return 0;
+}
+
+int ems_oracle_fdtd_set_adjoint(ems_oracle_fdtd *s, int which, const double *in)
+{
+    adj_state *a = adj_of(s);
+    if (!a) return -1;
+    if (which < 3) memcpy(a->e[which], in, sizeof(double) * ems_oracle_fdtd_field_size(s, which));
+    else if (which < 6) memcpy(a->h[which - 3], in, sizeof(double) * ems_oracle_fdtd_field_size(s, which));
+    else memcpy(a->psi[which - 6], in, sizeof(double) * s->nuni);
+    return 0;
+}
+
+/* forward CPML psi (slot 0..11, unified grid) */
+int ems_oracle_fdtd_get_psi(ems_oracle_fdtd *s, int slot, double *out)
+{
+    if (!s->psi[slot]) return -1;
+    memcpy(out, s->psi[slot], sizeof(double) * s->nuni);
+    return 0;
+}
+int ems_oracle_fdtd_set_psi(ems_oracle_fdtd *s, int slot, const double *in)
+{
+    if (!s->psi[slot]) return -1;
+    memcpy(s->psi[slot], in, sizeof(double) * s->nuni);
+    return 0;
+}
+size_t ems_oracle_fdtd_uni_size(const ems_oracle_fdtd *s) { return s->nuni; }
+
+/* Material derivative factors per material cell (eps_r, sigma may be NULL):
+ *   s = sigma dt / (2 eps),  Ca = (1 - s)/(1 + s),  Cb = (dt/eps/h)/(1 + s)
+ *   dCa/deps_r = 2 s / (eps_r (1 + s)^2),   (dCb/deps_r)/Cb = -1 / (eps_r (1 + s))
+ *   dCa/dsigma = -dt / (eps (1 + s)^2),     (dCb/dsigma)/Cb = -dt / (2 eps (1 + s))
+ * The source amplitude dt/(eps_src (1 + s_src)) = h Cb at the source cell, so
+ * with E^{n+1} - Ca E^n = Cb (curl H) - src its derivative folds into the
+ * second term: dE^{n+1} = dCa E^n + (dCb/Cb) (E^{n+1} - Ca E^n). */
+void ems_oracle_fdtd_material_derivs(const ems_oracle_fdtd_cfg *c, size_t nmat, const double *eps_r,
+                                     const double *sigma, double *dca_e, double *dcbn_e, double *dca_s,
+                                     double *dcbn_s)
+{
+    for (size_t m = 0; m < nmat; ++m) {
+        double er = eps_r ? eps_r[m] : 1.0, sg = sigma ? sigma[m] : 0.0;
+        double e = c->eps * er;
+        double hs = sg * c->dt / (2.0 * e);
+        double op = 1.0 + hs;
+        dca_e[m] = 2.0 * hs / (er * op * op);
+        dcbn_e[m] = -1.0 / (er * op);
+        dca_s[m] = -c->dt / (e * op * op);
+        dcbn_s[m] = -c->dt / (2.0 * e * op);
+    }
+}
+
+/* Misfit and gradient of one shot from a zero state at t = 0.
+ * d_obs: [nobs][nslot][3] with nslot = nsteps/tf + 1 (slot 0 unused);
+ * components whose point list is empty are not part of the misfit.
+ * Outputs: *J, geps/gsig on the material grid (overwritten), traces. */
+int ems_oracle_fdtd_gradient(ems_oracle_fdtd *s, int nsteps, const double *eps_r, const double *sigma,
+                             const double *d_obs, double *J, double *geps, double *gsig)
+{
+    const ems_oracle_fdtd_cfg *c = &s->c;
+    adj_state *a = adj_of(s);
+    if (!a) return -1;
+    /* zero forward and adjoint state */
+    double *fl[6] = {s->ex, s->ey, s->ez, s->hx, s->hy, s->hz};
+    for (int q = 0; q < 6; ++q) memset(fl[q], 0, sizeof(double) * ems_oracle_fdtd_field_size(s, q));
+    for (int p = 0; p < 12; ++p)
+        if (s->psi[p]) memset(s->psi[p], 0, sizeof(double) * s->nuni);
+    for (int q = 0; q < 3; ++q) {
+        memset(a->e[q], 0, sizeof(double) * ems_oracle_fdtd_field_size(s, q));
+        memset(a->h[q], 0, sizeof(double) * ems_oracle_fdtd_field_size(s, q + 3));
+    }
+    for (int p = 0; p < 12; ++p) memset(a->psi[p], 0, sizeof(double) * s->nuni);
+    free(s->traces);
+    s->traces = NULL;
+    s->nslot = 0;
+    s->t_now = 0.0;
+    /* forward, keeping E at every step */
+    const size_t ne[3] = {s->nex, s->ney, s->nez};
+    double **hist = malloc(sizeof(double *) * 3 * (size_t)(nsteps + 1));
+    for (int n = 0; n <= nsteps; ++n)
+        for (int q = 0; q < 3; ++q) hist[3 * n + q] = malloc(sizeof(double) * ne[q]);
+    for (int q = 0; q < 3; ++q) memcpy(hist[q], fl[q], sizeof(double) * ne[q]);
+    for (int n = 1; n <= nsteps; ++n) {
+        ems_oracle_fdtd_run(s, 1);
+        /* run(1) records slot 1 for step 1 only when tf == 1; record by hand */
+        for (int q = 0; q < 3; ++q) memcpy(hist[3 * n + q], fl[q], sizeof(double) * ne[q]);
+    }
+    const int nslot = nsteps / c->tf + 1;
+    /* traces from the stored history (the reference's averaging) */
+    double *tr = calloc((size_t)s->nobs * nslot * 3 + 1, sizeof(double));
+    for (int slot = 1; slot < nslot; ++slot) {
+        const int n = slot * c->tf;
+        for (int o = 0; o < s->nobs; ++o)
+            for (int cc = 0; cc < 3; ++cc) {
+                int cnt = s->obs_count[3 * o + cc], off = s->obs_off[3 * o + cc];
+                double v = 0.0;
+                for (int q = 0; q < cnt; ++q) v += hist[3 * n + cc][s->obs_idx[off + q]];
+                tr[((size_t)o * nslot + slot) * 3 + cc] = v / (double)cnt;
+            }
+    }
+    free(s->traces);
+    s->traces = tr;
+    s->nslot = nslot;
+    /* misfit */
+    double jsum = 0.0;
+    for (int o = 0; o < s->nobs; ++o)
+        for (int slot = 1; slot < nslot; ++slot)
+            for (int cc = 0; cc < 3; ++cc) {
+                if (s->obs_count[3 * o + cc] == 0) continue;
+                double r = tr[((size_t)o * nslot + slot) * 3 + cc] - d_obs[((size_t)o * nslot + slot) * 3 + cc];
+                jsum += 0.5 * r * r;
+            }
+    *J = jsum;
+    /* material derivative factors */
+    const size_t nm = s->nmat;
+    double *dca_e = malloc(sizeof(double) * nm), *dcbn_e = malloc(sizeof(double) * nm);
+    double *dca_s = malloc(sizeof(double) * nm), *dcbn_s = malloc(sizeof(double) * nm);
+    ems_oracle_fdtd_material_derivs(c, nm, eps_r, sigma, dca_e, dcbn_e, dca_s, dcbn_s);
+    memset(geps, 0, sizeof(double) * nm);
+    memset(gsig, 0, sizeof(double) * nm);
+    /* backward */
+    for (int n = nsteps - 1; n >= 0; --n) {
+        const int step = n + 1;
+        if (step % c->tf == 0) {
+            const int slot = step / c->tf;
+            for (int o = 0; o < s->nobs; ++o)
+                for (int cc = 0; cc < 3; ++cc) {
+                    int cnt = s->obs_count[3 * o + cc], off = s->obs_off[3 * o + cc];
+                    if (cnt == 0) continue;
+                    double r = tr[((size_t)o * nslot + slot) * 3 + cc] - d_obs[((size_t)o * nslot + slot) * 3 + cc];
+                    double w = r / (double)cnt;
+                    for (int q = 0; q < cnt; ++q) a->e[cc][s->obs_idx[off + q]] += w;
+                }
+        }
+        adjoint_step(s, a, &hist[3 * n], &hist[3 * (n + 1)], dca_e, dcbn_e, dca_s, dcbn_s, geps, gsig);
+    }
+    for (int n = 0; n <= nsteps; ++n)
+        for (int q = 0; q < 3; ++q) free(hist[3 * n + q]);
+    free(hist);
+    free(dca_e);
+    free(dcbn_e);
+    free(dca_s);
+    free(dcbn_s);
     return 0;
 }
