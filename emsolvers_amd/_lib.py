@@ -68,6 +68,14 @@ SIGNATURES = {
     "ems_fdtd_check_params": (_I, [_DP, _IP, _D, _D, _D, _D]),
     "ems_fdtd_gradient": (_I, [_P, _I, _DP, _SZ, _DP, _DP, _DP]),
     "ems_fdtd_set_checkpoint_interval": (_I, [_P, _I]),
+    "ems_batch_create": (_I, [_PP]),
+    "ems_batch_destroy": (None, [_P]),
+    "ems_batch_add": (_I, [_P, _P]),
+    "ems_batch_set_tiling": (_I, [_P, _I, _I]),
+    "ems_batch_autotune": (_I, [_P, _I, _IP, _IP]),
+    "ems_batch_set_graph": (_I, [_P, _I]),
+    "ems_batch_run": (_I, [_P, _I]),
+    "ems_batch_run_timed": (_I, [_P, _I, _DP]),
     # structured-hex FEM
     "ems_fem_load_kmat": (_I, [ctypes.c_char_p, _DP, _SZ, _IP]),
     "ems_fem_create": (_I, [ctypes.POINTER(FemDesc), _DP, _PP]),

@@ -420,10 +420,15 @@ template <typename T> void launch_step(ems_fdtd *s, T ch, T ce, const PmlView<T>
     }
 }
 
-// Runs num_steps steps; when ev != nullptr, records HIP events around the
-// Yee kernels of every step (ev[2*step], ev[2*step+1]).
-template <typename T>
-ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+// Host side of run(): trace resize (saved_electric_field_ semantics), the
+// per-step source values and the receiver tables.  Shared by single and
+// batched runs.
+struct RunPlan {
+    int nsrc = 0, npts = 0, nrec = 0;
+    double t_end = 0.0;
+};
+
+template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan &rp, bool fused_setup)
 {
     const int tf = s->d.tf;
     // trace slots: saved_electric_field_ resize semantics (old values kept)
@@ -438,6 +443,8 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         s->traces.swap(t);
         s->nslot = nslot_new;
     }
+    rp = RunPlan();
+    rp.t_end = s->t_now;
     if (num_steps <= 0) return EMS_OK;
     EMS_TRY(setup_pml(s));
 
@@ -489,16 +496,53 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
         EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
     }
-
-    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
-    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    if (use_fused(s)) {
+    if (fused_setup && use_fused(s)) {
         EMS_TRY(ensure_alt(s));
         EMS_TRY(build_tiles(s));
     }
+    rp.nsrc = nsrc;
+    rp.npts = npts;
+    rp.nrec = nrec;
+    rp.t_end = t;
+    return EMS_OK;
+}
 
-    // timed mode: one event pair around the Yee kernels of every step plus
-    // one pair around the whole sequence; read back after the last step
+// After the device steps of a run: combine the raw receiver samples into the
+// traces exactly as the reference (saved += p for each point, then /= count).
+ems_status run_finish(ems_fdtd *s, const RunPlan &rp)
+{
+    s->t_now = rp.t_end;
+    const int nobs = (int)s->obs.size();
+    if (rp.nrec > 0 && nobs > 0) {
+        std::vector<double> raw((size_t)rp.nrec * rp.npts);
+        if (rp.npts)
+            EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < rp.nrec; ++r) {
+            const int slot = r + 1;
+            int p = 0;
+            for (int o = 0; o < nobs; ++o)
+                for (int c = 0; c < 3; ++c) {
+                    double &dst = s->traces[((size_t)o * s->nslot + slot) * 3 + c];
+                    const size_t cnt = s->obs[o][c].size();
+                    for (size_t q = 0; q < cnt; ++q, ++p) dst += raw[(size_t)r * rp.npts + p];
+                    dst /= (double)cnt;
+                }
+        }
+    }
+    return EMS_OK;
+}
+
+// Runs num_steps steps; in timed mode records HIP events around the Yee
+// kernels of every step and around the whole sequence.
+template <typename T>
+ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    RunPlan rp;
+    EMS_TRY(run_prepare<T>(s, num_steps, rp, true));
+    if (num_steps <= 0) return EMS_OK;
+    const int tf = s->d.tf, nsrc = rp.nsrc, npts = rp.npts;
+    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
     const bool timed = total_ms != nullptr;
     std::vector<hipEvent_t> ev;
     if (timed) {
@@ -538,27 +582,7 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
     EMS_HIP(hipStreamSynchronize(s->stream));
-    s->t_now = t;
-
-    // combine raw samples into the traces exactly as the reference:
-    // saved += p for each point in order, then /= count
-    if (nrec > 0 && nobs > 0) {
-        std::vector<double> raw((size_t)nrec * npts);
-        if (npts)
-            EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
-        for (int r = 0; r < nrec; ++r) {
-            const int slot = r + 1;
-            int p = 0;
-            for (int o = 0; o < nobs; ++o)
-                for (int c = 0; c < 3; ++c) {
-                    double &dst = s->traces[((size_t)o * s->nslot + slot) * 3 + c];
-                    const size_t cnt = s->obs[o][c].size();
-                    for (size_t q = 0; q < cnt; ++q, ++p) dst += raw[(size_t)r * npts + p];
-                    dst /= (double)cnt;
-                }
-        }
-    }
-    return EMS_OK;
+    return run_finish(s, rp);
 }
 
 ems_status set_material_impl(ems_fdtd *s)
@@ -978,6 +1002,282 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
 }  // namespace
 
 // =========================================================================
+// Batched runs: many handles (shots, models of uneven size) advanced together
+// with one launch per sweep (no reference counterpart).
+// =========================================================================
+struct ems_batch {
+    std::vector<ems_fdtd *> mem;
+    hipStream_t stream = nullptr;
+    int by = 8, lz = 2;      // workgroup rows, planes per thread
+    int graph_steps = 0;     // steps per captured graph (0 = plain launches)
+    DevBuf d_mem, d_post, d_step;
+    ~ems_batch()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Give a handle the coefficient arrays of a more general material mode; the
+// results do not change (Cb = dt/eps/h and Ca = 1 reproduce the uniform
+// update bit for bit).
+template <typename T> ems_status ensure_mode(ems_fdtd *s, int mode)
+{
+    if (s->mat_mode >= mode) return EMS_OK;
+    if (s->mat_mode == 0) {
+        std::vector<double> cb(s->nalloc, s->d.dt / s->d.eps / s->d.h);
+        EMS_TRY(upload_as<T>(s->cb, cb));
+    }
+    if (mode == 2) {
+        std::vector<double> ca(s->nalloc, 1.0);
+        EMS_TRY(upload_as<T>(s->ca, ca));
+    }
+    s->mat_mode = mode;
+    return EMS_OK;
+}
+
+template <typename T>
+BatchMember<T> member_of(ems_fdtd *s, bool pml, int by, int lz, long base)
+{
+    BatchMember<T> m{};
+    const GridView &g = s->g;
+    m.g = g;
+    m.f = fields<T>(s);
+    m.pm = pml_view<T>(s);
+    m.ca = s->ca.as<T>();
+    m.cb = s->cb.as<T>();
+    m.ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    m.ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    const int P = pml ? std::max(s->npml, 0) : 0;
+    m.lz = lz;
+    m.npw = P > 0 ? (2 * P + 1 + 63) / 64 : 0;
+    m.nbx = P > 0 ? m.npw + (g.nx - 2 * P + 63) / 64 : (g.nx + 1 + 63) / 64;
+    m.nby = (g.ny + 1 + by - 1) / by;
+    m.base = base;
+    return m;
+}
+
+long member_blocks(const GridView &g, int nbx, int nby, int lz) { return (long)nbx * nby * ((g.nz + 1 + lz - 1) / lz); }
+
+template <typename T>
+ems_status build_members(ems_batch *b, bool pml, std::vector<BatchMember<T>> &ms, long &total)
+{
+    ms.clear();
+    total = 0;
+    for (auto *s : b->mem) {
+        BatchMember<T> m = member_of<T>(s, pml, b->by, b->lz, total);
+        total += member_blocks(s->g, m.nbx, m.nby, b->lz);
+        ms.push_back(m);
+    }
+    EMS_TRY(b->d_mem.reserve(sizeof(BatchMember<T>) * ms.size()));
+    EMS_HIP(hipMemcpy(b->d_mem.p, ms.data(), sizeof(BatchMember<T>) * ms.size(), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+template <typename T, int MAT, bool PML>
+void batch_sweeps(ems_batch *b, long total, int nm)
+{
+    if (b->by < 1 || 64 * b->by > 512) return;  // the sweeps are compiled for at most 512 threads
+    const dim3 blk(64, b->by, 1);
+    const BatchMember<T> *m = b->d_mem.as<BatchMember<T>>();
+    k_h_batch<T, PML><<<(unsigned)total, blk, 0, b->stream>>>(m, nm);
+    k_e_batch<T, MAT, PML><<<(unsigned)total, blk, 0, b->stream>>>(m, nm);
+}
+
+template <typename T> void batch_sweeps_any(ems_batch *b, int mat, bool pml, long total, int nm)
+{
+    switch (mat * 2 + (pml ? 1 : 0)) {
+    case 0: batch_sweeps<T, 0, false>(b, total, nm); break;
+    case 1: batch_sweeps<T, 0, true>(b, total, nm); break;
+    case 2: batch_sweeps<T, 1, false>(b, total, nm); break;
+    case 3: batch_sweeps<T, 1, true>(b, total, nm); break;
+    case 4: batch_sweeps<T, 2, false>(b, total, nm); break;
+    default: batch_sweeps<T, 2, true>(b, total, nm); break;
+    }
+}
+
+struct BatchModes {
+    int mat = 0;
+    bool pml = false;
+};
+
+template <typename T> ems_status batch_modes(ems_batch *b, BatchModes &md)
+{
+    md = BatchModes();
+    for (auto *s : b->mem) {
+        EMS_TRY(setup_pml(s));
+        md.mat = std::max(md.mat, s->mat_mode);
+        md.pml = md.pml || s->npml > 0;
+    }
+    for (auto *s : b->mem) EMS_TRY(ensure_mode<T>(s, md.mat));
+    return EMS_OK;
+}
+
+template <typename T> ems_status batch_run_impl(ems_batch *b, int num_steps, double *total_ms)
+{
+    const int nm = (int)b->mem.size();
+    std::vector<RunPlan> rp(nm);
+    for (int q = 0; q < nm; ++q) EMS_TRY(run_prepare<T>(b->mem[q], num_steps, rp[q], false));
+    if (num_steps <= 0 || nm == 0) {
+        if (total_ms) *total_ms = 0.0;
+        return EMS_OK;
+    }
+    BatchModes md;
+    EMS_TRY(batch_modes<T>(b, md));
+    std::vector<BatchMember<T>> ms;
+    long total = 0;
+    EMS_TRY(build_members<T>(b, md.pml, ms, total));
+    std::vector<BatchPost> post(nm);
+    for (int q = 0; q < nm; ++q) {
+        ems_fdtd *s = b->mem[q];
+        BatchPost &p = post[q];
+        p.nsrc = rp[q].nsrc;
+        p.src_comp = s->d_src_comp.as<int>();
+        p.src_u = s->d_src_u.as<long>();
+        p.src_val = s->d_src_val.as<double>();
+        p.npts = rp[q].npts;
+        p.pt_comp = s->d_pt_comp.as<int>();
+        p.pt_u = s->d_pt_u.as<long>();
+        p.rec = (rp[q].npts && rp[q].nrec) ? s->d_rec.as<double>() : nullptr;
+        p.tf = s->d.tf;
+        p.step0 = 0;
+    }
+    EMS_TRY(b->d_post.reserve(sizeof(BatchPost) * nm));
+    EMS_HIP(hipMemcpy(b->d_post.p, post.data(), sizeof(BatchPost) * nm, hipMemcpyHostToDevice));
+    EMS_TRY(b->d_step.reserve(sizeof(int)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (total_ms) {
+        EMS_HIP(hipEventCreate(&e0));
+        EMS_HIP(hipEventCreate(&e1));
+        EMS_HIP(hipEventRecord(e0, b->stream));
+    }
+    const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
+    const BatchPost *dp = b->d_post.as<BatchPost>();
+    int done = 0;
+    const int S = b->graph_steps;
+    if (S > 0 && num_steps >= S) {
+        // one captured graph of S steps, replayed; the step index lives on the device
+        const int one = 1;
+        EMS_HIP(hipMemcpy(b->d_step.p, &one, sizeof(int), hipMemcpyHostToDevice));
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        EMS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < S; ++q) {
+            batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
+            k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, 0, b->d_step.as<int>());
+            k_step_advance<<<1, 64, 0, b->stream>>>(b->d_step.as<int>());
+        }
+        EMS_HIP(hipStreamEndCapture(b->stream, &graph));
+        EMS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        const int reps = num_steps / S;
+        for (int r = 0; r < reps; ++r) EMS_HIP(hipGraphLaunch(exec, b->stream));
+        done = reps * S;
+        EMS_HIP(hipStreamSynchronize(b->stream));
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+    }
+    for (int step = done + 1; step <= num_steps; ++step) {
+        batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
+        k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, step, nullptr);
+    }
+    EMS_HIP(hipGetLastError());
+    if (total_ms) {
+        EMS_HIP(hipEventRecord(e1, b->stream));
+        EMS_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        EMS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *total_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    EMS_HIP(hipStreamSynchronize(b->stream));
+    for (int q = 0; q < nm; ++q) EMS_TRY(run_finish(b->mem[q], rp[q]));
+    return EMS_OK;
+}
+
+// Autotuner: times the batched sweeps for workgroup rows x planes per thread
+// candidates on zeroed scratch fields (the members' own state is untouched;
+// zero inputs stay zero) and keeps the fastest.
+template <typename T> ems_status batch_autotune_impl(ems_batch *b, int trial, int *rows, int *planes)
+{
+    const int nm = (int)b->mem.size();
+    if (nm == 0) return invalid("empty batch");
+    BatchModes md;
+    EMS_TRY(batch_modes<T>(b, md));
+    size_t fmax = 0, pmax = 0;
+    for (auto *s : b->mem) {
+        fmax = std::max(fmax, s->nalloc * s->esz);
+        for (int p = 0; p < 12; ++p) pmax = std::max(pmax, s->psi[p].bytes);
+    }
+    DevBuf scratch;
+    const size_t sbytes = 6 * fmax + 12 * std::max(pmax, (size_t)256);
+    EMS_TRY(scratch.reserve(sbytes));
+    EMS_HIP(hipMemset(scratch.p, 0, sbytes));
+    char *base = static_cast<char *>(scratch.p);
+    // workgroups of 64 x rows threads: rows <= 8 keeps them within the
+    // sweeps' 512-thread launch bound
+    const int cand_by[3] = {2, 4, 8}, cand_lz[4] = {1, 2, 4, 8};
+    float best = 1e30f;
+    int bb = b->by, bl = b->lz;
+    hipEvent_t e0, e1;
+    EMS_HIP(hipEventCreate(&e0));
+    EMS_HIP(hipEventCreate(&e1));
+    for (int by : cand_by)
+        for (int lz : cand_lz) {
+            b->by = by;
+            b->lz = lz;
+            std::vector<BatchMember<T>> ms;
+            long total = 0;
+            for (auto *s : b->mem) {
+                BatchMember<T> m = member_of<T>(s, md.pml, by, lz, total);
+                total += member_blocks(s->g, m.nbx, m.nby, lz);
+                T *f[6];
+                for (int w = 0; w < 6; ++w) f[w] = reinterpret_cast<T *>(base + w * fmax);
+                m.f = Fields<T>{f[0], f[1], f[2], f[3], f[4], f[5]};
+                for (int ax = 0; ax < 3; ++ax)
+                    for (int q = 0; q < 2; ++q) {
+                        m.pm.psi_h[ax][q] = reinterpret_cast<T *>(base + 6 * fmax + (ax * 4 + q) * pmax);
+                        m.pm.psi_e[ax][q] = reinterpret_cast<T *>(base + 6 * fmax + (ax * 4 + 2 + q) * pmax);
+                    }
+                ms.push_back(m);
+            }
+            EMS_TRY(b->d_mem.reserve(sizeof(BatchMember<T>) * ms.size()));
+            EMS_HIP(hipMemcpy(b->d_mem.p, ms.data(), sizeof(BatchMember<T>) * ms.size(), hipMemcpyHostToDevice));
+            batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);  // warm-up
+            EMS_HIP(hipEventRecord(e0, b->stream));
+            for (int q = 0; q < trial; ++q) batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
+            EMS_HIP(hipEventRecord(e1, b->stream));
+            EMS_HIP(hipEventSynchronize(e1));
+            EMS_HIP(hipGetLastError());
+            float ms_ = 0.f;
+            EMS_HIP(hipEventElapsedTime(&ms_, e0, e1));
+            if (ms_ < best) {
+                best = ms_;
+                bb = by;
+                bl = lz;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->by = bb;
+    b->lz = bl;
+    if (rows) *rows = bb;
+    if (planes) *planes = bl;
+    return EMS_OK;
+}
+
+ems_status batch_check(ems_batch *b)
+{
+    if (!b) return invalid("null batch");
+    for (auto *s : b->mem)
+        if (s->d.precision != b->mem[0]->d.precision) return invalid("batch members must share one precision");
+    return EMS_OK;
+}
+
+}  // namespace
+
+// =========================================================================
 // C ABI
 // =========================================================================
 extern "C" {
@@ -1268,6 +1568,90 @@ ems_status ems_fdtd_gradient(ems_fdtd *s, int num_steps, const double *d_obs, si
     if (count != need) return invalid("d_obs must hold nobs * (num_steps/tf + 1) * 3 values");
     return s->esz == 4 ? gradient_impl<float>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma)
                        : gradient_impl<double>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma);
+}
+
+ems_status ems_batch_create(ems_batch **out)
+{
+    clear_error();
+    if (!out) return invalid("null argument");
+    *out = nullptr;
+    if (ems_device_count() <= 0) {
+        set_error("no HIP device available");
+        return EMS_ERR_HIP;
+    }
+    ems_batch *b = new ems_batch();
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+    }
+    *out = b;
+    return EMS_OK;
+}
+
+void ems_batch_destroy(ems_batch *b) { delete b; }
+
+ems_status ems_batch_add(ems_batch *b, ems_fdtd *member)
+{
+    clear_error();
+    if (!b || !member) return invalid("null argument");
+    for (auto *s : b->mem)
+        if (s == member) return invalid("handle already in the batch");
+    if (!b->mem.empty() && b->mem[0]->d.precision != member->d.precision)
+        return invalid("batch members must share one precision");
+    b->mem.push_back(member);
+    return EMS_OK;
+}
+
+ems_status ems_batch_set_tiling(ems_batch *b, int rows, int planes)
+{
+    clear_error();
+    if (!b) return invalid("null batch");
+    if (rows != 2 && rows != 4 && rows != 8) return invalid("rows must be 2, 4 or 8");
+    if (planes < 1) return invalid("planes must be >= 1");
+    b->by = rows;
+    b->lz = planes;
+    return EMS_OK;
+}
+
+ems_status ems_batch_autotune(ems_batch *b, int trial_steps, int *rows, int *planes)
+{
+    clear_error();
+    EMS_TRY(batch_check(b));
+    if (trial_steps < 1) return invalid("trial_steps must be >= 1");
+    if (b->mem.empty()) return invalid("empty batch");
+    return b->mem[0]->esz == 4 ? batch_autotune_impl<float>(b, trial_steps, rows, planes)
+                               : batch_autotune_impl<double>(b, trial_steps, rows, planes);
+}
+
+ems_status ems_batch_set_graph(ems_batch *b, int steps_per_graph)
+{
+    clear_error();
+    if (!b) return invalid("null batch");
+    if (steps_per_graph < 0) return invalid("steps_per_graph must be >= 0");
+    b->graph_steps = steps_per_graph;
+    return EMS_OK;
+}
+
+ems_status ems_batch_run(ems_batch *b, int num_steps)
+{
+    clear_error();
+    EMS_TRY(batch_check(b));
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    if (b->mem.empty()) return EMS_OK;
+    return b->mem[0]->esz == 4 ? batch_run_impl<float>(b, num_steps, nullptr)
+                               : batch_run_impl<double>(b, num_steps, nullptr);
+}
+
+ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms)
+{
+    clear_error();
+    EMS_TRY(batch_check(b));
+    if (!total_ms) return invalid("null argument");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    if (b->mem.empty()) return invalid("empty batch");
+    return b->mem[0]->esz == 4 ? batch_run_impl<float>(b, num_steps, total_ms)
+                               : batch_run_impl<double>(b, num_steps, total_ms);
 }
 
 }  // extern "C"

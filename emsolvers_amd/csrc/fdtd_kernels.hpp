@@ -114,12 +114,13 @@ __device__ __forceinline__ int split_column(const GridView &g, int P, int npw, i
 }
 
 template <typename T, bool PML>
-__global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw)
+__device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fields<T> &f, const PmlView<T> &pm,
+                                             int lz, int npw, int bx, int by, int bz)
 {
-    const int I = split_column(g, pm.P, npw, blockIdx.x, threadIdx.x);
+    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
     // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
-    const int K0 = blockIdx.z * lz;
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * lz;
     const int K1 = min(K0 + lz, g.nz + 1);
     if (I < 0 || I > g.nx || J > g.ny) return;
     const T *__restrict__ ex = f.ex;
@@ -211,13 +212,14 @@ __global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, 
 }
 
 template <typename T, int MAT, bool PML>
-__global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
-                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
+__device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fields<T> &f, const T *__restrict__ ca,
+                                             const T *__restrict__ cb, const PmlView<T> &pm, int lz, int npw, int bx,
+                                             int by, int bz)
 {
-    const int I = split_column(g, pm.P, npw, blockIdx.x, threadIdx.x);
+    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
     // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
-    const int J = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
-    const int K0 = blockIdx.z * lz;
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * lz;
     const int K1 = min(K0 + lz, g.nz + 1);
     if (I < 0 || I > g.nx || J > g.ny) return;
     T *__restrict__ ex = f.ex;
@@ -313,6 +315,110 @@ __global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, 
         cbv = n_cb;
         cav = n_ca;
     }
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw)
+{
+    h_split_body<T, PML>(g, ch, f, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename T, int MAT, bool PML>
+__global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
+                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
+{
+    e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// Batched split sweeps: many independent grids (shots, models of uneven
+// size) in one launch per sweep.  Workgroup b belongs to the member whose
+// block range [base, base + nbx*nby*nbz) contains it (binary search over the
+// members' bases); inside its member it runs the single-grid body.
+// ---------------------------------------------------------------------------
+template <typename T> struct BatchMember {
+    GridView g;
+    Fields<T> f;
+    PmlView<T> pm;
+    const T *ca, *cb;
+    T ch, ce;
+    int lz, npw, nbx, nby;
+    long base;
+};
+
+template <typename T>
+__device__ __forceinline__ int batch_find(const BatchMember<T> *__restrict__ m, int nm, long b)
+{
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m[mid].base <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(512) void k_h_batch(const BatchMember<T> *__restrict__ mem, int nm)
+{
+    const long b = blockIdx.x;
+    const BatchMember<T> &m = mem[batch_find(mem, nm, b)];
+    const long lb = b - m.base;
+    const int bx = (int)(lb % m.nbx), by = (int)((lb / m.nbx) % m.nby), bz = (int)(lb / ((long)m.nbx * m.nby));
+    h_split_body<T, PML>(m.g, m.ch, m.f, m.pm, m.lz, m.npw, bx, by, bz);
+}
+
+template <typename T, int MAT, bool PML>
+__global__ __launch_bounds__(512) void k_e_batch(const BatchMember<T> *__restrict__ mem, int nm)
+{
+    const long b = blockIdx.x;
+    const BatchMember<T> &m = mem[batch_find(mem, nm, b)];
+    const long lb = b - m.base;
+    const int bx = (int)(lb % m.nbx), by = (int)((lb / m.nbx) % m.nby), bz = (int)(lb / ((long)m.nbx * m.nby));
+    e_split_body<T, MAT, PML>(m.g, m.ce, m.f, m.ca, m.cb, m.pm, m.lz, m.npw, bx, by, bz);
+}
+
+// Sources and receivers of every member (block = member).  The step index
+// comes from `step` or, when step_ptr is set (graph replay), from the device
+// counter *step_ptr (advanced by k_step_advance after the launch).
+struct BatchPost {
+    int nsrc;
+    const int *src_comp;
+    const long *src_u;
+    const double *src_val;  // [step][nsrc], step relative to the run's first
+    int npts;
+    const int *pt_comp;
+    const long *pt_u;
+    double *rec;            // [row][npts], row = step/tf - 1 (relative)
+    int tf, step0;          // tf of the member and its run's first global step
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_post_batch(const BatchMember<T> *__restrict__ mem,
+                                                    const BatchPost *__restrict__ post, int step,
+                                                    const int *__restrict__ step_ptr)
+{
+    const BatchMember<T> &m = mem[blockIdx.x];
+    const BatchPost &q = post[blockIdx.x];
+    const int n = step_ptr ? *step_ptr : step;  // 1-based step of this run
+    T *e[3] = {m.f.ex, m.f.ey, m.f.ez};
+    if (threadIdx.x == 0) {
+        const double *v = q.src_val + (size_t)(n - 1) * q.nsrc;
+        for (int s = 0; s < q.nsrc; ++s) {
+            T *p = e[q.src_comp[s]] + q.src_u[s];
+            *p = *p - (T)v[s];
+        }
+    }
+    __syncthreads();
+    if (q.rec && n % q.tf == 0) {
+        double *r = q.rec + (size_t)(n / q.tf - 1) * q.npts;
+        for (int p = threadIdx.x; p < q.npts; p += blockDim.x) r[p] = (double)e[q.pt_comp[p]][q.pt_u[p]];
+    }
+}
+
+__global__ void k_step_advance(int *step_ptr)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_ptr = *step_ptr + 1;
 }
 
 // ---------------------------------------------------------------------------

@@ -187,6 +187,53 @@ class FemSimulation:
         return FIELD_NAMES.index(w) if isinstance(w, str) else int(w)
 
 
+class Batch:
+    """Many FemSimulation handles advanced together (one launch per sweep for
+    all members); members keep their own sources, receivers and material."""
+
+    def __init__(self, members=()):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().ems_batch_create(ctypes.byref(h)))
+        self._h = h
+        self.members = []
+        for m in members:
+            self.add(m)
+
+    def add(self, sim):
+        _lib.check(_lib.load().ems_batch_add(self._h, sim._h))
+        self.members.append(sim)
+
+    def set_tiling(self, rows, planes):
+        _lib.check(_lib.load().ems_batch_set_tiling(self._h, int(rows), int(planes)))
+
+    def autotune(self, trial_steps=5):
+        r, p = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().ems_batch_autotune(self._h, int(trial_steps), ctypes.byref(r), ctypes.byref(p)))
+        return r.value, p.value
+
+    def set_graph(self, steps_per_graph):
+        _lib.check(_lib.load().ems_batch_set_graph(self._h, int(steps_per_graph)))
+
+    def run(self, num_steps):
+        _lib.check(_lib.load().ems_batch_run(self._h, int(num_steps)))
+
+    def run_timed(self, num_steps):
+        t = ctypes.c_double()
+        _lib.check(_lib.load().ems_batch_run_timed(self._h, int(num_steps), ctypes.byref(t)))
+        return t.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().ems_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def check_params(domain_sizes, grid_num, duration, time_step, domain_size, c):
     """Reference check_params (src/fdtd/fdtd_simulation.cpp:333-366)."""
     ds = np.ascontiguousarray(domain_sizes, dtype=np.float64)

@@ -165,6 +165,36 @@ ems_status ems_fdtd_gradient(ems_fdtd *s, int num_steps, const double *d_obs, si
 /* Checkpoint interval of ems_fdtd_gradient (0 = ceil(sqrt(num_steps))). */
 ems_status ems_fdtd_set_checkpoint_interval(ems_fdtd *s, int interval);
 
+/* ------------------------------------------------------- batched FDTD runs
+ * No reference counterpart (the reference runs one shot per process).  A
+ * batch advances many FDTD handles (shots and models of uneven size, one
+ * precision) together: one launch per sweep covers every member, on the
+ * batch's own stream.  Each member keeps its own sources, receivers,
+ * material, CPML and wavelet, and after ems_batch_run(b, n) it is in exactly
+ * the state ems_fdtd_run(member, n) would have left it in (same traces and
+ * fields, bit for bit).  Members must outlive the batch. */
+typedef struct ems_batch ems_batch;
+
+ems_status ems_batch_create(ems_batch **out);
+void ems_batch_destroy(ems_batch *b);
+ems_status ems_batch_add(ems_batch *b, ems_fdtd *member);
+
+/* Workgroup rows (2, 4, 8) and planes per thread of the batched sweeps. */
+ems_status ems_batch_set_tiling(ems_batch *b, int rows, int planes);
+
+/* Times the candidate tilings (trial_steps each, on zeroed scratch fields:
+ * the members' state is untouched) and keeps the fastest; reports it. */
+ems_status ems_batch_autotune(ems_batch *b, int trial_steps, int *rows, int *planes);
+
+/* Capture steps_per_graph steps into one HIP graph and replay it (0 = plain
+ * launches); the step index is kept in device memory. */
+ems_status ems_batch_set_graph(ems_batch *b, int steps_per_graph);
+
+ems_status ems_batch_run(ems_batch *b, int num_steps);
+
+/* As ems_batch_run; *total_ms = HIP-event time of the whole step sequence. */
+ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms);
+
 /* --------------------------------------------------------- structured-hex FEM
  * The fem_v2 / symplectic_2 / symplectic_4 family: order-p edge elements on a
  * structured hex grid (grid_size = fine sub-edges per axis, multiple of p),
