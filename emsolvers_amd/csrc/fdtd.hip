@@ -941,6 +941,14 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
                 for (size_t q = 0; q < cnt; ++q, ++p) w[(size_t)r * npts + p] = res / (double)cnt;
             }
     }
+    // points that repeat are injected serially (the misfit's order)
+    int serial = 0;
+    {
+        std::vector<std::pair<int, long>> keys;
+        for (int p = 0; p < npts; ++p) keys.emplace_back(pc[p], pu[p]);
+        std::sort(keys.begin(), keys.end());
+        serial = std::adjacent_find(keys.begin(), keys.end()) != keys.end() ? 1 : 0;
+    }
     EMS_TRY(A->w.reserve(sizeof(double) * w.size()));
     EMS_HIP(hipMemcpy(A->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
     const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
@@ -978,8 +986,9 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
         for (int n = s1 - 1; n >= s0; --n) {
             Adj3<T> e_cur{{A->e[0][0].as<T>(), A->e[0][1].as<T>(), A->e[0][2].as<T>()}};
             if (npts && (n + 1) % tf == 0)
-                k_adj_inject<T><<<1, 64, 0, st>>>(e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
-                                                  A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts);
+                k_adj_inject<T><<<serial ? 1 : (npts + 255) / 256, serial ? 64 : 256, 0, st>>>(
+                    e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
+                    A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial);
             adjoint_step<T>(s, cur, seg_e<T>(s, n - s0), seg_e<T>(s, n + 1 - s0), gv);
             EMS_HIP(hipGetLastError());
             for (int c = 0; c < 3; ++c) swap_buf(A->e[0][c], A->e[1][c]);
@@ -1011,6 +1020,7 @@ struct ems_batch {
     int by = 8, lz = 2;      // workgroup rows, planes per thread
     int graph_steps = 0;     // steps per captured graph (0 = plain launches)
     DevBuf d_mem, d_post, d_step;
+    DevBuf d_post2, d_shots[2], fac, grad;  // gradient: recompute posts, shot tables, shared FP64 arrays
     ~ems_batch()
     {
         if (stream) (void)hipStreamDestroy(stream);
@@ -1264,6 +1274,276 @@ template <typename T> ems_status batch_autotune_impl(ems_batch *b, int trial, in
     b->lz = bl;
     if (rows) *rows = bb;
     if (planes) *planes = bl;
+    return EMS_OK;
+}
+
+// Traces of a fresh run from the raw samples, the misfit and the residual
+// weights (r / count per point and recorded row); shared by the gradients.
+void misfit_of(const ems_fdtd *s, int N, const RunPlan &rp, const std::vector<double> &raw, const double *d_obs,
+               std::vector<double> &tr, double &J, std::vector<double> &w, int &serial,
+               const std::vector<int> &pc, const std::vector<long> &pu)
+{
+    const int tf = s->d.tf, nobs = (int)s->obs.size(), nslot = N / tf + 1, npts = rp.npts, nrec = rp.nrec;
+    tr.assign((size_t)nobs * nslot * 3, 0.0);
+    for (int r = 0; r < nrec; ++r) {
+        int p = 0;
+        for (int o = 0; o < nobs; ++o)
+            for (int c = 0; c < 3; ++c) {
+                double v = 0.0;
+                const size_t cnt = s->obs[o][c].size();
+                for (size_t q = 0; q < cnt; ++q, ++p) v += raw[(size_t)r * npts + p];
+                tr[((size_t)o * nslot + r + 1) * 3 + c] = v / (double)cnt;
+            }
+    }
+    J = 0.0;
+    for (int o = 0; o < nobs; ++o)
+        for (int sl = 1; sl < nslot; ++sl)
+            for (int c = 0; c < 3; ++c) {
+                if (s->obs[o][c].empty()) continue;
+                const double r = tr[((size_t)o * nslot + sl) * 3 + c] - d_obs[((size_t)o * nslot + sl) * 3 + c];
+                J += 0.5 * r * r;
+            }
+    w.assign((size_t)std::max(nrec, 1) * std::max(npts, 1), 0.0);
+    for (int r = 0; r < nrec; ++r) {
+        int p = 0;
+        for (int o = 0; o < nobs; ++o)
+            for (int c = 0; c < 3; ++c) {
+                const size_t cnt = s->obs[o][c].size();
+                const double res =
+                    cnt ? tr[((size_t)o * nslot + r + 1) * 3 + c] - d_obs[((size_t)o * nslot + r + 1) * 3 + c] : 0.0;
+                for (size_t q = 0; q < cnt; ++q, ++p) w[(size_t)r * npts + p] = res / (double)cnt;
+            }
+    }
+    std::vector<std::pair<int, long>> keys;
+    for (int p = 0; p < npts; ++p) keys.emplace_back(pc[p], pu[p]);
+    std::sort(keys.begin(), keys.end());
+    serial = std::adjacent_find(keys.begin(), keys.end()) != keys.end() ? 1 : 0;
+}
+
+bool same_model(const ems_fdtd *a, const ems_fdtd *b)
+{
+    return a->d.nx == b->d.nx && a->d.ny == b->d.ny && a->d.nz == b->d.nz && a->d.h == b->d.h && a->d.dt == b->d.dt &&
+           a->d.eps == b->d.eps && a->d.mu == b->d.mu && a->d.tf == b->d.tf && a->d.precision == b->d.precision &&
+           a->npml == b->npml && a->pml_order == b->pml_order && a->pml_r0 == b->pml_r0 &&
+           a->pml_kmax == b->pml_kmax && a->pml_amax == b->pml_amax && a->eps_r == b->eps_r && a->sigma == b->sigma;
+}
+
+// Gradient of the summed misfit of all members (shots on one model): batched
+// forward and recompute sweeps, each shot's own checkpoints, and group
+// adjoint passes that update the shared FP64 gradient once per point.
+template <typename T>
+ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, double *misfits, double *Jtot,
+                               double *g_eps, double *g_sig)
+{
+    const int ns = (int)b->mem.size();
+    ems_fdtd *s0 = b->mem[0];
+    for (auto *s : b->mem)
+        if (!same_model(s0, s)) return invalid("batch gradient: members must share grid, material, CPML, dt and tf");
+    const GridView &g = s0->g;
+    const int tf = s0->d.tf;
+    BatchModes md;
+    EMS_TRY(batch_modes<T>(b, md));
+    std::vector<hipStream_t> saved(ns);
+    for (int q = 0; q < ns; ++q) {
+        saved[q] = b->mem[q]->stream;
+        b->mem[q]->stream = b->stream;
+    }
+    struct Restore {
+        ems_batch *b;
+        std::vector<hipStream_t> &sv;
+        ~Restore()
+        {
+            for (size_t q = 0; q < b->mem.size(); ++q) b->mem[q]->stream = sv[q];
+        }
+    } restore{b, saved};
+    hipStream_t st = b->stream;
+    // fresh state and run tables per shot
+    std::vector<RunPlan> rp(ns);
+    std::vector<std::vector<int>> pcs(ns);
+    std::vector<std::vector<long>> pus(ns);
+    for (int q = 0; q < ns; ++q) {
+        ems_fdtd *s = b->mem[q];
+        for (int w = 0; w < 6; ++w) EMS_HIP(hipMemsetAsync(s->f[w].p, 0, s->nalloc * s->esz, st));
+        for (int p = 0; p < 12; ++p)
+            if (s->psi[p].bytes) EMS_HIP(hipMemsetAsync(s->psi[p].p, 0, s->psi[p].bytes, st));
+        if (s->alt_ready)
+            for (int w = 0; w < 6; ++w) EMS_HIP(hipMemsetAsync(s->f2[w].p, 0, s->nalloc * s->esz, st));
+        EMS_HIP(hipStreamSynchronize(st));
+        s->t_now = 0.0;
+        s->traces.clear();
+        s->nslot = 0;
+        EMS_TRY(run_prepare<T>(s, N, rp[q], false));
+        for (auto &o : s->obs)
+            for (int c = 0; c < 3; ++c)
+                for (auto &p : o[c]) {
+                    pcs[q].push_back(c);
+                    pus[q].push_back(ref_to_unified(g, c, p[0], p[1], p[2]));
+                }
+    }
+    std::vector<BatchMember<T>> ms;
+    long total = 0;
+    EMS_TRY(build_members<T>(b, md.pml, ms, total));
+    std::vector<BatchPost> post(ns), post2(ns);
+    for (int q = 0; q < ns; ++q) {
+        ems_fdtd *s = b->mem[q];
+        BatchPost &p = post[q];
+        p.nsrc = rp[q].nsrc;
+        p.src_comp = s->d_src_comp.as<int>();
+        p.src_u = s->d_src_u.as<long>();
+        p.src_val = s->d_src_val.as<double>();
+        p.npts = rp[q].npts;
+        p.pt_comp = s->d_pt_comp.as<int>();
+        p.pt_u = s->d_pt_u.as<long>();
+        p.rec = (rp[q].npts && rp[q].nrec) ? s->d_rec.as<double>() : nullptr;
+        p.tf = tf;
+        p.step0 = 0;
+        post2[q] = p;
+        post2[q].rec = nullptr;
+    }
+    EMS_TRY(b->d_post.reserve(sizeof(BatchPost) * ns));
+    EMS_TRY(b->d_post2.reserve(sizeof(BatchPost) * ns));
+    EMS_HIP(hipMemcpy(b->d_post.p, post.data(), sizeof(BatchPost) * ns, hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(b->d_post2.p, post2.data(), sizeof(BatchPost) * ns, hipMemcpyHostToDevice));
+    const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
+    // checkpoints
+    const int K = s0->ckpt_interval > 0 ? std::min(s0->ckpt_interval, N)
+                                        : std::max(1, (int)std::ceil(std::sqrt((double)N)));
+    const int nseg = (N + K - 1) / K;
+    for (auto *s : b->mem) {
+        if (!s->adj) s->adj = new Adj();
+        Adj *A = s->adj;
+        const size_t sb = state_bytes(s);
+        if ((int)A->ckpt.size() < nseg) A->ckpt.resize(nseg);
+        for (int q = 0; q < nseg; ++q) EMS_TRY(A->ckpt[q].reserve(sb));
+        EMS_TRY(A->fin.reserve(sb));
+        EMS_TRY(A->seg.reserve((size_t)(K + 1) * 3 * s->nalloc * s->esz));
+    }
+    // forward
+    for (int q = 0; q < nseg; ++q) {
+        const int a0 = q * K, a1 = std::min(N, a0 + K);
+        const bool last = q == nseg - 1;
+        for (auto *s : b->mem) {
+            EMS_TRY(copy_state(s, s->adj->ckpt[q], true));
+            if (last) EMS_TRY(snap_e(s, 0));
+        }
+        for (int n = a0; n < a1; ++n) {
+            batch_sweeps_any<T>(b, md.mat, md.pml, total, ns);
+            k_post_batch<T><<<ns, 256, 0, st>>>(dm, b->d_post.as<BatchPost>(), n + 1, nullptr);
+            if (last)
+                for (auto *s : b->mem) EMS_TRY(snap_e(s, n + 1 - a0));
+        }
+    }
+    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin, true));
+    EMS_HIP(hipStreamSynchronize(st));
+    EMS_HIP(hipGetLastError());
+    // traces, misfits and residual weights
+    std::vector<int> serial(ns, 0);
+    double Jsum = 0.0;
+    for (int q = 0; q < ns; ++q) {
+        ems_fdtd *s = b->mem[q];
+        std::vector<double> raw((size_t)rp[q].nrec * rp[q].npts), tr, w;
+        if (!raw.empty())
+            EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        double J = 0.0;
+        misfit_of(s, N, rp[q], raw, d_obs[q], tr, J, w, serial[q], pcs[q], pus[q]);
+        if (misfits) misfits[q] = J;
+        Jsum += J;
+        EMS_TRY(s->adj->w.reserve(sizeof(double) * w.size()));
+        EMS_HIP(hipMemcpy(s->adj->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+        s->traces.swap(tr);
+        s->nslot = N / tf + 1;
+        s->t_now = rp[q].t_end;
+    }
+    if (Jtot) *Jtot = Jsum;
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    const std::vector<double> fac = material_factors(s0);
+    EMS_TRY(b->fac.reserve(sizeof(double) * fac.size()));
+    EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+    EMS_TRY(zero(b->grad, sizeof(double) * 2 * nm, st));
+    // adjoint state per shot and the two parity tables
+    const size_t fb = s0->nalloc * s0->esz;
+    std::vector<AdjShot<T>> tab[2];
+    for (auto *s : b->mem) {
+        Adj *A = s->adj;
+        for (int c = 0; c < 3; ++c) {
+            EMS_TRY(zero(A->e[0][c], fb, st));
+            EMS_TRY(zero(A->e[1][c], fb, st));
+            EMS_TRY(zero(A->h[c], fb, st));
+        }
+        if (s->npml > 0)
+            for (int ax = 0; ax < 3; ++ax)
+                for (int q = 0; q < 2; ++q)
+                    for (int bb = 0; bb < 2; ++bb) {
+                        EMS_TRY(zero(A->pe[bb][ax * 2 + q], s->psi[ax * 4 + 2 + q].bytes, st));
+                        EMS_TRY(zero(A->ph[bb][ax * 2 + q], s->psi[ax * 4 + q].bytes, st));
+                    }
+        for (int par = 0; par < 2; ++par) {
+            AdjShot<T> sh{};
+            sh.ein = Adj3<T>{{A->e[par][0].as<T>(), A->e[par][1].as<T>(), A->e[par][2].as<T>()}};
+            sh.eout = Adj3<T>{{A->e[1 - par][0].as<T>(), A->e[1 - par][1].as<T>(), A->e[1 - par][2].as<T>()}};
+            sh.hb = Adj3<T>{{A->h[0].as<T>(), A->h[1].as<T>(), A->h[2].as<T>()}};
+            sh.seg = A->seg.as<T>();
+            sh.a = adj_pml<T>(s, par);
+            tab[par].push_back(sh);
+        }
+    }
+    for (int par = 0; par < 2; ++par) {
+        EMS_TRY(b->d_shots[par].reserve(sizeof(AdjShot<T>) * ns));
+        EMS_HIP(hipMemcpy(b->d_shots[par].p, tab[par].data(), sizeof(AdjShot<T>) * ns, hipMemcpyHostToDevice));
+    }
+    GradView gv{b->fac.as<double>(), b->fac.as<double>() + nm, b->fac.as<double>() + 2 * nm,
+                b->fac.as<double>() + 3 * nm, b->grad.as<double>(), b->grad.as<double>() + nm};
+    const dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    const T ce = (T)(s0->d.dt / s0->d.eps / s0->d.h);
+    const T nch = -(T)(s0->d.dt / s0->d.mu / s0->d.h);
+    int par = 0;
+    for (int q = nseg - 1; q >= 0; --q) {
+        const int a0 = q * K, a1 = std::min(N, a0 + K);
+        if (q != nseg - 1) {
+            for (auto *s : b->mem) {
+                EMS_TRY(copy_state(s, s->adj->ckpt[q], false));
+                EMS_TRY(snap_e(s, 0));
+            }
+            for (int n = a0; n < a1; ++n) {
+                batch_sweeps_any<T>(b, md.mat, md.pml, total, ns);
+                k_post_batch<T><<<ns, 256, 0, st>>>(dm, b->d_post2.as<BatchPost>(), n + 1, nullptr);
+                for (auto *s : b->mem) EMS_TRY(snap_e(s, n + 1 - a0));
+            }
+        }
+        for (int n = a1 - 1; n >= a0; --n) {
+            if ((n + 1) % tf == 0)
+                for (int m = 0; m < ns; ++m) {
+                    ems_fdtd *s = b->mem[m];
+                    const int npts = rp[m].npts;
+                    if (!npts) continue;
+                    Adj *A = s->adj;
+                    Adj3<T> e_cur{{A->e[par][0].as<T>(), A->e[par][1].as<T>(), A->e[par][2].as<T>()}};
+                    k_adj_inject<T><<<serial[m] ? 1 : (npts + 255) / 256, serial[m] ? 64 : 256, 0, st>>>(
+                        e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
+                        A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial[m]);
+                }
+            const AdjShot<T> *sh = b->d_shots[par].as<AdjShot<T>>();
+            const T *ca = s0->ca.as<T>(), *cb = s0->cb.as<T>();
+            switch (md.mat * 2 + (md.pml ? 1 : 0)) {
+            case 0: k_adj_a_group<T, 0, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            case 1: k_adj_a_group<T, 0, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            case 2: k_adj_a_group<T, 1, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            case 3: k_adj_a_group<T, 1, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            case 4: k_adj_a_group<T, 2, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            default: k_adj_a_group<T, 2, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
+            }
+            if (md.pml) k_adj_b_group<T, true><<<grd, blk, 0, st>>>(g, nch, sh, ns);
+            else k_adj_b_group<T, false><<<grd, blk, 0, st>>>(g, nch, sh, ns);
+            EMS_HIP(hipGetLastError());
+            par = 1 - par;
+        }
+    }
+    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin, false));
+    EMS_HIP(hipStreamSynchronize(st));
+    std::vector<double> gr(2 * nm);
+    EMS_HIP(hipMemcpy(gr.data(), b->grad.p, sizeof(double) * gr.size(), hipMemcpyDeviceToHost));
+    if (g_eps) std::copy(gr.begin(), gr.begin() + nm, g_eps);
+    if (g_sig) std::copy(gr.begin() + nm, gr.end(), g_sig);
     return EMS_OK;
 }
 
@@ -1652,6 +1932,21 @@ ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms)
     if (b->mem.empty()) return invalid("empty batch");
     return b->mem[0]->esz == 4 ? batch_run_impl<float>(b, num_steps, total_ms)
                                : batch_run_impl<double>(b, num_steps, total_ms);
+}
+
+ems_status ems_batch_gradient(ems_batch *b, int num_steps, const double *const *d_obs, double *misfits,
+                              double *misfit_total, double *grad_eps_r, double *grad_sigma)
+{
+    clear_error();
+    EMS_TRY(batch_check(b));
+    if (!d_obs) return invalid("null argument");
+    if (num_steps <= 0) return invalid("num_steps must be >= 1");
+    if (b->mem.empty()) return invalid("empty batch");
+    for (size_t q = 0; q < b->mem.size(); ++q)
+        if (!d_obs[q]) return invalid("null d_obs entry");
+    return b->mem[0]->esz == 4
+               ? batch_gradient_impl<float>(b, num_steps, d_obs, misfits, misfit_total, grad_eps_r, grad_sigma)
+               : batch_gradient_impl<double>(b, num_steps, d_obs, misfits, misfit_total, grad_eps_r, grad_sigma);
 }
 
 }  // extern "C"

@@ -796,16 +796,15 @@ struct GradView {
     double *g_eps, *g_sig;
 };
 
+// Pass A at unified point (I, J, K); returns the point's gradient terms
+// (summed over its E components from 0).
 template <typename T, int MAT, bool PML, bool GRAD>
-__global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Adj3<T> eout, Adj3<T> hb,
-                                              const T *__restrict__ ca, const T *__restrict__ cb, AdjPml<T> a,
-                                              Adj3<T> en, Adj3<T> e1, GradView gv)
+__device__ __forceinline__ void adj_a_body(const GridView &g, T ce, const Adj3<T> &ein, const Adj3<T> &eout,
+                                           const Adj3<T> &hb, const T *__restrict__ ca, const T *__restrict__ cb,
+                                           const AdjPml<T> &a, const Adj3<T> &en, const Adj3<T> &e1,
+                                           const GradView &gv, int I, int J, int K, long u, long m, double &tse,
+                                           double &tss)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
     const T *ex = ein.c[0], *ey = ein.c[1], *ez = ein.c[2];
     if (h_valid(g, 0, I, J, K)) {
         const T A = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 0, I, J, K);
@@ -828,8 +827,8 @@ __global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Ad
         const T D = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 1, I + 1, J, K);
         hb.c[2][u] = hb.c[2][u] + ((A - B) + (C - D));
     }
-    double tse = 0.0, tss = 0.0;
-    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
+    tse = 0.0;
+    tss = 0.0;
     for (int c = 0; c < 3; ++c) {
         if (!e_valid(g, c, I, J, K)) continue;
         const T lam = ein.c[c][u];
@@ -855,20 +854,68 @@ __global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Ad
             tss = tss + l * (gv.dca_s[m] * e0 + gv.dcbn_s[m] * de);
         }
     }
-    if (GRAD) {
-        gv.g_eps[m] += tse;
-        gv.g_sig[m] += tss;
-    }
 }
 
-template <typename T, bool PML>
-__global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Adj3<T> hb, AdjPml<T> a)
+template <typename T, int MAT, bool PML, bool GRAD>
+__global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Adj3<T> eout, Adj3<T> hb,
+                                              const T *__restrict__ ca, const T *__restrict__ cb, AdjPml<T> a,
+                                              Adj3<T> en, Adj3<T> e1, GradView gv)
 {
     const int I = blockIdx.x * 64 + threadIdx.x;
     const int J = blockIdx.y * 4 + threadIdx.y;
     const int K = blockIdx.z;
     if (I > g.nx || J > g.ny || K > g.nz) return;
     const long u = g.off + I + g.pitch * J + g.slab * K;
+    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
+    double tse, tss;
+    adj_a_body<T, MAT, PML, GRAD>(g, ce, ein, eout, hb, ca, cb, a, en, e1, gv, I, J, K, u, m, tse, tss);
+    if (GRAD) {
+        gv.g_eps[m] += tse;
+        gv.g_sig[m] += tss;
+    }
+}
+
+// Several shots on one model: the adjoint state of each shot, shared
+// material, CPML grading and gradient.
+template <typename T> struct AdjShot {
+    Adj3<T> ein, eout, hb;
+    T *seg;       // E^n snapshots of the segment: [slot][3][nalloc]
+    AdjPml<T> a;  // psi pointers of this shot (grading shared)
+};
+
+// Pass A for every shot of a model at snapshot slot `slot` (E^n) and
+// slot + 1 (E^{n+1}); the shots' gradient terms are summed in registers
+// (shot order) and the FP64 gradient is updated once per point.
+template <typename T, int MAT, bool PML>
+__global__ __launch_bounds__(256) void k_adj_a_group(GridView g, T ce, const AdjShot<T> *__restrict__ shots, int ns,
+                                                    const T *__restrict__ ca, const T *__restrict__ cb, GradView gv,
+                                                    long nalloc, int slot)
+{
+    const int I = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I > g.nx || J > g.ny || K > g.nz) return;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
+    double se = 0.0, ss = 0.0;
+    for (int q = 0; q < ns; ++q) {
+        const AdjShot<T> &sh = shots[q];
+        T *b0 = sh.seg + (long)slot * 3 * nalloc, *b1 = b0 + 3 * nalloc;
+        const Adj3<T> en{{b0, b0 + nalloc, b0 + 2 * nalloc}}, e1{{b1, b1 + nalloc, b1 + 2 * nalloc}};
+        double tse, tss;
+        adj_a_body<T, MAT, PML, true>(g, ce, sh.ein, sh.eout, sh.hb, ca, cb, sh.a, en, e1, gv, I, J, K, u, m, tse,
+                                      tss);
+        se = se + tse;
+        ss = ss + tss;
+    }
+    gv.g_eps[m] += se;
+    gv.g_sig[m] += ss;
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ void adj_b_body(const GridView &g, T nch, const Adj3<T> &eb, const Adj3<T> &hb,
+                                           const AdjPml<T> &a, int I, int J, int K, long u)
+{
     const T *hx = hb.c[0], *hy = hb.c[1], *hz = hb.c[2];
     if (e_valid(g, 0, I, J, K)) {
         const T A = adj_sbar<T, PML>(g, a, hy, nch, 1, 0, I, J, K - 1);
@@ -909,17 +956,46 @@ __global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Ad
     }
 }
 
-// Residual injection at a recorded step (one thread, the misfit's order):
-// ebar[point] += w[point], w = (trace - d_obs) / count.
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Adj3<T> hb, AdjPml<T> a)
+{
+    const int I = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I > g.nx || J > g.ny || K > g.nz) return;
+    adj_b_body<T, PML>(g, nch, eb, hb, a, I, J, K, g.off + I + g.pitch * J + g.slab * K);
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(256) void k_adj_b_group(GridView g, T nch, const AdjShot<T> *__restrict__ shots, int ns)
+{
+    const int I = blockIdx.x * 64 + threadIdx.x;
+    const int J = blockIdx.y * 4 + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I > g.nx || J > g.ny || K > g.nz) return;
+    const long u = g.off + I + g.pitch * J + g.slab * K;
+    for (int q = 0; q < ns; ++q) adj_b_body<T, PML>(g, nch, shots[q].eout, shots[q].hb, shots[q].a, I, J, K, u);
+}
+
+// Residual injection at a recorded step: ebar[point] += w[point],
+// w = (trace - d_obs) / count.  serial != 0: one thread in the misfit's
+// order (points that repeat); else one thread per (distinct) point.
 template <typename T>
 __global__ void k_adj_inject(Adj3<T> eb, int npts, const int *__restrict__ pt_comp, const long *__restrict__ pt_u,
-                             const double *__restrict__ w)
+                             const double *__restrict__ w, int serial)
 {
-    if (threadIdx.x != 0) return;
-    for (int p = 0; p < npts; ++p) {
-        T *q = eb.c[pt_comp[p]] + pt_u[p];
-        *q = *q + (T)w[p];
+    if (serial) {
+        if (threadIdx.x != 0 || blockIdx.x != 0) return;
+        for (int p = 0; p < npts; ++p) {
+            T *q = eb.c[pt_comp[p]] + pt_u[p];
+            *q = *q + (T)w[p];
+        }
+        return;
     }
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    T *q = eb.c[pt_comp[p]] + pt_u[p];
+    *q = *q + (T)w[p];
 }
 
 }  // namespace fdtd

@@ -222,6 +222,23 @@ class Batch:
         _lib.check(_lib.load().ems_batch_run_timed(self._h, int(num_steps), ctypes.byref(t)))
         return t.value
 
+    def gradient(self, num_steps, d_obs):
+        """Shots on one model: (sum of misfits, per-member misfits, dJ/deps_r,
+        dJ/dsigma of the sum); d_obs[q] as for FemSimulation.gradient."""
+        nx, ny, nz = self.members[0].grid
+        nm = (nx + 1) * (ny + 1) * (nz + 1)
+        ds = [np.ascontiguousarray(d, dtype=np.float64) for d in d_obs]
+        if len(ds) != len(self.members):
+            raise ValueError("one d_obs per member")
+        ptrs = (_lib._DP * len(ds))(*[_lib.dptr(d) for d in ds])
+        mis = np.zeros(len(ds))
+        ge, gs = np.zeros(nm), np.zeros(nm)
+        J = ctypes.c_double()
+        _lib.check(_lib.load().ems_batch_gradient(self._h, int(num_steps), ptrs, _lib.dptr(mis),
+                                                  ctypes.cast(ctypes.byref(J), _lib._DP), _lib.dptr(ge),
+                                                  _lib.dptr(gs)))
+        return J.value, mis, ge, gs
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().ems_batch_destroy(self._h)

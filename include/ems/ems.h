@@ -195,6 +195,16 @@ ems_status ems_batch_run(ems_batch *b, int num_steps);
 /* As ems_batch_run; *total_ms = HIP-event time of the whole step sequence. */
 ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms);
 
+/* Misfit gradient of several shots on one model (members must share grid,
+ * material, CPML, dt and tf): as ems_fdtd_gradient for every member, with
+ * d_obs[q] the observed traces of member q; returns each member's misfit
+ * (misfits may be NULL), their sum, and the gradient of the sum with respect
+ * to eps_r and sigma.  The forward and recompute sweeps are batched and the
+ * adjoint passes of all shots run in one launch that updates the FP64
+ * gradient once per cell. */
+ems_status ems_batch_gradient(ems_batch *b, int num_steps, const double *const *d_obs, double *misfits,
+                              double *misfit_total, double *grad_eps_r, double *grad_sigma);
+
 /* --------------------------------------------------------- structured-hex FEM
  * The fem_v2 / symplectic_2 / symplectic_4 family: order-p edge elements on a
  * structured hex grid (grid_size = fine sub-edges per axis, multiple of p),
