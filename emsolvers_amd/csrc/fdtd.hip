@@ -6,6 +6,7 @@
 // device resident; traces stay on the device until they are read.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -78,6 +79,7 @@ struct ems_fdtd {
     // misfit gradient: checkpoint interval (0 = ceil(sqrt(steps))) and buffers
     int ckpt_interval = 0;
     struct Adj *adj = nullptr;
+    long mat_version = 0;  // set by every set_material call (unique over all handles)
     ~ems_fdtd();
 };
 
@@ -675,6 +677,7 @@ struct Adj {
     DevBuf e[2][3], h[3];       // adjoint fields (e ping-pong)
     DevBuf pe[2][6], ph[2][6];  // psi adjoints (ping-pong), compact slabs
     DevBuf fac, grad;           // 4 x nmat derivative factors, 2 x nmat gradient (FP64)
+    long fac_version = -1;      // material version the factors were built for
     DevBuf w;                   // residual weights [rows][npts]
 };
 
@@ -713,6 +716,13 @@ ems_status snap_e(ems_fdtd *s, int slot)
     return EMS_OK;
 }
 
+// Workgroups of the adjoint passes (a multiple of 8, see adj_coords).
+unsigned adj_grid(const GridView &g)
+{
+    const long total = (long)((g.nx + 1 + 63) / 64) * ((g.ny + 1 + 3) / 4) * (g.nz + 1);
+    return (unsigned)((total + 7) / 8 * 8);
+}
+
 template <typename T> Adj3<T> seg_e(ems_fdtd *s, int slot)
 {
     T *q = s->adj->seg.as<T>() + (size_t)slot * 3 * s->nalloc;
@@ -745,7 +755,7 @@ void launch_adjoint(ems_fdtd *s, int cur, Adj3<T> en, Adj3<T> e1, GradView gv)
 {
     const GridView &g = s->g;
     Adj *A = s->adj;
-    const dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    const dim3 blk(64, 4, 1), grd(adj_grid(g), 1, 1);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
     const T nch = -(T)(s->d.dt / s->d.mu / s->d.h);
     const AdjPml<T> a = adj_pml<T>(s, cur);
@@ -952,9 +962,12 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
     EMS_TRY(A->w.reserve(sizeof(double) * w.size()));
     EMS_HIP(hipMemcpy(A->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
     const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-    const std::vector<double> fac = material_factors(s);
-    EMS_TRY(A->fac.reserve(sizeof(double) * fac.size()));
-    EMS_HIP(hipMemcpy(A->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+    if (A->fac_version != s->mat_version || !A->fac.p) {
+        const std::vector<double> fac = material_factors(s);
+        EMS_TRY(A->fac.reserve(sizeof(double) * fac.size()));
+        EMS_HIP(hipMemcpy(A->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+        A->fac_version = s->mat_version;
+    }
     EMS_TRY(zero(A->grad, sizeof(double) * 2 * nm, st));
     // adjoint state
     const size_t fb = s->nalloc * s->esz;
@@ -1021,6 +1034,8 @@ struct ems_batch {
     int graph_steps = 0;     // steps per captured graph (0 = plain launches)
     DevBuf d_mem, d_post, d_step;
     DevBuf d_post2, d_shots[2], fac, grad;  // gradient: recompute posts, shot tables, shared FP64 arrays
+    const ems_fdtd *fac_owner = nullptr;     // factors cached for this member's material version
+    long fac_version = -1;
     ~ems_batch()
     {
         if (stream) (void)hipStreamDestroy(stream);
@@ -1456,9 +1471,13 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     }
     if (Jtot) *Jtot = Jsum;
     const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-    const std::vector<double> fac = material_factors(s0);
-    EMS_TRY(b->fac.reserve(sizeof(double) * fac.size()));
-    EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+    if (b->fac_owner != s0 || b->fac_version != s0->mat_version || !b->fac.p) {
+        const std::vector<double> fac = material_factors(s0);
+        EMS_TRY(b->fac.reserve(sizeof(double) * fac.size()));
+        EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+        b->fac_owner = s0;
+        b->fac_version = s0->mat_version;
+    }
     EMS_TRY(zero(b->grad, sizeof(double) * 2 * nm, st));
     // adjoint state per shot and the two parity tables
     const size_t fb = s0->nalloc * s0->esz;
@@ -1493,7 +1512,7 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     }
     GradView gv{b->fac.as<double>(), b->fac.as<double>() + nm, b->fac.as<double>() + 2 * nm,
                 b->fac.as<double>() + 3 * nm, b->grad.as<double>(), b->grad.as<double>() + nm};
-    const dim3 blk(64, 4, 1), grd((g.nx + 1 + 63) / 64, (g.ny + 1 + 3) / 4, g.nz + 1);
+    const dim3 blk(64, 4, 1), grd(adj_grid(g), 1, 1);
     const T ce = (T)(s0->d.dt / s0->d.eps / s0->d.h);
     const T nch = -(T)(s0->d.dt / s0->d.mu / s0->d.h);
     int par = 0;
@@ -1647,6 +1666,8 @@ ems_status ems_fdtd_set_material(ems_fdtd *s, const double *eps_r, const double 
     else s->eps_r.clear();
     if (sigma) s->sigma.assign(sigma, sigma + n);
     else s->sigma.clear();
+    static std::atomic<long> versions{0};
+    s->mat_version = ++versions;  // unique over all handles: keys the cached gradient factors
     return set_material_impl(s);
 }
 

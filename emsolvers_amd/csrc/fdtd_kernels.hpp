@@ -796,6 +796,25 @@ struct GradView {
     double *g_eps, *g_sig;
 };
 
+// Adjoint launches: a 1-D grid of 64 x 4 workgroups over (I, J) blocks and
+// single planes K, K fastest.  Consecutive workgroups on one XCD (the
+// hardware deals workgroups round-robin over 8 XCDs) take consecutive planes
+// of one (I, J) block, so the K-1 / K+1 neighbours they read are in that
+// XCD's L2.  gridDim.x is a multiple of 8.
+__device__ __forceinline__ bool adj_coords(const GridView &g, int &I, int &J, int &K)
+{
+    const int nkz = g.nz + 1;
+    const int nbx = (g.nx + 1 + 63) / 64, nby = (g.ny + 1 + 3) / 4;
+    const long total = (long)nbx * nby * nkz;
+    const long lb = (long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    if (lb >= total) return false;
+    K = (int)(lb % nkz);
+    const long r = lb / nkz;
+    I = (int)(r % nbx) * 64 + threadIdx.x;
+    J = (int)(r / nbx) * 4 + threadIdx.y;
+    return I <= g.nx && J <= g.ny;
+}
+
 // Pass A at unified point (I, J, K); returns the point's gradient terms
 // (summed over its E components from 0).
 template <typename T, int MAT, bool PML, bool GRAD>
@@ -861,10 +880,8 @@ __global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Ad
                                               const T *__restrict__ ca, const T *__restrict__ cb, AdjPml<T> a,
                                               Adj3<T> en, Adj3<T> e1, GradView gv)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
+    int I, J, K;
+    if (!adj_coords(g, I, J, K)) return;
     const long u = g.off + I + g.pitch * J + g.slab * K;
     const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
     double tse, tss;
@@ -891,10 +908,8 @@ __global__ __launch_bounds__(256) void k_adj_a_group(GridView g, T ce, const Adj
                                                     const T *__restrict__ ca, const T *__restrict__ cb, GradView gv,
                                                     long nalloc, int slot)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
+    int I, J, K;
+    if (!adj_coords(g, I, J, K)) return;
     const long u = g.off + I + g.pitch * J + g.slab * K;
     const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
     double se = 0.0, ss = 0.0;
@@ -959,20 +974,16 @@ __device__ __forceinline__ void adj_b_body(const GridView &g, T nch, const Adj3<
 template <typename T, bool PML>
 __global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Adj3<T> hb, AdjPml<T> a)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
+    int I, J, K;
+    if (!adj_coords(g, I, J, K)) return;
     adj_b_body<T, PML>(g, nch, eb, hb, a, I, J, K, g.off + I + g.pitch * J + g.slab * K);
 }
 
 template <typename T, bool PML>
 __global__ __launch_bounds__(256) void k_adj_b_group(GridView g, T nch, const AdjShot<T> *__restrict__ shots, int ns)
 {
-    const int I = blockIdx.x * 64 + threadIdx.x;
-    const int J = blockIdx.y * 4 + threadIdx.y;
-    const int K = blockIdx.z;
-    if (I > g.nx || J > g.ny || K > g.nz) return;
+    int I, J, K;
+    if (!adj_coords(g, I, J, K)) return;
     const long u = g.off + I + g.pitch * J + g.slab * K;
     for (int q = 0; q < ns; ++q) adj_b_body<T, PML>(g, nch, shots[q].eout, shots[q].hb, shots[q].a, I, J, K, u);
 }
