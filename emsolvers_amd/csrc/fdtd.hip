@@ -716,6 +716,13 @@ ems_status snap_e(ems_fdtd *s, int slot)
     return EMS_OK;
 }
 
+// Material versions, unique over all handles (keys of cached gradient factors).
+long next_version()
+{
+    static std::atomic<long> versions{0};
+    return ++versions;
+}
+
 // Workgroups of the adjoint passes (a multiple of 8, see adj_coords).
 unsigned adj_grid(const GridView &g)
 {
@@ -1597,6 +1604,7 @@ ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out)
     }
     ems_fdtd *s = new ems_fdtd();
     s->d = *desc;
+    s->mat_version = next_version();
     s->esz = desc->precision == EMS_F32 ? 4 : 8;
     s->g = make_grid(desc->nx, desc->ny, desc->nz);
     s->nalloc = (size_t)(s->g.off + s->g.slab * (desc->nz + 2) + kFrontPad);
@@ -1666,8 +1674,7 @@ ems_status ems_fdtd_set_material(ems_fdtd *s, const double *eps_r, const double 
     else s->eps_r.clear();
     if (sigma) s->sigma.assign(sigma, sigma + n);
     else s->sigma.clear();
-    static std::atomic<long> versions{0};
-    s->mat_version = ++versions;  // unique over all handles: keys the cached gradient factors
+    s->mat_version = next_version();  // unique over all handles: keys the cached gradient factors
     return set_material_impl(s);
 }
 
