@@ -71,6 +71,102 @@ def gpu_fem(order, stepper, n, steps, precision="f64"):
                 grid=[n] * 3, dtype="fp64" if precision == "f64" else "fp32")
 
 
+def gpu_fem1(stepper, n, steps, precision="f64"):
+    """First-order stabilized hex FEM (src/fem leapfrog, src/fem_abm) on n^3
+    cells with a Hertz dipole; DOF-updates/s of the step kernels."""
+    from emsolvers_amd.fem1 import FemSimulation as Fem, AbmSimulation as Abm
+    from emsolvers_amd.workloads import EPS0, MU0, C0
+    h = 4.0 / n
+    dt = 0.3 * h / C0
+    s = Fem((n, n, n), h, dt, EPS0, MU0, 1, 0.3, 0, precision=precision) if stepper == "fem" else \
+        Abm((n, n, n), h, dt, EPS0, MU0, 1, precision=precision)
+    s.setSourceDipole_x((n // 2, n // 2, n // 2))
+    p = (n // 2 + 5, n // 2, n // 2)
+    s.setObservationPoint([[p], [p], [p]])
+    s.run(2)
+    total_ms, kernel_ms = s.run_timed(steps)
+    nd = s.ndof()
+    return dict(value=round(nd * steps / (kernel_ms / 1e3) / 1e6, 1), unit="MDOF-updates/s", dofs=nd,
+                ms_per_step=round(kernel_ms / steps, 4), order=1, stepper=stepper, grid=[n] * 3,
+                dtype="fp64" if precision == "f64" else "fp32")
+
+
+def _shot(n, seed, model, npml=10, receivers=(8, 4), z_rec=None, src=None, f0=None, precision="f32"):
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.fdtd import FemSimulation, WAVELET_RICKER
+    h = 1e-3
+    s = FemSimulation((n, n, n), h, wl.cfl_dt(h, 0.99), wl.EPS0, wl.MU0, 1, precision=precision)
+    s.set_cpml(npml, 3.0, 1e-6, 1.0, 0.0)
+    rng = np.random.default_rng(seed)
+    s.set_wavelet(WAVELET_RICKER, f0 if f0 else rng.uniform(5e9, 15e9), -1.0, 1.0)
+    s.set_material(*model)
+    if src is None:
+        src = tuple(int(v) for v in rng.integers(n // 4, 3 * n // 4, 3))
+    s.setSource_z(src)
+    ra, rb = receivers
+    for a in range(ra):
+        for b in range(rb):
+            p = (n // 8 + a * (3 * n // 4) // max(ra - 1, 1), n // 8 + b * (3 * n // 4) // max(rb - 1, 1),
+                 z_rec if z_rec is not None else n // 2)
+            s.setObservationPoint([[p], [p], [p]])
+    return s
+
+
+def gpu_batch_forward(steps, per_model=16, sizes=(96, 128, 192, 256)):
+    """Shots x models of uneven size (ellipsoid eps_r, CPML 10, Ricker f0 in
+    [5, 15] GHz, 32 receivers each): sequential single-shot runs against one
+    batch (autotuned tiling)."""
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.fdtd import Batch
+    models = {n: (wl.ellipsoid_model(n, n), None) for n in sizes}
+    mem = [_shot(n, 1000 * n + i, models[n]) for n in sizes for i in range(per_model)]
+    cells = sum(m.grid[0] * m.grid[1] * m.grid[2] for m in mem)
+    for m in mem:
+        m.run(1)
+    t0 = time.perf_counter()
+    for m in mem:
+        m.run(steps)
+    seq = time.perf_counter() - t0
+    b = Batch(mem)
+    rows, planes = b.autotune(3)
+    b.run(1)
+    bat_ms = b.run_timed(steps)
+    return dict(shots=len(mem), models=list(sizes), steps=steps, cells_per_step=cells,
+                sequential_s=round(seq, 4), batched_s=round(bat_ms / 1e3, 4),
+                sequential_gcells=round(cells * steps / seq / 1e9, 2),
+                batched_gcells=round(cells * steps / (bat_ms / 1e3) / 1e9, 2),
+                ratio=round(seq / (bat_ms / 1e3), 3), tiling=dict(rows=rows, planes=planes))
+
+
+def gpu_batch_gradient(n=192, shots=16, steps=100):
+    """Misfit gradient (eps_r and sigma) of `shots` shots on a 4 x 4 source grid
+    at z = 20 with 64 receivers on the plane z = n - 20, background eps_r 2 with
+    three Gaussian blobs: sequential single-shot gradients against one batched
+    gradient.  Wall-clock, second call of each (buffers allocated)."""
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.fdtd import Batch
+    model = wl.blob_model(n, 7)
+    obs = np.zeros((64, steps + 1, 3))
+    srcs = [(n // 2 - 45 + 30 * (i % 4), n // 2 - 45 + 30 * (i // 4), 20) for i in range(shots)]
+    sims = [_shot(n, i, model, receivers=(8, 8), z_rec=n - 20, src=srcs[i], f0=10e9) for i in range(shots)]
+    for s in sims:
+        s.gradient(steps, obs)
+    t0 = time.perf_counter()
+    for s in sims:
+        s.gradient(steps, obs)
+    seq = time.perf_counter() - t0
+    del sims
+    mem = [_shot(n, i, model, receivers=(8, 8), z_rec=n - 20, src=srcs[i], f0=10e9) for i in range(shots)]
+    b = Batch(mem)
+    b.gradient(steps, [obs] * shots)
+    t0 = time.perf_counter()
+    b.gradient(steps, [obs] * shots)
+    bat = time.perf_counter() - t0
+    return dict(grid=[n] * 3, shots=shots, steps=steps, sequential_s=round(seq, 3), batched_s=round(bat, 3),
+                ratio=round(seq / bat, 3), ms_per_shot_step_batched=round(bat * 1e3 / shots / steps, 4),
+                dtype="fp32 fields, fp64 gradient")
+
+
 def pmc_traffic(path):
     """HBM bytes per step from a committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a device copy of
@@ -144,6 +240,7 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 split, 2 fused")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-fem", action="store_true", help="skip the FEM secondary figures")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched forward / gradient figures")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -206,7 +303,11 @@ def main():
     }
     if not args.no_fem:
         out["fem"] = [gpu_fem(2, "symplectic_4", 128, 20), gpu_fem(1, "fem_v2", 128, 50),
-                      gpu_fem(1, "symplectic_2", 128, 50, "f32")]
+                      gpu_fem(1, "symplectic_2", 128, 50, "f32"), gpu_fem1("fem", 128, 50),
+                      gpu_fem1("fem_abm", 128, 30)]
+    if not args.no_batch:
+        out["batched_forward"] = gpu_batch_forward(20)
+        out["batched_gradient"] = gpu_batch_gradient()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_reference_sample(args.n)

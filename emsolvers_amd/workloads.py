@@ -40,6 +40,25 @@ def ellipsoid_model(n, seed):
     return er.reshape(-1)
 
 
+def blob_model(n, seed, background=2.0, count=3, d_eps=0.5, width=12.0, d_sigma=0.01):
+    """(eps_r, sigma) on the material grid: background eps_r plus `count`
+    seeded Gaussian blobs of amplitude d_eps (eps_r) and d_sigma (S/m) and
+    standard deviation `width` cells."""
+    rng = np.random.default_rng(seed)
+    c1 = np.arange(n + 1, dtype=np.float64) - 0.5
+    er = np.full((n + 1, n + 1, n + 1), background)
+    sg = np.zeros((n + 1, n + 1, n + 1))
+    for _ in range(count):
+        c = rng.uniform(0.3 * n, 0.7 * n, 3)
+        gz = np.exp(-0.5 * ((c1 - c[2]) / width) ** 2)
+        gy = np.exp(-0.5 * ((c1 - c[1]) / width) ** 2)
+        gx = np.exp(-0.5 * ((c1 - c[0]) / width) ** 2)
+        blob = gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+        er += d_eps * blob
+        sg += d_sigma * blob
+    return er.reshape(-1), sg.reshape(-1)
+
+
 def shot(n, seed):
     """Source cell (Ez), Ricker f0 and 32 receivers (8 x 4 plane) of one shot."""
     rng = np.random.default_rng(seed + 1000003)
