@@ -35,3 +35,22 @@ def rel_l2(a, b):
 @pytest.fixture(scope="session")
 def golden():
     return load_golden
+
+
+def tf_misfits(traces, ref, dt):
+    """Largest single-valued TF envelope and phase misfits (Kristekova et al.)
+    over the observation components of traces [nobs][nslot][3] against the
+    reference traces; dt is the slot spacing.  Components that are not finite
+    or are zero in the reference are skipped."""
+    from emsolvers_amd import tfmisfit
+    nslot = ref.shape[1]
+    fmin, fmax = 2.0 / (nslot * dt), 0.25 / dt
+    em = pm = 0.0
+    for o in range(ref.shape[0]):
+        for c in range(ref.shape[2]):
+            r, s = ref[o, :, c], traces[o, :, c]
+            if not np.all(np.isfinite(r)) or not np.any(r):
+                continue
+            m = tfmisfit.misfits(s, r, dt, fmin, fmax, nf=40)
+            em, pm = max(em, m["EM"]), max(pm, m["PM"])
+    return em, pm

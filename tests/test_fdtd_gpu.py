@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.conftest import rel_l2
+from tests.conftest import rel_l2, tf_misfits
 from tests import fdtd_cases as fc
 
 pytestmark = pytest.mark.gpu
@@ -97,6 +97,9 @@ def test_fp32_error_against_fp64_oracle(oracle_runs, name, kernel):
     err = rel_l2(ts[fin], to[fin])
     print("fp32 trace rel L2 %s kernel %d: %.3e" % (name, kernel, err))
     assert err <= FP32_TRACE_TOL
+    em, pm = tf_misfits(ts, to, fc.cfl_dt(case["h"], case["frac"]) * case["tf"])
+    print("fp32 TF envelope / phase misfit %s kernel %d: EM %.2e PM %.2e" % (name, kernel, em, pm))
+    assert em <= 1e-4 and pm <= 1e-4
     for w in range(6):
         assert rel_l2(s.field(w), o.field(w)) <= 1e-4, NAMES[w]
 

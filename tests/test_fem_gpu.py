@@ -8,7 +8,7 @@ order); FP32 traces within 1e-5 relative L2 of the FP64 oracle.
 import numpy as np
 import pytest
 
-from tests.conftest import rel_l2
+from tests.conftest import rel_l2, tf_misfits
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,10 @@ def test_fp32_error_against_fp64_oracle(oracle_cache, key):
     err = rel_l2(g.traces()[..., 1:], o.traces()[..., 1:])
     print("fp32 trace rel L2 p%d s%d: %.3e" % (order, stepper, err))
     assert err <= 1e-5
+    dt = g.traces()[0, 1, 0] - g.traces()[0, 0, 0]
+    em, pm = tf_misfits(g.traces()[..., 1:], o.traces()[..., 1:], dt)
+    print("fp32 TF envelope / phase misfit p%d s%d: EM %.2e PM %.2e" % (order, stepper, em, pm))
+    assert em <= 1e-4 and pm <= 1e-4
     assert np.array_equal(g.traces()[..., 0], o.traces()[..., 0])
 
 
