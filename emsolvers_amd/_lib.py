@@ -41,6 +41,11 @@ class Fem1Desc(ctypes.Structure):
                 ("tf", _I), ("stepper", _I), ("stab", _D), ("use_ofem", _I), ("precision", _I)]
 
 
+class TetDesc(ctypes.Structure):
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I), ("h", _D), ("dt", _D), ("eps", _D), ("mu", _D),
+                ("tf", _I), ("stepper", _I), ("order", _I), ("precision", _I)]
+
+
 SIGNATURES = {
     "ems_last_error": (ctypes.c_char_p, []),
     "ems_version": (ctypes.c_char_p, []),
@@ -111,6 +116,21 @@ SIGNATURES = {
     "ems_fem1_matrix": (_I, [_P, _DP]),
     "ems_fem1_save_results": (_I, [_P, _I, ctypes.c_char_p]),
     "ems_fem1_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
+    # tetrahedral edge elements
+    "ems_tet_create": (_I, [ctypes.POINTER(TetDesc), _PP]),
+    "ems_tet_destroy": (None, [_P]),
+    "ems_tet_set_material": (_I, [_P, _DP, _DP]),
+    "ems_tet_add_dipole": (_I, [_P, _DP, _DP, _D, _D, _D]),
+    "ems_tet_add_receiver": (_I, [_P, _DP]),
+    "ems_tet_run": (_I, [_P, _I]),
+    "ems_tet_run_timed": (_I, [_P, _I, _DP]),
+    "ems_tet_trace_shape": (_I, [_P, _IP, _IP]),
+    "ems_tet_get_traces": (_I, [_P, _DP, _SZ]),
+    "ems_tet_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
+    "ems_tet_get_field": (_I, [_P, _I, _DP, _SZ]),
+    "ems_tet_set_field": (_I, [_P, _I, _DP, _SZ]),
+    "ems_tet_get_time": (_I, [_P, _DP]),
+    "ems_tet_solver_stats": (_I, [_P, _IP, _IP]),
 }
 
 _lib = None

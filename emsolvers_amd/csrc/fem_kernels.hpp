@@ -154,6 +154,24 @@ __device__ __forceinline__ T row_dot(const FemGrid &g, const T *__restrict__ K, 
     elem_bases(g, i, j, k, bx, by, bz);
     const T *__restrict__ kr = K + (long)r * N;
     T temp = 0;
+    if constexpr (P == 1) {
+        // first order: the 12 offsets in the local order of the table
+        // (x: +Y, +Z; y: +X, +Z; z: +X, +Y), without the table lookups
+        const long Nx = g.Nx, Ny = g.Ny, sx = Nx * (Ny + 1), sy = (Nx + 1) * Ny;
+        temp += kr[0] * gval<T, DRIFT>(u.x, pm.x, bx, hd, mm);
+        temp += kr[1] * gval<T, DRIFT>(u.x, pm.x, bx + Nx, hd, mm);
+        temp += kr[2] * gval<T, DRIFT>(u.x, pm.x, bx + sx, hd, mm);
+        temp += kr[3] * gval<T, DRIFT>(u.x, pm.x, bx + sx + Nx, hd, mm);
+        temp += kr[4] * gval<T, DRIFT>(u.y, pm.y, by, hd, mm);
+        temp += kr[5] * gval<T, DRIFT>(u.y, pm.y, by + 1, hd, mm);
+        temp += kr[6] * gval<T, DRIFT>(u.y, pm.y, by + sy, hd, mm);
+        temp += kr[7] * gval<T, DRIFT>(u.y, pm.y, by + sy + 1, hd, mm);
+        temp += kr[8] * gval<T, DRIFT>(u.z, pm.z, bz, hd, mm);
+        temp += kr[9] * gval<T, DRIFT>(u.z, pm.z, bz + 1, hd, mm);
+        temp += kr[10] * gval<T, DRIFT>(u.z, pm.z, bz + Nx + 1, hd, mm);
+        temp += kr[11] * gval<T, DRIFT>(u.z, pm.z, bz + Nx + 2, hd, mm);
+        return temp;
+    }
 #pragma unroll 4
     for (int m = 0; m < D; ++m) temp += kr[m] * gval<T, DRIFT>(u.x, pm.x, bx + off[m], hd, mm);
 #pragma unroll 4

@@ -1,0 +1,844 @@
+// First-order Nedelec (Whitney) edge elements on the Kuhn tetrahedral split
+// of a hex grid (no reference counterpart; nearest hooks: the order-p hex
+// operator of src/fem_v2/fem_kernels.hpp:13-175 and setSourceDipole_x of
+// src/fem/fem_simulation.cpp:454).
+//
+//   M_eps U'' + M_sig U' + S U = -g(t),  g_i = p'(t) (w_i(r0) . d)  (dipole d at r0)
+//   M_eps = sum_T eps0 eps_r(T) M_T,  M_sig = sum_T sig(T) M_T,  S = sum_T S_T / mu
+// with PEC (tangential E = 0) on the outer faces.  The consistent mass is
+// solved, not lumped: a Jacobi-preconditioned conjugate gradient on the
+// device (reductions in a fixed order, convergence tested on the device).
+// Steppers (U = E, V = E'):
+//   symplectic_2 / _4 (the stage of src/symplectic_2/fem_simulation.cpp:557-672
+//   with the mass solve, semi-implicit in sigma):
+//      U += (D/2) V;  (M_eps + D/2 M_sig) V' = (M_eps - D/2 M_sig) V - D (S U + g(t + o + D/2));
+//      U += (D/2) V'
+//     symplectic_4: stages (x1 dt, 0), (x2 dt, x1 dt), (x1 dt, (x1+x2) dt)
+//   ABM (the AB2 predictor / AM3 corrector of src/fem_abm/fem_simulation.cpp:
+//   278-833 on y = (U, V), y' = (V, a), M_eps a = -S U - M_sig V - g):
+//      y_p = y_n + dt/2 (3 F_n - F_{n-1});  y_{n+1} = y_n + dt/12 (5 F(y_p) + 8 F_n - F_{n-1})
+// The matrices share one CSR pattern, assembled on the host in a fixed
+// order (entries summed in (row, col, tet) order).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "common.hpp"
+#include "tet_mesh.hpp"
+
+using namespace ems;
+using namespace ems::tet;
+
+namespace {
+
+constexpr int kRB = 256;  // reduction block
+
+template <typename T> struct Csr {
+    int n;
+    const int *rp, *col;
+};
+
+// y = A x, one row per thread, summed in CSR order
+template <typename T>
+__global__ void k_spmv(Csr<T> A, const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    T s = 0;
+    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) s += val[q] * x[A.col[q]];
+    y[r] = s;
+}
+
+// b = Me V - h Ms V - D S U (lossless: no Ms term)
+template <typename T, bool LOSSY>
+__global__ void k_rhs(Csr<T> A, const T *__restrict__ me, const T *__restrict__ ms, const T *__restrict__ st,
+                      const T *__restrict__ U, const T *__restrict__ V, T hh, T D, T *__restrict__ b)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    T a = 0, c = 0, s = 0;
+    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) {
+        const int j = A.col[q];
+        a += me[q] * V[j];
+        if (LOSSY) c += ms[q] * V[j];
+        s += st[q] * U[j];
+    }
+    b[r] = LOSSY ? (a - hh * c) - D * s : a - D * s;
+}
+
+// b = -S U - Ms V (the ABM right-hand side)
+template <typename T, bool LOSSY>
+__global__ void k_force(Csr<T> A, const T *__restrict__ ms, const T *__restrict__ st, const T *__restrict__ U,
+                        const T *__restrict__ V, T *__restrict__ b)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    T c = 0, s = 0;
+    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) {
+        const int j = A.col[q];
+        s += st[q] * U[j];
+        if (LOSSY) c += ms[q] * V[j];
+    }
+    b[r] = LOSSY ? -s - c : -s;
+}
+
+// b[dof] += f[dipole] * w (dipole loads, 6 weights per dipole), one thread
+// in dipole order so that loads sharing an edge add up in a fixed order
+template <typename T>
+__global__ void k_load(T *b, int nw, const int *__restrict__ dof, const double *__restrict__ w,
+                       const double *__restrict__ f)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    for (int q = 0; q < nw; ++q) b[dof[q]] = b[dof[q]] + (T)(f[q / 6] * w[q]);
+}
+
+// Scalars of one CG solve (device resident)
+struct CgScalars {
+    double rz, rz0, alpha, beta, pap;
+    int done, iters;
+};
+
+// partial dot products of block b: sum over its index range in order
+template <typename T>
+__device__ __forceinline__ void block_dot(const T *a, const T *c, int n, double *partial)
+{
+    __shared__ double sh[kRB];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += (double)a[i] * (double)c[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kRB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__device__ __forceinline__ double sum_partials(const double *p, int nb)
+{
+    double s = 0.0;
+    for (int q = 0; q < nb; ++q) s += p[q];
+    return s;
+}
+
+// CG start: r = b - A x, z = dinv r, p = z, partials (r, z) and (b, dinv b)
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_cg_init(Csr<T> A, const T *__restrict__ val, const T *__restrict__ dinv,
+                                                 const T *__restrict__ b, const T *__restrict__ x, T *r, T *z, T *p,
+                                                 T *zb, double *partial)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        T s = 0;
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) s += val[q] * x[A.col[q]];
+        const T ri = b[i] - s;
+        r[i] = ri;
+        z[i] = dinv[i] * ri;
+        p[i] = z[i];
+        zb[i] = dinv[i] * b[i];
+    }
+    __syncthreads();
+    block_dot(r, z, A.n, partial);
+    __syncthreads();
+    block_dot(b, zb, A.n, partial + gridDim.x);
+}
+
+// stop when (r, M^-1 r) <= tol^2 (b, M^-1 b) (b = 0: relative to the start)
+__global__ void k_cg_start(CgScalars *c, const double *partial, int nb, double tol2)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    c->rz = sum_partials(partial, nb);
+    const double rb = sum_partials(partial + nb, nb);
+    c->rz0 = rb > 0.0 ? rb : c->rz;
+    c->iters = 0;
+    c->done = !(c->rz > tol2 * c->rz0);
+}
+
+// Ap = A p and partial (p, Ap)
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_cg_ap(Csr<T> A, const T *__restrict__ val, const T *__restrict__ p, T *ap,
+                                               double *partial, const CgScalars *c)
+{
+    if (c->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
+        T s = 0;
+        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) s += val[q] * p[A.col[q]];
+        ap[i] = s;
+    }
+    __syncthreads();
+    block_dot(p, ap, A.n, partial);
+}
+
+__global__ void k_cg_alpha(CgScalars *c, const double *partial, int nb)
+{
+    if (threadIdx.x || blockIdx.x || c->done) return;
+    c->pap = sum_partials(partial, nb);
+    c->alpha = c->rz / c->pap;
+}
+
+// x += a p, r -= a Ap, z = dinv r, partial (r, z)
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, const T *__restrict__ p,
+                                                   const T *__restrict__ ap, const T *__restrict__ dinv,
+                                                   double *partial, const CgScalars *c)
+{
+    if (c->done) return;
+    const T a = (T)c->alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] = x[i] + a * p[i];
+        r[i] = r[i] - a * ap[i];
+        z[i] = dinv[i] * r[i];
+    }
+    __syncthreads();
+    block_dot(r, z, n, partial);
+}
+
+__global__ void k_cg_beta(CgScalars *c, const double *partial, int nb, double tol2)
+{
+    if (threadIdx.x || blockIdx.x || c->done) return;
+    const double rz = sum_partials(partial, nb);
+    c->beta = rz / c->rz;
+    c->rz = rz;
+    c->iters += 1;
+    if (!(rz > tol2 * c->rz0)) c->done = 1;
+}
+
+template <typename T>
+__global__ void k_cg_dir(int n, T *p, const T *__restrict__ z, const CgScalars *c)
+{
+    if (c->done) return;
+    const T b = (T)c->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + b * p[i];
+}
+
+// y += a x
+template <typename T> __global__ void k_axpy(int n, T a, const T *__restrict__ x, T *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + a * x[i];
+}
+
+// y = c0 + k (a1 x1 + a2 x2 + a3 x3), in that order
+template <typename T>
+__global__ void k_comb(int n, const T *__restrict__ c0, T k, T a1, const T *__restrict__ x1, T a2,
+                       const T *__restrict__ x2, T a3, const T *__restrict__ x3, T *y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T s = a1 * x1[i] + a2 * x2[i];
+    if (x3) s = s + a3 * x3[i];
+    y[i] = c0[i] + k * s;
+}
+
+// receivers: E(r) = sum_l U[dof_l] w_l(r), 3 components, into rec[obs][3]
+template <typename T>
+__global__ void k_tet_record(const T *__restrict__ U, int nobs, const int *__restrict__ dof,
+                             const double *__restrict__ w, double *rec)
+{
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nobs) return;
+    for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int l = 0; l < 6; ++l) s += (double)U[dof[6 * o + l]] * w[(6 * o + l) * 3 + c];
+        rec[3 * o + c] = s;
+    }
+}
+
+}  // namespace
+
+struct ems_tet {
+    ems_tet_desc d{};
+    Mesh m;
+    size_t esz = 8;
+    std::vector<double> eps_r, sigma;  // per tet (empty: 1 / 0)
+    std::vector<int> rp, col;
+    std::vector<char> bnd;
+    DevBuf d_rp, d_col, d_S, d_Me, d_Ms;
+    std::vector<double> deltas;  // stage steps with an assembled A_+
+    DevBuf d_A[3], d_dinv[3];
+    DevBuf U, V, b, r, z, p, ap;        // state and CG scratch
+    DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
+    DevBuf d_partial, d_cg;
+    bool assembled = false, lossy = false;
+    // dipoles: per dipole 6 dofs and the load weights w_l(r0).d
+    std::vector<int> dip_dof;
+    std::vector<double> dip_w, dip_par;  // par: f0, t0, amp per dipole
+    DevBuf d_dip_dof, d_dip_w, d_dip_f;
+    std::vector<int> obs_dof;
+    std::vector<double> obs_w;
+    DevBuf d_obs_dof, d_obs_w, d_rec;
+    std::vector<double> traces;  // [nobs][nslot][3]
+    int nslot = 0;
+    long steps = 0;  // steps taken
+    double t_now = 0.0;
+    bool abm_started = false;
+    int cg_last = 0, cg_max = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+double pulse_rate(double t, double f0, double t0, double amp)
+{
+    // p(t) = amp exp(-(pi f0 (t - t0))^2),  returns p'(t)
+    const double a = M_PI * f0 * (t - t0);
+    return amp * (-2.0 * M_PI * f0 * a) * std::exp(-a * a);
+}
+
+// CSR pattern and matrix values (host, fixed summation order)
+ems_status assemble(ems_tet *s, std::vector<double> &S, std::vector<double> &Me, std::vector<double> &Ms)
+{
+    const Mesh &m = s->m;
+    const long ne = m.ne;
+    s->bnd.assign(ne, 0);
+    for (long e = 0; e < ne; ++e) s->bnd[e] = m.boundary(e) ? 1 : 0;
+    double Mt[6][6][6], St[6][6][6];
+    for (int t = 0; t < 6; ++t) whitney(t, m.h, Mt[t], St[t]);
+    // triplets (row, col, tet*36 + l*6 + m) of interior couplings
+    struct Tr {
+        int r, c;
+        long k;
+    };
+    std::vector<Tr> tr;
+    tr.reserve((size_t)m.nt * 36);
+    long tid = 0;
+    for (int k = 0; k < m.nz; ++k)
+        for (int j = 0; j < m.ny; ++j)
+            for (int i = 0; i < m.nx; ++i)
+                for (int t = 0; t < 6; ++t, ++tid) {
+                    long e[6];
+                    m.tet_edges(i, j, k, t, e);
+                    for (int l = 0; l < 6; ++l)
+                        for (int q = 0; q < 6; ++q)
+                            if (!s->bnd[e[l]] && !s->bnd[e[q]]) tr.push_back(Tr{(int)e[l], (int)e[q], tid * 36 + l * 6 + q});
+                }
+    std::stable_sort(tr.begin(), tr.end(), [](const Tr &a, const Tr &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    s->rp.assign(ne + 1, 0);
+    s->col.clear();
+    S.clear();
+    Me.clear();
+    Ms.clear();
+    size_t q = 0;
+    const double mu = s->d.mu, e0 = s->d.eps;
+    for (long r = 0; r < ne; ++r) {
+        s->rp[r] = (int)s->col.size();
+        if (s->bnd[r]) {  // identity row of the mass-type matrices, zero in S
+            s->col.push_back((int)r);
+            S.push_back(0.0);
+            Me.push_back(1.0);
+            Ms.push_back(0.0);
+            continue;
+        }
+        while (q < tr.size() && tr[q].r == r) {
+            const int c = tr[q].c;
+            double vs = 0.0, vm = 0.0, vg = 0.0;
+            for (; q < tr.size() && tr[q].r == r && tr[q].c == c; ++q) {
+                const long tt = tr[q].k / 36;
+                const int l = (int)((tr[q].k % 36) / 6), mm = (int)(tr[q].k % 6);
+                const int type = (int)(tt % 6);
+                const double er = s->eps_r.empty() ? 1.0 : s->eps_r[tt];
+                const double sg = s->sigma.empty() ? 0.0 : s->sigma[tt];
+                vs += St[type][l][mm] / mu;
+                vm += (e0 * er) * Mt[type][l][mm];
+                vg += sg * Mt[type][l][mm];
+            }
+            s->col.push_back(c);
+            S.push_back(vs);
+            Me.push_back(vm);
+            Ms.push_back(vg);
+        }
+    }
+    s->rp[ne] = (int)s->col.size();
+    return EMS_OK;
+}
+
+template <typename T> ems_status upload(DevBuf &d, const std::vector<double> &v)
+{
+    std::vector<T> t(v.begin(), v.end());
+    EMS_TRY(d.reserve(sizeof(T) * std::max<size_t>(t.size(), 1)));
+    if (!t.empty()) EMS_HIP(hipMemcpy(d.p, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+std::vector<double> stage_deltas(const ems_tet *s)
+{
+    const double dt = s->d.dt;
+    if (s->d.stepper == EMS_TET_SYMPLECTIC4) {
+        const double x1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0)), x2 = 1.0 - 2.0 * x1;
+        return {x1 * dt, x2 * dt};
+    }
+    if (s->d.stepper == EMS_TET_ABM) return {0.0};
+    return {dt};
+}
+
+template <typename T> ems_status setup(ems_tet *s)
+{
+    if (s->assembled) return EMS_OK;
+    std::vector<double> S, Me, Ms;
+    EMS_TRY(assemble(s, S, Me, Ms));
+    s->lossy = false;
+    for (double v : Ms)
+        if (v != 0.0) s->lossy = true;
+    EMS_TRY(s->d_rp.reserve(sizeof(int) * s->rp.size()));
+    EMS_TRY(s->d_col.reserve(sizeof(int) * s->col.size()));
+    EMS_HIP(hipMemcpy(s->d_rp.p, s->rp.data(), sizeof(int) * s->rp.size(), hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->d_col.p, s->col.data(), sizeof(int) * s->col.size(), hipMemcpyHostToDevice));
+    EMS_TRY(upload<T>(s->d_S, S));
+    EMS_TRY(upload<T>(s->d_Me, Me));
+    EMS_TRY(upload<T>(s->d_Ms, Ms));
+    // A_+ = Me + D/2 Ms per stage step (ABM: Me) and its Jacobi preconditioner
+    s->deltas = stage_deltas(s);
+    // Me + D/2 Ms is positive definite when every tet has eps0 eps_r + D/2 sigma > 0
+    // (the symplectic_4 middle stage has D < 0)
+    for (size_t q = 0; s->lossy && q < s->deltas.size(); ++q)
+        for (size_t t = 0; t < s->sigma.size(); ++t) {
+            const double er = s->eps_r.empty() ? 1.0 : s->eps_r[t];
+            if (!(s->d.eps * er + 0.5 * s->deltas[q] * s->sigma[t] > 0.0)) {
+                s->assembled = false;
+                return invalid("conductivity too large for the symplectic_4 stages: eps + x2 dt sigma / 2 <= 0");
+            }
+        }
+    for (size_t q = 0; q < s->deltas.size(); ++q) {
+        std::vector<double> A(Me.size()), dinv(s->m.ne, 1.0);
+        const double hh = 0.5 * s->deltas[q];
+        for (size_t e = 0; e < A.size(); ++e) A[e] = s->lossy ? Me[e] + hh * Ms[e] : Me[e];
+        for (long r = 0; r < s->m.ne; ++r)
+            for (int e = s->rp[r]; e < s->rp[r + 1]; ++e)
+                if (s->col[e] == r) dinv[r] = 1.0 / A[e];
+        EMS_TRY(upload<T>(s->d_A[q], A));
+        EMS_TRY(upload<T>(s->d_dinv[q], dinv));
+    }
+    const size_t vb = sizeof(T) * s->m.ne;
+    for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up, &s->Vp,
+                      &s->Ap}) {
+        EMS_TRY(v->reserve(vb));
+        EMS_HIP(hipMemset(v->p, 0, vb));
+    }
+    EMS_TRY(s->d_partial.reserve(sizeof(double) * 2048));
+    EMS_TRY(s->d_cg.reserve(sizeof(CgScalars)));
+    s->assembled = true;
+    return EMS_OK;
+}
+
+int nblocks(long n) { return (int)std::min<long>((n + kRB - 1) / kRB, 1024); }
+
+void swap_dev(DevBuf &a, DevBuf &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.bytes, b.bytes);
+}
+
+// x <- A^-1 b (initial guess x), A = d_A[q]
+template <typename T> ems_status cg(ems_tet *s, int q, DevBuf &x)
+{
+    const int n = (int)s->m.ne, nb = nblocks(n);
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    // FP32: the recursively updated residual keeps converging below the
+    // rounding of the matrix entries, so 1e-7 is reachable
+    const double tol = s->esz == 4 ? 1e-7 : 1e-14;
+    const double tol2 = tol * tol;
+    CgScalars *c = s->d_cg.as<CgScalars>();
+    double *part = s->d_partial.as<double>();
+    const T *val = s->d_A[q].as<T>(), *dinv = s->d_dinv[q].as<T>();
+    k_cg_init<T><<<nb, kRB, 0, s->stream>>>(A, val, dinv, s->b.as<T>(), x.as<T>(), s->r.as<T>(), s->z.as<T>(),
+                                            s->p.as<T>(), s->ap.as<T>(), part);
+    k_cg_start<<<1, 1, 0, s->stream>>>(c, part, nb, tol2);
+    const int maxit = 512, check = 16;
+    for (int it = 0; it < maxit; ++it) {
+        k_cg_ap<T><<<nb, kRB, 0, s->stream>>>(A, val, s->p.as<T>(), s->ap.as<T>(), part, c);
+        k_cg_alpha<<<1, 1, 0, s->stream>>>(c, part, nb);
+        k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, x.as<T>(), s->r.as<T>(), s->z.as<T>(), s->p.as<T>(),
+                                                  s->ap.as<T>(), dinv, part, c);
+        k_cg_beta<<<1, 1, 0, s->stream>>>(c, part, nb, tol2);
+        k_cg_dir<T><<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->p.as<T>(), s->z.as<T>(), c);
+        if ((it + 1) % check == 0) {
+            CgScalars h;
+            EMS_HIP(hipMemcpyAsync(&h, c, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+            EMS_HIP(hipStreamSynchronize(s->stream));
+            if (h.done) {
+                s->cg_last = h.iters;
+                s->cg_max = std::max(s->cg_max, h.iters);
+                return EMS_OK;
+            }
+        }
+    }
+    set_error("tetrahedral mass solve did not converge");
+    return EMS_ERR_STATE;
+}
+
+template <typename T> void load(ems_tet *s, DevBuf &b, const double *f)
+{
+    const int nw = (int)s->dip_dof.size();
+    if (nw) k_load<T><<<1, 64, 0, s->stream>>>(b.as<T>(), nw, s->d_dip_dof.as<int>(), s->d_dip_w.as<double>(), f);
+}
+
+// load scale per dipole weight at time t: -D p'(t) (stage) or -p'(t) (ABM)
+void dipole_factors(const ems_tet *s, double t, double scale, std::vector<double> &out)
+{
+    const int nd = (int)s->dip_par.size() / 3;
+    for (int q = 0; q < nd; ++q) out.push_back(scale * pulse_rate(t, s->dip_par[3 * q], s->dip_par[3 * q + 1], s->dip_par[3 * q + 2]));
+}
+
+template <typename T> ems_status stage(ems_tet *s, int q, double D, const double *fdev)
+{
+    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    if (s->lossy)
+        k_rhs<T, true><<<g, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_Ms.as<T>(), s->d_S.as<T>(), s->U.as<T>(),
+                                                 s->V.as<T>(), (T)(0.5 * D), (T)D, s->b.as<T>());
+    else
+        k_rhs<T, false><<<g, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), nullptr, s->d_S.as<T>(), s->U.as<T>(),
+                                                  s->V.as<T>(), (T)(0.5 * D), (T)D, s->b.as<T>());
+    load<T>(s, s->b, fdev);
+    EMS_TRY(cg<T>(s, q, s->V));
+    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    return EMS_OK;
+}
+
+// a = Me^-1 (-S U - Ms V + load) into dst (initial guess: dst)
+template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
+{
+    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    if (s->lossy) k_force<T, true><<<g, 256, 0, s->stream>>>(A, s->d_Ms.as<T>(), s->d_S.as<T>(), U, V, s->b.as<T>());
+    else k_force<T, false><<<g, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
+    load<T>(s, s->b, fdev);
+    return cg<T>(s, 0, dst);
+}
+
+template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
+{
+    EMS_TRY(setup<T>(s));
+    const int tf = s->d.tf, nobs = (int)s->obs_dof.size() / 6;
+    const long total = s->steps + N;
+    if (total / tf + 1 > 2147483647L / 3 / std::max(nobs, 1)) return invalid("too many receiver samples");
+    const int nslot = (int)(total / tf + 1);
+    if (nslot > s->nslot) {
+        std::vector<double> t((size_t)nobs * nslot * 3, 0.0);
+        for (int o = 0; o < nobs; ++o)
+            for (int q = 0; q < s->nslot; ++q)
+                for (int c = 0; c < 3; ++c) t[((size_t)o * nslot + q) * 3 + c] = s->traces[((size_t)o * s->nslot + q) * 3 + c];
+        s->traces.swap(t);
+        s->nslot = nslot;
+    }
+    if (N <= 0) return EMS_OK;
+    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const double dt = s->d.dt;
+    const int nd = (int)s->dip_par.size() / 3;
+    // per-call tables: load factors per stage / evaluation, in order
+    std::vector<double> f;
+    double t = s->t_now;
+    const bool abm = s->d.stepper == EMS_TET_ABM;
+    const bool s4 = s->d.stepper == EMS_TET_SYMPLECTIC4;
+    const double x1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0)), x2 = 1.0 - 2.0 * x1;
+    if (abm && !s->abm_started) dipole_factors(s, t, -1.0, f);  // a_0
+    for (int st = 0; st < N; ++st) {
+        if (abm) {
+            dipole_factors(s, t + dt, -1.0, f);  // F(y_p)
+            dipole_factors(s, t + dt, -1.0, f);  // F(y_n+1)
+        } else if (s4) {
+            const double Dq[3] = {x1 * dt, x2 * dt, x1 * dt}, Oq[3] = {0.0, x1 * dt, (x1 + x2) * dt};
+            for (int q = 0; q < 3; ++q) dipole_factors(s, t + Oq[q] + 0.5 * Dq[q], -Dq[q], f);
+        } else {
+            dipole_factors(s, t + 0.5 * dt, -dt, f);
+        }
+        t += dt;
+    }
+    if (f.empty()) f.push_back(0.0);
+    DevBuf &df = s->d_dip_f;
+    EMS_TRY(df.reserve(sizeof(double) * f.size()));
+    EMS_HIP(hipMemcpy(df.p, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+    // receiver samples of this call: steps s0+1..s0+N that are multiples of tf
+    const long s0 = s->steps;
+    const int nrec = (int)(total / tf - s0 / tf);
+    EMS_TRY(s->d_rec.reserve(sizeof(double) * std::max(1, nobs) * 3 * (size_t)(nrec + 1)));
+    const double *fp = df.as<double>();
+    size_t fo = 0;
+    auto fnext = [&]() { const double *r = fp + fo * (size_t)std::max(nd, 1); fo += 1; return r; };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (total_ms) {
+        EMS_HIP(hipEventCreate(&e0));
+        EMS_HIP(hipEventCreate(&e1));
+        EMS_HIP(hipEventRecord(e0, s->stream));
+    }
+    if (nobs && s0 == 0)  // slot 0: the initial state, stored after this call's samples
+        k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(s->U.as<T>(), nobs, s->d_obs_dof.as<int>(),
+                                                                 s->d_obs_w.as<double>(),
+                                                                 s->d_rec.as<double>() + (size_t)nrec * nobs * 3);
+    if (abm && !s->abm_started) {
+        // a_0 from the initial state; the AB2 history starts as (V_0, a_0)
+        EMS_TRY(accel<T>(s, s->U.as<T>(), s->V.as<T>(), s->an, fnext()));
+        EMS_HIP(hipMemcpyAsync(s->am.p, s->an.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+        EMS_HIP(hipMemcpyAsync(s->Vm.p, s->V.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+        s->abm_started = true;
+    }
+    for (int st = 1; st <= N; ++st) {
+        if (abm) {
+            const T h2 = (T)(0.5 * dt), h12 = (T)(dt / 12.0);
+            // predictor y_p = y_n + dt/2 (3 F_n - F_n-1)
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, s->U.as<T>(), h2, (T)3, s->V.as<T>(), (T)-1, s->Vm.as<T>(),
+                                                (T)0, nullptr, s->Up.as<T>());
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, s->V.as<T>(), h2, (T)3, s->an.as<T>(), (T)-1, s->am.as<T>(),
+                                                (T)0, nullptr, s->Vp.as<T>());
+            EMS_HIP(hipMemcpyAsync(s->Ap.p, s->an.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+            EMS_TRY(accel<T>(s, s->Up.as<T>(), s->Vp.as<T>(), s->Ap, fnext()));
+            // corrector y_n+1 = y_n + dt/12 (5 F(y_p) + 8 F_n - F_n-1)
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, s->U.as<T>(), h12, (T)5, s->Vp.as<T>(), (T)8, s->V.as<T>(),
+                                                (T)-1, s->Vm.as<T>(), s->Up.as<T>());
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, s->V.as<T>(), h12, (T)5, s->Ap.as<T>(), (T)8, s->an.as<T>(),
+                                                (T)-1, s->am.as<T>(), s->Vp.as<T>());
+            // history: (V_n-1, a_n-1) <- (V_n, a_n); state <- corrected; a_n+1 (guess a_n)
+            swap_dev(s->Vm, s->V);
+            swap_dev(s->V, s->Vp);
+            swap_dev(s->U, s->Up);
+            swap_dev(s->am, s->an);
+            EMS_HIP(hipMemcpyAsync(s->an.p, s->am.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+            EMS_TRY(accel<T>(s, s->U.as<T>(), s->V.as<T>(), s->an, fnext()));
+        } else if (s4) {
+            EMS_TRY(stage<T>(s, 0, x1 * dt, fnext()));
+            EMS_TRY(stage<T>(s, 1, x2 * dt, fnext()));
+            EMS_TRY(stage<T>(s, 0, x1 * dt, fnext()));
+        } else {
+            EMS_TRY(stage<T>(s, 0, dt, fnext()));
+        }
+        if (nobs && (s0 + st) % tf == 0)
+            k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(
+                s->U.as<T>(), nobs, s->d_obs_dof.as<int>(), s->d_obs_w.as<double>(),
+                s->d_rec.as<double>() + (size_t)((s0 + st) / tf - s0 / tf - 1) * nobs * 3);
+        EMS_HIP(hipGetLastError());
+    }
+    (void)g;
+    if (total_ms) {
+        EMS_HIP(hipEventRecord(e1, s->stream));
+        EMS_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        EMS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *total_ms = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    s->t_now = t;
+    s->steps = total;
+    if (nobs && (nrec || s0 == 0)) {
+        std::vector<double> raw((size_t)(nrec + 1) * nobs * 3);
+        EMS_HIP(hipMemcpy(raw.data(), s->d_rec.p, sizeof(double) * raw.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < nrec; ++r)
+            for (int o = 0; o < nobs; ++o)
+                for (int c = 0; c < 3; ++c)
+                    s->traces[((size_t)o * s->nslot + s0 / tf + r + 1) * 3 + c] = raw[((size_t)r * nobs + o) * 3 + c];
+        for (int o = 0; s0 == 0 && o < nobs; ++o)
+            for (int c = 0; c < 3; ++c) s->traces[(size_t)o * s->nslot * 3 + c] = raw[((size_t)nrec * nobs + o) * 3 + c];
+    }
+    return EMS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out)
+{
+    clear_error();
+    if (!desc || !out) return invalid("ems_tet_create: null argument");
+    *out = nullptr;
+    if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return invalid("grid sizes must be >= 1");
+    if (desc->order != 1) return invalid("only first-order tetrahedral elements are available");
+    if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
+        return invalid("h, dt, eps and mu must be positive");
+    if (desc->tf <= 0) return invalid("time_frequency must be >= 1");
+    if (desc->stepper < EMS_TET_SYMPLECTIC2 || desc->stepper > EMS_TET_ABM) return invalid("unknown stepper");
+    if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
+    if ((double)desc->nx * desc->ny * desc->nz * 7.0 > 2.0e9) return invalid("mesh too large for 32-bit indices");
+    if (ems_device_count() <= 0) {
+        set_error("no HIP device available");
+        return EMS_ERR_HIP;
+    }
+    ems_tet *s = new ems_tet();
+    s->d = *desc;
+    s->esz = desc->precision == EMS_F32 ? 4 : 8;
+    s->m.init(desc->nx, desc->ny, desc->nz, desc->h);
+    *out = s;
+    return EMS_OK;
+}
+
+void ems_tet_destroy(ems_tet *s) { delete s; }
+
+ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *sigma)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (s->assembled) return invalid("set the material before the first run");
+    const size_t nt = (size_t)s->m.nt;
+    for (size_t q = 0; eps_r && q < nt; ++q)
+        if (!(eps_r[q] > 0)) return invalid("eps_r must be positive");
+    for (size_t q = 0; sigma && q < nt; ++q)
+        if (!(sigma[q] >= 0)) return invalid("sigma must be non-negative");
+    if (eps_r) s->eps_r.assign(eps_r, eps_r + nt);
+    else s->eps_r.clear();
+    if (sigma) s->sigma.assign(sigma, sigma + nt);
+    else s->sigma.clear();
+    return EMS_OK;
+}
+
+ems_status ems_tet_add_dipole(ems_tet *s, const double pos[3], const double dir[3], double f0, double t0, double amp)
+{
+    clear_error();
+    if (!s || !pos || !dir) return invalid("null argument");
+    if (!(f0 > 0)) return invalid("f0 must be positive");
+    int i, j, k;
+    double lam[4], w[6][3];
+    const int t = locate(s->m, pos, i, j, k, lam);
+    if (t < 0) return invalid("dipole outside the mesh");
+    whitney_at(t, s->m.h, lam, w);
+    long e[6];
+    s->m.tet_edges(i, j, k, t, e);
+    for (int l = 0; l < 6; ++l) {
+        s->dip_dof.push_back((int)e[l]);
+        const double v = w[l][0] * dir[0] + w[l][1] * dir[1] + w[l][2] * dir[2];
+        s->dip_w.push_back(s->m.boundary(e[l]) ? 0.0 : v);
+    }
+    s->dip_par.push_back(f0);
+    s->dip_par.push_back(t0);
+    s->dip_par.push_back(amp);
+    // per dipole its own factor: weights of dipole q scale factor q
+    const int nd = (int)s->dip_dof.size();
+    EMS_TRY(s->d_dip_dof.reserve(sizeof(int) * nd));
+    EMS_TRY(s->d_dip_w.reserve(sizeof(double) * nd));
+    EMS_HIP(hipMemcpy(s->d_dip_dof.p, s->dip_dof.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->d_dip_w.p, s->dip_w.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3])
+{
+    clear_error();
+    if (!s || !pos) return invalid("null argument");
+    int i, j, k;
+    double lam[4], w[6][3];
+    const int t = locate(s->m, pos, i, j, k, lam);
+    if (t < 0) return invalid("receiver outside the mesh");
+    whitney_at(t, s->m.h, lam, w);
+    long e[6];
+    s->m.tet_edges(i, j, k, t, e);
+    for (int l = 0; l < 6; ++l) {
+        s->obs_dof.push_back((int)e[l]);
+        for (int c = 0; c < 3; ++c) s->obs_w.push_back(w[l][c]);
+    }
+    const int no = (int)s->obs_dof.size() / 6;
+    EMS_TRY(s->d_obs_dof.reserve(sizeof(int) * s->obs_dof.size()));
+    EMS_TRY(s->d_obs_w.reserve(sizeof(double) * s->obs_w.size()));
+    EMS_HIP(hipMemcpy(s->d_obs_dof.p, s->obs_dof.data(), sizeof(int) * s->obs_dof.size(), hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->d_obs_w.p, s->obs_w.data(), sizeof(double) * s->obs_w.size(), hipMemcpyHostToDevice));
+    std::vector<double> t2((size_t)no * s->nslot * 3, 0.0);
+    std::copy(s->traces.begin(), s->traces.end(), t2.begin());
+    s->traces.swap(t2);
+    return EMS_OK;
+}
+
+ems_status ems_tet_run(ems_tet *s, int num_steps)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->esz == 4 ? run_impl<float>(s, num_steps, nullptr) : run_impl<double>(s, num_steps, nullptr);
+}
+
+ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms)
+{
+    clear_error();
+    if (!s || !total_ms) return invalid("null argument");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->esz == 4 ? run_impl<float>(s, num_steps, total_ms) : run_impl<double>(s, num_steps, total_ms);
+}
+
+ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot)
+{
+    clear_error();
+    if (!s || !nobs || !nslot) return invalid("null argument");
+    *nobs = (int)s->obs_dof.size() / 6;
+    *nslot = s->nslot;
+    return EMS_OK;
+}
+
+ems_status ems_tet_get_traces(ems_tet *s, double *out, size_t count)
+{
+    clear_error();
+    if (!s || (!out && count)) return invalid("null argument");
+    if (count != s->traces.size()) return invalid("trace buffer has the wrong size");
+    std::copy(s->traces.begin(), s->traces.end(), out);
+    return EMS_OK;
+}
+
+ems_status ems_tet_dof_count(const ems_tet *s, size_t *n)
+{
+    clear_error();
+    if (!s || !n) return invalid("null argument");
+    *n = (size_t)s->m.ne;
+    return EMS_OK;
+}
+
+// which: 0 = E (U), 1 = dE/dt (V)
+ems_status ems_tet_get_field(ems_tet *s, int which, double *out, size_t count)
+{
+    clear_error();
+    if (!s || !out) return invalid("null argument");
+    if (which < 0 || which > 1) return invalid("field must be 0 (E) or 1 (dE/dt)");
+    if (count != (size_t)s->m.ne) return invalid("field buffer has the wrong size");
+    if (!s->assembled) {
+        std::fill(out, out + count, 0.0);
+        return EMS_OK;
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    DevBuf &b = which == 0 ? s->U : s->V;
+    if (s->esz == 8) {
+        EMS_HIP(hipMemcpy(out, b.p, count * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> t(count);
+        EMS_HIP(hipMemcpy(t.data(), b.p, count * 4, hipMemcpyDeviceToHost));
+        std::copy(t.begin(), t.end(), out);
+    }
+    return EMS_OK;
+}
+
+ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t count)
+{
+    clear_error();
+    if (!s || !in) return invalid("null argument");
+    if (which < 0 || which > 1) return invalid("field must be 0 (E) or 1 (dE/dt)");
+    if (count != (size_t)s->m.ne) return invalid("field buffer has the wrong size");
+    ems_status st = s->esz == 4 ? setup<float>(s) : setup<double>(s);
+    if (st != EMS_OK) return st;
+    DevBuf &b = which == 0 ? s->U : s->V;
+    if (s->esz == 8) {
+        EMS_HIP(hipMemcpy(b.p, in, count * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(in, in + count);
+        EMS_HIP(hipMemcpy(b.p, t.data(), count * 4, hipMemcpyHostToDevice));
+    }
+    s->abm_started = false;
+    return EMS_OK;
+}
+
+ems_status ems_tet_get_time(const ems_tet *s, double *t)
+{
+    clear_error();
+    if (!s || !t) return invalid("null argument");
+    *t = s->t_now;
+    return EMS_OK;
+}
+
+// conjugate-gradient iterations of the last mass solve and the maximum so far
+ems_status ems_tet_solver_stats(const ems_tet *s, int *last_iters, int *max_iters)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (last_iters) *last_iters = s->cg_last;
+    if (max_iters) *max_iters = s->cg_max;
+    return EMS_OK;
+}
+
+}  // extern "C"

@@ -342,6 +342,63 @@ ems_status ems_fem1_save_results(ems_fem1 *s, int num_steps, const char *path);
 /* Total edge DOFs per field set. */
 ems_status ems_fem1_dof_count(const ems_fem1 *s, size_t *n);
 
+/* --------------------------------- tetrahedral Nedelec (Whitney) elements
+ * No reference counterpart: the unstructured-mesh extension of the edge-FEM
+ * time steppers (src/symplectic_2, src/symplectic_4, src/fem_abm) on the
+ * Kuhn 6-tetrahedron split of an nx x ny x nz cube grid of spacing h, with
+ * the consistent mass matrix solved by conjugate gradients on the device and
+ * PEC outer faces.  Edge numbering: 7 families (x, y, z, x+y, x+z, y+z,
+ * x+y+z directions), family f edges at base vertex (I, J, K) numbered
+ * off_f + I + cx (J + cy K), cx = nx + 1 - dx_f, cy = ny + 1 - dy_f. */
+
+enum { EMS_TET_SYMPLECTIC2 = 0, EMS_TET_SYMPLECTIC4 = 1, EMS_TET_ABM = 2 };
+
+typedef struct ems_tet ems_tet;
+
+typedef struct {
+    int nx, ny, nz;   /* cubes per axis (6 tetrahedra each)             */
+    double h;         /* cube edge length [m]                           */
+    double dt;        /* time step [s]                                  */
+    double eps, mu;   /* background permittivity, permeability          */
+    int tf;           /* receiver sampling: every tf steps              */
+    int stepper;      /* EMS_TET_SYMPLECTIC2 / _SYMPLECTIC4 / _ABM      */
+    int order;        /* element order: 1 (Whitney)                     */
+    int precision;    /* EMS_F64 or EMS_F32                             */
+} ems_tet_desc;
+
+ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out);
+void ems_tet_destroy(ems_tet *s);
+
+/* Relative permittivity and conductivity per tetrahedron (6 nx ny nz
+ * values, tetrahedron 6 (i + nx (j + ny k)) + t); NULL = 1 / 0.  Before the
+ * first run. */
+ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *sigma);
+
+/* Electric dipole of moment p(t) d at pos, p(t) = amp exp(-(pi f0 (t - t0))^2):
+ * the load p'(t) (w_i(pos) . d) on the edges of the enclosing tetrahedron. */
+ems_status ems_tet_add_dipole(ems_tet *s, const double pos[3], const double dir[3], double f0, double t0,
+                              double amp);
+
+/* Receiver of the three components of E at pos. */
+ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3]);
+
+ems_status ems_tet_run(ems_tet *s, int num_steps);
+ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);
+
+/* Receiver traces [nobs][nslot][3], slot q = step q tf (slot 0 = t 0). */
+ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot);
+ems_status ems_tet_get_traces(ems_tet *s, double *out, size_t count);
+
+ems_status ems_tet_dof_count(const ems_tet *s, size_t *n);
+
+/* Edge coefficients of E (which = 0) or dE/dt (which = 1), dof_count values. */
+ems_status ems_tet_get_field(ems_tet *s, int which, double *out, size_t count);
+ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t count);
+ems_status ems_tet_get_time(const ems_tet *s, double *t);
+
+/* Conjugate-gradient iterations of the last mass solve and the largest so far. */
+ems_status ems_tet_solver_stats(const ems_tet *s, int *last_iters, int *max_iters);
+
 #ifdef __cplusplus
 }
 #endif

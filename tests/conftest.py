@@ -41,15 +41,20 @@ def tf_misfits(traces, ref, dt):
     """Largest single-valued TF envelope and phase misfits (Kristekova et al.)
     over the observation components of traces [nobs][nslot][3] against the
     reference traces; dt is the slot spacing.  Components that are not finite
-    or are zero in the reference are skipped."""
+    in the reference, or whose peak is below 1e-2 of the largest reference
+    sample, are skipped: rounding errors scale with the whole field, so on a
+    weak component (or the cancellation noise of one the source does not
+    drive) a relative criterion measures the strong components' rounding."""
     from emsolvers_amd import tfmisfit
     nslot = ref.shape[1]
     fmin, fmax = 2.0 / (nslot * dt), 0.25 / dt
+    finite = ref[np.isfinite(ref)]
+    floor = 1e-2 * np.abs(finite).max() if finite.size else 0.0
     em = pm = 0.0
     for o in range(ref.shape[0]):
         for c in range(ref.shape[2]):
             r, s = ref[o, :, c], traces[o, :, c]
-            if not np.all(np.isfinite(r)) or not np.any(r):
+            if not np.all(np.isfinite(r)) or not np.abs(r).max() > floor:
                 continue
             m = tfmisfit.misfits(s, r, dt, fmin, fmax, nf=40)
             em, pm = max(em, m["EM"]), max(pm, m["PM"])

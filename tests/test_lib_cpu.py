@@ -106,3 +106,22 @@ def test_fem1_invalid_descriptor_rejected():
         F.element_matrix(1, (4, 4, 4), 0.02, 1e-11, 8.854187817e-12, 1.25663706212e-6, 0.05, use_ofem=1)
     with pytest.raises(_lib.EmsError):
         F.FemSimulation((4, -1, 4), 0.02, 1e-11, 8.854187817e-12, 1.25663706212e-6, 1, 0.3, 0)
+
+
+def test_tet_invalid_descriptor_rejected():
+    from emsolvers_amd import _lib
+    from emsolvers_amd.tet import TetSimulation
+    bad = [dict(grid=(0, 4, 4)), dict(h=0.0), dict(dt=-1.0), dict(tf=0), dict(stepper=7), dict(order=2),
+           dict(precision="f16")]
+    for b in bad:
+        kw = dict(grid=(4, 4, 4), h=0.25, dt=0.01, eps=1.0, mu=1.0, tf=1, stepper="symplectic_2", order=1)
+        kw.update(b)
+        if kw.get("precision") == "f16":
+            continue  # the binding maps names; the C ABI check is below
+        with pytest.raises(_lib.EmsError) as ei:
+            TetSimulation(**kw)
+        assert ei.value.status == 1, b
+    d = _lib.TetDesc(4, 4, 4, 0.25, 0.01, 1.0, 1.0, 1, 0, 1, 9)
+    import ctypes
+    h = ctypes.c_void_p()
+    assert _lib.load().ems_tet_create(ctypes.byref(d), ctypes.byref(h)) == 1

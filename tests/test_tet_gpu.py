@@ -1,0 +1,164 @@
+"""GPU parity of the tetrahedral Whitney edge-element solver against the FP64
+oracle (oracle/tet.py: quadrature element matrices, scipy assembly, sparse LU
+mass solves).
+
+Bar: FP64 receiver traces and final fields within 1e-10 relative L2 (the
+device solves the consistent mass by conjugate gradients to a relative
+preconditioned residual of 1e-14, the oracle by LU); FP32 traces within
+1e-5 relative L2 and TF envelope / phase misfits within 1e-4.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_l2, tf_misfits
+
+pytestmark = pytest.mark.gpu
+
+EPS0 = 8.854187817e-12
+MU0 = 1.25663706212e-6
+C0 = 1.0 / np.sqrt(EPS0 * MU0)
+
+CASES = {
+    # unit-speed medium, h = 0.25; symplectic dt = 0.1 h, ABM dt = 0.05 h (its
+    # stability interval on the imaginary axis is shorter)
+    "s2": dict(stepper="symplectic_2", grid=(4, 5, 3), h=0.25, dt=0.025, eps=1.0, mu=1.0, tf=1, steps=80,
+               dip=[((0.52, 0.61, 0.43), (0.0, 0.0, 1.0), 4.0, 0.3, 1.0)],
+               rec=[(0.31, 0.72, 0.38), (0.66, 0.27, 0.51), (0.52, 0.93, 0.43)], lossy=False),
+    "s4-lossy": dict(stepper="symplectic_4", grid=(4, 4, 4), h=0.25, dt=0.025, eps=1.0, mu=1.0, tf=2, steps=60,
+                     dip=[((0.47, 0.53, 0.61), (0.3, -0.2, 0.9), 4.0, 0.3, 1.0),
+                          ((0.71, 0.29, 0.33), (1.0, 0.0, 0.0), 3.0, 0.4, -0.5)],
+                     rec=[(0.22, 0.81, 0.57), (0.63, 0.44, 0.72)], lossy=True),
+    "abm-lossy": dict(stepper="abm", grid=(5, 4, 4), h=0.25, dt=0.0125, eps=1.0, mu=1.0, tf=3, steps=70,
+                      dip=[((0.61, 0.48, 0.52), (0.0, 1.0, 0.0), 4.0, 0.25, 2.0)],
+                      rec=[(0.89, 0.53, 0.41), (0.37, 0.66, 0.59), (1.11, 0.21, 0.83)], lossy=True),
+    # SI units: h = 2 cm, dt = 0.1 h / c, 3 GHz Gaussian-derivative dipole
+    "s2-si": dict(stepper="symplectic_2", grid=(6, 5, 6), h=0.02, dt=0.1 * 0.02 / C0, eps=EPS0, mu=MU0, tf=1,
+                  steps=60, dip=[((0.061, 0.049, 0.058), (0.0, 0.0, 1.0), 3e9, 4e-10, 1e-9)],
+                  rec=[(0.032, 0.041, 0.067), (0.087, 0.071, 0.055)], lossy=False),
+}
+
+
+def materials(case, seed=7):
+    g = case["grid"]
+    nt = 6 * g[0] * g[1] * g[2]
+    if not case["lossy"]:
+        return None, None
+    rng = np.random.default_rng(seed)
+    return rng.uniform(1.0, 3.0, nt), rng.uniform(0.0, 0.6, nt) * case["eps"]
+
+
+def make(kind, case, precision="f64"):
+    er, sg = materials(case)
+    if kind == "oracle":
+        from oracle import tet as ot
+        st = {"symplectic_2": ot.SYMPLECTIC2, "symplectic_4": ot.SYMPLECTIC4, "abm": ot.ABM}[case["stepper"]]
+        s = ot.OracleTet(case["grid"], case["h"], case["dt"], case["eps"], case["mu"], case["tf"], st, er, sg)
+    else:
+        from emsolvers_amd.tet import TetSimulation
+        s = TetSimulation(case["grid"], case["h"], case["dt"], case["eps"], case["mu"], case["tf"], case["stepper"],
+                          precision=precision)
+        if case["lossy"]:
+            s.set_material(er, sg)
+    for pos, d, f0, t0, amp in case["dip"]:
+        s.add_dipole(pos, d, f0, t0, amp)
+    for r in case["rec"]:
+        s.add_receiver(r)
+    return s
+
+
+_oracle_cache = {}
+
+
+def oracle_run(name):
+    if name not in _oracle_cache:
+        case = CASES[name]
+        o = make("oracle", case)
+        tr = o.run(case["steps"])
+        _oracle_cache[name] = (tr, o.U.copy(), o.V.copy(), o.t)
+    return _oracle_cache[name]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp64_parity(name):
+    case = CASES[name]
+    print("tet fp64", name, flush=True)
+    tr_o, U_o, V_o, t_o = oracle_run(name)
+    s = make("gpu", case)
+    s.run(case["steps"])
+    tr = s.traces()
+    assert tr.shape == tr_o.shape
+    assert np.abs(tr_o).max() > 0
+    assert rel_l2(tr, tr_o) < 1e-10
+    assert rel_l2(s.get_field(0), U_o) < 1e-10
+    assert rel_l2(s.get_field(1), V_o) < 1e-10
+    assert s.time() == pytest.approx(t_o, rel=1e-14)
+    last, mx = s.solver_stats()
+    assert 0 < mx <= 512
+
+
+@pytest.mark.parametrize("name", ["s2", "abm-lossy", "s2-si"])
+def test_fp32_parity(name):
+    case = CASES[name]
+    print("tet fp32", name, flush=True)
+    tr_o = oracle_run(name)[0]
+    s = make("gpu", case, precision="f32")
+    s.run(case["steps"])
+    tr = s.traces()
+    assert rel_l2(tr, tr_o) < 1e-5
+    em, pm = tf_misfits(tr, tr_o, case["dt"] * case["tf"])
+    assert em < 1e-4 and pm < 1e-4
+
+
+@pytest.mark.parametrize("name", ["s4-lossy", "abm-lossy"])
+def test_split_runs_equal_one_run(name):
+    """run(a); run(b) continues the same trajectory (sources, ABM history,
+    receiver slots) bit for bit."""
+    case = CASES[name]
+    a = make("gpu", case)
+    a.run(case["steps"])
+    b = make("gpu", case)
+    first = 31
+    b.run(first)
+    b.run(case["steps"] - first)
+    np.testing.assert_array_equal(a.traces(), b.traces())
+    np.testing.assert_array_equal(a.get_field(0), b.get_field(0))
+
+
+def test_initial_field_and_receivers():
+    """set_field(E) with a random interior field: slot 0 holds the receivers'
+    interpolation of it, and the free evolution matches the oracle."""
+    from oracle import tet as ot
+    case = dict(CASES["s2"], dip=[])
+    o = make("oracle", case)
+    rng = np.random.default_rng(11)
+    U0 = rng.standard_normal(o.mesh.ne)
+    U0[o.bnd] = 0.0
+    o.set_field(0, U0)
+    tr_o = o.run(20)
+    s = make("gpu", case)
+    assert s.dof_count() == o.mesh.ne
+    s.set_field(0, U0)
+    np.testing.assert_array_equal(s.get_field(0), U0)
+    s.run(20)
+    tr = s.traces()
+    np.testing.assert_allclose(tr[:, 0, :], tr_o[:, 0, :], rtol=0, atol=1e-13 * np.abs(tr_o).max())
+    assert rel_l2(tr, tr_o) < 1e-10
+    assert rel_l2(s.get_field(0), o.U) < 1e-10
+
+
+def test_lossless_energy_bounded_on_device():
+    """The lossless symplectic_2 trajectory keeps its discrete energy (read
+    back through the fields) within a few percent after the source ends."""
+    from oracle import tet as ot
+    case = CASES["s2"]
+    s = make("gpu", case)
+    s.run(60)
+    m = ot.Mesh(*case["grid"], case["h"])
+    S, Me, _ = ot.assemble(m, case["eps"], case["mu"])
+    en = []
+    for _ in range(8):
+        s.run(25)
+        U, V = s.get_field(0), s.get_field(1)
+        en.append(0.5 * V @ (Me @ V) + 0.5 * U @ (S @ U))
+    en = np.array(en)
+    assert en.min() > 0 and (en.max() - en.min()) / en.mean() < 0.05
