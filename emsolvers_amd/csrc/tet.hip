@@ -32,185 +32,256 @@ using namespace ems::tet;
 
 namespace {
 
-constexpr int kRB = 256;  // reduction block
-
 template <typename T> struct Csr {
     int n;
     const int *rp, *col;
 };
 
-// y = A x, one row per thread, summed in CSR order
-template <typename T>
-__global__ void k_spmv(Csr<T> A, const T *__restrict__ val, const T *__restrict__ x, T *__restrict__ y)
+// Sparse rows are processed by LPR consecutive lanes (LPR = 4 or 8, from the
+// longest row): lane l sums entries rp[r] + l, + 2 LPR, ... in order, and the
+// lanes combine by a fixed butterfly, so every product is deterministic.
+template <int LPR, typename T> __device__ __forceinline__ T lanes_sum(T v)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= A.n) return;
+#pragma unroll
+    for (int m = 1; m < LPR; m <<= 1) v += __shfl_xor(v, m, LPR);
+    return v;
+}
+
+template <int LPR, typename T>
+__device__ __forceinline__ T row_dot(const Csr<T> &A, const T *__restrict__ val, const T *__restrict__ x, int r,
+                                     int lane)
+{
     T s = 0;
-    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) s += val[q] * x[A.col[q]];
-    y[r] = s;
+    for (int q = A.rp[r] + lane; q < A.rp[r + 1]; q += LPR) s += val[q] * x[A.col[q]];
+    return lanes_sum<LPR>(s);
+}
+
+// y = A x
+template <int LPR, typename T>
+__global__ __launch_bounds__(256) void k_spmv(Csr<T> A, const T *__restrict__ val, const T *__restrict__ x,
+                                              T *__restrict__ y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / LPR, lane = t % LPR;
+    if (r >= A.n) return;
+    const T v = row_dot<LPR>(A, val, x, r, lane);
+    if (lane == 0) y[r] = v;
 }
 
 // b = Me V - h Ms V - D S U (lossless: no Ms term)
-template <typename T, bool LOSSY>
-__global__ void k_rhs(Csr<T> A, const T *__restrict__ me, const T *__restrict__ ms, const T *__restrict__ st,
-                      const T *__restrict__ U, const T *__restrict__ V, T hh, T D, T *__restrict__ b)
+template <int LPR, typename T, bool LOSSY>
+__global__ __launch_bounds__(256) void k_rhs(Csr<T> A, const T *__restrict__ me, const T *__restrict__ ms,
+                                             const T *__restrict__ st, const T *__restrict__ U,
+                                             const T *__restrict__ V, T hh, T D, T *__restrict__ b)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / LPR, lane = t % LPR;
     if (r >= A.n) return;
     T a = 0, c = 0, s = 0;
-    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) {
+    for (int q = A.rp[r] + lane; q < A.rp[r + 1]; q += LPR) {
         const int j = A.col[q];
         a += me[q] * V[j];
         if (LOSSY) c += ms[q] * V[j];
         s += st[q] * U[j];
     }
-    b[r] = LOSSY ? (a - hh * c) - D * s : a - D * s;
+    a = lanes_sum<LPR>(a);
+    s = lanes_sum<LPR>(s);
+    if (LOSSY) c = lanes_sum<LPR>(c);
+    if (lane == 0) b[r] = LOSSY ? (a - hh * c) - D * s : a - D * s;
 }
 
 // b = -S U - Ms V (the ABM right-hand side)
-template <typename T, bool LOSSY>
-__global__ void k_force(Csr<T> A, const T *__restrict__ ms, const T *__restrict__ st, const T *__restrict__ U,
-                        const T *__restrict__ V, T *__restrict__ b)
+template <int LPR, typename T, bool LOSSY>
+__global__ __launch_bounds__(256) void k_force(Csr<T> A, const T *__restrict__ ms, const T *__restrict__ st,
+                                               const T *__restrict__ U, const T *__restrict__ V, T *__restrict__ b)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / LPR, lane = t % LPR;
     if (r >= A.n) return;
     T c = 0, s = 0;
-    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q) {
+    for (int q = A.rp[r] + lane; q < A.rp[r + 1]; q += LPR) {
         const int j = A.col[q];
         s += st[q] * U[j];
         if (LOSSY) c += ms[q] * V[j];
     }
-    b[r] = LOSSY ? -s - c : -s;
+    s = lanes_sum<LPR>(s);
+    if (LOSSY) c = lanes_sum<LPR>(c);
+    if (lane == 0) b[r] = LOSSY ? -s - c : -s;
 }
 
-// b[dof] += f[dipole] * w (dipole loads, 6 weights per dipole), one thread
-// in dipole order so that loads sharing an edge add up in a fixed order
+// b[dof] += f[dipole] * w (dipole loads, nl weights per dipole), one thread
+// in dipole order so that loads sharing a dof add up in a fixed order
 template <typename T>
-__global__ void k_load(T *b, int nw, const int *__restrict__ dof, const double *__restrict__ w,
+__global__ void k_load(T *b, int nw, int nl, const int *__restrict__ dof, const double *__restrict__ w,
                        const double *__restrict__ f)
 {
     if (threadIdx.x || blockIdx.x) return;
-    for (int q = 0; q < nw; ++q) b[dof[q]] = b[dof[q]] + (T)(f[q / 6] * w[q]);
+    for (int q = 0; q < nw; ++q) b[dof[q]] = b[dof[q]] + (T)(f[q / nl] * w[q]);
 }
 
-// Scalars of one CG solve (device resident)
-struct CgScalars {
-    double rz, rz0, alpha, beta, pap;
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients, two kernels per iteration (the
+// direction recurrence Ap <- A z + beta Ap lets the product and the new
+// direction share one pass).  Dot products: per-block partials (rows in a
+// fixed order, then a fixed LDS tree); every block re-reduces the partials
+// it needs in the same fixed order, so all blocks see the same scalars and
+// no scalar kernels or grid barriers are needed.  Partial buffers:
+//   P[0], P[1]  (r, z) of even / odd updates,  P[2] (p, Ap),  P[3] (b, M^-1 b)
+// The solve stops when (r, z) <= tol^2 (b, M^-1 b); b = 0 gives x = 0.
+// Block 0 records done and the iteration count for the host.
+// ---------------------------------------------------------------------------
+constexpr int kRB = 256;    // threads per block
+constexpr int kCgNB = 512;  // blocks of the CG kernels (partials per dot)
+
+struct CgState {
     int done, iters;
 };
 
-// partial dot products of block b: sum over its index range in order
-template <typename T>
-__device__ __forceinline__ void block_dot(const T *a, const T *c, int n, double *partial)
+// sum of nb partials, the same fixed order in every block; all threads get it
+__device__ __forceinline__ double block_total(const double *__restrict__ P, int nb)
 {
     __shared__ double sh[kRB];
     double s = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        s += (double)a[i] * (double)c[i];
+    for (int q = threadIdx.x; q < nb; q += kRB) s += P[q];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = kRB / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+    const double out = sh[0];
+    __syncthreads();
+    return out;
 }
 
-__device__ __forceinline__ double sum_partials(const double *p, int nb)
+// per-block partial of the per-thread sums v
+__device__ __forceinline__ void block_partial(double v, double *P)
 {
-    double s = 0.0;
-    for (int q = 0; q < nb; ++q) s += p[q];
-    return s;
+    __shared__ double sh[kRB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kRB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) P[blockIdx.x] = sh[0];
+    __syncthreads();
 }
 
-// CG start: r = b - A x, z = dinv r, p = z, partials (r, z) and (b, dinv b)
-template <typename T>
+__device__ __forceinline__ bool cg_converged(double rz, double ref, double tol2) { return !(rz > tol2 * ref); }
+
+// r = b - A x, z = M^-1 r; partials (r, z) -> P0 and (b, M^-1 b) -> P3
+template <int LPR, typename T>
 __global__ __launch_bounds__(kRB) void k_cg_init(Csr<T> A, const T *__restrict__ val, const T *__restrict__ dinv,
-                                                 const T *__restrict__ b, const T *__restrict__ x, T *r, T *z, T *p,
-                                                 T *zb, double *partial)
+                                                 const T *__restrict__ b, const T *__restrict__ x, T *r, T *z,
+                                                 double *P, CgState *st)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
-        T s = 0;
-        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) s += val[q] * x[A.col[q]];
-        const T ri = b[i] - s;
-        r[i] = ri;
-        z[i] = dinv[i] * ri;
-        p[i] = z[i];
-        zb[i] = dinv[i] * b[i];
+    double rz = 0.0, bb = 0.0;
+    const int stride = gridDim.x * kRB / LPR;
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+        const int lane = threadIdx.x % LPR;
+        const T ax = row_dot<LPR>(A, val, x, i, lane);
+        if (lane == 0) {
+            const T ri = b[i] - ax, zi = dinv[i] * ri;
+            r[i] = ri;
+            z[i] = zi;
+            rz += (double)ri * (double)zi;
+            bb += (double)b[i] * (double)(dinv[i] * b[i]);
+        }
     }
-    __syncthreads();
-    block_dot(r, z, A.n, partial);
-    __syncthreads();
-    block_dot(b, zb, A.n, partial + gridDim.x);
-}
-
-// stop when (r, M^-1 r) <= tol^2 (b, M^-1 b) (b = 0: relative to the start)
-__global__ void k_cg_start(CgScalars *c, const double *partial, int nb, double tol2)
-{
-    if (threadIdx.x || blockIdx.x) return;
-    c->rz = sum_partials(partial, nb);
-    const double rb = sum_partials(partial + nb, nb);
-    c->rz0 = rb > 0.0 ? rb : c->rz;
-    c->iters = 0;
-    c->done = !(c->rz > tol2 * c->rz0);
-}
-
-// Ap = A p and partial (p, Ap)
-template <typename T>
-__global__ __launch_bounds__(kRB) void k_cg_ap(Csr<T> A, const T *__restrict__ val, const T *__restrict__ p, T *ap,
-                                               double *partial, const CgScalars *c)
-{
-    if (c->done) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gridDim.x * blockDim.x) {
-        T s = 0;
-        for (int q = A.rp[i]; q < A.rp[i + 1]; ++q) s += val[q] * p[A.col[q]];
-        ap[i] = s;
+    block_partial(rz, P);
+    block_partial(bb, P + 3 * kCgNB);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->done = 0;
+        st->iters = 0;
     }
-    __syncthreads();
-    block_dot(p, ap, A.n, partial);
 }
 
-__global__ void k_cg_alpha(CgScalars *c, const double *partial, int nb)
+// p = z, Ap = A z; partial (p, Ap) -> P2 (or done when (r, z) is small
+// already; x = 0 when b = 0)
+template <int LPR, typename T>
+__global__ __launch_bounds__(kRB) void k_cg_first(Csr<T> A, const T *__restrict__ val, const T *__restrict__ z,
+                                                  T *p, T *ap, T *x, double *P, CgState *st, double tol2)
 {
-    if (threadIdx.x || blockIdx.x || c->done) return;
-    c->pap = sum_partials(partial, nb);
-    c->alpha = c->rz / c->pap;
+    const double rz = block_total(P, kCgNB), ref = block_total(P + 3 * kCgNB, kCgNB);
+    if (!(ref > 0.0)) {
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < A.n; i += gridDim.x * kRB) x[i] = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+        return;
+    }
+    if (cg_converged(rz, ref, tol2)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+        return;
+    }
+    double pap = 0.0;
+    const int stride = gridDim.x * kRB / LPR;
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+        const int lane = threadIdx.x % LPR;
+        const T q = row_dot<LPR>(A, val, z, i, lane);
+        if (lane == 0) {
+            const T zi = z[i];
+            p[i] = zi;
+            ap[i] = q;
+            pap += (double)zi * (double)q;
+        }
+    }
+    block_partial(pap, P + 2 * kCgNB);
 }
 
-// x += a p, r -= a Ap, z = dinv r, partial (r, z)
+// alpha = (r, z) / (p, Ap); x += alpha p, r -= alpha Ap, z = M^-1 r;
+// partial (r, z) -> P[par ^ 1]
 template <typename T>
 __global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, const T *__restrict__ p,
-                                                   const T *__restrict__ ap, const T *__restrict__ dinv,
-                                                   double *partial, const CgScalars *c)
+                                                   const T *__restrict__ ap, const T *__restrict__ dinv, double *P,
+                                                   int par, const CgState *st)
 {
-    if (c->done) return;
-    const T a = (T)c->alpha;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (st->done) return;
+    const double rz = block_total(P + par * kCgNB, kCgNB), pap = block_total(P + 2 * kCgNB, kCgNB);
+    const T a = (T)(rz / pap);
+    double s = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
         x[i] = x[i] + a * p[i];
-        r[i] = r[i] - a * ap[i];
-        z[i] = dinv[i] * r[i];
+        const T ri = r[i] - a * ap[i], zi = dinv[i] * ri;
+        r[i] = ri;
+        z[i] = zi;
+        s += (double)ri * (double)zi;
     }
-    __syncthreads();
-    block_dot(r, z, n, partial);
+    block_partial(s, P + (par ^ 1) * kCgNB);
 }
 
-__global__ void k_cg_beta(CgScalars *c, const double *partial, int nb, double tol2)
+// converged?  else beta = (r, z)_new / (r, z)_old; p = z + beta p,
+// Ap = A z + beta Ap; partial (p, Ap) -> P2
+template <int LPR, typename T>
+__global__ __launch_bounds__(kRB) void k_cg_direction(Csr<T> A, const T *__restrict__ val, const T *__restrict__ z,
+                                                      T *p, T *ap, double *P, int par, CgState *st, int it,
+                                                      double tol2)
 {
-    if (threadIdx.x || blockIdx.x || c->done) return;
-    const double rz = sum_partials(partial, nb);
-    c->beta = rz / c->rz;
-    c->rz = rz;
-    c->iters += 1;
-    if (!(rz > tol2 * c->rz0)) c->done = 1;
-}
-
-template <typename T>
-__global__ void k_cg_dir(int n, T *p, const T *__restrict__ z, const CgScalars *c)
-{
-    if (c->done) return;
-    const T b = (T)c->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + b * p[i];
+    if (st->done) return;
+    const double rz_old = block_total(P + par * kCgNB, kCgNB);
+    const double rz = block_total(P + (par ^ 1) * kCgNB, kCgNB);
+    const double ref = block_total(P + 3 * kCgNB, kCgNB);
+    if (cg_converged(rz, ref, tol2) || !(rz_old > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = 1;
+            st->iters = it + 1;
+        }
+        return;
+    }
+    const T beta = (T)(rz / rz_old);
+    double pap = 0.0;
+    const int stride = gridDim.x * kRB / LPR;
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+        const int lane = threadIdx.x % LPR;
+        const T q = row_dot<LPR>(A, val, z, i, lane);
+        if (lane == 0) {
+            const T pi = z[i] + beta * p[i], api = q + beta * ap[i];
+            p[i] = pi;
+            ap[i] = api;
+            pap += (double)pi * (double)api;
+        }
+    }
+    block_partial(pap, P + 2 * kCgNB);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it + 1;
 }
 
 // y += a x
@@ -232,16 +303,17 @@ __global__ void k_comb(int n, const T *__restrict__ c0, T k, T a1, const T *__re
     y[i] = c0[i] + k * s;
 }
 
-// receivers: E(r) = sum_l U[dof_l] w_l(r), 3 components, into rec[obs][3]
+// receivers: E(r) = sum_l U[dof_l] w_l(r) over the nl basis functions of the
+// enclosing tetrahedron, 3 components, into rec[obs][3]
 template <typename T>
-__global__ void k_tet_record(const T *__restrict__ U, int nobs, const int *__restrict__ dof,
+__global__ void k_tet_record(const T *__restrict__ U, int nobs, int nl, const int *__restrict__ dof,
                              const double *__restrict__ w, double *rec)
 {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nobs) return;
     for (int c = 0; c < 3; ++c) {
         double s = 0.0;
-        for (int l = 0; l < 6; ++l) s += (double)U[dof[6 * o + l]] * w[(6 * o + l) * 3 + c];
+        for (int l = 0; l < nl; ++l) s += (double)U[dof[nl * o + l]] * w[(nl * o + l) * 3 + c];
         rec[3 * o + c] = s;
     }
 }
@@ -258,9 +330,12 @@ struct ems_tet {
     DevBuf d_rp, d_col, d_S, d_Me, d_Ms;
     std::vector<double> deltas;  // stage steps with an assembled A_+
     DevBuf d_A[3], d_dinv[3];
+    DevBuf d_dinv_me;  // Jacobi preconditioner of M_eps alone
     DevBuf U, V, b, r, z, p, ap;        // state and CG scratch
     DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
     DevBuf d_partial, d_cg;
+    int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
+    int nl = 6;   // basis functions per tetrahedron
     bool assembled = false, lossy = false;
     // dipoles: per dipole 6 dofs and the load weights w_l(r0).d
     std::vector<int> dip_dof;
@@ -410,19 +485,27 @@ template <typename T> ems_status setup(ems_tet *s)
         EMS_TRY(upload<T>(s->d_A[q], A));
         EMS_TRY(upload<T>(s->d_dinv[q], dinv));
     }
+    {
+        std::vector<double> dinv(s->m.ne, 1.0);
+        for (long r = 0; r < s->m.ne; ++r)
+            for (int e = s->rp[r]; e < s->rp[r + 1]; ++e)
+                if (s->col[e] == r) dinv[r] = 1.0 / Me[e];
+        EMS_TRY(upload<T>(s->d_dinv_me, dinv));
+    }
     const size_t vb = sizeof(T) * s->m.ne;
     for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up, &s->Vp,
                       &s->Ap}) {
         EMS_TRY(v->reserve(vb));
         EMS_HIP(hipMemset(v->p, 0, vb));
     }
-    EMS_TRY(s->d_partial.reserve(sizeof(double) * 2048));
-    EMS_TRY(s->d_cg.reserve(sizeof(CgScalars)));
+    EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgNB));
+    EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
+    int longest = 0;
+    for (long r = 0; r < s->m.ne; ++r) longest = std::max(longest, s->rp[r + 1] - s->rp[r]);
+    s->lpr = longest > 32 ? 8 : 4;
     s->assembled = true;
     return EMS_OK;
 }
-
-int nblocks(long n) { return (int)std::min<long>((n + kRB - 1) / kRB, 1024); }
 
 void swap_dev(DevBuf &a, DevBuf &b)
 {
@@ -430,32 +513,45 @@ void swap_dev(DevBuf &a, DevBuf &b)
     std::swap(a.bytes, b.bytes);
 }
 
-// x <- A^-1 b (initial guess x), A = d_A[q]
-template <typename T> ems_status cg(ems_tet *s, int q, DevBuf &x)
+// calls f(std::integral_constant<int, LPR>) for the handle's lanes per row
+template <typename F> void with_lpr(int lpr, F &&f)
 {
-    const int n = (int)s->m.ne, nb = nblocks(n);
+    if (lpr == 8) f(std::integral_constant<int, 8>{});
+    else f(std::integral_constant<int, 4>{});
+}
+
+// blocks of 256 threads covering n rows at LPR lanes each
+inline int row_blocks(long n, int lpr) { return (int)((n * lpr + 255) / 256); }
+
+// x <- A^-1 b (initial guess x) for the SPD matrix `val` on the common pattern
+template <typename T> ems_status cg(ems_tet *s, const T *val, const T *dinv, DevBuf &x)
+{
+    const int n = (int)s->m.ne;
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     // FP32: the recursively updated residual keeps converging below the
     // rounding of the matrix entries, so 1e-7 is reachable
     const double tol = s->esz == 4 ? 1e-7 : 1e-14;
     const double tol2 = tol * tol;
-    CgScalars *c = s->d_cg.as<CgScalars>();
-    double *part = s->d_partial.as<double>();
-    const T *val = s->d_A[q].as<T>(), *dinv = s->d_dinv[q].as<T>();
-    k_cg_init<T><<<nb, kRB, 0, s->stream>>>(A, val, dinv, s->b.as<T>(), x.as<T>(), s->r.as<T>(), s->z.as<T>(),
-                                            s->p.as<T>(), s->ap.as<T>(), part);
-    k_cg_start<<<1, 1, 0, s->stream>>>(c, part, nb, tol2);
+    CgState *st = s->d_cg.as<CgState>();
+    double *P = s->d_partial.as<double>();
+    T *r = s->r.as<T>(), *z = s->z.as<T>(), *p = s->p.as<T>(), *ap = s->ap.as<T>(), *xx = x.as<T>();
     const int maxit = 512, check = 16;
+    with_lpr(s->lpr, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        k_cg_init<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, dinv, s->b.as<T>(), xx, r, z, P, st);
+        k_cg_first<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, z, p, ap, xx, P, st, tol2);
+    });
+    EMS_HIP(hipGetLastError());
     for (int it = 0; it < maxit; ++it) {
-        k_cg_ap<T><<<nb, kRB, 0, s->stream>>>(A, val, s->p.as<T>(), s->ap.as<T>(), part, c);
-        k_cg_alpha<<<1, 1, 0, s->stream>>>(c, part, nb);
-        k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, x.as<T>(), s->r.as<T>(), s->z.as<T>(), s->p.as<T>(),
-                                                  s->ap.as<T>(), dinv, part, c);
-        k_cg_beta<<<1, 1, 0, s->stream>>>(c, part, nb, tol2);
-        k_cg_dir<T><<<(n + 255) / 256, 256, 0, s->stream>>>(n, s->p.as<T>(), s->z.as<T>(), c);
+        const int par = it & 1;
+        k_cg_update<T><<<kCgNB, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st);
+        with_lpr(s->lpr, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            k_cg_direction<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, z, p, ap, P, par, st, it, tol2);
+        });
         if ((it + 1) % check == 0) {
-            CgScalars h;
-            EMS_HIP(hipMemcpyAsync(&h, c, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+            CgState h;
+            EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
             EMS_HIP(hipStreamSynchronize(s->stream));
             if (h.done) {
                 s->cg_last = h.iters;
@@ -471,7 +567,7 @@ template <typename T> ems_status cg(ems_tet *s, int q, DevBuf &x)
 template <typename T> void load(ems_tet *s, DevBuf &b, const double *f)
 {
     const int nw = (int)s->dip_dof.size();
-    if (nw) k_load<T><<<1, 64, 0, s->stream>>>(b.as<T>(), nw, s->d_dip_dof.as<int>(), s->d_dip_w.as<double>(), f);
+    if (nw) k_load<T><<<1, 64, 0, s->stream>>>(b.as<T>(), nw, s->nl, s->d_dip_dof.as<int>(), s->d_dip_w.as<double>(), f);
 }
 
 // load scale per dipole weight at time t: -D p'(t) (stage) or -p'(t) (ABM)
@@ -483,17 +579,22 @@ void dipole_factors(const ems_tet *s, double t, double scale, std::vector<double
 
 template <typename T> ems_status stage(ems_tet *s, int q, double D, const double *fdev)
 {
-    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const int n = (int)s->m.ne, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
-    if (s->lossy)
-        k_rhs<T, true><<<g, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_Ms.as<T>(), s->d_S.as<T>(), s->U.as<T>(),
-                                                 s->V.as<T>(), (T)(0.5 * D), (T)D, s->b.as<T>());
-    else
-        k_rhs<T, false><<<g, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), nullptr, s->d_S.as<T>(), s->U.as<T>(),
-                                                  s->V.as<T>(), (T)(0.5 * D), (T)D, s->b.as<T>());
+    with_lpr(s->lpr, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        if (s->lossy)
+            k_rhs<LPR, T, true><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_Ms.as<T>(), s->d_S.as<T>(),
+                                                           s->U.as<T>(), s->V.as<T>(), (T)(0.5 * D), (T)D,
+                                                           s->b.as<T>());
+        else
+            k_rhs<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), nullptr, s->d_S.as<T>(),
+                                                            s->U.as<T>(), s->V.as<T>(), (T)(0.5 * D), (T)D,
+                                                            s->b.as<T>());
+    });
     load<T>(s, s->b, fdev);
-    EMS_TRY(cg<T>(s, q, s->V));
+    EMS_TRY(cg<T>(s, s->d_A[q].as<T>(), s->d_dinv[q].as<T>(), s->V));
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
     return EMS_OK;
 }
@@ -501,18 +602,68 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, const double
 // a = Me^-1 (-S U - Ms V + load) into dst (initial guess: dst)
 template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
 {
-    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const int n = (int)s->m.ne, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
-    if (s->lossy) k_force<T, true><<<g, 256, 0, s->stream>>>(A, s->d_Ms.as<T>(), s->d_S.as<T>(), U, V, s->b.as<T>());
-    else k_force<T, false><<<g, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
+    with_lpr(s->lpr, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        if (s->lossy)
+            k_force<LPR, T, true><<<gr, 256, 0, s->stream>>>(A, s->d_Ms.as<T>(), s->d_S.as<T>(), U, V, s->b.as<T>());
+        else
+            k_force<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
+    });
     load<T>(s, s->b, fdev);
-    return cg<T>(s, 0, dst);
+    return cg<T>(s, s->d_A[0].as<T>(), s->d_dinv[0].as<T>(), dst);
+}
+
+// Largest eigenvalue of M_eps^-1 S by power iteration from a fixed start
+// vector (interior edges 1 + (e mod 7) / 7); the Rayleigh quotient
+// x'Sx / x'M_eps x of the last iterate, a lower bound of the true value.
+template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *omega2)
+{
+    EMS_TRY(setup<T>(s));
+    const int n = (int)s->m.ne, gr = row_blocks(n, s->lpr);
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    std::vector<T> x(n), sx(n), mx(n);
+    for (int e = 0; e < n; ++e) x[e] = s->bnd[e] ? (T)0 : (T)(1.0 + (e % 7) / 7.0);
+    DevBuf dx, dy;
+    EMS_TRY(dx.reserve(sizeof(T) * n));
+    EMS_TRY(dy.reserve(sizeof(T) * n));
+    double lam = 0.0;
+    for (int it = 0; it <= iters; ++it) {
+        EMS_HIP(hipMemcpy(dx.p, x.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+        with_lpr(s->lpr, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, s->d_S.as<T>(), dx.as<T>(), s->b.as<T>());
+            k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), dx.as<T>(), dy.as<T>());
+        });
+        EMS_HIP(hipMemcpy(sx.data(), s->b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+        EMS_HIP(hipMemcpy(mx.data(), dy.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+        double xs = 0.0, xm = 0.0;
+        for (int e = 0; e < n; ++e) {
+            xs += (double)x[e] * (double)sx[e];
+            xm += (double)x[e] * (double)mx[e];
+        }
+        if (!(xm > 0.0)) return invalid("mass matrix is not positive on the start vector");
+        lam = xs / xm;
+        if (it == iters) break;
+        // x <- M_eps^-1 S x, normalised (b holds S x)
+        EMS_HIP(hipMemset(dy.p, 0, sizeof(T) * n));
+        EMS_TRY(cg<T>(s, s->d_Me.as<T>(), s->d_dinv_me.as<T>(), dy));
+        EMS_HIP(hipMemcpy(x.data(), dy.p, sizeof(T) * n, hipMemcpyDeviceToHost));
+        double nn = 0.0;
+        for (int e = 0; e < n; ++e) nn += (double)x[e] * (double)x[e];
+        if (!(nn > 0.0)) break;
+        const double sc = 1.0 / std::sqrt(nn);
+        for (int e = 0; e < n; ++e) x[e] = (T)(x[e] * sc);
+    }
+    *omega2 = lam;
+    return EMS_OK;
 }
 
 template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
 {
     EMS_TRY(setup<T>(s));
-    const int tf = s->d.tf, nobs = (int)s->obs_dof.size() / 6;
+    const int tf = s->d.tf, nobs = (int)s->obs_dof.size() / s->nl;
     const long total = s->steps + N;
     if (total / tf + 1 > 2147483647L / 3 / std::max(nobs, 1)) return invalid("too many receiver samples");
     const int nslot = (int)(total / tf + 1);
@@ -565,7 +716,7 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
         EMS_HIP(hipEventRecord(e0, s->stream));
     }
     if (nobs && s0 == 0)  // slot 0: the initial state, stored after this call's samples
-        k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(s->U.as<T>(), nobs, s->d_obs_dof.as<int>(),
+        k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(s->U.as<T>(), nobs, s->nl, s->d_obs_dof.as<int>(),
                                                                  s->d_obs_w.as<double>(),
                                                                  s->d_rec.as<double>() + (size_t)nrec * nobs * 3);
     if (abm && !s->abm_started) {
@@ -606,7 +757,7 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
         }
         if (nobs && (s0 + st) % tf == 0)
             k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(
-                s->U.as<T>(), nobs, s->d_obs_dof.as<int>(), s->d_obs_w.as<double>(),
+                s->U.as<T>(), nobs, s->nl, s->d_obs_dof.as<int>(), s->d_obs_w.as<double>(),
                 s->d_rec.as<double>() + (size_t)((s0 + st) / tf - s0 / tf - 1) * nobs * 3);
         EMS_HIP(hipGetLastError());
     }
@@ -728,7 +879,7 @@ ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3])
         s->obs_dof.push_back((int)e[l]);
         for (int c = 0; c < 3; ++c) s->obs_w.push_back(w[l][c]);
     }
-    const int no = (int)s->obs_dof.size() / 6;
+    const int no = (int)s->obs_dof.size() / s->nl;
     EMS_TRY(s->d_obs_dof.reserve(sizeof(int) * s->obs_dof.size()));
     EMS_TRY(s->d_obs_w.reserve(sizeof(double) * s->obs_w.size()));
     EMS_HIP(hipMemcpy(s->d_obs_dof.p, s->obs_dof.data(), sizeof(int) * s->obs_dof.size(), hipMemcpyHostToDevice));
@@ -759,7 +910,7 @@ ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot)
 {
     clear_error();
     if (!s || !nobs || !nslot) return invalid("null argument");
-    *nobs = (int)s->obs_dof.size() / 6;
+    *nobs = (int)s->obs_dof.size() / s->nl;
     *nslot = s->nslot;
     return EMS_OK;
 }
@@ -829,6 +980,14 @@ ems_status ems_tet_get_time(const ems_tet *s, double *t)
     if (!s || !t) return invalid("null argument");
     *t = s->t_now;
     return EMS_OK;
+}
+
+ems_status ems_tet_max_frequency(ems_tet *s, int iters, double *omega2)
+{
+    clear_error();
+    if (!s || !omega2) return invalid("null argument");
+    if (iters < 1 || iters > 100000) return invalid("iters must be in [1, 100000]");
+    return s->esz == 4 ? max_frequency<float>(s, iters, omega2) : max_frequency<double>(s, iters, omega2);
 }
 
 // conjugate-gradient iterations of the last mass solve and the maximum so far

@@ -15,6 +15,8 @@ from . import _lib
 
 SYMPLECTIC2, SYMPLECTIC4, ABM = 0, 1, 2
 STEPPERS = {"symplectic_2": SYMPLECTIC2, "symplectic_4": SYMPLECTIC4, "abm": ABM}
+# stability limits w_max dt < kappa of each stepper on y'' = -w^2 y
+KAPPA = {SYMPLECTIC2: 2.0, SYMPLECTIC4: 1.5735, ABM: 1.2005}
 
 
 class TetSimulation:
@@ -30,7 +32,7 @@ class TetSimulation:
         hnd = ctypes.c_void_p()
         _lib.check(L.ems_tet_create(ctypes.byref(d), ctypes.byref(hnd)))
         self._h = hnd
-        self.dt, self.tf = float(dt), int(tf)
+        self.dt, self.tf, self.stepper = float(dt), int(tf), st
 
     def close(self):
         if getattr(self, "_h", None):
@@ -104,6 +106,12 @@ class TetSimulation:
         t = ctypes.c_double()
         _lib.check(_lib.load().ems_tet_get_time(self._h, ctypes.byref(t)))
         return t.value
+
+    def max_frequency(self, iters=200):
+        """w_max (rad/s) of M_eps^-1 S by power iteration (a lower bound)."""
+        w2 = ctypes.c_double()
+        _lib.check(_lib.load().ems_tet_max_frequency(self._h, int(iters), ctypes.byref(w2)))
+        return float(np.sqrt(w2.value))
 
     def solver_stats(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -85,3 +85,28 @@ def fdtd_cell_bytes(n, npml, word, material_words):
     d = inpml[:, None, None] + inpml[None, :, None] + inpml[None, None, :]
     extra = int(d.sum()) * word * 8
     return base + extra, (base + extra) / cells
+
+
+def tet_model(n, seed, sigma_fraction=0.0, sigma_max=0.05):
+    """(eps_r, sigma) per tetrahedron of an n^3-cube Kuhn mesh: eps_r per cube
+    log-uniform in [1, 9], smoothed by a Gaussian of 3 cubes (reflecting
+    edges), inherited by the cube's 6 tetrahedra (tetrahedron id
+    6 (i + n (j + n k)) + t); sigma uniform in [0, sigma_max] S/m on a seeded
+    `sigma_fraction` of the tetrahedra (None when 0)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    er = np.exp(rng.uniform(0.0, np.log(9.0), (n, n, n)))  # [k][j][i]
+    er = gaussian_filter(er, 3.0, mode="reflect")
+    er = np.repeat(er.reshape(-1), 6)
+    sg = None
+    if sigma_fraction > 0:
+        sg = np.zeros(er.size)
+        pick = rng.random(er.size) < sigma_fraction
+        sg[pick] = rng.uniform(0.0, sigma_max, int(pick.sum()))
+    return er, sg
+
+
+def tet_receivers(n, h, count, seed):
+    """`count` seeded random interior points (inner 80 % of the cube)."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.1, 0.9, (count, 3)) * n * h

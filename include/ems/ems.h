@@ -396,6 +396,13 @@ ems_status ems_tet_get_field(ems_tet *s, int which, double *out, size_t count);
 ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t count);
 ems_status ems_tet_get_time(const ems_tet *s, double *t);
 
+/* Squared angular frequency w_max^2 = lambda_max(M_eps^-1 S) estimated by
+ * `iters` power iterations (the Rayleigh quotient of the last iterate, a lower
+ * bound).  Stable steps: dt < kappa / w_max with kappa = 2 (symplectic_2),
+ * 1.5735 (symplectic_4), 1.2005 (ABM), the limits of each stepper on
+ * y'' = -w^2 y. */
+ems_status ems_tet_max_frequency(ems_tet *s, int iters, double *omega2);
+
 /* Conjugate-gradient iterations of the last mass solve and the largest so far. */
 ems_status ems_tet_solver_stats(const ems_tet *s, int *last_iters, int *max_iters);
 
