@@ -91,36 +91,41 @@ def gpu_fem1(stepper, n, steps, precision="f64"):
                 dtype="fp64" if precision == "f64" else "fp32")
 
 
-def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, seed=5):
-    """Tetrahedral Whitney edge elements on the Kuhn split of an n^3 unit
-    cube: seeded smoothed log-uniform eps_r, x-dipole (Gaussian pulse) at the
-    centre, 16 random interior receivers, dt = 0.9 x the stepper's stable step
-    (kappa / w_max, w_max by power iteration); DOF-updates/s over the step
-    sequence (HIP events) and the conjugate-gradient iterations per solve."""
+def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, seed=5, sigma_fraction=0.0,
+            power_iters=300):
+    """Tetrahedral Nedelec elements (order 1 or 2) on the Kuhn split of an n^3
+    unit cube: seeded smoothed log-uniform eps_r (sigma uniform in [0, 0.05]
+    S/m on a seeded fraction of the tetrahedra), x-dipole (Gaussian pulse)
+    at the centre, 16 random interior receivers, dt = 0.9 x the stepper's
+    stable step (kappa / w_max, w_max by power iteration on the same handle);
+    DOF-updates/s over the step sequence (HIP events) and the conjugate-
+    gradient iterations per solve."""
     from emsolvers_amd import workloads as wl
     from emsolvers_amd.tet import TetSimulation, KAPPA, STEPPERS
     h = 1.0 / n
-    er, _ = wl.tet_model(n, seed)
-    probe = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, order, precision)
-    probe.set_material(er)
-    wmax = probe.max_frequency(300)
-    probe.close()
+    er, sg = wl.tet_model(n, seed, sigma_fraction)
+    t0 = time.time()
+    s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, order, precision)
+    s.set_material(er, sg)
+    wmax = s.max_frequency(power_iters)
+    setup_s = time.time() - t0
     dt = 0.9 * KAPPA[STEPPERS[stepper]] / wmax
-    s = TetSimulation((n, n, n), h, dt, wl.EPS0, wl.MU0, 1, stepper, order, precision)
-    s.set_material(er)
+    s.set_time_step(dt)
     f0 = 2.0 * wl.C0 / (h * n)  # two vacuum wavelengths across the cube
     s.add_dipole([0.5 + 0.1 * h, 0.5 + 0.2 * h, 0.5 + 0.3 * h], [1.0, 0.0, 0.0], f0, 1.2 / f0, 1e-9)
     for p in wl.tet_receivers(n, h, 16, seed + 1):
         s.add_receiver(p)
-    s.run(2)
     ms = s.run_timed(steps)
     last, mx = s.solver_stats()
     nd = s.dof_count()
     tr = s.traces()
-    return dict(value=round(nd * steps / (ms / 1e3) / 1e6, 2), unit="MDOF-updates/s", dofs=nd,
-                tets=6 * n ** 3, ms_per_step=round(ms / steps, 4), stepper=stepper, order=order,
-                dt=dt, omega_max=wmax, cg_iters_last=last, cg_iters_max=mx,
-                finite=bool(np.all(np.isfinite(tr))), dtype="fp64" if precision == "f64" else "fp32")
+    out = dict(value=round(nd * steps / (ms / 1e3) / 1e6, 2), unit="MDOF-updates/s", dofs=nd, tets=6 * n ** 3,
+               ms_per_step=round(ms / steps, 4), steps=steps, stepper=stepper, order=order, dt=dt,
+               omega_max=wmax, cg_iters_last=last, cg_iters_max=mx, setup_s=round(setup_s, 2),
+               finite=bool(np.all(np.isfinite(tr))), dtype="fp64" if precision == "f64" else "fp32",
+               sigma_fraction=sigma_fraction)
+    s.close()
+    return out
 
 
 def _shot(n, seed, model, npml=10, receivers=(8, 4), z_rec=None, src=None, f0=None, precision="f32"):
@@ -340,7 +345,9 @@ def main():
                       gpu_fem1("fem_abm", 128, 30)]
     if not args.no_fem:
         out["tet"] = [gpu_tet(40, "symplectic_2", args.tet_steps), gpu_tet(40, "abm", args.tet_steps),
-                      gpu_tet(40, "symplectic_2", args.tet_steps, "f32")]
+                      gpu_tet(40, "symplectic_2", args.tet_steps, "f32"),
+                      gpu_tet(64, "symplectic_4", 5, "f32", order=2, sigma_fraction=0.2, power_iters=40),
+                      gpu_tet(64, "symplectic_4", 3, "f64", order=2, sigma_fraction=0.2, power_iters=40)]
     if not args.no_batch:
         out["batched_forward"] = gpu_batch_forward(20)
         out["batched_gradient"] = gpu_batch_gradient()

@@ -117,11 +117,15 @@ SIGNATURES = {
     "ems_fem1_save_results": (_I, [_P, _I, ctypes.c_char_p]),
     "ems_fem1_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
     # tetrahedral edge elements
+    "ems_tet_element_matrices": (_I, [_I, _I, _D, _DP, _DP]),
+    "ems_tet_assemble": (_I, [ctypes.POINTER(TetDesc), _DP, _DP, _IP, _IP, _DP, _DP, _DP, _SZ,
+                              ctypes.POINTER(_SZ)]),
     "ems_tet_create": (_I, [ctypes.POINTER(TetDesc), _PP]),
     "ems_tet_destroy": (None, [_P]),
     "ems_tet_set_material": (_I, [_P, _DP, _DP]),
     "ems_tet_add_dipole": (_I, [_P, _DP, _DP, _D, _D, _D]),
     "ems_tet_add_receiver": (_I, [_P, _DP]),
+    "ems_tet_set_time_step": (_I, [_P, _D]),
     "ems_tet_run": (_I, [_P, _I]),
     "ems_tet_run_timed": (_I, [_P, _I, _DP]),
     "ems_tet_trace_shape": (_I, [_P, _IP, _IP]),

@@ -20,11 +20,14 @@
 // The matrices share one CSR pattern, assembled on the host in a fixed
 // order (entries summed in (row, col, tet) order).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
+#include "tet_basis.hpp"
 #include "tet_mesh.hpp"
 
 using namespace ems;
@@ -284,6 +287,25 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(Csr<T> A, const T *__restr
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it + 1;
 }
 
+// A = Me + hh Ms over the stored entries
+template <typename T>
+__global__ void k_stage_matrix(long nnz, const T *__restrict__ me, const T *__restrict__ ms, T hh, T *__restrict__ a)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnz) a[q] = me[q] + hh * ms[q];
+}
+
+// dinv[r] = 1 / A[r][r] (1 where a row stores no diagonal)
+template <typename T> __global__ void k_diag_inv(Csr<T> A, const T *__restrict__ val, T *__restrict__ dinv)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= A.n) return;
+    T d = 1;
+    for (int q = A.rp[r]; q < A.rp[r + 1]; ++q)
+        if (A.col[q] == r) d = (T)1 / val[q];
+    dinv[r] = d;
+}
+
 // y += a x
 template <typename T> __global__ void k_axpy(int n, T a, const T *__restrict__ x, T *y)
 {
@@ -330,12 +352,14 @@ struct ems_tet {
     DevBuf d_rp, d_col, d_S, d_Me, d_Ms;
     std::vector<double> deltas;  // stage steps with an assembled A_+
     DevBuf d_A[3], d_dinv[3];
+    const void *a_val[3] = {}, *a_dinv[3] = {};  // solve matrix per stage step (d_A or d_Me)
     DevBuf d_dinv_me;  // Jacobi preconditioner of M_eps alone
     DevBuf U, V, b, r, z, p, ap;        // state and CG scratch
     DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
     DevBuf d_partial, d_cg;
     int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
     int nl = 6;   // basis functions per tetrahedron
+    long nd = 0;  // dofs
     bool assembled = false, lossy = false;
     // dipoles: per dipole 6 dofs and the load weights w_l(r0).d
     std::vector<int> dip_dof;
@@ -362,70 +386,261 @@ double pulse_rate(double t, double f0, double t0, double amp)
     return amp * (-2.0 * M_PI * f0 * a) * std::exp(-a * a);
 }
 
-// CSR pattern and matrix values (host, fixed summation order)
+// ---------------------------------------------------------------------------
+// Degrees of freedom.  Order 1: one per edge (the Whitney function), global
+// index = edge.  Order 2: Whitney dofs [0, ne), edge-gradient dofs [ne, 2 ne)
+// (ne + edge), face dofs [2 ne, 2 ne + 2 nf) (2 ne + 2 face + kappa).  Local
+// dofs of a tetrahedron follow the basis order of tet_basis.hpp.
+// ---------------------------------------------------------------------------
+long dof_count(const Mesh &m, int order) { return order == 1 ? m.ne : 2 * m.ne + 2 * m.nf; }
+
+void tet_dofs(const Mesh &m, int order, int i, int j, int k, int t, long *d)
+{
+    long e[6];
+    m.tet_edges(i, j, k, t, e);
+    for (int l = 0; l < 6; ++l) d[l] = e[l];
+    if (order == 1) return;
+    long fc[4];
+    m.tet_faces(i, j, k, t, fc);
+    for (int l = 0; l < 6; ++l) d[6 + l] = m.ne + e[l];
+    for (int q = 0; q < 4; ++q) {
+        d[12 + 2 * q] = 2 * m.ne + 2 * fc[q];
+        d[13 + 2 * q] = 2 * m.ne + 2 * fc[q] + 1;
+    }
+}
+
+bool dof_boundary(const Mesh &m, long d)
+{
+    if (d < 2 * m.ne) return m.boundary(d % m.ne);
+    return m.face_boundary((d - 2 * m.ne) / 2);
+}
+
+// ---------------------------------------------------------------------------
+// Assembly, one CSR row per dof, without a global sort.  A dof belongs to a
+// row type (edge family and kind, or face family and kind); the tetrahedra
+// containing its entity sit at fixed cube offsets from the entity's base
+// vertex, so each row type has a fixed list of column slots (sorted in
+// global index order) and every (tetrahedron, column) contribution lands in
+// a precomputed slot.  A row sums its contributions in the fixed incidence
+// order; slots that no in-range tetrahedron touches and boundary columns are
+// dropped.  PEC: boundary rows are identity rows of the mass-type matrices
+// and zero rows of S.
+// ---------------------------------------------------------------------------
+struct SlotKey {
+    int cls, fam, rk, rj, ri, kap;  // cls 0 Whitney edge, 1 gradient edge, 2 face
+    bool operator<(const SlotKey &o) const
+    {
+        if (cls != o.cls) return cls < o.cls;
+        if (fam != o.fam) return fam < o.fam;
+        if (rk != o.rk) return rk < o.rk;
+        if (rj != o.rj) return rj < o.rj;
+        if (ri != o.ri) return ri < o.ri;
+        return kap < o.kap;
+    }
+    bool operator==(const SlotKey &o) const
+    {
+        return cls == o.cls && fam == o.fam && rk == o.rk && rj == o.rj && ri == o.ri && kap == o.kap;
+    }
+};
+
+struct RowType {
+    struct Inc {
+        int d[3], t, L;  // cube offset from the row's base vertex, tet type, row's local dof
+    };
+    std::vector<Inc> inc;
+    std::vector<SlotKey> slots;
+    std::vector<int> slot_of;  // [inc][nl]
+};
+
+// the entity (class, family, base offset, kappa) of local dof m of tet type t
+SlotKey local_key(int t, int m)
+{
+    int v[4][3];
+    Mesh::verts(t, v);
+    SlotKey k{};
+    if (m < 12) {
+        const int l = m % 6, a = kLocEdge[l][0], b = kLocEdge[l][1];
+        k.cls = m / 6;
+        k.fam = Mesh::family_of(v[b][0] - v[a][0], v[b][1] - v[a][1], v[b][2] - v[a][2]);
+        k.ri = v[a][0];
+        k.rj = v[a][1];
+        k.rk = v[a][2];
+        k.kap = 0;
+    } else {
+        const int q = (m - 12) / 2, a = kLocTri[q][0], b = kLocTri[q][1], c = kLocTri[q][2];
+        const int f1 = Mesh::family_of(v[b][0] - v[a][0], v[b][1] - v[a][1], v[b][2] - v[a][2]);
+        const int f2 = Mesh::family_of(v[c][0] - v[a][0], v[c][1] - v[a][1], v[c][2] - v[a][2]);
+        k.cls = 2;
+        k.fam = Mesh::face_family(f1, f2);
+        k.ri = v[a][0];
+        k.rj = v[a][1];
+        k.rk = v[a][2];
+        k.kap = (m - 12) % 2;
+    }
+    return k;
+}
+
+// row types: edges rt = f + 7 kappa (kappa < order), faces rt = 14 + 2 q + kappa
+std::vector<RowType> row_types(int order)
+{
+    const int nl = basis_size(order);
+    std::vector<RowType> rts(order == 1 ? 7 : 38);
+    for (int t = 0; t < 6; ++t)
+        for (int m = 0; m < nl; ++m) {
+            const SlotKey k = local_key(t, m);
+            const int rt = k.cls < 2 ? k.fam + 7 * k.cls : 14 + 2 * k.fam + k.kap;
+            rts[rt].inc.push_back(RowType::Inc{{-k.ri, -k.rj, -k.rk}, t, m});
+        }
+    for (RowType &R : rts) {
+        for (const RowType::Inc &ic : R.inc)
+            for (int m = 0; m < nl; ++m) {
+                SlotKey k = local_key(ic.t, m);
+                k.ri += ic.d[0];
+                k.rj += ic.d[1];
+                k.rk += ic.d[2];
+                R.slots.push_back(k);
+            }
+        std::sort(R.slots.begin(), R.slots.end());
+        R.slots.erase(std::unique(R.slots.begin(), R.slots.end()), R.slots.end());
+        for (const RowType::Inc &ic : R.inc)
+            for (int m = 0; m < nl; ++m) {
+                SlotKey k = local_key(ic.t, m);
+                k.ri += ic.d[0];
+                k.rj += ic.d[1];
+                k.rk += ic.d[2];
+                R.slot_of.push_back((int)(std::lower_bound(R.slots.begin(), R.slots.end(), k) - R.slots.begin()));
+            }
+    }
+    return rts;
+}
+
+// row type and base vertex of a global dof
+void dof_entity(const Mesh &m, long d, int &rt, int &I, int &J, int &K)
+{
+    if (d < 2 * m.ne) {
+        int f;
+        m.edge_coords(d % m.ne, f, I, J, K);
+        rt = f + 7 * (int)(d / m.ne);
+    } else {
+        int q;
+        m.face_coords((d - 2 * m.ne) / 2, q, I, J, K);
+        rt = 14 + 2 * q + (int)((d - 2 * m.ne) % 2);
+    }
+}
+
+long slot_column(const Mesh &m, const SlotKey &k, int I, int J, int K)
+{
+    if (k.cls < 2) return k.cls * m.ne + m.edge(k.fam, I + k.ri, J + k.rj, K + k.rk);
+    return 2 * m.ne + 2 * m.face(k.fam, I + k.ri, J + k.rj, K + k.rk) + k.kap;
+}
+
 ems_status assemble(ems_tet *s, std::vector<double> &S, std::vector<double> &Me, std::vector<double> &Ms)
 {
     const Mesh &m = s->m;
-    const long ne = m.ne;
-    s->bnd.assign(ne, 0);
-    for (long e = 0; e < ne; ++e) s->bnd[e] = m.boundary(e) ? 1 : 0;
-    double Mt[6][6][6], St[6][6][6];
-    for (int t = 0; t < 6; ++t) whitney(t, m.h, Mt[t], St[t]);
-    // triplets (row, col, tet*36 + l*6 + m) of interior couplings
-    struct Tr {
-        int r, c;
-        long k;
-    };
-    std::vector<Tr> tr;
-    tr.reserve((size_t)m.nt * 36);
-    long tid = 0;
-    for (int k = 0; k < m.nz; ++k)
-        for (int j = 0; j < m.ny; ++j)
-            for (int i = 0; i < m.nx; ++i)
-                for (int t = 0; t < 6; ++t, ++tid) {
-                    long e[6];
-                    m.tet_edges(i, j, k, t, e);
-                    for (int l = 0; l < 6; ++l)
-                        for (int q = 0; q < 6; ++q)
-                            if (!s->bnd[e[l]] && !s->bnd[e[q]]) tr.push_back(Tr{(int)e[l], (int)e[q], tid * 36 + l * 6 + q});
-                }
-    std::stable_sort(tr.begin(), tr.end(), [](const Tr &a, const Tr &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
-    s->rp.assign(ne + 1, 0);
-    s->col.clear();
-    S.clear();
-    Me.clear();
-    Ms.clear();
-    size_t q = 0;
+    const int order = s->d.order, nl = basis_size(order);
+    const long nd = dof_count(m, order);
+    s->bnd.assign(nd, 0);
+    for (long d = 0; d < nd; ++d) s->bnd[d] = dof_boundary(m, d) ? 1 : 0;
+    std::vector<double> Mt(6 * nl * nl), St(6 * nl * nl);
+    for (int t = 0; t < 6; ++t) element_matrices(order, t, m.h, &Mt[t * nl * nl], &St[t * nl * nl]);
+    const std::vector<RowType> rts = row_types(order);
     const double mu = s->d.mu, e0 = s->d.eps;
-    for (long r = 0; r < ne; ++r) {
-        s->rp[r] = (int)s->col.size();
-        if (s->bnd[r]) {  // identity row of the mass-type matrices, zero in S
-            s->col.push_back((int)r);
-            S.push_back(0.0);
-            Me.push_back(1.0);
-            Ms.push_back(0.0);
-            continue;
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto for_rows = [&](auto &&body) {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nth; ++w)
+            th.emplace_back([&, w]() {
+                const long r0 = nd * w / nth, r1 = nd * (w + 1) / nth;
+                std::vector<double> vs, vm, vg;
+                std::vector<char> touched;
+                for (long r = r0; r < r1; ++r) body(r, vs, vm, vg, touched);
+            });
+        for (std::thread &t : th) t.join();
+    };
+    // one row: slot values and touched flags; returns the row type
+    auto row_values = [&](long r, bool values, std::vector<double> &vs, std::vector<double> &vm,
+                          std::vector<double> &vg, std::vector<char> &touched, int &I, int &J, int &K) -> int {
+        int rt;
+        dof_entity(m, r, rt, I, J, K);
+        const RowType &R = rts[rt];
+        const size_t ns = R.slots.size();
+        touched.assign(ns, 0);
+        if (values) {
+            vs.assign(ns, 0.0);
+            vm.assign(ns, 0.0);
+            vg.assign(ns, 0.0);
         }
-        while (q < tr.size() && tr[q].r == r) {
-            const int c = tr[q].c;
-            double vs = 0.0, vm = 0.0, vg = 0.0;
-            for (; q < tr.size() && tr[q].r == r && tr[q].c == c; ++q) {
-                const long tt = tr[q].k / 36;
-                const int l = (int)((tr[q].k % 36) / 6), mm = (int)(tr[q].k % 6);
-                const int type = (int)(tt % 6);
-                const double er = s->eps_r.empty() ? 1.0 : s->eps_r[tt];
-                const double sg = s->sigma.empty() ? 0.0 : s->sigma[tt];
-                vs += St[type][l][mm] / mu;
-                vm += (e0 * er) * Mt[type][l][mm];
-                vg += sg * Mt[type][l][mm];
+        for (size_t ic = 0; ic < R.inc.size(); ++ic) {
+            const RowType::Inc &c = R.inc[ic];
+            const int ci = I + c.d[0], cj = J + c.d[1], ck = K + c.d[2];
+            if (ci < 0 || cj < 0 || ck < 0 || ci >= m.nx || cj >= m.ny || ck >= m.nz) continue;
+            const long tid = 6 * (ci + (long)m.nx * (cj + (long)m.ny * ck)) + c.t;
+            const double er = s->eps_r.empty() ? 1.0 : s->eps_r[tid];
+            const double sg = s->sigma.empty() ? 0.0 : s->sigma[tid];
+            const double *Mrow = &Mt[(c.t * nl + c.L) * nl], *Srow = &St[(c.t * nl + c.L) * nl];
+            const int *so = &R.slot_of[ic * nl];
+            for (int q = 0; q < nl; ++q) {
+                touched[so[q]] = 1;
+                if (values) {
+                    vs[so[q]] += Srow[q] / mu;
+                    vm[so[q]] += (e0 * er) * Mrow[q];
+                    vg[so[q]] += sg * Mrow[q];
+                }
             }
-            s->col.push_back(c);
-            S.push_back(vs);
-            Me.push_back(vm);
-            Ms.push_back(vg);
         }
+        return rt;
+    };
+    // pass 1: row lengths
+    std::vector<int> len(nd);
+    for_rows([&](long r, std::vector<double> &vs, std::vector<double> &vm, std::vector<double> &vg,
+                 std::vector<char> &touched) {
+        if (s->bnd[r]) {
+            len[r] = 1;
+            return;
+        }
+        int I, J, K;
+        const RowType &R = rts[row_values(r, false, vs, vm, vg, touched, I, J, K)];
+        int c = 0;
+        for (size_t q = 0; q < R.slots.size(); ++q)
+            if (touched[q] && !s->bnd[slot_column(m, R.slots[q], I, J, K)]) ++c;
+        len[r] = c;
+    });
+    s->rp.assign(nd + 1, 0);
+    long total = 0;
+    for (long r = 0; r < nd; ++r) {
+        s->rp[r] = (int)total;
+        total += len[r];
+        if (total > INT_MAX) return invalid("tetrahedral mesh too large: more than 2^31 matrix entries");
     }
-    s->rp[ne] = (int)s->col.size();
+    s->rp[nd] = (int)total;
+    s->col.assign(total, 0);
+    S.assign(total, 0.0);
+    Me.assign(total, 0.0);
+    Ms.assign(total, 0.0);
+    // pass 2: columns and values
+    for_rows([&](long r, std::vector<double> &vs, std::vector<double> &vm, std::vector<double> &vg,
+                 std::vector<char> &touched) {
+        long o = s->rp[r];
+        if (s->bnd[r]) {
+            s->col[o] = (int)r;
+            S[o] = 0.0;
+            Me[o] = 1.0;
+            Ms[o] = 0.0;
+            return;
+        }
+        int I, J, K;
+        const RowType &R = rts[row_values(r, true, vs, vm, vg, touched, I, J, K)];
+        for (size_t q = 0; q < R.slots.size(); ++q) {
+            if (!touched[q]) continue;
+            const long c = slot_column(m, R.slots[q], I, J, K);
+            if (s->bnd[c]) continue;
+            s->col[o] = (int)c;
+            S[o] = vs[q];
+            Me[o] = vm[q];
+            Ms[o] = vg[q];
+            ++o;
+        }
+    });
     return EMS_OK;
 }
 
@@ -448,51 +663,64 @@ std::vector<double> stage_deltas(const ems_tet *s)
     return {dt};
 }
 
-template <typename T> ems_status setup(ems_tet *s)
+// Per stage step D: the solve matrix A_+ = Me + D/2 Ms and its Jacobi
+// preconditioner, built on the device from the stored Me and Ms; lossless
+// and ABM solves use Me itself.
+template <typename T> ems_status stage_matrices(ems_tet *s)
 {
-    if (s->assembled) return EMS_OK;
-    std::vector<double> S, Me, Ms;
-    EMS_TRY(assemble(s, S, Me, Ms));
-    s->lossy = false;
-    for (double v : Ms)
-        if (v != 0.0) s->lossy = true;
-    EMS_TRY(s->d_rp.reserve(sizeof(int) * s->rp.size()));
-    EMS_TRY(s->d_col.reserve(sizeof(int) * s->col.size()));
-    EMS_HIP(hipMemcpy(s->d_rp.p, s->rp.data(), sizeof(int) * s->rp.size(), hipMemcpyHostToDevice));
-    EMS_HIP(hipMemcpy(s->d_col.p, s->col.data(), sizeof(int) * s->col.size(), hipMemcpyHostToDevice));
-    EMS_TRY(upload<T>(s->d_S, S));
-    EMS_TRY(upload<T>(s->d_Me, Me));
-    EMS_TRY(upload<T>(s->d_Ms, Ms));
-    // A_+ = Me + D/2 Ms per stage step (ABM: Me) and its Jacobi preconditioner
+    const long nd = s->nd, nnz = (long)s->col.size();
+    const Csr<T> A{(int)nd, s->d_rp.as<int>(), s->d_col.as<int>()};
     s->deltas = stage_deltas(s);
     // Me + D/2 Ms is positive definite when every tet has eps0 eps_r + D/2 sigma > 0
     // (the symplectic_4 middle stage has D < 0)
     for (size_t q = 0; s->lossy && q < s->deltas.size(); ++q)
         for (size_t t = 0; t < s->sigma.size(); ++t) {
             const double er = s->eps_r.empty() ? 1.0 : s->eps_r[t];
-            if (!(s->d.eps * er + 0.5 * s->deltas[q] * s->sigma[t] > 0.0)) {
-                s->assembled = false;
+            if (!(s->d.eps * er + 0.5 * s->deltas[q] * s->sigma[t] > 0.0))
                 return invalid("conductivity too large for the symplectic_4 stages: eps + x2 dt sigma / 2 <= 0");
-            }
         }
+    EMS_TRY(s->d_dinv_me.reserve(sizeof(T) * nd));
+    k_diag_inv<T><<<(int)((nd + 255) / 256), 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_dinv_me.as<T>());
     for (size_t q = 0; q < s->deltas.size(); ++q) {
-        std::vector<double> A(Me.size()), dinv(s->m.ne, 1.0);
-        const double hh = 0.5 * s->deltas[q];
-        for (size_t e = 0; e < A.size(); ++e) A[e] = s->lossy ? Me[e] + hh * Ms[e] : Me[e];
-        for (long r = 0; r < s->m.ne; ++r)
-            for (int e = s->rp[r]; e < s->rp[r + 1]; ++e)
-                if (s->col[e] == r) dinv[r] = 1.0 / A[e];
-        EMS_TRY(upload<T>(s->d_A[q], A));
-        EMS_TRY(upload<T>(s->d_dinv[q], dinv));
+        if (!s->lossy || s->deltas[q] == 0.0) {
+            s->a_val[q] = s->d_Me.p;
+            s->a_dinv[q] = s->d_dinv_me.p;
+            continue;
+        }
+        EMS_TRY(s->d_A[q].reserve(sizeof(T) * std::max<long>(nnz, 1)));
+        EMS_TRY(s->d_dinv[q].reserve(sizeof(T) * nd));
+        k_stage_matrix<T><<<(int)((nnz + 255) / 256), 256, 0, s->stream>>>(
+            nnz, s->d_Me.as<T>(), s->d_Ms.as<T>(), (T)(0.5 * s->deltas[q]), s->d_A[q].as<T>());
+        k_diag_inv<T><<<(int)((nd + 255) / 256), 256, 0, s->stream>>>(A, s->d_A[q].as<T>(), s->d_dinv[q].as<T>());
+        s->a_val[q] = s->d_A[q].p;
+        s->a_dinv[q] = s->d_dinv[q].p;
     }
-    {
-        std::vector<double> dinv(s->m.ne, 1.0);
-        for (long r = 0; r < s->m.ne; ++r)
-            for (int e = s->rp[r]; e < s->rp[r + 1]; ++e)
-                if (s->col[e] == r) dinv[r] = 1.0 / Me[e];
-        EMS_TRY(upload<T>(s->d_dinv_me, dinv));
-    }
-    const size_t vb = sizeof(T) * s->m.ne;
+    EMS_HIP(hipGetLastError());
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    return EMS_OK;
+}
+
+template <typename T> ems_status setup(ems_tet *s)
+{
+    if (s->assembled) return EMS_OK;
+    std::vector<double> S, Me, Ms;
+    EMS_TRY(assemble(s, S, Me, Ms));
+    const long nd = s->nd;
+    s->lossy = false;
+    for (double v : Ms)
+        if (v != 0.0) s->lossy = true;
+    EMS_TRY(s->d_rp.reserve(sizeof(int) * s->rp.size()));
+    EMS_TRY(s->d_col.reserve(sizeof(int) * std::max<size_t>(s->col.size(), 1)));
+    EMS_HIP(hipMemcpy(s->d_rp.p, s->rp.data(), sizeof(int) * s->rp.size(), hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->d_col.p, s->col.data(), sizeof(int) * s->col.size(), hipMemcpyHostToDevice));
+    EMS_TRY(upload<T>(s->d_S, S));
+    S = std::vector<double>();
+    EMS_TRY(upload<T>(s->d_Me, Me));
+    Me = std::vector<double>();
+    if (s->lossy) EMS_TRY(upload<T>(s->d_Ms, Ms));
+    Ms = std::vector<double>();
+    EMS_TRY(stage_matrices<T>(s));
+    const size_t vb = sizeof(T) * nd;
     for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up, &s->Vp,
                       &s->Ap}) {
         EMS_TRY(v->reserve(vb));
@@ -501,7 +729,7 @@ template <typename T> ems_status setup(ems_tet *s)
     EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgNB));
     EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
     int longest = 0;
-    for (long r = 0; r < s->m.ne; ++r) longest = std::max(longest, s->rp[r + 1] - s->rp[r]);
+    for (long r = 0; r < nd; ++r) longest = std::max(longest, s->rp[r + 1] - s->rp[r]);
     s->lpr = longest > 32 ? 8 : 4;
     s->assembled = true;
     return EMS_OK;
@@ -526,7 +754,7 @@ inline int row_blocks(long n, int lpr) { return (int)((n * lpr + 255) / 256); }
 // x <- A^-1 b (initial guess x) for the SPD matrix `val` on the common pattern
 template <typename T> ems_status cg(ems_tet *s, const T *val, const T *dinv, DevBuf &x)
 {
-    const int n = (int)s->m.ne;
+    const int n = (int)s->nd;
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     // FP32: the recursively updated residual keeps converging below the
     // rounding of the matrix entries, so 1e-7 is reachable
@@ -579,7 +807,7 @@ void dipole_factors(const ems_tet *s, double t, double scale, std::vector<double
 
 template <typename T> ems_status stage(ems_tet *s, int q, double D, const double *fdev)
 {
-    const int n = (int)s->m.ne, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
+    const int n = (int)s->nd, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
     with_lpr(s->lpr, [&](auto L) {
@@ -594,7 +822,7 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, const double
                                                             s->b.as<T>());
     });
     load<T>(s, s->b, fdev);
-    EMS_TRY(cg<T>(s, s->d_A[q].as<T>(), s->d_dinv[q].as<T>(), s->V));
+    EMS_TRY(cg<T>(s, (const T *)s->a_val[q], (const T *)s->a_dinv[q], s->V));
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
     return EMS_OK;
 }
@@ -602,7 +830,7 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, const double
 // a = Me^-1 (-S U - Ms V + load) into dst (initial guess: dst)
 template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
 {
-    const int n = (int)s->m.ne, gr = row_blocks(n, s->lpr);
+    const int n = (int)s->nd, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     with_lpr(s->lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
@@ -612,7 +840,7 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
             k_force<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
     });
     load<T>(s, s->b, fdev);
-    return cg<T>(s, s->d_A[0].as<T>(), s->d_dinv[0].as<T>(), dst);
+    return cg<T>(s, (const T *)s->a_val[0], (const T *)s->a_dinv[0], dst);
 }
 
 // Largest eigenvalue of M_eps^-1 S by power iteration from a fixed start
@@ -621,7 +849,7 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
 template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *omega2)
 {
     EMS_TRY(setup<T>(s));
-    const int n = (int)s->m.ne, gr = row_blocks(n, s->lpr);
+    const int n = (int)s->nd, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     std::vector<T> x(n), sx(n), mx(n);
     for (int e = 0; e < n; ++e) x[e] = s->bnd[e] ? (T)0 : (T)(1.0 + (e % 7) / 7.0);
@@ -676,7 +904,7 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
         s->nslot = nslot;
     }
     if (N <= 0) return EMS_OK;
-    const int n = (int)s->m.ne, g = (n + 255) / 256;
+    const int n = (int)s->nd, g = (n + 255) / 256;
     const double dt = s->d.dt;
     const int nd = (int)s->dip_par.size() / 3;
     // per-call tables: load factors per stage / evaluation, in order
@@ -797,13 +1025,14 @@ ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out)
     if (!desc || !out) return invalid("ems_tet_create: null argument");
     *out = nullptr;
     if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return invalid("grid sizes must be >= 1");
-    if (desc->order != 1) return invalid("only first-order tetrahedral elements are available");
+    if (desc->order != 1 && desc->order != 2) return invalid("tetrahedral element order must be 1 or 2");
     if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
         return invalid("h, dt, eps and mu must be positive");
     if (desc->tf <= 0) return invalid("time_frequency must be >= 1");
     if (desc->stepper < EMS_TET_SYMPLECTIC2 || desc->stepper > EMS_TET_ABM) return invalid("unknown stepper");
     if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
-    if ((double)desc->nx * desc->ny * desc->nz * 7.0 > 2.0e9) return invalid("mesh too large for 32-bit indices");
+    if ((double)desc->nx * desc->ny * desc->nz * (desc->order == 1 ? 7.0 : 38.0) > 2.0e9)
+        return invalid("mesh too large for 32-bit indices");
     if (ems_device_count() <= 0) {
         set_error("no HIP device available");
         return EMS_ERR_HIP;
@@ -812,6 +1041,8 @@ ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out)
     s->d = *desc;
     s->esz = desc->precision == EMS_F32 ? 4 : 8;
     s->m.init(desc->nx, desc->ny, desc->nz, desc->h);
+    s->nl = basis_size(desc->order);
+    s->nd = dof_count(s->m, desc->order);
     *out = s;
     return EMS_OK;
 }
@@ -841,16 +1072,16 @@ ems_status ems_tet_add_dipole(ems_tet *s, const double pos[3], const double dir[
     if (!s || !pos || !dir) return invalid("null argument");
     if (!(f0 > 0)) return invalid("f0 must be positive");
     int i, j, k;
-    double lam[4], w[6][3];
+    double lam[4], w[20][3];
     const int t = locate(s->m, pos, i, j, k, lam);
     if (t < 0) return invalid("dipole outside the mesh");
-    whitney_at(t, s->m.h, lam, w);
-    long e[6];
-    s->m.tet_edges(i, j, k, t, e);
-    for (int l = 0; l < 6; ++l) {
+    basis_at(s->d.order, t, s->m.h, lam, w);
+    long e[20];
+    tet_dofs(s->m, s->d.order, i, j, k, t, e);
+    for (int l = 0; l < s->nl; ++l) {
         s->dip_dof.push_back((int)e[l]);
         const double v = w[l][0] * dir[0] + w[l][1] * dir[1] + w[l][2] * dir[2];
-        s->dip_w.push_back(s->m.boundary(e[l]) ? 0.0 : v);
+        s->dip_w.push_back(dof_boundary(s->m, e[l]) ? 0.0 : v);
     }
     s->dip_par.push_back(f0);
     s->dip_par.push_back(t0);
@@ -869,13 +1100,13 @@ ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3])
     clear_error();
     if (!s || !pos) return invalid("null argument");
     int i, j, k;
-    double lam[4], w[6][3];
+    double lam[4], w[20][3];
     const int t = locate(s->m, pos, i, j, k, lam);
     if (t < 0) return invalid("receiver outside the mesh");
-    whitney_at(t, s->m.h, lam, w);
-    long e[6];
-    s->m.tet_edges(i, j, k, t, e);
-    for (int l = 0; l < 6; ++l) {
+    basis_at(s->d.order, t, s->m.h, lam, w);
+    long e[20];
+    tet_dofs(s->m, s->d.order, i, j, k, t, e);
+    for (int l = 0; l < s->nl; ++l) {
         s->obs_dof.push_back((int)e[l]);
         for (int c = 0; c < 3; ++c) s->obs_w.push_back(w[l][c]);
     }
@@ -906,6 +1137,24 @@ ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms)
     return s->esz == 4 ? run_impl<float>(s, num_steps, total_ms) : run_impl<double>(s, num_steps, total_ms);
 }
 
+ems_status ems_tet_set_time_step(ems_tet *s, double dt)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (!(dt > 0)) return invalid("dt must be positive");
+    if (s->steps > 0) return invalid("the time step can only change before the first step");
+    const double old = s->d.dt;
+    s->d.dt = dt;
+    if (!s->assembled) return EMS_OK;
+    const ems_status st = s->esz == 4 ? stage_matrices<float>(s) : stage_matrices<double>(s);
+    if (st != EMS_OK) {
+        s->d.dt = old;
+        const ems_status back = s->esz == 4 ? stage_matrices<float>(s) : stage_matrices<double>(s);
+        (void)back;
+    }
+    return st;
+}
+
 ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot)
 {
     clear_error();
@@ -928,7 +1177,7 @@ ems_status ems_tet_dof_count(const ems_tet *s, size_t *n)
 {
     clear_error();
     if (!s || !n) return invalid("null argument");
-    *n = (size_t)s->m.ne;
+    *n = (size_t)s->nd;
     return EMS_OK;
 }
 
@@ -938,7 +1187,7 @@ ems_status ems_tet_get_field(ems_tet *s, int which, double *out, size_t count)
     clear_error();
     if (!s || !out) return invalid("null argument");
     if (which < 0 || which > 1) return invalid("field must be 0 (E) or 1 (dE/dt)");
-    if (count != (size_t)s->m.ne) return invalid("field buffer has the wrong size");
+    if (count != (size_t)s->nd) return invalid("field buffer has the wrong size");
     if (!s->assembled) {
         std::fill(out, out + count, 0.0);
         return EMS_OK;
@@ -960,7 +1209,7 @@ ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t cou
     clear_error();
     if (!s || !in) return invalid("null argument");
     if (which < 0 || which > 1) return invalid("field must be 0 (E) or 1 (dE/dt)");
-    if (count != (size_t)s->m.ne) return invalid("field buffer has the wrong size");
+    if (count != (size_t)s->nd) return invalid("field buffer has the wrong size");
     ems_status st = s->esz == 4 ? setup<float>(s) : setup<double>(s);
     if (st != EMS_OK) return st;
     DevBuf &b = which == 0 ? s->U : s->V;
@@ -979,6 +1228,45 @@ ems_status ems_tet_get_time(const ems_tet *s, double *t)
     clear_error();
     if (!s || !t) return invalid("null argument");
     *t = s->t_now;
+    return EMS_OK;
+}
+
+ems_status ems_tet_assemble(const ems_tet_desc *desc, const double *eps_r, const double *sigma, int *rp, int *col,
+                            double *S, double *Me, double *Ms, size_t capacity, size_t *nnz)
+{
+    clear_error();
+    if (!desc || !nnz) return invalid("null argument");
+    if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return invalid("grid sizes must be >= 1");
+    if (desc->order != 1 && desc->order != 2) return invalid("tetrahedral element order must be 1 or 2");
+    if (!(desc->h > 0) || !(desc->eps > 0) || !(desc->mu > 0)) return invalid("h, eps and mu must be positive");
+    ems_tet s;
+    s.d = *desc;
+    s.m.init(desc->nx, desc->ny, desc->nz, desc->h);
+    s.nl = basis_size(desc->order);
+    s.nd = dof_count(s.m, desc->order);
+    if (eps_r) s.eps_r.assign(eps_r, eps_r + s.m.nt);
+    if (sigma) s.sigma.assign(sigma, sigma + s.m.nt);
+    std::vector<double> vS, vM, vG;
+    EMS_TRY(assemble(&s, vS, vM, vG));
+    *nnz = s.col.size();
+    if (capacity == 0) return EMS_OK;
+    if (capacity < s.col.size() || !rp || !col || !S || !Me || !Ms) return invalid("output arrays too small");
+    std::copy(s.rp.begin(), s.rp.end(), rp);
+    std::copy(s.col.begin(), s.col.end(), col);
+    std::copy(vS.begin(), vS.end(), S);
+    std::copy(vM.begin(), vM.end(), Me);
+    std::copy(vG.begin(), vG.end(), Ms);
+    return EMS_OK;
+}
+
+ems_status ems_tet_element_matrices(int order, int type, double h, double *M, double *S)
+{
+    clear_error();
+    if (order != 1 && order != 2) return invalid("order must be 1 or 2");
+    if (type < 0 || type > 5) return invalid("tetrahedron type must be in [0, 5]");
+    if (!(h > 0)) return invalid("h must be positive");
+    if (!M || !S) return invalid("null argument");
+    element_matrices(order, type, h, M, S);
     return EMS_OK;
 }
 

@@ -1,6 +1,6 @@
-// Structured tetrahedral mesh (Kuhn split of an nx x ny x nz hex grid) and
-// the first-order Nedelec (Whitney) element matrices.  Host-side geometry
-// and assembly of the tetrahedral edge-element solver; no device code.
+// Structured tetrahedral mesh (Kuhn split of an nx x ny x nz hex grid):
+// edge and face numbering, barycentric geometry and point location.  Host
+// code of the tetrahedral edge-element solver; no device code.
 //
 // Each cube (i,j,k) is split into 6 tetrahedra, one per permutation p of the
 // axes, with vertices v0 = 0, v1 = e_p0, v2 = e_p0 + e_p1, v3 = (1,1,1)
@@ -10,6 +10,7 @@
 // mesh fall into 7 families by direction: x, y, z, x+y, x+z, y+z, x+y+z.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <vector>
@@ -20,12 +21,20 @@ namespace tet {
 static const int kFam[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
 static const int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 static const int kLocEdge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+// Faces: a face with vertices p < q < r (global order) is identified by its
+// base vertex p and the edge families of q - p and r - p; the 12 families
+// that occur in the split, in this order (pairs of kFam indices):
+static const int kFFam[12][2] = {{0, 3}, {0, 4}, {0, 6}, {1, 3}, {1, 5}, {1, 6},
+                                 {2, 4}, {2, 5}, {2, 6}, {3, 6}, {4, 6}, {5, 6}};
+// local faces of a tetrahedron (local vertex triples)
+static const int kLocTri[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 
 struct Mesh {
     int nx = 0, ny = 0, nz = 0;
     double h = 0;
-    long off[8] = {};  // first edge of each family
-    long ne = 0, nt = 0;
+    long off[8] = {};    // first edge of each family
+    long foff[13] = {};  // first face of each face family
+    long ne = 0, nf = 0, nt = 0;
 
     void init(int x, int y, int z, double hh)
     {
@@ -40,14 +49,69 @@ struct Mesh {
         }
         off[7] = o;
         ne = o;
+        o = 0;
+        for (int q = 0; q < 12; ++q) {
+            foff[q] = o;
+            const int *d = kFam[kFFam[q][1]];  // the farther vertex spans the face's extent
+            o += (long)(nx + 1 - d[0]) * (ny + 1 - d[1]) * (nz + 1 - d[2]);
+        }
+        foff[12] = o;
+        nf = o;
         nt = 6L * nx * ny * nz;
     }
-    int family(int dx, int dy, int dz) const
+    static int face_family(int f1, int f2)
+    {
+        for (int q = 0; q < 12; ++q)
+            if (kFFam[q][0] == f1 && kFFam[q][1] == f2) return q;
+        return -1;
+    }
+    long face(int q, int I, int J, int K) const
+    {
+        const int *d = kFam[kFFam[q][1]];
+        const long cx = nx + 1 - d[0], cy = ny + 1 - d[1];
+        return foff[q] + I + cx * (J + cy * (long)K);
+    }
+    // global faces of tet (cube i,j,k; type t) in local face order
+    void tet_faces(int i, int j, int k, int t, long fc[4]) const
+    {
+        int v[4][3];
+        verts(t, v);
+        for (int q = 0; q < 4; ++q) {
+            const int a = kLocTri[q][0], b = kLocTri[q][1], c = kLocTri[q][2];
+            const int f1 = family(v[b][0] - v[a][0], v[b][1] - v[a][1], v[b][2] - v[a][2]);
+            const int f2 = family(v[c][0] - v[a][0], v[c][1] - v[a][1], v[c][2] - v[a][2]);
+            fc[q] = face(face_family(f1, f2), i + v[a][0], j + v[a][1], k + v[a][2]);
+        }
+    }
+    void face_coords(long fid, int &q, int &I, int &J, int &K) const
+    {
+        q = 0;
+        while (fid >= foff[q + 1]) ++q;
+        const long r = fid - foff[q];
+        const int *d = kFam[kFFam[q][1]];
+        const long cx = nx + 1 - d[0], cy = ny + 1 - d[1];
+        I = (int)(r % cx);
+        J = (int)((r / cx) % cy);
+        K = (int)(r / (cx * cy));
+    }
+    // a face lying in an outer face of the box
+    bool face_boundary(long fid) const
+    {
+        int q, I, J, K;
+        face_coords(fid, q, I, J, K);
+        const int *d1 = kFam[kFFam[q][0]], *d2 = kFam[kFFam[q][1]];
+        const int p[3] = {I, J, K}, n[3] = {nx, ny, nz};
+        for (int a = 0; a < 3; ++a)
+            if (d1[a] == 0 && d2[a] == 0 && (p[a] == 0 || p[a] == n[a])) return true;
+        return false;
+    }
+    static int family_of(int dx, int dy, int dz)
     {
         for (int f = 0; f < 7; ++f)
             if (kFam[f][0] == dx && kFam[f][1] == dy && kFam[f][2] == dz) return f;
         return -1;
     }
+    int family(int dx, int dy, int dz) const { return family_of(dx, dy, dz); }
     long edge(int f, int I, int J, int K) const
     {
         const long cx = nx + 1 - kFam[f][0], cy = ny + 1 - kFam[f][1];
@@ -126,32 +190,6 @@ inline void bary(int t, double h, double g[4][3], double &vol)
     vol = std::fabs(det) / 6.0;
 }
 
-// Whitney element matrices of tet type t (local edge order kLocEdge):
-//   M_lm = int w_l . w_m,   w_ab = l_a grad l_b - l_b grad l_a,
-//   int l_i l_j = V (1 + d_ij) / 20;
-//   S_lm = int curl w_l . curl w_m = V (2 g_a x g_b) . (2 g_c x g_d).
-inline void whitney(int t, double h, double M[6][6], double S[6][6])
-{
-    double g[4][3], V;
-    bary(t, h, g, V);
-    auto dot = [](const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-    auto I2 = [&](int i, int j) { return V * (i == j ? 2.0 : 1.0) / 20.0; };
-    double C[6][3];
-    for (int l = 0; l < 6; ++l) {
-        const int a = kLocEdge[l][0], b = kLocEdge[l][1];
-        C[l][0] = 2.0 * (g[a][1] * g[b][2] - g[a][2] * g[b][1]);
-        C[l][1] = 2.0 * (g[a][2] * g[b][0] - g[a][0] * g[b][2]);
-        C[l][2] = 2.0 * (g[a][0] * g[b][1] - g[a][1] * g[b][0]);
-    }
-    for (int l = 0; l < 6; ++l)
-        for (int m = 0; m < 6; ++m) {
-            const int a = kLocEdge[l][0], b = kLocEdge[l][1], c = kLocEdge[m][0], d = kLocEdge[m][1];
-            M[l][m] = dot(g[b], g[d]) * I2(a, c) - dot(g[b], g[c]) * I2(a, d) - dot(g[a], g[d]) * I2(b, c) +
-                      dot(g[a], g[c]) * I2(b, d);
-            S[l][m] = V * dot(C[l], C[m]);
-        }
-}
-
 // Tet (type) containing point x (physical) of cube (i,j,k) and the
 // barycentric coordinates there; returns -1 outside the mesh.
 inline int locate(const Mesh &m, const double x[3], int &i, int &j, int &k, double lam[4])
@@ -181,17 +219,6 @@ inline int locate(const Mesh &m, const double x[3], int &i, int &j, int &k, doub
     lam[2] = r[p[1]] - r[p[2]];
     lam[3] = r[p[2]];
     return t;
-}
-
-// Values of the 6 local Whitney functions at barycentric point lam.
-inline void whitney_at(int t, double h, const double lam[4], double w[6][3])
-{
-    double g[4][3], V;
-    bary(t, h, g, V);
-    for (int l = 0; l < 6; ++l) {
-        const int a = kLocEdge[l][0], b = kLocEdge[l][1];
-        for (int c = 0; c < 3; ++c) w[l][c] = lam[a] * g[b][c] - lam[b] * g[a][c];
-    }
 }
 
 }  // namespace tet

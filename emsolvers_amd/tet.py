@@ -1,4 +1,4 @@
-"""Tetrahedral first-order Nedelec (Whitney) edge elements over the C ABI.
+"""Tetrahedral Nedelec (first kind, order 1 and 2) edge elements over the C ABI.
 
 No reference counterpart: the structured-hex edge-FEM steppers of
 src/symplectic_2, src/symplectic_4 (symplectic stages) and src/fem_abm
@@ -17,6 +17,42 @@ SYMPLECTIC2, SYMPLECTIC4, ABM = 0, 1, 2
 STEPPERS = {"symplectic_2": SYMPLECTIC2, "symplectic_4": SYMPLECTIC4, "abm": ABM}
 # stability limits w_max dt < kappa of each stepper on y'' = -w^2 y
 KAPPA = {SYMPLECTIC2: 2.0, SYMPLECTIC4: 1.5735, ABM: 1.2005}
+
+
+def element_matrices(order, tet_type, h):
+    """(M, S) element matrices of Kuhn tetrahedron `tet_type` (host computation)."""
+    nl = 6 if order == 1 else 20
+    M = np.zeros((nl, nl))
+    S = np.zeros((nl, nl))
+    _lib.check(_lib.load().ems_tet_element_matrices(int(order), int(tet_type), float(h), _lib.dptr(M),
+                                                    _lib.dptr(S)))
+    return M, S
+
+
+def assemble(grid, h, eps, mu, order=1, eps_r=None, sigma=None):
+    """The library's assembled (S, M_eps, M_sig) as scipy CSR matrices (host computation)."""
+    import scipy.sparse as sp
+    g = [int(v) for v in grid]
+    d = _lib.TetDesc(g[0], g[1], g[2], float(h), 1.0, float(eps), float(mu), 1, 0, int(order), 0)
+    er = None if eps_r is None else np.ascontiguousarray(eps_r, dtype=np.float64)
+    sg = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
+    L = _lib.load()
+    nnz = ctypes.c_size_t()
+    args = (_lib.dptr(er) if er is not None else None, _lib.dptr(sg) if sg is not None else None)
+    _lib.check(L.ems_tet_assemble(ctypes.byref(d), *args, None, None, None, None, None, 0, ctypes.byref(nnz)))
+    n = nnz.value
+    ne = sum((g[0] + 1 - a) * (g[1] + 1 - b) * (g[2] + 1 - c)
+             for a, b, c in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)))
+    nf = sum((g[0] + 1 - a) * (g[1] + 1 - b) * (g[2] + 1 - c)
+             for a, b, c in ((1, 1, 0),) * 2 + ((1, 0, 1),) * 2 + ((0, 1, 1),) * 2 + ((1, 1, 1),) * 6)
+    nd = ne if order == 1 else 2 * ne + 2 * nf
+    rp = np.zeros(nd + 1, dtype=np.int32)
+    col = np.zeros(n, dtype=np.int32)
+    S, Me, Ms = np.zeros(n), np.zeros(n), np.zeros(n)
+    _lib.check(L.ems_tet_assemble(ctypes.byref(d), *args, _lib.iptr(rp), _lib.iptr(col), _lib.dptr(S),
+                                  _lib.dptr(Me), _lib.dptr(Ms), n, ctypes.byref(nnz)))
+    mk = lambda v: sp.csr_matrix((v, col, rp), shape=(nd, nd))
+    return mk(S), mk(Me), mk(Ms)
 
 
 class TetSimulation:
@@ -71,6 +107,11 @@ class TetSimulation:
     def add_receiver(self, pos):
         p = np.asarray(pos, dtype=np.float64)
         _lib.check(_lib.load().ems_tet_add_receiver(self._h, _lib.dptr(p)))
+
+    def set_time_step(self, dt):
+        """New dt before the first step (no reassembly)."""
+        _lib.check(_lib.load().ems_tet_set_time_step(self._h, float(dt)))
+        self.dt = float(dt)
 
     def run(self, num_steps):
         _lib.check(_lib.load().ems_tet_run(self._h, int(num_steps)))

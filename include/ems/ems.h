@@ -349,7 +349,14 @@ ems_status ems_fem1_dof_count(const ems_fem1 *s, size_t *n);
  * the consistent mass matrix solved by conjugate gradients on the device and
  * PEC outer faces.  Edge numbering: 7 families (x, y, z, x+y, x+z, y+z,
  * x+y+z directions), family f edges at base vertex (I, J, K) numbered
- * off_f + I + cx (J + cy K), cx = nx + 1 - dx_f, cy = ny + 1 - dy_f. */
+ * off_f + I + cx (J + cy K), cx = nx + 1 - dx_f, cy = ny + 1 - dy_f.
+ * Faces (order 2): a face with vertices p < q < r is numbered by the edge
+ * families of (q - p, r - p), 12 families in the order (x, x+y), (x, x+z),
+ * (x, x+y+z), (y, x+y), (y, y+z), (y, x+y+z), (z, x+z), (z, y+z),
+ * (z, x+y+z), (x+y, x+y+z), (x+z, x+y+z), (y+z, x+y+z), then base vertex
+ * p as for edges with the extent of r - p.  Dofs: order 1 = edges; order 2
+ * = Whitney dofs (edge e), edge-gradient dofs (ne + e), face dofs
+ * (2 ne + 2 face + kappa). */
 
 enum { EMS_TET_SYMPLECTIC2 = 0, EMS_TET_SYMPLECTIC4 = 1, EMS_TET_ABM = 2 };
 
@@ -362,9 +369,24 @@ typedef struct {
     double eps, mu;   /* background permittivity, permeability          */
     int tf;           /* receiver sampling: every tf steps              */
     int stepper;      /* EMS_TET_SYMPLECTIC2 / _SYMPLECTIC4 / _ABM      */
-    int order;        /* element order: 1 (Whitney)                     */
+    int order;        /* element order: 1 (Whitney) or 2                */
     int precision;    /* EMS_F64 or EMS_F32                             */
 } ems_tet_desc;
+
+/* Element mass M = int w_l . w_m and curl-curl S = int curl w_l . curl w_m
+ * (nl x nl row-major, nl = 6 for order 1, 20 for order 2) of Kuhn
+ * tetrahedron `type` (0..5: the axis permutation of its vertex path) of a
+ * cube of edge h.  Local dofs: order 1: the Whitney functions of local edges
+ * (01, 02, 03, 12, 13, 23); order 2: those 6, the edge gradients
+ * grad(l_a l_b) of the same edges, then per local face (012, 013, 023, 123)
+ * l_a w_bc and l_b w_ac.  Host only (no device needed). */
+ems_status ems_tet_element_matrices(int order, int type, double h, double *M, double *S);
+
+/* The assembled CSR matrices of a mesh (host only): rp (dofs + 1), col, S,
+ * M_eps, M_sig (nnz each), PEC rows as identity rows of the mass-type
+ * matrices.  Call with capacity 0 to get *nnz, then with arrays of that size. */
+ems_status ems_tet_assemble(const ems_tet_desc *desc, const double *eps_r, const double *sigma, int *rp, int *col,
+                            double *S, double *Me, double *Ms, size_t capacity, size_t *nnz);
 
 ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out);
 void ems_tet_destroy(ems_tet *s);
@@ -381,6 +403,10 @@ ems_status ems_tet_add_dipole(ems_tet *s, const double pos[3], const double dir[
 
 /* Receiver of the three components of E at pos. */
 ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3]);
+
+/* Change the time step before the first step (rebuilds the stage solve
+ * matrices on the device; no reassembly). */
+ems_status ems_tet_set_time_step(ems_tet *s, double dt);
 
 ems_status ems_tet_run(ems_tet *s, int num_steps);
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);

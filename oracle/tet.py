@@ -1,4 +1,4 @@
-"""FP64 oracle of the tetrahedral Whitney edge-element solver (numpy / scipy).
+"""FP64 oracle of the tetrahedral Nedelec edge-element solver (numpy / scipy).
 
 TEST INFRASTRUCTURE ONLY.  The solver has no reference counterpart; it
 applies the time steppers of src/symplectic_2/fem_simulation.cpp:557-672
@@ -9,10 +9,13 @@ edge-element semi-discretisation on tetrahedra
     M_eps U'' + M_sig U' + S U = -g(t),  g_i = p'(t) (w_i(r0) . d).
 
 This restatement is independent of the library's host code: the element
-matrices come from quadrature of the Whitney functions (not the closed forms),
+matrices come from quadrature of the basis functions and of their curls
+written out by hand (the library integrates barycentric monomials exactly),
 barycentric coordinates from a 4 x 4 inverse, assembly from scipy.sparse and
 the mass solves from a sparse LU factorisation (the library uses conjugate
-gradients on the device).
+gradients on the device).  Order 1: the 6 Whitney functions; order 2 (first
+kind, degree 2, 20 functions): the Whitney functions, the edge gradients
+grad(l_a l_b) and per face (a, b, c) the functions l_a w_bc, l_b w_ac.
 
 Pins (tests/test_oracle_tet.py):
   * null space: S G = 0 for the vertex-to-edge incidence G (a discrete
@@ -25,7 +28,10 @@ Pins (tests/test_oracle_tet.py):
   * interpolation: receivers reproduce a constant field exactly;
   * steppers: on a discrete eigenmode the exact solution is cos(w t)
     (damped for uniform sigma): symplectic_2 and ABM converge at second
-    order, symplectic_4 at fourth.
+    order, symplectic_4 at fourth;
+  * order 2: element curl-curl rank 11 (its kernel grad P2 has dimension
+    9), dim ker S = interior vertices + interior edges, the cavity modes
+    converge at fourth order in h, and L2 projection reproduces linear fields.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -39,10 +45,28 @@ FAM = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, 0, 1], [0, 1, 1]
 PERM = np.array([[0, 1, 2], [0, 2, 1], [1, 0, 2], [1, 2, 0], [2, 0, 1], [2, 1, 0]])
 LOC_EDGE = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
-# degree-2 exact quadrature on a tetrahedron (barycentric points, weights / volume)
-_QA, _QB = 0.5854101966249685, 0.1381966011250105
-QPTS = np.array([[_QA, _QB, _QB, _QB], [_QB, _QA, _QB, _QB], [_QB, _QB, _QA, _QB], [_QB, _QB, _QB, _QA]])
-QW = np.full(4, 0.25)
+# face families: (edge family of q - p, edge family of r - p) of a face p < q < r
+FFAM = [(0, 3), (0, 4), (0, 6), (1, 3), (1, 5), (1, 6), (2, 4), (2, 5), (2, 6), (3, 6), (4, 6), (5, 6)]
+LOC_TRI = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+
+
+def _collapsed_rule(n):
+    """Collapsed-coordinate Gauss rule on the reference tetrahedron: barycentric
+    points (npts x 4) and weights summing to 1 (fraction of the volume); exact
+    for polynomials of degree <= 2n - 3."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    x, w = 0.5 * (x + 1), 0.5 * w
+    pts, wts = [], []
+    for a, wa in zip(x, w):
+        for b, wb in zip(x, w):
+            for c, wc in zip(x, w):
+                X, Y, Z = a, (1 - a) * b, (1 - a) * (1 - b) * c
+                pts.append([1 - X - Y - Z, X, Y, Z])
+                wts.append(6.0 * wa * wb * wc * (1 - a) ** 2 * (1 - b))
+    return np.array(pts), np.array(wts)
+
+
+QPTS, QW = _collapsed_rule(5)
 
 
 def tet_vertices(t):
@@ -62,16 +86,38 @@ def bary_grad(X):
     return C[1:, :].T.copy(), abs(np.linalg.det(A)) / 6.0
 
 
-def element_matrices(t, h):
+def basis_values(lam, G, order=1):
+    """(nl x 3) basis values and curls at barycentric point lam."""
+    W, C = [], []
+    for a, b in LOC_EDGE:
+        W.append(lam[a] * G[b] - lam[b] * G[a])
+        C.append(2.0 * np.cross(G[a], G[b]))
+    if order == 2:
+        for a, b in LOC_EDGE:
+            W.append(lam[a] * G[b] + lam[b] * G[a])
+            C.append(np.zeros(3))
+        for a, b, c in LOC_TRI:
+            # l_a w_bc: curl = grad l_a x w_bc + l_a curl w_bc
+            wbc = lam[b] * G[c] - lam[c] * G[b]
+            W.append(lam[a] * wbc)
+            C.append(np.cross(G[a], wbc) + 2.0 * lam[a] * np.cross(G[b], G[c]))
+            wac = lam[a] * G[c] - lam[c] * G[a]
+            W.append(lam[b] * wac)
+            C.append(np.cross(G[b], wac) + 2.0 * lam[b] * np.cross(G[a], G[c]))
+    return np.array(W), np.array(C)
+
+
+def element_matrices(t, h, order=1):
     """(M, S) by quadrature: M_lm = int w_l . w_m, S_lm = int curl w_l . curl w_m."""
     X = tet_vertices(t) * float(h)
     G, V = bary_grad(X)
-    M = np.zeros((6, 6))
+    nl = 6 if order == 1 else 20
+    M = np.zeros((nl, nl))
+    S = np.zeros((nl, nl))
     for lam, wq in zip(QPTS, QW):
-        W = np.array([lam[a] * G[b] - lam[b] * G[a] for a, b in LOC_EDGE])
+        W, C = basis_values(lam, G, order)
         M += wq * V * (W @ W.T)
-    Cu = np.array([2.0 * np.cross(G[a], G[b]) for a, b in LOC_EDGE])
-    S = V * (Cu @ Cu.T)
+        S += wq * V * (C @ C.T)
     return M, S
 
 
@@ -79,13 +125,19 @@ class Mesh:
     """Kuhn split of an nx x ny x nz grid of cubes of edge h; edge numbering of
     family f at base vertex (I, J, K): off_f + I + cx (J + cy K)."""
 
-    def __init__(self, nx, ny, nz, h):
+    def __init__(self, nx, ny, nz, h, order=1):
         self.n = np.array([nx, ny, nz])
         self.h = float(h)
+        self.order = int(order)
         self.cnt = [(nx + 1 - d[0]) * (ny + 1 - d[1]) * (nz + 1 - d[2]) for d in FAM]
         self.off = np.concatenate([[0], np.cumsum(self.cnt)])
         self.ne = int(self.off[-1])
+        self.fcnt = [(nx + 1 - FAM[f2][0]) * (ny + 1 - FAM[f2][1]) * (nz + 1 - FAM[f2][2]) for _, f2 in FFAM]
+        self.foff = np.concatenate([[0], np.cumsum(self.fcnt)])
+        self.nf = int(self.foff[-1])
         self.nt = 6 * nx * ny * nz
+        self.nl = 6 if order == 1 else 20
+        self.ndof = self.ne if order == 1 else 2 * self.ne + 2 * self.nf
 
     def fam_of(self, d):
         for f in range(7):
@@ -97,6 +149,53 @@ class Mesh:
         cx = self.n[0] + 1 - FAM[f][0]
         cy = self.n[1] + 1 - FAM[f][1]
         return self.off[f] + I + cx * (J + cy * K)
+
+    def face(self, q, I, J, K):
+        d = FAM[FFAM[q][1]]
+        cx = self.n[0] + 1 - d[0]
+        cy = self.n[1] + 1 - d[1]
+        return self.foff[q] + I + cx * (J + cy * K)
+
+    def tet_faces(self):
+        """[nt][4] global faces in local face order."""
+        nx, ny, nz = self.n
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        i, j, k = i.ravel(), j.ravel(), k.ravel()
+        out = np.zeros((nx * ny * nz, 6, 4), dtype=np.int64)
+        for t in range(6):
+            v = tet_vertices(t)
+            for q, (a, b, c) in enumerate(LOC_TRI):
+                ff = FFAM.index((self.fam_of(v[b] - v[a]), self.fam_of(v[c] - v[a])))
+                out[:, t, q] = self.face(ff, i + v[a][0], j + v[a][1], k + v[a][2])
+        return out.reshape(-1, 4)
+
+    def tet_dofs(self):
+        """[nt][nl] global dofs in the local basis order."""
+        E = self.tet_edges()
+        if self.order == 1:
+            return E
+        F = self.tet_faces()
+        Fd = np.stack([2 * self.ne + 2 * F, 2 * self.ne + 2 * F + 1], axis=2).reshape(-1, 8)
+        return np.hstack([E, self.ne + E, Fd])
+
+    def dof_boundary(self):
+        be = self.boundary()
+        if self.order == 1:
+            return be
+        # a face is on the boundary when its three vertices lie in one outer plane
+        nx, ny, nz = self.n
+        bf = np.zeros(self.nf, dtype=bool)
+        for q, (f1, f2) in enumerate(FFAM):
+            d1, d2 = FAM[f1], FAM[f2]
+            cx, cy, cz = self.n[0] + 1 - d2[0], self.n[1] + 1 - d2[1], self.n[2] + 1 - d2[2]
+            K, J, I = np.meshgrid(np.arange(cz), np.arange(cy), np.arange(cx), indexing="ij")
+            P = np.stack([I.ravel(), J.ravel(), K.ravel()], axis=1)
+            flag = np.zeros(P.shape[0], dtype=bool)
+            for a in range(3):
+                if d1[a] == 0 and d2[a] == 0:
+                    flag |= (P[:, a] == 0) | (P[:, a] == self.n[a])
+            bf[self.foff[q]:self.foff[q + 1]] = flag
+        return np.concatenate([be, be, np.repeat(bf, 2)])
 
     def tet_edges(self):
         """[nt][6] global edges, tetrahedron id 6 (i + nx (j + ny k)) + t."""
@@ -163,34 +262,34 @@ class Mesh:
         raise RuntimeError("no tetrahedron contains the point")
 
     def basis_at(self, x):
-        """(global edges [6], values [6][3]) of the Whitney functions at point x."""
+        """(global dofs [nl], values [nl][3]) of the basis functions at point x."""
         c, t, lam = self.locate(x)
         X = (tet_vertices(t) + c) * self.h
         G, _ = bary_grad(X)
         i, j, k = c
         tid = 6 * (i + self.n[0] * (j + self.n[1] * k)) + t
-        e = self.tet_edges()[tid]
-        W = np.array([lam[a] * G[b] - lam[b] * G[a] for a, b in LOC_EDGE])
+        e = self.tet_dofs()[tid]
+        W, _ = basis_values(lam, G, self.order)
         return e, W
 
 
 def assemble(mesh, eps, mu, eps_r=None, sigma=None):
     """S, M_eps, M_sig (csr) with PEC: boundary rows/columns removed, identity
     rows in the mass-type matrices, zero rows in S."""
-    E = mesh.tet_edges()
-    nt = E.shape[0]
+    E = mesh.tet_dofs()
+    nt, nl = E.shape
     er = np.ones(nt) if eps_r is None else np.asarray(eps_r, dtype=float)
     sg = np.zeros(nt) if sigma is None else np.asarray(sigma, dtype=float)
-    Ms_loc = [element_matrices(t, mesh.h) for t in range(6)]
+    Ms_loc = [element_matrices(t, mesh.h, mesh.order) for t in range(6)]
     typ = np.arange(nt) % 6
-    Ml = np.stack([Ms_loc[t][0] for t in typ])  # [nt][6][6]
+    Ml = np.stack([Ms_loc[t][0] for t in typ])  # [nt][nl][nl]
     Sl = np.stack([Ms_loc[t][1] for t in typ])
-    rows = np.repeat(E, 6, axis=1).ravel()
-    cols = np.tile(E, (1, 6)).ravel()
-    bnd = mesh.boundary()
+    rows = np.repeat(E, nl, axis=1).ravel()
+    cols = np.tile(E, (1, nl)).ravel()
+    bnd = mesh.dof_boundary()
     keep = ~bnd[rows] & ~bnd[cols]
     r, c = rows[keep], cols[keep]
-    n = mesh.ne
+    n = mesh.ndof
 
     def build(vals, diag):
         A = sp.coo_matrix((vals.ravel()[keep], (r, c)), shape=(n, n)).tocsr()
@@ -209,12 +308,12 @@ def pulse_rate(t, f0, t0, amp):
 
 
 class OracleTet:
-    def __init__(self, grid, h, dt, eps, mu, tf=1, stepper=SYMPLECTIC2, eps_r=None, sigma=None):
-        self.mesh = Mesh(*grid, h)
+    def __init__(self, grid, h, dt, eps, mu, tf=1, stepper=SYMPLECTIC2, eps_r=None, sigma=None, order=1):
+        self.mesh = Mesh(*grid, h, order=order)
         self.dt, self.tf, self.stepper = float(dt), int(tf), int(stepper)
         self.S, self.Me, self.Ms = assemble(self.mesh, eps, mu, eps_r, sigma)
-        self.bnd = self.mesh.boundary()
-        n = self.mesh.ne
+        self.bnd = self.mesh.dof_boundary()
+        n = self.mesh.ndof
         self.U = np.zeros(n)
         self.V = np.zeros(n)
         self.t = 0.0
@@ -235,7 +334,7 @@ class OracleTet:
         self.recs.append(self.mesh.basis_at(pos))
 
     def load(self, t):
-        g = np.zeros(self.mesh.ne)
+        g = np.zeros(self.mesh.ndof)
         for e, w, f0, t0, amp in self.dips:
             np.add.at(g, e, pulse_rate(t, f0, t0, amp) * w)
         return g

@@ -111,7 +111,7 @@ def test_fem1_invalid_descriptor_rejected():
 def test_tet_invalid_descriptor_rejected():
     from emsolvers_amd import _lib
     from emsolvers_amd.tet import TetSimulation
-    bad = [dict(grid=(0, 4, 4)), dict(h=0.0), dict(dt=-1.0), dict(tf=0), dict(stepper=7), dict(order=2),
+    bad = [dict(grid=(0, 4, 4)), dict(h=0.0), dict(dt=-1.0), dict(tf=0), dict(stepper=7), dict(order=3),
            dict(precision="f16")]
     for b in bad:
         kw = dict(grid=(4, 4, 4), h=0.25, dt=0.01, eps=1.0, mu=1.0, tf=1, stepper="symplectic_2", order=1)

@@ -5,6 +5,7 @@ fields and the convergence orders of the steppers on a discrete eigenmode."""
 import numpy as np
 import pytest
 import scipy.linalg as sla
+import scipy.sparse as sp
 
 from oracle import tet as ot
 
@@ -183,3 +184,138 @@ def test_dipole_load_is_point_evaluation():
     U = D @ E0
     t = 0.37
     assert o.load(t) @ U == pytest.approx(ot.pulse_rate(t, 2.0, 0.5, 1.5) * (E0 @ d), rel=1e-12)
+
+
+# ------------------------------------------------------------------ order 2
+def test_order2_counts_match_the_64_cube_workload():
+    m = ot.Mesh(64, 64, 64, 1 / 64, order=2)
+    assert m.ndof == 10084736 and m.nt == 1572864
+    m1 = ot.Mesh(40, 40, 40, 1 / 40)
+    assert m1.ne == 462520 and m1.nt == 384000
+
+
+def test_order2_face_numbering():
+    m = ot.Mesh(2, 3, 2, 0.5, order=2)
+    F = m.tet_faces()
+    assert F.min() == 0 and F.max() == m.nf - 1
+    # every interior face is shared by exactly two tetrahedra, boundary faces by one
+    cnt = np.bincount(F.ravel(), minlength=m.nf)
+    bf = m.dof_boundary()[2 * m.ne::2]
+    assert np.all(cnt[bf] == 1) and np.all(cnt[~bf] == 2)
+    # Euler: V - E + F - T = 1 for a ball
+    nv = np.prod(m.n + 1)
+    assert nv - m.ne + m.nf - m.nt == 1
+
+
+@pytest.mark.parametrize("t", range(6))
+def test_order2_element_matrices(t):
+    M, S = ot.element_matrices(t, 0.3, order=2)
+    assert np.allclose(M, M.T, rtol=0, atol=1e-15) and np.allclose(S, S.T, rtol=0, atol=1e-13)
+    assert np.all(np.linalg.eigvalsh(M) > 0)
+    ev = np.linalg.eigvalsh(S)
+    assert np.sum(ev > 1e-9 * ev.max()) == 11  # 20 - dim grad P2 (9)
+    # the leading 6 x 6 block is the Whitney element
+    M1, S1 = ot.element_matrices(t, 0.3, order=1)
+    np.testing.assert_allclose(M[:6, :6], M1, rtol=1e-13)
+    np.testing.assert_allclose(S[:6, :6], S1, rtol=1e-13)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_library_element_matrices_equal_quadrature(order):
+    """The library's exact monomial integration against the oracle's
+    quadrature of the hand-written basis and curls (host-only call)."""
+    from emsolvers_amd.tet import element_matrices as lib_em
+    for t in range(6):
+        for h in (1.0, 0.013):
+            Ml, Sl = lib_em(order, t, h)
+            Mo, So = ot.element_matrices(t, h, order)
+            np.testing.assert_allclose(Ml, Mo, rtol=1e-12, atol=1e-14 * np.abs(Mo).max())
+            np.testing.assert_allclose(Sl, So, rtol=1e-12, atol=1e-12 * np.abs(So).max())
+
+
+def test_order2_null_space():
+    """ker S (PEC) = grad of P2 Lagrange functions vanishing on the boundary:
+    dimension = interior vertices + interior edges."""
+    n = 2
+    m = ot.Mesh(n, n, n, 1.0 / n, order=2)
+    S, Me, _ = ot.assemble(m, 1.0, 1.0)
+    keep = ~m.dof_boundary()
+    Sk = S[keep][:, keep].toarray()
+    Mk = Me[keep][:, keep].toarray()
+    w = sla.eigh(Sk, Mk, eigvals_only=True)
+    nz = np.sum(np.abs(w) < 1e-8 * w.max())
+    interior_edges = np.sum(~m.boundary())
+    assert nz == (n - 1) ** 3 + interior_edges
+
+
+@pytest.mark.slow
+def test_order2_cavity_fourth_order():
+    errs = []
+    for n in (2, 3):
+        m = ot.Mesh(n, n, n, 1.0 / n, order=2)
+        S, Me, _ = ot.assemble(m, 1.0, 1.0)
+        keep = ~m.dof_boundary()
+        w = sla.eigh(S[keep][:, keep].toarray(), Me[keep][:, keep].toarray(), eigvals_only=True)
+        w = np.sort(w[w > 1e-8 * w.max()])
+        lam110 = 2 * np.pi ** 2
+        errs.append(np.abs(w[:3] / lam110 - 1).max())
+        assert np.all(np.abs(w[3:5] / (3 * np.pi ** 2) - 1) < 0.1)
+    rate = np.log(errs[0] / errs[1]) / np.log(3 / 2)
+    assert rate > 3.5, (errs, rate)
+    assert errs[1] < 5e-3
+
+
+def test_order2_projection_reproduces_linear_fields():
+    """L2 projection (full system, no PEC) of E(x) = A x + c onto the order-2
+    space is exact; the receivers' evaluation then returns E(r)."""
+    m = ot.Mesh(2, 2, 1, 0.5, order=2)
+    Edofs = m.tet_dofs()
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((3, 3))
+    c0 = rng.standard_normal(3)
+    field = lambda x: A @ x + c0
+    M = np.zeros((m.ndof, m.ndof))
+    b = np.zeros(m.ndof)
+    for tid, e in enumerate(Edofs):
+        t = tid % 6
+        cube = tid // 6
+        ci = np.array([cube % m.n[0], (cube // m.n[0]) % m.n[1], cube // (m.n[0] * m.n[1])])
+        X = (ot.tet_vertices(t) + ci) * m.h
+        G, V = ot.bary_grad(X)
+        for lam, wq in zip(ot.QPTS, ot.QW):
+            W, _ = ot.basis_values(lam, G, 2)
+            x = lam @ X
+            M[np.ix_(e, e)] += wq * V * (W @ W.T)
+            b[e] += wq * V * (W @ field(x))
+    u = np.linalg.solve(M, b)
+    for _ in range(10):
+        x = rng.uniform(0.05, 0.95, 3) * m.n * m.h
+        dofs, W = m.basis_at(x)
+        np.testing.assert_allclose(u[dofs] @ W, field(x), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("order,grid", [(1, (3, 2, 4)), (2, (2, 3, 2)), (2, (3, 3, 3))])
+def test_library_assembly_equals_oracle(order, grid):
+    """The library's row-wise assembly (host-only call) against the oracle's
+    scipy assembly, with per-tetrahedron eps_r and sigma."""
+    from emsolvers_amd.tet import assemble as lib_assemble
+    rng = np.random.default_rng(2)
+    nt = 6 * int(np.prod(grid))
+    er, sg = rng.uniform(1, 4, nt), rng.uniform(0, 0.5, nt)
+    h, eps, mu = 0.31, 2.0, 0.7
+    m = ot.Mesh(*grid, h, order=order)
+    So, Mo, Go = ot.assemble(m, eps, mu, er, sg)
+    Sl, Ml, Gl = lib_assemble(grid, h, eps, mu, order, er, sg)
+    for A, B in ((Sl, So), (Ml, Mo), (Gl, Go)):
+        D = (A - B).toarray()
+        assert np.abs(D).max() <= 1e-12 * np.abs(B.toarray()).max()
+    # the pattern: every nonzero of the oracle is a stored entry of the library
+    Mo.eliminate_zeros()
+    pat_l = sp.csr_matrix((np.ones(Ml.nnz), Ml.indices, Ml.indptr), shape=Ml.shape)
+    pat_o = (abs(Mo) > 0).astype(float)
+    assert (pat_o - pat_o.multiply(pat_l)).nnz == 0
+    assert np.all(np.diff(Ml.indptr) >= 1)
+    # columns sorted within each row
+    for r in range(0, Ml.shape[0], 7):
+        c = Ml.indices[Ml.indptr[r]:Ml.indptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
