@@ -34,3 +34,28 @@ class FdtdMisfit(torch.autograd.Function):
 
 def fdtd_misfit(eps_r, sigma, sim, num_steps, d_obs):
     return FdtdMisfit.apply(eps_r, sigma, sim, num_steps, d_obs)
+
+
+class TetMisfit(torch.autograd.Function):
+    """J(eps_r) of a tetrahedral run (per-tetrahedron eps_r; lossless
+    symplectic model), backward = the discrete adjoint of ems_tet_gradient."""
+
+    @staticmethod
+    def forward(ctx, eps_r, sim, num_steps, d_obs):
+        er = eps_r.detach().to("cpu", torch.float64).numpy().ravel()
+        sim.set_material(er, None)
+        d = d_obs.detach().to("cpu", torch.float64).numpy() if torch.is_tensor(d_obs) else np.asarray(d_obs)
+        J, ge, _, _ = sim.gradient(int(num_steps), d)
+        ctx.save_for_backward(torch.from_numpy(ge).to(eps_r.device, eps_r.dtype).reshape(eps_r.shape))
+        return torch.tensor(J, dtype=eps_r.dtype, device=eps_r.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (ge,) = ctx.saved_tensors
+        return grad_out * ge, None, None, None
+
+
+def tet_misfit(eps_r, sim, num_steps, d_obs):
+    """``sim``: an ``emsolvers_amd.tet.TetSimulation`` that has not run yet
+    (the material is set before each evaluation)."""
+    return TetMisfit.apply(eps_r, sim, num_steps, d_obs)

@@ -36,6 +36,12 @@ inline ems_status invalid(const std::string &msg)
     return EMS_ERR_INVALID_ARG;
 }
 
+inline ems_status not_supported(const std::string &msg)
+{
+    set_error(msg);
+    return EMS_ERR_NOT_SUPPORTED;
+}
+
 // Grow-only device buffer.
 struct DevBuf {
     void *p = nullptr;

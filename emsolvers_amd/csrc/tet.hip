@@ -325,6 +325,57 @@ __global__ void k_comb(int n, const T *__restrict__ c0, T k, T a1, const T *__re
     y[i] = c0[i] + k * s;
 }
 
+// Misfit gradient per tetrahedron (one thread each, fixed order):
+//   ge[T] += eps0 mu_T' M_T (Vin - Vout)_T,  gs[T] += -D/2 mu_T' M_T (Vin + Vout)_T
+// over the interior dofs of T (boundary dofs are masked).
+template <typename T>
+__global__ void k_tet_grad(long nt, int nl, const int *__restrict__ t2d, const double *__restrict__ mloc,
+                           const unsigned char *__restrict__ bnd, const T *__restrict__ mu,
+                           const T *__restrict__ vin, const T *__restrict__ vout, double eps0, double mhalf,
+                           double *__restrict__ ge, double *__restrict__ gs)
+{
+    const long tt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= nt) return;
+    const int *d = t2d + tt * nl;
+    const double *M = mloc + (tt % 6) * nl * nl;
+    double m[20], dm[20], sm[20];
+    for (int l = 0; l < nl; ++l) {
+        const int q = d[l];
+        const bool in = !bnd[q];
+        m[l] = in ? (double)mu[q] : 0.0;
+        dm[l] = in ? (double)vin[q] - (double)vout[q] : 0.0;
+        sm[l] = in ? (double)vin[q] + (double)vout[q] : 0.0;
+    }
+    double a = 0.0, b = 0.0;
+    for (int l = 0; l < nl; ++l) {
+        double ra = 0.0, rb = 0.0;
+        for (int k = 0; k < nl; ++k) {
+            ra += M[l * nl + k] * dm[k];
+            rb += M[l * nl + k] * sm[k];
+        }
+        a += m[l] * ra;
+        b += m[l] * rb;
+    }
+    ge[tt] += eps0 * a;
+    gs[tt] += mhalf * b;
+}
+
+// lU[dof] += sum_c w_c res_c over the receivers' basis functions (one thread,
+// receiver order)
+template <typename T>
+__global__ void k_tet_inject(int nobs, int nl, const int *__restrict__ dof, const double *__restrict__ w,
+                             const double *__restrict__ res, T *lu)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    for (int o = 0; o < nobs; ++o)
+        for (int l = 0; l < nl; ++l) {
+            const int q = dof[nl * o + l];
+            const double *wl = w + (nl * o + l) * 3;
+            const double v = (wl[0] * res[3 * o] + wl[1] * res[3 * o + 1]) + wl[2] * res[3 * o + 2];
+            lu[q] = lu[q] + (T)v;
+        }
+}
+
 // receivers: E(r) = sum_l U[dof_l] w_l(r) over the nl basis functions of the
 // enclosing tetrahedron, 3 components, into rec[obs][3]
 template <typename T>
@@ -375,6 +426,8 @@ struct ems_tet {
     bool abm_started = false;
     int cg_last = 0, cg_max = 0;
     hipStream_t stream = nullptr;
+    // misfit gradient (allocated on first use)
+    DevBuf g_t2d, g_mloc, g_bnd, g_lU, g_lV, g_mu, g_y, g_vout, g_tmp, g_ge, g_gs, g_res, g_f;
 };
 
 namespace {
@@ -1015,6 +1068,165 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
     return EMS_OK;
 }
 
+// Misfit gradient of a run from rest at t = 0 (symplectic steppers, lossless
+// model): the forward run records the receivers; the backward pass recovers
+// each stage's input state by running the stage backwards,
+//   U1 = U' - D/2 V';  V = V' + D M_eps^-1 (S U1 + g(t_s));  U = U1 - D/2 V,
+// and applies the discrete adjoint of the stage,
+//   mu = M_eps^-1 (lV' + D/2 lU');  lU = lU' - D S mu;  lV = M_eps mu + D/2 lU,
+// accumulating per tetrahedron eps0 mu' M_T (V - V') (eps_r) and
+// -D/2 mu' M_T (V + V') (sigma, at sigma = 0).  `drift` = the recovered
+// initial state's largest entry relative to the final state's.
+template <typename T>
+ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, double *ge, double *gs, double *drift)
+{
+    if (s->d.stepper == EMS_TET_ABM) return not_supported("the tetrahedral gradient needs a symplectic stepper");
+    for (double v : s->sigma)
+        if (v != 0.0) return not_supported("the tetrahedral gradient runs the integrator backwards: lossless models only");
+    EMS_TRY(setup<T>(s));
+    const long nd = s->nd, nt = s->m.nt;
+    const int nl = s->nl, nobs = (int)s->obs_dof.size() / nl, tf = s->d.tf;
+    const int n = (int)nd, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    // tables: tet -> dofs, element masses, boundary mask
+    if (!s->g_t2d.p) {
+        std::vector<int> t2d((size_t)nt * nl);
+        long tid = 0, dd[20];
+        for (int k = 0; k < s->m.nz; ++k)
+            for (int j = 0; j < s->m.ny; ++j)
+                for (int i = 0; i < s->m.nx; ++i)
+                    for (int t = 0; t < 6; ++t, ++tid) {
+                        tet_dofs(s->m, s->d.order, i, j, k, t, dd);
+                        for (int l = 0; l < nl; ++l) t2d[tid * nl + l] = (int)dd[l];
+                    }
+        std::vector<double> mloc(6 * nl * nl), sl(nl * nl);
+        for (int t = 0; t < 6; ++t) element_matrices(s->d.order, t, s->m.h, &mloc[t * nl * nl], sl.data());
+        std::vector<unsigned char> bnd(s->bnd.begin(), s->bnd.end());
+        EMS_TRY(s->g_t2d.reserve(sizeof(int) * t2d.size()));
+        EMS_TRY(s->g_mloc.reserve(sizeof(double) * mloc.size()));
+        EMS_TRY(s->g_bnd.reserve(bnd.size()));
+        EMS_HIP(hipMemcpy(s->g_t2d.p, t2d.data(), sizeof(int) * t2d.size(), hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->g_mloc.p, mloc.data(), sizeof(double) * mloc.size(), hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(s->g_bnd.p, bnd.data(), bnd.size(), hipMemcpyHostToDevice));
+    }
+    const size_t vb = sizeof(T) * nd;
+    for (DevBuf *v : {&s->g_lU, &s->g_lV, &s->g_mu, &s->g_y, &s->g_vout, &s->g_tmp}) {
+        EMS_TRY(v->reserve(vb));
+        EMS_HIP(hipMemset(v->p, 0, vb));
+    }
+    EMS_TRY(s->g_ge.reserve(sizeof(double) * nt));
+    EMS_TRY(s->g_gs.reserve(sizeof(double) * nt));
+    EMS_HIP(hipMemset(s->g_ge.p, 0, sizeof(double) * nt));
+    EMS_HIP(hipMemset(s->g_gs.p, 0, sizeof(double) * nt));
+    // forward from rest
+    EMS_HIP(hipMemset(s->U.p, 0, vb));
+    EMS_HIP(hipMemset(s->V.p, 0, vb));
+    s->t_now = 0.0;
+    s->steps = 0;
+    s->nslot = 0;
+    s->traces.assign(0, 0.0);
+    EMS_TRY(run_impl<T>(s, N, nullptr));
+    const int nslot = N / tf + 1;
+    std::vector<double> res((size_t)nslot * nobs * 3, 0.0);  // [slot][obs][3]
+    double jj = 0.0;
+    for (int o = 0; o < nobs; ++o)
+        for (int q = 1; q < nslot; ++q)
+            for (int c = 0; c < 3; ++c) {
+                const size_t k = ((size_t)o * nslot + q) * 3 + c;
+                const double r = s->traces[k] - d_obs[k];
+                res[((size_t)q * nobs + o) * 3 + c] = r;
+                jj += 0.5 * r * r;
+            }
+    *J = jj;
+    EMS_TRY(s->g_res.reserve(sizeof(double) * std::max<size_t>(res.size(), 1)));
+    EMS_HIP(hipMemcpy(s->g_res.p, res.data(), sizeof(double) * res.size(), hipMemcpyHostToDevice));
+    // stage schedule and the forward step times (same accumulation as run)
+    std::vector<double> Dq, Oq;
+    const double dt = s->d.dt;
+    if (s->d.stepper == EMS_TET_SYMPLECTIC4) {
+        const double x1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0)), x2 = 1.0 - 2.0 * x1;
+        Dq = {x1 * dt, x2 * dt, x1 * dt};
+        Oq = {0.0, x1 * dt, (x1 + x2) * dt};
+    } else {
+        Dq = {dt};
+        Oq = {0.0};
+    }
+    const int ns = (int)Dq.size(), ndip = (int)s->dip_par.size() / 3;
+    std::vector<double> tstep(N + 1, 0.0);
+    for (int k = 1; k <= N; ++k) tstep[k] = tstep[k - 1] + dt;
+    // load factors +p'(t_s) of the backward stages, in backward order
+    std::vector<double> f;
+    for (int st = N; st >= 1; --st)
+        for (int q = ns - 1; q >= 0; --q) dipole_factors(s, tstep[st - 1] + Oq[q] + 0.5 * Dq[q], 1.0, f);
+    if (f.empty()) f.push_back(0.0);
+    EMS_TRY(s->g_f.reserve(sizeof(double) * f.size()));
+    EMS_HIP(hipMemcpy(s->g_f.p, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+    // scale of the final state for the drift
+    auto inf_norm = [&](DevBuf &b) -> double {
+        std::vector<T> h(nd);
+        if (hipMemcpy(h.data(), b.p, vb, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
+        double mx = 0.0;
+        for (T v : h) mx = std::max(mx, std::fabs((double)v));
+        return mx;
+    };
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    const double u_end = inf_norm(s->U), v_end = inf_norm(s->V);
+    T *U = s->U.as<T>(), *V = s->V.as<T>(), *lU = s->g_lU.as<T>(), *lV = s->g_lV.as<T>();
+    T *tmp = s->g_tmp.as<T>(), *vout = s->g_vout.as<T>();
+    const T *Sv = s->d_S.as<T>(), *Mv = s->d_Me.as<T>(), *Md = s->d_dinv_me.as<T>();
+    size_t fo = 0;
+    const int tb = (int)((nt + 127) / 128);
+    for (int st = N; st >= 1; --st) {
+        if (nobs && st % tf == 0)
+            k_tet_inject<T><<<1, 64, 0, s->stream>>>(nobs, nl, s->d_obs_dof.as<int>(), s->d_obs_w.as<double>(),
+                                                     s->g_res.as<double>() + (size_t)(st / tf) * nobs * 3, lU);
+        for (int q = ns - 1; q >= 0; --q, ++fo) {
+            const double D = Dq[q];
+            // recover the stage input
+            EMS_HIP(hipMemcpyAsync(vout, V, vb, hipMemcpyDeviceToDevice, s->stream));
+            k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);  // U1
+            with_lpr(s->lpr, [&](auto L) {
+                constexpr int LPR = decltype(L)::value;
+                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Sv, U, s->b.as<T>());
+            });
+            load<T>(s, s->b, s->g_f.as<double>() + fo * (size_t)std::max(ndip, 1));
+            EMS_TRY(cg<T>(s, Mv, Md, s->g_y));
+            k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)D, s->g_y.as<T>(), V);        // V_in
+            k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);            // U_in
+            // adjoint of the stage
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, lV, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr,
+                                                s->b.as<T>());
+            EMS_TRY(cg<T>(s, Mv, Md, s->g_mu));
+            k_tet_grad<T><<<tb, 128, 0, s->stream>>>(nt, nl, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
+                                                     s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
+                                                     s->d.eps, -0.5 * D, s->g_ge.as<double>(),
+                                                     s->g_gs.as<double>());
+            with_lpr(s->lpr, [&](auto L) {
+                constexpr int LPR = decltype(L)::value;
+                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Sv, s->g_mu.as<T>(), tmp);
+            });
+            k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-D), tmp, lU);  // lU1
+            with_lpr(s->lpr, [&](auto L) {
+                constexpr int LPR = decltype(L)::value;
+                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Mv, s->g_mu.as<T>(), tmp);
+            });
+            k_comb<T><<<g, 256, 0, s->stream>>>(n, tmp, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr, lV);
+            EMS_HIP(hipGetLastError());
+        }
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    const double u0 = inf_norm(s->U), v0 = inf_norm(s->V);
+    if (drift) *drift = std::max(u_end > 0 ? u0 / u_end : u0, v_end > 0 ? v0 / v_end : v0);
+    if (ge) EMS_HIP(hipMemcpy(ge, s->g_ge.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    if (gs) EMS_HIP(hipMemcpy(gs, s->g_gs.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    // the handle returns to rest at t = 0 and keeps the run's traces
+    EMS_HIP(hipMemset(s->U.p, 0, vb));
+    EMS_HIP(hipMemset(s->V.p, 0, vb));
+    s->t_now = 0.0;
+    s->steps = 0;
+    return EMS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1053,7 +1265,7 @@ ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *s
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (s->assembled) return invalid("set the material before the first run");
+    if (s->steps > 0) return invalid("the material can only change at rest (before the first step)");
     const size_t nt = (size_t)s->m.nt;
     for (size_t q = 0; eps_r && q < nt; ++q)
         if (!(eps_r[q] > 0)) return invalid("eps_r must be positive");
@@ -1063,6 +1275,7 @@ ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *s
     else s->eps_r.clear();
     if (sigma) s->sigma.assign(sigma, sigma + nt);
     else s->sigma.clear();
+    s->assembled = false;  // reassembled by the next run (the state is at rest)
     return EMS_OK;
 }
 
@@ -1153,6 +1366,18 @@ ems_status ems_tet_set_time_step(ems_tet *s, double dt)
         (void)back;
     }
     return st;
+}
+
+ems_status ems_tet_gradient(ems_tet *s, int num_steps, const double *d_obs, size_t count, double *misfit,
+                            double *grad_eps_r, double *grad_sigma, double *drift)
+{
+    clear_error();
+    if (!s || !d_obs || !misfit) return invalid("null argument");
+    if (num_steps < 1) return invalid("num_steps must be >= 1");
+    const int nobs = (int)s->obs_dof.size() / s->nl;
+    if (count != (size_t)nobs * (num_steps / s->d.tf + 1) * 3) return invalid("d_obs has the wrong size");
+    return s->esz == 4 ? gradient_impl<float>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift)
+                       : gradient_impl<double>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift);
 }
 
 ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot)

@@ -121,6 +121,17 @@ class TetSimulation:
         _lib.check(_lib.load().ems_tet_run_timed(self._h, int(num_steps), ctypes.byref(ms)))
         return ms.value
 
+    def gradient(self, num_steps, d_obs):
+        """(J, dJ/deps_r, dJ/dsigma, drift) of a run from rest (symplectic
+        steppers, lossless model); d_obs [nobs][num_steps/tf + 1][3]."""
+        d = np.ascontiguousarray(np.asarray(d_obs, dtype=np.float64))
+        J, dr = ctypes.c_double(), ctypes.c_double()
+        ge = np.zeros(self.num_tets)
+        gs = np.zeros(self.num_tets)
+        _lib.check(_lib.load().ems_tet_gradient(self._h, int(num_steps), _lib.dptr(d), d.size, ctypes.byref(J),
+                                                _lib.dptr(ge), _lib.dptr(gs), ctypes.byref(dr)))
+        return J.value, ge, gs, dr.value
+
     def traces(self):
         """[nobs][nslot][3] receiver samples, slot q at step q tf."""
         no, ns = ctypes.c_int(), ctypes.c_int()

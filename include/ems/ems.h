@@ -392,8 +392,9 @@ ems_status ems_tet_create(const ems_tet_desc *desc, ems_tet **out);
 void ems_tet_destroy(ems_tet *s);
 
 /* Relative permittivity and conductivity per tetrahedron (6 nx ny nz
- * values, tetrahedron 6 (i + nx (j + ny k)) + t); NULL = 1 / 0.  Before the
- * first run. */
+ * values, tetrahedron 6 (i + nx (j + ny k)) + t); NULL = 1 / 0.  At rest
+ * only (before the first step, or after ems_tet_gradient); the next run
+ * reassembles. */
 ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *sigma);
 
 /* Electric dipole of moment p(t) d at pos, p(t) = amp exp(-(pi f0 (t - t0))^2):
@@ -410,6 +411,19 @@ ems_status ems_tet_set_time_step(ems_tet *s, double dt);
 
 ems_status ems_tet_run(ems_tet *s, int num_steps);
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);
+
+/* Misfit gradient (symplectic steppers, lossless models): runs num_steps
+ * steps from rest at t = 0 with the handle's dipoles and receivers and
+ * returns J = 1/2 sum over slots 1..num_steps/tf of (trace - d_obs)^2
+ * (d_obs [nobs][num_steps/tf + 1][3], slot 0 unused) and dJ/deps_r,
+ * dJ/dsigma per tetrahedron (either may be NULL; dsigma at sigma = 0), FP64.
+ * The forward states are recovered by running the integrator backwards (no
+ * checkpoints); *drift (may be NULL) = the largest entry of the recovered
+ * initial state relative to the final state's.  Afterwards the handle is at
+ * rest at t = 0 and holds the run's traces.  No reference counterpart
+ * (extends run() of src/symplectic_4/fem_simulation.cpp:1278-1359). */
+ems_status ems_tet_gradient(ems_tet *s, int num_steps, const double *d_obs, size_t count, double *misfit,
+                            double *grad_eps_r, double *grad_sigma, double *drift);
 
 /* Receiver traces [nobs][nslot][3], slot q = step q tf (slot 0 = t 0). */
 ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot);

@@ -312,6 +312,7 @@ class OracleTet:
         self.mesh = Mesh(*grid, h, order=order)
         self.dt, self.tf, self.stepper = float(dt), int(tf), int(stepper)
         self.S, self.Me, self.Ms = assemble(self.mesh, eps, mu, eps_r, sigma)
+        self.Me_eps0 = float(eps)
         self.bnd = self.mesh.dof_boundary()
         n = self.mesh.ndof
         self.U = np.zeros(n)
@@ -400,3 +401,78 @@ class OracleTet:
         else:
             self.V = np.array(values, dtype=float)
         self._hist = None
+
+
+# ------------------------------------------------------------------ adjoint
+def stage_schedule(stepper, dt):
+    """(D, offset) of the symplectic stages of one step."""
+    if stepper == SYMPLECTIC2:
+        return [(dt, 0.0)]
+    x1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    x2 = 1.0 - 2.0 * x1
+    return [(x1 * dt, 0.0), (x2 * dt, x1 * dt), (x1 * dt, (x1 + x2) * dt)]
+
+
+def misfit_gradient(o, nsteps, d_obs):
+    """Misfit J = 1/2 sum over slots 1..nsteps/tf of |trace - d_obs|^2 of a
+    run from rest at t = 0, and dJ/deps_r, dJ/dsigma per tetrahedron, by the
+    discrete adjoint of the symplectic stages with stored forward states:
+        stage  U1 = U + D/2 V;  A V' = B V - D (S U1 + g);  U' = U1 + D/2 V'
+               (A = Me + D/2 Ms, B = Me - D/2 Ms)
+        adjoint mu = A^-1 (lV' + D/2 lU');  lU = lU' - D S mu;  lV = B mu + D/2 lU
+        dJ/deps_r(T) += eps0 mu_T' M_T (V - V')_T,
+        dJ/dsigma(T) += -D/2 mu_T' M_T (V + V')_T   (interior dofs only)."""
+    if o.stepper == ABM:
+        raise ValueError("the adjoint covers the symplectic steppers")
+    m = o.mesh
+    n = m.ndof
+    o.U = np.zeros(n)
+    o.V = np.zeros(n)
+    o.t = 0.0
+    sched = stage_schedule(o.stepper, o.dt)
+    states = []  # per stage: (U_in, V_in, V_out, D)
+    traces = [o.receive()]
+    for s in range(1, nsteps + 1):
+        t = o.t
+        for D, off in sched:
+            U_in, V_in = o.U.copy(), o.V.copy()
+            o._stage(D, t + off + 0.5 * D)
+            states.append((U_in, V_in, o.V.copy(), D))
+        o.t = t + o.dt
+        if s % o.tf == 0:
+            traces.append(o.receive())
+    tr = np.stack(traces, axis=1)
+    res = tr - d_obs
+    res[:, 0, :] = 0.0
+    J = 0.5 * float(np.sum(res[:, 1:, :] ** 2))
+    E = m.tet_dofs()
+    typ = np.arange(m.nt) % 6
+    Mloc = [element_matrices(t, m.h, m.order)[0] for t in range(6)]
+    inner = (~o.bnd).astype(float)
+    eps0 = o.Me_eps0
+    ge = np.zeros(m.nt)
+    gs = np.zeros(m.nt)
+    lU = np.zeros(n)
+    lV = np.zeros(n)
+    q = len(states)
+    for s in range(nsteps, 0, -1):
+        if s % o.tf == 0:
+            r = res[:, s // o.tf, :]
+            for (dofs, W), rr in zip(o.recs, r):
+                np.add.at(lU, dofs, W @ rr)
+        for _ in sched:
+            q -= 1
+            U_in, V_in, V_out, D = states[q]
+            mu = o._solve(D, lV + 0.5 * D * lU)
+            mu_i = mu * inner
+            dm = (V_in - V_out) * inner
+            sm = (V_in + V_out) * inner
+            for t in range(6):
+                sel = typ == t
+                Mu = mu_i[E[sel]]
+                ge[sel] += eps0 * np.einsum("ti,ij,tj->t", Mu, Mloc[t], dm[E[sel]])
+                gs[sel] += -0.5 * D * np.einsum("ti,ij,tj->t", Mu, Mloc[t], sm[E[sel]])
+            lU1 = lU - D * (o.S @ mu)
+            lV = (o.Me @ mu - 0.5 * D * (o.Ms @ mu)) + 0.5 * D * lU1
+            lU = lU1
+    return J, ge, gs, tr

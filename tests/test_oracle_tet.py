@@ -319,3 +319,40 @@ def test_library_assembly_equals_oracle(order, grid):
     for r in range(0, Ml.shape[0], 7):
         c = Ml.indices[Ml.indptr[r]:Ml.indptr[r + 1]]
         assert np.all(np.diff(c) > 0)
+
+
+# ------------------------------------------------------------------ adjoint
+def _grad_case(order, stepper, lossy, n=2, steps=30):
+    rng = np.random.default_rng(order * 10 + stepper)
+    nt = 6 * n ** 3
+    er = rng.uniform(1.0, 3.0, nt)
+    sg = rng.uniform(0.0, 0.3, nt) if lossy else None
+    dt = 0.02 if order == 1 else 0.01
+    def make(e, s):
+        o = ot.OracleTet((n, n, n), 1.0 / n, dt, 1.0, 1.0, tf=2, stepper=stepper, eps_r=e, sigma=s, order=order)
+        o.add_dipole([0.41, 0.57, 0.36], [0.3, 0.5, 0.8], f0=3.0, t0=0.3)
+        o.add_receiver([0.62, 0.31, 0.55])
+        o.add_receiver([0.27, 0.71, 0.66])
+        return o
+    o = make(er, sg)
+    d = np.zeros((2, steps // 2 + 1, 3))
+    d[:, 1:, :] = 0.01 * rng.standard_normal((2, steps // 2, 3))
+    return make, er, sg, d, steps
+
+
+@pytest.mark.parametrize("order,stepper,lossy", [(1, ot.SYMPLECTIC2, False), (1, ot.SYMPLECTIC4, True),
+                                                 (2, ot.SYMPLECTIC4, False)])
+def test_adjoint_gradient_matches_central_differences(order, stepper, lossy):
+    make, er, sg, d, steps = _grad_case(order, stepper, lossy)
+    J, ge, gs, _ = ot.misfit_gradient(make(er, sg), steps, d)
+    rng = np.random.default_rng(4)
+    sg0 = sg if sg is not None else np.zeros_like(er)
+    for which in ("eps", "sigma"):
+        v = rng.standard_normal(er.size)
+        h = 1e-4 if which == "eps" else 1e-4
+        def Jat(a):
+            e, s = (er + a * v, sg0) if which == "eps" else (er, sg0 + a * v)
+            return ot.misfit_gradient(make(e, s), steps, d)[0]
+        fd = (Jat(h) - Jat(-h)) / (2 * h)
+        g = (ge if which == "eps" else gs) @ v
+        assert abs(fd - g) <= 1e-6 * max(abs(fd), abs(g)) + 1e-14, (which, fd, g)
