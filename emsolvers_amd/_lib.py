@@ -126,6 +126,7 @@ SIGNATURES = {
     "ems_tet_add_dipole": (_I, [_P, _DP, _DP, _D, _D, _D]),
     "ems_tet_add_receiver": (_I, [_P, _DP]),
     "ems_tet_set_time_step": (_I, [_P, _D]),
+    "ems_tet_set_operator": (_I, [_P, _I]),
     "ems_tet_gradient": (_I, [_P, _I, _DP, _SZ, _DP, _DP, _DP, _DP]),
     "ems_tet_run": (_I, [_P, _I]),
     "ems_tet_run_timed": (_I, [_P, _I, _DP]),

@@ -59,6 +59,223 @@ __device__ __forceinline__ T row_dot(const Csr<T> &A, const T *__restrict__ val,
     return lanes_sum<LPR>(s);
 }
 
+// ---------------------------------------------------------------------------
+// Element-by-element operator (no assembled matrix).  Phase 1, one thread per
+// tetrahedron, waves of one tetrahedron type (so the type's nl x nl element
+// matrices are read at uniform addresses): gather the tetrahedron's nl dofs
+// (boundary dofs as 0), form the local product(s) and store them type-major,
+// tmp[(t nl + l) ncube + cube], coalesced over cubes.  Phase 2, one thread per
+// dof: sum the contributions of the tetrahedra containing its entity in the
+// fixed order of the row type's incidence list; boundary rows are identity
+// rows of the mass-type operators.  Nothing is accumulated with atomics.
+// ---------------------------------------------------------------------------
+struct EbeMesh {
+    int nx, ny, nz, nl;
+    long ne, nf, nd, ncube;
+    long off[8];       // first edge of each edge family
+    long foff[13];     // first face of each face family
+    int ecx[7], ecy[7];  // edge family extents (index = off + I + cx (J + cy K))
+    int fcx[12], fcy[12];
+    int tedge[6][6];   // per tet type and local edge: family | base offset bits << 4
+    int tface[6][4];   // per tet type and local face: face family | base offset bits << 4
+    int inc_start[39];  // per row type: first entry of its incidence list
+};
+
+__device__ __forceinline__ long ebe_at(const long *off, const int *cx, const int *cy, int code, int ci, int cj, int ck)
+{
+    const int f = code & 15;
+    return off[f] + (ci + ((code >> 4) & 1)) + (long)cx[f] * ((cj + ((code >> 5) & 1)) + (long)cy[f] * (ck + ((code >> 6) & 1)));
+}
+
+// global dof of local dof l of tetrahedron type t in cube (ci, cj, ck)
+__device__ __forceinline__ long ebe_dof(const EbeMesh &m, int t, int l, int ci, int cj, int ck)
+{
+    if (l < 6) return ebe_at(m.off, m.ecx, m.ecy, m.tedge[t][l], ci, cj, ck);
+    if (l < 12) return m.ne + ebe_at(m.off, m.ecx, m.ecy, m.tedge[t][l - 6], ci, cj, ck);
+    const int q = (l - 12) >> 1;
+    return 2 * m.ne + 2 * ebe_at(m.foff, m.fcx, m.fcy, m.tface[t][q], ci, cj, ck) + ((l - 12) & 1);
+}
+
+// row type and base vertex of a dof (see row_types on the host)
+__device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J, int &K)
+{
+    if (r < 2 * m.ne) {
+        const int kind = r >= m.ne;
+        const long e = kind ? r - m.ne : r;
+        int f = 0;
+        while (f < 6 && e >= m.off[f + 1]) ++f;
+        const int rem = (int)(e - m.off[f]);
+        I = rem % m.ecx[f];
+        J = (rem / m.ecx[f]) % m.ecy[f];
+        K = rem / (m.ecx[f] * m.ecy[f]);
+        return f + 7 * kind;
+    }
+    const long fd = r - 2 * m.ne;
+    const long fid = fd >> 1;
+    int q = 0;
+    while (q < 11 && fid >= m.foff[q + 1]) ++q;
+    const int rem = (int)(fid - m.foff[q]);
+    I = rem % m.fcx[q];
+    J = (rem / m.fcx[q]) % m.fcy[q];
+    K = rem / (m.fcx[q] * m.fcy[q]);
+    return 14 + 2 * q + (int)(fd & 1);
+}
+
+// phase 2: sum over the tetrahedra containing dof r
+template <typename T>
+__device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict__ inc, const T *__restrict__ tmp, long r)
+{
+    int I, J, K;
+    const int rt = ebe_row(m, r, I, J, K);
+    T s = 0;
+    for (int q = m.inc_start[rt]; q < m.inc_start[rt + 1]; ++q) {
+        const int c = inc[q];
+        const int ci = I + (c & 3) - 1, cj = J + ((c >> 2) & 3) - 1, ck = K + ((c >> 4) & 3) - 1;
+        if (ci < 0 || cj < 0 || ck < 0 || ci >= m.nx || cj >= m.ny || ck >= m.nz) continue;
+        const int t = (c >> 6) & 7, L = (c >> 9) & 31;
+        s += tmp[(long)(t * m.nl + L) * m.ncube + ci + (long)m.nx * (cj + (long)m.ny * ck)];
+    }
+    return s;
+}
+
+// Phase 1 modes: MASS y = (eps0 eps_r + hh sigma) M x1;  STIFF y = S x1;
+// RHS y = (eps0 eps_r - hh sigma) M x1 - D S x2;  FORCE y = -S x2 - sigma M x1
+// (S already divided by mu; x1 = V, x2 = U in the stepper modes).
+enum { EBE_MASS = 0, EBE_STIFF = 1, EBE_RHS = 2, EBE_FORCE = 3 };
+
+template <typename T, int NL, int MODE>
+__global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restrict__ Mt, const T *__restrict__ St,
+                                                   const double *__restrict__ er, const double *__restrict__ sg,
+                                                   double e0, double hh, double D,
+                                                   const unsigned char *__restrict__ bnd, const T *__restrict__ x1,
+                                                   const T *__restrict__ x2, T *__restrict__ tmp, const int *done)
+{
+    if (done && *done) return;
+    const long cube = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (cube >= m.ncube) return;
+    const int ci = (int)(cube % m.nx), cj = (int)((cube / m.nx) % m.ny), ck = (int)(cube / ((long)m.nx * m.ny));
+    const long tid = 6 * cube + t;
+    const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
+    constexpr bool USE2 = MODE == EBE_RHS || MODE == EBE_FORCE;
+    T a[NL], b[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+        const long d = ebe_dof(m, t, l, ci, cj, ck);
+        const bool in = !bnd[d];
+        a[l] = in ? x1[d] : (T)0;
+        b[l] = (USE2 && in) ? x2[d] : (T)0;
+    }
+    const T *M = Mt + t * NL * NL, *S = St + t * NL * NL;
+    T cm;
+    if (MODE == EBE_MASS) cm = (T)(e0 * epsr + hh * sig);
+    else if (MODE == EBE_RHS) cm = (T)(e0 * epsr - hh * sig);
+    else cm = (T)sig;
+#pragma unroll 4
+    for (int l = 0; l < NL; ++l) {
+        T y;
+        if (MODE == EBE_STIFF) {
+            T s = 0;
+#pragma unroll
+            for (int q = 0; q < NL; ++q) s += S[l * NL + q] * a[q];
+            y = s;
+        } else if (MODE == EBE_MASS) {
+            T s = 0;
+#pragma unroll
+            for (int q = 0; q < NL; ++q) s += M[l * NL + q] * a[q];
+            y = cm * s;
+        } else {
+            T sm = 0, ss = 0;
+#pragma unroll
+            for (int q = 0; q < NL; ++q) {
+                sm += M[l * NL + q] * a[q];
+                ss += S[l * NL + q] * b[q];
+            }
+            y = MODE == EBE_RHS ? cm * sm - (T)D * ss : -ss - cm * sm;
+        }
+        tmp[(long)(t * NL + l) * m.ncube + cube] = y;
+    }
+}
+
+// Operator policies of the solver kernels: apply(x, r, lane) returns (A x)_r
+// for lane 0 of the row's lanes.
+template <int LPR, typename T> struct CsrOp {
+    static constexpr int lanes = LPR;
+    Csr<T> A;
+    const T *val;
+    __device__ __forceinline__ int n() const { return A.n; }
+    __device__ __forceinline__ T apply(const T *__restrict__ x, int r, int lane) const
+    {
+        return row_dot<LPR>(A, val, x, r, lane);
+    }
+};
+
+template <typename T> struct EbeOp {
+    static constexpr int lanes = 1;
+    EbeMesh m;
+    const int *inc;
+    const T *tmp;
+    const unsigned char *bnd;
+    int ident;  // boundary rows: 1 identity (mass-type), 0 zero
+    __device__ __forceinline__ int n() const { return (int)m.nd; }
+    __device__ __forceinline__ T apply(const T *__restrict__ x, int r, int) const
+    {
+        if (bnd[r]) return ident ? x[r] : (T)0;
+        return ebe_gather(m, inc, tmp, r);
+    }
+};
+
+// y = A x (phase 2 / CSR row products)
+template <typename OP, typename T>
+__global__ __launch_bounds__(256) void k_apply(OP op, const T *__restrict__ x, T *__restrict__ y)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / OP::lanes, lane = t % OP::lanes;
+    if (r >= op.n()) return;
+    const T v = op.apply(x, r, lane);
+    if (lane == 0) y[r] = v;
+}
+
+// b = phase-2 gather with boundary rows b = V (the stage right-hand side) or
+// 0 (the ABM force)
+template <typename T>
+__global__ __launch_bounds__(256) void k_ebe_rhs(EbeMesh m, const int *__restrict__ inc, const T *__restrict__ tmp,
+                                                 const unsigned char *__restrict__ bnd, const T *__restrict__ V,
+                                                 int keep_v, T *__restrict__ b)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m.nd) return;
+    b[r] = bnd[r] ? (keep_v ? V[r] : (T)0) : ebe_gather(m, inc, tmp, r);
+}
+
+// Jacobi diagonal of the mass-type operator (eps0 eps_r + hh sigma) M: dinv
+template <typename T>
+__global__ __launch_bounds__(256) void k_ebe_diag(EbeMesh m, const int *__restrict__ inc, const T *__restrict__ Mt,
+                                                  const double *__restrict__ er, const double *__restrict__ sg,
+                                                  double e0, double hh, const unsigned char *__restrict__ bnd,
+                                                  T *__restrict__ dinv)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m.nd) return;
+    if (bnd[r]) {
+        dinv[r] = 1;
+        return;
+    }
+    int I, J, K;
+    const int rt = ebe_row(m, r, I, J, K);
+    double s = 0.0;
+    for (int q = m.inc_start[rt]; q < m.inc_start[rt + 1]; ++q) {
+        const int c = inc[q];
+        const int ci = I + (c & 3) - 1, cj = J + ((c >> 2) & 3) - 1, ck = K + ((c >> 4) & 3) - 1;
+        if (ci < 0 || cj < 0 || ck < 0 || ci >= m.nx || cj >= m.ny || ck >= m.nz) continue;
+        const int t = (c >> 6) & 7, L = (c >> 9) & 31;
+        const long tid = 6 * (ci + (long)m.nx * (cj + (long)m.ny * ck)) + t;
+        const double co = e0 * (er ? er[tid] : 1.0) + hh * (sg ? sg[tid] : 0.0);
+        s += co * (double)Mt[(t * m.nl + L) * m.nl + L];
+    }
+    dinv[r] = (T)(1.0 / s);
+}
+
 // y = A x
 template <int LPR, typename T>
 __global__ __launch_bounds__(256) void k_spmv(Csr<T> A, const T *__restrict__ val, const T *__restrict__ x,
@@ -174,16 +391,16 @@ __device__ __forceinline__ void block_partial(double v, double *P)
 __device__ __forceinline__ bool cg_converged(double rz, double ref, double tol2) { return !(rz > tol2 * ref); }
 
 // r = b - A x, z = M^-1 r; partials (r, z) -> P0 and (b, M^-1 b) -> P3
-template <int LPR, typename T>
-__global__ __launch_bounds__(kRB) void k_cg_init(Csr<T> A, const T *__restrict__ val, const T *__restrict__ dinv,
-                                                 const T *__restrict__ b, const T *__restrict__ x, T *r, T *z,
-                                                 double *P, CgState *st)
+template <typename OP, typename T>
+__global__ __launch_bounds__(kRB) void k_cg_init(OP op, const T *__restrict__ dinv, const T *__restrict__ b,
+                                                 const T *__restrict__ x, T *r, T *z, double *P, CgState *st)
 {
+    constexpr int LPR = OP::lanes;
     double rz = 0.0, bb = 0.0;
     const int stride = gridDim.x * kRB / LPR;
-    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < op.n(); i += stride) {
         const int lane = threadIdx.x % LPR;
-        const T ax = row_dot<LPR>(A, val, x, i, lane);
+        const T ax = op.apply(x, i, lane);
         if (lane == 0) {
             const T ri = b[i] - ax, zi = dinv[i] * ri;
             r[i] = ri;
@@ -202,13 +419,14 @@ __global__ __launch_bounds__(kRB) void k_cg_init(Csr<T> A, const T *__restrict__
 
 // p = z, Ap = A z; partial (p, Ap) -> P2 (or done when (r, z) is small
 // already; x = 0 when b = 0)
-template <int LPR, typename T>
-__global__ __launch_bounds__(kRB) void k_cg_first(Csr<T> A, const T *__restrict__ val, const T *__restrict__ z,
-                                                  T *p, T *ap, T *x, double *P, CgState *st, double tol2)
+template <typename OP, typename T>
+__global__ __launch_bounds__(kRB) void k_cg_first(OP op, const T *__restrict__ z, T *p, T *ap, T *x, double *P,
+                                                  CgState *st, double tol2)
 {
+    constexpr int LPR = OP::lanes;
     const double rz = block_total(P, kCgNB), ref = block_total(P + 3 * kCgNB, kCgNB);
     if (!(ref > 0.0)) {
-        for (int i = blockIdx.x * kRB + threadIdx.x; i < A.n; i += gridDim.x * kRB) x[i] = 0;
+        for (int i = blockIdx.x * kRB + threadIdx.x; i < op.n(); i += gridDim.x * kRB) x[i] = 0;
         if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
         return;
     }
@@ -218,9 +436,9 @@ __global__ __launch_bounds__(kRB) void k_cg_first(Csr<T> A, const T *__restrict_
     }
     double pap = 0.0;
     const int stride = gridDim.x * kRB / LPR;
-    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < op.n(); i += stride) {
         const int lane = threadIdx.x % LPR;
-        const T q = row_dot<LPR>(A, val, z, i, lane);
+        const T q = op.apply(z, i, lane);
         if (lane == 0) {
             const T zi = z[i];
             p[i] = zi;
@@ -254,11 +472,11 @@ __global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, cons
 
 // converged?  else beta = (r, z)_new / (r, z)_old; p = z + beta p,
 // Ap = A z + beta Ap; partial (p, Ap) -> P2
-template <int LPR, typename T>
-__global__ __launch_bounds__(kRB) void k_cg_direction(Csr<T> A, const T *__restrict__ val, const T *__restrict__ z,
-                                                      T *p, T *ap, double *P, int par, CgState *st, int it,
-                                                      double tol2)
+template <typename OP, typename T>
+__global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict__ z, T *p, T *ap, double *P, int par,
+                                                      CgState *st, int it, double tol2)
 {
+    constexpr int LPR = OP::lanes;
     if (st->done) return;
     const double rz_old = block_total(P + par * kCgNB, kCgNB);
     const double rz = block_total(P + (par ^ 1) * kCgNB, kCgNB);
@@ -273,9 +491,9 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(Csr<T> A, const T *__restr
     const T beta = (T)(rz / rz_old);
     double pap = 0.0;
     const int stride = gridDim.x * kRB / LPR;
-    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < A.n; i += stride) {
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < op.n(); i += stride) {
         const int lane = threadIdx.x % LPR;
-        const T q = row_dot<LPR>(A, val, z, i, lane);
+        const T q = op.apply(z, i, lane);
         if (lane == 0) {
             const T pi = z[i] + beta * p[i], api = q + beta * ap[i];
             p[i] = pi;
@@ -409,6 +627,10 @@ struct ems_tet {
     DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
     DevBuf d_partial, d_cg;
     int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
+    int op_kind = EMS_TET_OP_AUTO;
+    bool ebe = false;  // element-by-element operator (no assembled matrices)
+    EbeMesh em{};
+    DevBuf d_inc, d_Mt, d_St, d_er, d_sg, d_bnd, d_tmp;
     int nl = 6;   // basis functions per tetrahedron
     long nd = 0;  // dofs
     bool assembled = false, lossy = false;
@@ -724,6 +946,35 @@ template <typename T> ems_status stage_matrices(ems_tet *s)
     const long nd = s->nd, nnz = (long)s->col.size();
     const Csr<T> A{(int)nd, s->d_rp.as<int>(), s->d_col.as<int>()};
     s->deltas = stage_deltas(s);
+    if (s->ebe) {
+        // Jacobi diagonals of (eps0 eps_r + hh sigma) M per stage step (hh = 0: M_eps)
+        for (size_t q = 0; s->lossy && q < s->deltas.size(); ++q)
+            for (size_t t = 0; t < s->sigma.size(); ++t) {
+                const double er = s->eps_r.empty() ? 1.0 : s->eps_r[t];
+                if (!(s->d.eps * er + 0.5 * s->deltas[q] * s->sigma[t] > 0.0))
+                    return invalid("conductivity too large for the symplectic_4 stages: eps + x2 dt sigma / 2 <= 0");
+            }
+        const int g = (int)((nd + 255) / 256);
+        const double *er = s->eps_r.empty() ? nullptr : s->d_er.as<double>();
+        const double *sg = s->sigma.empty() ? nullptr : s->d_sg.as<double>();
+        EMS_TRY(s->d_dinv_me.reserve(sizeof(T) * nd));
+        k_ebe_diag<T><<<g, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_Mt.as<T>(), er, sg, s->d.eps, 0.0,
+                                                s->d_bnd.as<unsigned char>(), s->d_dinv_me.as<T>());
+        for (size_t q = 0; q < s->deltas.size(); ++q) {
+            if (!s->lossy || s->deltas[q] == 0.0) {
+                s->a_dinv[q] = s->d_dinv_me.p;
+                continue;
+            }
+            EMS_TRY(s->d_dinv[q].reserve(sizeof(T) * nd));
+            k_ebe_diag<T><<<g, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_Mt.as<T>(), er, sg, s->d.eps,
+                                                    0.5 * s->deltas[q], s->d_bnd.as<unsigned char>(),
+                                                    s->d_dinv[q].as<T>());
+            s->a_dinv[q] = s->d_dinv[q].p;
+        }
+        EMS_HIP(hipGetLastError());
+        EMS_HIP(hipStreamSynchronize(s->stream));
+        return EMS_OK;
+    }
     // Me + D/2 Ms is positive definite when every tet has eps0 eps_r + D/2 sigma > 0
     // (the symplectic_4 middle stage has D < 0)
     for (size_t q = 0; s->lossy && q < s->deltas.size(); ++q)
@@ -753,12 +1004,100 @@ template <typename T> ems_status stage_matrices(ems_tet *s)
     return EMS_OK;
 }
 
+// Element-by-element setup: boundary flags, mesh tables, incidence lists,
+// element matrices (S divided by mu), per-tetrahedron material.
+template <typename T> ems_status setup_ebe(ems_tet *s)
+{
+    const Mesh &m = s->m;
+    const int order = s->d.order, nl = s->nl;
+    const long nd = s->nd;
+    s->bnd.assign(nd, 0);
+    for (long d = 0; d < nd; ++d) s->bnd[d] = dof_boundary(m, d) ? 1 : 0;
+    EbeMesh &e = s->em;
+    e = EbeMesh{};
+    e.nx = m.nx;
+    e.ny = m.ny;
+    e.nz = m.nz;
+    e.nl = nl;
+    e.ne = m.ne;
+    e.nf = m.nf;
+    e.nd = nd;
+    e.ncube = (long)m.nx * m.ny * m.nz;
+    for (int f = 0; f < 8; ++f) e.off[f] = m.off[f];
+    for (int q = 0; q < 13; ++q) e.foff[q] = m.foff[q];
+    for (int f = 0; f < 7; ++f) {
+        e.ecx[f] = m.nx + 1 - kFam[f][0];
+        e.ecy[f] = m.ny + 1 - kFam[f][1];
+    }
+    for (int q = 0; q < 12; ++q) {
+        e.fcx[q] = m.nx + 1 - kFam[kFFam[q][1]][0];
+        e.fcy[q] = m.ny + 1 - kFam[kFFam[q][1]][1];
+    }
+    for (int t = 0; t < 6; ++t) {
+        for (int l = 0; l < 6; ++l) {
+            const SlotKey k = local_key(t, l);
+            e.tedge[t][l] = k.fam | (k.ri << 4) | (k.rj << 5) | (k.rk << 6);
+        }
+        for (int q = 0; q < 4; ++q) {
+            const SlotKey k = local_key(t, 12 + 2 * q);
+            e.tface[t][q] = k.fam | (k.ri << 4) | (k.rj << 5) | (k.rk << 6);
+        }
+    }
+    const std::vector<RowType> rts = row_types(order);
+    std::vector<int> inc;
+    for (size_t rt = 0; rt < 39; ++rt) {
+        e.inc_start[rt] = (int)inc.size();
+        if (rt >= rts.size()) continue;
+        for (const RowType::Inc &c : rts[rt].inc)
+            inc.push_back((c.d[0] + 1) | ((c.d[1] + 1) << 2) | ((c.d[2] + 1) << 4) | (c.t << 6) | (c.L << 9));
+    }
+    std::vector<double> Mt(6 * nl * nl), St(6 * nl * nl);
+    for (int t = 0; t < 6; ++t) element_matrices(order, t, m.h, &Mt[t * nl * nl], &St[t * nl * nl]);
+    for (double &v : St) v /= s->d.mu;
+    EMS_TRY(s->d_inc.reserve(sizeof(int) * inc.size()));
+    EMS_HIP(hipMemcpy(s->d_inc.p, inc.data(), sizeof(int) * inc.size(), hipMemcpyHostToDevice));
+    EMS_TRY(upload<T>(s->d_Mt, Mt));
+    EMS_TRY(upload<T>(s->d_St, St));
+    std::vector<unsigned char> b(s->bnd.begin(), s->bnd.end());
+    EMS_TRY(s->d_bnd.reserve(b.size()));
+    EMS_HIP(hipMemcpy(s->d_bnd.p, b.data(), b.size(), hipMemcpyHostToDevice));
+    if (!s->eps_r.empty()) {
+        EMS_TRY(s->d_er.reserve(sizeof(double) * s->eps_r.size()));
+        EMS_HIP(hipMemcpy(s->d_er.p, s->eps_r.data(), sizeof(double) * s->eps_r.size(), hipMemcpyHostToDevice));
+    }
+    s->lossy = false;
+    for (double v : s->sigma)
+        if (v != 0.0) s->lossy = true;
+    if (!s->sigma.empty()) {
+        EMS_TRY(s->d_sg.reserve(sizeof(double) * s->sigma.size()));
+        EMS_HIP(hipMemcpy(s->d_sg.p, s->sigma.data(), sizeof(double) * s->sigma.size(), hipMemcpyHostToDevice));
+    }
+    EMS_TRY(s->d_tmp.reserve(sizeof(T) * 6 * nl * (size_t)e.ncube));
+    s->col.clear();
+    return EMS_OK;
+}
+
 template <typename T> ems_status setup(ems_tet *s)
 {
     if (s->assembled) return EMS_OK;
+    s->ebe = s->op_kind != EMS_TET_OP_CSR;
+    const long nd = s->nd;
+    if (s->ebe) {
+        EMS_TRY(setup_ebe<T>(s));
+        EMS_TRY(stage_matrices<T>(s));
+        const size_t vb = sizeof(T) * nd;
+        for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up,
+                          &s->Vp, &s->Ap}) {
+            EMS_TRY(v->reserve(vb));
+            EMS_HIP(hipMemset(v->p, 0, vb));
+        }
+        EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgNB));
+        EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
+        s->assembled = true;
+        return EMS_OK;
+    }
     std::vector<double> S, Me, Ms;
     EMS_TRY(assemble(s, S, Me, Ms));
-    const long nd = s->nd;
     s->lossy = false;
     for (double v : Ms)
         if (v != 0.0) s->lossy = true;
@@ -804,11 +1143,57 @@ template <typename F> void with_lpr(int lpr, F &&f)
 // blocks of 256 threads covering n rows at LPR lanes each
 inline int row_blocks(long n, int lpr) { return (int)((n * lpr + 255) / 256); }
 
-// x <- A^-1 b (initial guess x) for the SPD matrix `val` on the common pattern
-template <typename T> ems_status cg(ems_tet *s, const T *val, const T *dinv, DevBuf &x)
+template <typename F> void with_nl(int nl, F &&f)
+{
+    if (nl == 20) f(std::integral_constant<int, 20>{});
+    else f(std::integral_constant<int, 6>{});
+}
+
+// phase 1 of the element-by-element operator
+template <typename T, int MODE>
+void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const int *done)
+{
+    const dim3 grid((unsigned)((s->em.ncube + 255) / 256), 6);
+    const double *er = s->eps_r.empty() ? nullptr : s->d_er.as<double>();
+    const double *sg = s->sigma.empty() ? nullptr : s->d_sg.as<double>();
+    with_nl(s->nl, [&](auto N) {
+        constexpr int NL = decltype(N)::value;
+        k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
+                                                             s->d.eps, hh, D, s->d_bnd.as<unsigned char>(), x1, x2,
+                                                             s->d_tmp.as<T>(), done);
+    });
+}
+
+template <typename T> EbeOp<T> ebe_op(ems_tet *s, int ident)
+{
+    return EbeOp<T>{s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(), s->d_bnd.as<unsigned char>(), ident};
+}
+
+// y = S x (which 0) or y = M_eps x (which 1)
+template <typename T> ems_status apply_op(ems_tet *s, int which, const T *x, T *y)
 {
     const int n = (int)s->nd;
-    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    if (s->ebe) {
+        if (which == 0) ebe_local<T, EBE_STIFF>(s, 0.0, 0.0, x, nullptr, nullptr);
+        else ebe_local<T, EBE_MASS>(s, 0.0, 0.0, x, nullptr, nullptr);
+        k_apply<EbeOp<T>, T><<<(n + 255) / 256, 256, 0, s->stream>>>(ebe_op<T>(s, which), x, y);
+    } else {
+        const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+        const T *val = which == 0 ? s->d_S.as<T>() : s->d_Me.as<T>();
+        with_lpr(s->lpr, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            k_spmv<LPR, T><<<row_blocks(n, LPR), 256, 0, s->stream>>>(A, val, x, y);
+        });
+    }
+    EMS_HIP(hipGetLastError());
+    return EMS_OK;
+}
+
+// x <- A^-1 b (initial guess x) with A = M_eps (which < 0) or the solve
+// matrix M_eps + D/2 M_sig of stage step `which`
+template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
+{
+    const int n = (int)s->nd;
     // FP32: the recursively updated residual keeps converging below the
     // rounding of the matrix entries, so 1e-7 is reachable
     const double tol = s->esz == 4 ? 1e-7 : 1e-14;
@@ -816,11 +1201,43 @@ template <typename T> ems_status cg(ems_tet *s, const T *val, const T *dinv, Dev
     CgState *st = s->d_cg.as<CgState>();
     double *P = s->d_partial.as<double>();
     T *r = s->r.as<T>(), *z = s->z.as<T>(), *p = s->p.as<T>(), *ap = s->ap.as<T>(), *xx = x.as<T>();
+    const T *b = s->b.as<T>();
+    const T *dinv = (const T *)(which < 0 ? s->d_dinv_me.p : s->a_dinv[which]);
     const int maxit = 512, check = 16;
+    if (s->ebe) {
+        const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
+        const EbeOp<T> op = ebe_op<T>(s, 1);
+        ebe_local<T, EBE_MASS>(s, hh, 0.0, xx, nullptr, nullptr);
+        k_cg_init<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st);
+        ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, nullptr);
+        k_cg_first<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2);
+        EMS_HIP(hipGetLastError());
+        for (int it = 0; it < maxit; ++it) {
+            const int par = it & 1;
+            k_cg_update<T><<<kCgNB, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st);
+            ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, &st->done);
+            k_cg_direction<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2);
+            if ((it + 1) % check == 0) {
+                CgState h;
+                EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+                EMS_HIP(hipStreamSynchronize(s->stream));
+                if (h.done) {
+                    s->cg_last = h.iters;
+                    s->cg_max = std::max(s->cg_max, h.iters);
+                    return EMS_OK;
+                }
+            }
+        }
+        set_error("tetrahedral mass solve did not converge");
+        return EMS_ERR_STATE;
+    }
+    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    const T *val = (const T *)(which < 0 ? s->d_Me.p : s->a_val[which]);
     with_lpr(s->lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
-        k_cg_init<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, dinv, s->b.as<T>(), xx, r, z, P, st);
-        k_cg_first<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, z, p, ap, xx, P, st, tol2);
+        const CsrOp<LPR, T> op{A, val};
+        k_cg_init<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st);
+        k_cg_first<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2);
     });
     EMS_HIP(hipGetLastError());
     for (int it = 0; it < maxit; ++it) {
@@ -828,7 +1245,8 @@ template <typename T> ems_status cg(ems_tet *s, const T *val, const T *dinv, Dev
         k_cg_update<T><<<kCgNB, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st);
         with_lpr(s->lpr, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
-            k_cg_direction<LPR, T><<<kCgNB, kRB, 0, s->stream>>>(A, val, z, p, ap, P, par, st, it, tol2);
+            const CsrOp<LPR, T> op{A, val};
+            k_cg_direction<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2);
         });
         if ((it + 1) % check == 0) {
             CgState h;
@@ -863,7 +1281,11 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, const double
     const int n = (int)s->nd, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
-    with_lpr(s->lpr, [&](auto L) {
+    if (s->ebe) {
+        ebe_local<T, EBE_RHS>(s, 0.5 * D, D, s->V.as<T>(), s->U.as<T>(), nullptr);
+        k_ebe_rhs<T><<<g, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(),
+                                               s->d_bnd.as<unsigned char>(), s->V.as<T>(), 1, s->b.as<T>());
+    } else with_lpr(s->lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         if (s->lossy)
             k_rhs<LPR, T, true><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_Ms.as<T>(), s->d_S.as<T>(),
@@ -875,7 +1297,7 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, const double
                                                             s->b.as<T>());
     });
     load<T>(s, s->b, fdev);
-    EMS_TRY(cg<T>(s, (const T *)s->a_val[q], (const T *)s->a_dinv[q], s->V));
+    EMS_TRY(cg<T>(s, q, s->V));
     k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
     return EMS_OK;
 }
@@ -885,7 +1307,11 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
 {
     const int n = (int)s->nd, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
-    with_lpr(s->lpr, [&](auto L) {
+    if (s->ebe) {
+        ebe_local<T, EBE_FORCE>(s, 0.0, 0.0, V, U, nullptr);
+        k_ebe_rhs<T><<<(n + 255) / 256, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(),
+                                                             s->d_bnd.as<unsigned char>(), V, 0, s->b.as<T>());
+    } else with_lpr(s->lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         if (s->lossy)
             k_force<LPR, T, true><<<gr, 256, 0, s->stream>>>(A, s->d_Ms.as<T>(), s->d_S.as<T>(), U, V, s->b.as<T>());
@@ -893,7 +1319,7 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
             k_force<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
     });
     load<T>(s, s->b, fdev);
-    return cg<T>(s, (const T *)s->a_val[0], (const T *)s->a_dinv[0], dst);
+    return cg<T>(s, 0, dst);
 }
 
 // Largest eigenvalue of M_eps^-1 S by power iteration from a fixed start
@@ -902,8 +1328,7 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
 template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *omega2)
 {
     EMS_TRY(setup<T>(s));
-    const int n = (int)s->nd, gr = row_blocks(n, s->lpr);
-    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    const int n = (int)s->nd;
     std::vector<T> x(n), sx(n), mx(n);
     for (int e = 0; e < n; ++e) x[e] = s->bnd[e] ? (T)0 : (T)(1.0 + (e % 7) / 7.0);
     DevBuf dx, dy;
@@ -912,11 +1337,8 @@ template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *om
     double lam = 0.0;
     for (int it = 0; it <= iters; ++it) {
         EMS_HIP(hipMemcpy(dx.p, x.data(), sizeof(T) * n, hipMemcpyHostToDevice));
-        with_lpr(s->lpr, [&](auto L) {
-            constexpr int LPR = decltype(L)::value;
-            k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, s->d_S.as<T>(), dx.as<T>(), s->b.as<T>());
-            k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), dx.as<T>(), dy.as<T>());
-        });
+        EMS_TRY(apply_op<T>(s, 0, dx.as<T>(), s->b.as<T>()));
+        EMS_TRY(apply_op<T>(s, 1, dx.as<T>(), dy.as<T>()));
         EMS_HIP(hipMemcpy(sx.data(), s->b.p, sizeof(T) * n, hipMemcpyDeviceToHost));
         EMS_HIP(hipMemcpy(mx.data(), dy.p, sizeof(T) * n, hipMemcpyDeviceToHost));
         double xs = 0.0, xm = 0.0;
@@ -929,7 +1351,7 @@ template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *om
         if (it == iters) break;
         // x <- M_eps^-1 S x, normalised (b holds S x)
         EMS_HIP(hipMemset(dy.p, 0, sizeof(T) * n));
-        EMS_TRY(cg<T>(s, s->d_Me.as<T>(), s->d_dinv_me.as<T>(), dy));
+        EMS_TRY(cg<T>(s, -1, dy));
         EMS_HIP(hipMemcpy(x.data(), dy.p, sizeof(T) * n, hipMemcpyDeviceToHost));
         double nn = 0.0;
         for (int e = 0; e < n; ++e) nn += (double)x[e] * (double)x[e];
@@ -1086,8 +1508,7 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     EMS_TRY(setup<T>(s));
     const long nd = s->nd, nt = s->m.nt;
     const int nl = s->nl, nobs = (int)s->obs_dof.size() / nl, tf = s->d.tf;
-    const int n = (int)nd, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
-    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+    const int n = (int)nd, g = (n + 255) / 256;
     // tables: tet -> dofs, element masses, boundary mask
     if (!s->g_t2d.p) {
         std::vector<int> t2d((size_t)nt * nl);
@@ -1173,7 +1594,6 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     const double u_end = inf_norm(s->U), v_end = inf_norm(s->V);
     T *U = s->U.as<T>(), *V = s->V.as<T>(), *lU = s->g_lU.as<T>(), *lV = s->g_lV.as<T>();
     T *tmp = s->g_tmp.as<T>(), *vout = s->g_vout.as<T>();
-    const T *Sv = s->d_S.as<T>(), *Mv = s->d_Me.as<T>(), *Md = s->d_dinv_me.as<T>();
     size_t fo = 0;
     const int tb = (int)((nt + 127) / 128);
     for (int st = N; st >= 1; --st) {
@@ -1185,31 +1605,22 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
             // recover the stage input
             EMS_HIP(hipMemcpyAsync(vout, V, vb, hipMemcpyDeviceToDevice, s->stream));
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);  // U1
-            with_lpr(s->lpr, [&](auto L) {
-                constexpr int LPR = decltype(L)::value;
-                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Sv, U, s->b.as<T>());
-            });
+            EMS_TRY(apply_op<T>(s, 0, U, s->b.as<T>()));
             load<T>(s, s->b, s->g_f.as<double>() + fo * (size_t)std::max(ndip, 1));
-            EMS_TRY(cg<T>(s, Mv, Md, s->g_y));
+            EMS_TRY(cg<T>(s, -1, s->g_y));
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)D, s->g_y.as<T>(), V);        // V_in
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);            // U_in
             // adjoint of the stage
             k_comb<T><<<g, 256, 0, s->stream>>>(n, lV, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr,
                                                 s->b.as<T>());
-            EMS_TRY(cg<T>(s, Mv, Md, s->g_mu));
+            EMS_TRY(cg<T>(s, -1, s->g_mu));
             k_tet_grad<T><<<tb, 128, 0, s->stream>>>(nt, nl, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
                                                      s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
                                                      s->d.eps, -0.5 * D, s->g_ge.as<double>(),
                                                      s->g_gs.as<double>());
-            with_lpr(s->lpr, [&](auto L) {
-                constexpr int LPR = decltype(L)::value;
-                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Sv, s->g_mu.as<T>(), tmp);
-            });
+            EMS_TRY(apply_op<T>(s, 0, s->g_mu.as<T>(), tmp));
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-D), tmp, lU);  // lU1
-            with_lpr(s->lpr, [&](auto L) {
-                constexpr int LPR = decltype(L)::value;
-                k_spmv<LPR, T><<<gr, 256, 0, s->stream>>>(A, Mv, s->g_mu.as<T>(), tmp);
-            });
+            EMS_TRY(apply_op<T>(s, 1, s->g_mu.as<T>(), tmp));
             k_comb<T><<<g, 256, 0, s->stream>>>(n, tmp, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr, lV);
             EMS_HIP(hipGetLastError());
         }
@@ -1348,6 +1759,17 @@ ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms)
     if (!s || !total_ms) return invalid("null argument");
     if (num_steps < 0) return invalid("num_steps must be >= 0");
     return s->esz == 4 ? run_impl<float>(s, num_steps, total_ms) : run_impl<double>(s, num_steps, total_ms);
+}
+
+ems_status ems_tet_set_operator(ems_tet *s, int op)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (op < EMS_TET_OP_AUTO || op > EMS_TET_OP_EBE) return invalid("unknown operator");
+    if (s->steps > 0) return invalid("the operator can only change at rest (before the first step)");
+    s->op_kind = op;
+    s->assembled = false;
+    return EMS_OK;
 }
 
 ems_status ems_tet_set_time_step(ems_tet *s, double dt)

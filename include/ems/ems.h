@@ -360,6 +360,12 @@ ems_status ems_fem1_dof_count(const ems_fem1 *s, size_t *n);
 
 enum { EMS_TET_SYMPLECTIC2 = 0, EMS_TET_SYMPLECTIC4 = 1, EMS_TET_ABM = 2 };
 
+/* Operator of the tetrahedral solver: assembled CSR matrices, or
+ * element-by-element products with the 6 shared element-type matrices (no
+ * assembled matrix; per-tetrahedron products, then a fixed-order gather per
+ * dof).  AUTO: element-by-element (measured faster at both orders). */
+enum { EMS_TET_OP_AUTO = 0, EMS_TET_OP_CSR = 1, EMS_TET_OP_EBE = 2 };
+
 typedef struct ems_tet ems_tet;
 
 typedef struct {
@@ -408,6 +414,9 @@ ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3]);
 /* Change the time step before the first step (rebuilds the stage solve
  * matrices on the device; no reassembly). */
 ems_status ems_tet_set_time_step(ems_tet *s, double dt);
+
+/* Operator choice (EMS_TET_OP_*), at rest only; the next run rebuilds. */
+ems_status ems_tet_set_operator(ems_tet *s, int op);
 
 ems_status ems_tet_run(ems_tet *s, int num_steps);
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);

@@ -57,7 +57,7 @@ def materials(case, seed=7):
     return rng.uniform(1.0, 3.0, nt), rng.uniform(0.0, 0.6, nt) * case["eps"]
 
 
-def make(kind, case, precision="f64"):
+def make(kind, case, precision="f64", op="auto"):
     er, sg = materials(case)
     if kind == "oracle":
         from oracle import tet as ot
@@ -68,6 +68,7 @@ def make(kind, case, precision="f64"):
         from emsolvers_amd.tet import TetSimulation
         s = TetSimulation(case["grid"], case["h"], case["dt"], case["eps"], case["mu"], case["tf"], case["stepper"],
                           order=case.get("order", 1), precision=precision)
+        s.set_operator(op)
         if case["lossy"]:
             s.set_material(er, sg)
     for pos, d, f0, t0, amp in case["dip"]:
@@ -89,12 +90,15 @@ def oracle_run(name):
     return _oracle_cache[name]
 
 
+@pytest.mark.parametrize("op", ["csr", "ebe"])
 @pytest.mark.parametrize("name", list(CASES))
-def test_fp64_parity(name):
+def test_fp64_parity(name, op):
+    """Both operators (assembled CSR; element-by-element products with the
+    shared element matrices) against the oracle."""
     case = CASES[name]
-    print("tet fp64", name, flush=True)
+    print("tet fp64", name, op, flush=True)
     tr_o, U_o, V_o, t_o = oracle_run(name)
-    s = make("gpu", case)
+    s = make("gpu", case, op=op)
     s.run(case["steps"])
     tr = s.traces()
     assert tr.shape == tr_o.shape
@@ -107,12 +111,13 @@ def test_fp64_parity(name):
     assert 0 < mx <= 512
 
 
+@pytest.mark.parametrize("op", ["csr", "ebe"])
 @pytest.mark.parametrize("name", ["s2", "abm-lossy", "s2-si", "p2-s4-lossy"])
-def test_fp32_parity(name):
+def test_fp32_parity(name, op):
     case = CASES[name]
-    print("tet fp32", name, flush=True)
+    print("tet fp32", name, op, flush=True)
     tr_o = oracle_run(name)[0]
-    s = make("gpu", case, precision="f32")
+    s = make("gpu", case, precision="f32", op=op)
     s.run(case["steps"])
     tr = s.traces()
     assert rel_l2(tr, tr_o) < 1e-5
@@ -120,14 +125,14 @@ def test_fp32_parity(name):
     assert em < 1e-4 and pm < 1e-4
 
 
-@pytest.mark.parametrize("name", ["s4-lossy", "abm-lossy"])
-def test_split_runs_equal_one_run(name):
+@pytest.mark.parametrize("name,op", [("s4-lossy", "csr"), ("abm-lossy", "csr"), ("p2-s4-lossy", "ebe")])
+def test_split_runs_equal_one_run(name, op):
     """run(a); run(b) continues the same trajectory (sources, ABM history,
-    receiver slots) bit for bit."""
+    receiver slots) bit for bit (both operators are deterministic)."""
     case = CASES[name]
-    a = make("gpu", case)
+    a = make("gpu", case, op=op)
     a.run(case["steps"])
-    b = make("gpu", case)
+    b = make("gpu", case, op=op)
     first = 31
     b.run(first)
     b.run(case["steps"] - first)

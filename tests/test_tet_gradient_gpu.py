@@ -20,7 +20,7 @@ CASES = {
 ST = {"symplectic_2": 0, "symplectic_4": 1, "abm": 2}
 
 
-def setup(case, kind, precision="f64", seed=3):
+def setup(case, kind, precision="f64", seed=3, op="auto"):
     n = case["n"]
     rng = np.random.default_rng(seed)
     er = rng.uniform(1.0, 3.0, 6 * n ** 3)
@@ -34,6 +34,7 @@ def setup(case, kind, precision="f64", seed=3):
         from emsolvers_amd.tet import TetSimulation
         s = TetSimulation((n, n, n), 1.0 / n, case["dt"], 1.0, 1.0, case["tf"], case["stepper"], case["order"],
                           precision)
+        s.set_operator(op)
         s.set_material(er)
     s.add_dipole(*pos_d)
     for r in recs:
@@ -56,12 +57,13 @@ def reference(name):
     return _ref[name]
 
 
+@pytest.mark.parametrize("op", ["csr", "ebe"])
 @pytest.mark.parametrize("name", list(CASES))
-def test_fp64_gradient_matches_oracle(name):
+def test_fp64_gradient_matches_oracle(name, op):
     case = CASES[name]
-    print("tet gradient fp64", name, flush=True)
+    print("tet gradient fp64", name, op, flush=True)
     J_o, ge_o, gs_o = reference(name)
-    s, d, _ = setup(case, "gpu")
+    s, d, _ = setup(case, "gpu", op=op)
     J, ge, gs, drift = s.gradient(case["steps"], d)
     print("misfit %.6e drift %.2e" % (J, drift))
     assert J == pytest.approx(J_o, rel=1e-12)
