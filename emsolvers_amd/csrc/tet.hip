@@ -345,13 +345,20 @@ __global__ void k_load(T *b, int nw, int nl, const int *__restrict__ dof, const 
 // direction share one pass).  Dot products: per-block partials (rows in a
 // fixed order, then a fixed LDS tree); every block re-reduces the partials
 // it needs in the same fixed order, so all blocks see the same scalars and
-// no scalar kernels or grid barriers are needed.  Partial buffers:
+// no scalar kernels or grid barriers are needed (a last-block reduction
+// would need a device-scope fence per block, which costs more here).  The
+// block count nb grows with the rows (cg_blocks); partial buffers of nb
+// entries each:
 //   P[0], P[1]  (r, z) of even / odd updates,  P[2] (p, Ap),  P[3] (b, M^-1 b)
 // The solve stops when (r, z) <= tol^2 (b, M^-1 b); b = 0 gives x = 0.
 // Block 0 records done and the iteration count for the host.
 // ---------------------------------------------------------------------------
-constexpr int kRB = 256;    // threads per block
-constexpr int kCgNB = 512;  // blocks of the CG kernels (partials per dot)
+constexpr int kRB = 256;       // threads per block
+constexpr int kCgMaxNB = 4096; // most blocks of the CG kernels (partials per dot)
+
+// CG blocks for n rows: about 1024 rows per block, 512 to kCgMaxNB (every
+// block re-reduces the partials, so the count stays moderate)
+inline int cg_blocks(long n) { return (int)std::min<long>(kCgMaxNB, std::max<long>(512, (n + 1023) / 1024)); }
 
 struct CgState {
     int done, iters;
@@ -393,7 +400,7 @@ __device__ __forceinline__ bool cg_converged(double rz, double ref, double tol2)
 // r = b - A x, z = M^-1 r; partials (r, z) -> P0 and (b, M^-1 b) -> P3
 template <typename OP, typename T>
 __global__ __launch_bounds__(kRB) void k_cg_init(OP op, const T *__restrict__ dinv, const T *__restrict__ b,
-                                                 const T *__restrict__ x, T *r, T *z, double *P, CgState *st)
+                                                 const T *__restrict__ x, T *r, T *z, double *P, CgState *st, int nb)
 {
     constexpr int LPR = OP::lanes;
     double rz = 0.0, bb = 0.0;
@@ -410,7 +417,7 @@ __global__ __launch_bounds__(kRB) void k_cg_init(OP op, const T *__restrict__ di
         }
     }
     block_partial(rz, P);
-    block_partial(bb, P + 3 * kCgNB);
+    block_partial(bb, P + 3 * nb);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->done = 0;
         st->iters = 0;
@@ -421,10 +428,10 @@ __global__ __launch_bounds__(kRB) void k_cg_init(OP op, const T *__restrict__ di
 // already; x = 0 when b = 0)
 template <typename OP, typename T>
 __global__ __launch_bounds__(kRB) void k_cg_first(OP op, const T *__restrict__ z, T *p, T *ap, T *x, double *P,
-                                                  CgState *st, double tol2)
+                                                  CgState *st, double tol2, int nb)
 {
     constexpr int LPR = OP::lanes;
-    const double rz = block_total(P, kCgNB), ref = block_total(P + 3 * kCgNB, kCgNB);
+    const double rz = block_total(P, nb), ref = block_total(P + 3 * nb, nb);
     if (!(ref > 0.0)) {
         for (int i = blockIdx.x * kRB + threadIdx.x; i < op.n(); i += gridDim.x * kRB) x[i] = 0;
         if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -446,7 +453,7 @@ __global__ __launch_bounds__(kRB) void k_cg_first(OP op, const T *__restrict__ z
             pap += (double)zi * (double)q;
         }
     }
-    block_partial(pap, P + 2 * kCgNB);
+    block_partial(pap, P + 2 * nb);
 }
 
 // alpha = (r, z) / (p, Ap); x += alpha p, r -= alpha Ap, z = M^-1 r;
@@ -454,10 +461,10 @@ __global__ __launch_bounds__(kRB) void k_cg_first(OP op, const T *__restrict__ z
 template <typename T>
 __global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, const T *__restrict__ p,
                                                    const T *__restrict__ ap, const T *__restrict__ dinv, double *P,
-                                                   int par, const CgState *st)
+                                                   int par, const CgState *st, int nb)
 {
     if (st->done) return;
-    const double rz = block_total(P + par * kCgNB, kCgNB), pap = block_total(P + 2 * kCgNB, kCgNB);
+    const double rz = block_total(P + par * nb, nb), pap = block_total(P + 2 * nb, nb);
     const T a = (T)(rz / pap);
     double s = 0.0;
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
@@ -467,20 +474,20 @@ __global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, cons
         z[i] = zi;
         s += (double)ri * (double)zi;
     }
-    block_partial(s, P + (par ^ 1) * kCgNB);
+    block_partial(s, P + (par ^ 1) * nb);
 }
 
 // converged?  else beta = (r, z)_new / (r, z)_old; p = z + beta p,
 // Ap = A z + beta Ap; partial (p, Ap) -> P2
 template <typename OP, typename T>
 __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict__ z, T *p, T *ap, double *P, int par,
-                                                      CgState *st, int it, double tol2)
+                                                      CgState *st, int it, double tol2, int nb)
 {
     constexpr int LPR = OP::lanes;
     if (st->done) return;
-    const double rz_old = block_total(P + par * kCgNB, kCgNB);
-    const double rz = block_total(P + (par ^ 1) * kCgNB, kCgNB);
-    const double ref = block_total(P + 3 * kCgNB, kCgNB);
+    const double rz_old = block_total(P + par * nb, nb);
+    const double rz = block_total(P + (par ^ 1) * nb, nb);
+    const double ref = block_total(P + 3 * nb, nb);
     if (cg_converged(rz, ref, tol2) || !(rz_old > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->done = 1;
@@ -501,7 +508,7 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict
             pap += (double)pi * (double)api;
         }
     }
-    block_partial(pap, P + 2 * kCgNB);
+    block_partial(pap, P + 2 * nb);
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it + 1;
 }
 
@@ -1091,7 +1098,7 @@ template <typename T> ems_status setup(ems_tet *s)
             EMS_TRY(v->reserve(vb));
             EMS_HIP(hipMemset(v->p, 0, vb));
         }
-        EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgNB));
+        EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgMaxNB));
         EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
         s->assembled = true;
         return EMS_OK;
@@ -1118,7 +1125,7 @@ template <typename T> ems_status setup(ems_tet *s)
         EMS_TRY(v->reserve(vb));
         EMS_HIP(hipMemset(v->p, 0, vb));
     }
-    EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgNB));
+    EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgMaxNB));
     EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
     int longest = 0;
     for (long r = 0; r < nd; ++r) longest = std::max(longest, s->rp[r + 1] - s->rp[r]);
@@ -1203,20 +1210,20 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     T *r = s->r.as<T>(), *z = s->z.as<T>(), *p = s->p.as<T>(), *ap = s->ap.as<T>(), *xx = x.as<T>();
     const T *b = s->b.as<T>();
     const T *dinv = (const T *)(which < 0 ? s->d_dinv_me.p : s->a_dinv[which]);
-    const int maxit = 512, check = 16;
+    const int maxit = 512, check = 16, nb = cg_blocks(n);
     if (s->ebe) {
         const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
         const EbeOp<T> op = ebe_op<T>(s, 1);
         ebe_local<T, EBE_MASS>(s, hh, 0.0, xx, nullptr, nullptr);
-        k_cg_init<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st);
+        k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
         ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, nullptr);
-        k_cg_first<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2);
+        k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
         EMS_HIP(hipGetLastError());
         for (int it = 0; it < maxit; ++it) {
             const int par = it & 1;
-            k_cg_update<T><<<kCgNB, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st);
+            k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
             ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, &st->done);
-            k_cg_direction<EbeOp<T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2);
+            k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
             if ((it + 1) % check == 0) {
                 CgState h;
                 EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
@@ -1236,17 +1243,17 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     with_lpr(s->lpr, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const CsrOp<LPR, T> op{A, val};
-        k_cg_init<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st);
-        k_cg_first<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2);
+        k_cg_init<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+        k_cg_first<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
     });
     EMS_HIP(hipGetLastError());
     for (int it = 0; it < maxit; ++it) {
         const int par = it & 1;
-        k_cg_update<T><<<kCgNB, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st);
+        k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
         with_lpr(s->lpr, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
             const CsrOp<LPR, T> op{A, val};
-            k_cg_direction<CsrOp<LPR, T>, T><<<kCgNB, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2);
+            k_cg_direction<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
         });
         if ((it + 1) % check == 0) {
             CgState h;
