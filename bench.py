@@ -92,7 +92,7 @@ def gpu_fem1(stepper, n, steps, precision="f64"):
 
 
 def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, seed=5, sigma_fraction=0.0,
-            power_iters=300):
+            power_iters=300, op="auto"):
     """Tetrahedral Nedelec elements (order 1 or 2) on the Kuhn split of an n^3
     unit cube: seeded smoothed log-uniform eps_r (sigma uniform in [0, 0.05]
     S/m on a seeded fraction of the tetrahedra), x-dipole (Gaussian pulse)
@@ -106,6 +106,7 @@ def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, 
     er, sg = wl.tet_model(n, seed, sigma_fraction)
     t0 = time.time()
     s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, order, precision)
+    s.set_operator(op)
     s.set_material(er, sg)
     wmax = s.max_frequency(power_iters)
     setup_s = time.time() - t0
@@ -123,7 +124,7 @@ def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, 
                ms_per_step=round(ms / steps, 4), steps=steps, stepper=stepper, order=order, dt=dt,
                omega_max=wmax, cg_iters_last=last, cg_iters_max=mx, setup_s=round(setup_s, 2),
                finite=bool(np.all(np.isfinite(tr))), dtype="fp64" if precision == "f64" else "fp32",
-               sigma_fraction=sigma_fraction)
+               sigma_fraction=sigma_fraction, operator=op)
     s.close()
     return out
 
@@ -384,6 +385,8 @@ def main():
         out["tet"] = [gpu_tet(40, "symplectic_2", args.tet_steps), gpu_tet(40, "abm", args.tet_steps),
                       gpu_tet(40, "symplectic_2", args.tet_steps, "f32"),
                       gpu_tet(64, "symplectic_4", 5, "f32", order=2, sigma_fraction=0.2, power_iters=40),
+                      gpu_tet(64, "symplectic_4", 5, "f32", order=2, sigma_fraction=0.2, power_iters=40,
+                              op="ebe_mfma"),
                       gpu_tet(64, "symplectic_4", 3, "f64", order=2, sigma_fraction=0.2, power_iters=40)]
     if not args.no_fem:
         out["tet_gradient"] = [gpu_tet_gradient(40, 100, "f64"), gpu_tet_gradient(40, 100, "f32")]

@@ -197,6 +197,82 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     }
 }
 
+// Phase 1 of the mass operator with matrix cores: a wave owns 16
+// tetrahedra of one type in consecutive cubes and forms Y = M_t X (NL x 16)
+// with 16x16x4 MFMA tiles (rows padded to 16, K in steps of 4), then scales
+// column n by the tetrahedron's eps0 eps_r + hh sigma.  Operand layouts of
+// the 16x16x4 builtins: A[i][k] in lane (k 16 + i), B[k][j] in lane (k 16 + j);
+// result element v of lane l is D[4 (l / 16) + v][l % 16] (f32) or
+// D[l / 16 + 4 v][l % 16] (f64).
+typedef float ems_f4 __attribute__((ext_vector_type(4)));
+typedef double ems_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<float> {
+    typedef ems_f4 V;
+    static __device__ __forceinline__ V step(float a, float b, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct MfmaAcc<double> {
+    typedef ems_d4 V;
+    static __device__ __forceinline__ V step(double a, double b, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+
+template <typename T, int NL>
+__global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__restrict__ Mt,
+                                                       const double *__restrict__ er, const double *__restrict__ sg,
+                                                       double e0, double hh, const unsigned char *__restrict__ bnd,
+                                                       const T *__restrict__ x, T *__restrict__ tmp, const int *done)
+{
+    if (done && *done) return;
+    constexpr int MT = (NL + 15) / 16, KS = (NL + 3) / 4;
+    const int t = blockIdx.y, lane = threadIdx.x & 63;
+    const long base = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;  // first cube of this wave
+    if (base >= m.ncube) return;  // uniform per wave; no barrier below
+    const int n = lane & 15, kq = lane >> 4;
+    const long cube = base + n;
+    const bool valid = cube < m.ncube;
+    const int ci = valid ? (int)(cube % m.nx) : 0, cj = valid ? (int)((cube / m.nx) % m.ny) : 0;
+    const int ck = valid ? (int)(cube / ((long)m.nx * m.ny)) : 0;
+    const T *M = Mt + t * NL * NL;
+    typedef typename MfmaAcc<T>::V V;
+    V acc[MT];
+#pragma unroll
+    for (int q = 0; q < MT; ++q) acc[q] = V{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int k = 4 * ks + kq;
+        T bv = 0;
+        if (valid && k < NL) {
+            const long d = ebe_dof(m, t, k, ci, cj, ck);
+            if (!bnd[d]) bv = x[d];
+        }
+#pragma unroll
+        for (int q = 0; q < MT; ++q) {
+            const int i = 16 * q + n;
+            const T av = (i < NL && k < NL) ? M[i * NL + k] : (T)0;
+            acc[q] = MfmaAcc<T>::step(av, bv, acc[q]);
+        }
+    }
+    if (!valid) return;
+    const long tid = 6 * cube + t;
+    const T c = (T)(e0 * (er ? er[tid] : 1.0) + hh * (sg ? sg[tid] : 0.0));
+#pragma unroll
+    for (int q = 0; q < MT; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * q + MfmaAcc<T>::row(lane, v);
+            if (row < NL) tmp[(long)(t * NL + row) * m.ncube + cube] = c * acc[q][v];
+        }
+}
+
 // Operator policies of the solver kernels: apply(x, r, lane) returns (A x)_r
 // for lane 0 of the row's lanes.
 template <int LPR, typename T> struct CsrOp {
@@ -635,7 +711,8 @@ struct ems_tet {
     DevBuf d_partial, d_cg;
     int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
     int op_kind = EMS_TET_OP_AUTO;
-    bool ebe = false;  // element-by-element operator (no assembled matrices)
+    bool ebe = false;   // element-by-element operator (no assembled matrices)
+    bool mfma = false;  // element products of the mass operator on matrix cores
     EbeMesh em{};
     DevBuf d_inc, d_Mt, d_St, d_er, d_sg, d_bnd, d_tmp;
     int nl = 6;   // basis functions per tetrahedron
@@ -1088,6 +1165,7 @@ template <typename T> ems_status setup(ems_tet *s)
 {
     if (s->assembled) return EMS_OK;
     s->ebe = s->op_kind != EMS_TET_OP_CSR;
+    s->mfma = s->op_kind == EMS_TET_OP_EBE_MFMA;
     const long nd = s->nd;
     if (s->ebe) {
         EMS_TRY(setup_ebe<T>(s));
@@ -1163,6 +1241,16 @@ void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const 
     const dim3 grid((unsigned)((s->em.ncube + 255) / 256), 6);
     const double *er = s->eps_r.empty() ? nullptr : s->d_er.as<double>();
     const double *sg = s->sigma.empty() ? nullptr : s->d_sg.as<double>();
+    if (MODE == EBE_MASS && s->mfma) {
+        const dim3 gm((unsigned)((s->em.ncube + 63) / 64), 6);
+        with_nl(s->nl, [&](auto N) {
+            constexpr int NL = decltype(N)::value;
+            k_ebe_mass_mfma<T, NL><<<gm, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), er, sg, s->d.eps, hh,
+                                                             s->d_bnd.as<unsigned char>(), x1, s->d_tmp.as<T>(),
+                                                             done);
+        });
+        return;
+    }
     with_nl(s->nl, [&](auto N) {
         constexpr int NL = decltype(N)::value;
         k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
@@ -1772,7 +1860,7 @@ ems_status ems_tet_set_operator(ems_tet *s, int op)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (op < EMS_TET_OP_AUTO || op > EMS_TET_OP_EBE) return invalid("unknown operator");
+    if (op < EMS_TET_OP_AUTO || op > EMS_TET_OP_EBE_MFMA) return invalid("unknown operator");
     if (s->steps > 0) return invalid("the operator can only change at rest (before the first step)");
     s->op_kind = op;
     s->assembled = false;

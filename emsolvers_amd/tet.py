@@ -18,7 +18,7 @@ STEPPERS = {"symplectic_2": SYMPLECTIC2, "symplectic_4": SYMPLECTIC4, "abm": ABM
 # stability limits w_max dt < kappa of each stepper on y'' = -w^2 y
 KAPPA = {SYMPLECTIC2: 2.0, SYMPLECTIC4: 1.5735, ABM: 1.2005}
 # operators: assembled CSR, element-by-element (auto: element-by-element)
-OPERATORS = {"auto": 0, "csr": 1, "ebe": 2}
+OPERATORS = {"auto": 0, "csr": 1, "ebe": 2, "ebe_mfma": 3}
 
 
 def element_matrices(order, tet_type, h):
@@ -111,7 +111,7 @@ class TetSimulation:
         _lib.check(_lib.load().ems_tet_add_receiver(self._h, _lib.dptr(p)))
 
     def set_operator(self, op):
-        """'auto', 'csr' or 'ebe' (at rest)."""
+        """'auto', 'csr', 'ebe' or 'ebe_mfma' (at rest)."""
         _lib.check(_lib.load().ems_tet_set_operator(self._h, OPERATORS[op] if isinstance(op, str) else int(op)))
 
     def set_time_step(self, dt):

@@ -363,8 +363,10 @@ enum { EMS_TET_SYMPLECTIC2 = 0, EMS_TET_SYMPLECTIC4 = 1, EMS_TET_ABM = 2 };
 /* Operator of the tetrahedral solver: assembled CSR matrices, or
  * element-by-element products with the 6 shared element-type matrices (no
  * assembled matrix; per-tetrahedron products, then a fixed-order gather per
- * dof).  AUTO: element-by-element (measured faster at both orders). */
-enum { EMS_TET_OP_AUTO = 0, EMS_TET_OP_CSR = 1, EMS_TET_OP_EBE = 2 };
+ * dof).  EBE_MFMA forms the mass operator's element products with matrix
+ * cores (16x16x4 MFMA over 16 tetrahedra of one type).  AUTO:
+ * element-by-element (measured faster than CSR at both orders). */
+enum { EMS_TET_OP_AUTO = 0, EMS_TET_OP_CSR = 1, EMS_TET_OP_EBE = 2, EMS_TET_OP_EBE_MFMA = 3 };
 
 typedef struct ems_tet ems_tet;
 

@@ -90,11 +90,12 @@ def oracle_run(name):
     return _oracle_cache[name]
 
 
-@pytest.mark.parametrize("op", ["csr", "ebe"])
+@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_fp64_parity(name, op):
-    """Both operators (assembled CSR; element-by-element products with the
-    shared element matrices) against the oracle."""
+    """All operators (assembled CSR; element-by-element products with the
+    shared element matrices, on vector units or matrix cores) against the
+    oracle."""
     case = CASES[name]
     print("tet fp64", name, op, flush=True)
     tr_o, U_o, V_o, t_o = oracle_run(name)
@@ -111,7 +112,7 @@ def test_fp64_parity(name, op):
     assert 0 < mx <= 512
 
 
-@pytest.mark.parametrize("op", ["csr", "ebe"])
+@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma"])
 @pytest.mark.parametrize("name", ["s2", "abm-lossy", "s2-si", "p2-s4-lossy"])
 def test_fp32_parity(name, op):
     case = CASES[name]
