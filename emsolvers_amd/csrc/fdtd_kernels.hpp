@@ -453,24 +453,24 @@ namespace ems {
 namespace fdtd {
 
 // ---------------------------------------------------------------------------
-// Fused single-sweep kernel, one independent wavefront per tile.
+// Fused single-sweep kernel.
 //
 // One launch per time step reads E^n, H^{n-1/2} from the "old" buffers and
 // writes H^{n+1/2}, E^{n+1} into the "new" buffers (ping-pong), so every
-// field word crosses HBM once in each direction per step.  A 64-lane wave
-// owns 63 columns (I), R rows (J) and a chunk of lz planes (K) and marches
-// J inside each plane and K across planes.  E(K) needs H^{n+1/2} at I-1, J-1
-// and K-1:
-//   I-1 : from the neighbouring lane (__shfl_up), lane 0 is a halo column,
-//   J-1 : from the previous row in registers, row 0 is a halo row,
-//   K-1 : from the previous plane in registers, the plane below the chunk is
-//         a halo plane;
-// halo H is recomputed from the old buffers and never stored.  No LDS and no
-// barriers: waves are independent, so load latency is hidden by occupancy.
-// CPML psi for H is ping-ponged too (halo recomputation reads old values);
-// psi for E is updated in place by the owning lane.  Tiles that touch the
-// CPML run a PML=true instantiation, the others one without CPML code.  The
-// arithmetic per output is that of the split kernels: same bits.
+// field word crosses HBM once in each direction per step.  A workgroup of
+// 64 x BY threads (one wave per row J) owns a 63 x (BY-1) tile of (I, J)
+// columns over a chunk of lz planes and marches K.  E(K) needs H^{n+1/2} at
+// I-1, J-1 and K-1:
+//   I-1, J-1 : exchanged through LDS (double-buffered by plane parity, one
+//              barrier per plane); lane 0 and row 0 form the halo and
+//              recompute H there from the old buffers without storing it;
+//   K-1      : carried in registers from the previous plane; the plane below
+//              the chunk is recomputed as a halo plane.
+// The next plane's loads are issued before the barrier.  CPML psi for H is
+// ping-ponged too (halo recomputation reads old values); psi for E is updated
+// in place by the owning lane.  Tiles that touch the CPML run a PML=true
+// instantiation, the others one without CPML code.  The arithmetic per
+// output is that of the split kernels: same bits.
 // ---------------------------------------------------------------------------
 template <typename T> struct PsiH {
     const T *old_[3][2];
