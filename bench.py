@@ -317,6 +317,8 @@ def main():
     ap.add_argument("--no-fem", action="store_true", help="skip the FEM secondary figures")
     ap.add_argument("--tet-steps", type=int, default=200, help="steps of the tetrahedral FEM figures")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched forward / gradient figures")
+    ap.add_argument("--fem-profile", action="store_true",
+                    help="run only short FEM legs (tet configs 2 and 3, fem_v2 and fem at 128^3), for rocprofv3")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -329,6 +331,12 @@ def main():
         print("bench.py: multi-rank launch is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.fem_profile:
+        legs = [gpu_tet(40, "symplectic_2", 50, "f64"),
+                gpu_tet(64, "symplectic_4", 2, "f32", order=2, sigma_fraction=0.2, power_iters=10),
+                gpu_fem(1, "fem_v2", 128, 20), gpu_fem1("fem", 128, 20)]
+        print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
+        return
     if args.copy_calibrate:
         import torch
         x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
