@@ -103,7 +103,8 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
         const int kind = r >= m.ne;
         const long e = kind ? r - m.ne : r;
         int f = 0;
-        while (f < 6 && e >= m.off[f + 1]) ++f;
+#pragma unroll
+        for (int q = 1; q < 7; ++q) f += e >= m.off[q];
         const int rem = (int)(e - m.off[f]);
         I = rem % m.ecx[f];
         J = (rem / m.ecx[f]) % m.ecy[f];
@@ -113,7 +114,8 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
     const long fd = r - 2 * m.ne;
     const long fid = fd >> 1;
     int q = 0;
-    while (q < 11 && fid >= m.foff[q + 1]) ++q;
+#pragma unroll
+    for (int w = 1; w < 12; ++w) q += fid >= m.foff[w];
     const int rem = (int)(fid - m.foff[q]);
     I = rem % m.fcx[q];
     J = (rem / m.fcx[q]) % m.fcy[q];
@@ -121,20 +123,33 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
     return 14 + 2 * q + (int)(fd & 1);
 }
 
-// phase 2: sum over the tetrahedra containing dof r
+// phase 2: sum over the tetrahedra containing dof r, in incidence order.
+// All loads of the row are issued before the sum (at most kMaxInc entries;
+// entries outside the mesh contribute +0).
+constexpr int kMaxInc = 8;
+
 template <typename T>
 __device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict__ inc, const T *__restrict__ tmp, long r)
 {
     int I, J, K;
     const int rt = ebe_row(m, r, I, J, K);
-    T s = 0;
-    for (int q = m.inc_start[rt]; q < m.inc_start[rt + 1]; ++q) {
-        const int c = inc[q];
-        const int ci = I + (c & 3) - 1, cj = J + ((c >> 2) & 3) - 1, ck = K + ((c >> 4) & 3) - 1;
-        if (ci < 0 || cj < 0 || ck < 0 || ci >= m.nx || cj >= m.ny || ck >= m.nz) continue;
-        const int t = (c >> 6) & 7, L = (c >> 9) & 31;
-        s += tmp[(long)(t * m.nl + L) * m.ncube + ci + (long)m.nx * (cj + (long)m.ny * ck)];
+    const int q0 = m.inc_start[rt], cnt = m.inc_start[rt + 1] - q0;
+    T v[kMaxInc];
+#pragma unroll
+    for (int q = 0; q < kMaxInc; ++q) {
+        v[q] = 0;
+        if (q < cnt) {
+            const int c = inc[q0 + q];
+            const int ci = I + (c & 3) - 1, cj = J + ((c >> 2) & 3) - 1, ck = K + ((c >> 4) & 3) - 1;
+            if (ci >= 0 && cj >= 0 && ck >= 0 && ci < m.nx && cj < m.ny && ck < m.nz) {
+                const int t = (c >> 6) & 7, L = (c >> 9) & 31;
+                v[q] = tmp[(long)(t * m.nl + L) * m.ncube + ci + (long)m.nx * (cj + (long)m.ny * ck)];
+            }
+        }
     }
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxInc; ++q) s += v[q];
     return s;
 }
 
@@ -147,7 +162,7 @@ template <typename T, int NL, int MODE>
 __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restrict__ Mt, const T *__restrict__ St,
                                                    const double *__restrict__ er, const double *__restrict__ sg,
                                                    double e0, double hh, double D,
-                                                   const unsigned char *__restrict__ bnd, const T *__restrict__ x1,
+                                                   const unsigned *__restrict__ tmask, const T *__restrict__ x1,
                                                    const T *__restrict__ x2, T *__restrict__ tmp, const int *done)
 {
     if (done && *done) return;
@@ -157,12 +172,13 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     const int ci = (int)(cube % m.nx), cj = (int)((cube / m.nx) % m.ny), ck = (int)(cube / ((long)m.nx * m.ny));
     const long tid = 6 * cube + t;
     const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
+    const unsigned bm = tmask[tid];  // bit l: local dof l is a boundary dof
     constexpr bool USE2 = MODE == EBE_RHS || MODE == EBE_FORCE;
     T a[NL], b[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
         const long d = ebe_dof(m, t, l, ci, cj, ck);
-        const bool in = !bnd[d];
+        const bool in = !((bm >> l) & 1u);
         a[l] = in ? x1[d] : (T)0;
         b[l] = (USE2 && in) ? x2[d] : (T)0;
     }
@@ -228,7 +244,7 @@ template <> struct MfmaAcc<double> {
 template <typename T, int NL>
 __global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__restrict__ Mt,
                                                        const double *__restrict__ er, const double *__restrict__ sg,
-                                                       double e0, double hh, const unsigned char *__restrict__ bnd,
+                                                       double e0, double hh, const unsigned *__restrict__ tmask,
                                                        const T *__restrict__ x, T *__restrict__ tmp, const int *done)
 {
     if (done && *done) return;
@@ -252,7 +268,7 @@ __global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__res
         T bv = 0;
         if (valid && k < NL) {
             const long d = ebe_dof(m, t, k, ci, cj, ck);
-            if (!bnd[d]) bv = x[d];
+            if (!((tmask[6 * cube + t] >> k) & 1u)) bv = x[d];
         }
 #pragma unroll
         for (int q = 0; q < MT; ++q) {
@@ -429,16 +445,26 @@ __global__ void k_load(T *b, int nw, int nl, const int *__restrict__ dof, const 
 // The solve stops when (r, z) <= tol^2 (b, M^-1 b); b = 0 gives x = 0.
 // Block 0 records done and the iteration count for the host.
 // ---------------------------------------------------------------------------
-constexpr int kRB = 256;       // threads per block
-constexpr int kCgMaxNB = 4096; // most blocks of the CG kernels (partials per dot)
+constexpr int kRB = 256;         // threads per block
+constexpr int kCgMaxNB = 16384;  // most blocks of the CG kernels (partials per dot)
+constexpr int kCgInlineNB = 512; // up to this many blocks every block re-reduces the partials
 
-// CG blocks for n rows: about 1024 rows per block, 512 to kCgMaxNB (every
-// block re-reduces the partials, so the count stays moderate)
+// CG blocks for n rows: about 1024 rows per block, 512 to kCgMaxNB.  Above
+// kCgInlineNB the partials are reduced once per dot by a one-block kernel
+// (the same fixed order), and the consumers read the scalar.
 inline int cg_blocks(long n) { return (int)std::min<long>(kCgMaxNB, std::max<long>(512, (n + 1023) / 1024)); }
 
 struct CgState {
     int done, iters;
+    double scal[4];  // reduced dots (slots as the partial buffers) when nb > kCgInlineNB
 };
+
+// scalar of partial slot `slot`: re-reduced in place or read after k_cg_finalize
+__device__ __forceinline__ double block_total(const double *__restrict__ P, int nb);
+__device__ __forceinline__ double cg_dot(const double *__restrict__ P, const CgState *st, int slot, int nb)
+{
+    return nb > kCgInlineNB ? st->scal[slot] : block_total(P + slot * nb, nb);
+}
 
 // sum of nb partials, the same fixed order in every block; all threads get it
 __device__ __forceinline__ double block_total(const double *__restrict__ P, int nb)
@@ -507,7 +533,7 @@ __global__ __launch_bounds__(kRB) void k_cg_first(OP op, const T *__restrict__ z
                                                   CgState *st, double tol2, int nb)
 {
     constexpr int LPR = OP::lanes;
-    const double rz = block_total(P, nb), ref = block_total(P + 3 * nb, nb);
+    const double rz = cg_dot(P, st, 0, nb), ref = cg_dot(P, st, 3, nb);
     if (!(ref > 0.0)) {
         for (int i = blockIdx.x * kRB + threadIdx.x; i < op.n(); i += gridDim.x * kRB) x[i] = 0;
         if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
@@ -540,7 +566,7 @@ __global__ __launch_bounds__(kRB) void k_cg_update(int n, T *x, T *r, T *z, cons
                                                    int par, const CgState *st, int nb)
 {
     if (st->done) return;
-    const double rz = block_total(P + par * nb, nb), pap = block_total(P + 2 * nb, nb);
+    const double rz = cg_dot(P, st, par, nb), pap = cg_dot(P, st, 2, nb);
     const T a = (T)(rz / pap);
     double s = 0.0;
     for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
@@ -561,9 +587,7 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict
 {
     constexpr int LPR = OP::lanes;
     if (st->done) return;
-    const double rz_old = block_total(P + par * nb, nb);
-    const double rz = block_total(P + (par ^ 1) * nb, nb);
-    const double ref = block_total(P + 3 * nb, nb);
+    const double rz_old = cg_dot(P, st, par, nb), rz = cg_dot(P, st, par ^ 1, nb), ref = cg_dot(P, st, 3, nb);
     if (cg_converged(rz, ref, tol2) || !(rz_old > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->done = 1;
@@ -586,6 +610,19 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict
     }
     block_partial(pap, P + 2 * nb);
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it + 1;
+}
+
+// one block: scal[slot] = fixed-order sum of the slot's nb partials
+__global__ __launch_bounds__(kRB) void k_cg_finalize(const double *__restrict__ P, CgState *st, int s0, int s1,
+                                                     int nb)
+{
+    if (st->done) return;
+    const double a = block_total(P + s0 * nb, nb);
+    const double b = s1 >= 0 ? block_total(P + s1 * nb, nb) : 0.0;
+    if (threadIdx.x == 0) {
+        st->scal[s0] = a;
+        if (s1 >= 0) st->scal[s1] = b;
+    }
 }
 
 // A = Me + hh Ms over the stored entries
@@ -714,7 +751,7 @@ struct ems_tet {
     bool ebe = false;   // element-by-element operator (no assembled matrices)
     bool mfma = false;  // element products of the mass operator on matrix cores
     EbeMesh em{};
-    DevBuf d_inc, d_Mt, d_St, d_er, d_sg, d_bnd, d_tmp;
+    DevBuf d_inc, d_Mt, d_St, d_er, d_sg, d_bnd, d_tmp, d_tmask;
     int nl = 6;   // basis functions per tetrahedron
     long nd = 0;  // dofs
     bool assembled = false, lossy = false;
@@ -1145,6 +1182,24 @@ template <typename T> ems_status setup_ebe(ems_tet *s)
     std::vector<unsigned char> b(s->bnd.begin(), s->bnd.end());
     EMS_TRY(s->d_bnd.reserve(b.size()));
     EMS_HIP(hipMemcpy(s->d_bnd.p, b.data(), b.size(), hipMemcpyHostToDevice));
+    // per tetrahedron: bit l set when local dof l is a boundary dof
+    {
+        std::vector<unsigned> tm((size_t)m.nt, 0u);
+        long tid = 0, dd[20];
+        for (int k = 0; k < m.nz; ++k)
+            for (int j = 0; j < m.ny; ++j)
+                for (int i = 0; i < m.nx; ++i)
+                    for (int t = 0; t < 6; ++t, ++tid) {
+                        tet_dofs(m, order, i, j, k, t, dd);
+                        unsigned w = 0;
+                        for (int l = 0; l < nl; ++l) w |= (s->bnd[dd[l]] ? 1u : 0u) << l;
+                        tm[tid] = w;
+                    }
+        EMS_TRY(s->d_tmask.reserve(sizeof(unsigned) * tm.size()));
+        EMS_HIP(hipMemcpy(s->d_tmask.p, tm.data(), sizeof(unsigned) * tm.size(), hipMemcpyHostToDevice));
+    }
+    for (size_t rt = 0; rt + 1 < 39; ++rt)
+        if (e.inc_start[rt + 1] - e.inc_start[rt] > kMaxInc) return invalid("internal: incidence list too long");
     if (!s->eps_r.empty()) {
         EMS_TRY(s->d_er.reserve(sizeof(double) * s->eps_r.size()));
         EMS_HIP(hipMemcpy(s->d_er.p, s->eps_r.data(), sizeof(double) * s->eps_r.size(), hipMemcpyHostToDevice));
@@ -1246,7 +1301,7 @@ void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const 
         with_nl(s->nl, [&](auto N) {
             constexpr int NL = decltype(N)::value;
             k_ebe_mass_mfma<T, NL><<<gm, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), er, sg, s->d.eps, hh,
-                                                             s->d_bnd.as<unsigned char>(), x1, s->d_tmp.as<T>(),
+                                                             s->d_tmask.as<unsigned>(), x1, s->d_tmp.as<T>(),
                                                              done);
         });
         return;
@@ -1254,7 +1309,7 @@ void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const 
     with_nl(s->nl, [&](auto N) {
         constexpr int NL = decltype(N)::value;
         k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
-                                                             s->d.eps, hh, D, s->d_bnd.as<unsigned char>(), x1, x2,
+                                                             s->d.eps, hh, D, s->d_tmask.as<unsigned>(), x1, x2,
                                                              s->d_tmp.as<T>(), done);
     });
 }
@@ -1299,19 +1354,24 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     const T *b = s->b.as<T>();
     const T *dinv = (const T *)(which < 0 ? s->d_dinv_me.p : s->a_dinv[which]);
     const int maxit = 512, check = 16, nb = cg_blocks(n);
+    const bool fin = nb > kCgInlineNB;
     if (s->ebe) {
         const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
         const EbeOp<T> op = ebe_op<T>(s, 1);
         ebe_local<T, EBE_MASS>(s, hh, 0.0, xx, nullptr, nullptr);
         k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 0, 3, nb);
         ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, nullptr);
         k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
         EMS_HIP(hipGetLastError());
         for (int it = 0; it < maxit; ++it) {
             const int par = it & 1;
             k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
+            if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
             ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, &st->done);
             k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
+            if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
             if ((it + 1) % check == 0) {
                 CgState h;
                 EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
@@ -1332,17 +1392,21 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
         constexpr int LPR = decltype(L)::value;
         const CsrOp<LPR, T> op{A, val};
         k_cg_init<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 0, 3, nb);
         k_cg_first<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
     });
     EMS_HIP(hipGetLastError());
     for (int it = 0; it < maxit; ++it) {
         const int par = it & 1;
         k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
         with_lpr(s->lpr, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
             const CsrOp<LPR, T> op{A, val};
             k_cg_direction<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
         });
+        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
         if ((it + 1) % check == 0) {
             CgState h;
             EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
