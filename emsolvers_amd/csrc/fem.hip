@@ -83,6 +83,15 @@ void launch_leapfrog(ems_fem *s, const double *sval)
     const CComp3<T> e0 = cc3<T>(s, s->t0), e1 = cc3<T>(s, s->t1);
     const Comp3<T> e2 = c3<T>(s, s->t2);
     const int ns = (int)s->src_z.size();
+    if (P == 1) {  // fields staged in LDS
+        k_fem_leapfrog1<T, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), e0, e1, e2, ns,
+                                                                     s->d_src.as<long>(), sval);
+        k_fem_leapfrog1<T, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), e0, e1, e2, ns,
+                                                                     s->d_src.as<long>(), sval);
+        k_fem_leapfrog1<T, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), e0, e1, e2, ns,
+                                                                     s->d_src.as<long>(), sval);
+        return;
+    }
     k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
                                                                      e2, ns, s->d_src.as<long>(), sval);
     k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
@@ -99,6 +108,17 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
     const CComp3<T> u = cc3<T>(s, 0), pm = cc3<T>(s, 1);
     const Comp3<T> uo = c3<T>(s, 2), po = c3<T>(s, 3);
     const int ns = (int)s->src_z.size();
+    if (P == 1) {  // fields staged in LDS
+        k_fem_symplectic1<T, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval);
+        k_fem_symplectic1<T, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval);
+        k_fem_symplectic1<T, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval);
+        swap_slots(s, 0, 2);
+        swap_slots(s, 1, 3);
+        return;
+    }
     k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
                                                                        po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
                                                                        s->d_src.as<long>(), sval);
