@@ -277,11 +277,60 @@ template <typename T> Fields<T> fields(ems_fdtd *s)
                      s->f[3].as<T>(), s->f[4].as<T>(), s->f[5].as<T>()};
 }
 
+// Boxes of the split sweeps with a CPML of P cells: the interior box (no
+// CPML cell) and the shell around it (z-slabs over whole planes, y-slabs over
+// the interior planes, x-slab columns over the interior rows and planes).
+// False if the grid has no interior box.
+bool split_boxes(const GridView &g, int P, int by, int lz, SplitBoxes &inner, int &n_inner, SplitBoxes &shell,
+                 int &n_shell)
+{
+    if (g.nx < 2 * P + 1 || g.ny < 2 * P + 1 || g.nz < 2 * P + 1) return false;
+    const int npw = (2 * P + 1 + 63) / 64;
+    const int nfull = npw + (g.nx - 2 * P + 63) / 64;
+    auto add = [&](SplitBoxes &bs, int &total, SplitBox b) {
+        const int rows = b.jpack ? 2 * P + 1 : b.j1 - b.j0;
+        const int per = b.xrows > 0 ? by * b.xrows : by;  // rows per workgroup
+        b.nby = (rows + per - 1) / per;
+        b.base = total;
+        bs.b[bs.n++] = b;
+        total += b.nbx * b.nby * ((b.k1 - b.k0 + lz - 1) / lz);
+    };
+    inner.n = shell.n = 0;
+    n_inner = n_shell = 0;
+    const int Ki0 = P + 1, Ki1 = g.nz - P + 1, Ji0 = P + 1, Ji1 = g.ny - P + 1;
+    // x-slab columns of several rows per wavefront (2P+1 <= 64 always here)
+    const int xrows = 64 / (2 * P + 1);
+    //              i0     i1            j0   j1        k0            k1        packed jpack xrows nbx
+    add(inner, n_inner, {P + 1, g.nx - P + 1, Ji0, Ji1, Ki0, Ki1, 0, 0, 0, (g.nx - 2 * P + 63) / 64, 0, 0});
+    add(shell, n_shell, {0, 0, 0, g.ny + 1, 0, P + 1, 1, 0, 0, nfull, 0, 0});
+    add(shell, n_shell, {0, 0, 0, g.ny + 1, g.nz - P + 1, g.nz + 1, 1, 0, 0, nfull, 0, 0});
+    add(shell, n_shell, {0, 0, 0, 0, Ki0, Ki1, 1, 1, 0, nfull, 0, 0});
+    if (xrows > 0) add(shell, n_shell, {0, 0, Ji0, Ji1, Ki0, Ki1, 0, 0, xrows, 1, 0, 0});
+    else add(shell, n_shell, {0, 0, Ji0, Ji1, Ki0, Ki1, 1, 0, 0, npw, 0, 0});
+    return true;
+}
+
 template <typename T, int MAT, bool PML>
 void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     const GridView &g = s->g;
     const int by = s->split_by, lz = s->split_lz;
+    if (PML && (s->tile_flags & 4)) {
+        SplitBoxes inner, shell;
+        int ni, ns;
+        if (split_boxes(g, s->npml, by, lz, inner, ni, shell, ns)) {
+            const int npw = (2 * s->npml + 1 + 63) / 64;
+            dim3 blk(64, by, 1);
+            Fields<T> f = fields<T>(s);
+            k_h_boxes<T, false><<<ni, blk, 0, s->stream>>>(g, ch, f, pm, lz, npw, inner);
+            k_h_boxes<T, true><<<ns, blk, 0, s->stream>>>(g, ch, f, pm, lz, npw, shell);
+            k_e_boxes<T, MAT, false><<<ni, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm, lz, npw,
+                                                                inner);
+            k_e_boxes<T, MAT, true><<<ns, blk, 0, s->stream>>>(g, ce, f, s->ca.as<T>(), s->cb.as<T>(), pm, lz, npw,
+                                                               shell);
+            return;
+        }
+    }
     // CPML: the x-slab columns get wavefronts of their own
     const int P = PML ? s->npml : 0;
     const int npw = PML ? (2 * P + 1 + 63) / 64 : 0;
@@ -1724,7 +1773,7 @@ ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes, int flags)
     clear_error();
     if (!s) return invalid("null handle");
     if (rows != 8 && rows != 16) return invalid("tile rows must be 8 or 16");
-    if (flags < 0 || flags > 3) return invalid("tile flags must be 0..3");
+    if (flags < 0 || flags > 7) return invalid("tile flags must be 0..7");
     s->tile_flags = flags;
     if (planes < 1) return invalid("tile planes must be >= 1");
     s->tile_by = rows;

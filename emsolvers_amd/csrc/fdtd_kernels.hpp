@@ -78,6 +78,45 @@ template <typename T> __device__ __forceinline__ T cpml_term(T *psi, long q, con
     return t * pr.k + p;
 }
 
+// The same recursion with psi already in a register (prefetched); stores the
+// new psi at q.  Bit for bit cpml_term.
+template <typename T> __device__ __forceinline__ T cpml_reg(T psi_old, T *psi, long q, const Prof<T> pr, T t)
+{
+    T p = pr.b * psi_old + pr.a * t;
+    psi[q] = p;
+    return t * pr.k + p;
+}
+
+// The split sweeps prefetch the next plane's psi with its fields in FP32
+// (measured faster at 256^3); in FP64 the extra registers cost more than the
+// hidden latency and psi is loaded at its plane.
+template <typename T> constexpr bool kPsiPrefetch = sizeof(T) == 4;
+
+// psi of one cell for the split sweeps, in the order the updates use them:
+//   [0] y of component x   [1] z of component x   [2] z of component y
+//   [3] x of component y   [4] x of component z   [5] y of component z
+// (ps[axis][0/1] as in PmlView: x: y z | y: x z | z: x y components)
+template <typename T>
+__device__ __forceinline__ void psi_load(const GridView &g, const PmlView<T> &pm, T *const (&ps)[3][2], int I, int J,
+                                         int K, bool px, bool py, bool pz, T (&v)[6])
+{
+    if (py) {
+        const long q = slab_y(g, pm.P, pm.W, I, J, K);
+        v[0] = ps[1][0][q];
+        v[5] = ps[1][1][q];
+    }
+    if (pz) {
+        const long q = slab_z(g, pm.P, pm.W, I, J, K);
+        v[1] = ps[2][0][q];
+        v[2] = ps[2][1][q];
+    }
+    if (px) {
+        const long q = slab_x(g, pm.P, pm.W, I, J, K);
+        v[3] = ps[0][0][q];
+        v[4] = ps[0][1][q];
+    }
+}
+
 // E <- update(E, bracket) for the three material modes
 template <typename T, int MAT>
 __device__ __forceinline__ T e_update(T e, T br, T ce, const T *__restrict__ ca,
@@ -113,16 +152,11 @@ __device__ __forceinline__ int split_column(const GridView &g, int P, int npw, i
     return I <= g.nx - P ? I : -1;
 }
 
+// H sweep over planes [K0, K1) of the column (I, J); J is wave-uniform.
 template <typename T, bool PML>
-__device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fields<T> &f, const PmlView<T> &pm,
-                                             int lz, int npw, int bx, int by, int bz)
+__device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T> &f, const PmlView<T> &pm, int I,
+                                         int J, int K0, int K1)
 {
-    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
-    // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
-    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
-    const int K0 = bz * lz;
-    const int K1 = min(K0 + lz, g.nz + 1);
-    if (I < 0 || I > g.nx || J > g.ny) return;
     const T *__restrict__ ex = f.ex;
     const T *__restrict__ ey = f.ey;
     const T *__restrict__ ez = f.ez;
@@ -140,9 +174,26 @@ __device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fiel
     // values of the current plane; the next plane is loaded before the stores
     T ex_k1 = ex[u + S], ey_k1 = ey[u + S], ez_k = ez[u], ez_j1 = ez[u + P], ez_i1 = ez[u + 1];
     T ey_i1 = ey[u + 1], ex_j1 = ex[u + P], hxo = hx[u], hyo = hy[u], hzo = hz[u];
+    // CPML: the column's x and y gradings once, psi and the z grading of the
+    // next plane prefetched with the fields (psi is private to its cell)
+    Prof<T> prx{}, pry{}, prz{};
+    T ps[6] = {};
+    bool pz = false;
+    if (PML) {
+        if (px) prx = pm.hp[0][I];
+        if (py) pry = pm.hp[1][J];
+    }
+    if (PML && kPsiPrefetch<T>) {
+        pz = in_pml(pm.P, g.nz, K0);
+        if (pz) prz = pm.hp[2][K0];
+        psi_load(g, pm, pm.psi_h, I, J, K0, px, py, pz, ps);
+    }
     for (int K = K0; K < K1; ++K, u += S) {
         T n_ex_k1 = 0, n_ey_k1 = 0, n_ez_k = 0, n_ez_j1 = 0, n_ez_i1 = 0, n_ey_i1 = 0, n_ex_j1 = 0;
         T n_hx = 0, n_hy = 0, n_hz = 0;
+        T nps[6] = {};
+        Prof<T> nprz{};
+        bool npz = false;
         if (K + 1 < K1) {
             const long v = u + S;
             n_ex_k1 = ex[v + S];
@@ -155,15 +206,30 @@ __device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fiel
             n_hx = hx[v];
             n_hy = hy[v];
             n_hz = hz[v];
+            if (PML && kPsiPrefetch<T>) {
+                npz = in_pml(pm.P, g.nz, K + 1);
+                if (npz) nprz = pm.hp[2][K + 1];
+                psi_load(g, pm, pm.psi_h, I, J, K + 1, px, py, npz, nps);
+            }
         }
-        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        if (PML && !kPsiPrefetch<T>) {
+            pz = in_pml(pm.P, g.nz, K);
+            if (pz) prz = pm.hp[2][K];
+            psi_load(g, pm, pm.psi_h, I, J, K, px, py, pz, ps);
+        }
+        long qx = 0, qy = 0, qz = 0;
+        if (PML) {
+            if (px) qx = slab_x(g, pm.P, pm.W, I, J, K);
+            if (py) qy = slab_y(g, pm.P, pm.W, I, J, K);
+            if (pz) qz = slab_z(g, pm.P, pm.W, I, J, K);
+        }
         if (I >= 1) {  // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
             const T a = ez_j1, b = ez_k, c = ey_k1, d = ey_k;
             T br;
             if (PML && (py || pz)) {
                 T t1 = a - b, t2 = c - d;
-                if (py) t1 = cpml_term(pm.psi_h[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.hp[1][J], t1);
-                if (pz) t2 = cpml_term(pm.psi_h[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.hp[2][K], t2);
+                if (py) t1 = cpml_reg(ps[0], pm.psi_h[1][0], qy, pry, t1);
+                if (pz) t2 = cpml_reg(ps[1], pm.psi_h[2][0], qz, prz, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
@@ -175,8 +241,8 @@ __device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fiel
             T br;
             if (PML && (pz || px)) {
                 T t1 = a - b, t2 = c - d;
-                if (pz) t1 = cpml_term(pm.psi_h[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.hp[2][K], t1);
-                if (px) t2 = cpml_term(pm.psi_h[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.hp[0][I], t2);
+                if (pz) t1 = cpml_reg(ps[2], pm.psi_h[2][1], qz, prz, t1);
+                if (px) t2 = cpml_reg(ps[3], pm.psi_h[0][0], qx, prx, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
@@ -188,13 +254,19 @@ __device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fiel
             T br;
             if (PML && (px || py)) {
                 T t1 = a - b, t2 = c - d;
-                if (px) t1 = cpml_term(pm.psi_h[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.hp[0][I], t1);
-                if (py) t2 = cpml_term(pm.psi_h[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.hp[1][J], t2);
+                if (px) t1 = cpml_reg(ps[4], pm.psi_h[0][1], qx, prx, t1);
+                if (py) t2 = cpml_reg(ps[5], pm.psi_h[1][1], qy, pry, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
             }
             hz[u] = hzo - ch * br;
+        }
+        if (PML && kPsiPrefetch<T>) {
+            pz = npz;
+            prz = nprz;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ps[c] = nps[c];
         }
         ex_k = ex_k1;
         ey_k = ey_k1;
@@ -211,17 +283,11 @@ __device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fiel
     }
 }
 
+// E sweep over planes [K0, K1) of the column (I, J); J is wave-uniform.
 template <typename T, int MAT, bool PML>
-__device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fields<T> &f, const T *__restrict__ ca,
-                                             const T *__restrict__ cb, const PmlView<T> &pm, int lz, int npw, int bx,
-                                             int by, int bz)
+__device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T> &f, const T *__restrict__ ca,
+                                         const T *__restrict__ cb, const PmlView<T> &pm, int I, int J, int K0, int K1)
 {
-    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
-    // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
-    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
-    const int K0 = bz * lz;
-    const int K1 = min(K0 + lz, g.nz + 1);
-    if (I < 0 || I > g.nx || J > g.ny) return;
     T *__restrict__ ex = f.ex;
     T *__restrict__ ey = f.ey;
     T *__restrict__ ez = f.ez;
@@ -242,9 +308,25 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
     T hx_k = hx[u], hy_k = hy[u], hz_k = hz[u], hz_jm = hz[u + dp], hy_im = hy[u + dm], hz_im = hz[u + dm];
     T hx_jm = hx[u + dp], exo = ex[u], eyo = ey[u], ezo = ez[u];
     T cbv = MAT >= 1 ? cb[u] : (T)0, cav = MAT == 2 ? ca[u] : (T)0;
+    // CPML as in the H sweep: gradings of the column once, psi prefetched
+    Prof<T> prx{}, pry{}, prz{};
+    T ps[6] = {};
+    bool pz = false;
+    if (PML) {
+        if (px) prx = pm.ep[0][I];
+        if (py) pry = pm.ep[1][J];
+    }
+    if (PML && kPsiPrefetch<T>) {
+        pz = in_pml(pm.P, g.nz, K0);
+        if (pz) prz = pm.ep[2][K0];
+        psi_load(g, pm, pm.psi_e, I, J, K0, px, py, pz, ps);
+    }
     for (int K = K0; K < K1; ++K, u += S) {
         T n_hx = 0, n_hy = 0, n_hz = 0, n_hz_jm = 0, n_hy_im = 0, n_hz_im = 0, n_hx_jm = 0;
         T n_ex = 0, n_ey = 0, n_ez = 0, n_cb = 0, n_ca = 0;
+        T nps[6] = {};
+        Prof<T> nprz{};
+        bool npz = false;
         if (K + 1 < K1) {
             const long v = u + S;
             n_hx = hx[v];
@@ -259,15 +341,30 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
             n_ez = ez[v];
             if (MAT >= 1) n_cb = cb[v];
             if (MAT == 2) n_ca = ca[v];
+            if (PML && kPsiPrefetch<T>) {
+                npz = in_pml(pm.P, g.nz, K + 1);
+                if (npz) nprz = pm.ep[2][K + 1];
+                psi_load(g, pm, pm.psi_e, I, J, K + 1, px, py, npz, nps);
+            }
         }
-        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        if (PML && !kPsiPrefetch<T>) {
+            pz = in_pml(pm.P, g.nz, K);
+            if (pz) prz = pm.ep[2][K];
+            psi_load(g, pm, pm.psi_e, I, J, K, px, py, pz, ps);
+        }
+        long qx = 0, qy = 0, qz = 0;
+        if (PML) {
+            if (px) qx = slab_x(g, pm.P, pm.W, I, J, K);
+            if (py) qy = slab_y(g, pm.P, pm.W, I, J, K);
+            if (pz) qz = slab_z(g, pm.P, pm.W, I, J, K);
+        }
         if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
             const T a = hz_k, b = hz_jm, c = hy_k, d = hy_km1;
             T br;
             if (PML && (py || pz)) {
                 T t1 = a - b, t2 = c - d;
-                if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t1);
-                if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t2);
+                if (py) t1 = cpml_reg(ps[0], pm.psi_e[1][0], qy, pry, t1);
+                if (pz) t2 = cpml_reg(ps[1], pm.psi_e[2][0], qz, prz, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
@@ -279,8 +376,8 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
             T br;
             if (PML && (pz || px)) {
                 T t1 = a - b, t2 = c - d;
-                if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), pm.ep[2][K], t1);
-                if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t2);
+                if (pz) t1 = cpml_reg(ps[2], pm.psi_e[2][1], qz, prz, t1);
+                if (px) t2 = cpml_reg(ps[3], pm.psi_e[0][0], qx, prx, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
@@ -292,13 +389,19 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
             T br;
             if (PML && (px || py)) {
                 T t1 = a - b, t2 = c - d;
-                if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), pm.ep[0][I], t1);
-                if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t2);
+                if (px) t1 = cpml_reg(ps[4], pm.psi_e[0][1], qx, prx, t1);
+                if (py) t2 = cpml_reg(ps[5], pm.psi_e[1][1], qy, pry, t2);
                 br = t1 - t2;
             } else {
                 br = a - b - c + d;
             }
             ez[u] = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
+        }
+        if (PML && kPsiPrefetch<T>) {
+            pz = npz;
+            prz = nprz;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) ps[c] = nps[c];
         }
         hx_km1 = hx_k;
         hy_km1 = hy_k;
@@ -318,6 +421,32 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
 }
 
 template <typename T, bool PML>
+__device__ __forceinline__ void h_split_body(const GridView &g, T ch, const Fields<T> &f, const PmlView<T> &pm,
+                                             int lz, int npw, int bx, int by, int bz)
+{
+    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
+    // a wavefront is one x-row (workgroup 64 x rows): J is wave-uniform
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * lz;
+    const int K1 = min(K0 + lz, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    h_column<T, PML>(g, ch, f, pm, I, J, K0, K1);
+}
+
+template <typename T, int MAT, bool PML>
+__device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fields<T> &f, const T *__restrict__ ca,
+                                             const T *__restrict__ cb, const PmlView<T> &pm, int lz, int npw, int bx,
+                                             int by, int bz)
+{
+    const int I = split_column(g, pm.P, npw, bx, threadIdx.x);
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * lz;
+    const int K1 = min(K0 + lz, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    e_column<T, MAT, PML>(g, ce, f, ca, cb, pm, I, J, K0, K1);
+}
+
+template <typename T, bool PML>
 __global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw)
 {
     h_split_body<T, PML>(g, ch, f, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
@@ -328,6 +457,92 @@ __global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, 
                                                  const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
 {
     e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// Split sweeps over boxes (an alternative to one launch per sweep, selected
+// by a tiling flag).  With a CPML the interior box (no CPML cell) runs the
+// CPML-free instantiation and the shell around it the CPML one, in a second
+// launch.  Measured at 256^3 this is slower than one launch: the shell
+// launch is small and cannot hide its latency behind the interior's
+// streaming, while the CPML instantiation costs interior cells only ~3%.
+// A box covers
+//   columns: I in [i0, i1), or (packed) the order of split_column over nbx
+//            wavefronts (nbx = npw: the x-slab columns only);
+//   rows:    J in [j0, j1), or (jpack) the 2P+1 rows of the two y-slabs;
+//   planes:  [k0, k1) in chunks of lz per thread.
+// With xrows = R > 0 a wavefront holds the 2P+1 x-slab columns of R
+// consecutive rows (lane = r*(2P+1) + l), so that the x-slab box does not run
+// wavefronts with 2P+1 of 64 lanes busy; J is then not wave-uniform.
+// Blocks of one launch are numbered box by box (x fastest, then y, then z).
+// ---------------------------------------------------------------------------
+struct SplitBox {
+    int i0, i1, j0, j1, k0, k1;
+    int packed, jpack, xrows;
+    int nbx, nby, base;
+};
+
+constexpr int kMaxBoxes = 4;
+struct SplitBoxes {
+    SplitBox b[kMaxBoxes];
+    int n;
+};
+
+// Cell column of this thread in the box list; false for idle lanes.  XR:
+// the list may hold a box with xrows > 0.
+template <bool XR>
+__device__ __forceinline__ bool box_column(const GridView &g, int P, int npw, int lz, const SplitBoxes &bs, int &I,
+                                           int &J, int &K0, int &K1)
+{
+    const int blk = blockIdx.x;
+    SplitBox b = bs.b[0];
+#pragma unroll
+    for (int q = 1; q < kMaxBoxes; ++q)
+        if (q < bs.n && blk >= bs.b[q].base) b = bs.b[q];
+    const int lb = blk - b.base;
+    const int bx = lb % b.nbx, by = (lb / b.nbx) % b.nby, bz = lb / (b.nbx * b.nby);
+    K0 = b.k0 + bz * lz;
+    K1 = min(K0 + lz, b.k1);
+    if (XR && b.xrows > 0) {
+        const int W = 2 * P + 1;
+        const int r = (int)threadIdx.x / W, l = (int)threadIdx.x - r * W;
+        if (r >= b.xrows) return false;
+        I = l <= P ? l : g.nx - 2 * P + l;
+        J = b.j0 + (by * (int)blockDim.y + (int)threadIdx.y) * b.xrows + r;
+        return J < b.j1;
+    }
+    I = b.packed ? split_column(g, P, npw, bx, threadIdx.x) : b.i0 + bx * 64 + (int)threadIdx.x;
+    const int jj = by * blockDim.y + threadIdx.y;  // wave-uniform
+    int Jv;
+    if (b.jpack) {
+        if (jj > 2 * P) return false;
+        Jv = jj <= P ? jj : g.ny - 2 * P + jj;
+    } else {
+        Jv = b.j0 + jj;
+        if (Jv >= b.j1) return false;
+    }
+    J = XR ? Jv : __builtin_amdgcn_readfirstlane(Jv);
+    if (b.packed) return I >= 0;
+    return I < b.i1;
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(512) void k_h_boxes(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw,
+                                                 SplitBoxes bs)
+{
+    int I, J, K0, K1;
+    if (!box_column<PML>(g, pm.P, npw, lz, bs, I, J, K0, K1)) return;
+    h_column<T, PML>(g, ch, f, pm, I, J, K0, K1);
+}
+
+template <typename T, int MAT, bool PML>
+__global__ __launch_bounds__(512) void k_e_boxes(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
+                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw,
+                                                 SplitBoxes bs)
+{
+    int I, J, K0, K1;
+    if (!box_column<PML>(g, pm.P, npw, lz, bs, I, J, K0, K1)) return;
+    e_column<T, MAT, PML>(g, ce, f, ca, cb, pm, I, J, K0, K1);
 }
 
 // ---------------------------------------------------------------------------

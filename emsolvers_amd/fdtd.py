@@ -114,8 +114,9 @@ class FemSimulation:
         _lib.check(_lib.load().ems_fdtd_set_kernel(self._h, int(variant)))
 
     def set_tiling(self, rows, planes, flags=0):
-        """Fused-kernel workgroup rows (8, 16), planes per workgroup and flags
-        (bit 0 prefetch, bit 1 non-temporal stores)."""
+        """Fused-kernel workgroup rows (8, 16), planes per workgroup (or per
+        split-kernel thread) and flags (bit 0 prefetch, bit 1 non-temporal
+        stores, bit 2 split sweeps with a CPML as interior + CPML-shell launches)."""
         _lib.check(_lib.load().ems_fdtd_set_tiling(self._h, int(rows), int(planes), int(flags)))
 
     def run_timed(self, num_steps):

@@ -107,7 +107,9 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
 /* Kernel tiling: fused-kernel workgroup rows (8 or 16; 63 x (rows-1) cells
  * per plane), planes per workgroup or per split-kernel thread (>= 1) and
  * flags (bit 0: prefetch the next plane into registers, bit 1: non-temporal
- * stores).  Results do not depend on it. */
+ * stores, bit 2: split sweeps with a CPML as an interior launch plus a
+ * CPML-shell launch per sweep instead of one launch).  Results do not depend
+ * on it. */
 ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes, int flags);
 
 /* FemSimulation::run (src/fdtd/fdtd_simulation.cpp:242-306): advances

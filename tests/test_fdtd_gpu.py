@@ -129,6 +129,38 @@ def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes
         assert np.array_equal(s.field(w), o.field(w)), NAMES[w]
 
 
+@pytest.mark.parametrize("planes,flags", [(1, 0), (3, 4), (5, 0), (2, 4)])
+def test_split_cpml_boxes_are_bitwise_identical_to_oracle(oracle_runs, planes, flags):
+    """Split sweeps with a CPML, as one launch per sweep (flags bit 2 clear) or
+    an interior launch plus a CPML-shell launch (set), for several K-chunk
+    lengths: every cell updated once, the oracle's FP64 bits."""
+    case = fc.PARITY_CASES["lossy_cpml_rect"]
+    print("split boxes", planes, flags, flush=True)
+    o = oracle_runs("lossy_cpml_rect")
+    s = build_gpu(case, "f64", 1)
+    s.set_tiling(8, planes, flags)
+    s.run(case["nsteps"])
+    assert np.array_equal(s.traces(), o.traces())
+    for w in range(6):
+        assert np.array_equal(s.field(w), o.field(w)), NAMES[w]
+
+
+def test_split_cpml_thinnest_interior_box():
+    """The thinnest grid a 5-cell CPML allows (nx = 2*5 + 1), both split
+    launch forms: the interior box is one column wide, one x-slab wavefront
+    holds all 11 CPML columns."""
+    case = dict(fc.PARITY_CASES["lossy_cpml_rect"], grid=(11, 14, 12), nsteps=40,
+                sources=[(0, (5, 7, 6))], observations=[[[(5, 7, 6)], [(5, 7, 6)], [(5, 7, 6)]]])
+    o = fc.run_oracle(case)
+    for flags in (1, 5):
+        s = build_gpu(case, "f64", 1)
+        s.set_tiling(8, 2, flags)
+        s.run(case["nsteps"])
+        assert np.array_equal(s.traces(), o.traces())
+        for w in range(6):
+            assert np.array_equal(s.field(w), o.field(w)), NAMES[w]
+
+
 def test_run_twice_accumulates_like_reference(golden):
     """run(n) twice: time keeps going, trace slots are added into (the
     reference resizes saved_electric_field_ without clearing it)."""
