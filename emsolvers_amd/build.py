@@ -3,7 +3,9 @@
     python -m emsolvers_amd.build          # or emsolvers_amd.build.build()
 
 Every HIP source under emsolvers_amd/csrc is compiled by hipcc for gfx950
-into emsolvers_amd/libems.so.  Packed FP32 instructions are disabled
+into emsolvers_amd/libems.so; the reference-compatible command-line drivers
+(emsolvers_amd/drivers) are compiled by the host C++ compiler against it into
+emsolvers_amd/bin.  Packed FP32 instructions are disabled
 (-packed-fp32-ops) and floating-point contraction is off, so that FP64
 kernels keep the reference's operation order; FP32 kernels request fused
 multiply-adds explicitly where they want them.
@@ -18,6 +20,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libems.so")
+BIN = os.path.join(HERE, "bin")
+DRIVERS = os.path.join(HERE, "drivers")
+# program name -> (source, extra defines); src/<solver>/main.cpp of the reference
+PROGRAMS = {
+    "ems_fdtd": ("fdtd_main.cpp", []),
+    "ems_fem": ("fem_main.cpp", []),
+    "ems_fem_abm": ("fem_abm_main.cpp", []),
+    "ems_fem_v2": ("hex_main.cpp", ["-DEMS_HEX_STEPPER=0"]),
+    "ems_symplectic_2": ("hex_main.cpp", ["-DEMS_HEX_STEPPER=1"]),
+    "ems_symplectic_4": ("hex_main.cpp", ["-DEMS_HEX_STEPPER=2"]),
+}
 ARCH = os.environ.get("EMS_OFFLOAD_ARCH", "gfx950")
 
 
@@ -74,7 +87,41 @@ def build(verbose=False, jobs=8):
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     os.replace(tmp, LIB)
+    build_drivers(verbose)
     return LIB
+
+
+def host_cxx():
+    for cand in (os.environ.get("CXX"), shutil.which("g++"), shutil.which("c++")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("no host C++ compiler for the drivers")
+
+
+def build_drivers(verbose=False):
+    """Compile the command-line drivers against libems.so (runpath
+    $ORIGIN/.., so the tree can move as a whole)."""
+    os.makedirs(BIN, exist_ok=True)
+    cxx = host_cxx()
+    procs = []
+    for prog, (src, defs) in PROGRAMS.items():
+        out = os.path.join(BIN, prog)
+        cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", DRIVERS] + defs + [
+            os.path.join(DRIVERS, src), "-o", out + ".tmp", "-L", HERE, "-lems",
+            "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((prog, out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+    failed = []
+    for prog, out, p in procs:
+        text = p.communicate()[0].decode(errors="replace")
+        if p.returncode != 0:
+            failed.append((prog, text))
+        else:
+            os.replace(out + ".tmp", out)
+    if failed:
+        raise RuntimeError("driver build failed:\n" + "\n".join("== %s ==\n%s" % f for f in failed))
+    return BIN
 
 
 if __name__ == "__main__":

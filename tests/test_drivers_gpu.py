@@ -1,0 +1,147 @@
+"""Reference-compatible command-line drivers (emsolvers_amd/bin) on the GPU:
+each runs the set-up of its reference main (src/<solver>/main.cpp) and writes
+the reference's CSV file; the values are checked against the compiled
+reference's golden traces (FDTD) or the FP64 oracle configured from the
+main's formulas, restated here, and the file names against known answers of
+the reference's naming rules (fem: fixed 3-decimal stabilization without
+trailing zeros; fem_v2 family: compress(), src/fem_v2/fem_simulation.cpp:
+1446-1475).
+
+Bar: values at the printed precision of the file (6 significant digits for
+fdtd / fem / fem_abm, 16 for the fem_v2 family), exact file names.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "emsolvers_amd", "bin")
+KMAT = os.path.join(ROOT, "tests", "golden", "kmat")
+
+
+def run(prog, args, cwd):
+    exe = os.path.join(BIN, prog)
+    assert os.path.exists(exe), "drivers not built"
+    print("driver", prog, *args, flush=True)
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, cwd=str(cwd), timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def read_csv(path):
+    lines = open(path).read().splitlines()
+    assert lines[0] == "time,Ex,Ey,Ez"
+    return np.array([[float(x) for x in l.split(",")] for l in lines[1:]])
+
+
+def close6(a, b):
+    """Equal at 6 significant digits (the default stream precision)."""
+    return np.all(np.abs(a - b) <= 5.01e-6 * np.abs(b) + 1e-300)
+
+
+def test_fdtd_driver_matches_reference_golden(tmp_path, golden):
+    """ems_fdtd 50: src/fdtd/main.cpp on 50^3 cells and 78 steps, the
+    golden vectors written by the compiled reference for that set-up."""
+    g = golden("fdtd_main50")
+    run("ems_fdtd", [50], tmp_path)
+    # results_<i>_<std::to_string(dt)>_<std::to_string(h)>_<N>.csv
+    assert sorted(os.listdir(tmp_path)) == ["results_0_0.000000_0.020000_78.csv"]
+    v = read_csv(tmp_path / "results_0_0.000000_0.020000_78.csv")
+    tr = g["traces"][0]
+    assert v.shape == (79, 4)
+    assert np.abs(tr).max() > 0
+    assert close6(v[:, 1:], tr)
+    assert close6(v[:, 0], np.arange(79) * (3.0e-9 / 78))
+
+
+def fem1_oracle(stepper, n, grid, steps, dt, h, tf, stab, use_ofem, src_pos, obs_pos):
+    from oracle.fem1 import OracleFem1
+    eps, mu = 8.854187817e-12, 1.25663706212e-6
+    o = OracleFem1((grid,) * 3, h, dt, eps, mu, tf, stepper, stab, use_ofem)
+    r = lambda v: int(np.floor(v / h + 0.5))  # std::round, positive values
+    o.setSourceDipole_x(tuple(r(p) - 1 for p in src_pos))
+    i, j, k = (r(p) for p in obs_pos)
+    o.setObservationPoint([[(i - 1, j, k)],
+                           [(i - 1, j - 1, k), (i - 1, j, k), (i, j - 1, k), (i, j, k)],
+                           [(i - 1, j, k - 1), (i - 1, j, k), (i, j, k - 1), (i, j, k)]])
+    o.run(steps)
+    return o.traces()[0]
+
+
+@pytest.mark.parametrize("n,stab,use_ofem,base,name", [(1, 0.3, 0, 10, "fem_0.3_1.csv"),
+                                                        (2, 0.25, 1, 6, "ofem_0.25_2.csv"),
+                                                        (1, 0.5, 0, 8, "fem_0.5_1.csv")])
+def test_fem_driver_matches_oracle(tmp_path, n, stab, use_ofem, base, name):
+    """ems_fem <n> <stab> <use_ofem> <base>: src/fem/main.cpp with base*n
+    cells per axis (200 in the reference), 2*base*n steps, tf = n."""
+    run("ems_fem", [n, stab, use_ofem, base], tmp_path)
+    assert sorted(os.listdir(tmp_path)) == [name]
+    g = base * n
+    steps = 2 * base * n
+    h, dt = 4.0 / g, 0.8e-8 / steps
+    tr = fem1_oracle(0, n, g, steps, dt, h, n, stab, use_ofem, (1.5,) * 3, (2.5,) * 3)
+    v = read_csv(tmp_path / name)
+    assert v.shape == (tr.shape[0], 4)
+    assert np.abs(tr).max() > 0
+    assert close6(v[:, 1:], tr)
+    assert close6(v[:, 0], np.arange(tr.shape[0]) * n * dt)
+
+
+def test_fem_abm_driver_matches_oracle(tmp_path):
+    """ems_fem_abm 20: src/fem_abm/main.cpp on 20^3 cells, round(1560*20/500)
+    = 62 steps."""
+    run("ems_fem_abm", [20], tmp_path)
+    # std::to_string(3e-9 / 62) = "0.000000", std::to_string(0.25) = "0.250000"
+    assert sorted(os.listdir(tmp_path)) == ["results_0_0.000000_0.250000_62.csv"]
+    tr = fem1_oracle(1, 1, 20, 62, 3.0e-9 / 62, 0.25, 1, 0.05, 0, (0.49,) * 3, (0.51,) * 3)
+    v = read_csv(tmp_path / "results_0_0.000000_0.250000_62.csv")
+    assert v.shape == (63, 4)
+    assert np.abs(tr).max() > 0
+    assert close6(v[:, 1:], tr)
+
+
+HEX = [
+    # program, stepper, argv, expected file name
+    ("ems_fem_v2", 0, [1, 1, 1, 1.0, 1.0, 1.0, 8, 2.0e-9, 0.5, 0.5, 0.5, 0.625, 0.5, 0.5, 1, 1, 1],
+     "ofem1_1_1_1_1_2e-09_5e-01_5e-01_5e-01_6.25e-01_5e-01_5e-01.csv"),
+    ("ems_symplectic_2", 1, [1, 1, 2, 3.0, 3.0, 3.0, 6, 9.0e-9, 1.4, 1.4, 1.4, 1.6, 1.6, 1.6, 1, 1, 1],
+     "ofem2_1_3_3_3_9e-09_1.4_1.4_1.4_1.6_1.6_1.6.csv"),
+    ("ems_symplectic_4", 2, [2, 0, 1, 2.0, 2.0, 2.0, 5, 3.0e-9, 1.0, 0.8, 1.2, 1.2, 1.0, 0.8, 1, 1, 1],
+     "fem1_2_2_2_2_3e-09_1_8e-01_1.2_1.2_1_8e-01.csv"),
+]
+
+
+@pytest.mark.parametrize("prog,stepper,argv,name", HEX, ids=[h[0] for h in HEX])
+def test_hex_driver_matches_oracle(tmp_path, prog, stepper, argv, name):
+    """src/fem_v2/main.cpp argv (and the symplectic mains'): the scale, order,
+    kmat choice, dt = h/(c sqrt 3)/2, tf = 2*scale, floor(duration/dt) + 1
+    steps, z source and the receiver at round(position / h)."""
+    from oracle.fem import OracleFEMv2, load_kmat, kmat_path
+    shutil.copytree(KMAT, tmp_path / "kmat")
+    run(prog, argv, tmp_path)
+    files = sorted(f for f in os.listdir(tmp_path) if f.endswith(".csv"))
+    assert files == [name]
+    scale, use_ofem, order = argv[0], argv[1], argv[2]
+    g = scale * argv[6]
+    h = argv[3] / g
+    eps, mu = 8.8541878128e-12, 1.25663706212e-6  # src/fem_v2/main.cpp:33-34
+    c = 1.0 / np.sqrt(eps * mu)
+    dt = h / (c * np.sqrt(3.0)) / 2.0
+    steps = int(np.floor(argv[7] / dt) + 1)
+    r = lambda v: int(np.floor(v / h + 0.5))
+    o = OracleFEMv2(order, (g,) * 3, h, dt, eps, mu, tf=2 * scale, stepper=stepper, use_ofem=use_ofem,
+                    kraw=load_kmat(kmat_path(order, use_ofem, root=KMAT)))
+    o.setSource_z(tuple(r(p) for p in argv[8:11]))
+    o.setObservationPoint(tuple(r(p) for p in argv[11:14]))
+    o.run(steps)
+    tr = o.traces()[0]
+    v = read_csv(tmp_path / name)
+    assert v.shape == tr.shape
+    assert np.abs(tr[:, 1:]).max() > 0
+    # 15 decimals in scientific notation: 16 significant digits
+    np.testing.assert_allclose(v, tr, rtol=2e-15, atol=0)
