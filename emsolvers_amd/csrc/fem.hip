@@ -26,7 +26,7 @@ struct ems_fem {
     DevBuf K, off;
     // leapfrog: slot[s][comp]; symplectic: U = slot 0, P = slot 1, and the
     // ping-pong partners slot 2 (U), slot 3 (P)
-    DevBuf f[4][3];
+    DevBuf f[5][3];  // slot 4: drifted field of a symplectic stage (order >= 2)
     int t0 = 0, t1 = 1, t2 = 2;  // leapfrog rotation
     std::vector<long> src_z;     // z sources (flat Ez index)
     int nsrc_xy = 0;             // x/y sources: stored by the reference, never injected
@@ -36,6 +36,11 @@ struct ems_fem {
     double t_now = 0.0;
     hipStream_t stream = nullptr;
     DevBuf d_src, d_sval, d_obs, d_rec;
+    // first order: z-marching all-component kernel with BY rows (4 or 8) and
+    // lz planes per workgroup; 0 rows: one gather kernel per component;
+    // -1: per stepper, as measured fastest at 128^3 on MI355X (leapfrog 8 x 8,
+    // symplectic stages 4 x 4)
+    int z1_by = -1, z1_lz = 8;
 };
 
 namespace {
@@ -76,6 +81,24 @@ template <int COMP> dim3 comp_grid(const ems_fem *s)
     return dim3((dx + 63) / 64, (dy + 3) / 4, dz);
 }
 
+// First order, one z-marching launch for all components (k_fem1z).
+template <typename T, bool DRIFT>
+void launch_z1(ems_fem *s, CComp3<T> u, CComp3<T> pm, CComp3<T> other, Comp3<T> o1, Comp3<T> o2, T deltat, T hd,
+               T mm, const double *sval)
+{
+    const FemGrid &g = s->g;
+    const int by = s->z1_by > 0 ? s->z1_by : DRIFT ? 4 : 8;
+    const int lz = s->z1_by > 0 ? s->z1_lz : DRIFT ? 4 : 8;
+    const int ns = (int)s->src_z.size();
+    const dim3 grd((g.Nx + 63) / 64, (g.Ny + by - 1) / by, (g.Nz + lz - 1) / lz);
+    if (by == 4)
+        k_fem1z<T, 4, DRIFT><<<grd, dim3(64, 4, 1), 0, s->stream>>>(g, s->K.as<T>(), u, pm, other, o1, o2, lz, deltat,
+                                                                   hd, mm, ns, s->d_src.as<long>(), sval);
+    else
+        k_fem1z<T, 8, DRIFT><<<grd, dim3(64, 8, 1), 0, s->stream>>>(g, s->K.as<T>(), u, pm, other, o1, o2, lz, deltat,
+                                                                   hd, mm, ns, s->d_src.as<long>(), sval);
+}
+
 template <typename T, int P>
 void launch_leapfrog(ems_fem *s, const double *sval)
 {
@@ -83,6 +106,10 @@ void launch_leapfrog(ems_fem *s, const double *sval)
     const CComp3<T> e0 = cc3<T>(s, s->t0), e1 = cc3<T>(s, s->t1);
     const Comp3<T> e2 = c3<T>(s, s->t2);
     const int ns = (int)s->src_z.size();
+    if (P == 1 && s->z1_by != 0) {
+        launch_z1<T, false>(s, e1, e1, e0, e2, e2, (T)0, (T)0, (T)1, sval);
+        return;
+    }
     if (P == 1) {  // fields staged in LDS
         k_fem_leapfrog1<T, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), e0, e1, e2, ns,
                                                                      s->d_src.as<long>(), sval);
@@ -108,6 +135,12 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
     const CComp3<T> u = cc3<T>(s, 0), pm = cc3<T>(s, 1);
     const Comp3<T> uo = c3<T>(s, 2), po = c3<T>(s, 3);
     const int ns = (int)s->src_z.size();
+    if (P == 1 && s->z1_by != 0) {
+        launch_z1<T, true>(s, u, pm, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, sval);
+        swap_slots(s, 0, 2);
+        swap_slots(s, 1, 3);
+        return;
+    }
     if (P == 1) {  // fields staged in LDS
         k_fem_symplectic1<T, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(
             s->g, s->K.as<T>(), u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval);
@@ -119,15 +152,22 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
         swap_slots(s, 1, 3);
         return;
     }
-    k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
-                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-                                                                       s->d_src.as<long>(), sval);
-    k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
-                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-                                                                       s->d_src.as<long>(), sval);
-    k_fem_symplectic<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, uo,
-                                                                       po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-                                                                       s->d_src.as<long>(), sval);
+    // the drifted field once per stage, then the gathers read it
+    {
+        const long nmax = std::max({s->g.NX, s->g.NY, s->g.NZ});
+        k_fem_drift<T><<<dim3((unsigned)((nmax + 255) / 256), 3, 1), 256, 0, s->stream>>>(
+            s->g, u, pm, c3<T>(s, 4), (T)(0.5 * deltat), (T)mm);
+    }
+    const CComp3<T> ud = cc3<T>(s, 4);
+    k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
+                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
+                                                                       ns, s->d_src.as<long>(), sval);
+    k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
+                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
+                                                                       ns, s->d_src.as<long>(), sval);
+    k_fem_symplectic<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
+                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
+                                                                       ns, s->d_src.as<long>(), sval);
     swap_slots(s, 0, 2);
     swap_slots(s, 1, 3);
 }
@@ -406,7 +446,7 @@ ems_status ems_fem_create(const ems_fem_desc *desc, const double *kmat, ems_fem 
     if (st == EMS_OK && hipMemcpy(s->off.p, off.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess)
         st = hip_fail(hipGetLastError(), "hipMemcpy", __FILE__, __LINE__);
     if (st == EMS_OK) st = desc->precision == EMS_F32 ? upload_kmat<float>(s) : upload_kmat<double>(s);
-    const int nslots = desc->stepper == EMS_FEM_LEAPFROG ? 3 : 4;
+    const int nslots = desc->stepper == EMS_FEM_LEAPFROG ? 3 : desc->order >= 2 ? 5 : 4;
     for (int sl = 0; sl < nslots && st == EMS_OK; ++sl)
         for (int cc = 0; cc < 3 && st == EMS_OK; ++cc) {
             st = s->f[sl][cc].reserve(comp_size(s, cc) * s->esz);
@@ -578,6 +618,17 @@ ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename)
             file << v[0] << "," << v[1] << "," << v[2] << "," << v[3] << std::endl;
         }
     }
+    return EMS_OK;
+}
+
+ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (rows != -1 && rows != 0 && rows != 4 && rows != 8) return invalid("tile rows must be -1, 0, 4 or 8");
+    if (planes < 1) return invalid("tile planes must be >= 1");
+    s->z1_by = rows;
+    s->z1_lz = planes;
     return EMS_OK;
 }
 

@@ -74,14 +74,11 @@ __device__ __forceinline__ bool pec1(const Hex1 &g, int comp, int X, int Y, int 
 // order, with the edge's local row.  For every component that order is
 // (1,1), (0,1), (1,0), (0,0) in the two transverse element offsets, i.e.
 // local rows 3, 2, 1, 0 (x), 3, 1, 2, 0 (y), 3, 2, 1, 0 (z) of the block.
-struct Cand1 {
-    int i, j, k, r;
-};
-
-template <int COMP>
-__device__ __forceinline__ int cands(const Hex1 &g, int X, int Y, int Z, Cand1 *c)
+// f(i, j, k, r) is called for each of them in that order (no candidate
+// array: an array filled with a data-dependent count lives in scratch memory).
+template <int COMP, typename F>
+__device__ __forceinline__ void for_cands(const Hex1 &g, int X, int Y, int Z, F &&f)
 {
-    int n = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int u = (q & 1) ? 0 : 1;  // inner offset: 1 then 0
@@ -103,15 +100,8 @@ __device__ __forceinline__ int cands(const Hex1 &g, int X, int Y, int Z, Cand1 *
             k = Z;
             r = 8 + u + 2 * v;
         }
-        if (i >= 0 && i < g.Nx && j >= 0 && j < g.Ny && k >= 0 && k < g.Nz) {
-            c[n].i = i;
-            c[n].j = j;
-            c[n].k = k;
-            c[n].r = r;
-            ++n;
-        }
+        if (i >= 0 && i < g.Nx && j >= 0 && j < g.Ny && k >= 0 && k < g.Nz) f(i, j, k, r);
     }
-    return n;
 }
 
 // Launch: workgroup 64 x 4, grid (ceil(dx/64), ceil(dy/4), dz) per component.
@@ -138,10 +128,8 @@ __global__ __launch_bounds__(256) void k_fem1_leap(Hex1 g, const T *__restrict__
         e2.c[COMP][idx] = (T)0;
         return;
     }
-    Cand1 c[4];
-    const int nc = cands<COMP>(g, X, Y, Z, c);
     T acc = 0;
-    for (int q = 0; q < nc; ++q) acc -= elem_dot<T>(g, K, c[q].r, e1, c[q].i, c[q].j, c[q].k);
+    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { acc -= elem_dot<T>(g, K, r, e1, i, j, k); });
     e2.c[COMP][idx] = acc + ((T)2 * e1.c[COMP][idx] - e0.c[COMP][idx]);
 }
 
@@ -180,17 +168,15 @@ __global__ __launch_bounds__(256) void k_fem1_abm(Hex1 g, const T *__restrict__ 
     int X, Y, Z;
     long idx;
     if (!coords<COMP>(g, X, Y, Z, idx)) return;
-    Cand1 c[4];
-    const int nc = cands<COMP>(g, X, Y, Z, c);
     T ah = 0;
-    for (int q = 0; q < nc; ++q) ah += elem_dot<T>(g, K, c[q].r, h3, c[q].i, c[q].j, c[q].k);
+    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ah += elem_dot<T>(g, K, r, h3, i, j, k); });
     ho.c[COMP][idx] = ah + h1.c[COMP][idx];
     if (PEC && pec1(g, COMP, X, Y, Z)) {
         eo.c[COMP][idx] = (T)0;
         return;
     }
     T ae = 0;
-    for (int q = 0; q < nc; ++q) ae += elem_dot<T>(g, Kt, c[q].r, e3, c[q].i, c[q].j, c[q].k);
+    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ae += elem_dot<T>(g, Kt, r, e3, i, j, k); });
     eo.c[COMP][idx] = ae + e1.c[COMP][idx];
 }
 

@@ -126,6 +126,97 @@ __device__ __forceinline__ int dof_elements(const FemGrid &g, int comp, int X, i
     return nc;
 }
 
+// Elements of a DOF without a candidate array (an array filled with a
+// data-dependent count lives in scratch memory), used at first order, where
+// it makes the kernels 2-3x faster; at orders 2 and 3 (long row dots) the
+// array above measured faster and is kept.  Along each of the DOF's
+// two transverse axes the DOF lies in the element at v / p, or, when v is on
+// an element boundary (v % p == 0), also in the element v / p - 1 (local
+// index p).  Code a | b << 1 picks the choice on the first (a) and second (b)
+// transverse axis (x edge: y, z; y edge: x, z; z edge: x, y); 1 = the lower
+// element.  Returns false when that element does not exist.
+template <int P>
+__device__ __forceinline__ bool elem_of(const FemGrid &g, int comp, int X, int Y, int Z, int code, int &i, int &j,
+                                        int &k, int &r)
+{
+    constexpr int p = P;
+    const int C[3] = {X, Y, Z};
+    const int ne[3] = {g.nex, g.ney, g.nez};
+    int e[3], l[3];
+    bool ok = true;
+    int t = 0;  // transverse axis counter
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int v = C[a];
+        if (a == comp) {
+            e[a] = v / p;
+            l[a] = v % p;
+            continue;
+        }
+        const int lower = (code >> t) & 1;
+        ++t;
+        if (v % p != 0) {
+            e[a] = v / p;
+            l[a] = v % p;
+            ok = ok && lower == 0;
+        } else if (lower == 0) {
+            e[a] = v / p;
+            l[a] = 0;
+            ok = ok && e[a] < ne[a];
+        } else {
+            e[a] = v / p - 1;
+            l[a] = p;
+            ok = ok && e[a] >= 0;
+        }
+    }
+    i = e[0];
+    j = e[1];
+    k = e[2];
+    if (comp == 0) r = l[0] + l[1] * p + l[2] * p * (p + 1);
+    else if (comp == 1) r = g.D + l[0] + l[1] * (p + 1) + l[2] * p * (p + 1);
+    else r = 2 * g.D + l[0] + l[1] * (p + 1) + l[2] * (p + 1) * (p + 1);
+    return ok;
+}
+
+constexpr long kNoRank = 0x7fffffffffffffffL;
+
+// The (up to 4) element codes of a DOF in the reference's serial order
+// (ascending element rank), missing elements last with rank kNoRank.  Inner
+// elements are visited k, j, i ascending, i.e. codes 3, 2, 1, 0; otherwise
+// the ranks are sorted by a 5-comparator network, in registers.
+template <int P>
+__device__ __forceinline__ void order4(const FemGrid &g, int comp, int X, int Y, int Z, int (&cd)[4], long (&rk)[4])
+{
+    bool inner = true;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int i, j, k, r;
+        cd[q] = 3 - q;
+        if (elem_of<P>(g, comp, X, Y, Z, 3 - q, i, j, k, r)) {
+            inner = inner && i >= 1 && j >= 1 && k >= 1 && i <= g.nex - 2 && j <= g.ney - 2 && k <= g.nez - 2;
+            rk[q] = elem_rank(g, i, j, k);
+        } else {
+            rk[q] = kNoRank;
+        }
+    }
+    if (inner) return;
+    auto cswap = [&](int a, int b) {
+        if (rk[b] < rk[a]) {
+            const long t = rk[a];
+            rk[a] = rk[b];
+            rk[b] = t;
+            const int c = cd[a];
+            cd[a] = cd[b];
+            cd[b] = c;
+        }
+    };
+    cswap(0, 1);
+    cswap(2, 3);
+    cswap(0, 2);
+    cswap(1, 3);
+    cswap(1, 2);
+}
+
 __device__ __forceinline__ void elem_bases(const FemGrid &g, int i, int j, int k, long &bx, long &by, long &bz)
 {
     const long p = g.p, Nx = g.Nx, Ny = g.Ny;
@@ -247,10 +338,28 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__rest
 // Reads (U, P) from one buffer pair and writes the other; neighbours' drift
 // is recomputed in the gather, so one pass per stage.
 // ---------------------------------------------------------------------------
+// Drifted field u + ((dt/2) P)/m of every DOF (the value the symplectic
+// gathers read), once per stage instead of once per gathered term: the same
+// expression, the same bits.  Grid (ceil(max count / 256), 3 components).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fem_drift(FemGrid g, CComp3<T> u, CComp3<T> pm, Comp3<T> d, T hd, T mm)
+{
+    const int c = blockIdx.y;
+    const long n = c == 0 ? g.NX : c == 1 ? g.NY : g.NZ;
+    const long i = blockIdx.x * 256L + threadIdx.x;
+    if (i >= n) return;
+    const T *uc = c == 0 ? u.x : c == 1 ? u.y : u.z;
+    const T *pc = c == 0 ? pm.x : c == 1 ? pm.y : pm.z;
+    T *dc = c == 0 ? d.x : c == 1 ? d.y : d.z;
+    dc[i] = gval<T, true>(uc, pc, i, hd, mm);
+}
+
+// ud: the drifted field of k_fem_drift (gathered); u, pm: the DOF's own values.
 template <typename T, int P, int COMP>
 __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__restrict__ K,
                                                         const int *__restrict__ off, CComp3<T> u, CComp3<T> pm,
-                                                        Comp3<T> u_out, Comp3<T> p_out, T deltat, T hd, T mm,
+                                                        CComp3<T> ud, Comp3<T> u_out, Comp3<T> p_out, T deltat, T hd,
+                                                        T mm,
                                                         int nsrc, const long *__restrict__ src_idx,
                                                         const double *__restrict__ src_val)
 {
@@ -281,7 +390,7 @@ __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__re
             for (int s = 0; s < nsrc_here; ++s) tmp += sv;
             src_done = true;
         }
-        tmp += row_dot<T, P, true>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, u, pm, hd, mm);
+        tmp += row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, ud, ud, hd, mm);
     }
     if (!src_done)
         for (int s = 0; s < nsrc_here; ++s) tmp += sv;
@@ -393,10 +502,16 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog1(FemGrid g, const T *__res
         out[idx] = (T)0;
         return;
     }
-    Cand c[4];
-    const int nc = dof_elements<1>(g, COMP, X, Y, Z, c);
+    int cd[4];
+    long rk[4];
+    order4<1>(g, COMP, X, Y, Z, cd, rk);
     T acc = 0;
-    for (int q = 0; q < nc; ++q) acc -= t1_row_dot<T, COMP>(sK, sE, c[q].r, c[q].i, c[q].j, c[q].k, ox, oy, oz);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int i, j, k, r;
+        if (!elem_of<1>(g, COMP, X, Y, Z, cd[q], i, j, k, r)) continue;
+        acc -= t1_row_dot<T, COMP>(sK, sE, r, i, j, k, ox, oy, oz);
+    }
     const T *a1 = COMP == 0 ? e1.x : COMP == 1 ? e1.y : e1.z;
     const T *a0 = COMP == 0 ? e0.x : COMP == 1 ? e0.y : e0.z;
     T v = acc + ((T)2 * a1[idx] - a0[idx]);
@@ -430,20 +545,24 @@ __global__ __launch_bounds__(256) void k_fem_symplectic1(FemGrid g, const T *__r
         po[idx] = (T)0;
         return;
     }
-    Cand c[4];
-    const int nc = dof_elements<1>(g, COMP, X, Y, Z, c);
+    int cd[4];
+    long rk[4];
+    order4<1>(g, COMP, X, Y, Z, cd, rk);
     int nsrc_here = 0;
     if (COMP == 2)
         for (int s = 0; s < nsrc; ++s) nsrc_here += src_idx[s] == idx;
     const T sv = (T)src_val[0];
     T tmp = 0;
     bool src_done = nsrc_here == 0;
-    for (int q = 0; q < nc; ++q) {
-        if (!src_done && c[q].rank >= g.OE) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int i, j, k, r;
+        if (!elem_of<1>(g, COMP, X, Y, Z, cd[q], i, j, k, r)) continue;
+        if (!src_done && rk[q] >= g.OE) {
             for (int s = 0; s < nsrc_here; ++s) tmp += sv;
             src_done = true;
         }
-        tmp += t1_row_dot<T, COMP>(sK, sE, c[q].r, c[q].i, c[q].j, c[q].k, ox, oy, oz);
+        tmp += t1_row_dot<T, COMP>(sK, sE, r, i, j, k, ox, oy, oz);
     }
     if (!src_done)
         for (int s = 0; s < nsrc_here; ++s) tmp += sv;
@@ -453,6 +572,239 @@ __global__ __launch_bounds__(256) void k_fem_symplectic1(FemGrid g, const T *__r
     const T p1 = pa[idx] - deltat * tmp;
     uo[idx] = u1 + hd * p1 / mm;
     po[idx] = p1;
+}
+
+// ---------------------------------------------------------------------------
+// First order, all three components in one z-marching pass.
+//
+// A workgroup of 64 x BY threads owns the points X0 <= X < X0 + 64,
+// Y0 <= Y < Y0 + BY of a chunk of planes and updates, at each plane Z, the
+// x, y and z edge DOFs based at (X, Y, Z).  Every DOF on the faces X = Nx,
+// Y = Ny, Z = Nz is PEC-tangential (zero): the threads at X = Nx-1,
+// Y = Ny-1, Z = Nz-1 write those, so the thread grid covers Nx x Ny x Nz.
+// The fields (drifted in symplectic stages, as gval) are staged in LDS as a
+// rolling window of 4 plane slots, each holding the x, y and z arrays on the
+// 66 x (BY+2) box (X0-1 .. X0+64, Y0-1 .. Y0+BY) of one plane; plane P lives
+// in slot P & 3.  Plane Z+2 is loaded into registers while plane Z is
+// computed from planes Z-1 .. Z+1 and stored after it: one barrier per plane.
+// Candidate order and row dots are those of the kernels above: same bits.
+// ---------------------------------------------------------------------------
+template <int BY> struct Z1Win {
+    static constexpr int EX = 66, EY = BY + 2;
+    static constexpr int PLANE = EX * EY;  // one array of one plane
+    static constexpr int SLOT = 3 * PLANE;  // x, y, z arrays of one plane
+    static constexpr int NT = 64 * BY;
+    static constexpr int NPF = (SLOT + NT - 1) / NT;  // entries per thread per plane
+};
+
+template <typename T, int BY>
+__device__ __forceinline__ T z1_row_dot(const T *sK, const T *win, int r, int i, int j, int k, int ox, int oy)
+{
+    using W = Z1Win<BY>;
+    const T *p0 = win + (k & 3) * W::SLOT, *p1 = win + ((k + 1) & 3) * W::SLOT;
+    const int b = (i - ox) + W::EX * (j - oy);
+    const T *kr = sK + r * 12;
+    T temp = 0;
+    temp += kr[0] * p0[b];
+    temp += kr[1] * p0[b + W::EX];
+    temp += kr[2] * p1[b];
+    temp += kr[3] * p1[b + W::EX];
+    temp += kr[4] * p0[W::PLANE + b];
+    temp += kr[5] * p0[W::PLANE + b + 1];
+    temp += kr[6] * p1[W::PLANE + b];
+    temp += kr[7] * p1[W::PLANE + b + 1];
+    temp += kr[8] * p0[2 * W::PLANE + b];
+    temp += kr[9] * p0[2 * W::PLANE + b + 1];
+    temp += kr[10] * p0[2 * W::PLANE + b + W::EX];
+    temp += kr[11] * p0[2 * W::PLANE + b + W::EX + 1];
+    return temp;
+}
+
+// Global index of window entry e of plane P, or -1 outside the array.
+template <int BY>
+__device__ __forceinline__ long z1_entry(const FemGrid &g, int e, int ox, int oy, int P, int &q)
+{
+    using W = Z1Win<BY>;
+    q = e / W::PLANE;
+    const int rem = e - q * W::PLANE;
+    const int ly = rem / W::EX, lx = rem - ly * W::EX;
+    const int X = ox + lx, Y = oy + ly;
+    const int dx = g.Nx + (q != 0), dy = g.Ny + (q != 1), dz = g.Nz + (q != 2);
+    if (X < 0 || Y < 0 || P < 0 || X >= dx || Y >= dy || P >= dz) return -1;
+    return X + (long)dx * (Y + (long)dy * P);
+}
+
+template <typename T, int BY, bool DRIFT> struct Z1Regs {
+    T v[Z1Win<BY>::NPF];
+    T m[DRIFT ? Z1Win<BY>::NPF : 1];
+};
+
+template <typename T, int BY, bool DRIFT>
+__device__ __forceinline__ void z1_fetch(const FemGrid &g, CComp3<T> u, CComp3<T> pm, int ox, int oy, int P,
+                                         Z1Regs<T, BY, DRIFT> &r)
+{
+    using W = Z1Win<BY>;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+#pragma unroll
+    for (int t = 0; t < W::NPF; ++t) {
+        const int e = tid + t * W::NT;
+        r.v[t] = 0;
+        if (DRIFT) r.m[t] = 0;
+        if (e < W::SLOT) {
+            int q;
+            const long a = z1_entry<BY>(g, e, ox, oy, P, q);
+            if (a >= 0) {
+                const T *uq = q == 0 ? u.x : q == 1 ? u.y : u.z;
+                r.v[t] = uq[a];
+                if (DRIFT) {
+                    const T *mq = q == 0 ? pm.x : q == 1 ? pm.y : pm.z;
+                    r.m[t] = mq[a];
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int BY, bool DRIFT>
+__device__ __forceinline__ void z1_store(T *win, int P, const Z1Regs<T, BY, DRIFT> &r, T hd, T mm)
+{
+    using W = Z1Win<BY>;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    T *slot = win + (P & 3) * W::SLOT;
+#pragma unroll
+    for (int t = 0; t < W::NPF; ++t) {
+        const int e = tid + t * W::NT;
+        if (e < W::SLOT) slot[e] = DRIFT ? r.v[t] + hd * r.m[t] / mm : r.v[t];
+    }
+}
+
+// The step of one DOF from its candidates: leapfrog (DRIFT = false)
+//   out = ((0 - sum K_e E1) + (2 E1 - E0)) - src
+// or symplectic stage (DRIFT = true), as k_fem_leapfrog / k_fem_symplectic.
+template <typename T, int BY, bool DRIFT>
+__device__ __forceinline__ void z1_dof(const FemGrid &g, const T *sK, const T *win, int comp, int X, int Y, int Z,
+                                       int ox, int oy, long idx, T own, const T *__restrict__ other, T *o1, T *o2,
+                                       T deltat, T hd, T mm, int nsrc, const long *__restrict__ src_idx, T sv)
+{
+    int cd[4];
+    long rk[4];
+    order4<1>(g, comp, X, Y, Z, cd, rk);
+    if (!DRIFT) {
+        T acc = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int i, j, k, r;
+            if (!elem_of<1>(g, comp, X, Y, Z, cd[q], i, j, k, r)) continue;
+            acc -= z1_row_dot<T, BY>(sK, win, r, i, j, k, ox, oy);
+        }
+        T v = acc + ((T)2 * own - other[idx]);
+        if (comp == 2)
+            for (int s = 0; s < nsrc; ++s)
+                if (src_idx[s] == idx) v -= sv;
+        o1[idx] = v;
+    } else {
+        int nsrc_here = 0;
+        if (comp == 2)
+            for (int s = 0; s < nsrc; ++s) nsrc_here += src_idx[s] == idx;
+        T tmp = 0;
+        bool src_done = nsrc_here == 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int i, j, k, r;
+            if (!elem_of<1>(g, comp, X, Y, Z, cd[q], i, j, k, r)) continue;
+            if (!src_done && rk[q] >= g.OE) {
+                for (int s = 0; s < nsrc_here; ++s) tmp += sv;
+                src_done = true;
+            }
+            tmp += z1_row_dot<T, BY>(sK, win, r, i, j, k, ox, oy);
+        }
+        if (!src_done)
+            for (int s = 0; s < nsrc_here; ++s) tmp += sv;
+        // own = u + ((dt/2) P)/m, the drifted value staged in the window
+        const T p1 = other[idx] - deltat * tmp;
+        o1[idx] = own + hd * p1 / mm;
+        o2[idx] = p1;
+    }
+}
+
+template <typename T, bool DRIFT>
+__device__ __forceinline__ void z1_zero(T *o1, T *o2, long idx)
+{
+    o1[idx] = (T)0;
+    if (DRIFT) o2[idx] = (T)0;
+}
+
+// Leapfrog: u = E1, other = E0, out1 = E2.  Symplectic stage: u = U, pm = P,
+// other = P (the DOF's own momentum), out1 = U', out2 = P'.
+template <typename T, int BY, bool DRIFT>
+__global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restrict__ K, CComp3<T> u, CComp3<T> pm,
+                                                    CComp3<T> other, Comp3<T> out1, Comp3<T> out2, int lz, T deltat,
+                                                    T hd, T mm, int nsrc, const long *__restrict__ src_idx,
+                                                    const double *__restrict__ src_val)
+{
+    using W = Z1Win<BY>;
+    __shared__ T sK[144];
+    __shared__ T win[4 * W::SLOT];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    if (tid < 144) sK[tid] = K[tid];
+    const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * BY;
+    const int ox = X0 - 1, oy = Y0 - 1;
+    const int Z0 = blockIdx.z * lz, Z1 = min(Z0 + lz, g.Nz);
+    const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
+    const bool act = X < g.Nx && Y < g.Ny;
+    const T sv = (T)src_val[0];
+    // warm-up: planes Z0-1 .. Z0+1
+    for (int P = Z0 - 1; P <= Z0 + 1; ++P) {
+        Z1Regs<T, BY, DRIFT> r;
+        z1_fetch<T, BY, DRIFT>(g, u, pm, ox, oy, P, r);
+        z1_store<T, BY, DRIFT>(win, P, r, hd, mm);
+    }
+    __syncthreads();
+    const long sx1 = g.Nx, sx2 = (long)g.Nx * (g.Ny + 1);             // x array strides
+    const long sy1 = g.Nx + 1, sy2 = (long)(g.Nx + 1) * g.Ny;         // y array
+    const long sz1 = g.Nx + 1, sz2 = (long)(g.Nx + 1) * (g.Ny + 1);   // z array
+    const int lo = ((int)threadIdx.x + 1) + W::EX * ((int)threadIdx.y + 1);  // own (X, Y) in the window
+    for (int Z = Z0; Z < Z1; ++Z) {
+        Z1Regs<T, BY, DRIFT> r;
+        z1_fetch<T, BY, DRIFT>(g, u, pm, ox, oy, Z + 2, r);
+        if (act) {
+            const T *cur = win + (Z & 3) * W::SLOT;
+            // the x, y, z edges based at (X, Y, Z), one code path (register
+            // pressure); PEC on the low faces here, on X = Nx, Y = Ny, Z = Nz below
+#pragma unroll 1
+            for (int comp = 0; comp < 3; ++comp) {
+                const long i1 = comp == 0 ? sx1 : comp == 1 ? sy1 : sz1;
+                const long i2 = comp == 0 ? sx2 : comp == 1 ? sy2 : sz2;
+                const long idx = X + i1 * Y + i2 * Z;
+                T *a1 = comp == 0 ? out1.x : comp == 1 ? out1.y : out1.z;
+                T *a2 = comp == 0 ? out2.x : comp == 1 ? out2.y : out2.z;
+                const T *ot = comp == 0 ? other.x : comp == 1 ? other.y : other.z;
+                const bool pec = comp == 0 ? (Y == 0 || Z == 0) : comp == 1 ? (X == 0 || Z == 0) : (X == 0 || Y == 0);
+                if (pec) z1_zero<T, DRIFT>(a1, a2, idx);
+                else z1_dof<T, BY, DRIFT>(g, sK, win, comp, X, Y, Z, ox, oy, idx, cur[comp * W::PLANE + lo], ot, a1,
+                                          a2, deltat, hd, mm, nsrc, src_idx, sv);
+            }
+            // the PEC faces X = Nx, Y = Ny, Z = Nz
+            const bool ex = X == g.Nx - 1, ey = Y == g.Ny - 1, ez = Z == g.Nz - 1;
+            if (ey) {
+                z1_zero<T, DRIFT>(out1.x, out2.x, X + sx1 * g.Ny + sx2 * Z);
+                z1_zero<T, DRIFT>(out1.z, out2.z, X + sz1 * g.Ny + sz2 * Z);
+            }
+            if (ex) {
+                z1_zero<T, DRIFT>(out1.y, out2.y, g.Nx + sy1 * Y + sy2 * Z);
+                z1_zero<T, DRIFT>(out1.z, out2.z, g.Nx + sz1 * Y + sz2 * Z);
+            }
+            if (ez) {
+                z1_zero<T, DRIFT>(out1.x, out2.x, X + sx1 * Y + sx2 * g.Nz);
+                z1_zero<T, DRIFT>(out1.y, out2.y, X + sy1 * Y + sy2 * g.Nz);
+            }
+            if (ey && ez) z1_zero<T, DRIFT>(out1.x, out2.x, X + sx1 * g.Ny + sx2 * g.Nz);
+            if (ex && ez) z1_zero<T, DRIFT>(out1.y, out2.y, g.Nx + sy1 * Y + sy2 * g.Nz);
+            if (ex && ey) z1_zero<T, DRIFT>(out1.z, out2.z, g.Nx + sz1 * g.Ny + sz2 * Z);
+        }
+        z1_store<T, BY, DRIFT>(win, Z + 2, r, hd, mm);
+        __syncthreads();
+    }
 }
 
 // Receivers (src/fem_v2/fem_simulation.cpp:1275-1310): the four x edges and

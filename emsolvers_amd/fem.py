@@ -126,6 +126,12 @@ class FemSimulation:
         self.set(1, slot, v[n0:n0 + n1])
         self.set(2, slot, v[n0 + n1:])
 
+    def set_tiling(self, rows, planes=8):
+        """First-order kernels: rows 4 or 8 = one z-marching pass over all
+        components (64 x rows workgroups, `planes` planes each), 0 = one
+        gather kernel per component.  Results do not depend on it."""
+        _lib.check(_lib.load().ems_fem_set_tiling(self._h, int(rows), int(planes)))
+
     def ndof(self):
         n = ctypes.c_size_t()
         _lib.check(_lib.load().ems_fem_dof_count(self._h, ctypes.byref(n)))

@@ -272,6 +272,13 @@ ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename)
 /* Total edge DOFs (NX + NY + NZ). */
 ems_status ems_fem_dof_count(const ems_fem *s, size_t *n);
 
+/* First-order kernels (no reference counterpart): rows = 4 or 8 selects the
+ * z-marching kernel that updates all three components in one pass with a
+ * 64 x rows workgroup and `planes` planes per workgroup; rows = 0 one gather
+ * kernel per component; rows = -1 (default) the measured choice per stepper.
+ * Results do not depend on it. */
+ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes);
+
 /* ------------------------------------------ first-order stabilized hex FEM
  * src/fem (leapfrog, element matrix 1/2 dt^2/(mu eps h^2) C_s B_s^-1 A_s with
  * stabilization factor s, Hertz-dipole source) and src/fem_abm

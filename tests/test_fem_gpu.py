@@ -61,6 +61,42 @@ def oracle_cache():
     return get
 
 
+@pytest.mark.parametrize("rows,planes", [(0, 8), (4, 1), (4, 3), (8, 5), (8, 8)])
+@pytest.mark.parametrize("stepper", [0, 1, 2])
+def test_first_order_tilings_bitwise(oracle_cache, stepper, rows, planes):
+    """First order: the z-marching all-component kernel (64 x rows workgroups,
+    any number of planes per workgroup) and the per-component gather kernels
+    (rows = 0) give the oracle's bits."""
+    key = (1, stepper)
+    case = CASES[key]
+    print("tiling", key, rows, planes, flush=True)
+    o = oracle_cache(key)
+    g = build("gpu", 1, stepper, case, "f64")
+    g.set_tiling(rows, planes)
+    g.run(case["steps"])
+    assert np.array_equal(g.traces(), o.traces())
+    for c in range(3):
+        for slot in (0, 1):
+            assert np.array_equal(g.get(c, slot), o.get(c, slot)), (c, slot)
+
+
+@pytest.mark.parametrize("stepper", [0, 2])
+def test_first_order_wide_grid_bitwise(stepper):
+    """A grid wider than one 64-column tile and taller than one row block,
+    with the source and receiver near tile edges."""
+    case = dict(grid=(70, 11, 9), steps=25, src=(64, 8, 4), obs=[(63, 8, 4), (65, 3, 4)], tf=1)
+    print("wide grid", stepper, flush=True)
+    o = build("oracle", 1, stepper, case)
+    o.run(case["steps"])
+    for rows in (4, 8):
+        g = build("gpu", 1, stepper, case, "f64")
+        g.set_tiling(rows, 4)
+        g.run(case["steps"])
+        assert np.array_equal(g.traces(), o.traces())
+        for c in range(3):
+            assert np.array_equal(g.get(c, 1), o.get(c, 1)), c
+
+
 @pytest.mark.parametrize("key", list(CASES), ids=lambda k: "p%d-s%d" % k)
 def test_fp64_bitwise_with_oracle(oracle_cache, key):
     order, stepper = key
