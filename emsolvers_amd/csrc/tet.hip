@@ -666,33 +666,40 @@ __global__ void k_comb(int n, const T *__restrict__ c0, T k, T a1, const T *__re
 // Misfit gradient per tetrahedron (one thread each, fixed order):
 //   ge[T] += eps0 mu_T' M_T (Vin - Vout)_T,  gs[T] += -D/2 mu_T' M_T (Vin + Vout)_T
 // over the interior dofs of T (boundary dofs are masked).
-template <typename T>
-__global__ void k_tet_grad(long nt, int nl, const int *__restrict__ t2d, const double *__restrict__ mloc,
+template <typename T, int NL>
+__global__ __launch_bounds__(128) void k_tet_grad(long nt, const int *__restrict__ t2d, const double *__restrict__ mloc,
                            const unsigned char *__restrict__ bnd, const T *__restrict__ mu,
                            const T *__restrict__ vin, const T *__restrict__ vout, double eps0, double mhalf,
                            double *__restrict__ ge, double *__restrict__ gs)
 {
+    // NL (6 or 20) known at compile time: the local vectors stay in registers
+    constexpr int nl = NL;
     const long tt = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (tt >= nt) return;
     const int *d = t2d + tt * nl;
     const double *M = mloc + (tt % 6) * nl * nl;
-    double m[20], dm[20], sm[20];
+    double dm[nl], sm[nl];
+#pragma unroll
     for (int l = 0; l < nl; ++l) {
         const int q = d[l];
         const bool in = !bnd[q];
-        m[l] = in ? (double)mu[q] : 0.0;
         dm[l] = in ? (double)vin[q] - (double)vout[q] : 0.0;
         sm[l] = in ? (double)vin[q] + (double)vout[q] : 0.0;
     }
     double a = 0.0, b = 0.0;
+    // rows rolled, columns unrolled: dm, sm are indexed only by the constant k
+#pragma unroll 1
     for (int l = 0; l < nl; ++l) {
+        const int q = d[l];
+        const double ml = !bnd[q] ? (double)mu[q] : 0.0;
         double ra = 0.0, rb = 0.0;
+#pragma unroll
         for (int k = 0; k < nl; ++k) {
             ra += M[l * nl + k] * dm[k];
             rb += M[l * nl + k] * sm[k];
         }
-        a += m[l] * ra;
-        b += m[l] * rb;
+        a += ml * ra;
+        b += ml * rb;
     }
     ge[tt] += eps0 * a;
     gs[tt] += mhalf * b;
@@ -1773,10 +1780,16 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
             k_comb<T><<<g, 256, 0, s->stream>>>(n, lV, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr,
                                                 s->b.as<T>());
             EMS_TRY(cg<T>(s, -1, s->g_mu));
-            k_tet_grad<T><<<tb, 128, 0, s->stream>>>(nt, nl, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
-                                                     s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
-                                                     s->d.eps, -0.5 * D, s->g_ge.as<double>(),
-                                                     s->g_gs.as<double>());
+            if (nl == 6)
+                k_tet_grad<T, 6><<<tb, 128, 0, s->stream>>>(nt, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
+                                                            s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
+                                                            s->d.eps, -0.5 * D, s->g_ge.as<double>(),
+                                                            s->g_gs.as<double>());
+            else
+                k_tet_grad<T, 20><<<tb, 128, 0, s->stream>>>(nt, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
+                                                             s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
+                                                             s->d.eps, -0.5 * D, s->g_ge.as<double>(),
+                                                             s->g_gs.as<double>());
             EMS_TRY(apply_op<T>(s, 0, s->g_mu.as<T>(), tmp));
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-D), tmp, lU);  // lU1
             EMS_TRY(apply_op<T>(s, 1, s->g_mu.as<T>(), tmp));
