@@ -27,6 +27,7 @@ struct ems_fem {
     // leapfrog: slot[s][comp]; symplectic: U = slot 0, P = slot 1, and the
     // ping-pong partners slot 2 (U), slot 3 (P)
     DevBuf f[5][3];  // slot 4: drifted field of a symplectic stage (order >= 2)
+    DevBuf etmp;     // orders 2, 3: per-element row dots [n][nelem] (k_fem_elem)
     int t0 = 0, t1 = 1, t2 = 2;  // leapfrog rotation
     std::vector<long> src_z;     // z sources (flat Ez index)
     int nsrc_xy = 0;             // x/y sources: stored by the reference, never injected
@@ -119,6 +120,19 @@ void launch_leapfrog(ems_fem *s, const double *sval)
                                                                      s->d_src.as<long>(), sval);
         return;
     }
+    if constexpr (P >= 2) {  // element by element: row dots once per element, then the gathers
+        const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
+        T *et = s->etmp.as<T>();
+        k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
+                                                                                  s->off.as<int>(), e1, et);
+        k_fem_leapfrog<T, P, 0, true><<<comp_grid<0>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
+        k_fem_leapfrog<T, P, 1, true><<<comp_grid<1>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
+        k_fem_leapfrog<T, P, 2, true><<<comp_grid<2>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
+        return;
+    }
     k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
                                                                      e2, ns, s->d_src.as<long>(), sval);
     k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
@@ -159,6 +173,24 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
             s->g, u, pm, c3<T>(s, 4), (T)(0.5 * deltat), (T)mm);
     }
     const CComp3<T> ud = cc3<T>(s, 4);
+    if constexpr (P >= 2) {  // element by element
+        const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
+        T *et = s->etmp.as<T>();
+        k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
+                                                                                  s->off.as<int>(), ud, et);
+        k_fem_symplectic<T, P, 0, true><<<comp_grid<0>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+            s->d_src.as<long>(), sval, et);
+        k_fem_symplectic<T, P, 1, true><<<comp_grid<1>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+            s->d_src.as<long>(), sval, et);
+        k_fem_symplectic<T, P, 2, true><<<comp_grid<2>(s), blk, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
+            s->d_src.as<long>(), sval, et);
+        swap_slots(s, 0, 2);
+        swap_slots(s, 1, 3);
+        return;
+    }
     k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
                                                                        uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
                                                                        ns, s->d_src.as<long>(), sval);
@@ -447,6 +479,7 @@ ems_status ems_fem_create(const ems_fem_desc *desc, const double *kmat, ems_fem 
         st = hip_fail(hipGetLastError(), "hipMemcpy", __FILE__, __LINE__);
     if (st == EMS_OK) st = desc->precision == EMS_F32 ? upload_kmat<float>(s) : upload_kmat<double>(s);
     const int nslots = desc->stepper == EMS_FEM_LEAPFROG ? 3 : desc->order >= 2 ? 5 : 4;
+    if (st == EMS_OK && desc->order >= 2) st = s->etmp.reserve((size_t)g.n * g.nex * g.ney * g.nez * s->esz);
     for (int sl = 0; sl < nslots && st == EMS_OK; ++sl)
         for (int cc = 0; cc < 3 && st == EMS_OK; ++cc) {
             st = s->f[sl][cc].reserve(comp_size(s, cc) * s->esz);

@@ -296,13 +296,64 @@ __device__ __forceinline__ T row_dot(const FemGrid &g, const T *__restrict__ K, 
 }
 
 // ---------------------------------------------------------------------------
+// Orders 2 and 3, element by element.  Phase 1 (k_fem_elem): one thread per element
+// loads its n field values once and stores all n row dots
+//   y_e[r] = sum_m K[r][m] x_e[m]   (summed from 0 over m, as row_dot)
+// at tmp[r * nelem + e]; the matrix is read at addresses uniform over the
+// wavefront.  Phase 2: the per-DOF kernels below with each element's row dot
+// read from tmp instead of gathered (EBE = true): the same bits, while the
+// per-DOF gather loads every element's n values once per DOF of it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long elem_slot(const FemGrid &g, int r, int i, int j, int k)
+{
+    const long nelem = (long)g.nex * g.ney * g.nez;
+    return r * nelem + i + (long)g.nex * (j + (long)g.ney * k);
+}
+
+// Workgroup size of k_fem_elem: at order 3 the 144 values of an element take
+// up to 288 registers, which one wave per workgroup can hold.
+template <int P> constexpr int kElemWG = P == 3 ? 64 : 256;
+
+template <typename T, int P>
+__global__ __launch_bounds__(kElemWG<P>) void k_fem_elem(FemGrid g, const T *__restrict__ K,
+                                                         const int *__restrict__ off, CComp3<T> x, T *__restrict__ tmp)
+{
+    constexpr int D = P * (P + 1) * (P + 1), N = 3 * D;
+    const long nelem = (long)g.nex * g.ney * g.nez;
+    const long e = (long)blockIdx.x * kElemWG<P> + threadIdx.x;
+    if (e >= nelem) return;
+    const int i = (int)(e % g.nex), j = (int)((e / g.nex) % g.ney), k = (int)(e / ((long)g.nex * g.ney));
+    long bx, by, bz;
+    elem_bases(g, i, j, k, bx, by, bz);
+    T v[N];
+#pragma unroll
+    for (int m = 0; m < D; ++m) v[m] = x.x[bx + off[m]];
+#pragma unroll
+    for (int m = D; m < 2 * D; ++m) v[m] = x.y[by + off[m]];
+#pragma unroll
+    for (int m = 2 * D; m < N; ++m) v[m] = x.z[bz + off[m]];
+    // rows rolled (one copy of the n-term dot), columns unrolled: v stays in
+    // registers
+#pragma unroll 1
+    for (int r = 0; r < N; ++r) {
+        const T *__restrict__ kr = K + r * N;
+        T t = 0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) t += kr[m] * v[m];
+        tmp[r * nelem + e] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // fem_v2 leapfrog step (src/fem_v2/fem_simulation.cpp:560-655 + BC):
 //   E2 = ((0 - sum_e K_e E1) + (2 E1 - E0)) - src,  PEC-tangential DOFs -> 0
 // ---------------------------------------------------------------------------
-template <typename T, int P, int COMP>
+// EBE: the row dots come from tmp (k_fem_elem) instead of being gathered.
+template <typename T, int P, int COMP, bool EBE = false>
 __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__restrict__ K, const int *__restrict__ off,
                                                       CComp3<T> e0, CComp3<T> e1, Comp3<T> e2, int nsrc,
-                                                      const long *__restrict__ src_idx, const double *__restrict__ src_val)
+                                                      const long *__restrict__ src_idx, const double *__restrict__ src_val,
+                                                      const T *__restrict__ etmp = nullptr)
 {
     int dx, dy, dz;
     comp_dims<COMP>(g, dx, dy, dz);
@@ -316,11 +367,23 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__rest
         out[idx] = (T)0;
         return;
     }
-    Cand c[4];
-    const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
     T acc = 0;
-    for (int q = 0; q < nc; ++q)
-        acc -= row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, e1, e1, (T)0, (T)1);
+    if constexpr (EBE) {  // one stored value per element: candidates in registers
+        int cd[4];
+        long rk[4];
+        order4<P>(g, COMP, X, Y, Z, cd, rk);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int i, j, k, r;
+            if (rk[q] == kNoRank || !elem_of<P>(g, COMP, X, Y, Z, cd[q], i, j, k, r)) continue;
+            acc -= etmp[elem_slot(g, r, i, j, k)];
+        }
+    } else {
+        Cand c[4];
+        const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
+        for (int q = 0; q < nc; ++q)
+            acc -= row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, e1, e1, (T)0, (T)1);
+    }
     const T *a1 = COMP == 0 ? e1.x : COMP == 1 ? e1.y : e1.z;
     const T *a0 = COMP == 0 ? e0.x : COMP == 1 ? e0.y : e0.z;
     T v = acc + ((T)2 * a1[idx] - a0[idx]);
@@ -354,14 +417,16 @@ __global__ __launch_bounds__(256) void k_fem_drift(FemGrid g, CComp3<T> u, CComp
     dc[i] = gval<T, true>(uc, pc, i, hd, mm);
 }
 
-// ud: the drifted field of k_fem_drift (gathered); u, pm: the DOF's own values.
-template <typename T, int P, int COMP>
+// ud: the drifted field of k_fem_drift (gathered); u, pm: the DOF's own
+// values.  EBE: the row dots of ud come from tmp (k_fem_elem).
+template <typename T, int P, int COMP, bool EBE = false>
 __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__restrict__ K,
                                                         const int *__restrict__ off, CComp3<T> u, CComp3<T> pm,
                                                         CComp3<T> ud, Comp3<T> u_out, Comp3<T> p_out, T deltat, T hd,
                                                         T mm,
                                                         int nsrc, const long *__restrict__ src_idx,
-                                                        const double *__restrict__ src_val)
+                                                        const double *__restrict__ src_val,
+                                                        const T *__restrict__ etmp = nullptr)
 {
     int dx, dy, dz;
     comp_dims<COMP>(g, dx, dy, dz);
@@ -377,20 +442,36 @@ __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__re
         po[idx] = (T)0;
         return;
     }
-    Cand c[4];
-    const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
     int nsrc_here = 0;
     if (COMP == 2)
         for (int s = 0; s < nsrc; ++s) nsrc_here += src_idx[s] == idx;
     const T sv = (T)src_val[0];
     T tmp = 0;
     bool src_done = nsrc_here == 0;
-    for (int q = 0; q < nc; ++q) {
-        if (!src_done && c[q].rank >= g.OE) {
-            for (int s = 0; s < nsrc_here; ++s) tmp += sv;
-            src_done = true;
+    if constexpr (EBE) {  // one stored value per element: candidates in registers
+        int cd[4];
+        long rk[4];
+        order4<P>(g, COMP, X, Y, Z, cd, rk);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            int i, j, k, r;
+            if (rk[q] == kNoRank || !elem_of<P>(g, COMP, X, Y, Z, cd[q], i, j, k, r)) continue;
+            if (!src_done && rk[q] >= g.OE) {
+                for (int s = 0; s < nsrc_here; ++s) tmp += sv;
+                src_done = true;
+            }
+            tmp += etmp[elem_slot(g, r, i, j, k)];
         }
-        tmp += row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, ud, ud, hd, mm);
+    } else {
+        Cand c[4];
+        const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
+        for (int q = 0; q < nc; ++q) {
+            if (!src_done && c[q].rank >= g.OE) {
+                for (int s = 0; s < nsrc_here; ++s) tmp += sv;
+                src_done = true;
+            }
+            tmp += row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, ud, ud, hd, mm);
+        }
     }
     if (!src_done)
         for (int s = 0; s < nsrc_here; ++s) tmp += sv;
@@ -753,7 +834,8 @@ __global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restric
     const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
     const bool act = X < g.Nx && Y < g.Ny;
     const T sv = (T)src_val[0];
-    // warm-up: planes Z0-1 .. Z0+1
+    // warm-up: planes Z0-1 .. Z0+1 (symplectic stages stage the drifted
+    // value; a separate drift pass measured slower at first order)
     for (int P = Z0 - 1; P <= Z0 + 1; ++P) {
         Z1Regs<T, BY, DRIFT> r;
         z1_fetch<T, BY, DRIFT>(g, u, pm, ox, oy, P, r);
