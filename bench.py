@@ -386,7 +386,8 @@ def main():
         "device_bytes": r["device_bytes"],
     }
     if not args.no_fem:
-        out["fem"] = [gpu_fem(2, "symplectic_4", 128, 20), gpu_fem(1, "fem_v2", 128, 50),
+        out["fem"] = [gpu_fem(2, "symplectic_4", 128, 20), gpu_fem(2, "symplectic_4", 128, 20, "f32"),
+                      gpu_fem(3, "fem_v2", 96, 20), gpu_fem(1, "fem_v2", 128, 50),
                       gpu_fem(1, "symplectic_2", 128, 50, "f32"), gpu_fem1("fem", 128, 50),
                       gpu_fem1("fem_abm", 128, 30)]
     if not args.no_fem:
