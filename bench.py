@@ -242,6 +242,143 @@ def gpu_batch_gradient(n=192, shots=16, steps=100):
                 dtype="fp32 fields, fp64 gradient")
 
 
+def _trace_errors(tr, ref, dt):
+    """Relative L2 of FP32 traces against FP64 ones, and the largest TF
+    envelope / phase misfit (Kristekova et al., emsolvers_amd.tfmisfit) over
+    the components carrying at least 1e-2 of the peak."""
+    from emsolvers_amd import tfmisfit
+    fin = np.isfinite(ref)
+    rel = float(np.linalg.norm((tr - ref)[fin]) / np.linalg.norm(ref[fin]))
+    nslot = ref.shape[1]
+    fmin, fmax = 2.0 / (nslot * dt), 0.25 / dt
+    floor = 1e-2 * np.abs(ref[fin]).max()
+    em = pm = 0.0
+    for o in range(ref.shape[0]):
+        for c in range(ref.shape[2]):
+            r = ref[o, :, c]
+            if not np.all(np.isfinite(r)) or not np.abs(r).max() > floor:
+                continue
+            m = tfmisfit.misfits(tr[o, :, c], r, dt, fmin, fmax, nf=40)
+            em, pm = max(em, m["EM"]), max(pm, m["PM"])
+    return dict(rel_l2=rel, tf_em=em, tf_pm=pm)
+
+
+def fp32_errors():
+    """FP32 against FP64 on the same inputs for each benchmark configuration
+    (the FP64 GPU path equals the FP64 oracle: bit for bit for the FDTD and
+    hex FEM, within 1e-10 for tetrahedra)."""
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.fdtd import FemSimulation as Fdtd, WAVELET_RICKER
+    from emsolvers_amd.tet import TetSimulation, KAPPA, STEPPERS
+    from emsolvers_amd.fem import FemSimulation as Hex
+    out = {}
+
+    # config 1: 64^3 FDTD, sphere eps_r 4 (r = 16), CPML 10, Ricker 10 GHz at
+    # Ez (16, 32, 32), 8 receivers on y = z = 32, 1000 steps
+    def cfg1(prec):
+        n, h = 64, 1e-3
+        s = Fdtd((n, n, n), h, wl.cfl_dt(h), wl.EPS0, wl.MU0, 1, precision=prec)
+        s.set_material(wl.sphere_model(n, 16.0, 4.0, (32, 32, 32)), None)
+        s.set_cpml(10, 3.0, 1e-6, 1.0, 0.0)
+        s.set_wavelet(WAVELET_RICKER, 10e9, -1.0, 1.0)
+        s.setSource_z((16, 32, 32))
+        for r in range(8):
+            p = (10 + int(round((r + 0.5) * 44 / 8)), 32, 32)
+            s.setObservationPoint([[p], [p], [p]])
+        s.run(1000)
+        return s.traces()
+    out["config1_fdtd_64_sphere_cpml"] = _trace_errors(cfg1("f32"), cfg1("f64"), wl.cfl_dt(1e-3))
+
+    # config 2: tetrahedra p = 1 on 40^3 cubes, symplectic_2 and ABM
+    def tet(n, order, stepper, prec, steps, sigma_fraction, dt=None):
+        h = 1.0 / n
+        er, sg = wl.tet_model(n, 5, sigma_fraction)
+        s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, order, prec)
+        s.set_material(er, sg)
+        if dt is None:
+            dt = 0.9 * KAPPA[STEPPERS[stepper]] / s.max_frequency(40)
+        s.set_time_step(dt)
+        f0 = 2.0 * wl.C0 / (h * n)
+        s.add_dipole([0.5 + 0.1 * h, 0.5 + 0.2 * h, 0.5 + 0.3 * h], [1.0, 0.0, 0.0], f0, 1.2 / f0, 1e-9)
+        for p in wl.tet_receivers(n, h, 16, 6):
+            s.add_receiver(p)
+        s.run(steps)
+        tr = s.traces()
+        s.close()
+        return tr, dt
+    for stepper in ("symplectic_2", "abm"):
+        ref, dt = tet(40, 1, stepper, "f64", 600, 0.0)
+        tr, _ = tet(40, 1, stepper, "f32", 600, 0.0, dt)
+        out["config2_tet_p1_40_" + stepper] = _trace_errors(tr, ref, dt)
+
+    # config 3: hex p = 2 (64^3 elements) and tetrahedra p = 2 (64^3 cubes, sigma
+    # on 20 % of the tetrahedra), symplectic_4
+    def hex2(prec):
+        kmat = os.path.join(ROOT, "tests", "golden", "kmat")
+        n, h = 128, 1e-3
+        dt = h / (wl.C0 * 3 ** 0.5) / 2.0
+        s = Hex(2, (n, n, n), h, dt, wl.EPS0, wl.MU0, 1, 1, stepper="symplectic_4", precision=prec, kmat_dir=kmat)
+        s.setSource_z((n // 2, n // 2, n // 2))
+        for d in (4, 12, 24):
+            s.setObservationPoint((n // 2 + d, n // 2, n // 2))
+        s.run(600)
+        return s.traces()[..., 1:], dt
+    ref, dt = hex2("f64")
+    out["config3_hex_p2_symplectic_4"] = _trace_errors(hex2("f32")[0], ref, dt)
+    # (the pulse peaks near step 230: 300 steps cover its arrival)
+    ref, dt = tet(64, 2, "symplectic_4", "f64", 300, 0.2)
+    tr, _ = tet(64, 2, "symplectic_4", "f32", 300, 0.2, dt)
+    out["config3_tet_p2_symplectic_4"] = _trace_errors(tr, ref, dt)
+
+    # config 4: one shot per model size of the batched workload (each batched
+    # shot equals its single run bit for bit)
+    for n in (96, 256):
+        er = wl.ellipsoid_model(n, n)
+        tr = {}
+        for prec in ("f32", "f64"):
+            s = _shot(n, 1000 * n, (er, None), precision=prec)
+            s.run(300)
+            tr[prec] = s.traces()
+        out["config4_fdtd_%d_ellipsoids_shot" % n] = _trace_errors(tr["f32"], tr["f64"], wl.cfl_dt(1e-3))
+
+    # config 5: misfit gradients with FP32 fields (FP64 accumulation) against
+    # FP64 fields; FDTD 192^3 blob model, one shot, 600 steps (the wave reaches
+    # the receiver plane after about 270); tetrahedral lossless symplectic_4
+    # on 40^3 cubes, 100 steps
+    n = 192
+    model = wl.blob_model(n, 7)
+    g = {}
+    for prec in ("f32", "f64"):
+        s = _shot(n, 0, model, receivers=(8, 8), z_rec=n - 20, src=(n // 2 - 45, n // 2 - 45, 20), f0=10e9,
+                  precision=prec)
+        J, ge, gs = s.gradient(600, np.zeros((64, 601, 3)))
+        g[prec] = (np.asarray(ge), np.asarray(gs))
+    out["config5_fdtd_192_gradient"] = dict(
+        rel_l2_eps=float(np.linalg.norm(g["f32"][0] - g["f64"][0]) / np.linalg.norm(g["f64"][0])),
+        rel_l2_sigma=float(np.linalg.norm(g["f32"][1] - g["f64"][1]) / np.linalg.norm(g["f64"][1])))
+    gt = {}
+    dt_t = None
+    for prec in ("f64", "f32"):
+        h = 1.0 / 40
+        er, _ = wl.tet_model(40, 5, 0.0)
+        s = TetSimulation((40, 40, 40), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, "symplectic_4", 1, prec)
+        s.set_material(er, np.zeros_like(er))
+        if dt_t is None:  # the FP64 handle's stable step, reused for FP32
+            dt_t = 0.9 * KAPPA[STEPPERS["symplectic_4"]] / s.max_frequency(40)
+        s.set_time_step(dt_t)
+        f0 = 2.0 * wl.C0 / (h * 40)
+        s.add_dipole([0.5 + 0.1 * h, 0.5 + 0.2 * h, 0.5 + 0.3 * h], [1.0, 0.0, 0.0], f0, 1.2 / f0, 1e-9)
+        for p in wl.tet_receivers(40, h, 16, 6):
+            s.add_receiver(p)
+        J, ge, gs, drift = s.gradient(100, np.zeros((16, 101, 3)))
+        gt[prec] = (np.asarray(ge), np.asarray(gs))
+        s.close()
+    out["config5_tet_p1_40_gradient"] = dict(
+        rel_l2_eps=float(np.linalg.norm(gt["f32"][0] - gt["f64"][0]) / np.linalg.norm(gt["f64"][0])),
+        rel_l2_sigma=float(np.linalg.norm(gt["f32"][1] - gt["f64"][1]) / np.linalg.norm(gt["f64"][1])))
+    return out
+
+
 def pmc_traffic(path):
     """HBM bytes per step from a committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a device copy of
@@ -319,6 +456,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched forward / gradient figures")
     ap.add_argument("--fem-profile", action="store_true",
                     help="run only short FEM legs (tet configs 2 and 3, fem_v2 and fem at 128^3), for rocprofv3")
+    ap.add_argument("--fp32-errors", action="store_true",
+                    help="FP32 against FP64 traces and gradients per benchmark configuration, then exit")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="first run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -331,6 +470,9 @@ def main():
         print("bench.py: multi-rank launch is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.fp32_errors:
+        print(json.dumps({"metric": "FP32 error against FP64", "configs": fp32_errors()}), flush=True)
+        return
     if args.fem_profile:
         legs = [gpu_tet(40, "symplectic_2", 50, "f64"),
                 gpu_tet(64, "symplectic_4", 2, "f32", order=2, sigma_fraction=0.2, power_iters=10),

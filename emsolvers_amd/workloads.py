@@ -40,6 +40,17 @@ def ellipsoid_model(n, seed):
     return er.reshape(-1)
 
 
+def sphere_model(n, radius=16.0, eps_r=4.0, centre=None):
+    """eps_r on the material grid: a sphere of the given radius (cells) and
+    eps_r in vacuum, centred at (n/2, n/2, n/2) unless given; sampled at the
+    cell centres like the other models."""
+    c = np.full(3, n / 2.0) if centre is None else np.asarray(centre, dtype=np.float64)
+    c1 = np.arange(n + 1, dtype=np.float64) - 0.5
+    d2 = ((c1 - c[2]) ** 2)[:, None, None] + ((c1 - c[1]) ** 2)[None, :, None] + ((c1 - c[0]) ** 2)[None, None, :]
+    er = np.where(d2 <= radius * radius, eps_r, 1.0)
+    return er.reshape(-1)
+
+
 def blob_model(n, seed, background=2.0, count=3, d_eps=0.5, width=12.0, d_sigma=0.01):
     """(eps_r, sigma) on the material grid: background eps_r plus `count`
     seeded Gaussian blobs of amplitude d_eps (eps_r) and d_sigma (S/m) and
