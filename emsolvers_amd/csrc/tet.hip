@@ -156,12 +156,17 @@ __device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict_
 // Phase 1 modes: MASS y = (eps0 eps_r + hh sigma) M x1;  STIFF y = S x1;
 // RHS y = (eps0 eps_r - hh sigma) M x1 - D S x2;  FORCE y = -S x2 - sigma M x1
 // (S already divided by mu; x1 = V, x2 = U in the stepper modes).
+// First order applies S = (V/mu) C^T C as the element curl C x followed by
+// (V/mu) C^T (Ct: [type][C | (V/mu) C], 3 x 6 each): a curl-free field then
+// gives a curl that is zero up to the rounding of C x alone, so the rounding
+// of S x no longer leaks ~ (V/mu) |C|^2 |x| into the curl-free null space,
+// which no oscillation removes (in FP32 it made the fields drift).
 enum { EBE_MASS = 0, EBE_STIFF = 1, EBE_RHS = 2, EBE_FORCE = 3 };
 
 template <typename T, int NL, int MODE>
 __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restrict__ Mt, const T *__restrict__ St,
                                                    const double *__restrict__ er, const double *__restrict__ sg,
-                                                   double e0, double hh, double D,
+                                                   const T *__restrict__ Ct, double e0, double hh, double D,
                                                    const unsigned *__restrict__ tmask, const T *__restrict__ x1,
                                                    const T *__restrict__ x2, T *__restrict__ tmp, const int *done)
 {
@@ -183,6 +188,19 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
         b[l] = (USE2 && in) ? x2[d] : (T)0;
     }
     const T *M = Mt + t * NL * NL, *S = St + t * NL * NL;
+    // first order: the element curl of the stiffness operand (x1 in STIFF,
+    // x2 in RHS and FORCE)
+    T cu[3] = {0, 0, 0};
+    if constexpr (NL == 6 && MODE != EBE_MASS) {
+        const T *C = Ct + t * 36;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            T v = 0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) v += C[k * 6 + q] * (MODE == EBE_STIFF ? a[q] : b[q]);
+            cu[k] = v;
+        }
+    }
     T cm;
     if (MODE == EBE_MASS) cm = (T)(e0 * epsr + hh * sig);
     else if (MODE == EBE_RHS) cm = (T)(e0 * epsr - hh * sig);
@@ -190,7 +208,10 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
 #pragma unroll 4
     for (int l = 0; l < NL; ++l) {
         T y;
-        if (MODE == EBE_STIFF) {
+        if (MODE == EBE_STIFF && NL == 6) {
+            const T *Cs = Ct + t * 36 + 18;
+            y = (Cs[l] * cu[0] + Cs[6 + l] * cu[1]) + Cs[12 + l] * cu[2];
+        } else if (MODE == EBE_STIFF) {
             T s = 0;
 #pragma unroll
             for (int q = 0; q < NL; ++q) s += S[l * NL + q] * a[q];
@@ -202,10 +223,17 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
             y = cm * s;
         } else {
             T sm = 0, ss = 0;
+            if constexpr (NL == 6) {
+                const T *Cs = Ct + t * 36 + 18;
+                ss = (Cs[l] * cu[0] + Cs[6 + l] * cu[1]) + Cs[12 + l] * cu[2];
 #pragma unroll
-            for (int q = 0; q < NL; ++q) {
-                sm += M[l * NL + q] * a[q];
-                ss += S[l * NL + q] * b[q];
+                for (int q = 0; q < NL; ++q) sm += M[l * NL + q] * a[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NL; ++q) {
+                    sm += M[l * NL + q] * a[q];
+                    ss += S[l * NL + q] * b[q];
+                }
             }
             y = MODE == EBE_RHS ? cm * sm - (T)D * ss : -ss - cm * sm;
         }
@@ -782,7 +810,7 @@ struct ems_tet {
     bool ebe = false;   // element-by-element operator (no assembled matrices)
     bool mfma = false;  // element products of the mass operator on matrix cores
     EbeMesh em{};
-    DevBuf d_inc, d_Mt, d_St, d_er, d_sg, d_bnd, d_tmp, d_tmask;
+    DevBuf d_inc, d_Mt, d_St, d_Ct, d_er, d_sg, d_bnd, d_tmp, d_tmask;
     int nl = 6;   // basis functions per tetrahedron
     long nd = 0;  // dofs
     bool assembled = false, lossy = false;
@@ -1206,6 +1234,20 @@ template <typename T> ems_status setup_ebe(ems_tet *s)
     std::vector<double> Mt(6 * nl * nl), St(6 * nl * nl);
     for (int t = 0; t < 6; ++t) element_matrices(order, t, m.h, &Mt[t * nl * nl], &St[t * nl * nl]);
     for (double &v : St) v /= s->d.mu;
+    {  // first order: element curls C and (V/mu) C per type (S = (V/mu) C^T C)
+        std::vector<double> ct(6 * 36, 0.0);
+        if (order == 1)
+            for (int t = 0; t < 6; ++t) {
+                double C[3][6], V;
+                whitney_curls(t, m.h, C, V);
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 6; ++l) {
+                        ct[t * 36 + k * 6 + l] = C[k][l];
+                        ct[t * 36 + 18 + k * 6 + l] = C[k][l] * (V / s->d.mu);
+                    }
+            }
+        EMS_TRY(upload<T>(s->d_Ct, ct));
+    }
     EMS_TRY(s->d_inc.reserve(sizeof(int) * inc.size()));
     EMS_HIP(hipMemcpy(s->d_inc.p, inc.data(), sizeof(int) * inc.size(), hipMemcpyHostToDevice));
     EMS_TRY(upload<T>(s->d_Mt, Mt));
@@ -1340,7 +1382,8 @@ void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const 
     with_nl(s->nl, [&](auto N) {
         constexpr int NL = decltype(N)::value;
         k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
-                                                             s->d.eps, hh, D, s->d_tmask.as<unsigned>(), x1, x2,
+                                                             s->d_Ct.as<T>(), s->d.eps, hh, D, s->d_tmask.as<unsigned>(),
+                                                             x1, x2,
                                                              s->d_tmp.as<T>(), done);
     });
 }

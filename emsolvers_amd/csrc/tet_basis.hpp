@@ -131,6 +131,20 @@ inline int element_matrices(int order, int t, double h, double *M, double *S)
     return nl;
 }
 
+// First order: the curls of the 6 Whitney functions of type t are constant,
+// curl w_ab = 2 grad l_a x grad l_b; C[k][l] is component k of the curl of
+// local function l, so that the element curl-curl matrix is S = V C^T C.
+inline void whitney_curls(int t, double h, double C[3][6], double &V)
+{
+    double G[4][3];
+    bary(t, h, G, V);
+    for (int l = 0; l < 6; ++l) {
+        double c[3];
+        cross3(G[kLocEdge[l][0]], G[kLocEdge[l][1]], c);
+        for (int k = 0; k < 3; ++k) C[k][l] = 2.0 * c[k];
+    }
+}
+
 // values of the nl basis functions at barycentric point lam
 inline int basis_at(int order, int t, double h, const double lam[4], double (*w)[3])
 {

@@ -356,3 +356,22 @@ def test_adjoint_gradient_matches_central_differences(order, stepper, lossy):
         fd = (Jat(h) - Jat(-h)) / (2 * h)
         g = (ge if which == "eps" else gs) @ v
         assert abs(fd - g) <= 1e-6 * max(abs(fd), abs(g)) + 1e-14, (which, fd, g)
+
+
+def test_whitney_stiffness_is_a_product_of_element_curls():
+    """First order: curl w_ab = 2 grad l_a x grad l_b is constant on a
+    tetrahedron, so the quadrature curl-curl matrix equals V C^T C (the
+    factorized form the device applies, curl then transposed curl).  Pinned
+    for all six Kuhn types against the oracle's quadrature matrices, and C
+    annihilates the gradients of the vertex functions (the null space)."""
+    from oracle import tet as ot
+    h = 0.37
+    for t in range(6):
+        G, V = ot.bary_grad(ot.tet_vertices(t) * h)
+        C = np.stack([2.0 * np.cross(G[a], G[b]) for a, b in ot.LOC_EDGE], axis=1)  # 3 x 6
+        M, S = ot.element_matrices(t, h, 1)
+        np.testing.assert_allclose(V * C.T @ C, S, rtol=1e-12, atol=1e-12 * np.abs(S).max())
+        # gradient of vertex function v as edge coefficients: (l_b - l_a)(grad) -> +-1
+        for v in range(4):
+            g = np.array([(1.0 if b == v else 0.0) - (1.0 if a == v else 0.0) for a, b in ot.LOC_EDGE])
+            assert np.abs(C @ g).max() < 1e-12 * np.abs(C).max()
