@@ -154,19 +154,18 @@ __device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict_
 }
 
 // Phase 1 modes: MASS y = (eps0 eps_r + hh sigma) M x1;  STIFF y = S x1;
-// RHS y = (eps0 eps_r - hh sigma) M x1 - D S x2;  FORCE y = -S x2 - sigma M x1
-// (S already divided by mu; x1 = V, x2 = U in the stepper modes).
+// FORCE y = -S x2 - sigma M x1 (S already divided by mu; x1 = V, x2 = U).
 // First order applies S = (V/mu) C^T C as the element curl C x followed by
 // (V/mu) C^T (Ct: [type][C | (V/mu) C], 3 x 6 each): a curl-free field then
 // gives a curl that is zero up to the rounding of C x alone, so the rounding
 // of S x no longer leaks ~ (V/mu) |C|^2 |x| into the curl-free null space,
 // which no oscillation removes (in FP32 it made the fields drift).
-enum { EBE_MASS = 0, EBE_STIFF = 1, EBE_RHS = 2, EBE_FORCE = 3 };
+enum { EBE_MASS = 0, EBE_STIFF = 1, EBE_FORCE = 3 };
 
 template <typename T, int NL, int MODE>
 __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restrict__ Mt, const T *__restrict__ St,
                                                    const double *__restrict__ er, const double *__restrict__ sg,
-                                                   const T *__restrict__ Ct, double e0, double hh, double D,
+                                                   const T *__restrict__ Ct, double e0, double hh,
                                                    const unsigned *__restrict__ tmask, const T *__restrict__ x1,
                                                    const T *__restrict__ x2, T *__restrict__ tmp, const int *done)
 {
@@ -178,7 +177,7 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     const long tid = 6 * cube + t;
     const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
     const unsigned bm = tmask[tid];  // bit l: local dof l is a boundary dof
-    constexpr bool USE2 = MODE == EBE_RHS || MODE == EBE_FORCE;
+    constexpr bool USE2 = MODE == EBE_FORCE;
     T a[NL], b[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
@@ -189,7 +188,7 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     }
     const T *M = Mt + t * NL * NL, *S = St + t * NL * NL;
     // first order: the element curl of the stiffness operand (x1 in STIFF,
-    // x2 in RHS and FORCE)
+    // x2 in FORCE)
     T cu[3] = {0, 0, 0};
     if constexpr (NL == 6 && MODE != EBE_MASS) {
         const T *C = Ct + t * 36;
@@ -203,7 +202,6 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     }
     T cm;
     if (MODE == EBE_MASS) cm = (T)(e0 * epsr + hh * sig);
-    else if (MODE == EBE_RHS) cm = (T)(e0 * epsr - hh * sig);
     else cm = (T)sig;
 #pragma unroll 4
     for (int l = 0; l < NL; ++l) {
@@ -235,7 +233,7 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
                     ss += S[l * NL + q] * b[q];
                 }
             }
-            y = MODE == EBE_RHS ? cm * sm - (T)D * ss : -ss - cm * sm;
+            y = -ss - cm * sm;
         }
         tmp[(long)(t * NL + l) * m.ncube + cube] = y;
     }
@@ -408,29 +406,7 @@ __global__ __launch_bounds__(256) void k_spmv(Csr<T> A, const T *__restrict__ va
     if (lane == 0) y[r] = v;
 }
 
-// b = Me V - h Ms V - D S U (lossless: no Ms term)
-template <int LPR, typename T, bool LOSSY>
-__global__ __launch_bounds__(256) void k_rhs(Csr<T> A, const T *__restrict__ me, const T *__restrict__ ms,
-                                             const T *__restrict__ st, const T *__restrict__ U,
-                                             const T *__restrict__ V, T hh, T D, T *__restrict__ b)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = t / LPR, lane = t % LPR;
-    if (r >= A.n) return;
-    T a = 0, c = 0, s = 0;
-    for (int q = A.rp[r] + lane; q < A.rp[r + 1]; q += LPR) {
-        const int j = A.col[q];
-        a += me[q] * V[j];
-        if (LOSSY) c += ms[q] * V[j];
-        s += st[q] * U[j];
-    }
-    a = lanes_sum<LPR>(a);
-    s = lanes_sum<LPR>(s);
-    if (LOSSY) c = lanes_sum<LPR>(c);
-    if (lane == 0) b[r] = LOSSY ? (a - hh * c) - D * s : a - D * s;
-}
-
-// b = -S U - Ms V (the ABM right-hand side)
+// b = -S U - Ms V (the force of the first-order system)
 template <int LPR, typename T, bool LOSSY>
 __global__ __launch_bounds__(256) void k_force(Csr<T> A, const T *__restrict__ ms, const T *__restrict__ st,
                                                const T *__restrict__ U, const T *__restrict__ V, T *__restrict__ b)
@@ -679,23 +655,12 @@ template <typename T> __global__ void k_axpy(int n, T a, const T *__restrict__ x
     if (i < n) y[i] = y[i] + a * x[i];
 }
 
-// Initial guess of a stage's mass solve: keep V, then V += ratio dVprev (the
-// previous stage's increment scaled by the ratio of the stage steps)
+// x = y + r (y - yp): linear extrapolation of the stage solutions in time
 template <typename T>
-__global__ void k_guess(int n, T *V, T *Vold, const T *__restrict__ dV, T ratio)
+__global__ void k_extrap(int n, const T *__restrict__ y, const T *__restrict__ yp, T r, T *__restrict__ x)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const T v = V[i];
-    Vold[i] = v;
-    if (ratio != (T)0) V[i] = v + ratio * dV[i];
-}
-
-// dV = V - Vold
-template <typename T> __global__ void k_delta(int n, const T *__restrict__ V, const T *__restrict__ Vold, T *dV)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dV[i] = V[i] - Vold[i];
+    if (i < n) x[i] = y[i] + r * (y[i] - yp[i]);
 }
 
 // y = c0 + k (a1 x1 + a2 x2 + a3 x3), in that order
@@ -799,11 +764,13 @@ struct ems_tet {
     DevBuf d_dinv_me;  // Jacobi preconditioner of M_eps alone
     DevBuf U, V, b, r, z, p, ap;        // state and CG scratch
     DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
-    // symplectic stages: the previous stage's increment of V and its step,
-    // for the second-order initial guess of the next mass solve
-    DevBuf Vold, dVprev;
-    double d_prev = 0.0;
-    bool have_prev = false;
+    // symplectic stages solve for y = (V' - V) / D, an acceleration at the
+    // stage midpoint: Y, Yp keep the last two (at times ty, typ; have_prev
+    // counts the valid ones) for the next solve's initial guess, X is the
+    // solve's buffer
+    DevBuf Y, Yp, X;
+    double ty = 0.0, typ = 0.0;
+    int have_prev = 0;
     DevBuf d_partial, d_cg;
     int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
     int op_kind = EMS_TET_OP_AUTO;
@@ -1364,7 +1331,7 @@ template <typename F> void with_nl(int nl, F &&f)
 
 // phase 1 of the element-by-element operator
 template <typename T, int MODE>
-void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const int *done)
+void ebe_local(ems_tet *s, double hh, const T *x1, const T *x2, const int *done)
 {
     const dim3 grid((unsigned)((s->em.ncube + 255) / 256), 6);
     const double *er = s->eps_r.empty() ? nullptr : s->d_er.as<double>();
@@ -1382,7 +1349,7 @@ void ebe_local(ems_tet *s, double hh, double D, const T *x1, const T *x2, const 
     with_nl(s->nl, [&](auto N) {
         constexpr int NL = decltype(N)::value;
         k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
-                                                             s->d_Ct.as<T>(), s->d.eps, hh, D, s->d_tmask.as<unsigned>(),
+                                                             s->d_Ct.as<T>(), s->d.eps, hh, s->d_tmask.as<unsigned>(),
                                                              x1, x2,
                                                              s->d_tmp.as<T>(), done);
     });
@@ -1398,8 +1365,8 @@ template <typename T> ems_status apply_op(ems_tet *s, int which, const T *x, T *
 {
     const int n = (int)s->nd;
     if (s->ebe) {
-        if (which == 0) ebe_local<T, EBE_STIFF>(s, 0.0, 0.0, x, nullptr, nullptr);
-        else ebe_local<T, EBE_MASS>(s, 0.0, 0.0, x, nullptr, nullptr);
+        if (which == 0) ebe_local<T, EBE_STIFF>(s, 0.0, x, nullptr, nullptr);
+        else ebe_local<T, EBE_MASS>(s, 0.0, x, nullptr, nullptr);
         k_apply<EbeOp<T>, T><<<(n + 255) / 256, 256, 0, s->stream>>>(ebe_op<T>(s, which), x, y);
     } else {
         const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
@@ -1432,10 +1399,10 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     if (s->ebe) {
         const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
         const EbeOp<T> op = ebe_op<T>(s, 1);
-        ebe_local<T, EBE_MASS>(s, hh, 0.0, xx, nullptr, nullptr);
+        ebe_local<T, EBE_MASS>(s, hh, xx, nullptr, nullptr);
         k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
         if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 0, 3, nb);
-        ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, nullptr);
+        ebe_local<T, EBE_MASS>(s, hh, z, nullptr, nullptr);
         k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
         if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
         EMS_HIP(hipGetLastError());
@@ -1443,7 +1410,7 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
             const int par = it & 1;
             k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
             if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
-            ebe_local<T, EBE_MASS>(s, hh, 0.0, z, nullptr, &st->done);
+            ebe_local<T, EBE_MASS>(s, hh, z, nullptr, &st->done);
             k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
             if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
             if ((it + 1) % check == 0) {
@@ -1502,59 +1469,20 @@ template <typename T> void load(ems_tet *s, DevBuf &b, const double *f)
     if (nw) k_load<T><<<1, 64, 0, s->stream>>>(b.as<T>(), nw, s->nl, s->d_dip_dof.as<int>(), s->d_dip_w.as<double>(), f);
 }
 
-// load scale per dipole weight at time t: -D p'(t) (stage) or -p'(t) (ABM)
+// load scale per dipole weight at time t: -p'(t) (the force of stages and ABM)
 void dipole_factors(const ems_tet *s, double t, double scale, std::vector<double> &out)
 {
     const int nd = (int)s->dip_par.size() / 3;
     for (int q = 0; q < nd; ++q) out.push_back(scale * pulse_rate(t, s->dip_par[3 * q], s->dip_par[3 * q + 1], s->dip_par[3 * q + 2]));
 }
 
-template <typename T> ems_status stage(ems_tet *s, int q, double D, const double *fdev)
-{
-    const int n = (int)s->nd, g = (n + 255) / 256, gr = row_blocks(n, s->lpr);
-    const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
-    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
-    if (s->ebe) {
-        ebe_local<T, EBE_RHS>(s, 0.5 * D, D, s->V.as<T>(), s->U.as<T>(), nullptr);
-        k_ebe_rhs<T><<<g, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(),
-                                               s->d_bnd.as<unsigned char>(), s->V.as<T>(), 1, s->b.as<T>());
-    } else with_lpr(s->lpr, [&](auto L) {
-        constexpr int LPR = decltype(L)::value;
-        if (s->lossy)
-            k_rhs<LPR, T, true><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), s->d_Ms.as<T>(), s->d_S.as<T>(),
-                                                           s->U.as<T>(), s->V.as<T>(), (T)(0.5 * D), (T)D,
-                                                           s->b.as<T>());
-        else
-            k_rhs<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, s->d_Me.as<T>(), nullptr, s->d_S.as<T>(),
-                                                            s->U.as<T>(), s->V.as<T>(), (T)(0.5 * D), (T)D,
-                                                            s->b.as<T>());
-    });
-    load<T>(s, s->b, fdev);
-    // initial guess V + (D / D_prev) dV_prev: V changes over a stage by about
-    // D times the acceleration, so the guess is second order in the step and
-    // the solve starts closer (the answer is the same to the CG tolerance)
-    if (!s->Vold.p) {
-        EMS_TRY(s->Vold.reserve(sizeof(T) * n));
-        EMS_TRY(s->dVprev.reserve(sizeof(T) * n));
-        s->have_prev = false;
-    }
-    const T ratio = s->have_prev && s->d_prev != 0.0 ? (T)(D / s->d_prev) : (T)0;
-    k_guess<T><<<g, 256, 0, s->stream>>>(n, s->V.as<T>(), s->Vold.as<T>(), s->dVprev.as<T>(), ratio);
-    EMS_TRY(cg<T>(s, q, s->V));
-    k_delta<T><<<g, 256, 0, s->stream>>>(n, s->V.as<T>(), s->Vold.as<T>(), s->dVprev.as<T>());
-    s->d_prev = D;
-    s->have_prev = true;
-    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
-    return EMS_OK;
-}
-
-// a = Me^-1 (-S U - Ms V + load) into dst (initial guess: dst)
-template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
+// b = -S U - Ms V + load (the force of the first-order system)
+template <typename T> void force(ems_tet *s, const T *U, const T *V, const double *fdev)
 {
     const int n = (int)s->nd, gr = row_blocks(n, s->lpr);
     const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
     if (s->ebe) {
-        ebe_local<T, EBE_FORCE>(s, 0.0, 0.0, V, U, nullptr);
+        ebe_local<T, EBE_FORCE>(s, 0.0, V, U, nullptr);
         k_ebe_rhs<T><<<(n + 255) / 256, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(),
                                                              s->d_bnd.as<unsigned char>(), V, 0, s->b.as<T>());
     } else with_lpr(s->lpr, [&](auto L) {
@@ -1565,7 +1493,55 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
             k_force<LPR, T, false><<<gr, 256, 0, s->stream>>>(A, nullptr, s->d_S.as<T>(), U, V, s->b.as<T>());
     });
     load<T>(s, s->b, fdev);
+}
+
+// a = Me^-1 (-S U - Ms V + load) into dst (initial guess: dst)
+template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
+{
+    force<T>(s, U, V, fdev);
     return cg<T>(s, 0, dst);
+}
+
+// One symplectic stage of step D, midpoint tm in units of dt (load factors -p'
+// at the midpoint):
+//   U += D/2 V;  (M_eps + D/2 M_sigma) V' = (M_eps - D/2 M_sigma) V - D (S U - load);
+//   U += D/2 V'
+// solved for the increment: (M_eps + D/2 M_sigma) y = -S U - M_sigma V + load,
+// V' = V + D y.  The solve's tolerance is then relative to the increment
+// rather than to V (in FP32 the error of solving for V' accumulated over the
+// steps).  Initial guess: the last two stage solutions extrapolated linearly
+// to tm (second order) when the span is moderate, else the last one.
+template <typename T> ems_status stage(ems_tet *s, int q, double D, double tm, const double *fdev)
+{
+    const int n = (int)s->nd, g = (n + 255) / 256;
+    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    force<T>(s, s->U.as<T>(), s->V.as<T>(), fdev);
+    if (!s->X.p) {
+        EMS_TRY(s->Y.reserve(sizeof(T) * n));
+        EMS_TRY(s->Yp.reserve(sizeof(T) * n));
+        EMS_TRY(s->X.reserve(sizeof(T) * n));
+        s->have_prev = 0;
+    }
+    // extrapolate only over moderate spans: symplectic_4's stage midpoints
+    // (0.675, 0.5, 0.325 of a step) would give a ratio of -7.7 into the next
+    // step, which amplifies the solutions' differences instead of predicting
+    const double r = s->have_prev >= 2 && s->ty != s->typ ? (tm - s->ty) / (s->ty - s->typ) : 0.0;
+    if (s->have_prev >= 2 && std::fabs(r) <= 1.5)
+        k_extrap<T><<<g, 256, 0, s->stream>>>(n, s->Y.as<T>(), s->Yp.as<T>(), (T)r, s->X.as<T>());
+    else if (s->have_prev >= 1)
+        EMS_HIP(hipMemcpyAsync(s->X.p, s->Y.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+    else
+        EMS_HIP(hipMemsetAsync(s->X.p, 0, sizeof(T) * n, s->stream));
+    EMS_TRY(cg<T>(s, q, s->X));
+    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)D, s->X.as<T>(), s->V.as<T>());
+    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    // history: Yp <- Y <- X
+    swap_dev(s->Yp, s->Y);
+    swap_dev(s->Y, s->X);
+    s->typ = s->ty;
+    s->ty = tm;
+    s->have_prev = std::min(s->have_prev + 1, 2);
+    return EMS_OK;
 }
 
 // Largest eigenvalue of M_eps^-1 S by power iteration from a fixed start
@@ -1641,9 +1617,9 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
             dipole_factors(s, t + dt, -1.0, f);  // F(y_n+1)
         } else if (s4) {
             const double Dq[3] = {x1 * dt, x2 * dt, x1 * dt}, Oq[3] = {0.0, x1 * dt, (x1 + x2) * dt};
-            for (int q = 0; q < 3; ++q) dipole_factors(s, t + Oq[q] + 0.5 * Dq[q], -Dq[q], f);
+            for (int q = 0; q < 3; ++q) dipole_factors(s, t + Oq[q] + 0.5 * Dq[q], -1.0, f);
         } else {
-            dipole_factors(s, t + 0.5 * dt, -dt, f);
+            dipole_factors(s, t + 0.5 * dt, -1.0, f);
         }
         t += dt;
     }
@@ -1698,11 +1674,14 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
             EMS_HIP(hipMemcpyAsync(s->an.p, s->am.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
             EMS_TRY(accel<T>(s, s->U.as<T>(), s->V.as<T>(), s->an, fnext()));
         } else if (s4) {
-            EMS_TRY(stage<T>(s, 0, x1 * dt, fnext()));
-            EMS_TRY(stage<T>(s, 1, x2 * dt, fnext()));
-            EMS_TRY(stage<T>(s, 0, x1 * dt, fnext()));
+            // stage midpoints in units of dt from the absolute step index, so
+            // that split runs extrapolate exactly as one run
+            const double tn = (double)(s0 + st - 1);
+            EMS_TRY(stage<T>(s, 0, x1 * dt, tn + 0.5 * x1, fnext()));
+            EMS_TRY(stage<T>(s, 1, x2 * dt, tn + x1 + 0.5 * x2, fnext()));
+            EMS_TRY(stage<T>(s, 0, x1 * dt, tn + (x1 + x2) + 0.5 * x1, fnext()));
         } else {
-            EMS_TRY(stage<T>(s, 0, dt, fnext()));
+            EMS_TRY(stage<T>(s, 0, dt, (double)(s0 + st - 1) + 0.5, fnext()));
         }
         if (nobs && (s0 + st) % tf == 0)
             k_tet_record<T><<<(nobs + 63) / 64, 64, 0, s->stream>>>(
@@ -1792,7 +1771,7 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     s->steps = 0;
     s->nslot = 0;
     s->traces.assign(0, 0.0);
-    s->have_prev = false;
+    s->have_prev = 0;
     EMS_TRY(run_impl<T>(s, N, nullptr));
     const int nslot = N / tf + 1;
     std::vector<double> res((size_t)nslot * nobs * 3, 0.0);  // [slot][obs][3]
@@ -1888,7 +1867,7 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     EMS_HIP(hipMemset(s->V.p, 0, vb));
     s->t_now = 0.0;
     s->steps = 0;
-    s->have_prev = false;
+    s->have_prev = 0;
     return EMS_OK;
 }
 
@@ -2121,7 +2100,7 @@ ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t cou
         EMS_HIP(hipMemcpy(b.p, t.data(), count * 4, hipMemcpyHostToDevice));
     }
     s->abm_started = false;
-    s->have_prev = false;  // a new state: no stage increment to extrapolate
+    s->have_prev = 0;  // a new state: no stage solution to extrapolate
     return EMS_OK;
 }
 
