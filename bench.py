@@ -476,7 +476,8 @@ def main():
     if args.fem_profile:
         legs = [gpu_tet(40, "symplectic_2", 50, "f64"),
                 gpu_tet(64, "symplectic_4", 2, "f32", order=2, sigma_fraction=0.2, power_iters=10),
-                gpu_fem(1, "fem_v2", 128, 20), gpu_fem1("fem", 128, 20)]
+                gpu_fem(1, "fem_v2", 128, 20), gpu_fem(2, "symplectic_4", 128, 10), gpu_fem1("fem", 128, 20),
+                gpu_fem1("fem_abm", 128, 10)]
         print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
         return
     if args.copy_calibrate:
