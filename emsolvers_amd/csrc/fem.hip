@@ -125,20 +125,13 @@ void launch_leapfrog(ems_fem *s, const double *sval)
         T *et = s->etmp.as<T>();
         k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
                                                                                   s->off.as<int>(), e1, et);
-        k_fem_leapfrog<T, P, 0, true><<<comp_grid<0>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
-        k_fem_leapfrog<T, P, 1, true><<<comp_grid<1>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
-        k_fem_leapfrog<T, P, 2, true><<<comp_grid<2>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), e0, e1, e2, ns, s->d_src.as<long>(), sval, et);
-        return;
+        k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, e0, e1, e2, ns, s->d_src.as<long>(), sval,
+                                                                         et);
+        k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, e0, e1, e2, ns, s->d_src.as<long>(), sval,
+                                                                         et);
+        k_fem_leapfrog<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, e0, e1, e2, ns, s->d_src.as<long>(), sval,
+                                                                         et);
     }
-    k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
-                                                                     e2, ns, s->d_src.as<long>(), sval);
-    k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
-                                                                     e2, ns, s->d_src.as<long>(), sval);
-    k_fem_leapfrog<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), e0, e1,
-                                                                     e2, ns, s->d_src.as<long>(), sval);
 }
 
 template <typename T, int P>
@@ -166,42 +159,24 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
         swap_slots(s, 1, 3);
         return;
     }
-    // the drifted field once per stage, then the gathers read it
-    {
+    if constexpr (P >= 2) {  // the drifted field once per stage, then element by element
         const long nmax = std::max({s->g.NX, s->g.NY, s->g.NZ});
         k_fem_drift<T><<<dim3((unsigned)((nmax + 255) / 256), 3, 1), 256, 0, s->stream>>>(
             s->g, u, pm, c3<T>(s, 4), (T)(0.5 * deltat), (T)mm);
-    }
-    const CComp3<T> ud = cc3<T>(s, 4);
-    if constexpr (P >= 2) {  // element by element
+        const CComp3<T> ud = cc3<T>(s, 4);
         const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
         T *et = s->etmp.as<T>();
         k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
                                                                                   s->off.as<int>(), ud, et);
-        k_fem_symplectic<T, P, 0, true><<<comp_grid<0>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-            s->d_src.as<long>(), sval, et);
-        k_fem_symplectic<T, P, 1, true><<<comp_grid<1>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-            s->d_src.as<long>(), sval, et);
-        k_fem_symplectic<T, P, 2, true><<<comp_grid<2>(s), blk, 0, s->stream>>>(
-            s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns,
-            s->d_src.as<long>(), sval, et);
+        k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(
+            s->g, u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval, et);
+        k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(
+            s->g, u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval, et);
+        k_fem_symplectic<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(
+            s->g, u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval, et);
         swap_slots(s, 0, 2);
         swap_slots(s, 1, 3);
-        return;
     }
-    k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
-                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
-                                                                       ns, s->d_src.as<long>(), sval);
-    k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
-                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
-                                                                       ns, s->d_src.as<long>(), sval);
-    k_fem_symplectic<T, P, 2><<<comp_grid<2>(s), blk, 0, s->stream>>>(s->g, s->K.as<T>(), s->off.as<int>(), u, pm, ud,
-                                                                       uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm,
-                                                                       ns, s->d_src.as<long>(), sval);
-    swap_slots(s, 0, 2);
-    swap_slots(s, 1, 3);
 }
 
 template <typename T> void step_leapfrog(ems_fem *s, const double *sval)

@@ -44,92 +44,8 @@ __device__ __forceinline__ long elem_rank(const FemGrid &g, int i, int j, int k)
     return g.OE + (i - 1) + (long)(j - 1) * (a - 2) + (long)(k - 1) * (a - 2) * (b - 2);
 }
 
-struct Cand {
-    long rank;
-    int i, j, k;  // element
-    int r;        // local row of the DOF in that element
-};
-
-// Elements containing DOF (comp, X, Y, Z), sorted by rank.  Returns count.
-template <int P>
-__device__ __forceinline__ int dof_elements(const FemGrid &g, int comp, int X, int Y, int Z, Cand *out)
-{
-    constexpr int p = P;
-    // per axis: up to two (element index, sub-index) choices
-    int ei[3][2], li[3][2], cnt[3];
-    const int C[3] = {X, Y, Z};
-    const int ne[3] = {g.nex, g.ney, g.nez};
-    for (int a = 0; a < 3; ++a) {
-        const int v = C[a];
-        if (a == comp || v % p != 0) {  // along the edge, or strictly inside
-            ei[a][0] = v / p;
-            li[a][0] = v % p;
-            cnt[a] = 1;
-        } else {
-            cnt[a] = 0;
-            if (v / p < ne[a]) {
-                ei[a][cnt[a]] = v / p;
-                li[a][cnt[a]] = 0;
-                cnt[a]++;
-            }
-            if (v / p - 1 >= 0) {
-                ei[a][cnt[a]] = v / p - 1;
-                li[a][cnt[a]] = p;
-                cnt[a]++;
-            }
-        }
-    }
-    // all candidates inner elements (1 <= index <= ne-2 on every axis): the
-    // serial order of inner elements is k, then j, then i ascending, so the
-    // candidates come out sorted without ranking them
-    bool inner = true;
-    for (int a = 0; a < 3; ++a)
-        for (int q = 0; q < cnt[a]; ++q) inner = inner && ei[a][q] >= 1 && ei[a][q] <= ne[a] - 2;
-    int nc = 0;
-    if (inner) {
-        for (int c = cnt[2] - 1; c >= 0; --c)      // ascending k (the +1 choice is stored first)
-            for (int b = cnt[1] - 1; b >= 0; --b)  // ascending j
-                for (int a = cnt[0] - 1; a >= 0; --a) {
-                    Cand q;
-                    q.i = ei[0][a];
-                    q.j = ei[1][b];
-                    q.k = ei[2][c];
-                    const int l = li[0][a], m = li[1][b], n = li[2][c];
-                    if (comp == 0) q.r = l + m * p + n * p * (p + 1);
-                    else if (comp == 1) q.r = g.D + l + m * (p + 1) + n * p * (p + 1);
-                    else q.r = 2 * g.D + l + m * (p + 1) + n * (p + 1) * (p + 1);
-                    q.rank = g.OE;  // inner (only compared against OE)
-                    out[nc++] = q;
-                }
-        return nc;
-    }
-    for (int a = 0; a < cnt[0]; ++a)
-        for (int b = 0; b < cnt[1]; ++b)
-            for (int c = 0; c < cnt[2]; ++c) {
-                Cand q;
-                q.i = ei[0][a];
-                q.j = ei[1][b];
-                q.k = ei[2][c];
-                const int l = li[0][a], m = li[1][b], n = li[2][c];
-                if (comp == 0) q.r = l + m * p + n * p * (p + 1);
-                else if (comp == 1) q.r = g.D + l + m * (p + 1) + n * p * (p + 1);
-                else q.r = 2 * g.D + l + m * (p + 1) + n * (p + 1) * (p + 1);
-                q.rank = elem_rank(g, q.i, q.j, q.k);
-                // insertion by rank
-                int s = nc++;
-                while (s > 0 && out[s - 1].rank > q.rank) {
-                    out[s] = out[s - 1];
-                    --s;
-                }
-                out[s] = q;
-            }
-    return nc;
-}
-
-// Elements of a DOF without a candidate array (an array filled with a
-// data-dependent count lives in scratch memory), used at first order, where
-// it makes the kernels 2-3x faster; at orders 2 and 3 (long row dots) the
-// array above measured faster and is kept.  Along each of the DOF's
+// Elements of a DOF, held in registers (a candidate array filled with a
+// data-dependent count would live in scratch memory).  Along each of the DOF's
 // two transverse axes the DOF lies in the element at v / p, or, when v is on
 // an element boundary (v % p == 0), also in the element v / p - 1 (local
 // index p).  Code a | b << 1 picks the choice on the first (a) and second (b)
@@ -257,52 +173,14 @@ __device__ __forceinline__ T gval(const T *__restrict__ u, const T *__restrict__
     return u[q];
 }
 
-// dot(K row r, x_e) for element (i, j, k), summed from 0 over m in order.
-template <typename T, int P, bool DRIFT>
-__device__ __forceinline__ T row_dot(const FemGrid &g, const T *__restrict__ K, const int *__restrict__ off, int r,
-                                     int i, int j, int k, CComp3<T> u, CComp3<T> pm, T hd, T mm)
-{
-    constexpr int D = P * (P + 1) * (P + 1);
-    constexpr int N = 3 * D;
-    long bx, by, bz;
-    elem_bases(g, i, j, k, bx, by, bz);
-    const T *__restrict__ kr = K + (long)r * N;
-    T temp = 0;
-    if constexpr (P == 1) {
-        // first order: the 12 offsets in the local order of the table
-        // (x: +Y, +Z; y: +X, +Z; z: +X, +Y), without the table lookups
-        const long Nx = g.Nx, Ny = g.Ny, sx = Nx * (Ny + 1), sy = (Nx + 1) * Ny;
-        temp += kr[0] * gval<T, DRIFT>(u.x, pm.x, bx, hd, mm);
-        temp += kr[1] * gval<T, DRIFT>(u.x, pm.x, bx + Nx, hd, mm);
-        temp += kr[2] * gval<T, DRIFT>(u.x, pm.x, bx + sx, hd, mm);
-        temp += kr[3] * gval<T, DRIFT>(u.x, pm.x, bx + sx + Nx, hd, mm);
-        temp += kr[4] * gval<T, DRIFT>(u.y, pm.y, by, hd, mm);
-        temp += kr[5] * gval<T, DRIFT>(u.y, pm.y, by + 1, hd, mm);
-        temp += kr[6] * gval<T, DRIFT>(u.y, pm.y, by + sy, hd, mm);
-        temp += kr[7] * gval<T, DRIFT>(u.y, pm.y, by + sy + 1, hd, mm);
-        temp += kr[8] * gval<T, DRIFT>(u.z, pm.z, bz, hd, mm);
-        temp += kr[9] * gval<T, DRIFT>(u.z, pm.z, bz + 1, hd, mm);
-        temp += kr[10] * gval<T, DRIFT>(u.z, pm.z, bz + Nx + 1, hd, mm);
-        temp += kr[11] * gval<T, DRIFT>(u.z, pm.z, bz + Nx + 2, hd, mm);
-        return temp;
-    }
-#pragma unroll 4
-    for (int m = 0; m < D; ++m) temp += kr[m] * gval<T, DRIFT>(u.x, pm.x, bx + off[m], hd, mm);
-#pragma unroll 4
-    for (int m = D; m < 2 * D; ++m) temp += kr[m] * gval<T, DRIFT>(u.y, pm.y, by + off[m], hd, mm);
-#pragma unroll 4
-    for (int m = 2 * D; m < N; ++m) temp += kr[m] * gval<T, DRIFT>(u.z, pm.z, bz + off[m], hd, mm);
-    return temp;
-}
-
 // ---------------------------------------------------------------------------
 // Orders 2 and 3, element by element.  Phase 1 (k_fem_elem): one thread per element
 // loads its n field values once and stores all n row dots
-//   y_e[r] = sum_m K[r][m] x_e[m]   (summed from 0 over m, as row_dot)
+//   y_e[r] = sum_m K[r][m] x_e[m]   (summed from 0 over m in order)
 // at tmp[r * nelem + e]; the matrix is read at addresses uniform over the
-// wavefront.  Phase 2: the per-DOF kernels below with each element's row dot
-// read from tmp instead of gathered (EBE = true): the same bits, while the
-// per-DOF gather loads every element's n values once per DOF of it.
+// wavefront.  Phase 2: the per-DOF kernels below add each element's row dot
+// read from tmp: the same bits as a per-DOF gather, which would load every
+// element's n values once per DOF of it.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ long elem_slot(const FemGrid &g, int r, int i, int j, int k)
 {
@@ -348,12 +226,11 @@ __global__ __launch_bounds__(kElemWG<P>) void k_fem_elem(FemGrid g, const T *__r
 // fem_v2 leapfrog step (src/fem_v2/fem_simulation.cpp:560-655 + BC):
 //   E2 = ((0 - sum_e K_e E1) + (2 E1 - E0)) - src,  PEC-tangential DOFs -> 0
 // ---------------------------------------------------------------------------
-// EBE: the row dots come from tmp (k_fem_elem) instead of being gathered.
-template <typename T, int P, int COMP, bool EBE = false>
-__global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__restrict__ K, const int *__restrict__ off,
-                                                      CComp3<T> e0, CComp3<T> e1, Comp3<T> e2, int nsrc,
+// Orders 2 and 3: the row dots come from etmp (k_fem_elem).
+template <typename T, int P, int COMP>
+__global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, CComp3<T> e0, CComp3<T> e1, Comp3<T> e2, int nsrc,
                                                       const long *__restrict__ src_idx, const double *__restrict__ src_val,
-                                                      const T *__restrict__ etmp = nullptr)
+                                                      const T *__restrict__ etmp)
 {
     int dx, dy, dz;
     comp_dims<COMP>(g, dx, dy, dz);
@@ -368,7 +245,7 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__rest
         return;
     }
     T acc = 0;
-    if constexpr (EBE) {  // one stored value per element: candidates in registers
+    {  // one stored value per element: candidates in registers
         int cd[4];
         long rk[4];
         order4<P>(g, COMP, X, Y, Z, cd, rk);
@@ -378,11 +255,6 @@ __global__ __launch_bounds__(256) void k_fem_leapfrog(FemGrid g, const T *__rest
             if (rk[q] == kNoRank || !elem_of<P>(g, COMP, X, Y, Z, cd[q], i, j, k, r)) continue;
             acc -= etmp[elem_slot(g, r, i, j, k)];
         }
-    } else {
-        Cand c[4];
-        const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
-        for (int q = 0; q < nc; ++q)
-            acc -= row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, e1, e1, (T)0, (T)1);
     }
     const T *a1 = COMP == 0 ? e1.x : COMP == 1 ? e1.y : e1.z;
     const T *a0 = COMP == 0 ? e0.x : COMP == 1 ? e0.y : e0.z;
@@ -417,16 +289,14 @@ __global__ __launch_bounds__(256) void k_fem_drift(FemGrid g, CComp3<T> u, CComp
     dc[i] = gval<T, true>(uc, pc, i, hd, mm);
 }
 
-// ud: the drifted field of k_fem_drift (gathered); u, pm: the DOF's own
-// values.  EBE: the row dots of ud come from tmp (k_fem_elem).
-template <typename T, int P, int COMP, bool EBE = false>
-__global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__restrict__ K,
-                                                        const int *__restrict__ off, CComp3<T> u, CComp3<T> pm,
-                                                        CComp3<T> ud, Comp3<T> u_out, Comp3<T> p_out, T deltat, T hd,
-                                                        T mm,
-                                                        int nsrc, const long *__restrict__ src_idx,
+// Orders 2 and 3: etmp holds the row dots of the drifted field of
+// k_fem_drift (k_fem_elem); u, pm: the DOF's own values.
+template <typename T, int P, int COMP>
+__global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, CComp3<T> u, CComp3<T> pm, Comp3<T> u_out,
+                                                        Comp3<T> p_out, T deltat, T hd, T mm, int nsrc,
+                                                        const long *__restrict__ src_idx,
                                                         const double *__restrict__ src_val,
-                                                        const T *__restrict__ etmp = nullptr)
+                                                        const T *__restrict__ etmp)
 {
     int dx, dy, dz;
     comp_dims<COMP>(g, dx, dy, dz);
@@ -448,7 +318,7 @@ __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__re
     const T sv = (T)src_val[0];
     T tmp = 0;
     bool src_done = nsrc_here == 0;
-    if constexpr (EBE) {  // one stored value per element: candidates in registers
+    {  // one stored value per element: candidates in registers
         int cd[4];
         long rk[4];
         order4<P>(g, COMP, X, Y, Z, cd, rk);
@@ -461,16 +331,6 @@ __global__ __launch_bounds__(256) void k_fem_symplectic(FemGrid g, const T *__re
                 src_done = true;
             }
             tmp += etmp[elem_slot(g, r, i, j, k)];
-        }
-    } else {
-        Cand c[4];
-        const int nc = dof_elements<P>(g, COMP, X, Y, Z, c);
-        for (int q = 0; q < nc; ++q) {
-            if (!src_done && c[q].rank >= g.OE) {
-                for (int s = 0; s < nsrc_here; ++s) tmp += sv;
-                src_done = true;
-            }
-            tmp += row_dot<T, P, false>(g, K, off, c[q].r, c[q].i, c[q].j, c[q].k, ud, ud, hd, mm);
         }
     }
     if (!src_done)
