@@ -21,7 +21,7 @@ using namespace ems::fdtd;
 
 namespace {
 
-constexpr long kPitchAlign = 16;  // elements: 64 B rows (FP32), 128 B rows (FP64)
+constexpr long kPitchAlign = 32;  // elements: rows start on 128-byte lines (256 B in FP64)
 constexpr long kFrontPad = 64;    // keeps (I-1) reads of row 0 inside the allocation
 
 struct Src {
@@ -58,6 +58,7 @@ struct ems_fdtd {
     int tile_flags = 1;  // bit 0: prefetch the next plane, bit 1: non-temporal stores
     int split_by = 8;    // split kernels: workgroup rows
     int split_lz = 2;    // split kernels: planes per thread
+    int fused_rev = 0;   // fused kernel: tiles of this step in reverse order
     // fused-kernel tile lists (interior / CPML-touching)
     DevBuf d_tiles_inner, d_tiles_pml;
     int n_inner = 0, n_pml = 0, ntx = 0, nty = 0;
@@ -286,7 +287,7 @@ bool split_boxes(const GridView &g, int P, int by, int lz, SplitBoxes &inner, in
 {
     if (g.nx < 2 * P + 1 || g.ny < 2 * P + 1 || g.nz < 2 * P + 1) return false;
     const int npw = (2 * P + 1 + 63) / 64;
-    const int nfull = npw + (g.nx - 2 * P + 63) / 64;
+    const int nfull = split_row_waves(g, P);
     auto add = [&](SplitBoxes &bs, int &total, SplitBox b) {
         const int rows = b.jpack ? 2 * P + 1 : b.j1 - b.j0;
         const int per = b.xrows > 0 ? by * b.xrows : by;  // rows per workgroup
@@ -334,7 +335,7 @@ void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
     // CPML: the x-slab columns get wavefronts of their own
     const int P = PML ? s->npml : 0;
     const int npw = PML ? (2 * P + 1 + 63) / 64 : 0;
-    const int nbx = PML ? npw + (g.nx - 2 * P + 63) / 64 : (g.nx + 1 + 63) / 64;
+    const int nbx = split_row_waves(g, P);
     dim3 blk(64, by, 1);
     dim3 grd(nbx, (g.ny + 1 + by - 1) / by, (g.nz + 1 + lz - 1) / lz);
     Fields<T> f = fields<T>(s);
@@ -388,7 +389,12 @@ template <typename T, int BY, int MAT, bool PML, bool PF, bool NT>
 void launch_tiles(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const PsiH<T> &ph, const int *list, int count)
 {
     if (count <= 0) return;
-    TileSet ts{list, count, s->ntx, s->nty, s->tile_lz};
+    // Workgroup order, measured at 256^3: XCD-banded (XCD c = b % 8 takes one
+    // contiguous eighth of the tiles) is faster in FP32 and slower in FP64
+    // than plain order; the tiles run in reverse every other step, so that a
+    // step starts where the previous one ended, on what the caches hold
+    // (2-3 % faster in both precisions).
+    TileSet ts{list, count, s->ntx, s->nty, s->tile_lz, sizeof(T) == 4 ? 1 : 0, s->fused_rev};
     const int grid = (count + 7) / 8 * 8;
     Fields<T> o = fields<T>(s);
     Fields<T> nw{s->f2[0].as<T>(), s->f2[1].as<T>(), s->f2[2].as<T>(),
@@ -427,6 +433,7 @@ void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
         if (PML) launch_tiles<T, R, MAT, true, true, true>(s, ch, ce, pm, ph, s->d_tiles_pml.as<int>(), s->n_pml);
         break;
     }
+    s->fused_rev ^= 1;
     // the new buffers become current
     for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
     if (PML)
@@ -440,14 +447,15 @@ template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch,
     else launch_fused<T, 8, MAT, PML>(s, ch, ce, pm);
 }
 
-// EMS_KERNEL_AUTO: measured on MI355X (256^3), the split sweeps with the CPML
-// columns on wavefronts of their own are fastest with a CPML, the fused
-// sweep without one.
+// EMS_KERNEL_AUTO: measured on MI355X (256^3), the split sweeps are fastest
+// with a CPML (0.38 vs 0.57 ms FP32, 0.67 vs 0.86 ms FP64) and in FP32
+// without one (0.260 vs 0.267 ms); the fused sweep in FP64 without a CPML
+// (0.50 vs 0.53 ms).
 bool use_fused(const ems_fdtd *s)
 {
     if (s->kernel == EMS_KERNEL_SPLIT) return false;
     if (s->kernel == EMS_KERNEL_FUSED) return true;
-    return s->npml <= 0;
+    return s->npml <= 0 && s->d.precision == EMS_F64;
 }
 
 template <typename T, int MAT, bool PML> void launch_variant(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
@@ -1133,7 +1141,7 @@ BatchMember<T> member_of(ems_fdtd *s, bool pml, int by, int lz, long base)
     const int P = pml ? std::max(s->npml, 0) : 0;
     m.lz = lz;
     m.npw = P > 0 ? (2 * P + 1 + 63) / 64 : 0;
-    m.nbx = P > 0 ? m.npw + (g.nx - 2 * P + 63) / 64 : (g.nx + 1 + 63) / 64;
+    m.nbx = split_row_waves(g, P);
     m.nby = (g.ny + 1 + by - 1) / by;
     m.base = base;
     return m;

@@ -3,8 +3,8 @@
 // Device layout ("unified grid"): all six components share one padded index
 // space (I, J, K) in [0, nx+1] x [0, ny+1] x [0, nz+1],
 //     u(I, J, K) = off + I + pitch*J + slab*K,   slab = pitch*(ny+2),
-// with pitch a multiple of 16 elements, so that every row starts on a 64-byte
-// (FP32) / 128-byte (FP64) boundary.  Reference arrays map into it as
+// with pitch a multiple of 32 elements, so that every row starts on a 128-byte
+// (FP32) / 256-byte (FP64) boundary.  Reference arrays map into it as
 //     E*[k][j][i] -> (i, j, k)
 //     Hx[k][j][i] -> (i+1, j, k)    Hy[k][j][i] -> (i, j+1, k)    Hz[k][j][i] -> (i, j, k+1)
 // (reference layouts: src/fdtd/fdtd_simulation.cpp:55-88), which turns the
@@ -137,8 +137,13 @@ __device__ __forceinline__ T e_update(T e, T br, T ce, const T *__restrict__ ca,
 // ---------------------------------------------------------------------------
 // Column of a split-kernel thread.  With a CPML the 2P+1 columns of the two
 // x-slabs (I <= P, I >= nx-P+1) are packed into the first npw wavefronts of
-// every row and the interior columns follow, so that no wavefront mixes CPML
-// and interior lanes (the CPML branch would otherwise run for most rows).
+// every row, so that no wavefront mixes CPML and interior lanes (the CPML
+// branch would otherwise run for most rows).  The interior wavefronts that
+// follow cover I = 64w .. 64w+63, aligned with the rows' 128-byte lines, with
+// the x-slab lanes idle: a wavefront starting at I = P+1 would straddle a line
+// at each end, and the neighbouring wavefront sits in another workgroup,
+// usually on another XCD with an L2 of its own (measured at 256^3 FP32
+// with a 10-cell CPML: 0.41 -> 0.38 ms per step).
 // Returns -1 for idle lanes.
 __device__ __forceinline__ int split_column(const GridView &g, int P, int npw, int bx, int tx)
 {
@@ -148,8 +153,16 @@ __device__ __forceinline__ int split_column(const GridView &g, int P, int npw, i
         if (L > 2 * P) return -1;
         return L <= P ? L : g.nx - 2 * P + L;
     }
-    const int I = P + 1 + (bx - npw) * 64 + tx;
-    return I <= g.nx - P ? I : -1;
+    const int I = (bx - npw) * 64 + tx;
+    return I > P && I <= g.nx - P ? I : -1;
+}
+
+// Wavefronts per row of the split sweeps: npw x-slab wavefronts plus the
+// aligned interior ones (P > 0), or all columns 0..nx (P = 0).
+__host__ __device__ inline int split_row_waves(const GridView &g, int P)
+{
+    if (P <= 0) return (g.nx + 1 + 63) / 64;
+    return (2 * P + 1 + 63) / 64 + (g.nx - P + 1 + 63) / 64;
 }
 
 // H sweep over planes [K0, K1) of the column (I, J); J is wave-uniform.
@@ -452,11 +465,15 @@ __global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, 
     h_split_body<T, PML>(g, ch, f, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// The E sweep runs its workgroups in reverse order, so that it starts where
+// the H sweep ended, on the planes the caches still hold (measured 1-6 %
+// faster at 256^3; the next H sweep then starts where this one ends).
 template <typename T, int MAT, bool PML>
 __global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
                                                  const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
 {
-    e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
+    e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, gridDim.x - 1 - blockIdx.x, gridDim.y - 1 - blockIdx.y,
+                              gridDim.z - 1 - blockIdx.z);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,7 +603,7 @@ __global__ __launch_bounds__(512) void k_h_batch(const BatchMember<T> *__restric
 template <typename T, int MAT, bool PML>
 __global__ __launch_bounds__(512) void k_e_batch(const BatchMember<T> *__restrict__ mem, int nm)
 {
-    const long b = blockIdx.x;
+    const long b = gridDim.x - 1L - blockIdx.x;  // reverse order, as k_e_split
     const BatchMember<T> &m = mem[batch_find(mem, nm, b)];
     const long lb = b - m.base;
     const int bx = (int)(lb % m.nbx), by = (int)((lb / m.nbx) % m.nby), bz = (int)(lb / ((long)m.nbx * m.nby));
@@ -697,6 +714,8 @@ struct TileSet {
     int count;        // tiles in this launch
     int ntx, nty;     // tiles along I and J
     int lz;           // planes per tile
+    int band;         // 1: XCD-banded workgroup order
+    int rev;          // 1: tiles in reverse order (every other step)
 };
 
 // Loads of one plane for the fused kernel (own cell, +I, +J neighbours).
@@ -737,8 +756,9 @@ __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Field
 {
     __shared__ T sH[2][3][BY][64];
     const int per = gridDim.x >> 3;
-    const int lb = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    int lb = ts.band ? (blockIdx.x & 7) * per + (blockIdx.x >> 3) : blockIdx.x;
     if (lb >= ts.count) return;  // uniform for the whole workgroup
+    if (ts.rev) lb = ts.count - 1 - lb;
     const int tile = ts.list ? ts.list[lb] : lb;
     const int ti = tile % ts.ntx;
     const int tj = (tile / ts.ntx) % ts.nty;
