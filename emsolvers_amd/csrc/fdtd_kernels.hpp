@@ -459,10 +459,33 @@ __device__ __forceinline__ void e_split_body(const GridView &g, T ce, const Fiel
     e_column<T, MAT, PML>(g, ce, f, ca, cb, pm, I, J, K0, K1);
 }
 
-template <typename T, bool PML>
-__global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw)
+// Workgroup order of the split sweeps.  Workgroups go to the 8 XCDs round
+// robin by linear id (workgroup b runs on XCD b % 8), each XCD with an L2 of
+// its own.  A tile (workgroup row block by, K chunk bz) has nbx workgroups
+// along x, which share cache lines: the x-slab wavefronts' lines with the
+// first and last interior ones, and each interior wavefront's I+1 line with
+// the next.  The 1-D grid therefore gives all nbx workgroups of a tile to one
+// XCD, b = 8 (bx + nbx (t / 8)) + t % 8, while the eight XCDs work on eight
+// consecutive tiles, as in plain order.  Returns false for the padding
+// workgroups of the last eight tiles.
+__device__ __forceinline__ bool split_tile(int b, int nbx, int nby, int ntiles, int &bx, int &by, int &bz)
 {
-    h_split_body<T, PML>(g, ch, f, pm, lz, npw, blockIdx.x, blockIdx.y, blockIdx.z);
+    const int c = b & 7, sl = b >> 3;
+    bx = sl % nbx;
+    const int t = (sl / nbx) * 8 + c;
+    if (t >= ntiles) return false;
+    by = t % nby;
+    bz = t / nby;
+    return true;
+}
+
+template <typename T, bool PML>
+__global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, PmlView<T> pm, int lz, int npw,
+                                                 int nbx, int nby, int ntiles)
+{
+    int bx, by, bz;
+    if (!split_tile(blockIdx.x, nbx, nby, ntiles, bx, by, bz)) return;
+    h_split_body<T, PML>(g, ch, f, pm, lz, npw, bx, by, bz);
 }
 
 // The E sweep runs its workgroups in reverse order, so that it starts where
@@ -470,10 +493,12 @@ __global__ __launch_bounds__(512) void k_h_split(GridView g, T ch, Fields<T> f, 
 // faster at 256^3; the next H sweep then starts where this one ends).
 template <typename T, int MAT, bool PML>
 __global__ __launch_bounds__(512) void k_e_split(GridView g, T ce, Fields<T> f, const T *__restrict__ ca,
-                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw)
+                                                 const T *__restrict__ cb, PmlView<T> pm, int lz, int npw, int nbx,
+                                                 int nby, int ntiles)
 {
-    e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, gridDim.x - 1 - blockIdx.x, gridDim.y - 1 - blockIdx.y,
-                              gridDim.z - 1 - blockIdx.z);
+    int bx, by, bz;
+    if (!split_tile(gridDim.x - 1 - blockIdx.x, nbx, nby, ntiles, bx, by, bz)) return;
+    e_split_body<T, MAT, PML>(g, ce, f, ca, cb, pm, lz, npw, bx, by, bz);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,6 +522,7 @@ struct SplitBox {
     int i0, i1, j0, j1, k0, k1;
     int packed, jpack, xrows;
     int nbx, nby, base;
+    int ilo;  // first column (unpacked boxes start at i0 = ilo rounded down to 64)
 };
 
 constexpr int kMaxBoxes = 4;
@@ -540,7 +566,7 @@ __device__ __forceinline__ bool box_column(const GridView &g, int P, int npw, in
     }
     J = XR ? Jv : __builtin_amdgcn_readfirstlane(Jv);
     if (b.packed) return I >= 0;
-    return I < b.i1;
+    return I >= b.ilo && I < b.i1;
 }
 
 template <typename T, bool PML>
