@@ -1151,11 +1151,13 @@ BatchMember<T> member_of(ems_fdtd *s, bool pml, int by, int lz, long base)
     m.npw = P > 0 ? (2 * P + 1 + 63) / 64 : 0;
     m.nbx = split_row_waves(g, P);
     m.nby = (g.ny + 1 + by - 1) / by;
+    m.ntiles = m.nby * ((g.nz + 1 + lz - 1) / lz);
     m.base = base;
     return m;
 }
 
-long member_blocks(const GridView &g, int nbx, int nby, int lz) { return (long)nbx * nby * ((g.nz + 1 + lz - 1) / lz); }
+// workgroups of a member: whole groups of 8 tiles (split_tile), so every base is a multiple of 8
+long member_blocks(int nbx, int ntiles) { return 8L * nbx * ((ntiles + 7) / 8); }
 
 template <typename T>
 ems_status build_members(ems_batch *b, bool pml, std::vector<BatchMember<T>> &ms, long &total)
@@ -1164,7 +1166,7 @@ ems_status build_members(ems_batch *b, bool pml, std::vector<BatchMember<T>> &ms
     total = 0;
     for (auto *s : b->mem) {
         BatchMember<T> m = member_of<T>(s, pml, b->by, b->lz, total);
-        total += member_blocks(s->g, m.nbx, m.nby, b->lz);
+        total += member_blocks(m.nbx, m.ntiles);
         ms.push_back(m);
     }
     EMS_TRY(b->d_mem.reserve(sizeof(BatchMember<T>) * ms.size()));
@@ -1328,7 +1330,7 @@ template <typename T> ems_status batch_autotune_impl(ems_batch *b, int trial, in
             long total = 0;
             for (auto *s : b->mem) {
                 BatchMember<T> m = member_of<T>(s, md.pml, by, lz, total);
-                total += member_blocks(s->g, m.nbx, m.nby, lz);
+                total += member_blocks(m.nbx, m.ntiles);
                 T *f[6];
                 for (int w = 0; w < 6; ++w) f[w] = reinterpret_cast<T *>(base + w * fmax);
                 m.f = Fields<T>{f[0], f[1], f[2], f[3], f[4], f[5]};

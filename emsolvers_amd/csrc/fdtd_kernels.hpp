@@ -591,8 +591,9 @@ __global__ __launch_bounds__(512) void k_e_boxes(GridView g, T ce, Fields<T> f, 
 // ---------------------------------------------------------------------------
 // Batched split sweeps: many independent grids (shots, models of uneven
 // size) in one launch per sweep.  Workgroup b belongs to the member whose
-// block range [base, base + nbx*nby*nbz) contains it (binary search over the
-// members' bases); inside its member it runs the single-grid body.
+// block range [base, base + 8*nbx*ceil(ntiles/8)) contains it (binary search
+// over the members' bases, all multiples of 8); inside its member it takes
+// the single-grid order of split_tile and runs the single-grid body.
 // ---------------------------------------------------------------------------
 template <typename T> struct BatchMember {
     GridView g;
@@ -600,7 +601,7 @@ template <typename T> struct BatchMember {
     PmlView<T> pm;
     const T *ca, *cb;
     T ch, ce;
-    int lz, npw, nbx, nby;
+    int lz, npw, nbx, nby, ntiles;
     long base;
 };
 
@@ -621,8 +622,8 @@ __global__ __launch_bounds__(512) void k_h_batch(const BatchMember<T> *__restric
 {
     const long b = blockIdx.x;
     const BatchMember<T> &m = mem[batch_find(mem, nm, b)];
-    const long lb = b - m.base;
-    const int bx = (int)(lb % m.nbx), by = (int)((lb / m.nbx) % m.nby), bz = (int)(lb / ((long)m.nbx * m.nby));
+    int bx, by, bz;
+    if (!split_tile((int)(b - m.base), m.nbx, m.nby, m.ntiles, bx, by, bz)) return;
     h_split_body<T, PML>(m.g, m.ch, m.f, m.pm, m.lz, m.npw, bx, by, bz);
 }
 
@@ -631,8 +632,8 @@ __global__ __launch_bounds__(512) void k_e_batch(const BatchMember<T> *__restric
 {
     const long b = gridDim.x - 1L - blockIdx.x;  // reverse order, as k_e_split
     const BatchMember<T> &m = mem[batch_find(mem, nm, b)];
-    const long lb = b - m.base;
-    const int bx = (int)(lb % m.nbx), by = (int)((lb / m.nbx) % m.nby), bz = (int)(lb / ((long)m.nbx * m.nby));
+    int bx, by, bz;
+    if (!split_tile((int)(b - m.base), m.nbx, m.nby, m.ntiles, bx, by, bz)) return;
     e_split_body<T, MAT, PML>(m.g, m.ce, m.f, m.ca, m.cb, m.pm, m.lz, m.npw, bx, by, bz);
 }
 
