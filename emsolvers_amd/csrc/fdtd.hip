@@ -57,7 +57,7 @@ struct ems_fdtd {
     int tile_lz = 4;   // planes per workgroup of the fused kernel
     int tile_flags = 1;  // bit 0: prefetch the next plane, bit 1: non-temporal stores
     int split_by = 8;    // split kernels: workgroup rows
-    int split_lz = 0;    // split kernels: planes per thread (0: 4 with a CPML, else 2)
+    int split_lz = 0;    // split kernels: planes per thread (0: 4 in FP64 with a CPML, else 2)
     int fused_rev = 0;   // fused kernel: tiles of this step in reverse order
     // fused-kernel tile lists (interior / CPML-touching)
     DevBuf d_tiles_inner, d_tiles_pml;
@@ -319,9 +319,9 @@ template <typename T, int MAT, bool PML>
 void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     const GridView &g = s->g;
-    // planes per thread, measured at 256^3: with a CPML 4 (FP64 2 %, FP32
-    // 0.5 % faster than 2), without one 2 (4 % faster than 4 in FP32)
-    const int by = s->split_by, lz = s->split_lz > 0 ? s->split_lz : PML ? 4 : 2;
+    // planes per thread, measured at 256^3: 2 (FP32 with a CPML 2 % faster
+    // than 4, without one 4 %), except FP64 with a CPML: 4 (1 % faster)
+    const int by = s->split_by, lz = s->split_lz > 0 ? s->split_lz : PML && sizeof(T) == 8 ? 4 : 2;
     if (PML && (s->tile_flags & 4)) {
         SplitBoxes inner, shell;
         int ni, ns;
@@ -788,11 +788,10 @@ long next_version()
     return ++versions;
 }
 
-// Workgroups of the adjoint passes (a multiple of 8, see adj_coords).
+// Workgroups of the adjoint passes (see adj_coords).
 unsigned adj_grid(const GridView &g)
 {
-    const long total = (long)((g.nx + 1 + 63) / 64) * ((g.ny + 1 + 3) / 4) * (g.nz + 1);
-    return (unsigned)((total + 7) / 8 * 8);
+    return (unsigned)((long)((g.nx + 1 + 63) / 64) * ((g.ny + 1 + 3) / 4) * (g.nz + 1));
 }
 
 template <typename T> Adj3<T> seg_e(ems_fdtd *s, int slot)
@@ -1753,6 +1752,11 @@ ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0, d
     if (ncells < 0) return invalid("CPML thickness must be >= 0");
     const int nmin = std::min({s->g.nx, s->g.ny, s->g.nz});
     if (ncells > 0 && 2 * ncells + 1 > nmin) return invalid("CPML thicker than half the grid");
+    {  // psi slabs are indexed with 32-bit integers
+        const long W = 2L * ncells + 1, ax = s->g.nx + 2L, ay = s->g.ny + 2L, az = s->g.nz + 2L;
+        if (ncells > 0 && std::max({W * ay * az, ax * W * az, ax * ay * W}) >= (1L << 31))
+            return invalid("CPML slabs too large (2^31 entries per slab)");
+    }
     if (ncells > 0 && (!(r0 > 0 && r0 < 1) || !(order >= 0) || !(kappa_max >= 1) || !(alpha_max >= 0)))
         return invalid("invalid CPML parameters");
     s->npml = ncells;

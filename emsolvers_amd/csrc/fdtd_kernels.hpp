@@ -54,20 +54,23 @@ template <typename T> struct PmlView {
 
 __device__ __forceinline__ bool in_pml(int P, int n, int I) { return P > 0 && (I <= P || I >= n - P + 1); }
 
-__device__ __forceinline__ long slab_x(const GridView &g, int P, int W, int I, int J, int K)
+// Index into a psi slab.  32-bit: every slab has fewer than 2^31 entries
+// (ems_fdtd_set_cpml checks it), and the narrower indices keep the CPML
+// instantiations of the split sweeps at 6 waves per SIMD in FP32.
+__device__ __forceinline__ int slab_x(const GridView &g, int P, int W, int I, int J, int K)
 {
     int l = I <= P ? I : I - g.nx + 2 * P;
-    return l + (long)W * (J + (long)(g.ny + 2) * K);
+    return l + W * (J + (g.ny + 2) * K);
 }
-__device__ __forceinline__ long slab_y(const GridView &g, int P, int W, int I, int J, int K)
+__device__ __forceinline__ int slab_y(const GridView &g, int P, int W, int I, int J, int K)
 {
     int l = J <= P ? J : J - g.ny + 2 * P;
-    return I + (long)(g.nx + 2) * (l + (long)W * K);
+    return I + (g.nx + 2) * (l + W * K);
 }
-__device__ __forceinline__ long slab_z(const GridView &g, int P, int W, int I, int J, int K)
+__device__ __forceinline__ int slab_z(const GridView &g, int P, int W, int I, int J, int K)
 {
     int l = K <= P ? K : K - g.nz + 2 * P;
-    return I + (long)(g.nx + 2) * (J + (long)(g.ny + 2) * l);
+    return I + (g.nx + 2) * (J + (g.ny + 2) * l);
 }
 
 // psi <- b*psi + a*t ; returns t/kappa + psi  (Roden & Gedney recursion)
@@ -80,17 +83,19 @@ template <typename T> __device__ __forceinline__ T cpml_term(T *psi, long q, con
 
 // The same recursion with psi already in a register (prefetched); stores the
 // new psi at q.  Bit for bit cpml_term.
-template <typename T> __device__ __forceinline__ T cpml_reg(T psi_old, T *psi, long q, const Prof<T> pr, T t)
+template <typename T> __device__ __forceinline__ T cpml_reg(T psi_old, T *psi, int q, const Prof<T> pr, T t)
 {
     T p = pr.b * psi_old + pr.a * t;
     psi[q] = p;
     return t * pr.k + p;
 }
 
-// The split sweeps prefetch the next plane's psi with its fields in FP32
-// (measured faster at 256^3); in FP64 the extra registers cost more than the
-// hidden latency and psi is loaded at its plane.
+// The H sweep prefetches the next plane's psi with its fields in FP32
+// (measured faster at 256^3); in FP64, and in the E sweep, the extra
+// registers cost more than the hidden latency (one wave per SIMD less) and
+// psi is loaded at its plane.
 template <typename T> constexpr bool kPsiPrefetch = sizeof(T) == 4;
+template <typename T> constexpr bool kPsiPrefetchE = false;
 
 // psi of one cell for the split sweeps, in the order the updates use them:
 //   [0] y of component x   [1] z of component x   [2] z of component y
@@ -101,17 +106,17 @@ __device__ __forceinline__ void psi_load(const GridView &g, const PmlView<T> &pm
                                          int K, bool px, bool py, bool pz, T (&v)[6])
 {
     if (py) {
-        const long q = slab_y(g, pm.P, pm.W, I, J, K);
+        const int q = slab_y(g, pm.P, pm.W, I, J, K);
         v[0] = ps[1][0][q];
         v[5] = ps[1][1][q];
     }
     if (pz) {
-        const long q = slab_z(g, pm.P, pm.W, I, J, K);
+        const int q = slab_z(g, pm.P, pm.W, I, J, K);
         v[1] = ps[2][0][q];
         v[2] = ps[2][1][q];
     }
     if (px) {
-        const long q = slab_x(g, pm.P, pm.W, I, J, K);
+        const int q = slab_x(g, pm.P, pm.W, I, J, K);
         v[3] = ps[0][0][q];
         v[4] = ps[0][1][q];
     }
@@ -230,7 +235,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             if (pz) prz = pm.hp[2][K];
             psi_load(g, pm, pm.psi_h, I, J, K, px, py, pz, ps);
         }
-        long qx = 0, qy = 0, qz = 0;
+        int qx = 0, qy = 0, qz = 0;
         if (PML) {
             if (px) qx = slab_x(g, pm.P, pm.W, I, J, K);
             if (py) qy = slab_y(g, pm.P, pm.W, I, J, K);
@@ -316,8 +321,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
     }
     // H at K-1 (zero below the grid: never used there)
     T hx_km1 = K0 >= 1 ? hx[u - S] : (T)0, hy_km1 = K0 >= 1 ? hy[u - S] : (T)0;
-    const long um = u - (I >= 1 ? 1 : 0), up = u - (J >= 1 ? P : 0);  // clamped neighbour bases
-    long dm = um - u, dp = up - u;
+    const int dm = I >= 1 ? -1 : 0, dp = J >= 1 ? -(int)P : 0;  // clamped neighbour offsets
     T hx_k = hx[u], hy_k = hy[u], hz_k = hz[u], hz_jm = hz[u + dp], hy_im = hy[u + dm], hz_im = hz[u + dm];
     T hx_jm = hx[u + dp], exo = ex[u], eyo = ey[u], ezo = ez[u];
     T cbv = MAT >= 1 ? cb[u] : (T)0, cav = MAT == 2 ? ca[u] : (T)0;
@@ -329,7 +333,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
         if (px) prx = pm.ep[0][I];
         if (py) pry = pm.ep[1][J];
     }
-    if (PML && kPsiPrefetch<T>) {
+    if (PML && kPsiPrefetchE<T>) {
         pz = in_pml(pm.P, g.nz, K0);
         if (pz) prz = pm.ep[2][K0];
         psi_load(g, pm, pm.psi_e, I, J, K0, px, py, pz, ps);
@@ -354,18 +358,18 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             n_ez = ez[v];
             if (MAT >= 1) n_cb = cb[v];
             if (MAT == 2) n_ca = ca[v];
-            if (PML && kPsiPrefetch<T>) {
+            if (PML && kPsiPrefetchE<T>) {
                 npz = in_pml(pm.P, g.nz, K + 1);
                 if (npz) nprz = pm.ep[2][K + 1];
                 psi_load(g, pm, pm.psi_e, I, J, K + 1, px, py, npz, nps);
             }
         }
-        if (PML && !kPsiPrefetch<T>) {
+        if (PML && !kPsiPrefetchE<T>) {
             pz = in_pml(pm.P, g.nz, K);
             if (pz) prz = pm.ep[2][K];
             psi_load(g, pm, pm.psi_e, I, J, K, px, py, pz, ps);
         }
-        long qx = 0, qy = 0, qz = 0;
+        int qx = 0, qy = 0, qz = 0;
         if (PML) {
             if (px) qx = slab_x(g, pm.P, pm.W, I, J, K);
             if (py) qy = slab_y(g, pm.P, pm.W, I, J, K);
@@ -410,7 +414,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             }
             ez[u] = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
         }
-        if (PML && kPsiPrefetch<T>) {
+        if (PML && kPsiPrefetchE<T>) {
             pz = npz;
             prz = nprz;
 #pragma unroll
@@ -1059,16 +1063,15 @@ struct GradView {
 };
 
 // Adjoint launches: a 1-D grid of 64 x 4 workgroups over (I, J) blocks and
-// single planes K, K fastest.  Consecutive workgroups on one XCD (the
-// hardware deals workgroups round-robin over 8 XCDs) take consecutive planes
-// of one (I, J) block, so the K-1 / K+1 neighbours they read are in that
-// XCD's L2.  gridDim.x is a multiple of 8.
+// single planes K, K fastest, in plain order (measured 3 % faster than
+// giving each XCD a contiguous range: the eight XCDs then work on
+// neighbouring planes, whose shared lines the caches behind the L2s serve).
 __device__ __forceinline__ bool adj_coords(const GridView &g, int &I, int &J, int &K)
 {
     const int nkz = g.nz + 1;
     const int nbx = (g.nx + 1 + 63) / 64, nby = (g.ny + 1 + 3) / 4;
     const long total = (long)nbx * nby * nkz;
-    const long lb = (long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const long lb = blockIdx.x;
     if (lb >= total) return false;
     K = (int)(lb % nkz);
     const long r = lb / nkz;
