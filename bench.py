@@ -511,7 +511,7 @@ def main():
             "grid": [args.n] * 3,
             "cpml_cells": 10,
             "material_words": 1,
-            "kernel": {0: "auto (split H/E sweeps when a CPML is present, fused otherwise)", 1: "split", 2: "fused"}[args.kernel],
+            "kernel": {0: "auto (split H/E sweeps; fused only in FP64 without a CPML)", 1: "split", 2: "fused"}[args.kernel],
             "parallelism": "single-gpu",
         },
         "roofline": {

@@ -81,7 +81,7 @@ template <typename T> __device__ __forceinline__ T cpml_term(T *psi, long q, con
     return t * pr.k + p;
 }
 
-// The same recursion with psi already in a register (prefetched); stores the
+// The same recursion with psi already in a register (psi_load); stores the
 // new psi at q.  Bit for bit cpml_term.
 template <typename T> __device__ __forceinline__ T cpml_reg(T psi_old, T *psi, int q, const Prof<T> pr, T t)
 {
@@ -90,12 +90,9 @@ template <typename T> __device__ __forceinline__ T cpml_reg(T psi_old, T *psi, i
     return t * pr.k + p;
 }
 
-// The H sweep prefetches the next plane's psi with its fields in FP32
-// (measured faster at 256^3); in FP64, and in the E sweep, the extra
-// registers cost more than the hidden latency (one wave per SIMD less) and
-// psi is loaded at its plane.
-template <typename T> constexpr bool kPsiPrefetch = sizeof(T) == 4;
-template <typename T> constexpr bool kPsiPrefetchE = false;
+// The split sweeps load psi at its plane: prefetching the next plane's psi
+// with its fields costs registers, and at 256^3 the wave per SIMD it takes
+// is worth more than the latency it hides (FP32 and FP64).
 
 // psi of one cell for the split sweeps, in the order the updates use them:
 //   [0] y of component x   [1] z of component x   [2] z of component y
@@ -192,8 +189,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
     // values of the current plane; the next plane is loaded before the stores
     T ex_k1 = ex[u + S], ey_k1 = ey[u + S], ez_k = ez[u], ez_j1 = ez[u + P], ez_i1 = ez[u + 1];
     T ey_i1 = ey[u + 1], ex_j1 = ex[u + P], hxo = hx[u], hyo = hy[u], hzo = hz[u];
-    // CPML: the column's x and y gradings once, psi and the z grading of the
-    // next plane prefetched with the fields (psi is private to its cell)
+    // CPML: the column's x and y gradings once (psi is private to its cell)
     Prof<T> prx{}, pry{}, prz{};
     T ps[6] = {};
     bool pz = false;
@@ -201,17 +197,9 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
         if (px) prx = pm.hp[0][I];
         if (py) pry = pm.hp[1][J];
     }
-    if (PML && kPsiPrefetch<T>) {
-        pz = in_pml(pm.P, g.nz, K0);
-        if (pz) prz = pm.hp[2][K0];
-        psi_load(g, pm, pm.psi_h, I, J, K0, px, py, pz, ps);
-    }
     for (int K = K0; K < K1; ++K, u += S) {
         T n_ex_k1 = 0, n_ey_k1 = 0, n_ez_k = 0, n_ez_j1 = 0, n_ez_i1 = 0, n_ey_i1 = 0, n_ex_j1 = 0;
         T n_hx = 0, n_hy = 0, n_hz = 0;
-        T nps[6] = {};
-        Prof<T> nprz{};
-        bool npz = false;
         if (K + 1 < K1) {
             const long v = u + S;
             n_ex_k1 = ex[v + S];
@@ -224,13 +212,8 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             n_hx = hx[v];
             n_hy = hy[v];
             n_hz = hz[v];
-            if (PML && kPsiPrefetch<T>) {
-                npz = in_pml(pm.P, g.nz, K + 1);
-                if (npz) nprz = pm.hp[2][K + 1];
-                psi_load(g, pm, pm.psi_h, I, J, K + 1, px, py, npz, nps);
-            }
         }
-        if (PML && !kPsiPrefetch<T>) {
+        if (PML) {
             pz = in_pml(pm.P, g.nz, K);
             if (pz) prz = pm.hp[2][K];
             psi_load(g, pm, pm.psi_h, I, J, K, px, py, pz, ps);
@@ -280,12 +263,6 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             }
             hz[u] = hzo - ch * br;
         }
-        if (PML && kPsiPrefetch<T>) {
-            pz = npz;
-            prz = nprz;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) ps[c] = nps[c];
-        }
         ex_k = ex_k1;
         ey_k = ey_k1;
         ex_k1 = n_ex_k1;
@@ -325,7 +302,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
     T hx_k = hx[u], hy_k = hy[u], hz_k = hz[u], hz_jm = hz[u + dp], hy_im = hy[u + dm], hz_im = hz[u + dm];
     T hx_jm = hx[u + dp], exo = ex[u], eyo = ey[u], ezo = ez[u];
     T cbv = MAT >= 1 ? cb[u] : (T)0, cav = MAT == 2 ? ca[u] : (T)0;
-    // CPML as in the H sweep: gradings of the column once, psi prefetched
+    // CPML as in the H sweep: gradings of the column once
     Prof<T> prx{}, pry{}, prz{};
     T ps[6] = {};
     bool pz = false;
@@ -333,17 +310,9 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
         if (px) prx = pm.ep[0][I];
         if (py) pry = pm.ep[1][J];
     }
-    if (PML && kPsiPrefetchE<T>) {
-        pz = in_pml(pm.P, g.nz, K0);
-        if (pz) prz = pm.ep[2][K0];
-        psi_load(g, pm, pm.psi_e, I, J, K0, px, py, pz, ps);
-    }
     for (int K = K0; K < K1; ++K, u += S) {
         T n_hx = 0, n_hy = 0, n_hz = 0, n_hz_jm = 0, n_hy_im = 0, n_hz_im = 0, n_hx_jm = 0;
         T n_ex = 0, n_ey = 0, n_ez = 0, n_cb = 0, n_ca = 0;
-        T nps[6] = {};
-        Prof<T> nprz{};
-        bool npz = false;
         if (K + 1 < K1) {
             const long v = u + S;
             n_hx = hx[v];
@@ -358,13 +327,8 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             n_ez = ez[v];
             if (MAT >= 1) n_cb = cb[v];
             if (MAT == 2) n_ca = ca[v];
-            if (PML && kPsiPrefetchE<T>) {
-                npz = in_pml(pm.P, g.nz, K + 1);
-                if (npz) nprz = pm.ep[2][K + 1];
-                psi_load(g, pm, pm.psi_e, I, J, K + 1, px, py, npz, nps);
-            }
         }
-        if (PML && !kPsiPrefetchE<T>) {
+        if (PML) {
             pz = in_pml(pm.P, g.nz, K);
             if (pz) prz = pm.ep[2][K];
             psi_load(g, pm, pm.psi_e, I, J, K, px, py, pz, ps);
@@ -413,12 +377,6 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
                 br = a - b - c + d;
             }
             ez[u] = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
-        }
-        if (PML && kPsiPrefetchE<T>) {
-            pz = npz;
-            prz = nprz;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) ps[c] = nps[c];
         }
         hx_km1 = hx_k;
         hy_km1 = hy_k;
