@@ -1664,6 +1664,11 @@ ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out)
     if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
         return invalid("h, dt, eps and mu must be positive");
     if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
+    {  // the split sweeps address a plane with 32-bit byte offsets
+        const GridView g = make_grid(desc->nx, desc->ny, desc->nz);
+        if ((double)g.slab * (desc->precision == EMS_F32 ? 4 : 8) >= 4294967296.0)
+            return invalid("grid plane too large (one z-plane must be under 4 GiB)");
+    }
     if (ems_device_count() <= 0) {
         set_error("no HIP device available");
         return EMS_ERR_HIP;

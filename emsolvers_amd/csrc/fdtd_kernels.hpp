@@ -167,7 +167,34 @@ __host__ __device__ inline int split_row_waves(const GridView &g, int P)
     return (2 * P + 1 + 63) / 64 + (g.nx - P + 1 + 63) / 64;
 }
 
-// H sweep over planes [K0, K1) of the column (I, J); J is wave-uniform.
+// Element at a wave-uniform base plus a per-lane byte offset.  The split
+// sweeps address every field as (uniform plane base) + (lane's offset in
+// the plane, under 4 GiB: ems_fdtd_create checks it): the base goes through
+// readfirstlane into scalar registers and the lane keeps 32-bit offsets,
+// instead of a 64-bit address per field and neighbour.  This took the FP32
+// CPML sweeps from 73-82 VGPRs (6 waves per SIMD) to 51-56 (8).  The base
+// must be the same on every active lane; the address space stays global.
+#define EMS_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ unsigned long long uniform_bits(const void *p)
+{
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+template <typename T> __device__ __forceinline__ EMS_GLOBAL T &lane_at(T *base, unsigned boff)
+{
+    EMS_GLOBAL char *b = reinterpret_cast<EMS_GLOBAL char *>(uniform_bits(base));
+    return *reinterpret_cast<EMS_GLOBAL T *>(b + boff);
+}
+template <typename T> __device__ __forceinline__ const EMS_GLOBAL T &lane_at(const T *base, unsigned boff)
+{
+    const EMS_GLOBAL char *b = reinterpret_cast<const EMS_GLOBAL char *>(uniform_bits(base));
+    return *reinterpret_cast<const EMS_GLOBAL T *>(b + boff);
+}
+
+// H sweep over planes [K0, K1) of the column (I, J).  K0 and K1 are
+// wave-uniform; I and J may differ between lanes.
 template <typename T, bool PML>
 __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T> &f, const PmlView<T> &pm, int I,
                                          int J, int K0, int K1)
@@ -179,16 +206,19 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
     T *__restrict__ hy = f.hy;
     T *__restrict__ hz = f.hz;
     const long P = g.pitch, S = g.slab;
-    long u = g.off + I + P * J + S * K0;
+    long ub = g.off + S * K0;                              // wave-uniform part of the cell index
+    const unsigned ui = (unsigned)(I + P * J) * sizeof(T);  // the lane's part, in bytes
     bool px = false, py = false;
     if (PML) {
         px = in_pml(pm.P, g.nx, I);
         py = in_pml(pm.P, g.ny, J);
     }
-    T ex_k = ex[u], ey_k = ey[u];
+    T ex_k = lane_at(ex + ub, ui), ey_k = lane_at(ey + ub, ui);
     // values of the current plane; the next plane is loaded before the stores
-    T ex_k1 = ex[u + S], ey_k1 = ey[u + S], ez_k = ez[u], ez_j1 = ez[u + P], ez_i1 = ez[u + 1];
-    T ey_i1 = ey[u + 1], ex_j1 = ex[u + P], hxo = hx[u], hyo = hy[u], hzo = hz[u];
+    T ex_k1 = lane_at(ex + (ub + S), ui), ey_k1 = lane_at(ey + (ub + S), ui);
+    T ez_k = lane_at(ez + ub, ui), ez_j1 = lane_at(ez + (ub + P), ui), ez_i1 = lane_at(ez + (ub + 1), ui);
+    T ey_i1 = lane_at(ey + (ub + 1), ui), ex_j1 = lane_at(ex + (ub + P), ui);
+    T hxo = lane_at(hx + ub, ui), hyo = lane_at(hy + ub, ui), hzo = lane_at(hz + ub, ui);
     // CPML: the column's x and y gradings once (psi is private to its cell)
     Prof<T> prx{}, pry{}, prz{};
     T ps[6] = {};
@@ -197,21 +227,21 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
         if (px) prx = pm.hp[0][I];
         if (py) pry = pm.hp[1][J];
     }
-    for (int K = K0; K < K1; ++K, u += S) {
+    for (int K = K0; K < K1; ++K, ub += S) {
         T n_ex_k1 = 0, n_ey_k1 = 0, n_ez_k = 0, n_ez_j1 = 0, n_ez_i1 = 0, n_ey_i1 = 0, n_ex_j1 = 0;
         T n_hx = 0, n_hy = 0, n_hz = 0;
         if (K + 1 < K1) {
-            const long v = u + S;
-            n_ex_k1 = ex[v + S];
-            n_ey_k1 = ey[v + S];
-            n_ez_k = ez[v];
-            n_ez_j1 = ez[v + P];
-            n_ez_i1 = ez[v + 1];
-            n_ey_i1 = ey[v + 1];
-            n_ex_j1 = ex[v + P];
-            n_hx = hx[v];
-            n_hy = hy[v];
-            n_hz = hz[v];
+            const long vb = ub + S;
+            n_ex_k1 = lane_at(ex + (vb + S), ui);
+            n_ey_k1 = lane_at(ey + (vb + S), ui);
+            n_ez_k = lane_at(ez + vb, ui);
+            n_ez_j1 = lane_at(ez + (vb + P), ui);
+            n_ez_i1 = lane_at(ez + (vb + 1), ui);
+            n_ey_i1 = lane_at(ey + (vb + 1), ui);
+            n_ex_j1 = lane_at(ex + (vb + P), ui);
+            n_hx = lane_at(hx + vb, ui);
+            n_hy = lane_at(hy + vb, ui);
+            n_hz = lane_at(hz + vb, ui);
         }
         if (PML) {
             pz = in_pml(pm.P, g.nz, K);
@@ -235,7 +265,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            hx[u] = hxo - ch * br;
+            lane_at(hx + ub, ui) = hxo - ch * br;
         }
         if (J >= 1) {  // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
             const T a = ex_k1, b = ex_k, c = ez_i1, d = ez_k;
@@ -248,7 +278,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            hy[u] = hyo - ch * br;
+            lane_at(hy + ub, ui) = hyo - ch * br;
         }
         if (K >= 1) {  // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
             const T a = ey_i1, b = ey_k, c = ex_j1, d = ex_k;
@@ -261,7 +291,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            hz[u] = hzo - ch * br;
+            lane_at(hz + ub, ui) = hzo - ch * br;
         }
         ex_k = ex_k1;
         ey_k = ey_k1;
@@ -278,7 +308,7 @@ __device__ __forceinline__ void h_column(const GridView &g, T ch, const Fields<T
     }
 }
 
-// E sweep over planes [K0, K1) of the column (I, J); J is wave-uniform.
+// E sweep over planes [K0, K1) of the column (I, J), as the H sweep.
 template <typename T, int MAT, bool PML>
 __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T> &f, const T *__restrict__ ca,
                                          const T *__restrict__ cb, const PmlView<T> &pm, int I, int J, int K0, int K1)
@@ -290,18 +320,22 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
     const T *__restrict__ hy = f.hy;
     const T *__restrict__ hz = f.hz;
     const long P = g.pitch, S = g.slab;
-    long u = g.off + I + P * J + S * K0;
+    long ub = g.off + S * K0;                                 // wave-uniform part of the cell index
+    const unsigned ui = (unsigned)(I + P * J) * sizeof(T);     // the lane's part, in bytes
+    const unsigned uim = ui - (I >= 1 ? (unsigned)sizeof(T) : 0u);       // I-1 (clamped)
+    const unsigned ujm = ui - (J >= 1 ? (unsigned)(P * sizeof(T)) : 0u);  // J-1 (clamped)
     bool px = false, py = false;
     if (PML) {
         px = in_pml(pm.P, g.nx, I);
         py = in_pml(pm.P, g.ny, J);
     }
     // H at K-1 (zero below the grid: never used there)
-    T hx_km1 = K0 >= 1 ? hx[u - S] : (T)0, hy_km1 = K0 >= 1 ? hy[u - S] : (T)0;
-    const int dm = I >= 1 ? -1 : 0, dp = J >= 1 ? -(int)P : 0;  // clamped neighbour offsets
-    T hx_k = hx[u], hy_k = hy[u], hz_k = hz[u], hz_jm = hz[u + dp], hy_im = hy[u + dm], hz_im = hz[u + dm];
-    T hx_jm = hx[u + dp], exo = ex[u], eyo = ey[u], ezo = ez[u];
-    T cbv = MAT >= 1 ? cb[u] : (T)0, cav = MAT == 2 ? ca[u] : (T)0;
+    T hx_km1 = K0 >= 1 ? lane_at(hx + (ub - S), ui) : (T)0, hy_km1 = K0 >= 1 ? lane_at(hy + (ub - S), ui) : (T)0;
+    T hx_k = lane_at(hx + ub, ui), hy_k = lane_at(hy + ub, ui), hz_k = lane_at(hz + ub, ui);
+    T hz_jm = lane_at(hz + ub, ujm), hy_im = lane_at(hy + ub, uim), hz_im = lane_at(hz + ub, uim);
+    T hx_jm = lane_at(hx + ub, ujm);
+    T exo = lane_at(ex + ub, ui), eyo = lane_at(ey + ub, ui), ezo = lane_at(ez + ub, ui);
+    T cbv = MAT >= 1 ? lane_at(cb + ub, ui) : (T)0, cav = MAT == 2 ? lane_at(ca + ub, ui) : (T)0;
     // CPML as in the H sweep: gradings of the column once
     Prof<T> prx{}, pry{}, prz{};
     T ps[6] = {};
@@ -310,23 +344,23 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
         if (px) prx = pm.ep[0][I];
         if (py) pry = pm.ep[1][J];
     }
-    for (int K = K0; K < K1; ++K, u += S) {
+    for (int K = K0; K < K1; ++K, ub += S) {
         T n_hx = 0, n_hy = 0, n_hz = 0, n_hz_jm = 0, n_hy_im = 0, n_hz_im = 0, n_hx_jm = 0;
         T n_ex = 0, n_ey = 0, n_ez = 0, n_cb = 0, n_ca = 0;
         if (K + 1 < K1) {
-            const long v = u + S;
-            n_hx = hx[v];
-            n_hy = hy[v];
-            n_hz = hz[v];
-            n_hz_jm = hz[v + dp];
-            n_hy_im = hy[v + dm];
-            n_hz_im = hz[v + dm];
-            n_hx_jm = hx[v + dp];
-            n_ex = ex[v];
-            n_ey = ey[v];
-            n_ez = ez[v];
-            if (MAT >= 1) n_cb = cb[v];
-            if (MAT == 2) n_ca = ca[v];
+            const long vb = ub + S;
+            n_hx = lane_at(hx + vb, ui);
+            n_hy = lane_at(hy + vb, ui);
+            n_hz = lane_at(hz + vb, ui);
+            n_hz_jm = lane_at(hz + vb, ujm);
+            n_hy_im = lane_at(hy + vb, uim);
+            n_hz_im = lane_at(hz + vb, uim);
+            n_hx_jm = lane_at(hx + vb, ujm);
+            n_ex = lane_at(ex + vb, ui);
+            n_ey = lane_at(ey + vb, ui);
+            n_ez = lane_at(ez + vb, ui);
+            if (MAT >= 1) n_cb = lane_at(cb + vb, ui);
+            if (MAT == 2) n_ca = lane_at(ca + vb, ui);
         }
         if (PML) {
             pz = in_pml(pm.P, g.nz, K);
@@ -350,7 +384,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            ex[u] = MAT == 0 ? exo + ce * br : MAT == 1 ? exo + cbv * br : cav * exo + cbv * br;
+            lane_at(ex + ub, ui) = MAT == 0 ? exo + ce * br : MAT == 1 ? exo + cbv * br : cav * exo + cbv * br;
         }
         if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
             const T a = hx_k, b = hx_km1, c = hz_k, d = hz_im;
@@ -363,7 +397,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            ey[u] = MAT == 0 ? eyo + ce * br : MAT == 1 ? eyo + cbv * br : cav * eyo + cbv * br;
+            lane_at(ey + ub, ui) = MAT == 0 ? eyo + ce * br : MAT == 1 ? eyo + cbv * br : cav * eyo + cbv * br;
         }
         if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
             const T a = hy_k, b = hy_im, c = hx_k, d = hx_jm;
@@ -376,7 +410,7 @@ __device__ __forceinline__ void e_column(const GridView &g, T ce, const Fields<T
             } else {
                 br = a - b - c + d;
             }
-            ez[u] = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
+            lane_at(ez + ub, ui) = MAT == 0 ? ezo + ce * br : MAT == 1 ? ezo + cbv * br : cav * ezo + cbv * br;
         }
         hx_km1 = hx_k;
         hy_km1 = hy_k;
