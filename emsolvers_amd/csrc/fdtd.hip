@@ -56,7 +56,7 @@ struct ems_fdtd {
     int tile_by = 8;   // workgroup rows of the fused kernel (4, 8 or 16)
     int tile_lz = 4;   // planes per workgroup of the fused kernel
     int tile_flags = 1;  // bit 0: prefetch the next plane, bit 1: non-temporal stores
-    int split_by = 8;    // split kernels: workgroup rows
+    int split_by = 0;    // split kernels: workgroup rows (0: 4 with a CPML, else 8)
     int split_lz = 0;    // split kernels: planes per thread (0: 4 in FP64 with a CPML, else 2)
     int fused_rev = 0;   // fused kernel: tiles of this step in reverse order
     // fused-kernel tile lists (interior / CPML-touching)
@@ -319,9 +319,12 @@ template <typename T, int MAT, bool PML>
 void launch_split(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     const GridView &g = s->g;
-    // planes per thread, measured at 256^3: 2 (FP32 with a CPML 2 % faster
-    // than 4, without one 4 %), except FP64 with a CPML: 4 (1 % faster)
-    const int by = s->split_by, lz = s->split_lz > 0 ? s->split_lz : PML && sizeof(T) == 8 ? 4 : 2;
+    // measured at 256^3: workgroups of 4 rows with a CPML (4 % faster than 8
+    // in FP32 and FP64; 2 rows are 1 % slower in FP32), of 8 without (1 %
+    // faster than 4); planes per thread 2 (FP32 with a CPML 2 % faster than
+    // 4, without one 4 %), except FP64 with a CPML: 4 (1 % faster)
+    const int by = s->split_by > 0 ? s->split_by : PML ? 4 : 8;
+    const int lz = s->split_lz > 0 ? s->split_lz : PML && sizeof(T) == 8 ? 4 : 2;
     if (PML && (s->tile_flags & 4)) {
         SplitBoxes inner, shell;
         int ni, ns;

@@ -110,7 +110,8 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
  * stores, bit 2: split sweeps with a CPML as an interior launch plus a
  * CPML-shell launch per sweep instead of one launch).  Results do not depend
  * on it.  Defaults: rows 8, 4 planes per fused workgroup, 2 planes per split
- * thread (4 in FP64 with a CPML), flags 1. */
+ * thread (4 in FP64 with a CPML), flags 1; split workgroups have 4 rows with
+ * a CPML and 8 without. */
 ems_status ems_fdtd_set_tiling(ems_fdtd *s, int rows, int planes, int flags);
 
 /* FemSimulation::run (src/fdtd/fdtd_simulation.cpp:242-306): advances
