@@ -75,7 +75,9 @@ typedef struct {
     int precision;    /* EMS_F64 or EMS_F32                                */
 } ems_fdtd_desc;
 
-/* FemSimulation::FemSimulation (src/fdtd/fdtd_simulation.cpp:38-53). */
+/* FemSimulation::FemSimulation (src/fdtd/fdtd_simulation.cpp:38-53).
+ * EMS_ERR_INVALID_ARG also when one padded z-plane of a field would reach
+ * 4 GiB (the kernels address a plane with 32-bit offsets). */
 ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out);
 void ems_fdtd_destroy(ems_fdtd *s);
 
@@ -93,7 +95,10 @@ ems_status ems_fdtd_add_observation(ems_fdtd *s, const int counts[3], const int 
  * counterpart: extends the scalar coefficient of fdtd_simulation.cpp:170-220. */
 ems_status ems_fdtd_set_material(ems_fdtd *s, const double *eps_r, const double *sigma);
 
-/* CPML on all six faces, ncells thick (0 disables).  No reference counterpart. */
+/* CPML on all six faces, ncells thick (0 disables).  No reference counterpart.
+ * EMS_ERR_INVALID_ARG when 2*ncells+1 exceeds the smallest grid size, for
+ * invalid grading parameters, or when one psi slab would hold 2^31 entries
+ * or more (32-bit slab indices). */
 ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0,
                              double kappa_max, double alpha_max);
 
