@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lane_addr.hpp"
+
 namespace ems {
 namespace fdtd {
 
@@ -167,31 +169,10 @@ __host__ __device__ inline int split_row_waves(const GridView &g, int P)
     return (2 * P + 1 + 63) / 64 + (g.nx - P + 1 + 63) / 64;
 }
 
-// Element at a wave-uniform base plus a per-lane byte offset.  The split
-// sweeps address every field as (uniform plane base) + (lane's offset in
-// the plane, under 4 GiB: ems_fdtd_create checks it): the base goes through
-// readfirstlane into scalar registers and the lane keeps 32-bit offsets,
-// instead of a 64-bit address per field and neighbour.  This took the FP32
-// CPML sweeps from 73-82 VGPRs (6 waves per SIMD) to 51-56 (8).  The base
-// must be the same on every active lane; the address space stays global.
-#define EMS_GLOBAL __attribute__((address_space(1)))
-__device__ __forceinline__ unsigned long long uniform_bits(const void *p)
-{
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
-template <typename T> __device__ __forceinline__ EMS_GLOBAL T &lane_at(T *base, unsigned boff)
-{
-    EMS_GLOBAL char *b = reinterpret_cast<EMS_GLOBAL char *>(uniform_bits(base));
-    return *reinterpret_cast<EMS_GLOBAL T *>(b + boff);
-}
-template <typename T> __device__ __forceinline__ const EMS_GLOBAL T &lane_at(const T *base, unsigned boff)
-{
-    const EMS_GLOBAL char *b = reinterpret_cast<const EMS_GLOBAL char *>(uniform_bits(base));
-    return *reinterpret_cast<const EMS_GLOBAL T *>(b + boff);
-}
+// The split sweeps address every field as (uniform plane base) + (lane's
+// offset in the plane, under 4 GiB: ems_fdtd_create checks it), through
+// lane_at.  This took the FP32 CPML sweeps from 73-82 VGPRs (6 waves per
+// SIMD) to 51-56 (8).
 
 // H sweep over planes [K0, K1) of the column (I, J).  K0 and K1 are
 // wave-uniform; I and J may differ between lanes.

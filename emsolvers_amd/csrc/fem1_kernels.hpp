@@ -17,6 +17,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lane_addr.hpp"
+
 namespace ems {
 namespace fem1 {
 
@@ -32,27 +34,10 @@ template <typename T> struct CF3 {
     const T *c[3];
 };
 
-// Element at a wave-uniform base plus a per-lane byte offset (base kept in
-// scalar registers through readfirstlane, one 32-bit offset per lane; the
-// address space stays global).
-__device__ __forceinline__ unsigned long long f1_uniform_bits(const void *p)
-{
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
-template <typename T> __device__ __forceinline__ T f1_at(const T *base, unsigned boff)
-{
-    const __attribute__((address_space(1))) char *b =
-        reinterpret_cast<const __attribute__((address_space(1))) char *>(f1_uniform_bits(base));
-    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(b + boff);
-}
-
 // Row r of the 12 x 12 matrix M dotted with the 12 edge values of element
 // (i, j, k), m = 0..11 in order, summed from 0.  The callers' j and k are
 // wave-uniform (only i varies over the lanes): in FP32 the addresses are
-// taken as uniform bases plus i (11 % faster at 128^3); in FP64 that form
+// taken as uniform bases plus i through lane_at (11 % faster at 128^3); in FP64 that form
 // measured 35 % slower and the plain 64-bit addresses are kept.
 template <typename T>
 __device__ __forceinline__ T elem_dot(const Hex1 &g, const T *__restrict__ M, int r, CF3<T> x, int i, int j, int k)
@@ -70,18 +55,18 @@ __device__ __forceinline__ T elem_dot(const Hex1 &g, const T *__restrict__ M, in
         const long by = (Nx + 1) * (j + Ny * (long)k);
         const long bz = (Nx + 1) * (j + (Ny + 1) * (long)k);
         const unsigned io = (unsigned)i * sizeof(T);
-        t += kr[0] * f1_at(X + bx, io);
-        t += kr[1] * f1_at(X + (bx + Nx), io);
-        t += kr[2] * f1_at(X + (bx + sx), io);
-        t += kr[3] * f1_at(X + (bx + sx + Nx), io);
-        t += kr[4] * f1_at(Y + by, io);
-        t += kr[5] * f1_at(Y + (by + sy), io);
-        t += kr[6] * f1_at(Y + (by + 1), io);
-        t += kr[7] * f1_at(Y + (by + 1 + sy), io);
-        t += kr[8] * f1_at(Z + bz, io);
-        t += kr[9] * f1_at(Z + (bz + 1), io);
-        t += kr[10] * f1_at(Z + (bz + Nx + 1), io);
-        t += kr[11] * f1_at(Z + (bz + Nx + 2), io);
+        t += kr[0] * lane_at(X + bx, io);
+        t += kr[1] * lane_at(X + (bx + Nx), io);
+        t += kr[2] * lane_at(X + (bx + sx), io);
+        t += kr[3] * lane_at(X + (bx + sx + Nx), io);
+        t += kr[4] * lane_at(Y + by, io);
+        t += kr[5] * lane_at(Y + (by + sy), io);
+        t += kr[6] * lane_at(Y + (by + 1), io);
+        t += kr[7] * lane_at(Y + (by + 1 + sy), io);
+        t += kr[8] * lane_at(Z + bz, io);
+        t += kr[9] * lane_at(Z + (bz + 1), io);
+        t += kr[10] * lane_at(Z + (bz + Nx + 1), io);
+        t += kr[11] * lane_at(Z + (bz + Nx + 2), io);
         return t;
     }
     const long bx = i + Nx * (j + (Ny + 1) * (long)k);
