@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "lane_addr.hpp"
 #include "tet_basis.hpp"
 #include "tet_mesh.hpp"
 
@@ -125,7 +126,12 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
 
 // phase 2: sum over the tetrahedra containing dof r, in incidence order.
 // All loads of the row are issued before the sum (at most kMaxInc entries;
-// entries outside the mesh contribute +0).
+// entries outside the mesh contribute +0).  When the wave holds one row type
+// (all but the waves at family boundaries) the incidence entries are
+// wave-uniform and each tmp address is a uniform base plus the lane's cube
+// offset (lane_at; 6-7 % faster at order 1); otherwise per-lane addresses.
+// The cube offset in bytes stays below 2^32 (the mesh-size limit of
+// ems_tet_create keeps ncube under 2.9e8).
 constexpr int kMaxInc = 8;
 
 template <typename T>
@@ -135,6 +141,30 @@ __device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict_
     const int rt = ebe_row(m, r, I, J, K);
     const int q0 = m.inc_start[rt], cnt = m.inc_start[rt + 1] - q0;
     T v[kMaxInc];
+    if (__all(rt == __builtin_amdgcn_readfirstlane(rt))) {
+        // one row type on the wave: the incidence entries are uniform, and
+        // each tmp address is a uniform base plus the lane's cube offset
+        const unsigned co = (unsigned)(I + m.nx * (J + m.ny * K)) * (unsigned)sizeof(T);
+        const int rtu = __builtin_amdgcn_readfirstlane(rt);
+        const int q0u = m.inc_start[rtu], cntu = m.inc_start[rtu + 1] - q0u;
+#pragma unroll
+        for (int q = 0; q < kMaxInc; ++q) {
+            v[q] = 0;
+            if (q < cntu) {
+                const int c = inc[q0u + q];
+                const int di = (c & 3) - 1, dj = ((c >> 2) & 3) - 1, dk = ((c >> 4) & 3) - 1;
+                const int ci = I + di, cj = J + dj, ck = K + dk;
+                const int t = (c >> 6) & 7, L = (c >> 9) & 31;
+                const long sh = (long)(t * m.nl + L) * m.ncube + di + (long)m.nx * (dj + (long)m.ny * dk);
+                if (ci >= 0 && cj >= 0 && ck >= 0 && ci < m.nx && cj < m.ny && ck < m.nz)
+                    v[q] = lane_at(tmp + sh, co);
+            }
+        }
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxInc; ++q) s += v[q];
+        return s;
+    }
 #pragma unroll
     for (int q = 0; q < kMaxInc; ++q) {
         v[q] = 0;
