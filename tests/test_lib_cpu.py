@@ -65,6 +65,12 @@ def test_invalid_descriptor_rejected_before_device_use():
     assert ei.value.status == 1
     with pytest.raises(_lib.EmsError):
         FemSimulation((8, 8, 8), 1e-3, 1e-12, 8.854187817e-12, 1.25663706212e-6, 0)
+    # one padded z-plane of 4 GiB or more: the kernels' 32-bit plane offsets
+    # would wrap (40032 x 40002 FP32 words = 6.4 GB; 23232 x 23202 FP64 = 4.31 GB)
+    for grid, prec in (((40000, 40000, 4), "f32"), ((23200, 23200, 4), "f64")):
+        with pytest.raises(_lib.EmsError) as ei:
+            FemSimulation(grid, 1e-3, 1e-12, 8.854187817e-12, 1.25663706212e-6, 1, precision=prec)
+        assert ei.value.status == 1 and "plane too large" in str(ei.value)
 
 
 def test_check_params_matches_oracle():
