@@ -685,6 +685,18 @@ template <typename T> __global__ void k_axpy(int n, T a, const T *__restrict__ x
     if (i < n) y[i] = y[i] + a * x[i];
 }
 
+// y += a x with Kahan compensation c (the low-order part y has lost so
+// far): the stored state does not accumulate the rounding of its updates.
+template <typename T> __global__ void k_axpy_comp(int n, T a, const T *__restrict__ x, T *y, T *c)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T yv = a * x[i] - c[i];
+    const T t = y[i] + yv;
+    c[i] = (t - y[i]) - yv;
+    y[i] = t;
+}
+
 // x = y + r (y - yp): linear extrapolation of the stage solutions in time
 template <typename T>
 __global__ void k_extrap(int n, const T *__restrict__ y, const T *__restrict__ yp, T r, T *__restrict__ x)
@@ -801,6 +813,10 @@ struct ems_tet {
     DevBuf Y, Yp, X;
     double ty = 0.0, typ = 0.0;
     int have_prev = 0;
+    // FP32 symplectic stages: Kahan compensation of the U and V updates,
+    // valid for the current state (cleared when the state is set)
+    DevBuf Uc, Vc;
+    bool comp_valid = false, comp_off = false;
     DevBuf d_partial, d_cg;
     int lpr = 4;  // lanes per sparse row (4, or 8 for rows longer than 32)
     int op_kind = EMS_TET_OP_AUTO;
@@ -1544,7 +1560,19 @@ template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBu
 template <typename T> ems_status stage(ems_tet *s, int q, double D, double tm, const double *fdev)
 {
     const int n = (int)s->nd, g = (n + 255) / 256;
-    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    const bool comp = sizeof(T) == 4 && !s->comp_off;
+    if (comp && !s->comp_valid) {
+        EMS_TRY(s->Uc.reserve(sizeof(T) * n));
+        EMS_TRY(s->Vc.reserve(sizeof(T) * n));
+        EMS_HIP(hipMemsetAsync(s->Uc.p, 0, sizeof(T) * n, s->stream));
+        EMS_HIP(hipMemsetAsync(s->Vc.p, 0, sizeof(T) * n, s->stream));
+        s->comp_valid = true;
+    }
+    auto update = [&](T a, const T *x, DevBuf &y, DevBuf &c) {
+        if (comp) k_axpy_comp<T><<<g, 256, 0, s->stream>>>(n, a, x, y.as<T>(), c.as<T>());
+        else k_axpy<T><<<g, 256, 0, s->stream>>>(n, a, x, y.as<T>());
+    };
+    update((T)(0.5 * D), s->V.as<T>(), s->U, s->Uc);
     force<T>(s, s->U.as<T>(), s->V.as<T>(), fdev);
     if (!s->X.p) {
         EMS_TRY(s->Y.reserve(sizeof(T) * n));
@@ -1563,8 +1591,8 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, double tm, c
     else
         EMS_HIP(hipMemsetAsync(s->X.p, 0, sizeof(T) * n, s->stream));
     EMS_TRY(cg<T>(s, q, s->X));
-    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)D, s->X.as<T>(), s->V.as<T>());
-    k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(0.5 * D), s->V.as<T>(), s->U.as<T>());
+    update((T)D, s->X.as<T>(), s->V, s->Vc);
+    update((T)(0.5 * D), s->V.as<T>(), s->U, s->Uc);
     // history: Yp <- Y <- X
     swap_dev(s->Yp, s->Y);
     swap_dev(s->Y, s->X);
@@ -1802,7 +1830,11 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     s->nslot = 0;
     s->traces.assign(0, 0.0);
     s->have_prev = 0;
-    EMS_TRY(run_impl<T>(s, N, nullptr));
+    s->comp_valid = false;
+    s->comp_off = true;  // the reversal below undoes plain updates
+    const ems_status fwd = run_impl<T>(s, N, nullptr);
+    s->comp_off = false;
+    EMS_TRY(fwd);
     const int nslot = N / tf + 1;
     std::vector<double> res((size_t)nslot * nobs * 3, 0.0);  // [slot][obs][3]
     double jj = 0.0;
@@ -1898,6 +1930,7 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     s->t_now = 0.0;
     s->steps = 0;
     s->have_prev = 0;
+    s->comp_valid = false;
     return EMS_OK;
 }
 
@@ -2131,6 +2164,7 @@ ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t cou
     }
     s->abm_started = false;
     s->have_prev = 0;  // a new state: no stage solution to extrapolate
+    s->comp_valid = false;
     return EMS_OK;
 }
 
