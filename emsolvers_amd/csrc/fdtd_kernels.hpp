@@ -134,10 +134,11 @@ __device__ __forceinline__ T e_update(T e, T br, T ce, const T *__restrict__ ca,
 // ---------------------------------------------------------------------------
 // Split kernels: an H sweep then an E sweep, in place.  Each thread owns one
 // (I, J) column and marches up a chunk of lz planes in K, carrying the
-// K-neighbour values in registers (each field word is loaded from HBM once
-// per sweep); I and J neighbours are served by L1 (same wavefront / same
-// workgroup).  Workgroup 64 x BY, grid (ceil((nx+1)/64), ceil((ny+1)/BY),
-// ceil((nz+1)/lz)).
+// K-neighbour values in registers and loading the next plane before the
+// current one's stores; I and J neighbours are served by the caches (same
+// wavefront / workgroup / XCD).  Workgroup 64 x BY (BY rows, one wavefront
+// per row); a 1-D grid of nbx workgroups per tile (row block, K chunk), in
+// the order of split_tile.
 // ---------------------------------------------------------------------------
 // Column of a split-kernel thread.  With a CPML the 2P+1 columns of the two
 // x-slabs (I <= P, I >= nx-P+1) are packed into the first npw wavefronts of
