@@ -179,3 +179,42 @@ def test_lossless_energy_bounded_on_device():
         en.append(0.5 * V @ (Me @ V) + 0.5 * U @ (S @ U))
     en = np.array(en)
     assert en.min() > 0 and (en.max() - en.min()) / en.mean() < 0.05
+
+
+@pytest.mark.parametrize("stepper", ["symplectic_2", "abm"])
+def test_fp32_error_on_benchmark_model(stepper):
+    """The 40^3 order-1 benchmark model (seeded smoothed eps_r, x-dipole, 16
+    receivers, dt = 0.9 of the stepper's stable step), 600 steps: FP32 traces
+    within 1e-5 relative L2 of FP64 on the same inputs (the FP64 path is the
+    one the oracle pins), TF envelope / phase misfits within 1e-4.  Guards the
+    curl-factorised stiffness, the increment-form stage solves and the
+    compensated FP32 stage updates, without which this error grows past 1e-5."""
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.tet import TetSimulation, KAPPA, STEPPERS
+    n = 40
+    h = 1.0 / n
+
+    def run(prec, dt=None):
+        er, sg = wl.tet_model(n, 5, 0.0)
+        s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, 1, prec)
+        s.set_material(er, sg)
+        if dt is None:
+            dt = 0.9 * KAPPA[STEPPERS[stepper]] / s.max_frequency(40)
+        s.set_time_step(dt)
+        f0 = 2.0 * wl.C0 / (h * n)
+        s.add_dipole([0.5 + 0.1 * h, 0.5 + 0.2 * h, 0.5 + 0.3 * h], [1.0, 0.0, 0.0], f0, 1.2 / f0, 1e-9)
+        for p in wl.tet_receivers(n, h, 16, 6):
+            s.add_receiver(p)
+        s.run(600)
+        tr = s.traces()
+        s.close()
+        return tr, dt
+
+    print("benchmark model fp32 error", stepper, flush=True)
+    ref, dt = run("f64")
+    tr, _ = run("f32", dt)
+    err = rel_l2(tr, ref)
+    em, pm = tf_misfits(tr, ref, dt)
+    print("fp32 rel L2 %.2e, TF EM %.2e PM %.2e" % (err, em, pm), flush=True)
+    assert err < 1e-5
+    assert em <= 1e-4 and pm <= 1e-4
