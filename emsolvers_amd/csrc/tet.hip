@@ -647,15 +647,34 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict
 }
 
 // one block: scal[slot] = fixed-order sum of the slot's nb partials
-__global__ __launch_bounds__(kRB) void k_cg_finalize(const double *__restrict__ P, CgState *st, int s0, int s1,
+// One block of kFB threads: each thread sums its strided partials of both
+// slots in one pass (fixed order), then a fixed LDS tree.
+constexpr int kFB = 1024;
+__global__ __launch_bounds__(kFB) void k_cg_finalize(const double *__restrict__ P, CgState *st, int s0, int s1,
                                                      int nb)
 {
     if (st->done) return;
-    const double a = block_total(P + s0 * nb, nb);
-    const double b = s1 >= 0 ? block_total(P + s1 * nb, nb) : 0.0;
-    if (threadIdx.x == 0) {
-        st->scal[s0] = a;
-        if (s1 >= 0) st->scal[s1] = b;
+    __shared__ double sa[kFB], sb[kFB];
+    const int t = threadIdx.x;
+    const double *p0 = P + s0 * nb, *p1 = P + (s1 >= 0 ? s1 : s0) * nb;
+    double a = 0.0, b = 0.0;
+    for (int q = t; q < nb; q += kFB) {
+        a += p0[q];
+        if (s1 >= 0) b += p1[q];
+    }
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int w = kFB / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sa[t] += sa[t + w];
+            sb[t] += sb[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        st->scal[s0] = sa[0];
+        if (s1 >= 0) st->scal[s1] = sb[0];
     }
 }
 
@@ -1447,18 +1466,18 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
         const EbeOp<T> op = ebe_op<T>(s, 1);
         ebe_local<T, EBE_MASS>(s, hh, xx, nullptr, nullptr);
         k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 0, 3, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 0, 3, nb);
         ebe_local<T, EBE_MASS>(s, hh, z, nullptr, nullptr);
         k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
         EMS_HIP(hipGetLastError());
         for (int it = 0; it < maxit; ++it) {
             const int par = it & 1;
             k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
-            if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
+            if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
             ebe_local<T, EBE_MASS>(s, hh, z, nullptr, &st->done);
             k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
-            if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
+            if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
             if ((it + 1) % check == 0) {
                 CgState h;
                 EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
@@ -1479,21 +1498,21 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
         constexpr int LPR = decltype(L)::value;
         const CsrOp<LPR, T> op{A, val};
         k_cg_init<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 0, 3, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 0, 3, nb);
         k_cg_first<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
     });
     EMS_HIP(hipGetLastError());
     for (int it = 0; it < maxit; ++it) {
         const int par = it & 1;
         k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
         with_lpr(s->lpr, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
             const CsrOp<LPR, T> op{A, val};
             k_cg_direction<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
         });
-        if (fin) k_cg_finalize<<<1, kRB, 0, s->stream>>>(P, st, 2, -1, nb);
+        if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
         if ((it + 1) % check == 0) {
             CgState h;
             EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));

@@ -218,3 +218,43 @@ def test_fp32_error_on_benchmark_model(stepper):
     print("fp32 rel L2 %.2e, TF EM %.2e PM %.2e" % (err, em, pm), flush=True)
     assert err < 1e-5
     assert em <= 1e-4 and pm <= 1e-4
+
+
+def test_large_mesh_reduction_path():
+    """A mesh above 512 CG blocks (48^3 cubes, order 1: 823,543 DOFs), where
+    each dot product is reduced once by a one-block kernel instead of in
+    every block: the solves converge in the iteration range of the smaller
+    benchmark mesh (31-36 there), and FP32 agrees with FP64 within 1e-5."""
+    from emsolvers_amd import workloads as wl
+    from emsolvers_amd.tet import TetSimulation, KAPPA, STEPPERS
+    n = 48
+    h = 1.0 / n
+
+    def run(prec, dt=None):
+        er, sg = wl.tet_model(n, 5, 0.0)
+        s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, "symplectic_2", 1, prec)
+        s.set_material(er, sg)
+        if dt is None:
+            dt = 0.9 * KAPPA[STEPPERS["symplectic_2"]] / s.max_frequency(40)
+        s.set_time_step(dt)
+        f0 = 2.0 * wl.C0 / (h * n)
+        s.add_dipole([0.5 + 0.1 * h, 0.5 + 0.2 * h, 0.5 + 0.3 * h], [1.0, 0.0, 0.0], f0, 1.2 / f0, 1e-9)
+        for p in wl.tet_receivers(n, h, 16, 6):
+            s.add_receiver(p)
+        s.run(300)
+        last, mx = s.solver_stats()
+        nd = s.dof_count()
+        tr = s.traces()
+        s.close()
+        return tr, dt, last, mx, nd
+
+    print("large mesh reduction path", flush=True)
+    ref, dt, last, mx, nd = run("f64")
+    assert nd > 512 * 1024
+    print("fp64 CG iterations last %d max %d" % (last, mx), flush=True)
+    assert 20 <= last <= mx <= 60
+    assert np.isfinite(ref).all()
+    tr, _, _, _, _ = run("f32", dt)
+    err = rel_l2(tr, ref)
+    print("fp32 rel L2 %.2e" % err, flush=True)
+    assert err < 1e-5
