@@ -192,12 +192,15 @@ __device__ __forceinline__ T ebe_gather(const EbeMesh &m, const int *__restrict_
 // which no oscillation removes (in FP32 it made the fields drift).
 enum { EBE_MASS = 0, EBE_STIFF = 1, EBE_FORCE = 3 };
 
+// cmt (MASS, optional): the coefficient (T)(e0 eps_r + hh sigma) per
+// tetrahedron, type-major, computed by k_mass_coef with the same expression.
 template <typename T, int NL, int MODE>
 __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restrict__ Mt, const T *__restrict__ St,
                                                    const double *__restrict__ er, const double *__restrict__ sg,
                                                    const T *__restrict__ Ct, double e0, double hh,
                                                    const unsigned *__restrict__ tmask, const T *__restrict__ x1,
-                                                   const T *__restrict__ x2, T *__restrict__ tmp, const int *done)
+                                                   const T *__restrict__ x2, T *__restrict__ tmp, const int *done,
+                                                   const T *__restrict__ cmt)
 {
     if (done && *done) return;
     const long cube = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,7 +208,8 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     if (cube >= m.ncube) return;
     const int ci = (int)(cube % m.nx), cj = (int)((cube / m.nx) % m.ny), ck = (int)(cube / ((long)m.nx * m.ny));
     const long tid = 6 * cube + t;
-    const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
+    const bool pre = MODE == EBE_MASS && cmt;
+    const double epsr = (!pre && er) ? er[tid] : 1.0, sig = (!pre && sg) ? sg[tid] : 0.0;
     const unsigned bm = tmask[tid];  // bit l: local dof l is a boundary dof
     constexpr bool USE2 = MODE == EBE_FORCE;
     T a[NL], b[NL];
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
         }
     }
     T cm;
-    if (MODE == EBE_MASS) cm = (T)(e0 * epsr + hh * sig);
+    if (MODE == EBE_MASS) cm = pre ? cmt[(long)t * m.ncube + cube] : (T)(e0 * epsr + hh * sig);
     else cm = (T)sig;
 #pragma unroll 4
     for (int l = 0; l < NL; ++l) {
@@ -278,6 +282,19 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
 // D[l / 16 + 4 v][l % 16] (f64).
 typedef float ems_f4 __attribute__((ext_vector_type(4)));
 typedef double ems_d4 __attribute__((ext_vector_type(4)));
+
+// Mass coefficient (T)(e0 eps_r + hh sigma) of every tetrahedron, type-major
+// (the expression of k_ebe_local, so the same bits).
+template <typename T>
+__global__ void k_mass_coef(long ncube, const double *__restrict__ er, const double *__restrict__ sg, double e0,
+                            double hh, T *__restrict__ out)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 6 * ncube) return;
+    const long t = q / ncube, cube = q - t * ncube, tid = 6 * cube + t;
+    const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
+    out[q] = (T)(e0 * epsr + hh * sig);
+}
 
 template <typename T> struct MfmaAcc;
 template <> struct MfmaAcc<float> {
@@ -822,6 +839,10 @@ struct ems_tet {
     std::vector<double> deltas;  // stage steps with an assembled A_+
     DevBuf d_A[3], d_dinv[3];
     const void *a_val[3] = {}, *a_dinv[3] = {};  // solve matrix per stage step (d_A or d_Me)
+    // element-by-element: per-tetrahedron mass coefficient eps0 eps_r + hh sigma
+    // (type-major, [t * ncube + cube]) for hh = 0 and per stage step
+    DevBuf d_cm_me, d_cm[3];
+    const void *a_cm[3] = {};
     DevBuf d_dinv_me;  // Jacobi preconditioner of M_eps alone
     DevBuf U, V, b, r, z, p, ap;        // state and CG scratch
     DevBuf Vm, am, an, Up, Vp, Ap;      // ABM: history (V_n-1, a_n-1, a_n) and predictor
@@ -1172,9 +1193,14 @@ template <typename T> ems_status stage_matrices(ems_tet *s)
         EMS_TRY(s->d_dinv_me.reserve(sizeof(T) * nd));
         k_ebe_diag<T><<<g, 256, 0, s->stream>>>(s->em, s->d_inc.as<int>(), s->d_Mt.as<T>(), er, sg, s->d.eps, 0.0,
                                                 s->d_bnd.as<unsigned char>(), s->d_dinv_me.as<T>());
+        const long nt = 6 * s->em.ncube;
+        const int gt = (int)((nt + 255) / 256);
+        EMS_TRY(s->d_cm_me.reserve(sizeof(T) * nt));
+        k_mass_coef<T><<<gt, 256, 0, s->stream>>>(s->em.ncube, er, sg, s->d.eps, 0.0, s->d_cm_me.as<T>());
         for (size_t q = 0; q < s->deltas.size(); ++q) {
             if (!s->lossy || s->deltas[q] == 0.0) {
                 s->a_dinv[q] = s->d_dinv_me.p;
+                s->a_cm[q] = s->d_cm_me.p;
                 continue;
             }
             EMS_TRY(s->d_dinv[q].reserve(sizeof(T) * nd));
@@ -1182,6 +1208,10 @@ template <typename T> ems_status stage_matrices(ems_tet *s)
                                                     0.5 * s->deltas[q], s->d_bnd.as<unsigned char>(),
                                                     s->d_dinv[q].as<T>());
             s->a_dinv[q] = s->d_dinv[q].p;
+            EMS_TRY(s->d_cm[q].reserve(sizeof(T) * nt));
+            k_mass_coef<T><<<gt, 256, 0, s->stream>>>(s->em.ncube, er, sg, s->d.eps, 0.5 * s->deltas[q],
+                                                      s->d_cm[q].as<T>());
+            s->a_cm[q] = s->d_cm[q].p;
         }
         EMS_HIP(hipGetLastError());
         EMS_HIP(hipStreamSynchronize(s->stream));
@@ -1396,7 +1426,7 @@ template <typename F> void with_nl(int nl, F &&f)
 
 // phase 1 of the element-by-element operator
 template <typename T, int MODE>
-void ebe_local(ems_tet *s, double hh, const T *x1, const T *x2, const int *done)
+void ebe_local(ems_tet *s, double hh, const T *x1, const T *x2, const int *done, const T *cmt = nullptr)
 {
     const dim3 grid((unsigned)((s->em.ncube + 255) / 256), 6);
     const double *er = s->eps_r.empty() ? nullptr : s->d_er.as<double>();
@@ -1416,7 +1446,7 @@ void ebe_local(ems_tet *s, double hh, const T *x1, const T *x2, const int *done)
         k_ebe_local<T, NL, MODE><<<grid, 256, 0, s->stream>>>(s->em, s->d_Mt.as<T>(), s->d_St.as<T>(), er, sg,
                                                              s->d_Ct.as<T>(), s->d.eps, hh, s->d_tmask.as<unsigned>(),
                                                              x1, x2,
-                                                             s->d_tmp.as<T>(), done);
+                                                             s->d_tmp.as<T>(), done, cmt);
     });
 }
 
@@ -1463,11 +1493,12 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     const bool fin = nb > kCgInlineNB;
     if (s->ebe) {
         const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
+        const T *cmt = (const T *)(which < 0 ? s->d_cm_me.p : s->a_cm[which]);  // null: from eps_r, sigma
         const EbeOp<T> op = ebe_op<T>(s, 1);
-        ebe_local<T, EBE_MASS>(s, hh, xx, nullptr, nullptr);
+        ebe_local<T, EBE_MASS>(s, hh, xx, nullptr, nullptr, cmt);
         k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
         if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 0, 3, nb);
-        ebe_local<T, EBE_MASS>(s, hh, z, nullptr, nullptr);
+        ebe_local<T, EBE_MASS>(s, hh, z, nullptr, nullptr, cmt);
         k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
         if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
         EMS_HIP(hipGetLastError());
@@ -1475,7 +1506,7 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
             const int par = it & 1;
             k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
             if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
-            ebe_local<T, EBE_MASS>(s, hh, z, nullptr, &st->done);
+            ebe_local<T, EBE_MASS>(s, hh, z, nullptr, &st->done, cmt);
             k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
             if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
             if ((it + 1) % check == 0) {
