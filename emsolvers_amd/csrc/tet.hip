@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     const long tid = 6 * cube + t;
     const bool pre = MODE == EBE_MASS && cmt;
     const double epsr = (!pre && er) ? er[tid] : 1.0, sig = (!pre && sg) ? sg[tid] : 0.0;
-    const unsigned bm = tmask[tid];  // bit l: local dof l is a boundary dof
+    const unsigned bm = tmask[(long)t * m.ncube + cube];  // bit l: local dof l is a boundary dof
     constexpr bool USE2 = MODE == EBE_FORCE;
     T a[NL], b[NL];
 #pragma unroll
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__res
         T bv = 0;
         if (valid && k < NL) {
             const long d = ebe_dof(m, t, k, ci, cj, ck);
-            if (!((tmask[6 * cube + t] >> k) & 1u)) bv = x[d];
+            if (!((tmask[(long)t * m.ncube + cube] >> k) & 1u)) bv = x[d];
         }
 #pragma unroll
         for (int q = 0; q < MT; ++q) {
@@ -1319,16 +1319,19 @@ template <typename T> ems_status setup_ebe(ems_tet *s)
     EMS_HIP(hipMemcpy(s->d_bnd.p, b.data(), b.size(), hipMemcpyHostToDevice));
     // per tetrahedron: bit l set when local dof l is a boundary dof
     {
+        // boundary mask per tetrahedron, type-major ([t * ncube + cube], as tmp)
         std::vector<unsigned> tm((size_t)m.nt, 0u);
-        long tid = 0, dd[20];
+        const size_t ncube = (size_t)m.nx * m.ny * m.nz;
+        long dd[20];
+        size_t cube = 0;
         for (int k = 0; k < m.nz; ++k)
             for (int j = 0; j < m.ny; ++j)
-                for (int i = 0; i < m.nx; ++i)
-                    for (int t = 0; t < 6; ++t, ++tid) {
+                for (int i = 0; i < m.nx; ++i, ++cube)
+                    for (int t = 0; t < 6; ++t) {
                         tet_dofs(m, order, i, j, k, t, dd);
                         unsigned w = 0;
                         for (int l = 0; l < nl; ++l) w |= (s->bnd[dd[l]] ? 1u : 0u) << l;
-                        tm[tid] = w;
+                        tm[(size_t)t * ncube + cube] = w;
                     }
         EMS_TRY(s->d_tmask.reserve(sizeof(unsigned) * tm.size()));
         EMS_HIP(hipMemcpy(s->d_tmask.p, tm.data(), sizeof(unsigned) * tm.size(), hipMemcpyHostToDevice));
