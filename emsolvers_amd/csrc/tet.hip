@@ -97,6 +97,17 @@ __device__ __forceinline__ long ebe_dof(const EbeMesh &m, int t, int l, int ci, 
     return 2 * m.ne + 2 * ebe_at(m.foff, m.fcx, m.fcy, m.tface[t][q], ci, cj, ck) + ((l - 12) & 1);
 }
 
+// (I, J, K) of index r = I + cx (J + cy K), in 32-bit unsigned arithmetic
+// (ems_tet_create keeps every cube and family index below 2^31; 64-bit
+// division expands to a long instruction sequence per lane)
+__device__ __forceinline__ void cube_ijk(unsigned r, unsigned cx, unsigned cy, int &I, int &J, int &K)
+{
+    const unsigned q = r / cx;
+    I = (int)(r - q * cx);
+    J = (int)(q % cy);
+    K = (int)(q / cy);
+}
+
 // row type and base vertex of a dof (see row_types on the host)
 __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J, int &K)
 {
@@ -106,10 +117,7 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
         int f = 0;
 #pragma unroll
         for (int q = 1; q < 7; ++q) f += e >= m.off[q];
-        const int rem = (int)(e - m.off[f]);
-        I = rem % m.ecx[f];
-        J = (rem / m.ecx[f]) % m.ecy[f];
-        K = rem / (m.ecx[f] * m.ecy[f]);
+        cube_ijk((unsigned)(e - m.off[f]), (unsigned)m.ecx[f], (unsigned)m.ecy[f], I, J, K);
         return f + 7 * kind;
     }
     const long fd = r - 2 * m.ne;
@@ -117,10 +125,7 @@ __device__ __forceinline__ int ebe_row(const EbeMesh &m, long r, int &I, int &J,
     int q = 0;
 #pragma unroll
     for (int w = 1; w < 12; ++w) q += fid >= m.foff[w];
-    const int rem = (int)(fid - m.foff[q]);
-    I = rem % m.fcx[q];
-    J = (rem / m.fcx[q]) % m.fcy[q];
-    K = rem / (m.fcx[q] * m.fcy[q]);
+    cube_ijk((unsigned)(fid - m.foff[q]), (unsigned)m.fcx[q], (unsigned)m.fcy[q], I, J, K);
     return 14 + 2 * q + (int)(fd & 1);
 }
 
@@ -206,7 +211,8 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
     const long cube = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     if (cube >= m.ncube) return;
-    const int ci = (int)(cube % m.nx), cj = (int)((cube / m.nx) % m.ny), ck = (int)(cube / ((long)m.nx * m.ny));
+    int ci, cj, ck;
+    cube_ijk((unsigned)cube, (unsigned)m.nx, (unsigned)m.ny, ci, cj, ck);
     const long tid = 6 * cube + t;
     const bool pre = MODE == EBE_MASS && cmt;
     const double epsr = (!pre && er) ? er[tid] : 1.0, sig = (!pre && sg) ? sg[tid] : 0.0;
@@ -328,8 +334,8 @@ __global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__res
     const int n = lane & 15, kq = lane >> 4;
     const long cube = base + n;
     const bool valid = cube < m.ncube;
-    const int ci = valid ? (int)(cube % m.nx) : 0, cj = valid ? (int)((cube / m.nx) % m.ny) : 0;
-    const int ck = valid ? (int)(cube / ((long)m.nx * m.ny)) : 0;
+    int ci = 0, cj = 0, ck = 0;
+    if (valid) cube_ijk((unsigned)cube, (unsigned)m.nx, (unsigned)m.ny, ci, cj, ck);
     const T *M = Mt + t * NL * NL;
     typedef typename MfmaAcc<T>::V V;
     V acc[MT];
