@@ -886,6 +886,9 @@ struct ems_tet {
     double t_now = 0.0;
     bool abm_started = false;
     int cg_last = 0, cg_max = 0;
+    // after a run or gradient failed part way the device state is undefined:
+    // bit 0 / bit 1 = E / dE/dt must be set again before the next run
+    int need_state = 0;
     hipStream_t stream = nullptr;
     // misfit gradient (allocated on first use)
     DevBuf g_t2d, g_mloc, g_bnd, g_lU, g_lV, g_mu, g_y, g_vout, g_tmp, g_ge, g_gs, g_res, g_f;
@@ -1360,6 +1363,26 @@ template <typename T> ems_status setup_ebe(ems_tet *s)
     return EMS_OK;
 }
 
+// State and solver vectors.  U and V hold the state: they are zeroed when
+// first allocated only, so that a field set at rest survives the reassembly
+// that a later set_material / set_operator triggers.
+template <typename T> ems_status alloc_vectors(ems_tet *s)
+{
+    const size_t vb = sizeof(T) * s->nd;
+    for (DevBuf *v : {&s->U, &s->V}) {
+        const bool fresh = v->p == nullptr || v->bytes < vb;
+        EMS_TRY(v->reserve(vb));
+        if (fresh) EMS_HIP(hipMemset(v->p, 0, vb));
+    }
+    for (DevBuf *v : {&s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up, &s->Vp, &s->Ap}) {
+        EMS_TRY(v->reserve(vb));
+        EMS_HIP(hipMemset(v->p, 0, vb));
+    }
+    EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgMaxNB));
+    EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
+    return EMS_OK;
+}
+
 template <typename T> ems_status setup(ems_tet *s)
 {
     if (s->assembled) return EMS_OK;
@@ -1369,14 +1392,7 @@ template <typename T> ems_status setup(ems_tet *s)
     if (s->ebe) {
         EMS_TRY(setup_ebe<T>(s));
         EMS_TRY(stage_matrices<T>(s));
-        const size_t vb = sizeof(T) * nd;
-        for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up,
-                          &s->Vp, &s->Ap}) {
-            EMS_TRY(v->reserve(vb));
-            EMS_HIP(hipMemset(v->p, 0, vb));
-        }
-        EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgMaxNB));
-        EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
+        EMS_TRY(alloc_vectors<T>(s));
         s->assembled = true;
         return EMS_OK;
     }
@@ -1396,14 +1412,7 @@ template <typename T> ems_status setup(ems_tet *s)
     if (s->lossy) EMS_TRY(upload<T>(s->d_Ms, Ms));
     Ms = std::vector<double>();
     EMS_TRY(stage_matrices<T>(s));
-    const size_t vb = sizeof(T) * nd;
-    for (DevBuf *v : {&s->U, &s->V, &s->b, &s->r, &s->z, &s->p, &s->ap, &s->Vm, &s->am, &s->an, &s->Up, &s->Vp,
-                      &s->Ap}) {
-        EMS_TRY(v->reserve(vb));
-        EMS_HIP(hipMemset(v->p, 0, vb));
-    }
-    EMS_TRY(s->d_partial.reserve(sizeof(double) * 4 * kCgMaxNB));
-    EMS_TRY(s->d_cg.reserve(sizeof(CgState)));
+    EMS_TRY(alloc_vectors<T>(s));
     int longest = 0;
     for (long r = 0; r < nd; ++r) longest = std::max(longest, s->rp[r + 1] - s->rp[r]);
     s->lpr = longest > 32 ? 8 : 4;
@@ -1709,7 +1718,12 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
     const long total = s->steps + N;
     if (total / tf + 1 > 2147483647L / 3 / std::max(nobs, 1)) return invalid("too many receiver samples");
     const int nslot = (int)(total / tf + 1);
-    if (nslot > s->nslot) {
+    if (s->steps == 0) {
+        // a run from rest defines every slot: none of an earlier (longer)
+        // run or gradient may survive in the tail
+        s->traces.assign((size_t)nobs * nslot * 3, 0.0);
+        s->nslot = nslot;
+    } else if (nslot > s->nslot) {
         std::vector<double> t((size_t)nobs * nslot * 3, 0.0);
         for (int o = 0; o < nobs; ++o)
             for (int q = 0; q < s->nslot; ++q)
@@ -2100,12 +2114,31 @@ ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3])
     return EMS_OK;
 }
 
+namespace {
+
+// A run that fails after its first device step leaves U, V and the solver
+// history part way through a step: mark the state as undefined (argument
+// errors are raised before any device work and leave it intact).
+ems_status run_checked(ems_tet *s, int num_steps, double *total_ms)
+{
+    if (s->need_state) {
+        set_error("the state is undefined after a failed run: set E and dE/dt (ems_tet_set_field) first");
+        return EMS_ERR_STATE;
+    }
+    const ems_status st =
+        s->esz == 4 ? run_impl<float>(s, num_steps, total_ms) : run_impl<double>(s, num_steps, total_ms);
+    if (st != EMS_OK && st != EMS_ERR_INVALID_ARG) s->need_state = 3;
+    return st;
+}
+
+}  // namespace
+
 ems_status ems_tet_run(ems_tet *s, int num_steps)
 {
     clear_error();
     if (!s) return invalid("null handle");
     if (num_steps < 0) return invalid("num_steps must be >= 0");
-    return s->esz == 4 ? run_impl<float>(s, num_steps, nullptr) : run_impl<double>(s, num_steps, nullptr);
+    return run_checked(s, num_steps, nullptr);
 }
 
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms)
@@ -2113,7 +2146,7 @@ ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms)
     clear_error();
     if (!s || !total_ms) return invalid("null argument");
     if (num_steps < 0) return invalid("num_steps must be >= 0");
-    return s->esz == 4 ? run_impl<float>(s, num_steps, total_ms) : run_impl<double>(s, num_steps, total_ms);
+    return run_checked(s, num_steps, total_ms);
 }
 
 ems_status ems_tet_set_operator(ems_tet *s, int op)
@@ -2153,8 +2186,13 @@ ems_status ems_tet_gradient(ems_tet *s, int num_steps, const double *d_obs, size
     if (num_steps < 1) return invalid("num_steps must be >= 1");
     const int nobs = (int)s->obs_dof.size() / s->nl;
     if (count != (size_t)nobs * (num_steps / s->d.tf + 1) * 3) return invalid("d_obs has the wrong size");
-    return s->esz == 4 ? gradient_impl<float>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift)
-                       : gradient_impl<double>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift);
+    // the gradient starts from rest and ends at rest: it defines the state
+    const ems_status st = s->esz == 4
+                              ? gradient_impl<float>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift)
+                              : gradient_impl<double>(s, num_steps, d_obs, misfit, grad_eps_r, grad_sigma, drift);
+    if (st == EMS_OK) s->need_state = 0;
+    else if (st != EMS_ERR_INVALID_ARG && st != EMS_ERR_NOT_SUPPORTED) s->need_state = 3;
+    return st;
 }
 
 ems_status ems_tet_trace_shape(const ems_tet *s, int *nobs, int *nslot)
@@ -2224,6 +2262,7 @@ ems_status ems_tet_set_field(ems_tet *s, int which, const double *in, size_t cou
     s->abm_started = false;
     s->have_prev = 0;  // a new state: no stage solution to extrapolate
     s->comp_valid = false;
+    s->need_state &= ~(1 << which);
     return EMS_OK;
 }
 

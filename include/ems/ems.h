@@ -418,7 +418,7 @@ void ems_tet_destroy(ems_tet *s);
 /* Relative permittivity and conductivity per tetrahedron (6 nx ny nz
  * values, tetrahedron 6 (i + nx (j + ny k)) + t); NULL = 1 / 0.  At rest
  * only (before the first step, or after ems_tet_gradient); the next run
- * reassembles. */
+ * reassembles and keeps a field set with ems_tet_set_field. */
 ems_status ems_tet_set_material(ems_tet *s, const double *eps_r, const double *sigma);
 
 /* Electric dipole of moment p(t) d at pos, p(t) = amp exp(-(pi f0 (t - t0))^2):
@@ -433,9 +433,18 @@ ems_status ems_tet_add_receiver(ems_tet *s, const double pos[3]);
  * matrices on the device; no reassembly). */
 ems_status ems_tet_set_time_step(ems_tet *s, double dt);
 
-/* Operator choice (EMS_TET_OP_*), at rest only; the next run rebuilds. */
+/* Operator choice (EMS_TET_OP_*), at rest only; the next run rebuilds (a
+ * field set with ems_tet_set_field is kept). */
 ems_status ems_tet_set_operator(ems_tet *s, int op);
 
+/* Advances num_steps steps.  A run from rest (no step taken yet, or after
+ * ems_tet_gradient) sizes the traces to exactly num_steps/tf + 1 slots; later
+ * runs append.  If a run fails after device work began (a mass solve that
+ * does not converge returns EMS_ERR_STATE, or a HIP error), the field state
+ * is undefined: the step count, time and traces keep the values of the last
+ * completed call, and run returns EMS_ERR_STATE until both E and dE/dt are
+ * set again with ems_tet_set_field (or a gradient succeeds, which leaves the
+ * handle at rest). */
 ems_status ems_tet_run(ems_tet *s, int num_steps);
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);
 

@@ -40,22 +40,29 @@ def golden():
 def tf_misfits(traces, ref, dt):
     """Largest single-valued TF envelope and phase misfits (Kristekova et al.)
     over the observation components of traces [nobs][nslot][3] against the
-    reference traces; dt is the slot spacing.  Components that are not finite
-    in the reference, or whose peak is below 1e-2 of the largest reference
-    sample, are skipped: rounding errors scale with the whole field, so on a
+    reference traces; dt is the slot spacing.  Both trace sets must be finite
+    (a NaN or Inf fails the assertion).  Components whose peak is below 1e-2
+    of the largest reference sample are skipped: rounding errors scale with
+    the whole field, so on a
     weak component (or the cancellation noise of one the source does not
     drive) a relative criterion measures the strong components' rounding."""
     from emsolvers_amd import tfmisfit
+    traces = np.asarray(traces, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    # a blown-up trace must fail, never fold away as a skipped component
+    assert np.all(np.isfinite(traces)), "tf_misfits: compared traces hold NaN/Inf"
+    assert np.all(np.isfinite(ref)), "tf_misfits: reference traces hold NaN/Inf"
     nslot = ref.shape[1]
     fmin, fmax = 2.0 / (nslot * dt), 0.25 / dt
-    finite = ref[np.isfinite(ref)]
-    floor = 1e-2 * np.abs(finite).max() if finite.size else 0.0
+    floor = 1e-2 * np.abs(ref).max() if ref.size else 0.0
     em = pm = 0.0
     for o in range(ref.shape[0]):
         for c in range(ref.shape[2]):
             r, s = ref[o, :, c], traces[o, :, c]
-            if not np.all(np.isfinite(r)) or not np.abs(r).max() > floor:
+            if not np.abs(r).max() > floor:
                 continue
             m = tfmisfit.misfits(s, r, dt, fmin, fmax, nf=40)
-            em, pm = max(em, m["EM"]), max(pm, m["PM"])
+            assert np.isfinite(m["EM"]) and np.isfinite(m["PM"]), \
+                "tf_misfits: non-finite misfit at observation %d component %d" % (o, c)
+            em, pm = float(np.fmax(em, m["EM"])), float(np.fmax(pm, m["PM"]))
     return em, pm

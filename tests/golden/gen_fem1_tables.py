@@ -15,7 +15,7 @@ Outputs (both committed):
   tests/golden/fem1_tables.json         numerator/denominator pairs (fixture)
   emsolvers_amd/csrc/fem1_tables.inc    the same as C integer tables
 
-Usage: python emsolvers_amd/csrc/gen_fem1_tables.py [/root/reference]
+Usage: python tests/golden/gen_fem1_tables.py [/root/reference]
 """
 import json
 import math
@@ -26,6 +26,7 @@ from fractions import Fraction
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "emsolvers_amd", "csrc")
 
 NAMES = ["Amat", "_Amat", "Bmat", "_Bmat", "Cmat", "_Cmat"]
 
@@ -86,7 +87,7 @@ def main():
         f.write("\n")
     # C include: one int numerator table per matrix and a common denominator
     lines = ["// Element-matrix tables of the first-order stabilized hex solvers,",
-             "// generated by emsolvers_amd/csrc/gen_fem1_tables.py from",
+             "// generated by tests/golden/gen_fem1_tables.py from",
              "// src/fem/fem_simulation.cpp:133-406 and src/fem_abm/fem_simulation.cpp:146-189.",
              "// value = num[r*12+c] / den (IEEE double division).", ""]
     for key, tabs in sets.items():
@@ -105,7 +106,7 @@ def main():
             for r in range(12):
                 lines.append("    " + ", ".join("%d" % v for v in nums[r * 12:(r + 1) * 12]) + ",")
             lines.append("};")
-    with open(os.path.join(HERE, "fem1_tables.inc"), "w") as f:
+    with open(os.path.join(CSRC, "fem1_tables.inc"), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("wrote tests/golden/fem1_tables.json and emsolvers_amd/csrc/fem1_tables.inc")
 

@@ -163,6 +163,52 @@ def test_initial_field_and_receivers():
     assert rel_l2(s.get_field(0), o.U) < 1e-10
 
 
+@pytest.mark.parametrize("then", ["material", "operator"])
+def test_field_survives_material_and_operator_change(then):
+    """set_field, then set_material or set_operator (both legal at rest and
+    both trigger a reassembly), then run: the set field is the initial state,
+    as in the oracle."""
+    case = dict(CASES["s4-lossy"], dip=[])
+    er, sg = materials(case)
+    o = make("oracle", case)
+    rng = np.random.default_rng(5)
+    U0 = rng.standard_normal(o.mesh.ne)
+    U0[o.bnd] = 0.0
+    o.set_field(0, U0)
+    tr_o = o.run(30)
+    from emsolvers_amd.tet import TetSimulation
+    s = TetSimulation(case["grid"], case["h"], case["dt"], case["eps"], case["mu"], case["tf"], case["stepper"])
+    for r in case["rec"]:
+        s.add_receiver(r)
+    s.set_material(er, sg)
+    s.run(0)  # assemble once, so that the change below is a reassembly
+    s.set_field(0, U0)
+    if then == "material":
+        s.set_material(er, sg)
+    else:
+        s.set_operator("csr")
+    np.testing.assert_array_equal(s.get_field(0), U0)
+    s.run(30)
+    assert rel_l2(s.traces(), tr_o) < 1e-10
+    assert rel_l2(s.get_field(0), o.U) < 1e-10
+
+
+def test_run_after_gradient_has_fresh_traces():
+    """A gradient over N steps, then a run from rest over M < N steps: the
+    traces hold exactly M/tf + 1 slots, all of the new run."""
+    case = CASES["s2"]
+    s = make("gpu", case)
+    s.run(40)
+    tr_ref = s.traces()
+    s2 = make("gpu", case)
+    d_obs = np.zeros((len(case["rec"]), 60 // case["tf"] + 1, 3))
+    s2.gradient(60, d_obs)
+    s2.run(40)
+    tr = s2.traces()
+    assert tr.shape == tr_ref.shape == (len(case["rec"]), 40 // case["tf"] + 1, 3)
+    np.testing.assert_array_equal(tr, tr_ref)
+
+
 def test_lossless_energy_bounded_on_device():
     """The lossless symplectic_2 trajectory keeps its discrete energy (read
     back through the fields) within a few percent after the source ends."""
