@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libems_oracle.so")
+# EMS_ORACLE_LIB selects another build of the same sources (the mutation
+# tests load deliberately altered copies through it)
+_LIB_PATH = os.environ.get("EMS_ORACLE_LIB") or os.path.join(_HERE, "_build", "libems_oracle.so")
 _lib = None
 
 WAVELET_HERTZ, WAVELET_RICKER, WAVELET_GAUSS = 0, 1, 2
