@@ -2228,12 +2228,12 @@ ems_status ems_tet_get_field(ems_tet *s, int which, double *out, size_t count)
     if (!s || !out) return invalid("null argument");
     if (which < 0 || which > 1) return invalid("field must be 0 (E) or 1 (dE/dt)");
     if (count != (size_t)s->nd) return invalid("field buffer has the wrong size");
-    if (!s->assembled) {
+    DevBuf &b = which == 0 ? s->U : s->V;
+    if (!b.p || b.bytes < count * s->esz) {  // never allocated: the state is at rest
         std::fill(out, out + count, 0.0);
         return EMS_OK;
     }
     EMS_HIP(hipStreamSynchronize(s->stream));
-    DevBuf &b = which == 0 ? s->U : s->V;
     if (s->esz == 8) {
         EMS_HIP(hipMemcpy(out, b.p, count * 8, hipMemcpyDeviceToHost));
     } else {

@@ -40,26 +40,30 @@ def golden():
 def tf_misfits(traces, ref, dt):
     """Largest single-valued TF envelope and phase misfits (Kristekova et al.)
     over the observation components of traces [nobs][nslot][3] against the
-    reference traces; dt is the slot spacing.  Both trace sets must be finite
-    (a NaN or Inf fails the assertion).  Components whose peak is below 1e-2
-    of the largest reference sample are skipped: rounding errors scale with
-    the whole field, so on a
-    weak component (or the cancellation noise of one the source does not
+    reference traces; dt is the slot spacing.  The traces must be finite
+    wherever the reference is (a NaN or Inf there fails the assertion);
+    components the reference leaves non-finite (the 0/0 average of an empty
+    receiver list) are skipped, as are components whose peak is below 1e-2
+    of the largest reference sample: rounding errors scale with the whole
+    field, so on a weak component (or the cancellation noise of one the source does not
     drive) a relative criterion measures the strong components' rounding."""
     from emsolvers_amd import tfmisfit
     traces = np.asarray(traces, dtype=np.float64)
     ref = np.asarray(ref, dtype=np.float64)
-    # a blown-up trace must fail, never fold away as a skipped component
-    assert np.all(np.isfinite(traces)), "tf_misfits: compared traces hold NaN/Inf"
-    assert np.all(np.isfinite(ref)), "tf_misfits: reference traces hold NaN/Inf"
+    # a blown-up trace must fail, never fold away as a skipped component: the
+    # only non-finite samples allowed are those of the reference itself (its
+    # 0/0 average of an empty receiver list), at the same places
+    bad = ~np.isfinite(ref)
+    assert np.array_equal(~np.isfinite(traces), bad), "tf_misfits: non-finite samples where the reference is finite"
     nslot = ref.shape[1]
     fmin, fmax = 2.0 / (nslot * dt), 0.25 / dt
-    floor = 1e-2 * np.abs(ref).max() if ref.size else 0.0
+    finite = ref[~bad]
+    floor = 1e-2 * np.abs(finite).max() if finite.size else 0.0
     em = pm = 0.0
     for o in range(ref.shape[0]):
         for c in range(ref.shape[2]):
             r, s = ref[o, :, c], traces[o, :, c]
-            if not np.abs(r).max() > floor:
+            if bad[o, :, c].any() or not np.abs(r).max() > floor:
                 continue
             m = tfmisfit.misfits(s, r, dt, fmin, fmax, nf=40)
             assert np.isfinite(m["EM"]) and np.isfinite(m["PM"]), \
