@@ -62,6 +62,7 @@ SIGNATURES = {
     "ems_fdtd_set_tiling": (_I, [_P, _I, _I, _I]),
     "ems_fdtd_run": (_I, [_P, _I]),
     "ems_fdtd_run_timed": (_I, [_P, _I, _DP, _DP]),
+    "ems_fdtd_run_static": (_I, [_P, _I]),
     "ems_fdtd_trace_shape": (_I, [_P, _IP, _IP]),
     "ems_fdtd_get_traces": (_I, [_P, _DP, _SZ]),
     "ems_fdtd_field_size": (_I, [_P, _I, ctypes.POINTER(_SZ)]),

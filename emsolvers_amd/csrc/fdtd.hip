@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -515,8 +516,15 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
     }
     rp = RunPlan();
     rp.t_end = s->t_now;
-    if (num_steps <= 0) return EMS_OK;
+    // the handle's own device state (CPML, fused-kernel ping-pong set and
+    // tiles) is set up even by a zero-step run, which so prepares a handle
+    // for ems_fdtd_run_static
     EMS_TRY(setup_pml(s));
+    if (fused_setup && use_fused(s)) {
+        EMS_TRY(ensure_alt(s));
+        EMS_TRY(build_tiles(s));
+    }
+    if (num_steps <= 0) return EMS_OK;
 
     // sources sorted by component (x list, then y, then z; :222-239)
     std::vector<Src> src = s->src;
@@ -565,10 +573,6 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
     if (npts) {
         EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
         EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
-    }
-    if (fused_setup && use_fused(s)) {
-        EMS_TRY(ensure_alt(s));
-        EMS_TRY(build_tiles(s));
     }
     rp.nsrc = nsrc;
     rp.npts = npts;
@@ -1653,6 +1657,210 @@ ems_status batch_check(ems_batch *b)
 }  // namespace
 
 // =========================================================================
+// ems_fdtd_run_static: ems_fdtd_run without device allocation
+// =========================================================================
+//
+// The per-call staging of a run (source component and index lists, the
+// per-step source values, the receiver point lists and the raw receiver
+// samples) lives in one file-scope device array per GPU instead of the
+// handle's grow-only buffers.  Every call on a GPU shares that GPU's array:
+// calls that overlap on one GPU (two threads, two streams) race; calls that
+// follow each other do not (each call waits for its kernels before it
+// returns).  Every byte a call reads from the array was written by the same
+// call first, so nothing needs clearing.
+namespace {
+
+constexpr size_t kStaticScratchBytes = (size_t)8 << 20;  // per GPU
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per GPU
+// (the array's host-side name is no device address).
+ems_status static_scratch(void **out)
+{
+    int dev = -1;
+    EMS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return invalid("ems_fdtd_run_static: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        EMS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_error("ems_fdtd_run_static: no device address for the static scratch array");
+            return EMS_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return EMS_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Copies n values produced by gen(i) to dst through a fixed host buffer.
+template <typename V, typename Gen> ems_status stage_values(V *dst, size_t n, Gen gen)
+{
+    constexpr size_t kChunk = 512;
+    V buf[kChunk];
+    for (size_t i0 = 0; i0 < n; i0 += kChunk) {
+        const size_t m = std::min(kChunk, n - i0);
+        for (size_t i = 0; i < m; ++i) buf[i] = gen(i0 + i);
+        EMS_HIP(hipMemcpy(dst + i0, buf, sizeof(V) * m, hipMemcpyHostToDevice));
+    }
+    return EMS_OK;
+}
+
+template <typename T> ems_status run_static_impl(ems_fdtd *s, int num_steps)
+{
+    if (s->npml > 0 && s->dirty_pml)
+        return (set_error("ems_fdtd_run_static: the CPML is not set up yet (run ems_fdtd_run(s, 0) first)"),
+                EMS_ERR_STATE);
+    if (use_fused(s) && (!s->alt_ready || !s->tiles_ok || s->tiles_key[0] != s->tile_by ||
+                         s->tiles_key[1] != s->tile_lz || s->tiles_key[2] != s->npml))
+        return (set_error("ems_fdtd_run_static: the fused kernel is not set up yet (run ems_fdtd_run(s, 0) first)"),
+                EMS_ERR_STATE);
+    const int tf = s->d.tf;
+    // sources in the order of run_prepare (stable by component)
+    int nsrc = (int)s->src.size();
+    auto src_at = [&](int q) -> const Src & {
+        int k = 0;
+        for (int c = 0; c < 3; ++c)
+            for (const Src &x : s->src)
+                if (x.comp == c && k++ == q) return x;
+        return s->src[0];
+    };
+    size_t npts = 0;
+    for (auto &o : s->obs)
+        for (int c = 0; c < 3; ++c) npts += o[c].size();
+    const size_t nrec = (size_t)(num_steps / tf);
+    const size_t b_sc = align256(sizeof(int) * nsrc), b_su = align256(sizeof(long) * nsrc);
+    const size_t b_pc = align256(sizeof(int) * npts), b_pu = align256(sizeof(long) * npts);
+    const size_t b_sv = align256(sizeof(double) * (size_t)num_steps * nsrc);
+    const size_t b_rec = align256(sizeof(double) * nrec * npts);
+    const size_t need = b_sc + b_su + b_pc + b_pu + b_sv + b_rec;
+    if (need > kStaticScratchBytes)
+        return invalid("ems_fdtd_run_static: this call needs " + std::to_string(need) +
+                       " bytes of staging, more than the static maximum of " + std::to_string(kStaticScratchBytes) +
+                       " (fewer steps per call, sources or receiver points)");
+    void *base = nullptr;
+    EMS_TRY(static_scratch(&base));
+    unsigned char *bp = static_cast<unsigned char *>(base);
+    int *d_sc = reinterpret_cast<int *>(bp);
+    long *d_su = reinterpret_cast<long *>(bp + b_sc);
+    int *d_pc = reinterpret_cast<int *>(bp + b_sc + b_su);
+    long *d_pu = reinterpret_cast<long *>(bp + b_sc + b_su + b_pc);
+    double *d_sv = reinterpret_cast<double *>(bp + b_sc + b_su + b_pc + b_pu);
+    double *d_rec = reinterpret_cast<double *>(bp + b_sc + b_su + b_pc + b_pu + b_sv);
+
+    // trace slots: the resize semantics of ems_fdtd_run (host bookkeeping)
+    const int nslot_new = num_steps / tf + 1;
+    const int nobs = (int)s->obs.size();
+    if (nslot_new > s->nslot) {
+        std::vector<double> t((size_t)nobs * nslot_new * 3, 0.0);
+        for (int o = 0; o < nobs; ++o)
+            for (int q = 0; q < s->nslot; ++q)
+                for (int c = 0; c < 3; ++c)
+                    t[((size_t)o * nslot_new + q) * 3 + c] = s->traces[((size_t)o * s->nslot + q) * 3 + c];
+        s->traces.swap(t);
+        s->nslot = nslot_new;
+    }
+    if (num_steps <= 0) return EMS_OK;
+    EMS_TRY(stage_values(d_sc, (size_t)nsrc, [&](size_t q) { return src_at((int)q).comp; }));
+    EMS_TRY(stage_values(d_su, (size_t)nsrc, [&](size_t q) { return src_at((int)q).u; }));
+    // per-step injected values, t accumulated as current_time_ (as run_prepare)
+    const double t0 = s->t_now, dt = s->d.dt;
+    double t_end = t0;
+    for (int st = 0; st < num_steps; ++st) t_end += dt;
+    if (nsrc) {
+        double t = t0;
+        int st_cur = 0;
+        double w = wavelet_value(s, t);
+        EMS_TRY(stage_values(d_sv, (size_t)num_steps * nsrc, [&](size_t i) {
+            const int st = (int)(i / nsrc), q = (int)(i % nsrc);
+            while (st_cur < st) {  // values are generated in step order
+                t += dt;
+                ++st_cur;
+                w = wavelet_value(s, t);
+            }
+            const Src &x = src_at(q);
+            double v = w * dt / x.eps;
+            if (x.scale != 1.0) v = v / x.scale;
+            return v;
+        }));
+    }
+    // receiver points in observation order
+    auto point_at = [&](size_t i, int &comp) -> long {
+        size_t k = 0;
+        for (auto &o : s->obs)
+            for (int c = 0; c < 3; ++c) {
+                if (i < k + o[c].size()) {
+                    const auto &p = o[c][i - k];
+                    comp = c;
+                    return ref_to_unified(s->g, c, p[0], p[1], p[2]);
+                }
+                k += o[c].size();
+            }
+        comp = 0;
+        return 0;
+    };
+    EMS_TRY(stage_values(d_pc, npts, [&](size_t i) { int c; point_at(i, c); return c; }));
+    EMS_TRY(stage_values(d_pu, npts, [&](size_t i) { int c; return point_at(i, c); }));
+
+    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    for (int step = 1; step <= num_steps; ++step) {
+        const PmlView<T> pm = pml_view<T>(s);
+        launch_step<T>(s, ch, ce, pm);
+        const Fields<T> f = fields<T>(s);
+        EMS_HIP(hipGetLastError());
+        double *rec = nullptr;
+        if (npts && step % tf == 0) rec = d_rec + (size_t)(step / tf - 1) * npts;
+        if (nsrc || rec) {
+            k_post<T><<<1, 256, 0, s->stream>>>(f, nsrc, d_sc, d_su, d_sv + (size_t)(step - 1) * nsrc, (int)npts,
+                                                d_pc, d_pu, rec);
+            EMS_HIP(hipGetLastError());
+        }
+    }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    // combine the raw samples into the traces as run_finish does, one
+    // receiver row at a time through a fixed host buffer
+    s->t_now = t_end;
+    if (nrec > 0 && nobs > 0) {
+        constexpr size_t kRow = 512;
+        double row[kRow];
+        for (size_t r = 0; r < nrec; ++r) {
+            const int slot = (int)r + 1;
+            size_t p = 0, have = 0, at = 0;
+            auto next = [&]() -> ems_status {
+                if (at == have) {
+                    have = std::min(kRow, npts - p);
+                    EMS_HIP(hipMemcpy(row, d_rec + r * npts + p, sizeof(double) * have, hipMemcpyDeviceToHost));
+                    at = 0;
+                }
+                return EMS_OK;
+            };
+            for (int o = 0; o < nobs; ++o)
+                for (int c = 0; c < 3; ++c) {
+                    double &dst = s->traces[((size_t)o * s->nslot + slot) * 3 + c];
+                    const size_t cnt = s->obs[o][c].size();
+                    for (size_t q = 0; q < cnt; ++q, ++p) {
+                        EMS_TRY(next());
+                        dst += row[at++];
+                    }
+                    dst /= (double)cnt;
+                }
+        }
+    }
+    return EMS_OK;
+}
+
+}  // namespace
+
+// =========================================================================
 // C ABI
 // =========================================================================
 extern "C" {
@@ -1820,6 +2028,14 @@ ems_status ems_fdtd_run(ems_fdtd *s, int num_steps)
     if (num_steps < 0) return invalid("num_steps must be >= 0");
     return s->d.precision == EMS_F32 ? run_impl<float>(s, num_steps, nullptr, nullptr)
                                      : run_impl<double>(s, num_steps, nullptr, nullptr);
+}
+
+ems_status ems_fdtd_run_static(ems_fdtd *s, int num_steps)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (num_steps < 0) return invalid("num_steps must be >= 0");
+    return s->d.precision == EMS_F32 ? run_static_impl<float>(s, num_steps) : run_static_impl<double>(s, num_steps);
 }
 
 ems_status ems_fdtd_run_timed(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)

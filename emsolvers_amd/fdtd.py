@@ -88,6 +88,11 @@ class FemSimulation:
     def run(self, num_steps):
         _lib.check(_lib.load().ems_fdtd_run(self._h, int(num_steps)))
 
+    def run_static(self, num_steps):
+        """ems_fdtd_run_static: the same run with its staging in the
+        library's static device array (no device allocation)."""
+        _lib.check(_lib.load().ems_fdtd_run_static(self._h, int(num_steps)))
+
     def saveResults(self, num_steps, c, directory=""):
         _lib.check(_lib.load().ems_fdtd_save_results(self._h, int(num_steps), float(c),
                                                      directory.encode()))

@@ -129,6 +129,22 @@ ems_status ems_fdtd_run(ems_fdtd *s, int num_steps);
  * the Yee update kernel(s) alone.  Same results as ems_fdtd_run. */
 ems_status ems_fdtd_run_timed(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same steps, traces and fields as ems_fdtd_run, bit for bit,
+ * but the per-call staging (source and receiver lists, the per-step source
+ * values, the raw receiver samples) lives in a static 8 MiB device array per
+ * GPU instead of the handle's buffers: no hipMalloc, hipFree or other
+ * device-allocating call.  The only host allocation is the growth of the
+ * trace array, as in ems_fdtd_run.  EMS_ERR_INVALID_ARG when one call's
+ * staging exceeds 8 MiB (e.g. 32 receiver points x 32768 recorded steps);
+ * EMS_ERR_STATE until the handle's own device state exists: call
+ * ems_fdtd_run(s, 0) once after set_cpml / set_kernel / set_tiling.  All
+ * calls on one GPU share its array: calls that overlap on one GPU (two
+ * threads, two streams) race; consecutive calls do not (each returns after
+ * its kernels finish). */
+ems_status ems_fdtd_run_static(ems_fdtd *s, int num_steps);
+
 /* Trace shape: nobs observation sets, nslot = num_steps/tf + 1 slots. */
 ems_status ems_fdtd_trace_shape(const ems_fdtd *s, int *nobs, int *nslot);
 
