@@ -60,6 +60,7 @@ struct ems_fdtd {
     int split_by = 0;    // split kernels: workgroup rows (0: 4 with a CPML, else 8)
     int split_lz = 0;    // split kernels: planes per thread (0: 4 in FP64 with a CPML, else 2)
     int fused_rev = 0;   // fused kernel: tiles of this step in reverse order
+    int z_ty = 4, z_lz = 8, z_nw = 1;  // z-march kernel: strip rows, planes per chunk, waves per workgroup
     // fused-kernel tile lists (interior / CPML-touching)
     DevBuf d_tiles_inner, d_tiles_pml;
     int n_inner = 0, n_pml = 0, ntx = 0, nty = 0;
@@ -463,17 +464,65 @@ template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch,
 // with a CPML (0.38 vs 0.57 ms FP32, 0.67 vs 0.86 ms FP64) and in FP32
 // without one (0.260 vs 0.267 ms); the fused sweep in FP64 without a CPML
 // (0.50 vs 0.53 ms).
-bool use_fused(const ems_fdtd *s)
+int kernel_kind(const ems_fdtd *s)
 {
-    if (s->kernel == EMS_KERNEL_SPLIT) return false;
-    if (s->kernel == EMS_KERNEL_FUSED) return true;
-    return s->npml <= 0 && s->d.precision == EMS_F64;
+    if (s->kernel != EMS_KERNEL_AUTO) return s->kernel;
+    return s->npml <= 0 && s->d.precision == EMS_F64 ? EMS_KERNEL_FUSED : EMS_KERNEL_SPLIT;
+}
+
+// The fused and z-march sweeps ping-pong between two field sets; only the
+// fused sweep uses tile lists.
+bool use_fused(const ems_fdtd *s) { return kernel_kind(s) != EMS_KERNEL_SPLIT; }
+bool use_tiles(const ems_fdtd *s) { return kernel_kind(s) == EMS_KERNEL_FUSED; }
+
+template <typename T, int TY, int NW, int MAT, bool PML>
+void launch_zmarch_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    const GridView &g = s->g;
+    PsiH<T> ph{};
+    if (PML)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q) {
+                ph.old_[ax][q] = s->psi[ax * 4 + q].as<T>();
+                ph.new_[ax][q] = s->psi2[ax * 2 + q].as<T>();
+            }
+    ZTiles zt;
+    zt.ntx = (g.nx + 1 + 61) / 62;
+    const int nstrip = (g.ny + 1 + TY - 1) / TY;
+    zt.nsg = (nstrip + NW - 1) / NW;
+    zt.lz = s->z_lz;
+    const int nkc = (g.nz + 1 + zt.lz - 1) / zt.lz;
+    zt.count = zt.ntx * zt.nsg * nkc;
+    zt.per = (zt.count + 7) / 8;
+    Fields<T> o = fields<T>(s);
+    Fields<T> nw{s->f2[0].as<T>(), s->f2[1].as<T>(), s->f2[2].as<T>(),
+                 s->f2[3].as<T>(), s->f2[4].as<T>(), s->f2[5].as<T>()};
+    k_zmarch<T, TY, NW, MAT, PML><<<zt.per * 8, dim3(64, NW, 1), 0, s->stream>>>(s->g, ch, ce, o, nw, s->ca.as<T>(),
+                                                                                s->cb.as<T>(), pm, ph, zt);
+    for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
+    if (PML)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 2 + q]);
+}
+
+template <typename T, int MAT, bool PML> void launch_zmarch(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
+{
+    if (s->z_ty == 2) {
+        if (s->z_nw == 1) launch_zmarch_t<T, 2, 1, MAT, PML>(s, ch, ce, pm);
+        else launch_zmarch_t<T, 2, 4, MAT, PML>(s, ch, ce, pm);
+    } else {
+        if (s->z_nw == 1) launch_zmarch_t<T, 4, 1, MAT, PML>(s, ch, ce, pm);
+        else launch_zmarch_t<T, 4, 4, MAT, PML>(s, ch, ce, pm);
+    }
 }
 
 template <typename T, int MAT, bool PML> void launch_variant(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
-    if (!use_fused(s)) launch_split<T, MAT, PML>(s, ch, ce, pm);
-    else launch_fused_by<T, MAT, PML>(s, ch, ce, pm);
+    switch (kernel_kind(s)) {
+    case EMS_KERNEL_FUSED: launch_fused_by<T, MAT, PML>(s, ch, ce, pm); break;
+    case EMS_KERNEL_ZMARCH: launch_zmarch<T, MAT, PML>(s, ch, ce, pm); break;
+    default: launch_split<T, MAT, PML>(s, ch, ce, pm); break;
+    }
 }
 
 // One Yee step; the psi pointers of pm are refreshed by the caller after a
@@ -520,10 +569,8 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
     // tiles) is set up even by a zero-step run, which so prepares a handle
     // for ems_fdtd_run_static
     EMS_TRY(setup_pml(s));
-    if (fused_setup && use_fused(s)) {
-        EMS_TRY(ensure_alt(s));
-        EMS_TRY(build_tiles(s));
-    }
+    if (fused_setup && use_fused(s)) EMS_TRY(ensure_alt(s));
+    if (fused_setup && use_tiles(s)) EMS_TRY(build_tiles(s));
     if (num_steps <= 0) return EMS_OK;
 
     // sources sorted by component (x list, then y, then z; :222-239)
@@ -953,10 +1000,8 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
     }
     const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    if (use_fused(s)) {
-        EMS_TRY(ensure_alt(s));
-        EMS_TRY(build_tiles(s));
-    }
+    if (use_fused(s)) EMS_TRY(ensure_alt(s));
+    if (use_tiles(s)) EMS_TRY(build_tiles(s));
     // one forward step n -> n+1 (sources; receivers when `record`)
     auto fstep = [&](int n, bool record) -> ems_status {
         const PmlView<T> pm = pml_view<T>(s);
@@ -1719,8 +1764,9 @@ template <typename T> ems_status run_static_impl(ems_fdtd *s, int num_steps)
     if (s->npml > 0 && s->dirty_pml)
         return (set_error("ems_fdtd_run_static: the CPML is not set up yet (run ems_fdtd_run(s, 0) first)"),
                 EMS_ERR_STATE);
-    if (use_fused(s) && (!s->alt_ready || !s->tiles_ok || s->tiles_key[0] != s->tile_by ||
-                         s->tiles_key[1] != s->tile_lz || s->tiles_key[2] != s->npml))
+    if ((use_fused(s) && !s->alt_ready) ||
+        (use_tiles(s) && (!s->tiles_ok || s->tiles_key[0] != s->tile_by || s->tiles_key[1] != s->tile_lz ||
+                          s->tiles_key[2] != s->npml)))
         return (set_error("ems_fdtd_run_static: the fused kernel is not set up yet (run ems_fdtd_run(s, 0) first)"),
                 EMS_ERR_STATE);
     const int tf = s->d.tf;
@@ -2001,8 +2047,21 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_FUSED) return invalid("unknown kernel variant");
+    if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_ZMARCH) return invalid("unknown kernel variant");
     s->kernel = variant;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_zmarch(ems_fdtd *s, int rows, int planes, int waves)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (rows != 2 && rows != 4) return invalid("z-march strip rows must be 2 or 4");
+    if (planes < 1) return invalid("z-march planes per chunk must be >= 1");
+    if (waves != 1 && waves != 4) return invalid("z-march waves per workgroup must be 1 or 4");
+    s->z_ty = rows;
+    s->z_lz = planes;
+    s->z_nw = waves;
     return EMS_OK;
 }
 

@@ -928,6 +928,310 @@ __global__ __launch_bounds__(64 * BY) void k_fused(GridView g, T ch, T ce, Field
     }
 }
 
+// ---------------------------------------------------------------------------
+// Z-marching fused sweep: one launch per step, no LDS and no barriers.
+//
+// Reads E^n, H^{n-1/2} from the old buffers and writes H^{n+1/2}, E^{n+1}
+// into the new ones (ping-pong, as k_fused; the H-psi too), so every field
+// word crosses HBM about once in each direction per step.  Every wavefront is
+// independent: lane l holds column I = 62 t - 1 + l of x-tile t, the wave
+// owns the 62 columns of lanes 1..62 and a strip of TY rows J0 .. J0+TY-1,
+// and it marches a chunk of lz planes.
+//   I+1 / I-1: the next / previous lane's registers by DPP wave shifts; lane
+//              0 (column I0-1) only computes H, lane 63 only provides E^n;
+//   J+1 / J-1: the strip's rows are unrolled in registers; row J0-1 is an H
+//              halo row, row J0+TY provides Ex, Ez;
+//   K+1 / K-1: the next plane's E^n is loaded a plane ahead, H^{n+1/2} of the
+//              plane below stays in registers; the plane below the chunk is
+//              an H halo plane.
+// Halo H values are computed from the old buffers (old psi) and never
+// stored.  Per output the arithmetic is that of k_fused and the split
+// sweeps: the same bits.  Fields are addressed as wave-uniform row bases plus
+// the lane's 32-bit byte offset (lane_at).
+// ---------------------------------------------------------------------------
+
+// lane l receives v of lane l + 1 (DPP wave_shl:1); lane 63 receives 0
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+// lane l receives v of lane l - 1 (DPP wave_shr:1); lane 0 receives 0
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x130, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+struct ZTiles {
+    int ntx;    // x-tiles of 62 columns
+    int nsg;    // strip groups (NW strips of TY rows each)
+    int lz;     // planes per chunk
+    int count;  // workgroups
+    int per;    // workgroups per XCD: XCD c = b % 8 takes tiles [c per, (c+1) per)
+};
+
+template <typename T, int TY, int NW, int MAT, bool PML>
+__global__ __launch_bounds__(64 * NW) void k_zmarch(GridView g, T ch, T ce, Fields<T> o, Fields<T> nw,
+                                                     const T *__restrict__ ca, const T *__restrict__ cb,
+                                                     PmlView<T> pm, PsiH<T> ph, ZTiles zt)
+{
+    const int lb = (blockIdx.x & 7) * zt.per + (blockIdx.x >> 3);
+    if (lb >= zt.count) return;  // uniform for the workgroup
+    const int ti = lb % zt.ntx;
+    const int sg = (lb / zt.ntx) % (zt.nsg);
+    const int kc = lb / (zt.ntx * zt.nsg);
+    const int lane = threadIdx.x;
+    const int I = ti * 62 - 1 + lane;
+    const int J0 = __builtin_amdgcn_readfirstlane((sg * NW + (int)threadIdx.y) * TY);
+    if (J0 > g.ny) return;  // uniform for the wave; no barriers follow
+    const int K0 = kc * zt.lz;
+    const int K1 = min(K0 + zt.lz, g.nz + 1);
+    const int Kst = K0 > 0 ? K0 - 1 : 0;
+    const bool xin = I >= 0 && I <= g.nx;                  // a column of the update box
+    const bool xown = lane >= 1 && lane <= 62 && I <= g.nx;  // H stored and E computed here
+    const unsigned lo = (unsigned)(I + 1) * (unsigned)sizeof(T);
+    bool px = false;  // x-slab column (update box only)
+    Prof<T> hpx{}, epx{};
+    if (PML && xin) {
+        px = in_pml(pm.P, g.nx, I);
+        if (px) {
+            hpx = pm.hp[0][I];
+            epx = pm.ep[0][I];
+        }
+    }
+    // row r of the strip is J = J0 - 1 + r: r = 0 the H halo row, 1..TY own,
+    // TY + 1 the Ex / Ez provider
+    bool rv[TY + 2], ro[TY + 2];
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) {
+        const int J = J0 - 1 + r;
+        rv[r] = J >= 0 && J <= g.ny + 1;
+        ro[r] = r >= 1 && r <= TY && J <= g.ny;
+    }
+    // addresses: a wave-uniform plane base per array (plane K, column -1)
+    // plus the lane's byte offset within the plane, per row (32-bit: a plane
+    // is under 4 GiB); keeps the scalar registers to one base per array
+    unsigned lro[TY + 2];
+#pragma unroll
+    for (int r = 0; r < TY + 2; ++r) lro[r] = lo + (unsigned)((long)(J0 - 1 + r) * g.pitch) * (unsigned)sizeof(T);
+    auto pl = [&](auto *a, int K) { return a + (g.off - 1 + g.slab * (long)K); };
+    // Loads run ahead of their use so that a wave keeps a plane's worth of
+    // loads in flight: E^n two planes ahead (H(K) needs E(K+1)), H^{n-1/2}
+    // and the material one plane ahead.  All loads of a plane are issued
+    // before its stores (the old and new buffers never alias, but the
+    // compiler cannot see it through the lane addressing).
+    T ex0[TY + 2], ey0[TY + 2], ez0[TY + 2];  // E^n of plane K
+    T ex1[TY + 2], ey1[TY + 2], ez1[TY + 2];  // E^n of plane K + 1
+    T hx0[TY + 1], hy0[TY + 1], hz0[TY + 1];  // H^{n-1/2} of plane K
+    T cb0[TY + 1], ca0[TY + 1];               // material of plane K (own rows)
+    T hxp[TY + 1], hyp[TY + 1];               // H^{n+1/2} of the plane below, per own row
+    auto load_e = [&](int K, T (&ex)[TY + 2], T (&ey)[TY + 2], T (&ez)[TY + 2]) {
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            ex[r] = ey[r] = ez[r] = (T)0;
+            if (rv[r] && K <= K1) {  // H(K1 - 1) needs E(K1); K1 <= nz + 1
+                ex[r] = lane_at(pl(o.ex, K), lro[r]);
+                ez[r] = lane_at(pl(o.ez, K), lro[r]);
+                if (r <= TY) ey[r] = lane_at(pl(o.ey, K), lro[r]);
+            }
+        }
+    };
+    auto load_h = [&](int K, T (&hx)[TY + 1], T (&hy)[TY + 1], T (&hz)[TY + 1], T (&cbv)[TY + 1],
+                      T (&cav)[TY + 1]) {
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            hx[r] = hy[r] = hz[r] = cbv[r] = cav[r] = (T)0;
+            if (rv[r] && K < K1) {
+                hx[r] = lane_at(pl(o.hx, K), lro[r]);
+                if (r > 0) hy[r] = lane_at(pl(o.hy, K), lro[r]);
+                hz[r] = lane_at(pl(o.hz, K), lro[r]);
+                if (ro[r] && K >= K0) {
+                    if (MAT >= 1) cbv[r] = lane_at(pl(cb, K), lro[r]);
+                    if (MAT == 2) cav[r] = lane_at(pl(ca, K), lro[r]);
+                }
+            }
+        }
+    };
+#pragma unroll
+    for (int r = 0; r <= TY; ++r) hxp[r] = hyp[r] = (T)0;
+    load_e(Kst, ex0, ey0, ez0);
+    load_e(Kst + 1, ex1, ey1, ez1);
+    load_h(Kst, hx0, hy0, hz0, cb0, ca0);
+    for (int K = Kst; K < K1; ++K) {
+        const bool eplane = K >= K0;
+        const bool pz = PML && in_pml(pm.P, g.nz, K);
+        Prof<T> hpz{}, epz{};
+        if (pz) {
+            hpz = pm.hp[2][K];
+            epz = pm.ep[2][K];
+        }
+        // in flight while this plane is computed
+        T ex2[TY + 2], ey2[TY + 2], ez2[TY + 2];
+        T hx1[TY + 1], hy1[TY + 1], hz1[TY + 1], cb1[TY + 1], ca1[TY + 1];
+        load_e(K + 2, ex2, ey2, ez2);
+        load_h(K + 1, hx1, hy1, hz1, cb1, ca1);
+        T hzm = 0, hxm = 0;  // H^{n+1/2} of the row below in this plane
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            if (!rv[r]) continue;  // uniform; only row 0 of the first strip and rows past ny + 1
+            const int J = J0 - 1 + r;
+            const bool own = ro[r] && eplane;  // uniform
+            const bool wr = own && xown;
+            const T ez_i1 = lane_next(ez0[r]);
+            const T ey_i1 = lane_next(ey0[r]);
+            const T ex_k = ex0[r], ey_k = ey0[r], ez_k = ez0[r];
+            const T ez_j1 = ez0[r + 1], ex_j1 = ex0[r + 1];
+            const T ex_k1 = ex1[r], ey_k1 = ey1[r];
+            const T hxo = hx0[r], hyo = hy0[r], hzo = hz0[r];
+            const bool py = PML && in_pml(pm.P, g.ny, J);
+            // CPML terms only inside the update box: H at I = nx + 1 or
+            // J = ny + 1 is neither stored nor used
+            const bool yin = J <= g.ny;
+            const bool pyl = py && xin && yin, pzl = pz && xin && yin, pxl = px && yin;
+            T hxn, hyn, hzn;
+            // Hx: (Ez(J+1) - Ez(J)) - Ey(K+1) + Ey(K)
+            if (PML && (pyl || pzl)) {
+                T t1 = ez_j1 - ez_k, t2 = ey_k1 - ey_k;
+                if (pyl) {
+                    const int q = slab_y(g, pm.P, pm.W, I, J, K);
+                    const Prof<T> pr = pm.hp[1][J];
+                    const T p = pr.b * ph.old_[1][0][q] + pr.a * t1;
+                    if (wr) ph.new_[1][0][q] = p;
+                    t1 = t1 * pr.k + p;
+                }
+                if (pzl) {
+                    const int q = slab_z(g, pm.P, pm.W, I, J, K);
+                    const T p = hpz.b * ph.old_[2][0][q] + hpz.a * t2;
+                    if (wr) ph.new_[2][0][q] = p;
+                    t2 = t2 * hpz.k + p;
+                }
+                hxn = hxo - ch * (t1 - t2);
+            } else {
+                hxn = hxo - ch * (ez_j1 - ez_k - ey_k1 + ey_k);
+            }
+            // Hy: (Ex(K+1) - Ex(K)) - Ez(I+1) + Ez(I)
+            if (PML && (pzl || pxl)) {
+                T t1 = ex_k1 - ex_k, t2 = ez_i1 - ez_k;
+                if (pzl) {
+                    const int q = slab_z(g, pm.P, pm.W, I, J, K);
+                    const T p = hpz.b * ph.old_[2][1][q] + hpz.a * t1;
+                    if (wr) ph.new_[2][1][q] = p;
+                    t1 = t1 * hpz.k + p;
+                }
+                if (pxl) {
+                    const int q = slab_x(g, pm.P, pm.W, I, J, K);
+                    const T p = hpx.b * ph.old_[0][0][q] + hpx.a * t2;
+                    if (wr) ph.new_[0][0][q] = p;
+                    t2 = t2 * hpx.k + p;
+                }
+                hyn = hyo - ch * (t1 - t2);
+            } else {
+                hyn = hyo - ch * (ex_k1 - ex_k - ez_i1 + ez_k);
+            }
+            // Hz: (Ey(I+1) - Ey(I)) - Ex(J+1) + Ex(J)
+            if (PML && (pxl || pyl)) {
+                T t1 = ey_i1 - ey_k, t2 = ex_j1 - ex_k;
+                if (pxl) {
+                    const int q = slab_x(g, pm.P, pm.W, I, J, K);
+                    const T p = hpx.b * ph.old_[0][1][q] + hpx.a * t1;
+                    if (wr) ph.new_[0][1][q] = p;
+                    t1 = t1 * hpx.k + p;
+                }
+                if (pyl) {
+                    const int q = slab_y(g, pm.P, pm.W, I, J, K);
+                    const Prof<T> pr = pm.hp[1][J];
+                    const T p = pr.b * ph.old_[1][1][q] + pr.a * t2;
+                    if (wr) ph.new_[1][1][q] = p;
+                    t2 = t2 * pr.k + p;
+                }
+                hzn = hzo - ch * (t1 - t2);
+            } else {
+                hzn = hzo - ch * (ey_i1 - ey_k - ex_j1 + ex_k);
+            }
+            if (own) {
+                const T hz_im1 = lane_prev(hzn), hy_im1 = lane_prev(hyn);
+                if (xown) {
+                    const T cbu = cb0[r], cau = ca0[r];
+                    if (I >= 1) lane_at(pl(nw.hx, K), lro[r]) = hxn;
+                    if (J >= 1) lane_at(pl(nw.hy, K), lro[r]) = hyn;
+                    if (K >= 1) lane_at(pl(nw.hz, K), lro[r]) = hzn;
+                    const T hy_km1 = hyp[r], hx_km1 = hxp[r];
+                    if (J >= 1 && K >= 1) {  // Ex: (Hz(J) - Hz(J-1)) - Hy(K) + Hy(K-1)
+                        T br;
+                        if (PML && (py || pz)) {
+                            T t1 = hzn - hzm, t2 = hyn - hy_km1;
+                            if (py) t1 = cpml_term(pm.psi_e[1][0], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t1);
+                            if (pz) t2 = cpml_term(pm.psi_e[2][0], slab_z(g, pm.P, pm.W, I, J, K), epz, t2);
+                            br = t1 - t2;
+                        } else {
+                            br = hzn - hzm - hyn + hy_km1;
+                        }
+                        lane_at(pl(nw.ex, K), lro[r]) = MAT == 0 ? ex_k + ce * br : MAT == 1 ? ex_k + cbu * br : cau * ex_k + cbu * br;
+                    }
+                    if (I >= 1 && K >= 1) {  // Ey: (Hx(K) - Hx(K-1)) - Hz(I) + Hz(I-1)
+                        T br;
+                        if (PML && (pz || px)) {
+                            T t1 = hxn - hx_km1, t2 = hzn - hz_im1;
+                            if (pz) t1 = cpml_term(pm.psi_e[2][1], slab_z(g, pm.P, pm.W, I, J, K), epz, t1);
+                            if (px) t2 = cpml_term(pm.psi_e[0][0], slab_x(g, pm.P, pm.W, I, J, K), epx, t2);
+                            br = t1 - t2;
+                        } else {
+                            br = hxn - hx_km1 - hzn + hz_im1;
+                        }
+                        lane_at(pl(nw.ey, K), lro[r]) = MAT == 0 ? ey_k + ce * br : MAT == 1 ? ey_k + cbu * br : cau * ey_k + cbu * br;
+                    }
+                    if (I >= 1 && J >= 1) {  // Ez: (Hy(I) - Hy(I-1)) - Hx(J) + Hx(J-1)
+                        T br;
+                        if (PML && (px || py)) {
+                            T t1 = hyn - hy_im1, t2 = hxn - hxm;
+                            if (px) t1 = cpml_term(pm.psi_e[0][1], slab_x(g, pm.P, pm.W, I, J, K), epx, t1);
+                            if (py) t2 = cpml_term(pm.psi_e[1][1], slab_y(g, pm.P, pm.W, I, J, K), pm.ep[1][J], t2);
+                            br = t1 - t2;
+                        } else {
+                            br = hyn - hy_im1 - hxn + hxm;
+                        }
+                        lane_at(pl(nw.ez, K), lro[r]) = MAT == 0 ? ez_k + ce * br : MAT == 1 ? ez_k + cbu * br : cau * ez_k + cbu * br;
+                    }
+                }
+            }
+            hxp[r] = hxn;
+            hyp[r] = hyn;
+            hzm = hzn;
+            hxm = hxn;
+        }
+#pragma unroll
+        for (int r = 0; r < TY + 2; ++r) {
+            ex0[r] = ex1[r];
+            ey0[r] = ey1[r];
+            ez0[r] = ez1[r];
+            ex1[r] = ex2[r];
+            ey1[r] = ey2[r];
+            ez1[r] = ez2[r];
+        }
+#pragma unroll
+        for (int r = 0; r <= TY; ++r) {
+            hx0[r] = hx1[r];
+            hy0[r] = hy1[r];
+            hz0[r] = hz1[r];
+            cb0[r] = cb1[r];
+            ca0[r] = ca1[r];
+        }
+    }
+}
+
 }  // namespace fdtd
 }  // namespace ems
 

@@ -124,6 +124,11 @@ class FemSimulation:
         stores, bit 2 split sweeps with a CPML as interior + CPML-shell launches)."""
         _lib.check(_lib.load().ems_fdtd_set_tiling(self._h, int(rows), int(planes), int(flags)))
 
+    def set_zmarch(self, rows=4, planes=8, waves=1):
+        """Tiling of the z-march sweep (kernel 3): strip rows (2, 4), planes
+        per chunk, wavefronts per workgroup (1, 4)."""
+        _lib.check(_lib.load().ems_fdtd_set_zmarch(self._h, int(rows), int(planes), int(waves)))
+
     def run_timed(self, num_steps):
         """Returns (total_ms, kernel_ms) measured with HIP events."""
         tot, ker = ctypes.c_double(), ctypes.c_double()

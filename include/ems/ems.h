@@ -48,7 +48,7 @@ enum {
 };
 
 /* kernel variants for ems_fdtd_set_kernel */
-enum { EMS_KERNEL_AUTO = 0, EMS_KERNEL_SPLIT = 1, EMS_KERNEL_FUSED = 2 };
+enum { EMS_KERNEL_AUTO = 0, EMS_KERNEL_SPLIT = 1, EMS_KERNEL_FUSED = 2, EMS_KERNEL_ZMARCH = 3 };
 
 /* Last error message of the calling thread ("" if none). */
 const char *ems_last_error(void);
@@ -106,8 +106,17 @@ ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0,
  * reference).  t0 < 0 selects sqrt(2)/f0. */
 ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, double amp);
 
-/* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT or EMS_KERNEL_FUSED. */
+/* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT (an H sweep and an E sweep
+ * in place), EMS_KERNEL_FUSED (one tiled sweep with LDS exchange, ping-pong
+ * buffers) or EMS_KERNEL_ZMARCH (one sweep of independent wavefronts: DPP
+ * lane shifts in x, unrolled rows in y, register z-march; ping-pong
+ * buffers).  Results do not depend on it. */
 ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
+
+/* Tiling of EMS_KERNEL_ZMARCH: rows per wavefront strip (2 or 4), planes
+ * per chunk (>= 1), wavefronts per workgroup (1 or 4).  Defaults 4, 8, 1.
+ * No reference counterpart. */
+ems_status ems_fdtd_set_zmarch(ems_fdtd *s, int rows, int planes, int waves);
 
 /* Kernel tiling: fused-kernel workgroup rows (8 or 16; 63 x (rows-1) cells
  * per plane), planes per workgroup or per split-kernel thread (>= 1) and

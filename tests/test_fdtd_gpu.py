@@ -27,7 +27,7 @@ def gpu_sim(grid, h, dt, eps, mu, tf, precision="f64", kernel=None):
     return s
 
 
-KERNELS = [1, 2]  # split, fused
+KERNELS = [1, 2, 3]  # split, fused, z-march
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -112,6 +112,22 @@ def test_split_and_fused_agree_bitwise_fp32():
     b.run(case["nsteps"])
     for w in range(6):
         assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+
+
+@pytest.mark.parametrize("rows,planes,waves", [(2, 1, 1), (2, 5, 4), (4, 3, 4), (4, 1000, 1)])
+def test_zmarch_tilings_agree_bitwise_with_split_fp32(rows, planes, waves):
+    """The z-march sweep's strips, chunks and workgroups change only its halo
+    recomputation: FP32 fields equal the split sweeps' bit for bit."""
+    case = fc.PARITY_CASES["lossy_cpml_rect"]
+    print("z-march tiling", rows, planes, waves, flush=True)
+    a = build_gpu(case, "f32", 1)
+    b = build_gpu(case, "f32", 3)
+    b.set_zmarch(rows, planes, waves)
+    a.run(case["nsteps"])
+    b.run(case["nsteps"])
+    for w in range(6):
+        assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+    assert np.array_equal(a.traces(), b.traces())
 
 
 @pytest.mark.parametrize("rows,planes,flags", [(8, 1, 0), (8, 7, 1), (8, 8, 2), (16, 3, 3), (16, 1000, 1)])
