@@ -92,7 +92,7 @@ def gpu_fem1(stepper, n, steps, precision="f64"):
 
 
 def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, seed=5, sigma_fraction=0.0,
-            power_iters=300, op="auto"):
+            power_iters=300, op="auto", solver="auto"):
     """Tetrahedral Nedelec elements (order 1 or 2) on the Kuhn split of an n^3
     unit cube: seeded smoothed log-uniform eps_r (sigma uniform in [0, 0.05]
     S/m on a seeded fraction of the tetrahedra), x-dipole (Gaussian pulse)
@@ -107,6 +107,7 @@ def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, 
     t0 = time.time()
     s = TetSimulation((n, n, n), h, 1e-3 * h / wl.C0, wl.EPS0, wl.MU0, 1, stepper, order, precision)
     s.set_operator(op)
+    s.set_solver(solver)
     s.set_material(er, sg)
     wmax = s.max_frequency(power_iters)
     setup_s = time.time() - t0
@@ -124,7 +125,7 @@ def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, 
                ms_per_step=round(ms / steps, 4), steps=steps, stepper=stepper, order=order, dt=dt,
                omega_max=wmax, cg_iters_last=last, cg_iters_max=mx, setup_s=round(setup_s, 2),
                finite=bool(np.all(np.isfinite(tr))), dtype="fp64" if precision == "f64" else "fp32",
-               sigma_fraction=sigma_fraction, operator=op)
+               sigma_fraction=sigma_fraction, operator=op, solver=solver)
     s.close()
     return out
 

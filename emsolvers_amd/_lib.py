@@ -139,6 +139,7 @@ SIGNATURES = {
     "ems_tet_get_field": (_I, [_P, _I, _DP, _SZ]),
     "ems_tet_set_field": (_I, [_P, _I, _DP, _SZ]),
     "ems_tet_get_time": (_I, [_P, _DP]),
+    "ems_tet_set_solver": (_I, [_P, _I]),
     "ems_tet_solver_stats": (_I, [_P, _IP, _IP]),
     "ems_tet_max_frequency": (_I, [_P, _I, _DP]),
 }

@@ -396,6 +396,56 @@ template <typename T> struct EbeOp {
     }
 };
 
+// The mass-type operator (eps0 eps_r + hh sigma) M in one pass per row, with
+// no per-tetrahedron buffer: row r forms, for each tetrahedron of its
+// incidence list (fixed order), that tetrahedron's local product row L,
+// cm * sum_l M_t[L][l] x_l (boundary dofs as 0), and sums them.  The same
+// expressions in the same order as phase 1 + phase 2 of the element-by-element
+// operator, so the same bits; the tetrahedron's dofs are gathered once per
+// row instead of once per tetrahedron (cache hits), and the tmp round trip
+// through HBM disappears.  Boundary rows are identity rows.
+template <typename T, int NL> struct EbeRowOp {
+    static constexpr int lanes = 1;
+    EbeMesh m;
+    const int *inc;
+    const T *Mt;             // [type][NL][NL]
+    const unsigned *tmask;   // [type][cube]: bit l = local dof l on the boundary
+    const T *cmt;            // [type][cube]: eps0 eps_r + hh sigma
+    const unsigned char *bnd;
+    __device__ __forceinline__ int n() const { return (int)m.nd; }
+    __device__ __forceinline__ T apply(const T *__restrict__ x, int r, int) const
+    {
+        if (bnd[r]) return x[r];
+        int I, J, K;
+        const int rt = ebe_row(m, r, I, J, K);
+        const int q0 = m.inc_start[rt], cnt = m.inc_start[rt + 1] - q0;
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < kMaxInc; ++q) {
+            T v = 0;
+            if (q < cnt) {
+                const int c = inc[q0 + q];
+                const int ci = I + (c & 3) - 1, cj = J + ((c >> 2) & 3) - 1, ck = K + ((c >> 4) & 3) - 1;
+                if (ci >= 0 && cj >= 0 && ck >= 0 && ci < m.nx && cj < m.ny && ck < m.nz) {
+                    const int t = (c >> 6) & 7, L = (c >> 9) & 31;
+                    const long cube = ci + (long)m.nx * (cj + (long)m.ny * ck);
+                    const unsigned bm = tmask[(long)t * m.ncube + cube];
+                    const T *Mr = Mt + (t * NL + L) * NL;
+                    T acc = 0;
+#pragma unroll
+                    for (int l = 0; l < NL; ++l) {
+                        const T a = ((bm >> l) & 1u) ? (T)0 : x[ebe_dof(m, t, l, ci, cj, ck)];
+                        acc += Mr[l] * a;
+                    }
+                    v = cmt[(long)t * m.ncube + cube] * acc;
+                }
+            }
+            s += v;
+        }
+        return s;
+    }
+};
+
 // y = A x (phase 2 / CSR row products)
 template <typename OP, typename T>
 __global__ __launch_bounds__(256) void k_apply(OP op, const T *__restrict__ x, T *__restrict__ y)
@@ -669,6 +719,80 @@ __global__ __launch_bounds__(kRB) void k_cg_direction(OP op, const T *__restrict
     if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it + 1;
 }
 
+// ---------------------------------------------------------------------------
+// Chebyshev acceleration of Jacobi for the mass solves (Saad, Iterative
+// Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1): with the spectrum
+// of D^-1 A inside [lo, hi], theta = (hi + lo)/2, delta = (hi - lo)/2,
+// sigma = theta/delta, rho_0 = 1/sigma, d_0 = D^-1 r_0 / theta and, for k >= 1,
+// rho_k = 1/(2 sigma - rho_k-1), d_k = rho_k rho_k-1 d_k-1 + (2 rho_k/delta)
+// D^-1 r_k, x_k+1 = x_k + d_k.  The coefficients are known in advance: no
+// inner products (no reductions, no host polls) between the first and the
+// last residual; the iteration count follows from the Chebyshev bound
+// 2 q^k, q = (sqrt(kappa) - 1)/(sqrt(kappa) + 1), kappa = hi/lo.
+// ---------------------------------------------------------------------------
+// r = b - A x (row i), z = D^-1 r, d = c1 d + c2 z, xn = x + d  (first: d = c2 z)
+template <typename OP, typename T>
+__global__ __launch_bounds__(kRB) void k_cheb_step(OP op, const T *__restrict__ b, const T *__restrict__ dinv,
+                                                   const T *__restrict__ x, T *__restrict__ d, T *__restrict__ xn,
+                                                   T c1, T c2, int first)
+{
+    constexpr int LPR = OP::lanes;
+    const int stride = gridDim.x * kRB / LPR;
+    for (int i = (blockIdx.x * kRB + threadIdx.x) / LPR; i < op.n(); i += stride) {
+        const int lane = threadIdx.x % LPR;
+        const T ax = op.apply(x, i, lane);
+        if (lane == 0) {
+            const T zi = dinv[i] * (b[i] - ax);
+            const T di = first ? c2 * zi : c1 * d[i] + c2 * zi;
+            d[i] = di;
+            xn[i] = x[i] + di;
+        }
+    }
+}
+
+// per-block partials of sum a_i b_i (w: 0) or sum a_i b_i / dinv_i (w: 1)
+template <typename T>
+__global__ __launch_bounds__(kRB) void k_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                             const T *__restrict__ dinv, int w, double *P)
+{
+    double s = 0.0;
+    for (int i = blockIdx.x * kRB + threadIdx.x; i < n; i += gridDim.x * kRB) {
+        const double v = (double)a[i] * (double)b[i];
+        s += w ? v / (double)dinv[i] : v;
+    }
+    block_partial(s, P);
+}
+
+// Lanczos vector update of D^-1 A in the D inner product:
+// w = D^-1 y - alpha v - beta vp
+template <typename T>
+__global__ void k_lanczos_w(int n, const T *__restrict__ dinv, const T *__restrict__ y, const T *__restrict__ v,
+                            const T *__restrict__ vp, double alpha, double beta, T *__restrict__ w)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    w[i] = (T)((double)dinv[i] * (double)y[i] - alpha * (double)v[i] - beta * (double)vp[i]);
+}
+
+// v = x / s
+template <typename T> __global__ void k_scale_to(int n, const T *__restrict__ x, double s, T *__restrict__ v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (T)((double)x[i] / s);
+}
+
+// deterministic start vector in [-1/2, 1/2] (integer hash of the index)
+template <typename T> __global__ void k_hash_fill(int n, T *__restrict__ v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned h = (unsigned)i * 2654435761u + 0x9e3779b9u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    v[i] = (T)((double)(h & 0xffffff) / 16777216.0 - 0.5);
+}
+
 // one block: scal[slot] = fixed-order sum of the slot's nb partials
 // One block of kFB threads: each thread sums its strided partials of both
 // slots in one pass (fixed order), then a fixed LDS tree.
@@ -868,6 +992,7 @@ struct ems_tet {
     int op_kind = EMS_TET_OP_AUTO;
     bool ebe = false;   // element-by-element operator (no assembled matrices)
     bool mfma = false;  // element products of the mass operator on matrix cores
+    bool rows = false;  // mass operator fused per row (EbeRowOp), no tmp buffer
     EbeMesh em{};
     DevBuf d_inc, d_Mt, d_St, d_Ct, d_er, d_sg, d_bnd, d_tmp, d_tmask;
     int nl = 6;   // basis functions per tetrahedron
@@ -886,6 +1011,14 @@ struct ems_tet {
     double t_now = 0.0;
     bool abm_started = false;
     int cg_last = 0, cg_max = 0;
+    // mass solver: EMS_TET_SOLVER_AUTO (Chebyshev-accelerated Jacobi with
+    // spectral bounds from Lanczos, CG where that fails) or _CG.  Bounds per
+    // solve matrix: slot 3 = M_eps, 0..2 = the stage steps' matrices.
+    int solver = EMS_TET_SOLVER_AUTO;
+    double cheb_lo[4] = {}, cheb_hi[4] = {};
+    int cheb_state[4] = {};  // 0 unknown, 1 bounds valid, -1 use CG
+    int cheb_steps = 0;      // Lanczos steps of the last bounds
+    DevBuf cd, cx, cx0;      // Chebyshev direction, ping-pong iterate, initial guess
     // after a run or gradient failed part way the device state is undefined:
     // bit 0 / bit 1 = E / dE/dt must be set again before the next run
     int need_state = 0;
@@ -1188,6 +1321,7 @@ template <typename T> ems_status stage_matrices(ems_tet *s)
     const long nd = s->nd, nnz = (long)s->col.size();
     const Csr<T> A{(int)nd, s->d_rp.as<int>(), s->d_col.as<int>()};
     s->deltas = stage_deltas(s);
+    for (int q = 0; q < 4; ++q) s->cheb_state[q] = 0;  // new matrices: new spectral bounds
     if (s->ebe) {
         // Jacobi diagonals of (eps0 eps_r + hh sigma) M per stage step (hh = 0: M_eps)
         for (size_t q = 0; s->lossy && q < s->deltas.size(); ++q)
@@ -1388,6 +1522,7 @@ template <typename T> ems_status setup(ems_tet *s)
     if (s->assembled) return EMS_OK;
     s->ebe = s->op_kind != EMS_TET_OP_CSR;
     s->mfma = s->op_kind == EMS_TET_OP_EBE_MFMA;
+    s->rows = s->op_kind == EMS_TET_OP_EBE_ROWS;
     const long nd = s->nd;
     if (s->ebe) {
         EMS_TRY(setup_ebe<T>(s));
@@ -1473,6 +1608,25 @@ template <typename T> EbeOp<T> ebe_op(ems_tet *s, int ident)
     return EbeOp<T>{s->em, s->d_inc.as<int>(), s->d_tmp.as<T>(), s->d_bnd.as<unsigned char>(), ident};
 }
 
+// Element-by-element mass operator of solve matrix `which` (M_eps for
+// which < 0) applied to x: f(op) launches the kernel that consumes op.  The
+// two-phase form runs phase 1 on x first; the per-row form needs nothing.
+template <typename T, typename F> void with_mass_op(ems_tet *s, int which, const T *x, const int *done, F &&f)
+{
+    const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
+    const T *cmt = (const T *)(which < 0 ? s->d_cm_me.p : s->a_cm[which]);
+    if (s->rows) {
+        with_nl(s->nl, [&](auto N) {
+            constexpr int NL = decltype(N)::value;
+            f(EbeRowOp<T, NL>{s->em, s->d_inc.as<int>(), s->d_Mt.as<T>(), s->d_tmask.as<unsigned>(), cmt,
+                              s->d_bnd.as<unsigned char>()});
+        });
+        return;
+    }
+    ebe_local<T, EBE_MASS>(s, hh, x, nullptr, done, cmt);
+    f(ebe_op<T>(s, 1));
+}
+
 // y = S x (which 0) or y = M_eps x (which 1)
 template <typename T> ems_status apply_op(ems_tet *s, int which, const T *x, T *y)
 {
@@ -1510,22 +1664,22 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     const int maxit = 512, check = 16, nb = cg_blocks(n);
     const bool fin = nb > kCgInlineNB;
     if (s->ebe) {
-        const double hh = (which < 0 || !s->lossy) ? 0.0 : 0.5 * s->deltas[which];
-        const T *cmt = (const T *)(which < 0 ? s->d_cm_me.p : s->a_cm[which]);  // null: from eps_r, sigma
-        const EbeOp<T> op = ebe_op<T>(s, 1);
-        ebe_local<T, EBE_MASS>(s, hh, xx, nullptr, nullptr, cmt);
-        k_cg_init<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+        with_mass_op<T>(s, which, xx, nullptr, [&](auto op) {
+            k_cg_init<decltype(op), T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+        });
         if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 0, 3, nb);
-        ebe_local<T, EBE_MASS>(s, hh, z, nullptr, nullptr, cmt);
-        k_cg_first<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
+        with_mass_op<T>(s, which, z, nullptr, [&](auto op) {
+            k_cg_first<decltype(op), T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, xx, P, st, tol2, nb);
+        });
         if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
         EMS_HIP(hipGetLastError());
         for (int it = 0; it < maxit; ++it) {
             const int par = it & 1;
             k_cg_update<T><<<nb, kRB, 0, s->stream>>>(n, xx, r, z, p, ap, dinv, P, par, st, nb);
             if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, par ^ 1, -1, nb);
-            ebe_local<T, EBE_MASS>(s, hh, z, nullptr, &st->done, cmt);
-            k_cg_direction<EbeOp<T>, T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
+            with_mass_op<T>(s, which, z, &st->done, [&](auto op) {
+                k_cg_direction<decltype(op), T><<<nb, kRB, 0, s->stream>>>(op, z, p, ap, P, par, st, it, tol2, nb);
+            });
             if (fin) k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 2, -1, nb);
             if ((it + 1) % check == 0) {
                 CgState h;
@@ -1577,6 +1731,241 @@ template <typename T> ems_status cg(ems_tet *s, int which, DevBuf &x)
     return EMS_ERR_STATE;
 }
 
+// y = A x with A = M_eps (which < 0) or the solve matrix of stage step `which`
+template <typename T> void apply_solve(ems_tet *s, int which, const T *x, T *y)
+{
+    const int n = (int)s->nd;
+    if (s->ebe) {
+        with_mass_op<T>(s, which, x, nullptr, [&](auto op) {
+            k_apply<decltype(op), T><<<(n + 255) / 256, 256, 0, s->stream>>>(op, x, y);
+        });
+    } else {
+        const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+        const T *val = (const T *)(which < 0 ? s->d_Me.p : s->a_val[which]);
+        with_lpr(s->lpr, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            k_spmv<LPR, T><<<row_blocks(n, LPR), 256, 0, s->stream>>>(A, val, x, y);
+        });
+    }
+}
+
+// sum a_i b_i (w 0) or sum a_i b_i / dinv_i (w 1), block partials summed on
+// the host in a fixed order
+template <typename T> ems_status dot_host(ems_tet *s, const T *a, const T *b, const T *dinv, int w, double &out)
+{
+    const int n = (int)s->nd, nb = cg_blocks(n);
+    double *P = s->d_partial.as<double>();
+    k_dot<T><<<nb, kRB, 0, s->stream>>>(n, a, b, dinv, w, P);
+    EMS_HIP(hipGetLastError());
+    std::vector<double> h((size_t)nb);
+    EMS_HIP(hipMemcpyAsync(h.data(), P, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    double t = 0.0;
+    for (double v : h) t += v;
+    out = t;
+    return EMS_OK;
+}
+
+// Eigenvalues of the symmetric tridiagonal matrix (diagonal a, off-diagonal
+// b[i] between rows i-1 and i): number below x by the Sturm sequence, and the
+// k-th smallest by bisection inside the Gershgorin interval.
+int sturm_count(const std::vector<double> &a, const std::vector<double> &b, double x)
+{
+    int c = 0;
+    double q = 1.0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        q = a[i] - x - (i > 0 ? b[i] * b[i] / q : 0.0);
+        if (q == 0.0) q = -1e-300;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+
+double tridiag_eig(const std::vector<double> &a, const std::vector<double> &b, int k)
+{
+    double lo = 1e300, hi = -1e300;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const double r = (i > 0 ? std::fabs(b[i]) : 0.0) + (i + 1 < a.size() ? std::fabs(b[i + 1]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (sturm_count(a, b, mid) > k) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// Spectral bounds of D^-1 A by Lanczos in the D inner product from a fixed
+// pseudo-random start, until both extreme Ritz values change by under 1e-4
+// over 8 steps (at most 240): the extreme Ritz values widened by 5 % (Ritz
+// values lie inside the spectrum).  Uses the CG scratch vectors.
+template <typename T> ems_status cheb_bounds(ems_tet *s, int which, double &lo, double &hi)
+{
+    const int n = (int)s->nd, g = (n + 255) / 256;
+    const T *dinv = (const T *)(which < 0 ? s->d_dinv_me.p : s->a_dinv[which]);
+    T *bufA = s->p.as<T>(), *bufB = s->ap.as<T>(), *w = s->r.as<T>(), *y = s->z.as<T>();
+    k_hash_fill<T><<<g, 256, 0, s->stream>>>(n, w);
+    double nn = 0.0;
+    EMS_TRY(dot_host(s, w, w, dinv, 1, nn));
+    if (!(nn > 0.0)) return not_supported("Lanczos start vector vanished");
+    T *vp = bufA, *v = bufB;
+    k_scale_to<T><<<g, 256, 0, s->stream>>>(n, w, std::sqrt(nn), v);
+    k_scale_to<T><<<g, 256, 0, s->stream>>>(n, w, HUGE_VAL, vp);  // 0
+    std::vector<double> al, be;
+    double beta = 0.0, last_lo = 0.0, last_hi = 0.0;
+    for (int j = 0; j < 240; ++j) {
+        apply_solve<T>(s, which, v, y);
+        double alpha = 0.0;
+        EMS_TRY(dot_host(s, y, v, dinv, 0, alpha));
+        k_lanczos_w<T><<<g, 256, 0, s->stream>>>(n, dinv, y, v, vp, alpha, beta, w);
+        double ww = 0.0;
+        EMS_TRY(dot_host(s, w, w, dinv, 1, ww));
+        al.push_back(alpha);
+        be.push_back(beta);
+        beta = std::sqrt(std::max(ww, 0.0));
+        if (!(beta > 1e-10 * std::fabs(alpha))) break;
+        k_scale_to<T><<<g, 256, 0, s->stream>>>(n, w, beta, vp);  // the next v, in the old vp
+        std::swap(v, vp);
+        // every 8 steps: stop once both extreme Ritz values are settled
+        if ((j + 1) % 8 == 0) {
+            const double a = tridiag_eig(al, be, 0), b = tridiag_eig(al, be, (int)al.size() - 1);
+            const bool settled = std::fabs(a - last_lo) <= 1e-4 * a && std::fabs(b - last_hi) <= 1e-4 * b;
+            last_lo = a;
+            last_hi = b;
+            if (settled) break;
+        }
+    }
+    EMS_HIP(hipGetLastError());
+    const int m = (int)al.size();
+    lo = 0.95 * tridiag_eig(al, be, 0);
+    hi = 1.05 * tridiag_eig(al, be, m - 1);
+    s->cheb_steps = m;
+    return EMS_OK;
+}
+
+// x <- A^-1 b (initial guess x) by Chebyshev-accelerated Jacobi: one residual
+// (with its inner products) before and one after a fixed number of
+// dot-free iterations; if the final residual misses the tolerance (bounds too
+// narrow), the matrix falls back to CG from the saved initial guess.
+template <typename T> ems_status solve(ems_tet *s, int which, DevBuf &x)
+{
+    // FP32: the true residual b - A x of a fixed-coefficient iteration
+    // stagnates near the FP32 rounding of A x, above the 1e-7 tolerance that
+    // CG's recursively updated residual reaches; CG there
+    // Order 2 (Jacobi condition number ~130): about 10 % more iterations
+    // than CG and slower per solve (measured); CG there too.
+    if (s->solver == EMS_TET_SOLVER_CG || s->esz == 4 || s->nl != 6) return cg<T>(s, which, x);
+    const int slot = which < 0 ? 3 : which;
+    if (s->cheb_state[slot] == 0) {
+        double lo = 0.0, hi = 0.0;
+        EMS_TRY(cheb_bounds<T>(s, which, lo, hi));
+        if (lo > 0.0 && hi > lo && std::isfinite(hi)) {
+            s->cheb_lo[slot] = lo;
+            s->cheb_hi[slot] = hi;
+            s->cheb_state[slot] = 1;
+        } else {
+            s->cheb_state[slot] = -1;
+        }
+    }
+    if (s->cheb_state[slot] < 0) return cg<T>(s, which, x);
+    const int n = (int)s->nd, nb = cg_blocks(n);
+    const double tol = s->esz == 4 ? 1e-7 : 1e-14;
+    CgState *st = s->d_cg.as<CgState>();
+    double *P = s->d_partial.as<double>();
+    T *r = s->r.as<T>(), *z = s->z.as<T>();
+    const T *b = s->b.as<T>();
+    const T *dinv = (const T *)(which < 0 ? s->d_dinv_me.p : s->a_dinv[which]);
+    const size_t vb = sizeof(T) * n;
+    EMS_TRY(s->cd.reserve(vb));
+    EMS_TRY(s->cx.reserve(vb));
+    EMS_TRY(s->cx0.reserve(vb));
+    EMS_HIP(hipMemcpyAsync(s->cx0.p, x.p, vb, hipMemcpyDeviceToDevice, s->stream));
+    // r = b - A xx, z = D^-1 r; (r, z) and (b, D^-1 b) to the host
+    auto residual = [&](const T *xx, double &rz, double &bb) -> ems_status {
+        if (s->ebe) {
+            with_mass_op<T>(s, which, xx, nullptr, [&](auto op) {
+                k_cg_init<decltype(op), T><<<nb, kRB, 0, s->stream>>>(op, dinv, b, xx, r, z, P, st, nb);
+            });
+        } else {
+            const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+            const T *val = (const T *)(which < 0 ? s->d_Me.p : s->a_val[which]);
+            with_lpr(s->lpr, [&](auto L) {
+                constexpr int LPR = decltype(L)::value;
+                k_cg_init<CsrOp<LPR, T>, T><<<nb, kRB, 0, s->stream>>>(CsrOp<LPR, T>{A, val}, dinv, b, xx, r, z, P, st,
+                                                                      nb);
+            });
+        }
+        k_cg_finalize<<<1, kFB, 0, s->stream>>>(P, st, 0, 3, nb);
+        EMS_HIP(hipGetLastError());
+        CgState h;
+        EMS_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+        EMS_HIP(hipStreamSynchronize(s->stream));
+        rz = h.scal[0];
+        bb = h.scal[3];
+        return EMS_OK;
+    };
+    double rz = 0.0, bb = 0.0;
+    EMS_TRY(residual(x.as<T>(), rz, bb));
+    if (!(bb > 0.0)) {  // b = 0: x = 0
+        k_scale_to<T><<<(n + 255) / 256, 256, 0, s->stream>>>(n, b, HUGE_VAL, x.as<T>());
+        s->cg_last = 0;
+        return EMS_OK;
+    }
+    if (!(rz > tol * tol * bb)) {
+        s->cg_last = 0;
+        return EMS_OK;
+    }
+    const double lo = s->cheb_lo[slot], hi = s->cheb_hi[slot];
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    const double sk = std::sqrt(hi / lo), q = (sk - 1.0) / (sk + 1.0);
+    const double need = 0.5 * tol * std::sqrt(bb / rz);
+    const int K = std::max(1, std::min(4000, (int)std::ceil(std::log(need) / std::log(q)) + 2));
+    T *cur = x.as<T>(), *nxt = s->cx.as<T>(), *d = s->cd.as<T>();
+    double rho_prev = 1.0 / sigma;
+    // no reductions in the steps: a full grid (latency is hidden by waves,
+    // not by the grid-stride loop of the CG kernels)
+    const int gs = (int)std::min<long>(((long)n + kRB - 1) / kRB, 1L << 20);
+    for (int k = 0; k < K; ++k) {
+        double c1 = 0.0, c2 = 1.0 / theta;
+        if (k > 0) {
+            const double rho = 1.0 / (2.0 * sigma - rho_prev);
+            c1 = rho * rho_prev;
+            c2 = 2.0 * rho / delta;
+            rho_prev = rho;
+        }
+        if (s->ebe) {
+            with_mass_op<T>(s, which, cur, nullptr, [&](auto op) {
+                k_cheb_step<decltype(op), T><<<gs, kRB, 0, s->stream>>>(op, b, dinv, cur, d, nxt, (T)c1, (T)c2, k == 0);
+            });
+        } else {
+            const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+            const T *val = (const T *)(which < 0 ? s->d_Me.p : s->a_val[which]);
+            with_lpr(s->lpr, [&](auto L) {
+                constexpr int LPR = decltype(L)::value;
+                const int gl = (int)std::min<long>(((long)n * LPR + kRB - 1) / kRB, 1L << 20);
+                k_cheb_step<CsrOp<LPR, T>, T><<<gl, kRB, 0, s->stream>>>(CsrOp<LPR, T>{A, val}, b, dinv, cur, d, nxt,
+                                                                         (T)c1, (T)c2, k == 0);
+            });
+        }
+        std::swap(cur, nxt);
+    }
+    EMS_HIP(hipGetLastError());
+    if (cur != x.as<T>()) EMS_HIP(hipMemcpyAsync(x.p, cur, vb, hipMemcpyDeviceToDevice, s->stream));
+    double rz2 = 0.0;
+    EMS_TRY(residual(x.as<T>(), rz2, bb));
+    if (!(rz2 > tol * tol * bb)) {
+        s->cg_last = K;
+        s->cg_max = std::max(s->cg_max, K);
+        return EMS_OK;
+    }
+    // bounds too narrow for this matrix: CG from the initial guess from now on
+    s->cheb_state[slot] = -1;
+    EMS_HIP(hipMemcpyAsync(x.p, s->cx0.p, vb, hipMemcpyDeviceToDevice, s->stream));
+    return cg<T>(s, which, x);
+}
+
 template <typename T> void load(ems_tet *s, DevBuf &b, const double *f)
 {
     const int nw = (int)s->dip_dof.size();
@@ -1613,7 +2002,7 @@ template <typename T> void force(ems_tet *s, const T *U, const T *V, const doubl
 template <typename T> ems_status accel(ems_tet *s, const T *U, const T *V, DevBuf &dst, const double *fdev)
 {
     force<T>(s, U, V, fdev);
-    return cg<T>(s, 0, dst);
+    return solve<T>(s, 0, dst);
 }
 
 // One symplectic stage of step D, midpoint tm in units of dt (load factors -p'
@@ -1658,7 +2047,7 @@ template <typename T> ems_status stage(ems_tet *s, int q, double D, double tm, c
         EMS_HIP(hipMemcpyAsync(s->X.p, s->Y.p, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
     else
         EMS_HIP(hipMemsetAsync(s->X.p, 0, sizeof(T) * n, s->stream));
-    EMS_TRY(cg<T>(s, q, s->X));
+    EMS_TRY(solve<T>(s, q, s->X));
     update((T)D, s->X.as<T>(), s->V, s->Vc);
     update((T)(0.5 * D), s->V.as<T>(), s->U, s->Uc);
     // history: Yp <- Y <- X
@@ -1699,7 +2088,7 @@ template <typename T> ems_status max_frequency(ems_tet *s, int iters, double *om
         if (it == iters) break;
         // x <- M_eps^-1 S x, normalised (b holds S x)
         EMS_HIP(hipMemset(dy.p, 0, sizeof(T) * n));
-        EMS_TRY(cg<T>(s, -1, dy));
+        EMS_TRY(solve<T>(s, -1, dy));
         EMS_HIP(hipMemcpy(x.data(), dy.p, sizeof(T) * n, hipMemcpyDeviceToHost));
         double nn = 0.0;
         for (int e = 0; e < n; ++e) nn += (double)x[e] * (double)x[e];
@@ -1968,13 +2357,13 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);  // U1
             EMS_TRY(apply_op<T>(s, 0, U, s->b.as<T>()));
             load<T>(s, s->b, s->g_f.as<double>() + fo * (size_t)std::max(ndip, 1));
-            EMS_TRY(cg<T>(s, -1, s->g_y));
+            EMS_TRY(solve<T>(s, -1, s->g_y));
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)D, s->g_y.as<T>(), V);        // V_in
             k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-0.5 * D), V, U);            // U_in
             // adjoint of the stage
             k_comb<T><<<g, 256, 0, s->stream>>>(n, lV, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr,
                                                 s->b.as<T>());
-            EMS_TRY(cg<T>(s, -1, s->g_mu));
+            EMS_TRY(solve<T>(s, -1, s->g_mu));
             if (nl == 6)
                 k_tet_grad<T, 6><<<tb, 128, 0, s->stream>>>(nt, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
                                                             s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), V, vout,
@@ -2153,7 +2542,7 @@ ems_status ems_tet_set_operator(ems_tet *s, int op)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (op < EMS_TET_OP_AUTO || op > EMS_TET_OP_EBE_MFMA) return invalid("unknown operator");
+    if (op < EMS_TET_OP_AUTO || op > EMS_TET_OP_EBE_ROWS) return invalid("unknown operator");
     if (s->steps > 0) return invalid("the operator can only change at rest (before the first step)");
     s->op_kind = op;
     s->assembled = false;
@@ -2322,6 +2711,15 @@ ems_status ems_tet_max_frequency(ems_tet *s, int iters, double *omega2)
 }
 
 // conjugate-gradient iterations of the last mass solve and the maximum so far
+ems_status ems_tet_set_solver(ems_tet *s, int solver)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (solver != EMS_TET_SOLVER_AUTO && solver != EMS_TET_SOLVER_CG) return invalid("unknown mass solver");
+    s->solver = solver;
+    return EMS_OK;
+}
+
 ems_status ems_tet_solver_stats(const ems_tet *s, int *last_iters, int *max_iters)
 {
     clear_error();

@@ -18,7 +18,7 @@ STEPPERS = {"symplectic_2": SYMPLECTIC2, "symplectic_4": SYMPLECTIC4, "abm": ABM
 # stability limits w_max dt < kappa of each stepper on y'' = -w^2 y
 KAPPA = {SYMPLECTIC2: 2.0, SYMPLECTIC4: 1.5735, ABM: 1.2005}
 # operators: assembled CSR, element-by-element (auto: element-by-element)
-OPERATORS = {"auto": 0, "csr": 1, "ebe": 2, "ebe_mfma": 3}
+OPERATORS = {"auto": 0, "csr": 1, "ebe": 2, "ebe_mfma": 3, "ebe_rows": 4}
 
 
 def element_matrices(order, tet_type, h):
@@ -111,7 +111,7 @@ class TetSimulation:
         _lib.check(_lib.load().ems_tet_add_receiver(self._h, _lib.dptr(p)))
 
     def set_operator(self, op):
-        """'auto', 'csr', 'ebe' or 'ebe_mfma' (at rest)."""
+        """'auto', 'csr', 'ebe', 'ebe_mfma' or 'ebe_rows' (at rest)."""
         _lib.check(_lib.load().ems_tet_set_operator(self._h, OPERATORS[op] if isinstance(op, str) else int(op)))
 
     def set_time_step(self, dt):
@@ -170,6 +170,11 @@ class TetSimulation:
         w2 = ctypes.c_double()
         _lib.check(_lib.load().ems_tet_max_frequency(self._h, int(iters), ctypes.byref(w2)))
         return float(np.sqrt(w2.value))
+
+    def set_solver(self, solver):
+        """'auto' (Chebyshev-accelerated Jacobi, CG fallback) or 'cg'."""
+        _lib.check(_lib.load().ems_tet_set_solver(self._h, {"auto": 0, "cg": 1}[solver] if isinstance(solver, str)
+                                                  else int(solver)))
 
     def solver_stats(self):
         a, b = ctypes.c_int(), ctypes.c_int()

@@ -405,9 +405,12 @@ enum { EMS_TET_SYMPLECTIC2 = 0, EMS_TET_SYMPLECTIC4 = 1, EMS_TET_ABM = 2 };
  * element-by-element products with the 6 shared element-type matrices (no
  * assembled matrix; per-tetrahedron products, then a fixed-order gather per
  * dof).  EBE_MFMA forms the mass operator's element products with matrix
- * cores (16x16x4 MFMA over 16 tetrahedra of one type).  AUTO:
+ * cores (16x16x4 MFMA over 16 tetrahedra of one type).  EBE_ROWS applies the
+ * mass operator in one pass per row (each row forms the local products of
+ * its incident tetrahedra; no per-tetrahedron buffer), with the same bits
+ * as EBE.  AUTO:
  * element-by-element (measured faster than CSR at both orders). */
-enum { EMS_TET_OP_AUTO = 0, EMS_TET_OP_CSR = 1, EMS_TET_OP_EBE = 2, EMS_TET_OP_EBE_MFMA = 3 };
+enum { EMS_TET_OP_AUTO = 0, EMS_TET_OP_CSR = 1, EMS_TET_OP_EBE = 2, EMS_TET_OP_EBE_MFMA = 3, EMS_TET_OP_EBE_ROWS = 4 };
 
 typedef struct ems_tet ems_tet;
 
@@ -504,7 +507,17 @@ ems_status ems_tet_get_time(const ems_tet *s, double *t);
  * y'' = -w^2 y. */
 ems_status ems_tet_max_frequency(ems_tet *s, int iters, double *omega2);
 
-/* Conjugate-gradient iterations of the last mass solve and the largest so far. */
+/* Mass solver: EMS_TET_SOLVER_AUTO (default) or EMS_TET_SOLVER_CG
+ * (Jacobi-preconditioned conjugate gradients).  AUTO solves FP64 order-1
+ * systems by Jacobi with Chebyshev acceleration (spectral bounds of each
+ * solve matrix by Lanczos, a fixed dot-free iteration count per solve from
+ * the Chebyshev bound, CG for a matrix whose final residual misses the
+ * tolerance) and everything else by CG.  Both stop at the same relative
+ * tolerance (1e-14 FP64, 1e-7 FP32). */
+enum { EMS_TET_SOLVER_AUTO = 0, EMS_TET_SOLVER_CG = 1 };
+ems_status ems_tet_set_solver(ems_tet *s, int solver);
+
+/* Iterations of the last mass solve and the largest so far. */
 ems_status ems_tet_solver_stats(const ems_tet *s, int *last_iters, int *max_iters);
 
 #ifdef __cplusplus

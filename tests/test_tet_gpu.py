@@ -90,7 +90,7 @@ def oracle_run(name):
     return _oracle_cache[name]
 
 
-@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma"])
+@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma", "ebe_rows"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_fp64_parity(name, op):
     """All operators (assembled CSR; element-by-element products with the
@@ -112,7 +112,7 @@ def test_fp64_parity(name, op):
     assert 0 < mx <= 512
 
 
-@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma"])
+@pytest.mark.parametrize("op", ["csr", "ebe", "ebe_mfma", "ebe_rows"])
 @pytest.mark.parametrize("name", ["s2", "abm-lossy", "s2-si", "p2-s4-lossy"])
 def test_fp32_parity(name, op):
     case = CASES[name]
@@ -124,6 +124,36 @@ def test_fp32_parity(name, op):
     assert rel_l2(tr, tr_o) < 1e-5
     em, pm = tf_misfits(tr, tr_o, case["dt"] * case["tf"])
     assert em < 1e-4 and pm < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("name", ["s4-lossy", "abm-lossy", "p2-s4-lossy", "p2-abm"])
+def test_row_operator_equals_two_phase_bitwise(name, precision):
+    """The per-row mass operator forms the same local products in the same
+    order as the two-phase element-by-element one: identical bits."""
+    case = CASES[name]
+    a = make("gpu", case, precision=precision, op="ebe")
+    b = make("gpu", case, precision=precision, op="ebe_rows")
+    for s in (a, b):
+        s.set_solver("cg")
+        s.run(case["steps"])
+    np.testing.assert_array_equal(a.traces(), b.traces())
+    np.testing.assert_array_equal(a.get_field(0), b.get_field(0))
+
+
+@pytest.mark.parametrize("name", ["s2", "s4-lossy", "abm-lossy"])
+def test_chebyshev_and_cg_solvers_agree(name):
+    """FP64 order 1: the Chebyshev-accelerated and the CG mass solves stop at
+    the same tolerance; the runs agree to it."""
+    case = CASES[name]
+    a = make("gpu", case, op="ebe")
+    b = make("gpu", case, op="ebe")
+    a.set_solver("cg")
+    b.set_solver("auto")
+    a.run(case["steps"])
+    b.run(case["steps"])
+    assert rel_l2(b.traces(), a.traces()) < 1e-11
+    assert rel_l2(b.get_field(0), a.get_field(0)) < 1e-11
 
 
 @pytest.mark.parametrize("name,op", [("s4-lossy", "csr"), ("abm-lossy", "csr"), ("p2-s4-lossy", "ebe")])
