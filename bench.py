@@ -207,11 +207,18 @@ def gpu_batch_forward(steps, per_model=16, sizes=(96, 128, 192, 256)):
     rows, planes = b.autotune(3)
     b.run(1)
     bat_ms = b.run_timed(steps)
+    # the same steps from a HIP graph of 10 steps, captured and instantiated
+    # inside the timed call, then replayed
+    b.set_graph(10)
+    graph_ms = b.run_timed(steps)
     return dict(shots=len(mem), models=list(sizes), steps=steps, cells_per_step=cells,
                 sequential_s=round(seq, 4), batched_s=round(bat_ms / 1e3, 4),
+                batched_graph_s=round(graph_ms / 1e3, 4),
                 sequential_gcells=round(cells * steps / seq / 1e9, 2),
                 batched_gcells=round(cells * steps / (bat_ms / 1e3) / 1e9, 2),
-                ratio=round(seq / (bat_ms / 1e3), 3), tiling=dict(rows=rows, planes=planes))
+                batched_graph_gcells=round(cells * steps / (graph_ms / 1e3) / 1e9, 2),
+                ratio=round(seq / (bat_ms / 1e3), 3), ratio_graph=round(seq / (graph_ms / 1e3), 3),
+                tiling=dict(rows=rows, planes=planes))
 
 
 def gpu_batch_gradient(n=192, shots=16, steps=100):
@@ -223,7 +230,8 @@ def gpu_batch_gradient(n=192, shots=16, steps=100):
     from emsolvers_amd.fdtd import Batch
     model = wl.blob_model(n, 7)
     obs = np.zeros((64, steps + 1, 3))
-    srcs = [(n // 2 - 45 + 30 * (i % 4), n // 2 - 45 + 30 * (i // 4), 20) for i in range(shots)]
+    # 4 x 4 source grid (repeated with a 1-cell shift per further 16 shots)
+    srcs = [(n // 2 - 45 + 30 * (i % 4) + i // 16, n // 2 - 45 + 30 * ((i // 4) % 4), 20) for i in range(shots)]
     sims = [_shot(n, i, model, receivers=(8, 8), z_rec=n - 20, src=srcs[i], f0=10e9) for i in range(shots)]
     for s in sims:
         s.gradient(steps, obs)
@@ -424,23 +432,59 @@ def cpu_reference_sample(n=256, steps=None, budget_s=12.0):
                        % (n, steps, s))
 
 
-def cpu_port_sample(n=256, steps=4, threads=None):
-    """FP64 CPU oracle on the same model (CPML + ellipsoids), OpenMP threads."""
+def host_info():
+    """Logical CPUs usable by this process (affinity), physical cores (lscpu)
+    and the CPU model."""
+    info = {"nproc": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        cpus = [l.split(",") for l in out.splitlines() if l and not l.startswith("#")]
+        allowed = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else None
+        cores = {(c[2], c[1]) for c in cpus if allowed is None or int(c[0]) in allowed}
+        info["physical_cores"] = len(cores)
+        model = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for l in model.splitlines():
+            if l.startswith("Model name:"):
+                info["model"] = l.split(":", 1)[1].strip()
+    except Exception as e:  # reported, not fatal
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_port_sample(n=256, budget_s=10.0):
+    """FP64 CPU oracle on the same model (CPML + ellipsoids), timed for about
+    budget_s each on all physical cores of this process's CPU share (capped by
+    OMP_NUM_THREADS when set) and on 1 thread."""
+    from oracle import fdtd as of
     from oracle.fdtd import OracleFDTD, WAVELET_RICKER
     from emsolvers_amd import workloads as wl
-    threads = threads or min(os.cpu_count() or 1, 16)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
+    info = host_info()
+    allc = info.get("physical_cores") or info["nproc"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        allc = min(allc, int(os.environ["OMP_NUM_THREADS"]))
     h = 1e-3
-    o = OracleFDTD((n, n, n), h, wl.cfl_dt(h), wl.EPS0, wl.MU0, wavelet=WAVELET_RICKER, f0=10e9,
-                   npml=10)
+    o = OracleFDTD((n, n, n), h, wl.cfl_dt(h), wl.EPS0, wl.MU0, wavelet=WAVELET_RICKER, f0=10e9, npml=10)
     o.set_material(wl.ellipsoid_model(n, 11), None)
     o.setSource_z((n // 2, n // 2, n // 2))
-    o.run(1)
-    t0 = time.perf_counter()
-    o.run(steps)
-    s = time.perf_counter() - t0
-    return dict(value=n ** 3 * steps / s / 1e6, unit="Mcell-updates/s", cores=threads, kind="port",
-                sample="%d^3 CPML+ellipsoids FP64 oracle, %d steps, %.1f s" % (n, steps, s))
+    legs = []
+    for threads in (allc, 1):
+        of.lib().ems_oracle_set_threads(int(threads))
+        o.run(1)  # warm-up (first touch, thread start)
+        t0 = time.perf_counter()
+        o.run(2)  # calibration
+        per = (time.perf_counter() - t0) / 2
+        steps = max(1, int(budget_s / max(per, 1e-6)))
+        t0 = time.perf_counter()
+        o.run(steps)
+        s = time.perf_counter() - t0
+        legs.append(dict(value=round(n ** 3 * steps / s / 1e6, 2), cores=int(threads), steps=steps,
+                         seconds=round(s, 2)))
+    best = legs[0]
+    return dict(value=best["value"], unit="Mcell-updates/s", cores=best["cores"], kind="port",
+                sample="%d^3 CPML+ellipsoids FP64 oracle, %d steps, %.1f s on %d threads; 1 thread: %.1f Mcell/s"
+                       " (%d steps, %.1f s)" % (n, best["steps"], best["seconds"], best["cores"], legs[1]["value"],
+                                                legs[1]["steps"], legs[1]["seconds"]),
+                one_thread=legs[1], host=info)
 
 
 def main():
@@ -455,6 +499,8 @@ def main():
     ap.add_argument("--no-fem", action="store_true", help="skip the FEM secondary figures")
     ap.add_argument("--tet-steps", type=int, default=200, help="steps of the tetrahedral FEM figures")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched forward / gradient figures")
+    ap.add_argument("--gradient64", action="store_true",
+                    help="also time the 64-shot batched gradient against 64 sequential gradients")
     ap.add_argument("--fem-profile", action="store_true",
                     help="run only short FEM legs (tet configs 2 and 3, fem_v2 and fem at 128^3), for rocprofv3")
     ap.add_argument("--fp32-errors", action="store_true",
@@ -546,6 +592,8 @@ def main():
     if not args.no_batch:
         out["batched_forward"] = gpu_batch_forward(20)
         out["batched_gradient"] = gpu_batch_gradient()
+        if args.gradient64:
+            out["batched_gradient_64"] = gpu_batch_gradient(shots=64, steps=50)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_reference_sample(args.n)

@@ -56,6 +56,8 @@ def lib():
         L.ems_oracle_fdtd_run.argtypes = [P, ctypes.c_int]
         L.ems_oracle_fdtd_nslot.argtypes = [P]
         L.ems_oracle_fdtd_nobs.argtypes = [P]
+        L.ems_oracle_set_threads.argtypes = [ctypes.c_int]
+        L.ems_oracle_get_threads.restype = ctypes.c_int
         L.ems_oracle_fdtd_time.argtypes = [P]
         L.ems_oracle_fdtd_time.restype = ctypes.c_double
         L.ems_oracle_fdtd_field_size.argtypes = [P, ctypes.c_int]

@@ -39,6 +39,9 @@
  * closed forms (cavity resonances, energy conservation, Fresnel reflection,
  * lossy decay) in tests/test_oracle_fdtd.py.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -304,6 +307,26 @@ int ems_oracle_fdtd_set_material(ems_oracle_fdtd *s, const double *eps_r, const 
 }
 
 /* setSource_{x,y,z}: src/fdtd/fdtd_simulation.cpp:90-106 */
+/* OpenMP threads of the oracle's parallel loops (timing legs only; the
+ * results do not depend on it: every loop writes disjoint entries) */
+void ems_oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+int ems_oracle_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 int ems_oracle_fdtd_add_source(ems_oracle_fdtd *s, int comp, int i, int j, int k)
 {
     const size_t nx = s->c.nx, ny = s->c.ny;
