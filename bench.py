@@ -221,18 +221,23 @@ def gpu_batch_forward(steps, per_model=16, sizes=(96, 128, 192, 256)):
                 tiling=dict(rows=rows, planes=planes))
 
 
-def gpu_batch_gradient(n=192, shots=16, steps=100):
+def gpu_batch_gradient(n=192, shots=16, steps=500):
     """Misfit gradient (eps_r and sigma) of `shots` shots on a 4 x 4 source grid
     at z = 20 with 64 receivers on the plane z = n - 20, background eps_r 2 with
     three Gaussian blobs: sequential single-shot gradients against one batched
-    gradient.  Wall-clock, second call of each (buffers allocated)."""
+    gradient, and the batched forward run of the same shots.  Wall-clock,
+    second call of each (buffers allocated); default checkpoint interval."""
     from emsolvers_amd import workloads as wl
     from emsolvers_amd.fdtd import Batch
     model = wl.blob_model(n, 7)
     obs = np.zeros((64, steps + 1, 3))
-    # 4 x 4 source grid (repeated with a 1-cell shift per further 16 shots)
-    srcs = [(n // 2 - 45 + 30 * (i % 4) + i // 16, n // 2 - 45 + 30 * ((i // 4) % 4), 20) for i in range(shots)]
-    sims = [_shot(n, i, model, receivers=(8, 8), z_rec=n - 20, src=srcs[i], f0=10e9) for i in range(shots)]
+    # 4 x 4 source grid, spacing 30 cells at n = 192 (scaled with n), at z = 20
+    # (scaled, at least 12), repeated with a 1-cell shift per further 16 shots
+    sp = max(1, round(30 * n / 192))
+    zs = max(12, round(20 * n / 192))
+    srcs = [(n // 2 - (3 * sp) // 2 + sp * (i % 4) + i // 16, n // 2 - (3 * sp) // 2 + sp * ((i // 4) % 4), zs)
+            for i in range(shots)]
+    sims = [_shot(n, i, model, receivers=(8, 8), z_rec=n - zs, src=srcs[i], f0=10e9) for i in range(shots)]
     for s in sims:
         s.gradient(steps, obs)
     t0 = time.perf_counter()
@@ -240,15 +245,17 @@ def gpu_batch_gradient(n=192, shots=16, steps=100):
         s.gradient(steps, obs)
     seq = time.perf_counter() - t0
     del sims
-    mem = [_shot(n, i, model, receivers=(8, 8), z_rec=n - 20, src=srcs[i], f0=10e9) for i in range(shots)]
+    mem = [_shot(n, i, model, receivers=(8, 8), z_rec=n - zs, src=srcs[i], f0=10e9) for i in range(shots)]
     b = Batch(mem)
     b.gradient(steps, [obs] * shots)
     t0 = time.perf_counter()
     b.gradient(steps, [obs] * shots)
     bat = time.perf_counter() - t0
+    fwd_ms = b.run_timed(steps)
     return dict(grid=[n] * 3, shots=shots, steps=steps, sequential_s=round(seq, 3), batched_s=round(bat, 3),
                 ratio=round(seq / bat, 3), ms_per_shot_step_batched=round(bat * 1e3 / shots / steps, 4),
-                dtype="fp32 fields, fp64 gradient")
+                forward_ms_per_shot_step=round(fwd_ms / shots / steps, 4),
+                gradient_to_forward=round(bat * 1e3 / fwd_ms, 3), dtype="fp32 fields, fp64 gradient")
 
 
 def _trace_errors(tr, ref, dt):
@@ -452,8 +459,8 @@ def host_info():
 
 
 def cpu_port_sample(n=256, budget_s=10.0):
-    """FP64 CPU oracle on the same model (CPML + ellipsoids), timed for about
-    budget_s each on all physical cores of this process's CPU share (capped by
+    """FP64 CPU oracle on the same model (CPML + ellipsoids), timed for at
+    least budget_s each on all physical cores of this process's CPU share (capped by
     OMP_NUM_THREADS when set) and on 1 thread."""
     from oracle import fdtd as of
     from oracle.fdtd import OracleFDTD, WAVELET_RICKER
@@ -473,10 +480,16 @@ def cpu_port_sample(n=256, budget_s=10.0):
         t0 = time.perf_counter()
         o.run(2)  # calibration
         per = (time.perf_counter() - t0) / 2
-        steps = max(1, int(budget_s / max(per, 1e-6)))
+        # chunks of about a quarter of the budget until at least budget_s has passed
+        chunk = max(1, int(0.25 * budget_s / max(per, 1e-6)))
+        steps = 0
         t0 = time.perf_counter()
-        o.run(steps)
-        s = time.perf_counter() - t0
+        while True:
+            o.run(chunk)
+            steps += chunk
+            s = time.perf_counter() - t0
+            if s >= budget_s:
+                break
         legs.append(dict(value=round(n ** 3 * steps / s / 1e6, 2), cores=int(threads), steps=steps,
                          seconds=round(s, 2)))
     best = legs[0]
@@ -534,7 +547,14 @@ def main():
         y.copy_(x)
         torch.cuda.synchronize()
         del x, y
+    t_start = time.perf_counter()
+
+    def note(leg):
+        # one progress line per finished leg, on stderr (stdout carries the JSON line)
+        print("bench.py: %s done at %.1f s" % (leg, time.perf_counter() - t_start), file=sys.stderr, flush=True)
+
     r = gpu_fdtd(args.n, args.steps, args.warmup, args.precision, args.kernel)
+    note("fdtd")
     cells = r["cells"]
     ms_per_step = r["total_ms"] / args.steps
     value = cells * args.steps / (r["total_ms"] / 1e3) / 1e6
@@ -580,6 +600,7 @@ def main():
                       gpu_fem(3, "fem_v2", 96, 20), gpu_fem(1, "fem_v2", 128, 50),
                       gpu_fem(1, "symplectic_2", 128, 50, "f32"), gpu_fem1("fem", 128, 50),
                       gpu_fem1("fem_abm", 128, 30)]
+        note("hex fem")
     if not args.no_fem:
         out["tet"] = [gpu_tet(40, "symplectic_2", args.tet_steps), gpu_tet(40, "abm", args.tet_steps),
                       gpu_tet(40, "symplectic_2", args.tet_steps, "f32"),
@@ -587,18 +608,26 @@ def main():
                       gpu_tet(64, "symplectic_4", 5, "f32", order=2, sigma_fraction=0.2, power_iters=40,
                               op="ebe_mfma"),
                       gpu_tet(64, "symplectic_4", 3, "f64", order=2, sigma_fraction=0.2, power_iters=40)]
+        note("tet")
     if not args.no_fem:
         out["tet_gradient"] = [gpu_tet_gradient(40, 100, "f64"), gpu_tet_gradient(40, 100, "f32")]
+        note("tet gradient")
     if not args.no_batch:
         out["batched_forward"] = gpu_batch_forward(20)
+        note("batched forward")
         out["batched_gradient"] = gpu_batch_gradient()
+        note("batched gradient")
+        out["batched_gradient_small"] = gpu_batch_gradient(n=64, shots=64, steps=200)
+        note("small-grid batched gradient")
         if args.gradient64:
-            out["batched_gradient_64"] = gpu_batch_gradient(shots=64, steps=50)
+            out["batched_gradient_64"] = gpu_batch_gradient(shots=64, steps=100)
+            note("64-shot gradient")
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_reference_sample(args.n)
         except Exception as e:  # reported, not fatal
             out["cpu_baseline"] = {"error": str(e)}
+        note("cpu reference")
         try:
             out["cpu_port"] = cpu_port_sample(args.n)
         except Exception as e:

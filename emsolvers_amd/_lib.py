@@ -75,6 +75,8 @@ SIGNATURES = {
     "ems_fdtd_check_params": (_I, [_DP, _IP, _D, _D, _D, _D]),
     "ems_fdtd_gradient": (_I, [_P, _I, _DP, _SZ, _DP, _DP, _DP]),
     "ems_fdtd_set_checkpoint_interval": (_I, [_P, _I]),
+    "ems_fdtd_set_gradient_memory": (_I, [_P, _SZ]),
+    "ems_fdtd_gradient_plan": (_I, [_P, _I, _IP, _IP, ctypes.POINTER(_SZ)]),
     "ems_batch_create": (_I, [_PP]),
     "ems_batch_destroy": (None, [_P]),
     "ems_batch_add": (_I, [_P, _P]),

@@ -81,6 +81,7 @@ struct ems_fdtd {
     bool dirty_pml = true;
     // misfit gradient: checkpoint interval (0 = ceil(sqrt(steps))) and buffers
     int ckpt_interval = 0;
+    size_t grad_mem = 0;  // cap on the gradient's snapshot and checkpoint bytes (0: none)
     struct Adj *adj = nullptr;
     long mat_version = 0;  // set by every set_material call (unique over all handles)
     ~ems_fdtd();
@@ -790,12 +791,13 @@ template <typename T> ems_status set_field_t(ems_fdtd *s, int w, const double *i
 // (no reference counterpart).
 // =========================================================================
 struct Adj {
-    std::vector<DevBuf> ckpt;   // per checkpoint: 6 fields + 12 psi, packed
     DevBuf fin;                 // the final forward state
-    DevBuf seg;                 // E^n of one segment: [K+1][3][nalloc]
-    DevBuf e[2][3], h[3];       // adjoint fields (e ping-pong)
+    void *segp = nullptr;       // this shot's E snapshots in the gradient store: [slot][3][nalloc]
+    DevBuf e[3], h[3];          // adjoint fields (e updated in place)
     DevBuf pe[2][6], ph[2][6];  // psi adjoints (ping-pong), compact slabs
     DevBuf fac, grad;           // 4 x nmat derivative factors, 2 x nmat gradient (FP64)
+    DevBuf sums;                // gradient sums P, Q (FP64, unified grid)
+    DevBuf tab;                 // shot tables of the adjoint sweeps, one per psi parity
     long fac_version = -1;      // material version the factors were built for
     DevBuf w;                   // residual weights [rows][npts]
 };
@@ -811,9 +813,10 @@ size_t state_bytes(const ems_fdtd *s)
     return b;
 }
 
-ems_status copy_state(ems_fdtd *s, DevBuf &dst, bool save)
+// The state (6 fields, then the 12 psi slabs that exist) to or from dst.
+ems_status copy_state(ems_fdtd *s, void *dst, bool save)
 {
-    char *q = static_cast<char *>(dst.p);
+    char *q = static_cast<char *>(dst);
     const size_t fb = s->nalloc * s->esz;
     for (int w = 0; w < 6; ++w, q += fb)
         EMS_HIP(hipMemcpyAsync(save ? q : s->f[w].p, save ? s->f[w].p : q, fb, hipMemcpyDeviceToDevice, s->stream));
@@ -829,29 +832,221 @@ ems_status copy_state(ems_fdtd *s, DevBuf &dst, bool save)
 ems_status snap_e(ems_fdtd *s, int slot)
 {
     const size_t fb = s->nalloc * s->esz;
-    char *q = static_cast<char *>(s->adj->seg.p) + (size_t)slot * 3 * fb;
+    char *q = static_cast<char *>(s->adj->segp) + (size_t)slot * 3 * fb;
     for (int c = 0; c < 3; ++c)
         EMS_HIP(hipMemcpyAsync(q + c * fb, s->f[c].p, fb, hipMemcpyDeviceToDevice, s->stream));
     return EMS_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Checkpoint schedule of a gradient over N steps.  The adjoint of step n
+// needs E^n and E^{n+1}; they come from a snapshot store that holds the E of
+// Ks+1 consecutive steps per shot, filled by running a segment of Ks steps
+// forward from a checkpoint (full state).  Two levels:
+//   level 1: checkpoints every K1 steps, written in the first forward pass;
+//   level 2: checkpoints every Ks steps inside one level-1 segment, written
+//            when that segment is (re)run.
+// Backwards over level-1 segments: rerun the segment from its checkpoint
+// writing level-2 checkpoints (the last segment is not rerun: the first pass
+// wrote them), then backwards over its Ks-step sub-segments: rerun each from
+// its checkpoint into the snapshot store (the last one is not rerun) and run
+// its adjoint steps.  K1 = N is the one-level scheme.  The last segment of a
+// level needs no checkpoint.
+// ---------------------------------------------------------------------------
+struct GradPlan {
+    int K1 = 1, Ks = 1;
+};
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Forward steps of a plan (first pass included) and its store bytes, for
+// snapshot bytes `snap` and checkpoint bytes `ck` per step / state of all shots.
+double plan_work(int N, const GradPlan &p)
+{
+    const int n1 = ceil_div(N, p.K1), last1 = N - (n1 - 1) * p.K1;
+    auto l2 = [&](int len) { return (double)(ceil_div(len, p.Ks) - 1) * p.Ks; };  // steps rerun inside a segment
+    return (double)N + (double)(N - last1) + (n1 - 1) * l2(p.K1) + l2(last1);
+}
+double plan_bytes(int N, const GradPlan &p, double snap, double ck)
+{
+    const int n1 = ceil_div(N, p.K1);
+    const int n2 = ceil_div(std::min(p.K1, N), p.Ks);
+    return (p.Ks + 1.0) * snap + ((n1 - 1) + (n2 - 1)) * ck;
+}
+
+// The plan with the fewest forward steps (then the fewest bytes) within
+// `budget`; a set interval gives the one-level plan with Ks = interval.
+// Nothing fits: the plan with the fewest bytes (its allocation then reports
+// the shortage).
+GradPlan plan_gradient(int N, int interval, double snap, double ck, double budget)
+{
+    GradPlan best, least;
+    if (interval > 0) {
+        best.K1 = N;
+        best.Ks = std::min(interval, N);
+        return best;
+    }
+    double bw = 0.0, bb = 0.0, lb = 0.0;
+    bool found = false, any = false;
+    auto consider = [&](int K1, int Ks) {
+        const GradPlan p{K1, Ks};
+        const double b = plan_bytes(N, p, snap, ck);
+        if (!any || b < lb) {
+            least = p;
+            lb = b;
+            any = true;
+        }
+        if (b > budget) return;
+        const double w = plan_work(N, p);
+        if (!found || w < bw || (w == bw && b < bb)) {
+            best = p;
+            bw = w;
+            bb = b;
+            found = true;
+        }
+    };
+    for (int Ks = 1; Ks <= N; ++Ks) {
+        consider(N, Ks);
+        for (long K1 = 2L * Ks; K1 < N; K1 += Ks) consider((int)K1, Ks);
+    }
+    return found ? best : least;
+}
+
+// One store per GPU for the snapshots and checkpoints of the gradient that
+// runs on it: grown on demand and kept for later calls (allocating and
+// freeing stores of tens of GB per call costs more than the calls' host
+// work); gradients on one GPU take turns on it.
+struct GradStore {
+    std::mutex mu;
+    DevBuf buf;  // one call's snapshots ([shot][slot][3][nalloc]), then its checkpoints
+};
+
+constexpr int kStoreDevices = 64;
+GradStore g_grad_store[kStoreDevices];
+
+ems_status grad_store(GradStore **out)
+{
+    int dev = -1;
+    EMS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStoreDevices) return invalid("gradient: device index out of range");
+    *out = &g_grad_store[dev];
+    return EMS_OK;
+}
+
+// Plan and store of a gradient over N steps of the shots `mem` (one grid),
+// with the store's lock held by the caller: the budget is 3/4 of the device
+// memory that is free or already held by the store, or the first shot's cap
+// if that is lower.  Sets every shot's
+// snapshot base; *ck_base = the checkpoint area ([idx][shot] states, level-1
+// indices first, then n1-1 .. level-2).
+ems_status grad_plan_for(const GradStore &gs, const ems_fdtd *s, int shots, int N, GradPlan &plan, double &bytes)
+{
+    const double snap = shots * 3.0 * (double)s->nalloc * (double)s->esz;
+    const double ck = shots * (double)state_bytes(s);
+    size_t fr = 0, tot = 0;
+    EMS_HIP(hipMemGetInfo(&fr, &tot));
+    double budget = 0.75 * ((double)fr + (double)gs.buf.bytes);
+    if (s->grad_mem > 0) budget = std::min(budget, (double)s->grad_mem);
+    plan = plan_gradient(N, s->ckpt_interval, snap, ck, budget);
+    bytes = plan_bytes(N, plan, snap, ck);
+    if (s->grad_mem > 0 && s->ckpt_interval <= 0 && bytes > (double)s->grad_mem)
+        return invalid("gradient: no checkpoint schedule of " + std::to_string(N) + " steps fits the memory cap of " +
+                       std::to_string(s->grad_mem) + " bytes");
+    return EMS_OK;
+}
+
+ems_status grad_prepare(GradStore &gs, const std::vector<ems_fdtd *> &mem, int N, GradPlan &plan, char **ck_base)
+{
+    const ems_fdtd *s = mem[0];
+    double bytes = 0.0;
+    EMS_TRY(grad_plan_for(gs, s, (int)mem.size(), N, plan, bytes));
+    const size_t per_shot = (size_t)(plan.Ks + 1) * 3 * s->nalloc * s->esz;
+    const int n1 = ceil_div(N, plan.K1), n2 = ceil_div(std::min(plan.K1, N), plan.Ks);
+    const size_t nck = (size_t)std::max(0, n1 - 1) + (size_t)std::max(0, n2 - 1);
+    const size_t snap_bytes = (per_shot * mem.size() + 255) / 256 * 256;
+    const size_t want = snap_bytes + nck * mem.size() * state_bytes(s);
+    if (gs.buf.bytes < want) {
+        // grow (the old store is released first, so the two are never held together)
+        gs.buf.release();
+        if (gs.buf.reserve(want) != EMS_OK) {
+            set_error("gradient: the snapshot and checkpoint store (" + std::to_string(want >> 20) +
+                      " MiB for segments of " + std::to_string(plan.Ks) + " steps) does not fit in device memory");
+            return EMS_ERR_NO_MEMORY;
+        }
+    }
+    for (size_t q = 0; q < mem.size(); ++q) mem[q]->adj->segp = static_cast<char *>(gs.buf.p) + q * per_shot;
+    *ck_base = static_cast<char *>(gs.buf.p) + snap_bytes;
+    return EMS_OK;
+}
+
+// Drives the schedule.  Ops:
+//   adv(n, record, slot): forward step n -> n+1 of every shot (receivers when
+//       record), then E^{n+1} into snapshot slot `slot` when slot >= 0
+//   snap(slot): the current E of every shot into snapshot slot `slot`
+//   save(idx) / load(idx): the state of every shot to / from checkpoint idx
+//   adjoint(n, slot): adjoint step n, E^n in slot `slot`, E^{n+1} in slot + 1
+template <typename Ops> struct GradSchedule {
+    Ops &op;
+    GradPlan p;
+    int N;
+    int l2_base() const { return std::max(0, ceil_div(N, p.K1) - 1); }
+    // a level-1 segment [a, b) from its start state: level-2 checkpoints and
+    // the last sub-segment's snapshots
+    ems_status level2_forward(int a, int b, bool record)
+    {
+        const int n2 = ceil_div(b - a, p.Ks);
+        for (int q = 0; q < n2; ++q) {
+            const int c = a + q * p.Ks, d = std::min(b, c + p.Ks);
+            const bool last = q == n2 - 1;
+            if (!last) EMS_TRY(op.save(l2_base() + q));
+            if (last) EMS_TRY(op.snap(0));
+            for (int n = c; n < d; ++n) EMS_TRY(op.adv(n, record, last ? n + 1 - c : -1));
+        }
+        return EMS_OK;
+    }
+    ems_status first_pass()
+    {
+        const int n1 = ceil_div(N, p.K1);
+        for (int q = 0; q < n1; ++q) {
+            const int a = q * p.K1, b = std::min(N, a + p.K1);
+            if (q < n1 - 1) {
+                EMS_TRY(op.save(q));
+                for (int n = a; n < b; ++n) EMS_TRY(op.adv(n, true, -1));
+            } else {
+                EMS_TRY(level2_forward(a, b, true));
+            }
+        }
+        return EMS_OK;
+    }
+    ems_status backward()
+    {
+        const int n1 = ceil_div(N, p.K1);
+        for (int q1 = n1 - 1; q1 >= 0; --q1) {
+            const int a = q1 * p.K1, b = std::min(N, a + p.K1);
+            if (q1 < n1 - 1) {
+                EMS_TRY(op.load(q1));
+                EMS_TRY(level2_forward(a, b, false));
+            }
+            const int n2 = ceil_div(b - a, p.Ks);
+            for (int q2 = n2 - 1; q2 >= 0; --q2) {
+                const int c = a + q2 * p.Ks, d = std::min(b, c + p.Ks);
+                if (q2 < n2 - 1) {
+                    EMS_TRY(op.load(l2_base() + q2));
+                    EMS_TRY(op.snap(0));
+                    for (int n = c; n < d; ++n) EMS_TRY(op.adv(n, false, n + 1 - c));
+                }
+                for (int n = d - 1; n >= c; --n) EMS_TRY(op.adjoint(n, n - c));
+            }
+        }
+        return EMS_OK;
+    }
+};
 
 // Material versions, unique over all handles (keys of cached gradient factors).
 long next_version()
 {
     static std::atomic<long> versions{0};
     return ++versions;
-}
-
-// Workgroups of the adjoint passes (see adj_coords).
-unsigned adj_grid(const GridView &g)
-{
-    return (unsigned)((long)((g.nx + 1 + 63) / 64) * ((g.ny + 1 + 3) / 4) * (g.nz + 1));
-}
-
-template <typename T> Adj3<T> seg_e(ems_fdtd *s, int slot)
-{
-    T *q = s->adj->seg.as<T>() + (size_t)slot * 3 * s->nalloc;
-    return Adj3<T>{{q, q + s->nalloc, q + 2 * s->nalloc}};
 }
 
 template <typename T> AdjPml<T> adj_pml(ems_fdtd *s, int cur)
@@ -875,33 +1070,60 @@ template <typename T> AdjPml<T> adj_pml(ems_fdtd *s, int cur)
     return a;
 }
 
-template <typename T, int MAT, bool PML>
-void launch_adjoint(ems_fdtd *s, int cur, Adj3<T> en, Adj3<T> e1, GradView gv)
+// Adjoint state of one shot for the sweeps, psi adjoints of parity `par` old.
+template <typename T> AdjShot<T> adj_shot(ems_fdtd *s, int par)
 {
-    const GridView &g = s->g;
     Adj *A = s->adj;
-    const dim3 blk(64, 4, 1), grd(adj_grid(g), 1, 1);
-    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    const T nch = -(T)(s->d.dt / s->d.mu / s->d.h);
-    const AdjPml<T> a = adj_pml<T>(s, cur);
-    Adj3<T> ein{{A->e[0][0].as<T>(), A->e[0][1].as<T>(), A->e[0][2].as<T>()}};
-    Adj3<T> eout{{A->e[1][0].as<T>(), A->e[1][1].as<T>(), A->e[1][2].as<T>()}};
-    Adj3<T> hb{{A->h[0].as<T>(), A->h[1].as<T>(), A->h[2].as<T>()}};
-    k_adj_a<T, MAT, PML, true><<<grd, blk, 0, s->stream>>>(g, ce, ein, eout, hb, s->ca.as<T>(), s->cb.as<T>(), a, en,
-                                                          e1, gv);
-    k_adj_b<T, PML><<<grd, blk, 0, s->stream>>>(g, nch, eout, hb, a);
+    AdjShot<T> sh{};
+    sh.e = Adj3<T>{{A->e[0].as<T>(), A->e[1].as<T>(), A->e[2].as<T>()}};
+    sh.hb = Adj3<T>{{A->h[0].as<T>(), A->h[1].as<T>(), A->h[2].as<T>()}};
+    sh.seg = static_cast<T *>(A->segp);
+    sh.a = adj_pml<T>(s, par);
+    return sh;
 }
 
-template <typename T> void adjoint_step(ems_fdtd *s, int cur, Adj3<T> en, Adj3<T> e1, GradView gv)
+// Launch geometry of the adjoint sweeps: the split sweeps' tiling with 4-row
+// workgroups and ADJ_LZ planes per thread.
+struct AdjGeom {
+    int P, npw, nbx, nby, ntiles;
+    unsigned grd;
+};
+
+AdjGeom adj_geom(const GridView &g, int npml)
 {
-    const bool pml = s->npml > 0;
-    switch (s->mat_mode * 2 + (pml ? 1 : 0)) {
-    case 0: launch_adjoint<T, 0, false>(s, cur, en, e1, gv); break;
-    case 1: launch_adjoint<T, 0, true>(s, cur, en, e1, gv); break;
-    case 2: launch_adjoint<T, 1, false>(s, cur, en, e1, gv); break;
-    case 3: launch_adjoint<T, 1, true>(s, cur, en, e1, gv); break;
-    case 4: launch_adjoint<T, 2, false>(s, cur, en, e1, gv); break;
-    default: launch_adjoint<T, 2, true>(s, cur, en, e1, gv); break;
+    AdjGeom a{};
+    a.P = npml > 0 ? npml : 0;
+    a.npw = a.P > 0 ? (2 * a.P + 1 + 63) / 64 : 0;
+    a.nbx = split_row_waves(g, a.P);
+    a.nby = (g.ny + 1 + 3) / 4;
+    a.ntiles = a.nby * ((g.nz + 1 + ADJ_LZ - 1) / ADJ_LZ);
+    a.grd = 8u * (unsigned)a.nbx * (unsigned)((a.ntiles + 7) / 8);  // see split_tile
+    return a;
+}
+
+// Both adjoint sweeps of one step for the ns shots of the device table sh,
+// accumulating the gradient sums of E snapshot slots `slot` and slot + 1.
+template <typename T, int MAT, bool PML>
+void adj_sweeps_t(const GridView &g, const AdjGeom &ag, hipStream_t st, T ce, T nch, const AdjShot<T> *sh, int ns,
+                  const T *ca, const T *cb, double *gp, double *gq, long nalloc, int slot)
+{
+    const dim3 blk(64, 4, 1);
+    k_adj_h<T, MAT, PML><<<ag.grd, blk, 0, st>>>(g, ce, sh, ns, cb, ag.P, ag.npw, ag.nbx, ag.nby, ag.ntiles);
+    k_adj_e<T, MAT, PML, true><<<ag.grd, blk, 0, st>>>(g, nch, sh, ns, ca, gp, gq, nalloc, slot, ag.P, ag.npw, ag.nbx,
+                                                       ag.nby, ag.ntiles);
+}
+
+template <typename T>
+void adj_sweeps(int mat, bool pml, const GridView &g, const AdjGeom &ag, hipStream_t st, T ce, T nch,
+                const AdjShot<T> *sh, int ns, const T *ca, const T *cb, double *gp, double *gq, long nalloc, int slot)
+{
+    switch (mat * 2 + (pml ? 1 : 0)) {
+    case 0: adj_sweeps_t<T, 0, false>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
+    case 1: adj_sweeps_t<T, 0, true>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
+    case 2: adj_sweeps_t<T, 1, false>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
+    case 3: adj_sweeps_t<T, 1, true>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
+    case 4: adj_sweeps_t<T, 2, false>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
+    default: adj_sweeps_t<T, 2, true>(g, ag, st, ce, nch, sh, ns, ca, cb, gp, gq, nalloc, slot); break;
     }
 }
 
@@ -909,6 +1131,25 @@ ems_status zero(DevBuf &b, size_t bytes, hipStream_t st)
 {
     EMS_TRY(b.reserve(bytes));
     EMS_HIP(hipMemsetAsync(b.p, 0, bytes, st));
+    return EMS_OK;
+}
+
+// Zero adjoint fields and psi adjoints (both parities) of one shot.
+ems_status adj_zero(ems_fdtd *s, hipStream_t st)
+{
+    Adj *A = s->adj;
+    const size_t fb = s->nalloc * s->esz;
+    for (int c = 0; c < 3; ++c) {
+        EMS_TRY(zero(A->e[c], fb, st));
+        EMS_TRY(zero(A->h[c], fb, st));
+    }
+    if (s->npml > 0)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q)
+                for (int b = 0; b < 2; ++b) {
+                    EMS_TRY(zero(A->pe[b][ax * 2 + q], s->psi[ax * 4 + 2 + q].bytes, st));
+                    EMS_TRY(zero(A->ph[b][ax * 2 + q], s->psi[ax * 4 + q].bytes, st));
+                }
     return EMS_OK;
 }
 
@@ -935,6 +1176,19 @@ std::vector<double> material_factors(const ems_fdtd *s)
         f[3 * nm + m] = -s->d.dt / (2.0 * e * op);
     }
     return f;
+}
+
+template <typename A, typename B, typename C, typename D, typename E> struct GradOps {
+    A adv;
+    B snap;
+    C save;
+    D load;
+    E adjoint;
+};
+template <typename A, typename B, typename C, typename D, typename E>
+GradOps<A, B, C, D, E> grad_ops(A a, B b, C c, D d, E e)
+{
+    return GradOps<A, B, C, D, E>{a, b, c, d, e};
 }
 
 template <typename T>
@@ -1016,27 +1270,51 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
         EMS_HIP(hipGetLastError());
         return EMS_OK;
     };
-    // checkpoints
-    const int K = s->ckpt_interval > 0 ? std::min(s->ckpt_interval, N)
-                                       : std::max(1, (int)std::ceil(std::sqrt((double)N)));
-    const int nseg = (N + K - 1) / K;
+    // gradient buffers outside the store first: the store takes its share of
+    // the memory that is left
     const size_t sb = state_bytes(s);
-    if ((int)A->ckpt.size() < nseg) A->ckpt.resize(nseg);
-    for (int q = 0; q < nseg; ++q) EMS_TRY(A->ckpt[q].reserve(sb));
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
     EMS_TRY(A->fin.reserve(sb));
-    EMS_TRY(A->seg.reserve((size_t)(K + 1) * 3 * s->nalloc * s->esz));
-    // forward: checkpoints at segment starts; the last segment keeps its E
-    for (int q = 0; q < nseg; ++q) {
-        const int s0 = q * K, s1 = std::min(N, s0 + K);
-        EMS_TRY(copy_state(s, A->ckpt[q], true));
-        const bool last = q == nseg - 1;
-        if (last) EMS_TRY(snap_e(s, 0));
-        for (int n = s0; n < s1; ++n) {
-            EMS_TRY(fstep(n, true));
-            if (last) EMS_TRY(snap_e(s, n + 1 - s0));
-        }
-    }
-    EMS_TRY(copy_state(s, A->fin, true));
+    EMS_TRY(A->grad.reserve(sizeof(double) * 2 * nm));
+    EMS_TRY(A->sums.reserve(sizeof(double) * 2 * s->nalloc));
+    EMS_TRY(A->fac.reserve(sizeof(double) * 4 * nm));
+    EMS_TRY(A->w.reserve(sizeof(double) * (size_t)std::max(nrec, 1) * std::max(npts, 1)));
+    EMS_TRY(adj_zero(s, st));
+    GradStore *gs = nullptr;
+    EMS_TRY(grad_store(&gs));
+    std::lock_guard<std::mutex> store_lock(gs->mu);
+    GradPlan plan;
+    char *ckb = nullptr;
+    EMS_TRY(grad_prepare(*gs, std::vector<ems_fdtd *>{s}, N, plan, &ckb));
+    // adjoint launch data (set once the residual weights are known)
+    int serial = 0, cur = 0;
+    double *gp = nullptr, *gq = nullptr;
+    const AdjGeom ag = adj_geom(g, s->npml);
+    const T nch = -ch;
+    const Adj3<T> e_cur{{A->e[0].as<T>(), A->e[1].as<T>(), A->e[2].as<T>()}};
+    auto ops = grad_ops(
+        [&](int n, bool record, int slot) -> ems_status {
+            EMS_TRY(fstep(n, record));
+            if (slot >= 0) EMS_TRY(snap_e(s, slot));
+            return EMS_OK;
+        },
+        [&](int slot) { return snap_e(s, slot); },
+        [&](int idx) { return copy_state(s, ckb + (size_t)idx * sb, true); },
+        [&](int idx) { return copy_state(s, ckb + (size_t)idx * sb, false); },
+        [&](int n, int slot) -> ems_status {
+            if (npts && (n + 1) % tf == 0)
+                k_adj_inject<T><<<serial ? 1 : (npts + 255) / 256, serial ? 64 : 256, 0, st>>>(
+                    e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
+                    A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial);
+            adj_sweeps<T>(s->mat_mode, s->npml > 0, g, ag, st, ce, nch, A->tab.as<AdjShot<T>>() + cur, 1,
+                          s->ca.as<T>(), s->cb.as<T>(), gp, gq, (long)s->nalloc, slot);
+            EMS_HIP(hipGetLastError());
+            cur = 1 - cur;
+            return EMS_OK;
+        });
+    GradSchedule<decltype(ops)> sched{ops, plan, N};
+    EMS_TRY(sched.first_pass());
+    EMS_TRY(copy_state(s, A->fin.p, true));
     EMS_HIP(hipStreamSynchronize(st));
     // traces (the reference's averaging) and the misfit
     const int nobs = (int)s->obs.size();
@@ -1075,68 +1353,35 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
             }
     }
     // points that repeat are injected serially (the misfit's order)
-    int serial = 0;
     {
         std::vector<std::pair<int, long>> keys;
         for (int p = 0; p < npts; ++p) keys.emplace_back(pc[p], pu[p]);
         std::sort(keys.begin(), keys.end());
         serial = std::adjacent_find(keys.begin(), keys.end()) != keys.end() ? 1 : 0;
     }
-    EMS_TRY(A->w.reserve(sizeof(double) * w.size()));
     EMS_HIP(hipMemcpy(A->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
-    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-    if (A->fac_version != s->mat_version || !A->fac.p) {
+    if (A->fac_version != s->mat_version) {
         const std::vector<double> fac = material_factors(s);
         EMS_TRY(A->fac.reserve(sizeof(double) * fac.size()));
         EMS_HIP(hipMemcpy(A->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
         A->fac_version = s->mat_version;
     }
-    EMS_TRY(zero(A->grad, sizeof(double) * 2 * nm, st));
-    // adjoint state
-    const size_t fb = s->nalloc * s->esz;
-    for (int c = 0; c < 3; ++c) {
-        EMS_TRY(zero(A->e[0][c], fb, st));
-        EMS_TRY(zero(A->e[1][c], fb, st));
-        EMS_TRY(zero(A->h[c], fb, st));
-    }
-    if (s->npml > 0)
-        for (int ax = 0; ax < 3; ++ax)
-            for (int q = 0; q < 2; ++q)
-                for (int b = 0; b < 2; ++b) {
-                    EMS_TRY(zero(A->pe[b][ax * 2 + q], s->psi[ax * 4 + 2 + q].bytes, st));
-                    EMS_TRY(zero(A->ph[b][ax * 2 + q], s->psi[ax * 4 + q].bytes, st));
-                }
+    EMS_TRY(zero(A->sums, sizeof(double) * 2 * s->nalloc, st));
+    const AdjShot<T> tab[2] = {adj_shot<T>(s, 0), adj_shot<T>(s, 1)};
+    EMS_TRY(A->tab.reserve(sizeof(tab)));
+    EMS_HIP(hipMemcpy(A->tab.p, tab, sizeof(tab), hipMemcpyHostToDevice));
     GradView gv{A->fac.as<double>(), A->fac.as<double>() + nm, A->fac.as<double>() + 2 * nm,
                 A->fac.as<double>() + 3 * nm, A->grad.as<double>(), A->grad.as<double>() + nm};
-    int cur = 0;  // psi adjoint ping-pong
-    for (int q = nseg - 1; q >= 0; --q) {
-        const int s0 = q * K, s1 = std::min(N, s0 + K);
-        if (q != nseg - 1) {
-            EMS_TRY(copy_state(s, A->ckpt[q], false));
-            EMS_TRY(snap_e(s, 0));
-            for (int n = s0; n < s1; ++n) {
-                EMS_TRY(fstep(n, false));
-                EMS_TRY(snap_e(s, n + 1 - s0));
-            }
-        }
-        for (int n = s1 - 1; n >= s0; --n) {
-            Adj3<T> e_cur{{A->e[0][0].as<T>(), A->e[0][1].as<T>(), A->e[0][2].as<T>()}};
-            if (npts && (n + 1) % tf == 0)
-                k_adj_inject<T><<<serial ? 1 : (npts + 255) / 256, serial ? 64 : 256, 0, st>>>(
-                    e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
-                    A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial);
-            adjoint_step<T>(s, cur, seg_e<T>(s, n - s0), seg_e<T>(s, n + 1 - s0), gv);
-            EMS_HIP(hipGetLastError());
-            for (int c = 0; c < 3; ++c) swap_buf(A->e[0][c], A->e[1][c]);
-            cur = 1 - cur;
-        }
-    }
-    EMS_TRY(copy_state(s, A->fin, false));
+    gp = A->sums.as<double>();
+    gq = gp + s->nalloc;
+    EMS_TRY(sched.backward());
+    k_adj_grad<<<(unsigned)((nm + 255) / 256), 256, 0, st>>>(g, gp, gq, gv);
+    EMS_HIP(hipGetLastError());
+    EMS_TRY(copy_state(s, A->fin.p, false));
     EMS_HIP(hipStreamSynchronize(st));
-    std::vector<double> gr(2 * nm);
-    EMS_HIP(hipMemcpy(gr.data(), A->grad.p, sizeof(double) * gr.size(), hipMemcpyDeviceToHost));
-    if (g_eps) std::copy(gr.begin(), gr.begin() + nm, g_eps);
-    if (g_sig) std::copy(gr.begin() + nm, gr.end(), g_sig);
+    // straight into the caller's arrays (no staging copy)
+    if (g_eps) EMS_HIP(hipMemcpy(g_eps, A->grad.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
+    if (g_sig) EMS_HIP(hipMemcpy(g_sig, A->grad.as<double>() + nm, sizeof(double) * nm, hipMemcpyDeviceToHost));
     if (misfit) *misfit = J;
     s->traces.swap(tr);
     s->nslot = nslot;
@@ -1156,7 +1401,7 @@ struct ems_batch {
     int by = 8, lz = 2;      // workgroup rows, planes per thread
     int graph_steps = 0;     // steps per captured graph (0 = plain launches)
     DevBuf d_mem, d_post, d_step;
-    DevBuf d_post2, d_shots[2], fac, grad;  // gradient: recompute posts, shot tables, shared FP64 arrays
+    DevBuf d_post2, d_shots[2], fac, grad, sums;  // gradient: recompute posts, shot tables, shared FP64 arrays
     const ems_fdtd *fac_owner = nullptr;     // factors cached for this member's material version
     long fac_version = -1;
     ~ems_batch()
@@ -1545,39 +1790,73 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     EMS_HIP(hipMemcpy(b->d_post.p, post.data(), sizeof(BatchPost) * ns, hipMemcpyHostToDevice));
     EMS_HIP(hipMemcpy(b->d_post2.p, post2.data(), sizeof(BatchPost) * ns, hipMemcpyHostToDevice));
     const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
-    // checkpoints
-    const int K = s0->ckpt_interval > 0 ? std::min(s0->ckpt_interval, N)
-                                        : std::max(1, (int)std::ceil(std::sqrt((double)N)));
-    const int nseg = (N + K - 1) / K;
-    for (auto *s : b->mem) {
+    // gradient buffers outside the store first
+    const size_t sb = state_bytes(s0);
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    for (int q = 0; q < ns; ++q) {
+        ems_fdtd *s = b->mem[q];
         if (!s->adj) s->adj = new Adj();
-        Adj *A = s->adj;
-        const size_t sb = state_bytes(s);
-        if ((int)A->ckpt.size() < nseg) A->ckpt.resize(nseg);
-        for (int q = 0; q < nseg; ++q) EMS_TRY(A->ckpt[q].reserve(sb));
-        EMS_TRY(A->fin.reserve(sb));
-        EMS_TRY(A->seg.reserve((size_t)(K + 1) * 3 * s->nalloc * s->esz));
+        EMS_TRY(s->adj->fin.reserve(sb));
+        EMS_TRY(s->adj->w.reserve(sizeof(double) * (size_t)std::max(rp[q].nrec, 1) * std::max(rp[q].npts, 1)));
+        EMS_TRY(adj_zero(s, st));
     }
-    // forward
-    for (int q = 0; q < nseg; ++q) {
-        const int a0 = q * K, a1 = std::min(N, a0 + K);
-        const bool last = q == nseg - 1;
-        for (auto *s : b->mem) {
-            EMS_TRY(copy_state(s, s->adj->ckpt[q], true));
-            if (last) EMS_TRY(snap_e(s, 0));
-        }
-        for (int n = a0; n < a1; ++n) {
+    EMS_TRY(b->grad.reserve(sizeof(double) * 2 * nm));
+    EMS_TRY(b->sums.reserve(sizeof(double) * 2 * s0->nalloc));
+    EMS_TRY(b->fac.reserve(sizeof(double) * 4 * nm));
+    GradStore *gs = nullptr;
+    EMS_TRY(grad_store(&gs));
+    std::lock_guard<std::mutex> store_lock(gs->mu);
+    GradPlan plan;
+    char *ckb = nullptr;
+    EMS_TRY(grad_prepare(*gs, b->mem, N, plan, &ckb));
+    // adjoint launch data (set once the residual weights are known)
+    std::vector<int> serial(ns, 0);
+    int par = 0;
+    double *gp = nullptr, *gq = nullptr;
+    const AdjGeom ag = adj_geom(g, s0->npml);
+    const T ce = (T)(s0->d.dt / s0->d.eps / s0->d.h);
+    const T nch = -(T)(s0->d.dt / s0->d.mu / s0->d.h);
+    auto snap_all = [&](int slot) -> ems_status {
+        for (auto *s : b->mem) EMS_TRY(snap_e(s, slot));
+        return EMS_OK;
+    };
+    auto state_all = [&](int idx, bool save) -> ems_status {
+        for (int q = 0; q < ns; ++q) EMS_TRY(copy_state(b->mem[q], ckb + ((size_t)idx * ns + q) * sb, save));
+        return EMS_OK;
+    };
+    auto ops = grad_ops(
+        [&](int n, bool record, int slot) -> ems_status {
             batch_sweeps_any<T>(b, md.mat, md.pml, total, ns);
-            k_post_batch<T><<<ns, 256, 0, st>>>(dm, b->d_post.as<BatchPost>(), n + 1, nullptr);
-            if (last)
-                for (auto *s : b->mem) EMS_TRY(snap_e(s, n + 1 - a0));
-        }
-    }
-    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin, true));
+            k_post_batch<T><<<ns, 256, 0, st>>>(dm, (record ? b->d_post : b->d_post2).as<BatchPost>(), n + 1, nullptr);
+            EMS_HIP(hipGetLastError());
+            if (slot >= 0) EMS_TRY(snap_all(slot));
+            return EMS_OK;
+        },
+        snap_all, [&](int idx) { return state_all(idx, true); }, [&](int idx) { return state_all(idx, false); },
+        [&](int n, int slot) -> ems_status {
+            if ((n + 1) % tf == 0)
+                for (int m = 0; m < ns; ++m) {
+                    ems_fdtd *s = b->mem[m];
+                    const int npts = rp[m].npts;
+                    if (!npts) continue;
+                    Adj *A = s->adj;
+                    const Adj3<T> e_cur{{A->e[0].as<T>(), A->e[1].as<T>(), A->e[2].as<T>()}};
+                    k_adj_inject<T><<<serial[m] ? 1 : (npts + 255) / 256, serial[m] ? 64 : 256, 0, st>>>(
+                        e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
+                        A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial[m]);
+                }
+            adj_sweeps<T>(md.mat, md.pml, g, ag, st, ce, nch, b->d_shots[par].as<AdjShot<T>>(), ns, s0->ca.as<T>(),
+                          s0->cb.as<T>(), gp, gq, (long)s0->nalloc, slot);
+            EMS_HIP(hipGetLastError());
+            par = 1 - par;
+            return EMS_OK;
+        });
+    GradSchedule<decltype(ops)> sched{ops, plan, N};
+    EMS_TRY(sched.first_pass());
+    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin.p, true));
     EMS_HIP(hipStreamSynchronize(st));
     EMS_HIP(hipGetLastError());
     // traces, misfits and residual weights
-    std::vector<int> serial(ns, 0);
     double Jsum = 0.0;
     for (int q = 0; q < ns; ++q) {
         ems_fdtd *s = b->mem[q];
@@ -1588,106 +1867,40 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
         misfit_of(s, N, rp[q], raw, d_obs[q], tr, J, w, serial[q], pcs[q], pus[q]);
         if (misfits) misfits[q] = J;
         Jsum += J;
-        EMS_TRY(s->adj->w.reserve(sizeof(double) * w.size()));
         EMS_HIP(hipMemcpy(s->adj->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
         s->traces.swap(tr);
         s->nslot = N / tf + 1;
         s->t_now = rp[q].t_end;
     }
     if (Jtot) *Jtot = Jsum;
-    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
-    if (b->fac_owner != s0 || b->fac_version != s0->mat_version || !b->fac.p) {
+    if (b->fac_owner != s0 || b->fac_version != s0->mat_version) {
         const std::vector<double> fac = material_factors(s0);
         EMS_TRY(b->fac.reserve(sizeof(double) * fac.size()));
         EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
         b->fac_owner = s0;
         b->fac_version = s0->mat_version;
     }
-    EMS_TRY(zero(b->grad, sizeof(double) * 2 * nm, st));
-    // adjoint state per shot and the two parity tables
-    const size_t fb = s0->nalloc * s0->esz;
+    EMS_TRY(zero(b->sums, sizeof(double) * 2 * s0->nalloc, st));
+    // the two parity tables of the shots' adjoint state
     std::vector<AdjShot<T>> tab[2];
-    for (auto *s : b->mem) {
-        Adj *A = s->adj;
-        for (int c = 0; c < 3; ++c) {
-            EMS_TRY(zero(A->e[0][c], fb, st));
-            EMS_TRY(zero(A->e[1][c], fb, st));
-            EMS_TRY(zero(A->h[c], fb, st));
-        }
-        if (s->npml > 0)
-            for (int ax = 0; ax < 3; ++ax)
-                for (int q = 0; q < 2; ++q)
-                    for (int bb = 0; bb < 2; ++bb) {
-                        EMS_TRY(zero(A->pe[bb][ax * 2 + q], s->psi[ax * 4 + 2 + q].bytes, st));
-                        EMS_TRY(zero(A->ph[bb][ax * 2 + q], s->psi[ax * 4 + q].bytes, st));
-                    }
-        for (int par = 0; par < 2; ++par) {
-            AdjShot<T> sh{};
-            sh.ein = Adj3<T>{{A->e[par][0].as<T>(), A->e[par][1].as<T>(), A->e[par][2].as<T>()}};
-            sh.eout = Adj3<T>{{A->e[1 - par][0].as<T>(), A->e[1 - par][1].as<T>(), A->e[1 - par][2].as<T>()}};
-            sh.hb = Adj3<T>{{A->h[0].as<T>(), A->h[1].as<T>(), A->h[2].as<T>()}};
-            sh.seg = A->seg.as<T>();
-            sh.a = adj_pml<T>(s, par);
-            tab[par].push_back(sh);
-        }
-    }
-    for (int par = 0; par < 2; ++par) {
-        EMS_TRY(b->d_shots[par].reserve(sizeof(AdjShot<T>) * ns));
-        EMS_HIP(hipMemcpy(b->d_shots[par].p, tab[par].data(), sizeof(AdjShot<T>) * ns, hipMemcpyHostToDevice));
+    for (auto *s : b->mem)
+        for (int pp = 0; pp < 2; ++pp) tab[pp].push_back(adj_shot<T>(s, pp));
+    for (int pp = 0; pp < 2; ++pp) {
+        EMS_TRY(b->d_shots[pp].reserve(sizeof(AdjShot<T>) * ns));
+        EMS_HIP(hipMemcpy(b->d_shots[pp].p, tab[pp].data(), sizeof(AdjShot<T>) * ns, hipMemcpyHostToDevice));
     }
     GradView gv{b->fac.as<double>(), b->fac.as<double>() + nm, b->fac.as<double>() + 2 * nm,
                 b->fac.as<double>() + 3 * nm, b->grad.as<double>(), b->grad.as<double>() + nm};
-    const dim3 blk(64, 4, 1), grd(adj_grid(g), 1, 1);
-    const T ce = (T)(s0->d.dt / s0->d.eps / s0->d.h);
-    const T nch = -(T)(s0->d.dt / s0->d.mu / s0->d.h);
-    int par = 0;
-    for (int q = nseg - 1; q >= 0; --q) {
-        const int a0 = q * K, a1 = std::min(N, a0 + K);
-        if (q != nseg - 1) {
-            for (auto *s : b->mem) {
-                EMS_TRY(copy_state(s, s->adj->ckpt[q], false));
-                EMS_TRY(snap_e(s, 0));
-            }
-            for (int n = a0; n < a1; ++n) {
-                batch_sweeps_any<T>(b, md.mat, md.pml, total, ns);
-                k_post_batch<T><<<ns, 256, 0, st>>>(dm, b->d_post2.as<BatchPost>(), n + 1, nullptr);
-                for (auto *s : b->mem) EMS_TRY(snap_e(s, n + 1 - a0));
-            }
-        }
-        for (int n = a1 - 1; n >= a0; --n) {
-            if ((n + 1) % tf == 0)
-                for (int m = 0; m < ns; ++m) {
-                    ems_fdtd *s = b->mem[m];
-                    const int npts = rp[m].npts;
-                    if (!npts) continue;
-                    Adj *A = s->adj;
-                    Adj3<T> e_cur{{A->e[par][0].as<T>(), A->e[par][1].as<T>(), A->e[par][2].as<T>()}};
-                    k_adj_inject<T><<<serial[m] ? 1 : (npts + 255) / 256, serial[m] ? 64 : 256, 0, st>>>(
-                        e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
-                        A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial[m]);
-                }
-            const AdjShot<T> *sh = b->d_shots[par].as<AdjShot<T>>();
-            const T *ca = s0->ca.as<T>(), *cb = s0->cb.as<T>();
-            switch (md.mat * 2 + (md.pml ? 1 : 0)) {
-            case 0: k_adj_a_group<T, 0, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            case 1: k_adj_a_group<T, 0, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            case 2: k_adj_a_group<T, 1, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            case 3: k_adj_a_group<T, 1, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            case 4: k_adj_a_group<T, 2, false><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            default: k_adj_a_group<T, 2, true><<<grd, blk, 0, st>>>(g, ce, sh, ns, ca, cb, gv, (long)s0->nalloc, n - a0); break;
-            }
-            if (md.pml) k_adj_b_group<T, true><<<grd, blk, 0, st>>>(g, nch, sh, ns);
-            else k_adj_b_group<T, false><<<grd, blk, 0, st>>>(g, nch, sh, ns);
-            EMS_HIP(hipGetLastError());
-            par = 1 - par;
-        }
-    }
-    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin, false));
+    gp = b->sums.as<double>();
+    gq = gp + s0->nalloc;
+    EMS_TRY(sched.backward());
+    k_adj_grad<<<(unsigned)((nm + 255) / 256), 256, 0, st>>>(g, gp, gq, gv);
+    EMS_HIP(hipGetLastError());
+    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin.p, false));
     EMS_HIP(hipStreamSynchronize(st));
-    std::vector<double> gr(2 * nm);
-    EMS_HIP(hipMemcpy(gr.data(), b->grad.p, sizeof(double) * gr.size(), hipMemcpyDeviceToHost));
-    if (g_eps) std::copy(gr.begin(), gr.begin() + nm, g_eps);
-    if (g_sig) std::copy(gr.begin() + nm, gr.end(), g_sig);
+    // straight into the caller's arrays (no staging copy)
+    if (g_eps) EMS_HIP(hipMemcpy(g_eps, b->grad.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
+    if (g_sig) EMS_HIP(hipMemcpy(g_sig, b->grad.as<double>() + nm, sizeof(double) * nm, hipMemcpyDeviceToHost));
     return EMS_OK;
 }
 
@@ -2217,6 +2430,30 @@ ems_status ems_fdtd_set_checkpoint_interval(ems_fdtd *s, int interval)
     if (!s) return invalid("null handle");
     if (interval < 0) return invalid("checkpoint interval must be >= 0");
     s->ckpt_interval = interval;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_gradient_memory(ems_fdtd *s, size_t bytes)
+{
+    if (!s) return invalid("null handle");
+    s->grad_mem = bytes;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_gradient_plan(ems_fdtd *s, int num_steps, int *interval1, int *interval, size_t *bytes)
+{
+    if (!s) return invalid("null handle");
+    if (num_steps < 1) return invalid("gradient plan: num_steps must be >= 1");
+    EMS_TRY(setup_pml(s));  // the checkpoints hold the CPML state
+    GradStore *gs = nullptr;
+    EMS_TRY(grad_store(&gs));
+    std::lock_guard<std::mutex> lock(gs->mu);
+    GradPlan p;
+    double b = 0.0;
+    EMS_TRY(grad_plan_for(*gs, s, 1, num_steps, p, b));
+    if (interval1) *interval1 = p.K1;
+    if (interval) *interval = p.Ks;
+    if (bytes) *bytes = (size_t)b;
     return EMS_OK;
 }
 

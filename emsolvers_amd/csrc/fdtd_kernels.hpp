@@ -1248,18 +1248,18 @@ namespace fdtd {
 // differences:
 //   pass A (E update transposed):  mu = Cb e'; per derivative u = +-mu,
 //     in the CPML of its axis  p = psiE' + u, tbar = u k + a p, psiE = b p,
-//     else tbar = u;  hbar += (tbar(own) - tbar(+1)) + (tbar(own) - tbar(+1));
-//     e = Ca e'; gradient += e' (dCa E^n + (dCb/Cb)(E^{n+1} - Ca E^n)).
+//     else tbar = u;  hbar += (tbar(own) - tbar(+1)) + (tbar(own) - tbar(+1)).
 //   pass B (H update transposed):  nu = -ch hbar, the same CPML transpose with
-//     the H-type profiles, e += (sbar(-1) - sbar(own)) + (sbar(-1) - sbar(own)).
-// psi adjoints and e are ping-ponged between old and new buffers; every term
-// of a cell outside a component's update range is 0.  One thread per unified
-// point, workgroup 64 x 4, grid (ceil((nx+1)/64), ceil((ny+1)/4), nz+1).
+//     the H-type profiles, e = Ca e' + (sbar(-1) - sbar(own)) + (sbar(-1) - sbar(own));
+//     gradient sums P += e' E^n, Q += e' (E^{n+1} - Ca E^n) (FP64), whence
+//     dJ/deps_r = dCa/deps_r P + (dCb/deps_r)/Cb Q and likewise for sigma.
+// Every term of a cell outside a component's update range is 0.
 // ---------------------------------------------------------------------------
 template <typename T> struct Adj3 {
     T *c[3];
 };
 
+// psi adjoint slabs (as PmlView; old and new buffers) and CPML gradings
 template <typename T> struct AdjPml {
     int P, W;
     const Prof<T> *hp[3], *ep[3];
@@ -1269,266 +1269,450 @@ template <typename T> struct AdjPml {
     T *ph_new[3][2];
 };
 
-// derivative d of component c: axis and slot within the axis' psi pair
-__device__ __forceinline__ int adj_axis(int c, int d) { return c == 0 ? 1 + d : c == 1 ? (d == 0 ? 2 : 0) : (d == 0 ? 0 : 1); }
-__device__ __forceinline__ int adj_slot(int c, int d) { return c == 0 ? 0 : c == 1 ? (d == 0 ? 1 : 0) : 1; }
-
-__device__ __forceinline__ bool e_valid(const GridView &g, int c, int I, int J, int K)
-{
-    if (c == 0) return I <= g.nx && J >= 1 && J <= g.ny && K >= 1 && K <= g.nz;
-    if (c == 1) return I >= 1 && I <= g.nx && J <= g.ny && K >= 1 && K <= g.nz;
-    return I >= 1 && I <= g.nx && J >= 1 && J <= g.ny && K <= g.nz;
-}
-__device__ __forceinline__ bool h_valid(const GridView &g, int c, int I, int J, int K)
-{
-    if (I < 0 || J < 0 || K < 0) return false;
-    if (c == 0) return I >= 1 && I <= g.nx && J <= g.ny && K <= g.nz;
-    if (c == 1) return I <= g.nx && J >= 1 && J <= g.ny && K <= g.nz;
-    return I <= g.nx && J <= g.ny && K >= 1 && K <= g.nz;
-}
-
-__device__ __forceinline__ long slab_of(const GridView &g, int P, int W, int ax, int I, int J, int K)
-{
-    return ax == 0 ? slab_x(g, P, W, I, J, K) : ax == 1 ? slab_y(g, P, W, I, J, K) : slab_z(g, P, W, I, J, K);
-}
-
-template <typename T, int MAT, bool PML>
-__device__ __forceinline__ T adj_tbar(const GridView &g, const AdjPml<T> &a, const T *__restrict__ eb, T ce,
-                                      const T *__restrict__ cb, int c, int d, int I, int J, int K)
-{
-    if (!e_valid(g, c, I, J, K)) return (T)0;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    const T mu = (MAT == 0 ? ce : cb[u]) * eb[u];
-    const T v = d == 0 ? mu : -mu;
-    if (PML) {
-        const int ax = adj_axis(c, d);
-        const int pos = ax == 0 ? I : ax == 1 ? J : K;
-        const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
-        if (in_pml(a.P, n, pos)) {
-            const T p = a.pe_old[ax][adj_slot(c, d)][slab_of(g, a.P, a.W, ax, I, J, K)] + v;
-            const Prof<T> pr = a.ep[ax][pos];
-            return v * pr.k + pr.a * p;
-        }
-    }
-    return v;
-}
-
-template <typename T, bool PML>
-__device__ __forceinline__ T adj_sbar(const GridView &g, const AdjPml<T> &a, const T *__restrict__ hb, T nch, int c,
-                                      int d, int I, int J, int K)
-{
-    if (!h_valid(g, c, I, J, K)) return (T)0;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    const T nu = nch * hb[u];
-    const T v = d == 0 ? nu : -nu;
-    if (PML) {
-        const int ax = adj_axis(c, d);
-        const int pos = ax == 0 ? I : ax == 1 ? J : K;
-        const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
-        if (in_pml(a.P, n, pos)) {
-            const T p = a.ph_old[ax][adj_slot(c, d)][slab_of(g, a.P, a.W, ax, I, J, K)] + v;
-            const Prof<T> pr = a.hp[ax][pos];
-            return v * pr.k + pr.a * p;
-        }
-    }
-    return v;
-}
-
 // Material derivative factors on the (nx+1)(ny+1)(nz+1) material grid.
 struct GradView {
     const double *dca_e, *dcbn_e, *dca_s, *dcbn_s;
     double *g_eps, *g_sig;
 };
 
-// Adjoint launches: a 1-D grid of 64 x 4 workgroups over (I, J) blocks and
-// single planes K, K fastest, in plain order (measured 3 % faster than
-// giving each XCD a contiguous range: the eight XCDs then work on
-// neighbouring planes, whose shared lines the caches behind the L2s serve).
-__device__ __forceinline__ bool adj_coords(const GridView &g, int &I, int &J, int &K)
-{
-    const int nkz = g.nz + 1;
-    const int nbx = (g.nx + 1 + 63) / 64, nby = (g.ny + 1 + 3) / 4;
-    const long total = (long)nbx * nby * nkz;
-    const long lb = blockIdx.x;
-    if (lb >= total) return false;
-    K = (int)(lb % nkz);
-    const long r = lb / nkz;
-    I = (int)(r % nbx) * 64 + threadIdx.x;
-    J = (int)(r / nbx) * 4 + threadIdx.y;
-    return I <= g.nx && J <= g.ny;
-}
+// ---------------------------------------------------------------------------
+// Adjoint sweeps over columns, in the layout and workgroup order of the
+// forward split sweeps (split_column, split_tile; 64 x 4 workgroups).  Each
+// thread owns one (I, J) column and ADJ_LZ planes of it.
+//   k_adj_h (transpose of the E update): tbar of the three E components at
+//     the cell and at its I+1, J+1 and K+1 neighbours, hbar += forward
+//     differences, new psi_E adjoints (ping-pong: the neighbours read the old
+//     ones).
+//   k_adj_e (transpose of the H update): sbar at the cell and at its I-1,
+//     J-1 and K-1 neighbours; the adjoint E is updated in place,
+//     e <- Ca e + backward differences (each thread reads only its own e);
+//     new psi_H adjoints; with GRAD the gradient sums of the cell,
+//       P += sum_c e_c E^n_c,   Q += sum_c e_c (E^{n+1}_c - Ca E^n_c),
+//     from the adjoint e before its update (FP64; the factors of eps_r and
+//     sigma multiply P and Q once, at the end).
+// The arithmetic of every term is that of the point-wise definition: terms of
+// cells outside a component's update range come out as (signed) zeros, since
+// the adjoint fields and psi adjoints are zero there and every material
+// coefficient read is a cell of the box.  All loads of a thread's planes are
+// issued before its first store.
+// Several shots on one model run in one launch: the table holds each shot's
+// adjoint state, the shots of a cell are visited in table order, and the
+// gradient sums of all shots are added once per cell (shot order).
+// ---------------------------------------------------------------------------
+constexpr int ADJ_LZ = 2;
 
-// Pass A at unified point (I, J, K); returns the point's gradient terms
-// (summed over its E components from 0).
-template <typename T, int MAT, bool PML, bool GRAD>
-__device__ __forceinline__ void adj_a_body(const GridView &g, T ce, const Adj3<T> &ein, const Adj3<T> &eout,
-                                           const Adj3<T> &hb, const T *__restrict__ ca, const T *__restrict__ cb,
-                                           const AdjPml<T> &a, const Adj3<T> &en, const Adj3<T> &e1,
-                                           const GradView &gv, int I, int J, int K, long u, long m, double &tse,
-                                           double &tss)
-{
-    const T *ex = ein.c[0], *ey = ein.c[1], *ez = ein.c[2];
-    if (h_valid(g, 0, I, J, K)) {
-        const T A = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 0, I, J, K);
-        const T B = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 0, I, J, K + 1);
-        const T C = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 1, I, J, K);
-        const T D = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 1, I, J + 1, K);
-        hb.c[0][u] = hb.c[0][u] + ((A - B) + (C - D));
-    }
-    if (h_valid(g, 1, I, J, K)) {
-        const T A = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 0, I, J, K);
-        const T B = adj_tbar<T, MAT, PML>(g, a, ez, ce, cb, 2, 0, I + 1, J, K);
-        const T C = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 1, I, J, K);
-        const T D = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 1, I, J, K + 1);
-        hb.c[1][u] = hb.c[1][u] + ((A - B) + (C - D));
-    }
-    if (h_valid(g, 2, I, J, K)) {
-        const T A = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 0, I, J, K);
-        const T B = adj_tbar<T, MAT, PML>(g, a, ex, ce, cb, 0, 0, I, J + 1, K);
-        const T C = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 1, I, J, K);
-        const T D = adj_tbar<T, MAT, PML>(g, a, ey, ce, cb, 1, 1, I + 1, J, K);
-        hb.c[2][u] = hb.c[2][u] + ((A - B) + (C - D));
-    }
-    tse = 0.0;
-    tss = 0.0;
-    for (int c = 0; c < 3; ++c) {
-        if (!e_valid(g, c, I, J, K)) continue;
-        const T lam = ein.c[c][u];
-        if (PML) {
-            const T mu = (MAT == 0 ? ce : cb[u]) * lam;
-            for (int d = 0; d < 2; ++d) {
-                const int ax = adj_axis(c, d);
-                const int pos = ax == 0 ? I : ax == 1 ? J : K;
-                const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
-                if (!in_pml(a.P, n, pos)) continue;
-                const long q = slab_of(g, a.P, a.W, ax, I, J, K);
-                const int sl = adj_slot(c, d);
-                const T v = d == 0 ? mu : -mu;
-                a.pe_new[ax][sl][q] = a.ep[ax][pos].b * (a.pe_old[ax][sl][q] + v);
-            }
-        }
-        const T cav = MAT == 2 ? ca[u] : (T)1;
-        eout.c[c][u] = cav * lam;
-        if (GRAD) {
-            const double l = (double)lam, e0 = (double)en.c[c][u], ep = (double)e1.c[c][u];
-            const double de = ep - (double)cav * e0;
-            tse = tse + l * (gv.dca_e[m] * e0 + gv.dcbn_e[m] * de);
-            tss = tss + l * (gv.dca_s[m] * e0 + gv.dcbn_s[m] * de);
-        }
-    }
-}
-
-template <typename T, int MAT, bool PML, bool GRAD>
-__global__ __launch_bounds__(256) void k_adj_a(GridView g, T ce, Adj3<T> ein, Adj3<T> eout, Adj3<T> hb,
-                                              const T *__restrict__ ca, const T *__restrict__ cb, AdjPml<T> a,
-                                              Adj3<T> en, Adj3<T> e1, GradView gv)
-{
-    int I, J, K;
-    if (!adj_coords(g, I, J, K)) return;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
-    double tse, tss;
-    adj_a_body<T, MAT, PML, GRAD>(g, ce, ein, eout, hb, ca, cb, a, en, e1, gv, I, J, K, u, m, tse, tss);
-    if (GRAD) {
-        gv.g_eps[m] += tse;
-        gv.g_sig[m] += tss;
-    }
-}
-
-// Several shots on one model: the adjoint state of each shot, shared
-// material, CPML grading and gradient.
 template <typename T> struct AdjShot {
-    Adj3<T> ein, eout, hb;
-    T *seg;       // E^n snapshots of the segment: [slot][3][nalloc]
-    AdjPml<T> a;  // psi pointers of this shot (grading shared)
+    Adj3<T> e, hb;  // adjoint E (in place) and H
+    T *seg;         // E^n snapshots of the segment: [slot][3][nalloc]
+    AdjPml<T> a;    // psi adjoints of this shot (old, new), shared grading
 };
 
-// Pass A for every shot of a model at snapshot slot `slot` (E^n) and
-// slot + 1 (E^{n+1}); the shots' gradient terms are summed in registers
-// (shot order) and the FP64 gradient is updated once per point.
-template <typename T, int MAT, bool PML>
-__global__ __launch_bounds__(256) void k_adj_a_group(GridView g, T ce, const AdjShot<T> *__restrict__ shots, int ns,
-                                                    const T *__restrict__ ca, const T *__restrict__ cb, GradView gv,
-                                                    long nalloc, int slot)
+// CPML transpose of one derivative term v at a cell inside the CPML of the
+// derivative's axis (profile pr, old psi adjoint p0): v k + a (p0 + v).
+template <typename T> __device__ __forceinline__ T adj_term(T v, bool in, T p0, const Prof<T> &pr)
 {
-    int I, J, K;
-    if (!adj_coords(g, I, J, K)) return;
-    const long u = g.off + I + g.pitch * J + g.slab * K;
-    const long m = I + (long)(g.nx + 1) * (J + (long)(g.ny + 1) * K);
-    double se = 0.0, ss = 0.0;
-    for (int q = 0; q < ns; ++q) {
-        const AdjShot<T> &sh = shots[q];
-        T *b0 = sh.seg + (long)slot * 3 * nalloc, *b1 = b0 + 3 * nalloc;
-        const Adj3<T> en{{b0, b0 + nalloc, b0 + 2 * nalloc}}, e1{{b1, b1 + nalloc, b1 + 2 * nalloc}};
-        double tse, tss;
-        adj_a_body<T, MAT, PML, true>(g, ce, sh.ein, sh.eout, sh.hb, ca, cb, sh.a, en, e1, gv, I, J, K, u, m, tse,
-                                      tss);
-        se = se + tse;
-        ss = ss + tss;
-    }
-    gv.g_eps[m] += se;
-    gv.g_sig[m] += ss;
+    if (!in) return v;
+    const T p = p0 + v;
+    return v * pr.k + pr.a * p;
 }
 
-template <typename T, bool PML>
-__device__ __forceinline__ void adj_b_body(const GridView &g, T nch, const Adj3<T> &eb, const Adj3<T> &hb,
-                                           const AdjPml<T> &a, int I, int J, int K, long u)
+template <typename T, int MAT, bool PML>
+__device__ __forceinline__ void adj_h_column(const GridView &g, T ce, const AdjShot<T> &sh, const T *__restrict__ cb,
+                                             int I, int J, int K0, int K1)
 {
-    const T *hx = hb.c[0], *hy = hb.c[1], *hz = hb.c[2];
-    if (e_valid(g, 0, I, J, K)) {
-        const T A = adj_sbar<T, PML>(g, a, hy, nch, 1, 0, I, J, K - 1);
-        const T B = adj_sbar<T, PML>(g, a, hy, nch, 1, 0, I, J, K);
-        const T C = adj_sbar<T, PML>(g, a, hz, nch, 2, 1, I, J - 1, K);
-        const T D = adj_sbar<T, PML>(g, a, hz, nch, 2, 1, I, J, K);
-        eb.c[0][u] = eb.c[0][u] + ((A - B) + (C - D));
+    constexpr int L = ADJ_LZ;
+    const long P = g.pitch, S = g.slab;
+    const AdjPml<T> &a = sh.a;
+    const T *ex = sh.e.c[0], *ey = sh.e.c[1], *ez = sh.e.c[2];
+    T *hx = sh.hb.c[0], *hy = sh.hb.c[1], *hz = sh.hb.c[2];
+    const long ub0 = g.off + S * K0;
+    const unsigned ui = (unsigned)(I + P * J) * sizeof(T);
+    const unsigned uI = ui + (unsigned)sizeof(T), uJ = ui + (unsigned)(P * sizeof(T));
+    const bool okI = I + 1 <= g.nx, okJ = J + 1 <= g.ny;
+    // planes K0 .. K0+L of the cell (the last one only for the K+1 terms),
+    // planes K0 .. K0+L-1 of the neighbours and of hbar
+    T lx[L + 1], ly[L + 1], cbo[L + 1], lz[L], zI[L], yI[L], cI[L], zJ[L], xJ[L], cJ[L], hbx[L], hby[L], hbz[L];
+#pragma unroll
+    for (int k = 0; k <= L; ++k) {
+        lx[k] = ly[k] = cbo[k] = (T)0;
+        if (K0 + k <= g.nz + 1) {
+            const long b = ub0 + S * k;
+            lx[k] = lane_at(ex + b, ui);
+            ly[k] = lane_at(ey + b, ui);
+            if (MAT >= 1) cbo[k] = lane_at(cb + b, ui);
+        }
     }
-    if (e_valid(g, 1, I, J, K)) {
-        const T A = adj_sbar<T, PML>(g, a, hx, nch, 0, 1, I, J, K - 1);
-        const T B = adj_sbar<T, PML>(g, a, hx, nch, 0, 1, I, J, K);
-        const T C = adj_sbar<T, PML>(g, a, hz, nch, 2, 0, I - 1, J, K);
-        const T D = adj_sbar<T, PML>(g, a, hz, nch, 2, 0, I, J, K);
-        eb.c[1][u] = eb.c[1][u] + ((A - B) + (C - D));
-    }
-    if (e_valid(g, 2, I, J, K)) {
-        const T A = adj_sbar<T, PML>(g, a, hx, nch, 0, 0, I, J - 1, K);
-        const T B = adj_sbar<T, PML>(g, a, hx, nch, 0, 0, I, J, K);
-        const T C = adj_sbar<T, PML>(g, a, hy, nch, 1, 1, I - 1, J, K);
-        const T D = adj_sbar<T, PML>(g, a, hy, nch, 1, 1, I, J, K);
-        eb.c[2][u] = eb.c[2][u] + ((A - B) + (C - D));
-    }
-    if (PML) {
-        for (int c = 0; c < 3; ++c) {
-            if (!h_valid(g, c, I, J, K)) continue;
-            const T nu = nch * hb.c[c][u];
-            for (int d = 0; d < 2; ++d) {
-                const int ax = adj_axis(c, d);
-                const int pos = ax == 0 ? I : ax == 1 ? J : K;
-                const int n = ax == 0 ? g.nx : ax == 1 ? g.ny : g.nz;
-                if (!in_pml(a.P, n, pos)) continue;
-                const long q = slab_of(g, a.P, a.W, ax, I, J, K);
-                const int sl = adj_slot(c, d);
-                const T v = d == 0 ? nu : -nu;
-                a.ph_new[ax][sl][q] = a.hp[ax][pos].b * (a.ph_old[ax][sl][q] + v);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        lz[k] = zI[k] = yI[k] = cI[k] = zJ[k] = xJ[k] = cJ[k] = hbx[k] = hby[k] = hbz[k] = (T)0;
+        if (K0 + k < K1) {
+            const long b = ub0 + S * k;
+            lz[k] = lane_at(ez + b, ui);
+            zI[k] = lane_at(ez + b, uI);
+            yI[k] = lane_at(ey + b, uI);
+            zJ[k] = lane_at(ez + b, uJ);
+            xJ[k] = lane_at(ex + b, uJ);
+            if (MAT >= 1) {
+                cI[k] = lane_at(cb + b, uI);
+                cJ[k] = lane_at(cb + b, uJ);
             }
+            hbx[k] = lane_at(hx + b, ui);
+            hby[k] = lane_at(hy + b, ui);
+            hbz[k] = lane_at(hz + b, ui);
+        }
+    }
+    bool px = false, py = false, pxn = false, pyn = false;
+    Prof<T> prx{}, pry{}, prxn{}, pryn{};
+    int qx = 0, qy = 0, qxn = 0, qyn = 0;
+    if (PML) {
+        px = in_pml(a.P, g.nx, I);
+        py = in_pml(a.P, g.ny, J);
+        pxn = okI && in_pml(a.P, g.nx, I + 1);
+        pyn = okJ && in_pml(a.P, g.ny, J + 1);
+        if (px) prx = a.ep[0][I];
+        if (py) pry = a.ep[1][J];
+        if (pxn) prxn = a.ep[0][I + 1];
+        if (pyn) pryn = a.ep[1][J + 1];
+    }
+    // the z-derivative terms of plane K0 (tbar of Ey, d = 0, and Ex, d = 1)
+    T tby0, tbx1;
+    {
+        const T mx = (MAT == 0 ? ce : cbo[0]) * lx[0], my = (MAT == 0 ? ce : cbo[0]) * ly[0];
+        tby0 = my;
+        tbx1 = -mx;
+        if (PML && in_pml(a.P, g.nz, K0)) {
+            const int qz = slab_z(g, a.P, a.W, I, J, K0);
+            const Prof<T> prz = a.ep[2][K0];
+            tby0 = adj_term(my, true, a.pe_old[2][1][qz], prz);
+            tbx1 = adj_term(-mx, true, a.pe_old[2][0][qz], prz);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const int K = K0 + k;
+        if (K >= K1) break;
+        const long b = ub0 + S * k;
+        const T c0 = MAT == 0 ? ce : cbo[k], c1 = MAT == 0 ? ce : cbo[k + 1];
+        const T mx = c0 * lx[k], my = c0 * ly[k], mz = c0 * lz[k];
+        // K+1 terms (zero above the update range)
+        T tby0n = (T)0, tbx1n = (T)0;
+        if (K + 1 <= g.nz) {
+            const T mx1 = c1 * lx[k + 1], my1 = c1 * ly[k + 1];
+            tby0n = my1;
+            tbx1n = -mx1;
+            if (PML && in_pml(a.P, g.nz, K + 1)) {
+                const int qz = slab_z(g, a.P, a.W, I, J, K + 1);
+                const Prof<T> prz = a.ep[2][K + 1];
+                tby0n = adj_term(my1, true, a.pe_old[2][1][qz], prz);
+                tbx1n = adj_term(-mx1, true, a.pe_old[2][0][qz], prz);
+            }
+        }
+        // x- and y-derivative terms of the cell
+        T ox0 = (T)0, ox1 = (T)0, oy0 = (T)0, oy1 = (T)0;  // old psi: x (Ey, Ez), y (Ex, Ez)
+        if (PML) {
+            if (px) {
+                qx = slab_x(g, a.P, a.W, I, J, K);
+                ox0 = a.pe_old[0][0][qx];
+                ox1 = a.pe_old[0][1][qx];
+            }
+            if (py) {
+                qy = slab_y(g, a.P, a.W, I, J, K);
+                oy0 = a.pe_old[1][0][qy];
+                oy1 = a.pe_old[1][1][qy];
+            }
+        }
+        const T tbz0 = adj_term(mz, PML && px, ox1, prx);
+        const T tby1 = adj_term(-my, PML && px, ox0, prx);
+        const T tbz1 = adj_term(-mz, PML && py, oy1, pry);
+        const T tbx0 = adj_term(mx, PML && py, oy0, pry);
+        // neighbours I+1 (Ez d = 0, Ey d = 1) and J+1 (Ez d = 1, Ex d = 0)
+        T tbz0I = (T)0, tby1I = (T)0, tbz1J = (T)0, tbx0J = (T)0;
+        if (okI) {
+            const T cc = MAT == 0 ? ce : cI[k];
+            const T mzI = cc * zI[k], myI = cc * yI[k];
+            T o0 = (T)0, o1 = (T)0;
+            if (PML && pxn) {
+                qxn = slab_x(g, a.P, a.W, I + 1, J, K);
+                o0 = a.pe_old[0][0][qxn];
+                o1 = a.pe_old[0][1][qxn];
+            }
+            tbz0I = adj_term(mzI, PML && pxn, o1, prxn);
+            tby1I = adj_term(-myI, PML && pxn, o0, prxn);
+        }
+        if (okJ) {
+            const T cc = MAT == 0 ? ce : cJ[k];
+            const T mzJ = cc * zJ[k], mxJ = cc * xJ[k];
+            T o0 = (T)0, o1 = (T)0;
+            if (PML && pyn) {
+                qyn = slab_y(g, a.P, a.W, I, J + 1, K);
+                o0 = a.pe_old[1][0][qyn];
+                o1 = a.pe_old[1][1][qyn];
+            }
+            tbz1J = adj_term(-mzJ, PML && pyn, o1, pryn);
+            tbx0J = adj_term(mxJ, PML && pyn, o0, pryn);
+        }
+        if (I >= 1) lane_at(hx + b, ui) = hbx[k] + ((tby0 - tby0n) + (tbz1 - tbz1J));
+        if (J >= 1) lane_at(hy + b, ui) = hby[k] + ((tbz0 - tbz0I) + (tbx1 - tbx1n));
+        if (K >= 1) lane_at(hz + b, ui) = hbz[k] + ((tbx0 - tbx0J) + (tby1 - tby1I));
+        // new psi_E adjoints of the cell's updated components
+        if (PML) {
+            const bool vx = J >= 1 && K >= 1, vy = I >= 1 && K >= 1, vz = I >= 1 && J >= 1;
+            if (px) {
+                if (vy) a.pe_new[0][0][qx] = prx.b * (ox0 + -my);
+                if (vz) a.pe_new[0][1][qx] = prx.b * (ox1 + mz);
+            }
+            if (py) {
+                if (vx) a.pe_new[1][0][qy] = pry.b * (oy0 + mx);
+                if (vz) a.pe_new[1][1][qy] = pry.b * (oy1 + -mz);
+            }
+            if (in_pml(a.P, g.nz, K)) {
+                const int qz = slab_z(g, a.P, a.W, I, J, K);
+                const Prof<T> prz = a.ep[2][K];
+                if (vx) a.pe_new[2][0][qz] = prz.b * (a.pe_old[2][0][qz] + -mx);
+                if (vy) a.pe_new[2][1][qz] = prz.b * (a.pe_old[2][1][qz] + my);
+            }
+        }
+        tby0 = tby0n;
+        tbx1 = tbx1n;
+    }
+}
+
+template <typename T, int MAT, bool PML, bool GRAD>
+__device__ __forceinline__ void adj_e_column(const GridView &g, T nch, const AdjShot<T> &sh, const T *__restrict__ ca,
+                                             long nalloc, int slot, int I, int J, int K0, int K1,
+                                             double (&tp)[ADJ_LZ], double (&tq)[ADJ_LZ])
+{
+    constexpr int L = ADJ_LZ;
+    const long P = g.pitch, S = g.slab;
+    const AdjPml<T> &a = sh.a;
+    T *ex = sh.e.c[0], *ey = sh.e.c[1], *ez = sh.e.c[2];
+    const T *hx = sh.hb.c[0], *hy = sh.hb.c[1], *hz = sh.hb.c[2];
+    const T *e0x = sh.seg + (long)slot * 3 * nalloc, *e0y = e0x + nalloc, *e0z = e0y + nalloc;
+    const T *e1x = e0z + nalloc, *e1y = e1x + nalloc, *e1z = e1y + nalloc;
+    const long ub0 = g.off + S * K0;
+    const unsigned ui = (unsigned)(I + P * J) * sizeof(T);
+    const unsigned uIm = ui - (I >= 1 ? (unsigned)sizeof(T) : 0u);
+    const unsigned uJm = ui - (J >= 1 ? (unsigned)(P * sizeof(T)) : 0u);
+    // plane K0-1 of Hx, Hy (the K-1 terms of plane K0) and planes K0 .. K0+L-1
+    T hxm = (T)0, hym = (T)0;
+    if (K0 >= 1) {
+        hxm = lane_at(hx + (ub0 - S), ui);
+        hym = lane_at(hy + (ub0 - S), ui);
+    }
+    T hxo[L], hyo[L], hzo[L], zIm[L], yIm[L], zJm[L], xJm[L], lx[L], ly[L], lz[L], cav[L];
+    T fx0[L], fy0[L], fz0[L], fx1[L], fy1[L], fz1[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        hxo[k] = hyo[k] = hzo[k] = zIm[k] = yIm[k] = zJm[k] = xJm[k] = lx[k] = ly[k] = lz[k] = cav[k] = (T)0;
+        fx0[k] = fy0[k] = fz0[k] = fx1[k] = fy1[k] = fz1[k] = (T)0;
+        if (K0 + k < K1) {
+            const long b = ub0 + S * k;
+            hxo[k] = lane_at(hx + b, ui);
+            hyo[k] = lane_at(hy + b, ui);
+            hzo[k] = lane_at(hz + b, ui);
+            zIm[k] = lane_at(hz + b, uIm);
+            yIm[k] = lane_at(hy + b, uIm);
+            zJm[k] = lane_at(hz + b, uJm);
+            xJm[k] = lane_at(hx + b, uJm);
+            lx[k] = lane_at(ex + b, ui);
+            ly[k] = lane_at(ey + b, ui);
+            lz[k] = lane_at(ez + b, ui);
+            if (MAT == 2) cav[k] = lane_at(ca + b, ui);
+            if (GRAD) {
+                fx0[k] = lane_at(e0x + b, ui);
+                fy0[k] = lane_at(e0y + b, ui);
+                fz0[k] = lane_at(e0z + b, ui);
+                fx1[k] = lane_at(e1x + b, ui);
+                fy1[k] = lane_at(e1y + b, ui);
+                fz1[k] = lane_at(e1z + b, ui);
+            }
+        }
+    }
+    bool px = false, py = false, pxm = false, pym = false;
+    Prof<T> prx{}, pry{}, prxm{}, prym{};
+    int qx = 0, qy = 0, qxm = 0, qym = 0;
+    if (PML) {
+        px = in_pml(a.P, g.nx, I);
+        py = in_pml(a.P, g.ny, J);
+        pxm = I >= 1 && in_pml(a.P, g.nx, I - 1);
+        pym = J >= 1 && in_pml(a.P, g.ny, J - 1);
+        if (px) prx = a.hp[0][I];
+        if (py) pry = a.hp[1][J];
+        if (pxm) prxm = a.hp[0][I - 1];
+        if (pym) prym = a.hp[1][J - 1];
+    }
+    // the z-derivative terms of plane K0-1 (sbar of Hy, d = 0, and Hx, d = 1)
+    T sby0m = (T)0, sbx1m = (T)0;
+    if (K0 >= 1) {
+        const T nux = nch * hxm, nuy = nch * hym;
+        sby0m = nuy;
+        sbx1m = -nux;
+        if (PML && in_pml(a.P, g.nz, K0 - 1)) {
+            const int qz = slab_z(g, a.P, a.W, I, J, K0 - 1);
+            const Prof<T> prz = a.hp[2][K0 - 1];
+            sby0m = adj_term(nuy, true, a.ph_old[2][1][qz], prz);
+            sbx1m = adj_term(-nux, true, a.ph_old[2][0][qz], prz);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const int K = K0 + k;
+        if (K >= K1) break;
+        const long b = ub0 + S * k;
+        const T nux = nch * hxo[k], nuy = nch * hyo[k], nuz = nch * hzo[k];
+        T oz0 = (T)0, oz1 = (T)0, ox0 = (T)0, ox1 = (T)0, oy0 = (T)0, oy1 = (T)0;
+        bool pz = false;
+        Prof<T> prz{};
+        int qz = 0;
+        if (PML) {
+            pz = in_pml(a.P, g.nz, K);
+            if (pz) {
+                qz = slab_z(g, a.P, a.W, I, J, K);
+                prz = a.hp[2][K];
+                oz0 = a.ph_old[2][0][qz];
+                oz1 = a.ph_old[2][1][qz];
+            }
+            if (px) {
+                qx = slab_x(g, a.P, a.W, I, J, K);
+                ox0 = a.ph_old[0][0][qx];
+                ox1 = a.ph_old[0][1][qx];
+            }
+            if (py) {
+                qy = slab_y(g, a.P, a.W, I, J, K);
+                oy0 = a.ph_old[1][0][qy];
+                oy1 = a.ph_old[1][1][qy];
+            }
+        }
+        const T sby0 = adj_term(nuy, PML && pz, oz1, prz);
+        const T sbx1 = adj_term(-nux, PML && pz, oz0, prz);
+        const T sbz0 = adj_term(nuz, PML && px, ox1, prx);
+        const T sby1 = adj_term(-nuy, PML && px, ox0, prx);
+        const T sbz1 = adj_term(-nuz, PML && py, oy1, pry);
+        const T sbx0 = adj_term(nux, PML && py, oy0, pry);
+        // neighbours I-1 (Hz d = 0, Hy d = 1) and J-1 (Hz d = 1, Hx d = 0)
+        T sbz0I = (T)0, sby1I = (T)0, sbz1J = (T)0, sbx0J = (T)0;
+        if (I >= 1) {
+            T o0 = (T)0, o1 = (T)0;
+            if (PML && pxm) {
+                qxm = slab_x(g, a.P, a.W, I - 1, J, K);
+                o0 = a.ph_old[0][0][qxm];
+                o1 = a.ph_old[0][1][qxm];
+            }
+            sbz0I = adj_term(nch * zIm[k], PML && pxm, o1, prxm);
+            sby1I = adj_term(-(nch * yIm[k]), PML && pxm, o0, prxm);
+        }
+        if (J >= 1) {
+            T o0 = (T)0, o1 = (T)0;
+            if (PML && pym) {
+                qym = slab_y(g, a.P, a.W, I, J - 1, K);
+                o0 = a.ph_old[1][0][qym];
+                o1 = a.ph_old[1][1][qym];
+            }
+            sbz1J = adj_term(-(nch * zJm[k]), PML && pym, o1, prym);
+            sbx0J = adj_term(nch * xJm[k], PML && pym, o0, prym);
+        }
+        const bool vx = J >= 1 && K >= 1, vy = I >= 1 && K >= 1, vz = I >= 1 && J >= 1;
+        if (GRAD) {
+            // gradient sums from the adjoint e before its update
+            double sp = 0.0, sq = 0.0;
+            const double c = MAT == 2 ? (double)cav[k] : 1.0;
+            if (vx) {
+                const double l = (double)lx[k], f0 = (double)fx0[k];
+                sp = sp + l * f0;
+                sq = sq + l * ((double)fx1[k] - (MAT == 2 ? c * f0 : f0));
+            }
+            if (vy) {
+                const double l = (double)ly[k], f0 = (double)fy0[k];
+                sp = sp + l * f0;
+                sq = sq + l * ((double)fy1[k] - (MAT == 2 ? c * f0 : f0));
+            }
+            if (vz) {
+                const double l = (double)lz[k], f0 = (double)fz0[k];
+                sp = sp + l * f0;
+                sq = sq + l * ((double)fz1[k] - (MAT == 2 ? c * f0 : f0));
+            }
+            tp[k] = tp[k] + sp;
+            tq[k] = tq[k] + sq;
+        }
+        if (vx) lane_at(ex + b, ui) = (MAT == 2 ? cav[k] * lx[k] : lx[k]) + ((sby0m - sby0) + (sbz1J - sbz1));
+        if (vy) lane_at(ey + b, ui) = (MAT == 2 ? cav[k] * ly[k] : ly[k]) + ((sbx1m - sbx1) + (sbz0I - sbz0));
+        if (vz) lane_at(ez + b, ui) = (MAT == 2 ? cav[k] * lz[k] : lz[k]) + ((sbx0J - sbx0) + (sby1I - sby1));
+        // new psi_H adjoints of the cell's H components
+        if (PML) {
+            if (px) {
+                if (J >= 1) a.ph_new[0][0][qx] = prx.b * (ox0 + -nuy);
+                if (K >= 1) a.ph_new[0][1][qx] = prx.b * (ox1 + nuz);
+            }
+            if (py) {
+                if (I >= 1) a.ph_new[1][0][qy] = pry.b * (oy0 + nux);
+                if (K >= 1) a.ph_new[1][1][qy] = pry.b * (oy1 + -nuz);
+            }
+            if (pz) {
+                if (I >= 1) a.ph_new[2][0][qz] = prz.b * (oz0 + -nux);
+                if (J >= 1) a.ph_new[2][1][qz] = prz.b * (oz1 + nuy);
+            }
+        }
+        sby0m = sby0;
+        sbx1m = sbx1;
+    }
+}
+
+template <typename T, int MAT, bool PML>
+__global__ __launch_bounds__(256) void k_adj_h(GridView g, T ce, const AdjShot<T> *__restrict__ shots, int ns,
+                                              const T *__restrict__ cb, int P, int npw, int nbx, int nby, int ntiles)
+{
+    int bx, by, bz;
+    if (!split_tile(blockIdx.x, nbx, nby, ntiles, bx, by, bz)) return;
+    const int I = split_column(g, P, npw, bx, threadIdx.x);
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * ADJ_LZ, K1 = min(K0 + ADJ_LZ, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    for (int q = 0; q < ns; ++q) adj_h_column<T, MAT, PML>(g, ce, shots[q], cb, I, J, K0, K1);
+}
+
+// Reverse workgroup order, as k_e_split: it starts on the planes k_adj_h ended on.
+template <typename T, int MAT, bool PML, bool GRAD>
+__global__ __launch_bounds__(256) void k_adj_e(GridView g, T nch, const AdjShot<T> *__restrict__ shots, int ns,
+                                              const T *__restrict__ ca, double *__restrict__ gp,
+                                              double *__restrict__ gq, long nalloc, int slot, int P, int npw, int nbx,
+                                              int nby, int ntiles)
+{
+    int bx, by, bz;
+    if (!split_tile((int)(gridDim.x - 1u - blockIdx.x), nbx, nby, ntiles, bx, by, bz)) return;
+    const int I = split_column(g, P, npw, bx, threadIdx.x);
+    const int J = __builtin_amdgcn_readfirstlane(by * blockDim.y + threadIdx.y);
+    const int K0 = bz * ADJ_LZ, K1 = min(K0 + ADJ_LZ, g.nz + 1);
+    if (I < 0 || I > g.nx || J > g.ny) return;
+    double tp[ADJ_LZ], tq[ADJ_LZ];
+#pragma unroll
+    for (int k = 0; k < ADJ_LZ; ++k) tp[k] = tq[k] = 0.0;
+    for (int q = 0; q < ns; ++q) adj_e_column<T, MAT, PML, GRAD>(g, nch, shots[q], ca, nalloc, slot, I, J, K0, K1, tp, tq);
+    if (GRAD) {
+        const unsigned ud = (unsigned)(I + g.pitch * J) * sizeof(double);
+#pragma unroll
+        for (int k = 0; k < ADJ_LZ; ++k) {
+            if (K0 + k >= K1) break;
+            const long b = g.off + g.slab * (K0 + k);
+            lane_at(gp + b, ud) = lane_at(gp + b, ud) + tp[k];
+            lane_at(gq + b, ud) = lane_at(gq + b, ud) + tq[k];
         }
     }
 }
 
-template <typename T, bool PML>
-__global__ __launch_bounds__(256) void k_adj_b(GridView g, T nch, Adj3<T> eb, Adj3<T> hb, AdjPml<T> a)
+// Gradient from the sums: g_eps = dCa/deps_r P + (dCb/deps_r)/Cb Q, likewise
+// for sigma, on the material grid (P, Q on the unified grid).
+__global__ void k_adj_grad(GridView g, const double *__restrict__ gp, const double *__restrict__ gq, GradView gv)
 {
-    int I, J, K;
-    if (!adj_coords(g, I, J, K)) return;
-    adj_b_body<T, PML>(g, nch, eb, hb, a, I, J, K, g.off + I + g.pitch * J + g.slab * K);
-}
-
-template <typename T, bool PML>
-__global__ __launch_bounds__(256) void k_adj_b_group(GridView g, T nch, const AdjShot<T> *__restrict__ shots, int ns)
-{
-    int I, J, K;
-    if (!adj_coords(g, I, J, K)) return;
+    const long nm = (long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const int I = (int)(m % (g.nx + 1));
+    const long r = m / (g.nx + 1);
+    const int J = (int)(r % (g.ny + 1)), K = (int)(r / (g.ny + 1));
     const long u = g.off + I + g.pitch * J + g.slab * K;
-    for (int q = 0; q < ns; ++q) adj_b_body<T, PML>(g, nch, shots[q].eout, shots[q].hb, shots[q].a, I, J, K, u);
+    const double p = gp[u], q = gq[u];
+    gv.g_eps[m] = gv.dca_e[m] * p + gv.dcbn_e[m] * q;
+    gv.g_sig[m] = gv.dca_s[m] * p + gv.dcbn_s[m] * q;
 }
 
 // Residual injection at a recorded step: ebar[point] += w[point],

@@ -171,8 +171,24 @@ class FemSimulation:
         return t.value
 
     def set_checkpoint_interval(self, interval):
-        """Checkpoint interval of gradient() (0 = ceil(sqrt(num_steps)))."""
+        """One-level checkpoint interval of gradient() (0: chosen per call
+        within the memory cap; results do not depend on it)."""
         _lib.check(_lib.load().ems_fdtd_set_checkpoint_interval(self._h, int(interval)))
+
+    def set_gradient_memory(self, nbytes):
+        """Cap on the snapshot and checkpoint bytes of gradient() (0: 3/4 of
+        the free device memory); the schedule with the fewest recomputed
+        steps within it is taken (results do not depend on it)."""
+        _lib.check(_lib.load().ems_fdtd_set_gradient_memory(self._h, int(nbytes)))
+
+    def gradient_plan(self, num_steps):
+        """(level-1 interval, snapshot segment length, bytes) of the schedule
+        gradient(num_steps) would take now (level-1 interval = num_steps: one
+        level)."""
+        k1, ks, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _lib.check(_lib.load().ems_fdtd_gradient_plan(self._h, int(num_steps), ctypes.byref(k1), ctypes.byref(ks),
+                                                      ctypes.byref(b)))
+        return k1.value, ks.value, b.value
 
     def gradient(self, num_steps, d_obs):
         """Misfit J = 1/2 sum (trace - d_obs)^2 of a run of num_steps from a

@@ -189,15 +189,38 @@ int ems_fdtd_check_params(const double domain_sizes[3], const int grid_num[3], d
  *       components c with a non-empty point list of (trace - d_obs)^2
  * and dJ/deps_r, dJ/dsigma on the (nx+1)(ny+1)(nz+1) material grid (either
  * output may be NULL), accumulated in FP64.  d_obs is [nobs][num_steps/tf+1][3]
- * (slot 0 unused).  Forward states are recovered by checkpointing every
- * `interval` steps and recomputing one segment at a time.  Afterwards the
+ * (slot 0 unused).  Forward states are recovered from checkpoints by
+ * recomputing one segment at a time (see ems_fdtd_set_checkpoint_interval
+ * and ems_fdtd_set_gradient_memory).  Afterwards the
  * handle holds the final forward state, time num_steps*dt and the run's
  * traces (not accumulated onto earlier ones). */
 ems_status ems_fdtd_gradient(ems_fdtd *s, int num_steps, const double *d_obs, size_t count,
                              double *misfit, double *grad_eps_r, double *grad_sigma);
 
-/* Checkpoint interval of ems_fdtd_gradient (0 = ceil(sqrt(num_steps))). */
+/* Checkpoint interval of ems_fdtd_gradient and ems_batch_gradient: one
+ * level of checkpoints every `interval` steps, E snapshots of interval+1
+ * steps per shot, every segment but the last run twice.  0 (the default):
+ * chosen per call within the memory cap (ems_fdtd_set_gradient_memory).
+ * The interval does not change any result. */
 ems_status ems_fdtd_set_checkpoint_interval(ems_fdtd *s, int interval);
+
+/* Cap on the bytes of E snapshots and checkpoints that ems_fdtd_gradient
+ * (and ems_batch_gradient, with its first member's cap) may use (0, the
+ * default: 3/4 of the device memory that is free or already held by the
+ * gradient store of the GPU).  With the interval left at 0 the gradient takes
+ * the schedule with the fewest recomputed steps within the cap: one level of
+ * checkpoints, or two (every segment rerun once more) when one does not fit.
+ * A cap that no schedule fits makes the gradient fail (EMS_ERR_INVALID_ARG).
+ * The store is one per GPU, kept from call to call and shared by all
+ * handles (gradients on one GPU take turns on it).  Does not change any
+ * result. */
+ems_status ems_fdtd_set_gradient_memory(ems_fdtd *s, size_t bytes);
+
+/* The schedule ems_fdtd_gradient would take now for num_steps steps:
+ * *interval1 = level-1 checkpoint interval (num_steps: one level),
+ * *interval = snapshot segment length, *bytes = its snapshot and checkpoint
+ * bytes (outputs may be NULL). */
+ems_status ems_fdtd_gradient_plan(ems_fdtd *s, int num_steps, int *interval1, int *interval, size_t *bytes);
 
 /* ------------------------------------------------------- batched FDTD runs
  * No reference counterpart (the reference runs one shot per process).  A

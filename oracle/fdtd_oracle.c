@@ -814,11 +814,12 @@ static double sbar_h(ems_oracle_fdtd *s, adj_state *a, int comp, int d, int I, i
 }
 
 /* the transpose of one bare step (update_h then update_e) applied to the
- * adjoint state; `grad` (may be NULL): accumulate the material gradient from
- * E^n = en[3], E^{n+1} = e1[3] (reference layouts) */
-static void adjoint_step(ems_oracle_fdtd *s, adj_state *a, double *const *en, double *const *e1,
-                         const double *dca_e, const double *dcbn_e, const double *dca_s,
-                         const double *dcbn_s, double *geps, double *gsig)
+ * adjoint state; `gp`, `gq` (may be NULL): accumulate the gradient sums of
+ * every material cell from e' and E^n = en[3], E^{n+1} = e1[3] (reference
+ * layouts), in component order:
+ *     P += sum_c e'_c E^n_c,   Q += sum_c e'_c (E^{n+1}_c - Ca E^n_c) */
+static void adjoint_step(ems_oracle_fdtd *s, adj_state *a, double *const *en, double *const *e1, double *gp,
+                         double *gq)
 {
     const ems_oracle_fdtd_cfg *c = &s->c;
     const int nx = c->nx, ny = c->ny, nz = c->nz;
@@ -833,12 +834,12 @@ static void adjoint_step(ems_oracle_fdtd *s, adj_state *a, double *const *en, do
             for (int I = 0; I <= nx + 1; ++I)
                 for (int q = 0; q < 3; ++q)
                     for (int d = 0; d < 2; ++d) tb[q][d][UNI(I, J, K)] = tbar_e(s, a, q, d, I, J, K);
-    /* gradient (uses e' = the adjoint of E^{n+1}) */
-    if (geps && en && e1) {
+    /* gradient sums (use e' = the adjoint of E^{n+1}) */
+    if (gp && en && e1) {
         for (int K = 0; K <= nz; ++K)
             for (int J = 0; J <= ny; ++J)
                 for (int I = 0; I <= nx; ++I) {
-                    double tse = 0.0, tss = 0.0;
+                    double tp = 0.0, tq = 0.0;
                     size_t m = I + J * (nxs + 1) + (size_t)K * (nxs + 1) * (nys + 1);
                     for (int q = 0; q < 3; ++q) {
                         if (!e_in(c, q, I, J, K)) continue;
@@ -846,11 +847,11 @@ static void adjoint_step(ems_oracle_fdtd *s, adj_state *a, double *const *en, do
                         double lam = a->e[q][r];
                         double ca = ca_at(s, I, J, K);
                         double de = e1[q][r] - ca * en[q][r];
-                        tse = tse + lam * (dca_e[m] * en[q][r] + dcbn_e[m] * de);
-                        tss = tss + lam * (dca_s[m] * en[q][r] + dcbn_s[m] * de);
+                        tp = tp + lam * en[q][r];
+                        tq = tq + lam * de;
                     }
-                    geps[m] += tse;
-                    gsig[m] += tss;
+                    gp[m] += tp;
+                    gq[m] += tq;
                 }
     }
     for (int K = 0; K <= nz + 1; ++K)
@@ -951,7 +952,7 @@ int ems_oracle_fdtd_adjoint_bare_step(ems_oracle_fdtd *s)
 {
     adj_state *a = adj_of(s);
     if (!a) return -1;
-    adjoint_step(s, a, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
+    adjoint_step(s, a, NULL, NULL, NULL, NULL);
     return 0;
 }
 
@@ -1081,8 +1082,7 @@ int ems_oracle_fdtd_gradient(ems_oracle_fdtd *s, int nsteps, const double *eps_r
     double *dca_e = malloc(sizeof(double) * nm), *dcbn_e = malloc(sizeof(double) * nm);
     double *dca_s = malloc(sizeof(double) * nm), *dcbn_s = malloc(sizeof(double) * nm);
     ems_oracle_fdtd_material_derivs(c, nm, eps_r, sigma, dca_e, dcbn_e, dca_s, dcbn_s);
-    memset(geps, 0, sizeof(double) * nm);
-    memset(gsig, 0, sizeof(double) * nm);
+    double *gp = calloc(nm, sizeof(double)), *gq = calloc(nm, sizeof(double));
     /* backward */
     for (int n = nsteps - 1; n >= 0; --n) {
         const int step = n + 1;
@@ -1097,8 +1097,15 @@ int ems_oracle_fdtd_gradient(ems_oracle_fdtd *s, int nsteps, const double *eps_r
                     for (int q = 0; q < cnt; ++q) a->e[cc][s->obs_idx[off + q]] += w;
                 }
         }
-        adjoint_step(s, a, &hist[3 * n], &hist[3 * (n + 1)], dca_e, dcbn_e, dca_s, dcbn_s, geps, gsig);
+        adjoint_step(s, a, &hist[3 * n], &hist[3 * (n + 1)], gp, gq);
     }
+    /* dJ/deps_r = dCa/deps_r P + (dCb/deps_r)/Cb Q, likewise for sigma */
+    for (size_t m = 0; m < nm; ++m) {
+        geps[m] = dca_e[m] * gp[m] + dcbn_e[m] * gq[m];
+        gsig[m] = dca_s[m] * gp[m] + dcbn_s[m] * gq[m];
+    }
+    free(gp);
+    free(gq);
     for (int n = 0; n <= nsteps; ++n)
         for (int q = 0; q < 3; ++q) free(hist[3 * n + q]);
     free(hist);

@@ -112,6 +112,43 @@ def test_checkpoint_interval_does_not_change_results():
     assert s.current_time == p.current_time
 
 
+def test_memory_caps_do_not_change_results():
+    """Memory caps that leave one-level schedules of several intervals and
+    two-level ones (every segment rerun twice): each plan within its cap, and
+    misfit, gradients and final state bit for bit those of the uncapped run.
+    A cap that nothing fits is an error."""
+    case = CASES["cpml-lossy"]
+    er, sg = _material(case)
+    dobs = _observed(case, er, sg)
+    N = case["steps"]
+    s = _gpu(case, er, sg)
+    k1, ks, full = s.gradient_plan(N)
+    assert (k1, ks) == (N, N)  # a small grid: every step's snapshot fits
+    ref = s.gradient(N, dobs)
+    kinds = set()
+    for frac in (0.6, 0.45, 0.4, 0.35, 0.3):
+        c = _gpu(case, er, sg)
+        cap = int(full * frac)
+        c.set_gradient_memory(cap)
+        try:
+            k1, ks, b = c.gradient_plan(N)
+        except Exception:
+            continue  # nothing fits this cap
+        print("cap %.2f: level-1 interval %d, segment %d, %d bytes" % (frac, k1, ks, b), flush=True)
+        assert b <= cap and ks < N
+        kinds.add(k1 < N)
+        out = c.gradient(N, dobs)
+        assert out[0] == ref[0]
+        assert np.array_equal(out[1], ref[1]) and np.array_equal(out[2], ref[2])
+        for w in range(6):
+            assert np.array_equal(c.field(w), s.field(w))
+    assert kinds == {False, True}, kinds
+    c = _gpu(case, er, sg)
+    c.set_gradient_memory(1)
+    with pytest.raises(Exception):
+        c.gradient(N, dobs)
+
+
 def test_fp32_gradient_against_finite_differences():
     case = CASES["cpml-lossy"]
     er, sg = _material(case)
