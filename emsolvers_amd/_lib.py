@@ -85,6 +85,7 @@ SIGNATURES = {
     "ems_batch_set_graph": (_I, [_P, _I]),
     "ems_batch_run": (_I, [_P, _I]),
     "ems_batch_run_timed": (_I, [_P, _I, _DP]),
+    "ems_batch_set_gradient_group": (_I, [_P, _I]),
     "ems_batch_gradient": (_I, [_P, _I, ctypes.POINTER(_DP), _DP, _DP, _DP, _DP]),
     # structured-hex FEM
     "ems_fem_load_kmat": (_I, [ctypes.c_char_p, _DP, _SZ, _IP]),

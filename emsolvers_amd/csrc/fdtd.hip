@@ -829,12 +829,58 @@ ems_status copy_state(ems_fdtd *s, void *dst, bool save)
     return EMS_OK;
 }
 
-ems_status snap_e(ems_fdtd *s, int slot)
+// The current E of the shots `mem` (one grid) into snapshot slot `slot`,
+// one launch per kShotArgs shots.
+template <typename T> ems_status snap_shots(const std::vector<ems_fdtd *> &mem, int slot, hipStream_t st)
 {
-    const size_t fb = s->nalloc * s->esz;
-    char *q = static_cast<char *>(s->adj->segp) + (size_t)slot * 3 * fb;
-    for (int c = 0; c < 3; ++c)
-        EMS_HIP(hipMemcpyAsync(q + c * fb, s->f[c].p, fb, hipMemcpyDeviceToDevice, s->stream));
+    const long nalloc = (long)mem[0]->nalloc;
+    const long nv = nalloc * (long)sizeof(T) / 16;
+    const unsigned gx = (unsigned)std::min<long>((nv + 255) / 256, 512);
+    for (size_t q0 = 0; q0 < mem.size(); q0 += kShotArgs) {
+        const int cnt = (int)std::min<size_t>(kShotArgs, mem.size() - q0);
+        SnapArgs<T> a{};
+        for (int q = 0; q < cnt; ++q) {
+            const ems_fdtd *s = mem[q0 + q];
+            for (int c = 0; c < 3; ++c) a.e[q][c] = s->f[c].as<T>();
+            a.seg[q] = static_cast<T *>(s->adj->segp);
+        }
+        k_snap<T><<<dim3(gx, 3u * cnt), 256, 0, st>>>(a, nalloc, slot);
+    }
+    EMS_HIP(hipGetLastError());
+    return EMS_OK;
+}
+
+// Residual injection arguments of the shots `mem`, kShotArgs per launch
+// (residual weights w and point lists in place).
+template <typename T>
+std::vector<InjectArgs<T>> inject_args(const std::vector<ems_fdtd *> &mem, const std::vector<int> &npts,
+                                       const std::vector<int> &serial)
+{
+    std::vector<InjectArgs<T>> out;
+    for (size_t q0 = 0; q0 < mem.size(); q0 += kShotArgs) {
+        const int cnt = (int)std::min<size_t>(kShotArgs, mem.size() - q0);
+        InjectArgs<T> a{};
+        for (int q = 0; q < cnt; ++q) {
+            const ems_fdtd *s = mem[q0 + q];
+            for (int c = 0; c < 3; ++c) a.e[q][c] = s->adj->e[c].as<T>();
+            a.pt_comp[q] = s->d_pt_comp.as<int>();
+            a.pt_u[q] = s->d_pt_u.as<long>();
+            a.w[q] = s->adj->w.as<double>();
+            a.npts[q] = npts[q0 + q];
+            a.serial[q] = serial[q0 + q];
+        }
+        out.push_back(a);
+    }
+    return out;
+}
+
+template <typename T> ems_status inject_shots(const std::vector<InjectArgs<T>> &args, int nshots, int row, hipStream_t st)
+{
+    for (size_t i = 0; i < args.size(); ++i) {
+        const int cnt = (int)std::min<size_t>(kShotArgs, (size_t)nshots - i * kShotArgs);
+        k_adj_inject_shots<T><<<cnt, 256, 0, st>>>(args[i], row);
+    }
+    EMS_HIP(hipGetLastError());
     return EMS_OK;
 }
 
@@ -1291,21 +1337,19 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
     double *gp = nullptr, *gq = nullptr;
     const AdjGeom ag = adj_geom(g, s->npml);
     const T nch = -ch;
-    const Adj3<T> e_cur{{A->e[0].as<T>(), A->e[1].as<T>(), A->e[2].as<T>()}};
+    const std::vector<ems_fdtd *> one{s};
+    std::vector<InjectArgs<T>> inj;
     auto ops = grad_ops(
         [&](int n, bool record, int slot) -> ems_status {
             EMS_TRY(fstep(n, record));
-            if (slot >= 0) EMS_TRY(snap_e(s, slot));
+            if (slot >= 0) EMS_TRY(snap_shots<T>(one, slot, st));
             return EMS_OK;
         },
-        [&](int slot) { return snap_e(s, slot); },
+        [&](int slot) { return snap_shots<T>(one, slot, st); },
         [&](int idx) { return copy_state(s, ckb + (size_t)idx * sb, true); },
         [&](int idx) { return copy_state(s, ckb + (size_t)idx * sb, false); },
         [&](int n, int slot) -> ems_status {
-            if (npts && (n + 1) % tf == 0)
-                k_adj_inject<T><<<serial ? 1 : (npts + 255) / 256, serial ? 64 : 256, 0, st>>>(
-                    e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
-                    A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial);
+            if (npts && (n + 1) % tf == 0) EMS_TRY(inject_shots<T>(inj, 1, (n + 1) / tf - 1, st));
             adj_sweeps<T>(s->mat_mode, s->npml > 0, g, ag, st, ce, nch, A->tab.as<AdjShot<T>>() + cur, 1,
                           s->ca.as<T>(), s->cb.as<T>(), gp, gq, (long)s->nalloc, slot);
             EMS_HIP(hipGetLastError());
@@ -1360,6 +1404,7 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
         serial = std::adjacent_find(keys.begin(), keys.end()) != keys.end() ? 1 : 0;
     }
     EMS_HIP(hipMemcpy(A->w.p, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    inj = inject_args<T>(one, std::vector<int>{npts}, std::vector<int>{serial});
     if (A->fac_version != s->mat_version) {
         const std::vector<double> fac = material_factors(s);
         EMS_TRY(A->fac.reserve(sizeof(double) * fac.size()));
@@ -1400,6 +1445,7 @@ struct ems_batch {
     hipStream_t stream = nullptr;
     int by = 8, lz = 2;      // workgroup rows, planes per thread
     int graph_steps = 0;     // steps per captured graph (0 = plain launches)
+    int grad_group = 0;      // shots per gradient group (0: automatic)
     DevBuf d_mem, d_post, d_step;
     DevBuf d_post2, d_shots[2], fac, grad, sums;  // gradient: recompute posts, shot tables, shared FP64 arrays
     const ems_fdtd *fac_owner = nullptr;     // factors cached for this member's material version
@@ -1713,12 +1759,13 @@ bool same_model(const ems_fdtd *a, const ems_fdtd *b)
            a->pml_kmax == b->pml_kmax && a->pml_amax == b->pml_amax && a->eps_r == b->eps_r && a->sigma == b->sigma;
 }
 
-// Gradient of the summed misfit of all members (shots on one model): batched
-// forward and recompute sweeps, each shot's own checkpoints, and group
-// adjoint passes that update the shared FP64 gradient once per point.
+// Misfits of all members of b (shots on one model) and their gradient sums,
+// added to P = gp, Q = gq (unified grid, FP64): batched forward and
+// recompute sweeps, each shot's own checkpoints, and adjoint passes over all
+// shots that add the shots' sums to P and Q once per point and step.
 template <typename T>
-ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, double *misfits, double *Jtot,
-                               double *g_eps, double *g_sig)
+ems_status batch_gradient_group(ems_batch *b, int N, const double *const *d_obs, double *misfits, double &Jsum,
+                                double *gp_out, double *gq_out)
 {
     const int ns = (int)b->mem.size();
     ems_fdtd *s0 = b->mem[0];
@@ -1792,7 +1839,6 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
     // gradient buffers outside the store first
     const size_t sb = state_bytes(s0);
-    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
     for (int q = 0; q < ns; ++q) {
         ems_fdtd *s = b->mem[q];
         if (!s->adj) s->adj = new Adj();
@@ -1800,9 +1846,6 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
         EMS_TRY(s->adj->w.reserve(sizeof(double) * (size_t)std::max(rp[q].nrec, 1) * std::max(rp[q].npts, 1)));
         EMS_TRY(adj_zero(s, st));
     }
-    EMS_TRY(b->grad.reserve(sizeof(double) * 2 * nm));
-    EMS_TRY(b->sums.reserve(sizeof(double) * 2 * s0->nalloc));
-    EMS_TRY(b->fac.reserve(sizeof(double) * 4 * nm));
     GradStore *gs = nullptr;
     EMS_TRY(grad_store(&gs));
     std::lock_guard<std::mutex> store_lock(gs->mu);
@@ -1816,10 +1859,8 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     const AdjGeom ag = adj_geom(g, s0->npml);
     const T ce = (T)(s0->d.dt / s0->d.eps / s0->d.h);
     const T nch = -(T)(s0->d.dt / s0->d.mu / s0->d.h);
-    auto snap_all = [&](int slot) -> ems_status {
-        for (auto *s : b->mem) EMS_TRY(snap_e(s, slot));
-        return EMS_OK;
-    };
+    auto snap_all = [&](int slot) { return snap_shots<T>(b->mem, slot, st); };
+    std::vector<InjectArgs<T>> inj;
     auto state_all = [&](int idx, bool save) -> ems_status {
         for (int q = 0; q < ns; ++q) EMS_TRY(copy_state(b->mem[q], ckb + ((size_t)idx * ns + q) * sb, save));
         return EMS_OK;
@@ -1834,17 +1875,7 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
         },
         snap_all, [&](int idx) { return state_all(idx, true); }, [&](int idx) { return state_all(idx, false); },
         [&](int n, int slot) -> ems_status {
-            if ((n + 1) % tf == 0)
-                for (int m = 0; m < ns; ++m) {
-                    ems_fdtd *s = b->mem[m];
-                    const int npts = rp[m].npts;
-                    if (!npts) continue;
-                    Adj *A = s->adj;
-                    const Adj3<T> e_cur{{A->e[0].as<T>(), A->e[1].as<T>(), A->e[2].as<T>()}};
-                    k_adj_inject<T><<<serial[m] ? 1 : (npts + 255) / 256, serial[m] ? 64 : 256, 0, st>>>(
-                        e_cur, npts, s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(),
-                        A->w.as<double>() + (size_t)((n + 1) / tf - 1) * npts, serial[m]);
-                }
+            if ((n + 1) % tf == 0) EMS_TRY(inject_shots<T>(inj, ns, (n + 1) / tf - 1, st));
             adj_sweeps<T>(md.mat, md.pml, g, ag, st, ce, nch, b->d_shots[par].as<AdjShot<T>>(), ns, s0->ca.as<T>(),
                           s0->cb.as<T>(), gp, gq, (long)s0->nalloc, slot);
             EMS_HIP(hipGetLastError());
@@ -1857,7 +1888,7 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
     EMS_HIP(hipStreamSynchronize(st));
     EMS_HIP(hipGetLastError());
     // traces, misfits and residual weights
-    double Jsum = 0.0;
+    Jsum = 0.0;
     for (int q = 0; q < ns; ++q) {
         ems_fdtd *s = b->mem[q];
         std::vector<double> raw((size_t)rp[q].nrec * rp[q].npts), tr, w;
@@ -1872,15 +1903,11 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
         s->nslot = N / tf + 1;
         s->t_now = rp[q].t_end;
     }
-    if (Jtot) *Jtot = Jsum;
-    if (b->fac_owner != s0 || b->fac_version != s0->mat_version) {
-        const std::vector<double> fac = material_factors(s0);
-        EMS_TRY(b->fac.reserve(sizeof(double) * fac.size()));
-        EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
-        b->fac_owner = s0;
-        b->fac_version = s0->mat_version;
+    {
+        std::vector<int> npts(ns);
+        for (int q = 0; q < ns; ++q) npts[q] = rp[q].npts;
+        inj = inject_args<T>(b->mem, npts, serial);
     }
-    EMS_TRY(zero(b->sums, sizeof(double) * 2 * s0->nalloc, st));
     // the two parity tables of the shots' adjoint state
     std::vector<AdjShot<T>> tab[2];
     for (auto *s : b->mem)
@@ -1889,14 +1916,85 @@ ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, 
         EMS_TRY(b->d_shots[pp].reserve(sizeof(AdjShot<T>) * ns));
         EMS_HIP(hipMemcpy(b->d_shots[pp].p, tab[pp].data(), sizeof(AdjShot<T>) * ns, hipMemcpyHostToDevice));
     }
+    gp = gp_out;
+    gq = gq_out;
+    EMS_TRY(sched.backward());
+    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin.p, false));
+    EMS_HIP(hipStreamSynchronize(st));
+    return EMS_OK;
+}
+
+// Shots per gradient group: the set size, else the most shots whose E
+// snapshots of all N+1 steps take at most half of the device's memory (no
+// step is then recomputed), at least 1.  Depends on the device and the
+// workload only, so that the summation order (group by group) does too.
+int gradient_group(const ems_batch *b, int N)
+{
+    const int ns = (int)b->mem.size();
+    if (b->grad_group > 0) return std::min(b->grad_group, ns);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        (void)hipGetLastError();
+        return ns;
+    }
+    const ems_fdtd *s = b->mem[0];
+    const double per_shot = (N + 1.0) * 3.0 * (double)s->nalloc * (double)s->esz;
+    return (int)std::max(1.0, std::min((double)ns, std::floor(0.5 * (double)tot / per_shot)));
+}
+
+// Gradient of the summed misfit of all members: the members in groups of
+// gradient_group shots (as even as possible, in member order), each group's
+// sums added to one P, Q, then the eps_r / sigma factors once.
+template <typename T>
+ems_status batch_gradient_impl(ems_batch *b, int N, const double *const *d_obs, double *misfits, double *Jtot,
+                               double *g_eps, double *g_sig)
+{
+    const int ns = (int)b->mem.size();
+    ems_fdtd *s0 = b->mem[0];
+    for (auto *s : b->mem)
+        if (!same_model(s0, s)) return invalid("batch gradient: members must share grid, material, CPML, dt and tf");
+    const GridView &g = s0->g;
+    const size_t nm = (size_t)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+    hipStream_t st = b->stream;
+    EMS_TRY(b->grad.reserve(sizeof(double) * 2 * nm));
+    EMS_TRY(b->fac.reserve(sizeof(double) * 4 * nm));
+    EMS_TRY(zero(b->sums, sizeof(double) * 2 * s0->nalloc, st));
+    double *gp = b->sums.as<double>(), *gq = gp + s0->nalloc;
+    const int gsz = gradient_group(b, N);
+    const int ngroups = (ns + gsz - 1) / gsz;
+    std::vector<double> mis(ns, 0.0);
+    double Jg = 0.0;
+    if (ngroups == 1) {
+        EMS_TRY(batch_gradient_group<T>(b, N, d_obs, mis.data(), Jg, gp, gq));
+    } else {
+        for (int k = 0; k < ngroups; ++k) {
+            const int q0 = (int)((long)k * ns / ngroups), q1 = (int)((long)(k + 1) * ns / ngroups);
+            ems_batch sub;
+            sub.mem.assign(b->mem.begin() + q0, b->mem.begin() + q1);
+            sub.stream = st;
+            sub.by = b->by;
+            sub.lz = b->lz;
+            struct Lend {  // the parent's stream, not the group's to destroy
+                ems_batch &b;
+                ~Lend() { b.stream = nullptr; }
+            } lend{sub};
+            EMS_TRY(batch_gradient_group<T>(&sub, N, d_obs + q0, mis.data() + q0, Jg, gp, gq));
+        }
+    }
+    double Jsum = 0.0;  // member order, whatever the groups
+    for (int q = 0; q < ns; ++q) Jsum += mis[q];
+    if (misfits) std::copy(mis.begin(), mis.end(), misfits);
+    if (Jtot) *Jtot = Jsum;
+    if (b->fac_owner != s0 || b->fac_version != s0->mat_version) {
+        const std::vector<double> fac = material_factors(s0);
+        EMS_HIP(hipMemcpy(b->fac.p, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice));
+        b->fac_owner = s0;
+        b->fac_version = s0->mat_version;
+    }
     GradView gv{b->fac.as<double>(), b->fac.as<double>() + nm, b->fac.as<double>() + 2 * nm,
                 b->fac.as<double>() + 3 * nm, b->grad.as<double>(), b->grad.as<double>() + nm};
-    gp = b->sums.as<double>();
-    gq = gp + s0->nalloc;
-    EMS_TRY(sched.backward());
     k_adj_grad<<<(unsigned)((nm + 255) / 256), 256, 0, st>>>(g, gp, gq, gv);
     EMS_HIP(hipGetLastError());
-    for (auto *s : b->mem) EMS_TRY(copy_state(s, s->adj->fin.p, false));
     EMS_HIP(hipStreamSynchronize(st));
     // straight into the caller's arrays (no staging copy)
     if (g_eps) EMS_HIP(hipMemcpy(g_eps, b->grad.p, sizeof(double) * nm, hipMemcpyDeviceToHost));
@@ -2551,6 +2649,15 @@ ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms)
     if (b->mem.empty()) return invalid("empty batch");
     return b->mem[0]->esz == 4 ? batch_run_impl<float>(b, num_steps, total_ms)
                                : batch_run_impl<double>(b, num_steps, total_ms);
+}
+
+ems_status ems_batch_set_gradient_group(ems_batch *b, int shots)
+{
+    clear_error();
+    if (!b) return invalid("null batch");
+    if (shots < 0) return invalid("gradient group must be >= 0");
+    b->grad_group = shots;
+    return EMS_OK;
 }
 
 ems_status ems_batch_gradient(ems_batch *b, int num_steps, const double *const *d_obs, double *misfits,

@@ -1715,25 +1715,57 @@ __global__ void k_adj_grad(GridView g, const double *__restrict__ gp, const doub
     gv.g_sig[m] = gv.dca_s[m] * p + gv.dcbn_s[m] * q;
 }
 
-// Residual injection at a recorded step: ebar[point] += w[point],
-// w = (trace - d_obs) / count.  serial != 0: one thread in the misfit's
-// order (points that repeat); else one thread per (distinct) point.
+// E of up to kShotArgs shots into their snapshot slots: block row 3q + c
+// copies component c of shot q (nalloc elements, a multiple of 32) to
+// seg_q + (3 slot + c) nalloc, 16 bytes per thread and iteration.  The
+// pointers travel as kernel arguments (the forward fields of some sweeps
+// change buffers from step to step).
+constexpr int kShotArgs = 64;
+
+template <typename T> struct SnapArgs {
+    const T *e[kShotArgs][3];
+    T *seg[kShotArgs];
+};
+
 template <typename T>
-__global__ void k_adj_inject(Adj3<T> eb, int npts, const int *__restrict__ pt_comp, const long *__restrict__ pt_u,
-                             const double *__restrict__ w, int serial)
+__global__ __launch_bounds__(256) void k_snap(SnapArgs<T> a, long nalloc, int slot)
 {
-    if (serial) {
-        if (threadIdx.x != 0 || blockIdx.x != 0) return;
-        for (int p = 0; p < npts; ++p) {
-            T *q = eb.c[pt_comp[p]] + pt_u[p];
-            *q = *q + (T)w[p];
+    const int q = blockIdx.y / 3, c = blockIdx.y % 3;
+    const float4 *src = reinterpret_cast<const float4 *>(a.e[q][c]);
+    float4 *dst = reinterpret_cast<float4 *>(a.seg[q] + ((long)slot * 3 + c) * nalloc);
+    const long nv = nalloc * (long)sizeof(T) / 16;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// Residual injection of up to kShotArgs shots at recorded row `row`: block q
+// adds w_q[row][p] to the adjoint E of shot q at its points (serial: one
+// thread, in the misfit's order, for point lists that repeat a point).
+template <typename T> struct InjectArgs {
+    T *e[kShotArgs][3];
+    const int *pt_comp[kShotArgs];
+    const long *pt_u[kShotArgs];
+    const double *w[kShotArgs];
+    int npts[kShotArgs], serial[kShotArgs];
+};
+
+template <typename T> __global__ __launch_bounds__(256) void k_adj_inject_shots(InjectArgs<T> a, int row)
+{
+    const int q = blockIdx.x, np = a.npts[q];
+    if (np == 0) return;
+    const double *w = a.w[q] + (long)row * np;
+    if (a.serial[q]) {
+        if (threadIdx.x != 0) return;
+        for (int p = 0; p < np; ++p) {
+            T *x = a.e[q][a.pt_comp[q][p]] + a.pt_u[q][p];
+            *x = *x + (T)w[p];
         }
         return;
     }
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    T *q = eb.c[pt_comp[p]] + pt_u[p];
-    *q = *q + (T)w[p];
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+        T *x = a.e[q][a.pt_comp[q][p]] + a.pt_u[q][p];
+        *x = *x + (T)w[p];
+    }
 }
 
 }  // namespace fdtd

@@ -249,6 +249,12 @@ class Batch:
         _lib.check(_lib.load().ems_batch_run_timed(self._h, int(num_steps), ctypes.byref(t)))
         return t.value
 
+    def set_gradient_group(self, shots):
+        """Shots per group of gradient() (0: the most whose snapshots of every
+        step fit in half of the device memory); the summation order, and so
+        the gradient's last bits, depend on it."""
+        _lib.check(_lib.load().ems_batch_set_gradient_group(self._h, int(shots)))
+
     def gradient(self, num_steps, d_obs):
         """Shots on one model: (sum of misfits, per-member misfits, dJ/deps_r,
         dJ/dsigma of the sum); d_obs[q] as for FemSimulation.gradient."""

@@ -256,11 +256,20 @@ ems_status ems_batch_run_timed(ems_batch *b, int num_steps, double *total_ms);
  * material, CPML, dt and tf): as ems_fdtd_gradient for every member, with
  * d_obs[q] the observed traces of member q; returns each member's misfit
  * (misfits may be NULL), their sum, and the gradient of the sum with respect
- * to eps_r and sigma.  The forward and recompute sweeps are batched and the
- * adjoint passes of all shots run in one launch that updates the FP64
- * gradient once per cell. */
+ * to eps_r and sigma.  The members run in groups (ems_batch_set_gradient_group),
+ * one after the other; in a group the forward and recompute sweeps of all
+ * shots are batched, and each adjoint pass runs over all of them in one
+ * launch that adds their FP64 gradient sums once per cell.  The summation
+ * order, and so the last bits of the gradient, depend on the group size. */
 ems_status ems_batch_gradient(ems_batch *b, int num_steps, const double *const *d_obs, double *misfits,
                               double *misfit_total, double *grad_eps_r, double *grad_sigma);
+
+/* Shots per group of ems_batch_gradient (member order, groups as even as
+ * possible).  0, the default: the most shots whose E snapshots of every step
+ * fit in half of the device's memory (so that no step is recomputed), at
+ * least one; a function of the device and the workload, not of the memory
+ * free at the call. */
+ems_status ems_batch_set_gradient_group(ems_batch *b, int shots);
 
 /* --------------------------------------------------------- structured-hex FEM
  * The fem_v2 / symplectic_2 / symplectic_4 family: order-p edge elements on a

@@ -1,7 +1,8 @@
 """Batched FDTD runs (many shots and models of uneven size in one launch per
 sweep, optional HIP-graph replay, autotuned tiling): every member must end in
 exactly the state of the same handle run on its own (FP64 and FP32, bit for
-bit: each member's cells see the same operations in the same order)."""
+bit: each member's cells see the same operations in the same order), and in
+FP64 exactly the state of the oracle run of the same shot."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,25 @@ def build_all(precision):
     return [member(g, sh, precision, seed=i, **kw) for i, (sh, kw, g) in enumerate(SHOTS)]
 
 
+def oracle_member(grid, shot, mat=None, cpml=0, lossy=False, seed=0):
+    """The oracle run of member(grid, shot, ...) (same material draw)."""
+    from oracle.fdtd import OracleFDTD, WAVELET_RICKER, WAVELET_GAUSS
+    h = 1e-3
+    o = OracleFDTD(grid, h, 0.95 * h / (C0 * np.sqrt(3)), EPS0, MU0, shot.get("tf", 1),
+                   wavelet=WAVELET_RICKER if shot["w"] == 0 else WAVELET_GAUSS, f0=shot["f0"], npml=cpml,
+                   pml_order=3.0, pml_r0=1e-6, pml_kappa_max=1.5 if cpml else 1.0,
+                   pml_alpha_max=0.02 if cpml else 0.0)
+    nx, ny, nz = grid
+    if mat:
+        rng = np.random.default_rng(seed)
+        nm = (nx + 1) * (ny + 1) * (nz + 1)
+        o.set_material(1 + rng.uniform(0, 4, nm), rng.uniform(0, 1, nm) if lossy else None)
+    [o.setSource_x, o.setSource_y, o.setSource_z][shot["comp"]](shot["pos"])
+    for r in shot["rec"]:
+        o.setObservationPoint([[r], [r], [r]])
+    return o
+
+
 @pytest.mark.parametrize("precision,graph", [("f64", 0), ("f64", 8), ("f32", 5)])
 def test_batch_members_equal_single_runs(precision, graph):
     from emsolvers_amd.fdtd import Batch
@@ -60,6 +80,14 @@ def test_batch_members_equal_single_runs(precision, graph):
         for w in range(6):
             assert np.array_equal(s.field(w), r.field(w)), (q, w)
         assert s.current_time == r.current_time
+    if precision == "f64":  # the oracle, the same two runs of 37 and 23 steps
+        for q, (sh, kw, g) in enumerate(SHOTS):
+            o = oracle_member(g, sh, seed=q, **kw)
+            o.run(37)
+            o.run(23)
+            assert np.array_equal(mem[q].traces(), o.traces()), q
+            for w in range(6):
+                assert np.array_equal(mem[q].field(w), o.field(w)), (q, w)
 
 
 def test_autotune_keeps_results():

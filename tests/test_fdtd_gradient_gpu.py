@@ -192,10 +192,13 @@ def test_torch_autograd_binding():
                                     atol=1e-9, rtol=1e-4, fast_mode=True)
 
 
-def test_batched_gradient_equals_sum_of_single_shots():
-    """Several shots on one model in one batch: per-shot misfits and traces
-    bit-exact with single runs; the summed gradient equal to the sum of the
-    single-shot gradients up to the summation order (1e-12)."""
+@pytest.mark.parametrize("group", [0, 1, 2])
+def test_batched_gradient_equals_sum_of_single_shots(group):
+    """Several shots on one model in one batch (one group of 3, or groups of
+    1 or 2 shots): per-shot misfits and traces bit-exact with single runs and
+    with the oracle; the summed gradient equal to the sum of the single-shot
+    gradients up to the summation order (1e-12), and to the sum of the
+    oracle's single-shot gradients likewise."""
     from emsolvers_amd.fdtd import Batch
     case = CASES["cpml-lossy"]
     er, sg = _material(case)
@@ -217,6 +220,7 @@ def test_batched_gradient_equals_sum_of_single_shots():
     singles = [shot(k, "gpu").gradient(case["steps"], obs[k]) for k in range(3)]
     mem = [shot(k, "gpu") for k in range(3)]
     b = Batch(mem)
+    b.set_gradient_group(group)
     J, mis, ge, gs = b.gradient(case["steps"], obs)
     for k in range(3):
         assert mis[k] == singles[k][0]
@@ -225,6 +229,14 @@ def test_batched_gradient_equals_sum_of_single_shots():
     ss = singles[0][2] + singles[1][2] + singles[2][2]
     assert np.abs(ge - se).max() <= 1e-12 * np.abs(se).max()
     assert np.abs(gs - ss).max() <= 1e-12 * np.abs(ss).max()
+    # the oracle's single-shot misfits and gradients
+    oe, osg = 0.0, 0.0
+    for k in range(3):
+        Jo, geo, gso = shot(k, "oracle").gradient(case["steps"], obs[k])
+        assert mis[k] == Jo
+        oe, osg = oe + geo, osg + gso
+    assert np.abs(ge - oe).max() <= 1e-12 * np.abs(oe).max()
+    assert np.abs(gs - osg).max() <= 1e-12 * np.abs(osg).max()
     ref = shot(1, "gpu")
     ref.run(case["steps"])
     np.testing.assert_array_equal(mem[1].traces(), ref.traces())
