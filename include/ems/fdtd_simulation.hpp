@@ -37,6 +37,7 @@ public:
         ems_fdtd_desc d{grid_size[0], grid_size[1], grid_size[2], domain_size, time_step, permittivity,
                         permeability, time_frequency, precision};
         check(ems_fdtd_create(&d, &h_));
+        grid_ = grid_size;
     }
     ~FemSimulation() { ems_fdtd_destroy(h_); }
     FemSimulation(const FemSimulation &) = delete;
@@ -99,10 +100,99 @@ public:
         check(ems_fdtd_get_field(h_, which, v.data(), n));
         return v;
     }
+    void setField(int which, const std::vector<double> &values)
+    {
+        check(ems_fdtd_set_field(h_, which, values.data(), values.size()));
+    }
+    double currentTime() const
+    {
+        double t = 0.0;
+        check(ems_fdtd_get_time(h_, &t));
+        return t;
+    }
+
+    // Misfit J = 1/2 sum (trace - d_obs)^2 of a run of num_steps from rest
+    // and dJ/deps_r, dJ/dsigma on the (nx+1)(ny+1)(nz+1) material grid
+    // (ems_fdtd_gradient).  d_obs: [observation][slot][3] flattened, slots
+    // 0..num_steps/tf.  Either gradient pointer may be null.
+    double gradient(int num_steps, const std::vector<double> &d_obs, std::vector<double> *grad_eps_r,
+                    std::vector<double> *grad_sigma)
+    {
+        const size_t nm = materialSize();
+        if (grad_eps_r) grad_eps_r->assign(nm, 0.0);
+        if (grad_sigma) grad_sigma->assign(nm, 0.0);
+        double J = 0.0;
+        check(ems_fdtd_gradient(h_, num_steps, d_obs.data(), d_obs.size(), &J,
+                                grad_eps_r ? grad_eps_r->data() : nullptr, grad_sigma ? grad_sigma->data() : nullptr));
+        return J;
+    }
+    void setCheckpointInterval(int interval) { check(ems_fdtd_set_checkpoint_interval(h_, interval)); }
+    void setGradientMemory(size_t bytes) { check(ems_fdtd_set_gradient_memory(h_, bytes)); }
+    // {level-1 interval, segment length, bytes} of the schedule gradient(num_steps) would take now
+    std::array<size_t, 3> gradientPlan(int num_steps)
+    {
+        int k1 = 0, ks = 0;
+        size_t b = 0;
+        check(ems_fdtd_gradient_plan(h_, num_steps, &k1, &ks, &b));
+        return {(size_t)k1, (size_t)ks, b};
+    }
+    // points of the material grid, (nx+1)(ny+1)(nz+1)
+    size_t materialSize() const { return (size_t)(grid_[0] + 1) * (grid_[1] + 1) * (grid_[2] + 1); }
     ems_fdtd *handle() const { return h_; }
 
 private:
+    std::array<int, 3> grid_{};
     ems_fdtd *h_ = nullptr;
+};
+
+// Many handles advanced together (ems_batch_*): one launch per sweep for all
+// members, optional HIP-graph replay, batched gradients of shots on one
+// model.  Members stay owned by the caller and must outlive the batch.
+class Batch {
+public:
+    Batch() { check(ems_batch_create(&b_)); }
+    ~Batch() { ems_batch_destroy(b_); }
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    void add(FemSimulation &sim)
+    {
+        check(ems_batch_add(b_, sim.handle()));
+        mem_.push_back(&sim);
+    }
+    void setTiling(int rows, int planes) { check(ems_batch_set_tiling(b_, rows, planes)); }
+    std::array<int, 2> autotune(int trial_steps = 5)
+    {
+        int r = 0, p = 0;
+        check(ems_batch_autotune(b_, trial_steps, &r, &p));
+        return {r, p};
+    }
+    void setGraph(int steps_per_graph) { check(ems_batch_set_graph(b_, steps_per_graph)); }
+    void setGradientGroup(int shots) { check(ems_batch_set_gradient_group(b_, shots)); }
+    void run(int num_steps) { check(ems_batch_run(b_, num_steps)); }
+
+    // Sum of the members' misfits (each in *misfits when given) and the
+    // gradient of the sum; d_obs[q] as for FemSimulation::gradient.
+    double gradient(int num_steps, const std::vector<std::vector<double>> &d_obs, std::vector<double> *misfits,
+                    std::vector<double> *grad_eps_r, std::vector<double> *grad_sigma)
+    {
+        if (d_obs.size() != mem_.size()) throw std::runtime_error("libems: one d_obs per batch member");
+        std::vector<const double *> ptrs;
+        for (auto &d : d_obs) ptrs.push_back(d.data());
+        const size_t nm = mem_.empty() ? 0 : mem_[0]->materialSize();
+        if (misfits) misfits->assign(mem_.size(), 0.0);
+        if (grad_eps_r) grad_eps_r->assign(nm, 0.0);
+        if (grad_sigma) grad_sigma->assign(nm, 0.0);
+        double J = 0.0;
+        check(ems_batch_gradient(b_, num_steps, ptrs.data(), misfits ? misfits->data() : nullptr, &J,
+                                 grad_eps_r ? grad_eps_r->data() : nullptr,
+                                 grad_sigma ? grad_sigma->data() : nullptr));
+        return J;
+    }
+
+private:
+    ems_batch *b_ = nullptr;
+    std::vector<FemSimulation *> mem_;
 };
 
 // check_params (src/fdtd/fdtd_simulation.cpp:333-366)
