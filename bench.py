@@ -51,9 +51,10 @@ def gpu_fdtd(n, steps, warmup, precision, kernel, npml=10, seed=11):
                 cells=n ** 3, device_bytes=sim.device_bytes())
 
 
-def gpu_fem(order, stepper, n, steps, precision="f64"):
+def gpu_fem(order, stepper, n, steps, precision="f64", elem="auto"):
     """Structured-hex FEM (kmat operators) on an n^3 fine grid; returns
-    DOF-updates/s of the step kernels (HIP events)."""
+    DOF-updates/s of the step kernels (HIP events).  elem: the element
+    products of orders 2 and 3 ('auto', 'rows' or 'mfma')."""
     from emsolvers_amd.fem import FemSimulation
     from emsolvers_amd.workloads import EPS0, MU0, C0
     kmat = os.path.join(ROOT, "tests", "golden", "kmat")
@@ -61,6 +62,7 @@ def gpu_fem(order, stepper, n, steps, precision="f64"):
     dt = h / (C0 * 3 ** 0.5) / 2.0
     sim = FemSimulation(order, (n, n, n), h, dt, EPS0, MU0, 1, 1, stepper=stepper, precision=precision,
                         kmat_dir=kmat)
+    sim.set_element_product(elem)
     sim.setSource_z((n // 2, n // 2, n // 2))
     sim.setObservationPoint((n // 2 + order, n // 2, n // 2))
     sim.run(2)
@@ -68,7 +70,7 @@ def gpu_fem(order, stepper, n, steps, precision="f64"):
     ndof = sim.ndof()
     return dict(value=round(ndof * steps / (kernel_ms / 1e3) / 1e6, 1), unit="MDOF-updates/s",
                 dofs=ndof, ms_per_step=round(kernel_ms / steps, 4), order=order, stepper=stepper,
-                grid=[n] * 3, dtype="fp64" if precision == "f64" else "fp32")
+                grid=[n] * 3, dtype="fp64" if precision == "f64" else "fp32", element_product=elem)
 
 
 def gpu_fem1(stepper, n, steps, precision="f64"):
@@ -516,6 +518,8 @@ def main():
                     help="also time the 64-shot batched gradient against 64 sequential gradients")
     ap.add_argument("--fem-profile", action="store_true",
                     help="run only short FEM legs (tet configs 2 and 3, fem_v2 and fem at 128^3), for rocprofv3")
+    ap.add_argument("--fem-elem-profile", action="store_true",
+                    help="run only the hex FEM order 2 and 3 legs with both element products, for rocprofv3")
     ap.add_argument("--fp32-errors", action="store_true",
                     help="FP32 against FP64 traces and gradients per benchmark configuration, then exit")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -538,6 +542,11 @@ def main():
                 gpu_tet(64, "symplectic_4", 2, "f32", order=2, sigma_fraction=0.2, power_iters=10),
                 gpu_fem(1, "fem_v2", 128, 20), gpu_fem(2, "symplectic_4", 128, 10), gpu_fem1("fem", 128, 20),
                 gpu_fem1("fem_abm", 128, 10)]
+        print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
+        return
+    if args.fem_elem_profile:
+        legs = [gpu_fem(o, st, n, k, p, e) for (o, st, n, k) in ((3, "fem_v2", 96, 20), (2, "symplectic_4", 128, 10))
+                for p in ("f64", "f32") for e in ("rows", "mfma")]
         print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
         return
     if args.copy_calibrate:
@@ -597,7 +606,9 @@ def main():
     }
     if not args.no_fem:
         out["fem"] = [gpu_fem(2, "symplectic_4", 128, 20), gpu_fem(2, "symplectic_4", 128, 20, "f32"),
-                      gpu_fem(3, "fem_v2", 96, 20), gpu_fem(1, "fem_v2", 128, 50),
+                      gpu_fem(3, "fem_v2", 96, 20), gpu_fem(3, "fem_v2", 96, 20, "f64", "rows"),
+                      gpu_fem(3, "fem_v2", 96, 20, "f32"), gpu_fem(3, "symplectic_4", 96, 10),
+                      gpu_fem(1, "fem_v2", 128, 50),
                       gpu_fem(1, "symplectic_2", 128, 50, "f32"), gpu_fem1("fem", 128, 50),
                       gpu_fem1("fem_abm", 128, 30)]
         note("hex fem")

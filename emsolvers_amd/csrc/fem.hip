@@ -24,6 +24,8 @@ struct ems_fem {
     size_t esz = 8;
     std::vector<double> kscaled;  // n x n, FP64
     DevBuf K, off;
+    DevBuf Kp;          // orders 2, 3: K in the MFMA A-operand layout (k_fem_elem_mfma)
+    int elem_kind = EMS_FEM_ELEM_AUTO;  // orders 2, 3: element products (per element thread or matrix cores)
     // leapfrog: slot[s][comp]; symplectic: U = slot 0, P = slot 1, and the
     // ping-pong partners slot 2 (U), slot 3 (P)
     DevBuf f[5][3];  // slot 4: drifted field of a symplectic stage (order >= 2)
@@ -100,6 +102,21 @@ void launch_z1(ems_fem *s, CComp3<T> u, CComp3<T> pm, CComp3<T> other, Comp3<T> 
                                                                    hd, mm, ns, s->d_src.as<long>(), sval);
 }
 
+// Row dots of every element (orders 2, 3) into etmp: per-element threads
+// (the oracle's sums) or matrix cores.
+template <typename T, int P> void launch_elem(ems_fem *s, CComp3<T> x)
+{
+    const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
+    T *et = s->etmp.as<T>();
+    const bool mfma = s->elem_kind == EMS_FEM_ELEM_MFMA || (s->elem_kind == EMS_FEM_ELEM_AUTO && P == 3);
+    if (mfma)
+        k_fem_elem_mfma<T, P><<<(unsigned)((nelem + 63) / 64), 256, 0, s->stream>>>(s->g, s->Kp.as<T>(),
+                                                                                   s->off.as<int>(), x, et);
+    else
+        k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(
+            s->g, s->K.as<T>(), s->off.as<int>(), x, et);
+}
+
 template <typename T, int P>
 void launch_leapfrog(ems_fem *s, const double *sval)
 {
@@ -121,10 +138,8 @@ void launch_leapfrog(ems_fem *s, const double *sval)
         return;
     }
     if constexpr (P >= 2) {  // element by element: row dots once per element, then the gathers
-        const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
         T *et = s->etmp.as<T>();
-        k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
-                                                                                  s->off.as<int>(), e1, et);
+        launch_elem<T, P>(s, e1);
         k_fem_leapfrog<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(s->g, e0, e1, e2, ns, s->d_src.as<long>(), sval,
                                                                          et);
         k_fem_leapfrog<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(s->g, e0, e1, e2, ns, s->d_src.as<long>(), sval,
@@ -164,10 +179,8 @@ void launch_stage(ems_fem *s, double deltat, const double *sval)
         k_fem_drift<T><<<dim3((unsigned)((nmax + 255) / 256), 3, 1), 256, 0, s->stream>>>(
             s->g, u, pm, c3<T>(s, 4), (T)(0.5 * deltat), (T)mm);
         const CComp3<T> ud = cc3<T>(s, 4);
-        const long nelem = (long)s->g.nex * s->g.ney * s->g.nez;
         T *et = s->etmp.as<T>();
-        k_fem_elem<T, P><<<(unsigned)((nelem + kElemWG<P> - 1) / kElemWG<P>), kElemWG<P>, 0, s->stream>>>(s->g, s->K.as<T>(),
-                                                                                  s->off.as<int>(), ud, et);
+        launch_elem<T, P>(s, ud);
         k_fem_symplectic<T, P, 0><<<comp_grid<0>(s), blk, 0, s->stream>>>(
             s->g, u, pm, uo, po, (T)deltat, (T)(0.5 * deltat), (T)mm, ns, s->d_src.as<long>(), sval, et);
         k_fem_symplectic<T, P, 1><<<comp_grid<1>(s), blk, 0, s->stream>>>(
@@ -324,8 +337,22 @@ template <typename T> ems_status upload_kmat(ems_fem *s)
     std::vector<T> k(s->kscaled.begin(), s->kscaled.end());
     EMS_TRY(s->K.reserve(sizeof(T) * k.size()));
     EMS_HIP(hipMemcpy(s->K.p, k.data(), sizeof(T) * k.size(), hipMemcpyHostToDevice));
+    if (s->g.p >= 2) {  // the A-operand layout of k_fem_elem_mfma
+        const int n = (int)std::lround(std::sqrt((double)k.size()));
+        const int mt = (n + 15) / 16, ks = (n + 3) / 4;
+        std::vector<T> kp((size_t)mt * ks * 64, (T)0);
+        for (int q = 0; q < mt; ++q)
+            for (int c = 0; c < ks; ++c)
+                for (int l = 0; l < 64; ++l) {
+                    const int r = 16 * q + (l & 15), m = 4 * c + (l >> 4);
+                    if (r < n && m < n) kp[((size_t)q * ks + c) * 64 + l] = k[(size_t)r * n + m];
+                }
+        EMS_TRY(s->Kp.reserve(sizeof(T) * kp.size()));
+        EMS_HIP(hipMemcpy(s->Kp.p, kp.data(), sizeof(T) * kp.size(), hipMemcpyHostToDevice));
+    }
     return EMS_OK;
 }
+
 
 int logical_slot(const ems_fem *s, int logical)
 {
@@ -637,6 +664,16 @@ ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes)
     if (planes < 1) return invalid("tile planes must be >= 1");
     s->z1_by = rows;
     s->z1_lz = planes;
+    return EMS_OK;
+}
+
+ems_status ems_fem_set_element_product(ems_fem *s, int kind)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (kind != EMS_FEM_ELEM_AUTO && kind != EMS_FEM_ELEM_ROWS && kind != EMS_FEM_ELEM_MFMA)
+        return invalid("unknown element product");
+    s->elem_kind = kind;
     return EMS_OK;
 }
 

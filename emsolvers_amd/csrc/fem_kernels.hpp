@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mfma.hpp"
+
 namespace ems {
 namespace fem {
 
@@ -220,6 +222,58 @@ __global__ __launch_bounds__(kElemWG<P>) void k_fem_elem(FemGrid g, const T *__r
         for (int m = 0; m < N; ++m) t += kr[m] * v[m];
         tmp[r * nelem + e] = t;
     }
+}
+
+// Orders 2 and 3 on matrix cores: the row dots of k_fem_elem (tmp[r nelem +
+// e]) as the product K X of the n x n element matrix and the n x 16 block of
+// 16 elements' local vectors, one block per wavefront, in 16 x 16 x 4 MFMA
+// steps (row tiles q of 16 rows, k steps of 4 columns).  The sums run in the
+// instructions' order with fused multiply-adds, not in k_fem_elem's order, so
+// the row dots agree with it to rounding, not bit for bit.  Kp holds K in
+// the A-operand layout, Kp[(q KS + ks) 64 + lane] = K[16 q + lane % 16][4 ks +
+// lane / 16] (zero outside n x n); workgroups of 4 wavefronts.
+template <int P> struct ElemTiles {
+    static constexpr int D = P * (P + 1) * (P + 1), N = 3 * D, MT = (N + 15) / 16, KS = (N + 3) / 4;
+};
+
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_fem_elem_mfma(FemGrid g, const T *__restrict__ Kp,
+                                                       const int *__restrict__ off, CComp3<T> x, T *__restrict__ tmp)
+{
+    constexpr int D = ElemTiles<P>::D, N = ElemTiles<P>::N, MT = ElemTiles<P>::MT, KS = ElemTiles<P>::KS;
+    typedef typename MfmaAcc<T>::V V;
+    const long nelem = (long)g.nex * g.ney * g.nez;
+    const int lane = threadIdx.x & 63;
+    const long base = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;  // first element of this wavefront
+    if (base >= nelem) return;  // uniform per wavefront; no barrier below
+    const int n = lane & 15, kq = lane >> 4;
+    const long e = base + n;
+    const bool valid = e < nelem;
+    long bx = 0, by = 0, bz = 0;
+    if (valid) {
+        const int i = (int)(e % g.nex), j = (int)((e / g.nex) % g.ney), k = (int)(e / ((long)g.nex * g.ney));
+        elem_bases(g, i, j, k, bx, by, bz);
+    }
+    V acc[MT];
+#pragma unroll
+    for (int q = 0; q < MT; ++q) acc[q] = V{0, 0, 0, 0};
+#pragma unroll 4
+    for (int ks = 0; ks < KS; ++ks) {
+        const int m = 4 * ks + kq;
+        T bv = 0;
+        if (valid && m < N) bv = m < D ? x.x[bx + off[m]] : m < 2 * D ? x.y[by + off[m]] : x.z[bz + off[m]];
+        const T *kp = Kp + (long)ks * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < MT; ++q) acc[q] = MfmaAcc<T>::step(kp[(long)q * KS * 64], bv, acc[q]);
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int q = 0; q < MT; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * q + MfmaAcc<T>::row(lane, v);
+            if (r < N) tmp[r * nelem + e] = acc[q][v];
+        }
 }
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,7 @@
 
 #include "common.hpp"
 #include "lane_addr.hpp"
+#include "mfma.hpp"
 #include "tet_basis.hpp"
 #include "tet_mesh.hpp"
 
@@ -286,8 +287,6 @@ __global__ __launch_bounds__(256) void k_ebe_local(EbeMesh m, const T *__restric
 // the 16x16x4 builtins: A[i][k] in lane (k 16 + i), B[k][j] in lane (k 16 + j);
 // result element v of lane l is D[4 (l / 16) + v][l % 16] (f32) or
 // D[l / 16 + 4 v][l % 16] (f64).
-typedef float ems_f4 __attribute__((ext_vector_type(4)));
-typedef double ems_d4 __attribute__((ext_vector_type(4)));
 
 // Mass coefficient (T)(e0 eps_r + hh sigma) of every tetrahedron, type-major
 // (the expression of k_ebe_local, so the same bits).
@@ -301,24 +300,6 @@ __global__ void k_mass_coef(long ncube, const double *__restrict__ er, const dou
     const double epsr = er ? er[tid] : 1.0, sig = sg ? sg[tid] : 0.0;
     out[q] = (T)(e0 * epsr + hh * sig);
 }
-
-template <typename T> struct MfmaAcc;
-template <> struct MfmaAcc<float> {
-    typedef ems_f4 V;
-    static __device__ __forceinline__ V step(float a, float b, V c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-template <> struct MfmaAcc<double> {
-    typedef ems_d4 V;
-    static __device__ __forceinline__ V step(double a, double b, V c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
 
 template <typename T, int NL>
 __global__ __launch_bounds__(256) void k_ebe_mass_mfma(EbeMesh m, const T *__restrict__ Mt,

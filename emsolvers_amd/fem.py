@@ -132,6 +132,13 @@ class FemSimulation:
         gather kernel per component.  Results do not depend on it."""
         _lib.check(_lib.load().ems_fem_set_tiling(self._h, int(rows), int(planes)))
 
+    def set_element_product(self, kind):
+        """Orders 2 and 3: 'rows' (one thread per element, the reference's
+        sums), 'mfma' (matrix cores; equal to rounding) or 'auto' (the
+        default: mfma at order 3, rows at order 2)."""
+        k = {"auto": 0, "rows": 1, "mfma": 2}[kind] if isinstance(kind, str) else int(kind)
+        _lib.check(_lib.load().ems_fem_set_element_product(self._h, k))
+
     def ndof(self):
         n = ctypes.c_size_t()
         _lib.check(_lib.load().ems_fem_dof_count(self._h, ctypes.byref(n)))

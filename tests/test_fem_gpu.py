@@ -104,6 +104,8 @@ def test_fp64_bitwise_with_oracle(oracle_cache, key):
     print("case order", order, "stepper", stepper, flush=True)
     o = oracle_cache(key)
     g = build("gpu", order, stepper, case, "f64")
+    if order >= 2:
+        g.set_element_product("rows")  # the reference's sums (the default at order 3 is mfma)
     g.run(case["steps"])
     to, tg = o.traces(), g.traces()
     assert to.shape == tg.shape
@@ -113,6 +115,35 @@ def test_fp64_bitwise_with_oracle(oracle_cache, key):
         for slot in (0, 1):
             assert np.array_equal(g.get(c, slot), o.get(c, slot)), (c, slot)
     assert g.current_time == o.current_time
+
+
+@pytest.mark.parametrize("key", [k for k in CASES if k[0] >= 2], ids=lambda k: "p%d-s%d" % k)
+def test_mfma_element_products_against_oracle(oracle_cache, key):
+    """Orders 2 and 3 with the element products on matrix cores: FP64 traces
+    and final fields within 1e-10 relative L2 of the oracle (the MFMA sums
+    run in another order than the reference's), FP32 traces within 1e-5 of
+    the FP64 oracle and within 1e-5 of the FP32 per-element-thread run."""
+    order, stepper = key
+    case = CASES[key]
+    print("case order", order, "stepper", stepper, flush=True)
+    o = oracle_cache(key)
+    g = build("gpu", order, stepper, case, "f64")
+    g.set_element_product("mfma")
+    g.run(case["steps"])
+    err = rel_l2(g.traces()[..., 1:], o.traces()[..., 1:])
+    ferr = max(rel_l2(g.get(c, slot), o.get(c, slot)) for c in range(3) for slot in (0, 1))
+    f32 = build("gpu", order, stepper, case, "f32")
+    f32.set_element_product("mfma")
+    f32.run(case["steps"])
+    r32 = build("gpu", order, stepper, case, "f32")
+    r32.set_element_product("rows")
+    r32.run(case["steps"])
+    e32 = rel_l2(f32.traces()[..., 1:], o.traces()[..., 1:])
+    e32r = rel_l2(f32.traces()[..., 1:], r32.traces()[..., 1:])
+    print("mfma p%d s%d: fp64 traces %.2e fields %.2e; fp32 vs oracle %.2e vs rows %.2e" % (order, stepper, err, ferr,
+                                                                                         e32, e32r), flush=True)
+    assert err <= 1e-10 and ferr <= 1e-10
+    assert e32 <= 1e-5 and e32r <= 1e-5
 
 
 @pytest.mark.parametrize("key", list(CASES), ids=lambda k: "p%d-s%d" % k)
