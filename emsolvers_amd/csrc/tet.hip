@@ -1006,6 +1006,7 @@ struct ems_tet {
     hipStream_t stream = nullptr;
     // misfit gradient (allocated on first use)
     DevBuf g_t2d, g_mloc, g_bnd, g_lU, g_lV, g_mu, g_y, g_vout, g_tmp, g_ge, g_gs, g_res, g_f;
+    DevBuf g_ck, g_st, g_fwd_f;  // checkpointed gradient: (U, V) checkpoints, stage states, forward load factors
 };
 
 namespace {
@@ -2225,37 +2226,48 @@ template <typename T> ems_status run_impl(ems_tet *s, int N, double *total_ms)
 // accumulating per tetrahedron eps0 mu' M_T (V - V') (eps_r) and
 // -D/2 mu' M_T (V + V') (sigma, at sigma = 0).  `drift` = the recovered
 // initial state's largest entry relative to the final state's.
+// Tables of the gradient kernels (once per handle): tet -> dofs, element
+// masses, boundary mask.
+template <typename T> ems_status gradient_tables(ems_tet *s)
+{
+    if (s->g_t2d.p) return EMS_OK;
+    const long nt = s->m.nt;
+    const int nl = s->nl;
+    std::vector<int> t2d((size_t)nt * nl);
+    long tid = 0, dd[20];
+    for (int k = 0; k < s->m.nz; ++k)
+        for (int j = 0; j < s->m.ny; ++j)
+            for (int i = 0; i < s->m.nx; ++i)
+                for (int t = 0; t < 6; ++t, ++tid) {
+                    tet_dofs(s->m, s->d.order, i, j, k, t, dd);
+                    for (int l = 0; l < nl; ++l) t2d[tid * nl + l] = (int)dd[l];
+                }
+    std::vector<double> mloc(6 * nl * nl), sl(nl * nl);
+    for (int t = 0; t < 6; ++t) element_matrices(s->d.order, t, s->m.h, &mloc[t * nl * nl], sl.data());
+    std::vector<unsigned char> bnd(s->bnd.begin(), s->bnd.end());
+    EMS_TRY(s->g_t2d.reserve(sizeof(int) * t2d.size()));
+    EMS_TRY(s->g_mloc.reserve(sizeof(double) * mloc.size()));
+    EMS_TRY(s->g_bnd.reserve(bnd.size()));
+    EMS_HIP(hipMemcpy(s->g_t2d.p, t2d.data(), sizeof(int) * t2d.size(), hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->g_mloc.p, mloc.data(), sizeof(double) * mloc.size(), hipMemcpyHostToDevice));
+    EMS_HIP(hipMemcpy(s->g_bnd.p, bnd.data(), bnd.size(), hipMemcpyHostToDevice));
+    return EMS_OK;
+}
+
+template <typename T>
+ems_status gradient_ckpt(ems_tet *s, int N, const double *d_obs, double *J, double *ge, double *gs, double *drift);
+
 template <typename T>
 ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, double *ge, double *gs, double *drift)
 {
     if (s->d.stepper == EMS_TET_ABM) return not_supported("the tetrahedral gradient needs a symplectic stepper");
     for (double v : s->sigma)
-        if (v != 0.0) return not_supported("the tetrahedral gradient runs the integrator backwards: lossless models only");
+        if (v != 0.0) return gradient_ckpt<T>(s, N, d_obs, J, ge, gs, drift);  // lossy: no time reversal
     EMS_TRY(setup<T>(s));
     const long nd = s->nd, nt = s->m.nt;
     const int nl = s->nl, nobs = (int)s->obs_dof.size() / nl, tf = s->d.tf;
     const int n = (int)nd, g = (n + 255) / 256;
-    // tables: tet -> dofs, element masses, boundary mask
-    if (!s->g_t2d.p) {
-        std::vector<int> t2d((size_t)nt * nl);
-        long tid = 0, dd[20];
-        for (int k = 0; k < s->m.nz; ++k)
-            for (int j = 0; j < s->m.ny; ++j)
-                for (int i = 0; i < s->m.nx; ++i)
-                    for (int t = 0; t < 6; ++t, ++tid) {
-                        tet_dofs(s->m, s->d.order, i, j, k, t, dd);
-                        for (int l = 0; l < nl; ++l) t2d[tid * nl + l] = (int)dd[l];
-                    }
-        std::vector<double> mloc(6 * nl * nl), sl(nl * nl);
-        for (int t = 0; t < 6; ++t) element_matrices(s->d.order, t, s->m.h, &mloc[t * nl * nl], sl.data());
-        std::vector<unsigned char> bnd(s->bnd.begin(), s->bnd.end());
-        EMS_TRY(s->g_t2d.reserve(sizeof(int) * t2d.size()));
-        EMS_TRY(s->g_mloc.reserve(sizeof(double) * mloc.size()));
-        EMS_TRY(s->g_bnd.reserve(bnd.size()));
-        EMS_HIP(hipMemcpy(s->g_t2d.p, t2d.data(), sizeof(int) * t2d.size(), hipMemcpyHostToDevice));
-        EMS_HIP(hipMemcpy(s->g_mloc.p, mloc.data(), sizeof(double) * mloc.size(), hipMemcpyHostToDevice));
-        EMS_HIP(hipMemcpy(s->g_bnd.p, bnd.data(), bnd.size(), hipMemcpyHostToDevice));
-    }
+    EMS_TRY(gradient_tables<T>(s));
     const size_t vb = sizeof(T) * nd;
     for (DevBuf *v : {&s->g_lU, &s->g_lV, &s->g_mu, &s->g_y, &s->g_vout, &s->g_tmp}) {
         EMS_TRY(v->reserve(vb));
@@ -2365,6 +2377,187 @@ ems_status gradient_impl(ems_tet *s, int N, const double *d_obs, double *J, doub
     EMS_HIP(hipStreamSynchronize(s->stream));
     const double u0 = inf_norm(s->U), v0 = inf_norm(s->V);
     if (drift) *drift = std::max(u_end > 0 ? u0 / u_end : u0, v_end > 0 ? v0 / v_end : v0);
+    if (ge) EMS_HIP(hipMemcpy(ge, s->g_ge.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    if (gs) EMS_HIP(hipMemcpy(gs, s->g_gs.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    // the handle returns to rest at t = 0 and keeps the run's traces
+    EMS_HIP(hipMemset(s->U.p, 0, vb));
+    EMS_HIP(hipMemset(s->V.p, 0, vb));
+    s->t_now = 0.0;
+    s->steps = 0;
+    s->have_prev = 0;
+    s->comp_valid = false;
+    return EMS_OK;
+}
+
+// y = (eps0 eps_r + hh sigma) M x, PEC rows identity (the stage matrices
+// A_+- = M_eps +- D/2 M_sig for hh = +-D/2).
+template <typename T> ems_status apply_mass(ems_tet *s, double hh, const T *x, T *y, T *work)
+{
+    const int n = (int)s->nd;
+    if (s->ebe) {
+        ebe_local<T, EBE_MASS>(s, hh, x, nullptr, nullptr);
+        k_apply<EbeOp<T>, T><<<(n + 255) / 256, 256, 0, s->stream>>>(ebe_op<T>(s, 1), x, y);
+    } else {
+        const Csr<T> A{n, s->d_rp.as<int>(), s->d_col.as<int>()};
+        with_lpr(s->lpr, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            k_spmv<LPR, T><<<row_blocks(n, LPR), 256, 0, s->stream>>>(A, s->d_Me.as<T>(), x, y);
+            k_spmv<LPR, T><<<row_blocks(n, LPR), 256, 0, s->stream>>>(A, s->d_Ms.as<T>(), x, work);
+        });
+        k_axpy<T><<<(n + 255) / 256, 256, 0, s->stream>>>(n, (T)hh, work, y);
+    }
+    EMS_HIP(hipGetLastError());
+    return EMS_OK;
+}
+
+// Misfit gradient with checkpoints (symplectic steppers; the path for lossy
+// models, whose integrator cannot run backwards).  The forward run keeps
+// (U, V) every K = ceil(sqrt(N)) steps.  Backwards over the segments, each is
+// rerun from its checkpoint keeping every stage's input (U, V) and output V,
+// then the discrete adjoint of the stages runs over it:
+//   mu = A_+^-1 (lV' + D/2 lU');  lU = lU' - D S mu;  lV = A_- mu + D/2 lU,
+//   A_+- = M_eps +- D/2 M_sig,
+// accumulating eps0 mu' M_T (V_in - V_out) (eps_r) and -D/2 mu' M_T (V_in +
+// V_out) (sigma) per tetrahedron.  *drift = 0 (nothing is reconstructed).
+template <typename T>
+ems_status gradient_ckpt(ems_tet *s, int N, const double *d_obs, double *J, double *ge, double *gs, double *drift)
+{
+    EMS_TRY(setup<T>(s));
+    EMS_TRY(gradient_tables<T>(s));
+    const long nd = s->nd, nt = s->m.nt;
+    const int nl = s->nl, nobs = (int)s->obs_dof.size() / nl, tf = s->d.tf;
+    const int n = (int)nd, g = (n + 255) / 256;
+    const size_t vb = sizeof(T) * nd;
+    for (DevBuf *v : {&s->g_lU, &s->g_lV, &s->g_mu, &s->g_y, &s->g_vout, &s->g_tmp}) {
+        EMS_TRY(v->reserve(vb));
+        EMS_HIP(hipMemset(v->p, 0, vb));
+    }
+    EMS_TRY(s->g_ge.reserve(sizeof(double) * nt));
+    EMS_TRY(s->g_gs.reserve(sizeof(double) * nt));
+    EMS_HIP(hipMemset(s->g_ge.p, 0, sizeof(double) * nt));
+    EMS_HIP(hipMemset(s->g_gs.p, 0, sizeof(double) * nt));
+    // stage schedule: step D, offset, stage-matrix index
+    const double dt = s->d.dt;
+    std::vector<double> Dq, Oq;
+    std::vector<int> Wq;
+    if (s->d.stepper == EMS_TET_SYMPLECTIC4) {
+        const double x1 = 1.0 / (2.0 - std::pow(2.0, 1.0 / 3.0)), x2 = 1.0 - 2.0 * x1;
+        Dq = {x1 * dt, x2 * dt, x1 * dt};
+        Oq = {0.0, x1 * dt, (x1 + x2) * dt};
+        Wq = {0, 1, 0};
+    } else {
+        Dq = {dt};
+        Oq = {0.0};
+        Wq = {0};
+    }
+    const int ns = (int)Dq.size(), ndip = (int)s->dip_par.size() / 3;
+    const int K = std::max(1, (int)std::ceil(std::sqrt((double)N)));
+    const int nseg = (N + K - 1) / K;
+    EMS_TRY(s->g_ck.reserve(2 * vb * (size_t)nseg));
+    EMS_TRY(s->g_st.reserve(3 * vb * (size_t)K * ns));
+    char *ck = static_cast<char *>(s->g_ck.p), *stq = static_cast<char *>(s->g_st.p);
+    auto cpy = [&](void *dst, const void *src) { return hipMemcpyAsync(dst, src, vb, hipMemcpyDeviceToDevice, s->stream); };
+    // forward from rest, a checkpoint at every segment start (plain updates
+    // throughout: the reruns then repeat the forward's arithmetic)
+    EMS_HIP(hipMemset(s->U.p, 0, vb));
+    EMS_HIP(hipMemset(s->V.p, 0, vb));
+    s->t_now = 0.0;
+    s->steps = 0;
+    s->nslot = 0;
+    s->traces.assign(0, 0.0);
+    s->have_prev = 0;
+    s->comp_valid = false;
+    s->comp_off = true;
+    struct PlainUpdates {  // restored on every exit
+        ems_tet *s;
+        ~PlainUpdates() { s->comp_off = false; }
+    } plain{s};
+    ems_status fwd = EMS_OK;
+    for (int q = 0; q < nseg && fwd == EMS_OK; ++q) {
+        EMS_HIP(cpy(ck + 2 * vb * q, s->U.p));
+        EMS_HIP(cpy(ck + 2 * vb * q + vb, s->V.p));
+        fwd = run_impl<T>(s, std::min(K, N - q * K), nullptr);
+    }
+    EMS_TRY(fwd);
+    const int nslot = N / tf + 1;
+    std::vector<double> res((size_t)nslot * nobs * 3, 0.0);  // [slot][obs][3]
+    double jj = 0.0;
+    for (int o = 0; o < nobs; ++o)
+        for (int q = 1; q < nslot; ++q)
+            for (int c = 0; c < 3; ++c) {
+                const size_t k = ((size_t)o * nslot + q) * 3 + c;
+                const double r = s->traces[k] - d_obs[k];
+                res[((size_t)q * nobs + o) * 3 + c] = r;
+                jj += 0.5 * r * r;
+            }
+    *J = jj;
+    EMS_TRY(s->g_res.reserve(sizeof(double) * std::max<size_t>(res.size(), 1)));
+    EMS_HIP(hipMemcpy(s->g_res.p, res.data(), sizeof(double) * res.size(), hipMemcpyHostToDevice));
+    // load factors of every forward stage (the run's times and signs)
+    std::vector<double> f;
+    {
+        double t = 0.0;
+        for (int st = 0; st < N; ++st) {
+            for (int q = 0; q < ns; ++q) dipole_factors(s, t + Oq[q] + 0.5 * Dq[q], -1.0, f);
+            t += dt;
+        }
+    }
+    if (f.empty()) f.push_back(0.0);
+    EMS_TRY(s->g_fwd_f.reserve(sizeof(double) * f.size()));
+    EMS_HIP(hipMemcpy(s->g_fwd_f.p, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+    const double *fw = s->g_fwd_f.as<double>();
+    T *lU = s->g_lU.as<T>(), *lV = s->g_lV.as<T>(), *tmp = s->g_tmp.as<T>(), *work = s->g_y.as<T>();
+    const int tb = (int)((nt + 127) / 128);
+    ems_status st_err = EMS_OK;
+    for (int q = nseg - 1; q >= 0 && st_err == EMS_OK; --q) {
+        const int a = q * K, b = std::min(N, a + K);
+        // rerun the segment, keeping each stage's (U_in, V_in, V_out)
+        EMS_HIP(cpy(s->U.p, ck + 2 * vb * q));
+        EMS_HIP(cpy(s->V.p, ck + 2 * vb * q + vb));
+        s->have_prev = 0;
+        for (int st = a; st < b && st_err == EMS_OK; ++st)
+            for (int k = 0; k < ns && st_err == EMS_OK; ++k) {
+                char *slot = stq + 3 * vb * ((size_t)(st - a) * ns + k);
+                EMS_HIP(cpy(slot, s->U.p));
+                EMS_HIP(cpy(slot + vb, s->V.p));
+                const double tm = (double)st + (Oq[k] + 0.5 * Dq[k]) / dt;
+                st_err = stage<T>(s, Wq[k], Dq[k], tm, fw + ((size_t)st * ns + k) * (size_t)std::max(ndip, 1));
+                if (st_err == EMS_OK) EMS_HIP(cpy(slot + 2 * vb, s->V.p));
+            }
+        // adjoint steps b .. a+1
+        for (int st = b; st > a && st_err == EMS_OK; --st) {
+            if (nobs && st % tf == 0)
+                k_tet_inject<T><<<1, 64, 0, s->stream>>>(nobs, nl, s->d_obs_dof.as<int>(), s->d_obs_w.as<double>(),
+                                                         s->g_res.as<double>() + (size_t)(st / tf) * nobs * 3, lU);
+            for (int k = ns - 1; k >= 0 && st_err == EMS_OK; --k) {
+                const double D = Dq[k];
+                const char *slot = stq + 3 * vb * ((size_t)(st - 1 - a) * ns + k);
+                const T *Vin = reinterpret_cast<const T *>(slot + vb), *Vout = reinterpret_cast<const T *>(slot + 2 * vb);
+                k_comb<T><<<g, 256, 0, s->stream>>>(n, lV, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr,
+                                                    s->b.as<T>());
+                st_err = solve<T>(s, Wq[k], s->g_mu);
+                if (st_err != EMS_OK) break;
+                if (nl == 6)
+                    k_tet_grad<T, 6><<<tb, 128, 0, s->stream>>>(nt, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
+                                                                s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), Vin,
+                                                                Vout, s->d.eps, -0.5 * D, s->g_ge.as<double>(),
+                                                                s->g_gs.as<double>());
+                else
+                    k_tet_grad<T, 20><<<tb, 128, 0, s->stream>>>(nt, s->g_t2d.as<int>(), s->g_mloc.as<double>(),
+                                                                 s->g_bnd.as<unsigned char>(), s->g_mu.as<T>(), Vin,
+                                                                 Vout, s->d.eps, -0.5 * D, s->g_ge.as<double>(),
+                                                                 s->g_gs.as<double>());
+                EMS_TRY(apply_op<T>(s, 0, s->g_mu.as<T>(), tmp));
+                k_axpy<T><<<g, 256, 0, s->stream>>>(n, (T)(-D), tmp, lU);  // lU1
+                EMS_TRY(apply_mass<T>(s, -0.5 * D, s->g_mu.as<T>(), tmp, work));
+                k_comb<T><<<g, 256, 0, s->stream>>>(n, tmp, (T)(0.5 * D), (T)1, lU, (T)0, lU, (T)0, nullptr, lV);
+                EMS_HIP(hipGetLastError());
+            }
+        }
+    }
+    EMS_TRY(st_err);
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    if (drift) *drift = 0.0;
     if (ge) EMS_HIP(hipMemcpy(ge, s->g_ge.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
     if (gs) EMS_HIP(hipMemcpy(gs, s->g_gs.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
     // the handle returns to rest at t = 0 and keeps the run's traces

@@ -129,7 +129,8 @@ class TetSimulation:
 
     def gradient(self, num_steps, d_obs):
         """(J, dJ/deps_r, dJ/dsigma, drift) of a run from rest (symplectic
-        steppers, lossless model); d_obs [nobs][num_steps/tf + 1][3]."""
+        steppers; lossless models by running the integrator backwards,
+        lossy ones by checkpoints, drift 0); d_obs [nobs][num_steps/tf + 1][3]."""
         d = np.ascontiguousarray(np.asarray(d_obs, dtype=np.float64))
         J, dr = ctypes.c_double(), ctypes.c_double()
         ge = np.zeros(self.num_tets)

@@ -519,16 +519,18 @@ ems_status ems_tet_set_operator(ems_tet *s, int op);
 ems_status ems_tet_run(ems_tet *s, int num_steps);
 ems_status ems_tet_run_timed(ems_tet *s, int num_steps, double *total_ms);
 
-/* Misfit gradient (symplectic steppers, lossless models): runs num_steps
- * steps from rest at t = 0 with the handle's dipoles and receivers and
- * returns J = 1/2 sum over slots 1..num_steps/tf of (trace - d_obs)^2
- * (d_obs [nobs][num_steps/tf + 1][3], slot 0 unused) and dJ/deps_r,
- * dJ/dsigma per tetrahedron (either may be NULL; dsigma at sigma = 0), FP64.
- * The forward states are recovered by running the integrator backwards (no
- * checkpoints); *drift (may be NULL) = the largest entry of the recovered
- * initial state relative to the final state's.  Afterwards the handle is at
- * rest at t = 0 and holds the run's traces.  No reference counterpart
- * (extends run() of src/symplectic_4/fem_simulation.cpp:1278-1359). */
+/* Misfit gradient (symplectic steppers): runs num_steps steps from rest at
+ * t = 0 with the handle's dipoles and receivers and returns J = 1/2 sum over
+ * slots 1..num_steps/tf of (trace - d_obs)^2 (d_obs [nobs][num_steps/tf +
+ * 1][3], slot 0 unused) and dJ/deps_r, dJ/dsigma per tetrahedron (either may
+ * be NULL), FP64.  Lossless models: the forward states are recovered by
+ * running the integrator backwards (no checkpoints); *drift (may be NULL) =
+ * the largest entry of the recovered initial state relative to the final
+ * state's.  Lossy models (any sigma > 0): checkpoints every
+ * ceil(sqrt(num_steps)) steps, each segment rerun once with its stages' states
+ * kept (*drift = 0).  Afterwards the handle is at rest at t = 0 and holds the
+ * run's traces.  No reference counterpart (extends run() of
+ * src/symplectic_4/fem_simulation.cpp:1278-1359). */
 ems_status ems_tet_gradient(ems_tet *s, int num_steps, const double *d_obs, size_t count, double *misfit,
                             double *grad_eps_r, double *grad_sigma, double *drift);
 
