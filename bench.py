@@ -520,6 +520,8 @@ def main():
                     help="run only short FEM legs (tet configs 2 and 3, fem_v2 and fem at 128^3), for rocprofv3")
     ap.add_argument("--fem-elem-profile", action="store_true",
                     help="run only the hex FEM order 2 and 3 legs with both element products, for rocprofv3")
+    ap.add_argument("--gradient-profile", action="store_true",
+                    help="run only a short FDTD gradient leg (192^3, 4 shots, 60 steps), for rocprofv3")
     ap.add_argument("--fp32-errors", action="store_true",
                     help="FP32 against FP64 traces and gradients per benchmark configuration, then exit")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -534,6 +536,13 @@ def main():
         print("bench.py: multi-rank launch is not supported", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibrate:
+        import torch
+        x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     if args.fp32_errors:
         print(json.dumps({"metric": "FP32 error against FP64", "configs": fp32_errors()}), flush=True)
         return
@@ -544,18 +553,14 @@ def main():
                 gpu_fem1("fem_abm", 128, 10)]
         print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
         return
+    if args.gradient_profile:
+        print(json.dumps({"metric": "FDTD gradient", "legs": [gpu_batch_gradient(shots=4, steps=60)]}), flush=True)
+        return
     if args.fem_elem_profile:
         legs = [gpu_fem(o, st, n, k, p, e) for (o, st, n, k) in ((3, "fem_v2", 96, 20), (2, "symplectic_4", 128, 10))
                 for p in ("f64", "f32") for e in ("rows", "mfma")]
         print(json.dumps({"metric": "FEM DOF-updates/s", "legs": legs}), flush=True)
         return
-    if args.copy_calibrate:
-        import torch
-        x = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
     t_start = time.perf_counter()
 
     def note(leg):
