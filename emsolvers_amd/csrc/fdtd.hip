@@ -16,6 +16,7 @@
 
 #include "common.hpp"
 #include "fdtd_kernels.hpp"
+#include "fdtd_tb2.hpp"
 
 using namespace ems;
 using namespace ems::fdtd;
@@ -50,9 +51,10 @@ struct ems_fdtd {
     double pml_order = 3.0, pml_r0 = 1e-6, pml_kmax = 1.0, pml_amax = 0.0;
     DevBuf prof;    // packed CPML grading tables
     DevBuf psi[12];
-    // fused kernel: second field set and second H-psi set (ping-pong)
+    // ping-pong kernels (fused, z-march, two-step): second field set and
+    // second psi set (same indexing as psi; fused and z-march swap the H-psi)
     DevBuf f2[6];
-    DevBuf psi2[6];
+    DevBuf psi2[12];
     bool alt_ready = false;
     int tile_by = 8;   // workgroup rows of the fused kernel (4, 8 or 16)
     int tile_lz = 4;   // planes per workgroup of the fused kernel
@@ -61,6 +63,14 @@ struct ems_fdtd {
     int split_lz = 0;    // split kernels: planes per thread (0: 4 in FP64 with a CPML, else 2)
     int fused_rev = 0;   // fused kernel: tiles of this step in reverse order
     int z_ty = 4, z_lz = 8, z_nw = 1;  // z-march kernel: strip rows, planes per chunk, waves per workgroup
+    int tb_txw = 0, tb_tyw = 0, tb_lz = 0;  // two-step kernel: thread tile columns, rows, planes per chunk (0: default)
+    // two-step kernel: sources and receiver points sorted by plane (CSR over K)
+    DevBuf d_tb_skptr, d_tb_scomp, d_tb_su, d_tb_sidx, d_tb_pkptr, d_tb_pcomp, d_tb_pu, d_tb_pidx;
+    int tb_sk[2] = {0, -1}, tb_pk[2] = {0, -1};  // first and last plane holding sources / receiver points
+    // two-step kernel with a CPML: the shell launches run on a second stream,
+    // concurrently with the interior one (fork / join events)
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     // fused-kernel tile lists (interior / CPML-touching)
     DevBuf d_tiles_inner, d_tiles_pml;
     int n_inner = 0, n_pml = 0, ntx = 0, nty = 0;
@@ -229,8 +239,8 @@ ems_status setup_pml(ems_fdtd *s)
             EMS_HIP(hipMemset(b.p, 0, nslab[ax] * s->esz));
         }
     for (int ax = 0; ax < 3; ++ax)
-        for (int q = 0; q < 2; ++q) {
-            DevBuf &b = s->psi2[ax * 2 + q];
+        for (int q = 0; q < 4; ++q) {
+            DevBuf &b = s->psi2[ax * 4 + q];
             EMS_TRY(b.reserve(nslab[ax] * s->esz));
             EMS_HIP(hipMemset(b.p, 0, nslab[ax] * s->esz));
         }
@@ -425,7 +435,7 @@ void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
         for (int ax = 0; ax < 3; ++ax)
             for (int q = 0; q < 2; ++q) {
                 ph.old_[ax][q] = s->psi[ax * 4 + q].as<T>();
-                ph.new_[ax][q] = s->psi2[ax * 2 + q].as<T>();
+                ph.new_[ax][q] = s->psi2[ax * 4 + q].as<T>();
             }
     // interior tiles without CPML code, then the CPML tiles; both read the
     // old buffers and write disjoint cells of the new ones
@@ -452,7 +462,7 @@ void launch_fused(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
     for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
     if (PML)
         for (int ax = 0; ax < 3; ++ax)
-            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 2 + q]);
+            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 4 + q]);
 }
 
 template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
@@ -485,7 +495,7 @@ void launch_zmarch_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
         for (int ax = 0; ax < 3; ++ax)
             for (int q = 0; q < 2; ++q) {
                 ph.old_[ax][q] = s->psi[ax * 4 + q].as<T>();
-                ph.new_[ax][q] = s->psi2[ax * 2 + q].as<T>();
+                ph.new_[ax][q] = s->psi2[ax * 4 + q].as<T>();
             }
     ZTiles zt;
     zt.ntx = (g.nx + 1 + 61) / 62;
@@ -503,7 +513,7 @@ void launch_zmarch_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
     for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
     if (PML)
         for (int ax = 0; ax < 3; ++ax)
-            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 2 + q]);
+            for (int q = 0; q < 2; ++q) swap_buf(s->psi[ax * 4 + q], s->psi2[ax * 4 + q]);
 }
 
 template <typename T, int MAT, bool PML> void launch_zmarch(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
@@ -517,6 +527,203 @@ template <typename T, int MAT, bool PML> void launch_zmarch(ems_fdtd *s, T ch, T
     }
 }
 
+// Tile of the two-step kernel: the handle's setting or the default for the
+// precision (FP32 68 x 15 threads, FP64 36 x 14; 8 planes per workgroup).
+template <typename T> void tb_tile(const ems_fdtd *s, int &txw, int &tyw, int &lz)
+{
+    txw = s->tb_txw > 0 ? s->tb_txw : sizeof(T) == 4 ? 68 : 36;
+    tyw = s->tb_tyw > 0 ? s->tb_tyw : sizeof(T) == 4 ? 15 : 14;
+    lz = s->tb_lz > 0 ? s->tb_lz : 8;
+}
+
+// Launches of one two-step advance.  Without a CPML: one launch over the grid.
+// With a CPML of P cells: the interior box, whose tiles (with their 2 + 1
+// halo and the 2 + 1 planes below / above a chunk) meet no CPML cell, without
+// CPML code; the shell in three launches of two slabs each, whose
+// instantiations carry only the psi of the axes they meet:
+//     z-slabs  K < P+3 or K > n-P-1, whole planes                 (x, y, z)
+//     x-slabs  the other planes, I < P+3 or I > n-P-1, all rows   (x, y)
+//     y-slabs  the other planes and columns, J < P+3 or J > n-P-1 (y)
+// Every cell of the update box is stored by exactly one launch.
+struct TbLaunch {
+    int pm;  // CPML axes (bit 0 x, 1 y, 2 z)
+    TbTiles tt;
+};
+
+template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
+{
+    const GridView &g = s->g;
+    int txw, tyw, lz;
+    tb_tile<T>(s, txw, tyw, lz);
+    // shell launches use workgroups of at most 512 threads (so that their
+    // CPML instantiations get 256 registers)
+    const int maxt = 512;
+    auto box = [&](int i0, int i1, int j0, int j1, int k0, int k1, int bw, int bh, int bz) {
+        TbBox b{};
+        b.i0 = i0;
+        b.j0 = j0;
+        b.k0 = k0;
+        b.ni = i1 - i0;
+        b.nj = j1 - j0;
+        b.nk = k1 - k0;
+        b.ntx = (b.ni + bw - 4) / (bw - 3);
+        b.nty = (b.nj + bh - 4) / (bh - 3);
+        b.nkc = (b.nk + bz - 1) / bz;
+        b.count = b.ni > 0 && b.nj > 0 && b.nk > 0 ? b.ntx * b.nty * b.nkc : 0;
+        return b;
+    };
+    auto launch = [&](int pm, int bw, int bh, int bz, TbBox b0, TbBox b1) {
+        TbLaunch l{};
+        l.pm = pm;
+        l.tt.txw = bw;
+        l.tt.tyw = bh;
+        l.tt.lz = bz;
+        l.tt.box[0] = b0;
+        l.tt.box[1] = b1;
+        l.tt.count = b0.count + b1.count;
+        l.tt.per = (l.tt.count + 7) / 8;
+        return l;
+    };
+    std::vector<TbLaunch> v;
+    const int P = s->npml > 0 ? s->npml : 0;
+    const int n1x = g.nx + 1, n1y = g.ny + 1, n1z = g.nz + 1;
+    const TbBox none{};
+    if (P == 0) {
+        v.push_back(launch(0, txw, tyw, lz, box(0, n1x, 0, n1y, 0, n1z, txw, tyw, lz), none));
+        return v;
+    }
+    // interior cores [P+3, n-P-1] on each axis
+    const int m = P + 3;
+    const int ex = g.nx - P, ey = g.ny - P, ez = g.nz - P;  // one past the interior cores
+    if (ex <= m || ey <= m || ez <= m) {
+        const int zw = txw * tyw <= maxt ? txw : 36, zh = txw * tyw <= maxt ? tyw : 14;
+        v.push_back(launch(7, zw, zh, lz, box(0, n1x, 0, n1y, 0, n1z, zw, zh, lz), none));
+        return v;
+    }
+    v.push_back(launch(0, txw, tyw, lz, box(m, ex, m, ey, m, ez, txw, tyw, lz), none));
+    // z-slabs: one chunk per slab
+    const int zl = std::max(m, n1z - ez);
+    const int zw = txw * tyw <= maxt ? txw : 36, zh = txw * tyw <= maxt ? tyw : 14;
+    v.push_back(launch(7, zw, zh, zl, box(0, n1x, 0, n1y, 0, m, zw, zh, zl), box(0, n1x, 0, n1y, ez, n1z, zw, zh, zl)));
+    // x-slabs: narrow tiles across I
+    const int xw = std::max(m, n1x - ex) + 3;
+    const int xh = std::min(64, maxt / xw);
+    v.push_back(launch(3, xw, xh, lz, box(0, m, 0, n1y, m, ez, xw, xh, lz), box(ex, n1x, 0, n1y, m, ez, xw, xh, lz)));
+    // y-slabs: narrow tiles across J
+    const int yh = std::max(m, n1y - ey) + 3;
+    const int yw = std::min(64, maxt / yh);
+    v.push_back(launch(2, yw, yh, lz, box(m, ex, 0, m, m, ez, yw, yh, lz), box(m, ex, ey, n1y, m, ez, yw, yh, lz)));
+    return v;
+}
+
+// Two steps in one advance (EMS_KERNEL_TB2): every launch reads the current
+// sets and writes the other field set and the other psi set (all twelve
+// slabs), which become current.
+template <typename T, int MAT, int PM, int MAXT>
+void launch_tb2_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &pp, const TbTiles &tt,
+                  hipStream_t st)
+{
+    const GridView &g = s->g;
+    if (tt.count == 0) return;
+    PsiSet<T> ps{};
+    if (PM != 0)
+        for (int ax = 0; ax < 3; ++ax)
+            for (int q = 0; q < 2; ++q) {
+                ps.oh[ax][q] = s->psi[ax * 4 + q].as<T>();
+                ps.oe[ax][q] = s->psi[ax * 4 + 2 + q].as<T>();
+                ps.nh[ax][q] = s->psi2[ax * 4 + q].as<T>();
+                ps.ne[ax][q] = s->psi2[ax * 4 + 2 + q].as<T>();
+            }
+    TbArrays<T> ar{};
+    const long off = g.off;
+    ar.ex = s->f[0].as<T>() + off;
+    ar.ey = s->f[1].as<T>() + off;
+    ar.ez = s->f[2].as<T>() + off;
+    ar.hx = s->f[3].as<T>() + off;
+    ar.hy = s->f[4].as<T>() + off;
+    ar.hz = s->f[5].as<T>() + off;
+    ar.nex = s->f2[0].as<T>() + off;
+    ar.ney = s->f2[1].as<T>() + off;
+    ar.nez = s->f2[2].as<T>() + off;
+    ar.nhx = s->f2[3].as<T>() + off;
+    ar.nhy = s->f2[4].as<T>() + off;
+    ar.nhz = s->f2[5].as<T>() + off;
+    ar.ca = s->ca.p ? s->ca.as<T>() + off : nullptr;
+    ar.cb = s->cb.p ? s->cb.as<T>() + off : nullptr;
+    ar.di = (unsigned)sizeof(T);
+    ar.dj = (unsigned)(g.pitch * sizeof(T));
+    ar.dk = (unsigned)(g.slab * sizeof(T));
+    const int threads = (tt.txw * tt.tyw + 63) / 64 * 64;
+    const size_t lds = (size_t)18 * threads * sizeof(T);
+    static std::once_flag attr;  // per instantiation: allow the full exchange size
+    std::call_once(attr, [] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2<T, MAT, PM, MAXT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 18 * MAXT * (int)sizeof(T));
+    });
+    k_tb2<T, MAT, PM, MAXT><<<tt.per * 8, threads, lds, st>>>(g, ch, ce, ar, pm, ps, tt, pp);
+}
+
+template <typename T, int MAT>
+void launch_tb2_m(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &pp, const TbLaunch &l, hipStream_t st)
+{
+    // the CPML-free interior of an FP32 grid may use workgroups of 1024
+    // threads; everything else at most 512
+    constexpr int M0 = sizeof(T) == 4 ? 1024 : 512;
+    switch (l.pm) {
+    case 0: launch_tb2_t<T, MAT, 0, M0>(s, ch, ce, pm, pp, l.tt, st); break;
+    case 2: launch_tb2_t<T, MAT, 2, 512>(s, ch, ce, pm, pp, l.tt, st); break;
+    case 3: launch_tb2_t<T, MAT, 3, 512>(s, ch, ce, pm, pp, l.tt, st); break;
+    default: launch_tb2_t<T, MAT, 7, 512>(s, ch, ce, pm, pp, l.tt, st); break;
+    }
+}
+
+template <typename T> void launch_tb2_one(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &pp,
+                                          const TbLaunch &l, hipStream_t st)
+{
+    switch (s->mat_mode) {
+    case 0: launch_tb2_m<T, 0>(s, ch, ce, pm, pp, l, st); break;
+    case 1: launch_tb2_m<T, 1>(s, ch, ce, pm, pp, l, st); break;
+    default: launch_tb2_m<T, 2>(s, ch, ce, pm, pp, l, st); break;
+    }
+}
+
+// One two-step advance.  With a CPML each shell launch goes to a stream of
+// its own (three streams of the handle, default priority) while the interior
+// runs on the handle's stream: they read the same old sets and write disjoint
+// cells, and the shell's small launches would otherwise leave most of the GPU
+// idle (measured at 256^3 FP32: 0.32 ms per step with the four launches in
+// sequence, 0.27 with the shell on one second stream).
+template <typename T> ems_status launch_tb2(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &pp)
+{
+    const std::vector<TbLaunch> plan = tb_plan<T>(s);
+    if (plan.size() > 1) {
+        const size_t ns = std::min<size_t>(plan.size() - 1, 3);
+        if (!s->ev_fork) EMS_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+        for (size_t q = 0; q < ns; ++q) {
+            if (!s->side[q]) EMS_HIP(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
+            if (!s->ev_join[q]) EMS_HIP(hipEventCreateWithFlags(&s->ev_join[q], hipEventDisableTiming));
+        }
+        EMS_HIP(hipEventRecord(s->ev_fork, s->stream));
+        for (size_t q = 0; q < ns; ++q) EMS_HIP(hipStreamWaitEvent(s->side[q], s->ev_fork, 0));
+        for (size_t q = 1; q < plan.size(); ++q) launch_tb2_one<T>(s, ch, ce, pm, pp, plan[q], s->side[(q - 1) % ns]);
+        EMS_HIP(hipGetLastError());
+        for (size_t q = 0; q < ns; ++q) EMS_HIP(hipEventRecord(s->ev_join[q], s->side[q]));
+        launch_tb2_one<T>(s, ch, ce, pm, pp, plan[0], s->stream);
+        EMS_HIP(hipGetLastError());
+        for (size_t q = 0; q < ns; ++q) EMS_HIP(hipStreamWaitEvent(s->stream, s->ev_join[q], 0));
+    } else {
+        launch_tb2_one<T>(s, ch, ce, pm, pp, plan[0], s->stream);
+        EMS_HIP(hipGetLastError());
+    }
+    for (int w = 0; w < 6; ++w) swap_buf(s->f[w], s->f2[w]);
+    if (s->npml > 0)
+        for (int q = 0; q < 12; ++q) swap_buf(s->psi[q], s->psi2[q]);
+    return EMS_OK;
+}
+
+// Single steps of a two-step handle (odd step counts, the gradient's
+// recompute, ems_fdtd_run_static) take the split sweeps: they work in place on
+// the current sets, which the two-step launches keep current.
 template <typename T, int MAT, bool PML> void launch_variant(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm)
 {
     switch (kernel_kind(s)) {
@@ -622,6 +829,47 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
         EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
         EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
     }
+    if (kernel_kind(s) == EMS_KERNEL_TB2 && num_steps >= 2) {
+        // the two-step kernel finds a plane's sources and receiver points
+        // through CSR lists over K (stable: sources of one cell keep their order)
+        const GridView &g = s->g;
+        auto plane_of = [&](long u) { return (int)((u - g.off) / g.slab); };
+        auto csr = [&](int n, const std::vector<int> &comp, const std::vector<long> &uu, DevBuf &kptr, DevBuf &dc,
+                       DevBuf &du, DevBuf &di, int (&krange)[2]) -> ems_status {
+            krange[0] = 0;
+            krange[1] = -1;
+            for (int q = 0; q < n; ++q) {
+                const int K = plane_of(uu[q]);
+                krange[0] = q == 0 ? K : std::min(krange[0], K);
+                krange[1] = q == 0 ? K : std::max(krange[1], K);
+            }
+            std::vector<int> order(n);
+            for (int q = 0; q < n; ++q) order[q] = q;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return plane_of(uu[a]) < plane_of(uu[b]); });
+            std::vector<int> kp(g.nz + 3, 0), c2(std::max(n, 1)), i2(std::max(n, 1));
+            std::vector<long> u2(std::max(n, 1));
+            for (int q = 0; q < n; ++q) {
+                const int o = order[q];
+                c2[q] = comp[o];
+                u2[q] = uu[o];
+                i2[q] = o;
+                kp[plane_of(uu[o]) + 1]++;
+            }
+            for (int K = 0; K < g.nz + 2; ++K) kp[K + 1] += kp[K];
+            EMS_TRY(kptr.reserve(sizeof(int) * kp.size()));
+            EMS_TRY(dc.reserve(sizeof(int) * c2.size()));
+            EMS_TRY(du.reserve(sizeof(long) * u2.size()));
+            EMS_TRY(di.reserve(sizeof(int) * i2.size()));
+            EMS_HIP(hipMemcpy(kptr.p, kp.data(), sizeof(int) * kp.size(), hipMemcpyHostToDevice));
+            EMS_HIP(hipMemcpy(dc.p, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice));
+            EMS_HIP(hipMemcpy(du.p, u2.data(), sizeof(long) * u2.size(), hipMemcpyHostToDevice));
+            EMS_HIP(hipMemcpy(di.p, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice));
+            return EMS_OK;
+        };
+        EMS_TRY(csr(nsrc, sc, su, s->d_tb_skptr, s->d_tb_scomp, s->d_tb_su, s->d_tb_sidx, s->tb_sk));
+        EMS_TRY(csr(npts, pc, pu, s->d_tb_pkptr, s->d_tb_pcomp, s->d_tb_pu, s->d_tb_pidx, s->tb_pk));
+    }
     rp.nsrc = nsrc;
     rp.npts = npts;
     rp.nrec = nrec;
@@ -666,19 +914,57 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
     const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
     const bool timed = total_ms != nullptr;
+    const bool tb2 = kernel_kind(s) == EMS_KERNEL_TB2;
+    if (tb2) {
+        int txw, tyw, lz;
+        tb_tile<T>(s, txw, tyw, lz);
+        if (txw * tyw > tb_max_threads<T>()) return invalid("two-step tile: at most 512 threads in FP64");
+        if ((double)s->g.slab * (s->g.nz + 2) * sizeof(T) >= 4294967296.0)
+            return invalid("two-step kernel: a field array reaches 4 GiB (32-bit offsets); use the split sweeps");
+    }
+    // events: [2 l, 2 l + 1] around the Yee launch(es) of launch group l
+    // (one step, or two for the two-step kernel); the last pair brackets all
     std::vector<hipEvent_t> ev;
+    int ngroups = 0;
     if (timed) {
         ev.resize(2 * (size_t)num_steps + 2, nullptr);
         for (auto &e : ev) EMS_HIP(hipEventCreate(&e));
         EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps], s->stream));
     }
-    for (int step = 1; step <= num_steps; ++step) {
-        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1)], s->stream));
+    for (int step = 1; step <= num_steps;) {
+        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups], s->stream));
         const PmlView<T> pm = pml_view<T>(s);
+        if (tb2 && step + 1 <= num_steps) {
+            // steps `step` and `step + 1`, sources and receivers inside
+            TbPost pp{};
+            pp.s_kptr = s->d_tb_skptr.as<int>();
+            pp.s_comp = s->d_tb_scomp.as<int>();
+            pp.s_u = s->d_tb_su.as<long>();
+            pp.s_idx = s->d_tb_sidx.as<int>();
+            pp.v1 = s->d_src_val.as<double>() + (size_t)(step - 1) * nsrc;
+            pp.v2 = s->d_src_val.as<double>() + (size_t)step * nsrc;
+            pp.p_kptr = s->d_tb_pkptr.as<int>();
+            pp.p_comp = s->d_tb_pcomp.as<int>();
+            pp.p_u = s->d_tb_pu.as<long>();
+            pp.p_idx = s->d_tb_pidx.as<int>();
+            pp.s_k0 = s->tb_sk[0];
+            pp.s_k1 = s->tb_sk[1];
+            pp.p_k0 = s->tb_pk[0];
+            pp.p_k1 = s->tb_pk[1];
+            if (npts && step % tf == 0) pp.rec1 = s->d_rec.as<double>() + (size_t)(step / tf - 1) * npts;
+            if (npts && (step + 1) % tf == 0) pp.rec2 = s->d_rec.as<double>() + (size_t)((step + 1) / tf - 1) * npts;
+            EMS_TRY(launch_tb2<T>(s, ch, ce, pm, pp));
+            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups + 1], s->stream));
+            EMS_HIP(hipGetLastError());
+            ++ngroups;
+            step += 2;
+            continue;
+        }
         launch_step<T>(s, ch, ce, pm);
         const Fields<T> f = fields<T>(s);
-        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)(step - 1) + 1], s->stream));
+        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups + 1], s->stream));
         EMS_HIP(hipGetLastError());
+        ++ngroups;
         double *rec = nullptr;
         if (npts && step % tf == 0) rec = s->d_rec.as<double>() + (size_t)(step / tf - 1) * npts;
         if (nsrc || rec) {
@@ -687,12 +973,13 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
                                                 s->d_pt_comp.as<int>(), s->d_pt_u.as<long>(), rec);
             EMS_HIP(hipGetLastError());
         }
+        ++step;
     }
     if (timed) {
         EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps + 1], s->stream));
         EMS_HIP(hipEventSynchronize(ev[2 * (size_t)num_steps + 1]));
         double ksum = 0.0;
-        for (int st = 0; st < num_steps; ++st) {
+        for (int st = 0; st < ngroups; ++st) {
             float ms = 0.f;
             EMS_HIP(hipEventElapsedTime(&ms, ev[2 * (size_t)st], ev[2 * (size_t)st + 1]));
             ksum += ms;
@@ -802,7 +1089,15 @@ struct Adj {
     DevBuf w;                   // residual weights [rows][npts]
 };
 
-ems_fdtd::~ems_fdtd() { delete adj; }
+ems_fdtd::~ems_fdtd()
+{
+    delete adj;
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (int q = 0; q < 3; ++q) {
+        if (ev_join[q]) (void)hipEventDestroy(ev_join[q]);
+        if (side[q]) (void)hipStreamDestroy(side[q]);
+    }
+}
 
 namespace {
 
@@ -2358,8 +2653,21 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_ZMARCH) return invalid("unknown kernel variant");
+    if (variant < EMS_KERNEL_AUTO || variant > EMS_KERNEL_TB2) return invalid("unknown kernel variant");
     s->kernel = variant;
+    return EMS_OK;
+}
+
+ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (!(cols == 0 && rows == 0) && (cols < 5 || rows < 5 || cols * rows > kTbMaxThreads))
+        return invalid("two-step tile: cols, rows >= 5 and cols * rows <= 1024 (512 in FP64), or both 0");
+    if (planes < 0) return invalid("two-step planes per workgroup must be >= 1 (0: default)");
+    s->tb_txw = cols;
+    s->tb_tyw = rows;
+    s->tb_lz = planes;
     return EMS_OK;
 }
 

@@ -13,7 +13,7 @@ from . import _lib
 
 F64, F32 = 0, 1
 WAVELET_HERTZ, WAVELET_RICKER, WAVELET_GAUSS = 0, 1, 2
-KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED = 0, 1, 2
+KERNEL_AUTO, KERNEL_SPLIT, KERNEL_FUSED, KERNEL_ZMARCH, KERNEL_TB2 = 0, 1, 2, 3, 4
 FIELD_NAMES = ("ex", "ey", "ez", "hx", "hy", "hz")
 
 # Reference physical constants (src/fdtd/main.cpp:24-27)
@@ -128,6 +128,12 @@ class FemSimulation:
         """Tiling of the z-march sweep (kernel 3): strip rows (2, 4), planes
         per chunk, wavefronts per workgroup (1, 4)."""
         _lib.check(_lib.load().ems_fdtd_set_zmarch(self._h, int(rows), int(planes), int(waves)))
+
+    def set_tb2(self, cols=0, rows=0, planes=0):
+        """Tiling of the two-step sweep (kernel 4): thread tile cols x rows
+        (cols * rows <= 1024; stored core (cols-3) x (rows-3)), planes per
+        workgroup; 0 = default."""
+        _lib.check(_lib.load().ems_fdtd_set_tb2(self._h, int(cols), int(rows), int(planes)))
 
     def run_timed(self, num_steps):
         """Returns (total_ms, kernel_ms) measured with HIP events."""

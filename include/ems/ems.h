@@ -48,7 +48,7 @@ enum {
 };
 
 /* kernel variants for ems_fdtd_set_kernel */
-enum { EMS_KERNEL_AUTO = 0, EMS_KERNEL_SPLIT = 1, EMS_KERNEL_FUSED = 2, EMS_KERNEL_ZMARCH = 3 };
+enum { EMS_KERNEL_AUTO = 0, EMS_KERNEL_SPLIT = 1, EMS_KERNEL_FUSED = 2, EMS_KERNEL_ZMARCH = 3, EMS_KERNEL_TB2 = 4 };
 
 /* Last error message of the calling thread ("" if none). */
 const char *ems_last_error(void);
@@ -108,10 +108,23 @@ ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, dou
 
 /* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT (an H sweep and an E sweep
  * in place), EMS_KERNEL_FUSED (one tiled sweep with LDS exchange, ping-pong
- * buffers) or EMS_KERNEL_ZMARCH (one sweep of independent wavefronts: DPP
+ * buffers), EMS_KERNEL_ZMARCH (one sweep of independent wavefronts: DPP
  * lane shifts in x, unrolled rows in y, register z-march; ping-pong
- * buffers).  Results do not depend on it. */
+ * buffers) or EMS_KERNEL_TB2 (two time steps per launch: overlapped tiles
+ * with a halo of 2 + 1 columns, LDS exchange, a lagged z-march; sources and
+ * receivers inside; ping-pong field and psi sets; odd remaining steps take
+ * the split sweeps).  Results do not depend on it. */
 ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
+
+/* Tiling of EMS_KERNEL_TB2: thread tile of cols x rows (I, J) columns,
+ * cols, rows >= 5 and cols * rows <= 1024 (512 in FP64; a larger FP64 tile
+ * makes ems_fdtd_run fail with EMS_ERR_INVALID_ARG), its stored core
+ * (cols - 3) x (rows - 3), and planes per workgroup (>= 1).  0 for
+ * cols = rows or planes selects the default (FP32 68 x 15, FP64 36 x 14,
+ * 8 planes).  With a CPML the tile applies to the CPML-free interior and the
+ * z-slabs of the shell; the x- and y-slabs take narrow tiles of their own.
+ * Results do not depend on it.  No reference counterpart. */
+ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes);
 
 /* Tiling of EMS_KERNEL_ZMARCH: rows per wavefront strip (2 or 4), planes
  * per chunk (>= 1), wavefronts per workgroup (1 or 4).  Defaults 4, 8, 1.

@@ -27,7 +27,7 @@ def gpu_sim(grid, h, dt, eps, mu, tf, precision="f64", kernel=None):
     return s
 
 
-KERNELS = [1, 2, 3]  # split, fused, z-march
+KERNELS = [1, 2, 3, 4]  # split, fused, z-march, two-step
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -128,6 +128,41 @@ def test_zmarch_tilings_agree_bitwise_with_split_fp32(rows, planes, waves):
     for w in range(6):
         assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
     assert np.array_equal(a.traces(), b.traces())
+
+
+@pytest.mark.parametrize("cols,rows,planes", [(5, 5, 1), (68, 11, 32), (20, 7, 3), (40, 25, 1000), (13, 9, 2)])
+@pytest.mark.parametrize("name", ["lossy_cpml_rect", "pec_rect_nonsquare"])
+def test_tb2_tilings_agree_bitwise_with_split_fp32(name, cols, rows, planes):
+    """The two-step sweep's tiles and chunks change only its halo
+    recomputation: FP32 fields, psi-driven CPML fields and traces equal the
+    split sweeps' bit for bit (tf = 2 records on either step of a pair)."""
+    case = fc.PARITY_CASES[name]
+    print("two-step tiling", name, cols, rows, planes, flush=True)
+    a = build_gpu(case, "f32", 1)
+    b = build_gpu(case, "f32", 4)
+    b.set_tb2(cols, rows, planes)
+    a.run(case["nsteps"])
+    b.run(case["nsteps"])
+    for w in range(6):
+        assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+    assert np.array_equal(a.traces(), b.traces(), equal_nan=True)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_tb2_odd_steps_and_split_runs(precision):
+    """Odd step counts end with one split step: runs of 3, 1, 2, 4 and 3
+    steps in a row give the fields, traces (slots restart per run, as the
+    reference's) and time of the same runs on the split sweeps."""
+    case = fc.PARITY_CASES["lossy_cpml_rect"]
+    a = build_gpu(case, precision, 1)
+    b = build_gpu(case, precision, 4)
+    for n in (3, 1, 2, 4, 3):
+        a.run(n)
+        b.run(n)
+    for w in range(6):
+        assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+    assert np.array_equal(a.traces(), b.traces(), equal_nan=True)
+    assert a.current_time == b.current_time
 
 
 @pytest.mark.parametrize("rows,planes,flags", [(8, 1, 0), (8, 7, 1), (8, 8, 2), (16, 3, 3), (16, 1000, 1)])
