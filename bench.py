@@ -411,11 +411,12 @@ def pmc_traffic(path):
             continue
         if t[0] == "kernel":
             kern.append((float(t[3]), float(t[5])))
-        else:
+        elif t[0] in ("copy_bytes", "copy_fetch_kib", "copy_write_kib", "steps_per_group"):
             kv[t[0]] = float(t[1])
     fs = kv["copy_bytes"] / (kv["copy_fetch_kib"] * 1024.0)
     ws = kv["copy_bytes"] / (kv["copy_write_kib"] * 1024.0)
-    return sum(f * 1024.0 * fs + w * 1024.0 * ws for f, w in kern)
+    # the kernels listed make up one launch group of steps_per_group steps
+    return sum(f * 1024.0 * fs + w * 1024.0 * ws for f, w in kern) / kv.get("steps_per_group", 1.0)
 
 
 def cpu_reference_sample(n=256, steps=None, budget_s=12.0):
@@ -509,7 +510,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", default="f32")
-    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 split, 2 fused")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 split, 2 fused, 3 z-march, 4 two-step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-fem", action="store_true", help="skip the FEM secondary figures")
     ap.add_argument("--tet-steps", type=int, default=200, help="steps of the tetrahedral FEM figures")
@@ -592,7 +593,9 @@ def main():
             "grid": [args.n] * 3,
             "cpml_cells": 10,
             "material_words": 1,
-            "kernel": {0: "auto (split H/E sweeps; fused only in FP64 without a CPML)", 1: "split", 2: "fused"}[args.kernel],
+            "kernel": {0: "auto (two-step sweep: two time steps per launch group, CPML-free interior plus "
+                          "CPML shell slabs on side streams)", 1: "split", 2: "fused", 3: "z-march",
+                       4: "two-step"}[args.kernel],
             "parallelism": "single-gpu",
         },
         "roofline": {
@@ -601,8 +604,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": pmc_traffic(os.path.join(ROOT, "profiles", "fdtd_yee_256_f32_cpml10_pmc.txt"))
-            if (args.n == 256 and args.precision == "f32" and args.kernel in (0, 1)) else None,
+            "traffic": pmc_traffic(os.path.join(ROOT, "profiles", "fdtd_yee_256_f32_cpml10_tb2_kernel_stats_pmc.txt"))
+            if (args.n == 256 and args.precision == "f32" and args.kernel in (0, 4)) else None,
             "kernel_avg_ms": round(k_avg_ms, 5),
             "algorithmic_bytes_per_step": int(r["step_bytes"]),
             "bytes_per_cell": round(r["per_cell"], 3),

@@ -471,13 +471,26 @@ template <typename T, int MAT, bool PML> void launch_fused_by(ems_fdtd *s, T ch,
     else launch_fused<T, 8, MAT, PML>(s, ch, ce, pm);
 }
 
-// EMS_KERNEL_AUTO: measured on MI355X (256^3), the split sweeps are fastest
-// with a CPML (0.38 vs 0.57 ms FP32, 0.67 vs 0.86 ms FP64) and in FP32
-// without one (0.260 vs 0.267 ms); the fused sweep in FP64 without a CPML
-// (0.50 vs 0.53 ms).
+// EMS_KERNEL_AUTO: measured on MI355X (eps_r model, ms per step), the
+// two-step sweep is fastest on large grids, in both precisions:
+//     256^3 FP32  no CPML  0.193 (split 0.267)   CPML 10  0.267 (split 0.324)
+//     256^3 FP64  no CPML  0.354 (fused 0.498)   CPML 10  0.485 (split 0.622)
+//     192^3 FP32  CPML 10  0.152 (split 0.162)
+// but not on small ones, where the CPML shell and short launches dominate
+// (FP32 CPML 10: 128^3 0.068 vs 0.051, 96^3 0.055 vs 0.035, 64^3 0.052 vs
+// 0.021).  So it is chosen when the CPML-free interior spans at least 144
+// cells on every axis and the field arrays stay under 4 GiB (32-bit byte
+// offsets); otherwise the split sweeps run, or the fused sweep in FP64
+// without a CPML (split 0.38 vs fused 0.57 ms FP32 with a CPML, fused 0.50
+// vs split 0.53 ms FP64 without, at 256^3).
+bool tb2_fits(const ems_fdtd *s) { return (double)s->g.slab * (s->g.nz + 2) * s->esz < 4294967296.0; }
+
 int kernel_kind(const ems_fdtd *s)
 {
     if (s->kernel != EMS_KERNEL_AUTO) return s->kernel;
+    const int shell = s->npml > 0 ? 2 * s->npml + 4 : 0;
+    const int inner = std::min({s->g.nx, s->g.ny, s->g.nz}) - shell;
+    if (tb2_fits(s) && inner >= 144) return EMS_KERNEL_TB2;
     return s->npml <= 0 && s->d.precision == EMS_F64 ? EMS_KERNEL_FUSED : EMS_KERNEL_SPLIT;
 }
 
@@ -528,12 +541,14 @@ template <typename T, int MAT, bool PML> void launch_zmarch(ems_fdtd *s, T ch, T
 }
 
 // Tile of the two-step kernel: the handle's setting or the default for the
-// precision (FP32 68 x 15 threads, FP64 36 x 14; 8 planes per workgroup).
+// precision (FP32 68 x 15 threads, FP64 36 x 14; 12 planes per workgroup;
+// measured best at 256^3 with a 10-cell CPML among 68 x 7..15, 36 x 14..28,
+// 44 x 11..23, 100 x 10, 132 x 7 and 8..32 planes).
 template <typename T> void tb_tile(const ems_fdtd *s, int &txw, int &tyw, int &lz)
 {
     txw = s->tb_txw > 0 ? s->tb_txw : sizeof(T) == 4 ? 68 : 36;
     tyw = s->tb_tyw > 0 ? s->tb_tyw : sizeof(T) == 4 ? 15 : 14;
-    lz = s->tb_lz > 0 ? s->tb_lz : 8;
+    lz = s->tb_lz > 0 ? s->tb_lz : 12;
 }
 
 // Launches of one two-step advance.  Without a CPML: one launch over the grid.
@@ -586,7 +601,9 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
     };
     std::vector<TbLaunch> v;
     const int P = s->npml > 0 ? s->npml : 0;
-    const int n1x = g.nx + 1, n1y = g.ny + 1, n1z = g.nz + 1;
+    // the boxes cover the update box and its ghost layer n+1, whose cells are
+    // never updated but may receive sources
+    const int n1x = g.nx + 2, n1y = g.ny + 2, n1z = g.nz + 2;
     const TbBox none{};
     if (P == 0) {
         v.push_back(launch(0, txw, tyw, lz, box(0, n1x, 0, n1y, 0, n1z, txw, tyw, lz), none));
@@ -687,6 +704,19 @@ template <typename T> void launch_tb2_one(ems_fdtd *s, T ch, T ce, const PmlView
     }
 }
 
+// The side streams and fork / join events of a two-step handle with a CPML
+// (created once, at the handle's set-up).
+ems_status tb_streams(ems_fdtd *s)
+{
+    if (s->npml <= 0) return EMS_OK;
+    if (!s->ev_fork) EMS_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+    for (int q = 0; q < 3; ++q) {
+        if (!s->side[q]) EMS_HIP(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
+        if (!s->ev_join[q]) EMS_HIP(hipEventCreateWithFlags(&s->ev_join[q], hipEventDisableTiming));
+    }
+    return EMS_OK;
+}
+
 // One two-step advance.  With a CPML each shell launch goes to a stream of
 // its own (three streams of the handle, default priority) while the interior
 // runs on the handle's stream: they read the same old sets and write disjoint
@@ -698,11 +728,7 @@ template <typename T> ems_status launch_tb2(ems_fdtd *s, T ch, T ce, const PmlVi
     const std::vector<TbLaunch> plan = tb_plan<T>(s);
     if (plan.size() > 1) {
         const size_t ns = std::min<size_t>(plan.size() - 1, 3);
-        if (!s->ev_fork) EMS_HIP(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-        for (size_t q = 0; q < ns; ++q) {
-            if (!s->side[q]) EMS_HIP(hipStreamCreateWithFlags(&s->side[q], hipStreamNonBlocking));
-            if (!s->ev_join[q]) EMS_HIP(hipEventCreateWithFlags(&s->ev_join[q], hipEventDisableTiming));
-        }
+        EMS_TRY(tb_streams(s));
         EMS_HIP(hipEventRecord(s->ev_fork, s->stream));
         for (size_t q = 0; q < ns; ++q) EMS_HIP(hipStreamWaitEvent(s->side[q], s->ev_fork, 0));
         for (size_t q = 1; q < plan.size(); ++q) launch_tb2_one<T>(s, ch, ce, pm, pp, plan[q], s->side[(q - 1) % ns]);
@@ -779,6 +805,7 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
     EMS_TRY(setup_pml(s));
     if (fused_setup && use_fused(s)) EMS_TRY(ensure_alt(s));
     if (fused_setup && use_tiles(s)) EMS_TRY(build_tiles(s));
+    if (fused_setup && kernel_kind(s) == EMS_KERNEL_TB2) EMS_TRY(tb_streams(s));
     if (num_steps <= 0) return EMS_OK;
 
     // sources sorted by component (x list, then y, then z; :222-239)
@@ -919,7 +946,7 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         int txw, tyw, lz;
         tb_tile<T>(s, txw, tyw, lz);
         if (txw * tyw > tb_max_threads<T>()) return invalid("two-step tile: at most 512 threads in FP64");
-        if ((double)s->g.slab * (s->g.nz + 2) * sizeof(T) >= 4294967296.0)
+        if (!tb2_fits(s))
             return invalid("two-step kernel: a field array reaches 4 GiB (32-bit offsets); use the split sweeps");
     }
     // events: [2 l, 2 l + 1] around the Yee launch(es) of launch group l

@@ -355,22 +355,28 @@ template <typename T> struct TbArrays {
 // Loads of the two-step sweep are unconditional (clamped addresses): a load
 // under a per-lane condition merged with a default makes the compiler wait
 // for it where it is issued, which would serialise the march.
+// ui, ui_i1, ui_j1: the lane's cell and its I+1 / J+1 neighbours in a plane,
+// clamped to the allocated layer n+1.
 template <typename T, int MAT>
-__device__ __forceinline__ void tb_load(const TbArrays<T> &a, int nz, unsigned ui, int K, TbSet<T> &v)
+__device__ __forceinline__ void tb_load(const TbArrays<T> &a, int nz, unsigned ui, unsigned ui_i1, unsigned ui_j1,
+                                        int K, TbSet<T> &v)
 {
-    const int Kc = min(max(K, 0), nz + 1);  // planes beyond the grid read the zero ghost plane
-    const int K1c = min(K + 1, nz + 1);
-    const int K2c = max(K - 2, 0);
-    const unsigned u0 = ui + (unsigned)Kc * a.dk, u1 = ui + (unsigned)K1c * a.dk, u2 = ui + (unsigned)K2c * a.dk;
+    // every plane index clamped to the allocated planes 0 .. nz+1 (planes
+    // beyond the grid read the ghost plane nz+1; their values are not used)
+    const int Kc = min(max(K, 0), nz + 1);
+    const int K1c = min(max(K + 1, 0), nz + 1);
+    const int K2c = min(max(K - 2, 0), nz + 1);
+    const unsigned pk = (unsigned)Kc * a.dk;
+    const unsigned u0 = ui + pk, u1 = ui + (unsigned)K1c * a.dk, u2 = ui + (unsigned)K2c * a.dk;
     v.ex = lane_at(a.ex, u0);
     v.ey = lane_at(a.ey, u0);
     v.ez = lane_at(a.ez, u0);
     v.ex1 = lane_at(a.ex, u1);
     v.ey1 = lane_at(a.ey, u1);
-    v.ez_i1 = lane_at(a.ez, u0 + a.di);
-    v.ey_i1 = lane_at(a.ey, u0 + a.di);
-    v.ez_j1 = lane_at(a.ez, u0 + a.dj);
-    v.ex_j1 = lane_at(a.ex, u0 + a.dj);
+    v.ez_i1 = lane_at(a.ez, ui_i1 + pk);
+    v.ey_i1 = lane_at(a.ey, ui_i1 + pk);
+    v.ez_j1 = lane_at(a.ez, ui_j1 + pk);
+    v.ex_j1 = lane_at(a.ex, ui_j1 + pk);
     v.hx = lane_at(a.hx, u0);
     v.hy = lane_at(a.hy, u0);
     v.hz = lane_at(a.hz, u0);
@@ -419,7 +425,11 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     const int I = bx.i0 + ti * (tt.txw - 3) - 2 + tx;
     const int J = bx.j0 + tj * (tt.tyw - 3) - 2 + ty;
     const bool inb = tv && I >= 0 && I <= g.nx && J >= 0 && J <= g.ny;
-    const bool core = inb && tx >= 2 && tx <= tt.txw - 2 && ty >= 2 && ty <= tt.tyw - 2 && I < bx.i0 + bx.ni &&
+    // the ghost layer I = nx+1 / J = ny+1 holds values (never updated) that
+    // the cells I = nx / J = ny read: its threads pass their old values on
+    const bool ing = tv && I >= 0 && I <= g.nx + 1 && J >= 0 && J <= g.ny + 1;
+    // stored: the box's cells of the update box and of its ghost layer n+1
+    const bool core = ing && tx >= 2 && tx <= tt.txw - 2 && ty >= 2 && ty <= tt.tyw - 2 && I < bx.i0 + bx.ni &&
                       J < bx.j0 + bx.nj;
     // LDS neighbours (clamped: out-of-tile neighbours only feed invalid values)
     const int tim = t >= 1 ? t - 1 : t;
@@ -429,12 +439,16 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     const int K0 = bx.k0 + kc * tt.lz;
     const int K1 = min(K0 + tt.lz, bx.k0 + bx.nk);
     const int Ks = max(K0 - 2, 0);
-    // update masks of the column
-    const bool cI = I >= 1, cJ = J >= 1;
-    // threads outside the update box compute from a clamped cell; their
-    // values enter the exchange as zeros and are never stored
-    const int Ic = min(max(I, 0), g.nx), Jc = min(max(J, 0), g.ny);
+    // update masks of the column (no update outside the update box)
+    const bool cI = inb && I >= 1, cJ = inb && J >= 1, c0 = inb;
+    // threads outside the update box read a clamped cell: on the ghost layer
+    // their own cell, whose values pass unchanged through both levels; further
+    // out the values enter the exchange as zeros.  Nothing outside the box is
+    // stored.
+    const int Ic = min(max(I, 0), g.nx + 1), Jc = min(max(J, 0), g.ny + 1);
     const unsigned ui = (unsigned)Ic * a.di + (unsigned)Jc * a.dj;
+    const unsigned ui_i1 = (unsigned)min(Ic + 1, g.nx + 1) * a.di + (unsigned)Jc * a.dj;
+    const unsigned ui_j1 = (unsigned)Ic * a.di + (unsigned)min(Jc + 1, g.ny + 1) * a.dj;
     // CPML: the column's x and y gradings
     bool px = false, py = false;
     Prof<T> hpx{}, hpy{}, epx{}, epy{};
@@ -466,50 +480,56 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     auto plane = [&](auto parc, const int K, const TbSet<T> &v) {
         constexpr int par = decltype(parc)::value, p1 = par ^ 1;
         auto slot = [&](int l, int pr, int c) { return L + ((l * 2 + pr) * 3 + c) * NTP; };
-        // uniform plane conditions (lanes outside the box compute garbage
-        // that is masked where it enters the exchange)
-        const bool kin = K <= g.nz;                        // plane K inside the update box
-        const bool do1 = kin && K <= K1;                   // level 1 at plane K
-        const bool doe1 = do1 && K >= K0 - 1;              // E1 needed at plane K
-        const bool doh2 = K >= K0 && K <= K1;              // H2 at plane K-1 (K-1 <= nz)
-        const bool doe2 = K >= K0 + 2 && K - 2 <= g.nz;    // E2 at plane K-2
+        // Uniform plane conditions.  Every level is produced on its planes;
+        // its updates run on planes of the update box only (K <= nz), the
+        // ghost plane nz+1 passing its values on.  Lanes outside the box
+        // compute garbage that is masked where it enters the exchange.
+        const bool kin = K <= g.nz;
+        const bool do1 = K <= K1;                          // H1 at plane K
+        const bool doe1 = do1 && K >= K0 - 1;              // E1 at plane K
+        const bool doh2 = K >= K0 && K <= K1 && K >= 1;    // H2 at plane K-1
+        const bool doe2 = K >= K0 + 2;                     // E2 at plane K-2 (<= K1-1 by the loop bound)
 
         // ---- a. H1(K) ------------------------------------------------------
         T hx1 = v.hx, hy1 = v.hy, hz1 = v.hz;
         T psh[6] = {};
-        const bool pz = (PM & 4) && inb && in_pml(pm.P, g.nz, K);
-        if (do1) {
+        const bool pz = (PM & 4) && inb && kin && in_pml(pm.P, g.nz, K);
+        if (do1 && kin) {
             Prof<T> hpz{};
             if (PML && pz) hpz = pm.hp[2][K];
             if (PML) tb_psi_load(g, pm.P, pm.W, ps.oh, I, J, K, px, py, pz, psh);
             tb_h<T, PML>(hx1, hy1, hz1, v.ez_j1, v.ez, v.ey1, v.ey, v.ex1, v.ex, v.ez_i1, v.ey_i1, v.ex_j1, ch, cI, cJ,
-                         K >= 1, px, py, pz, hpx, hpy, hpz, psh);
+                         c0 && K >= 1, px, py, pz, hpx, hpy, hpz, psh);
         }
-        slot(0, par, 0)[t] = inb ? hx1 : (T)0;
-        slot(0, par, 1)[t] = inb ? hy1 : (T)0;
-        slot(0, par, 2)[t] = inb ? hz1 : (T)0;
+        slot(0, par, 0)[t] = ing ? hx1 : (T)0;
+        slot(0, par, 1)[t] = ing ? hy1 : (T)0;
+        slot(0, par, 2)[t] = ing ? hz1 : (T)0;
         __syncthreads();
 
         // ---- e. E2(K-2): from H2(K-2) (own and at I-1 / J-1), H2(K-3), E1(K-2)
         if (doe2) {
             const int Km = K - 2;
-            const bool pzm = (PM & 4) && inb && in_pml(pm.P, g.nz, Km);
-            Prof<T> epz{};
-            if (PML && pzm) epz = pm.ep[2][Km];
+            const bool pzm = (PM & 4) && inb && Km <= g.nz && in_pml(pm.P, g.nz, Km);
             T pvs[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) pvs[q] = pse1mm[q];
             T ex2 = slot(1, par, 0)[t], ey2 = slot(1, par, 1)[t], ez2 = slot(1, par, 2)[t];
-            tb_e<T, MAT, PML>(ex2, ey2, ez2, slot(2, par, 0)[t], slot(2, par, 1)[t], slot(2, par, 2)[t],
-                              slot(2, p1, 0)[t], slot(2, p1, 1)[t], slot(2, par, 2)[tim], slot(2, par, 1)[tim],
-                              slot(2, par, 2)[tjm], slot(2, par, 0)[tjm], ce, v.cb2, v.ca2, cJ && Km >= 1,
-                              cI && Km >= 1, cI && cJ, px, py, pzm, epx, epy, epz, pvs);
+            if (Km <= g.nz) {
+                Prof<T> epz{};
+                if (PML && pzm) epz = pm.ep[2][Km];
+                tb_e<T, MAT, PML>(ex2, ey2, ez2, slot(2, par, 0)[t], slot(2, par, 1)[t], slot(2, par, 2)[t],
+                                  slot(2, p1, 0)[t], slot(2, p1, 1)[t], slot(2, par, 2)[tim], slot(2, par, 1)[tim],
+                                  slot(2, par, 2)[tjm], slot(2, par, 0)[tjm], ce, v.cb2, v.ca2, cJ && Km >= 1,
+                                  cI && Km >= 1, cI && cJ, px, py, pzm, epx, epy, epz, pvs);
+            }
             if (Km >= pp.s_k0 && Km <= pp.s_k1) tb_sources(pp, pp.v2, Km, uidx_dev(g, I, J, Km), ex2, ey2, ez2);
             if (core) {
+                // every component, updated or not (sources reach cells that
+                // are not updated; the other buffer is two steps old)
                 const unsigned uo = ui + (unsigned)Km * a.dk;
-                if (cJ && Km >= 1) lane_at(a.nex, uo) = ex2;
-                if (cI && Km >= 1) lane_at(a.ney, uo) = ey2;
-                if (cI && cJ) lane_at(a.nez, uo) = ez2;
+                lane_at(a.nex, uo) = ex2;
+                lane_at(a.ney, uo) = ey2;
+                lane_at(a.nez, uo) = ez2;
                 if (PML) tb_psi_store(g, pm.P, pm.W, ps.ne, I, J, Km, px, py, pzm, pvs);
                 if (pp.rec2 && Km >= pp.p_k0 && Km <= pp.p_k1)
                     tb_receivers(pp, pp.rec2, Km, uidx_dev(g, I, J, Km), ex2, ey2, ez2);
@@ -522,46 +542,50 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
         T ex1 = v.ex, ey1 = v.ey, ez1 = v.ez;
         T pse[6] = {};
         if (doe1) {
-            Prof<T> epz{};
-            if (PML && pz) epz = pm.ep[2][K];
-            if (PML) tb_psi_load(g, pm.P, pm.W, ps.oe, I, J, K, px, py, pz, pse);
-            tb_e<T, MAT, PML>(ex1, ey1, ez1, hx1, hy1, hz1, hx1m, hy1m, slot(0, par, 2)[tim], slot(0, par, 1)[tim],
-                              slot(0, par, 2)[tjm], slot(0, par, 0)[tjm], ce, v.cb, v.ca, cJ && K >= 1,
-                              cI && K >= 1, cI && cJ, px, py, pz, epx, epy, epz, pse);
+            if (kin) {
+                Prof<T> epz{};
+                if (PML && pz) epz = pm.ep[2][K];
+                if (PML) tb_psi_load(g, pm.P, pm.W, ps.oe, I, J, K, px, py, pz, pse);
+                tb_e<T, MAT, PML>(ex1, ey1, ez1, hx1, hy1, hz1, hx1m, hy1m, slot(0, par, 2)[tim],
+                                  slot(0, par, 1)[tim], slot(0, par, 2)[tjm], slot(0, par, 0)[tjm], ce, v.cb, v.ca,
+                                  cJ && K >= 1, cI && K >= 1, cI && cJ, px, py, pz, epx, epy, epz, pse);
+            }
             if (K >= pp.s_k0 && K <= pp.s_k1) tb_sources(pp, pp.v1, K, uidx_dev(g, I, J, K), ex1, ey1, ez1);
             if (pp.rec1 && K >= K0 && K < K1 && K >= pp.p_k0 && K <= pp.p_k1 && core)
                 tb_receivers(pp, pp.rec1, K, uidx_dev(g, I, J, K), ex1, ey1, ez1);
         }
         // E1(K-2) of this slot was read above (by this thread only)
-        slot(1, par, 0)[t] = inb ? ex1 : (T)0;
-        slot(1, par, 1)[t] = inb ? ey1 : (T)0;
-        slot(1, par, 2)[t] = inb ? ez1 : (T)0;
+        slot(1, par, 0)[t] = ing ? ex1 : (T)0;
+        slot(1, par, 1)[t] = ing ? ey1 : (T)0;
+        slot(1, par, 2)[t] = ing ? ez1 : (T)0;
 
         // ---- d. H2(K-1): from H1(K-1), E1(K-1) (own and at I+1 / J+1), E1(K) -----
         T hx2 = hx1m, hy2 = hy1m, hz2 = hz1m;
         if (doh2) {
             const int Km = K - 1;
-            const bool pzm = (PM & 4) && inb && in_pml(pm.P, g.nz, Km);
-            Prof<T> hpz{};
-            if (PML && pzm) hpz = pm.hp[2][Km];
+            const bool pzm = (PM & 4) && inb && Km <= g.nz && in_pml(pm.P, g.nz, Km);
             T pvs[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) pvs[q] = psh1m[q];
-            tb_h<T, PML>(hx2, hy2, hz2, slot(1, p1, 2)[tjp], ez1m, ey1, ey1m, ex1, ex1m, slot(1, p1, 2)[tip],
-                         slot(1, p1, 1)[tip], slot(1, p1, 0)[tjp], ch, cI, cJ, Km >= 1, px, py, pzm, hpx, hpy, hpz,
-                         pvs);
+            if (Km <= g.nz) {
+                Prof<T> hpz{};
+                if (PML && pzm) hpz = pm.hp[2][Km];
+                tb_h<T, PML>(hx2, hy2, hz2, slot(1, p1, 2)[tjp], ez1m, ey1, ey1m, ex1, ex1m, slot(1, p1, 2)[tip],
+                             slot(1, p1, 1)[tip], slot(1, p1, 0)[tjp], ch, cI, cJ, c0 && Km >= 1, px, py, pzm, hpx,
+                             hpy, hpz, pvs);
+            }
             if (core && Km >= K0) {
                 const unsigned uo = ui + (unsigned)Km * a.dk;
-                if (cI) lane_at(a.nhx, uo) = hx2;
-                if (cJ) lane_at(a.nhy, uo) = hy2;
-                if (Km >= 1) lane_at(a.nhz, uo) = hz2;
+                lane_at(a.nhx, uo) = hx2;
+                lane_at(a.nhy, uo) = hy2;
+                lane_at(a.nhz, uo) = hz2;
                 if (PML) tb_psi_store(g, pm.P, pm.W, ps.nh, I, J, Km, px, py, pzm, pvs);
             }
         }
         // H2(K-3) of this slot was read above (by this thread only)
-        slot(2, p1, 0)[t] = inb ? hx2 : (T)0;
-        slot(2, p1, 1)[t] = inb ? hy2 : (T)0;
-        slot(2, p1, 2)[t] = inb ? hz2 : (T)0;
+        slot(2, p1, 0)[t] = ing ? hx2 : (T)0;
+        slot(2, p1, 1)[t] = ing ? hy2 : (T)0;
+        slot(2, p1, 2)[t] = ing ? hz2 : (T)0;
 
         if (PML) {
 #pragma unroll
@@ -580,17 +604,17 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     TbSet<T> A, B;
     int K = Ks;
     if (K & 1) {
-        tb_load<T, MAT>(a, g.nz, ui, K, B);
-        tb_load<T, MAT>(a, g.nz, ui, K + 1, A);
+        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K, B);
+        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 1, A);
         plane(tb_par<1>{}, K, B);
         ++K;
     } else {
-        tb_load<T, MAT>(a, g.nz, ui, K, A);
+        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K, A);
     }
     for (; K + 1 <= Ke; K += 2) {
-        tb_load<T, MAT>(a, g.nz, ui, K + 1, B);
+        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 1, B);
         plane(tb_par<0>{}, K, A);
-        tb_load<T, MAT>(a, g.nz, ui, K + 2, A);
+        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 2, A);
         plane(tb_par<1>{}, K + 1, B);
     }
     if (K <= Ke) plane(tb_par<0>{}, K, A);

@@ -106,7 +106,9 @@ ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0,
  * reference).  t0 < 0 selects sqrt(2)/f0. */
 ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, double amp);
 
-/* EMS_KERNEL_AUTO (default), EMS_KERNEL_SPLIT (an H sweep and an E sweep
+/* EMS_KERNEL_AUTO (default: EMS_KERNEL_TB2 when a padded field array is
+ * under 4 GiB, else the split sweeps, or the fused sweep in FP64 without a
+ * CPML), EMS_KERNEL_SPLIT (an H sweep and an E sweep
  * in place), EMS_KERNEL_FUSED (one tiled sweep with LDS exchange, ping-pong
  * buffers), EMS_KERNEL_ZMARCH (one sweep of independent wavefronts: DPP
  * lane shifts in x, unrolled rows in y, register z-march; ping-pong
@@ -121,7 +123,7 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
  * makes ems_fdtd_run fail with EMS_ERR_INVALID_ARG), its stored core
  * (cols - 3) x (rows - 3), and planes per workgroup (>= 1).  0 for
  * cols = rows or planes selects the default (FP32 68 x 15, FP64 36 x 14,
- * 8 planes).  With a CPML the tile applies to the CPML-free interior and the
+ * 12 planes).  With a CPML the tile applies to the CPML-free interior and the
  * z-slabs of the shell; the x- and y-slabs take narrow tiles of their own.
  * Results do not depend on it.  No reference counterpart. */
 ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes);

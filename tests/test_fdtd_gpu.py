@@ -165,6 +165,36 @@ def test_tb2_odd_steps_and_split_runs(precision):
     assert a.current_time == b.current_time
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("npml", [0, 2])
+def test_tb2_ghost_cells_and_sources_there(precision, npml):
+    """Cells that are never updated (the reference arrays' ghost entries at
+    index 0 and n+1) keep values set by set_field and accumulate sources; the
+    two-step sweep reads and stores them as the split sweeps do, bit for bit."""
+    grid = (12, 14, 10)
+    nx, ny, nz = grid
+    h, dt = 1e-3, fc.cfl_dt(1e-3, 0.9)
+    rng = np.random.default_rng(9)
+    sims = [gpu_sim(grid, h, dt, fc.EPS0, fc.MU0, 1, precision, k) for k in (1, 4)]
+    vals = None
+    for s in sims:
+        if npml:
+            s.set_cpml(npml)
+        if vals is None:
+            vals = [rng.standard_normal(s.field(w).size) for w in range(6)]
+        for w in range(6):
+            s.set_field(w, vals[w])
+        s.setSource_y((nx + 1, 3, 4))   # ghost layer I = nx+1
+        s.setSource_x((5, 0, 3))        # J = 0: never updated
+        s.setSource_z((4, ny + 1, 2))   # ghost layer J = ny+1
+        s.setSource_z((6, 7, nz))       # top plane of Ez
+        s.setObservationPoint([[(5, 0, 3)], [(nx + 1, 3, 4)], [(4, ny + 1, 2), (6, 7, nz)]])
+        s.run(7)
+    for w in range(6):
+        assert np.array_equal(sims[0].field(w), sims[1].field(w)), NAMES[w]
+    assert np.array_equal(sims[0].traces(), sims[1].traces())
+
+
 @pytest.mark.parametrize("rows,planes,flags", [(8, 1, 0), (8, 7, 1), (8, 8, 2), (16, 3, 3), (16, 1000, 1)])
 def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes, flags):
     """Tile shape and K-chunking change only the halo recomputation, never the
