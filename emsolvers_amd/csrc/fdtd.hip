@@ -541,14 +541,15 @@ template <typename T, int MAT, bool PML> void launch_zmarch(ems_fdtd *s, T ch, T
 }
 
 // Tile of the two-step kernel: the handle's setting or the default for the
-// precision (FP32 68 x 15 threads, FP64 36 x 14; 12 planes per workgroup;
-// measured best at 256^3 with a 10-cell CPML among 68 x 7..15, 36 x 14..28,
-// 44 x 11..23, 100 x 10, 132 x 7 and 8..32 planes).
+// precision (FP32 64 x 16 threads and 12 planes, FP64 64 x 8 and 16 planes:
+// one wavefront per tile row; measured best at 256^3 with a 10-cell CPML
+// among 64 x 8..16, 68 x 7..15, 36 x 14..28, 32 x 16, 28 x 18, 24 x 21,
+// 44 x 11..23, 100 x 10, 128 x 8, 132 x 7 and 8..32 planes).
 template <typename T> void tb_tile(const ems_fdtd *s, int &txw, int &tyw, int &lz)
 {
-    txw = s->tb_txw > 0 ? s->tb_txw : sizeof(T) == 4 ? 68 : 36;
-    tyw = s->tb_tyw > 0 ? s->tb_tyw : sizeof(T) == 4 ? 15 : 14;
-    lz = s->tb_lz > 0 ? s->tb_lz : 12;
+    txw = s->tb_txw > 0 ? s->tb_txw : 64;
+    tyw = s->tb_tyw > 0 ? s->tb_tyw : sizeof(T) == 4 ? 16 : 8;
+    lz = s->tb_lz > 0 ? s->tb_lz : sizeof(T) == 4 ? 12 : 16;
 }
 
 // Launches of one two-step advance.  Without a CPML: one launch over the grid.
@@ -671,11 +672,12 @@ void launch_tb2_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &p
     ar.dj = (unsigned)(g.pitch * sizeof(T));
     ar.dk = (unsigned)(g.slab * sizeof(T));
     const int threads = (tt.txw * tt.tyw + 63) / 64 * 64;
-    const size_t lds = (size_t)18 * threads * sizeof(T);
+    const size_t lds = ((size_t)18 * MAXT + 2 * (tt.txw + 1)) * sizeof(T);
     static std::once_flag attr;  // per instantiation: allow the full exchange size
     std::call_once(attr, [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2<T, MAT, PM, MAXT>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 18 * MAXT * (int)sizeof(T));
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (18 * MAXT + 2 * (MAXT + 1)) * (int)sizeof(T));
     });
     k_tb2<T, MAT, PM, MAXT><<<tt.per * 8, threads, lds, st>>>(g, ch, ce, ar, pm, ps, tt, pp);
 }

@@ -394,7 +394,9 @@ __device__ __forceinline__ void tb_load(const TbArrays<T> &a, int nz, unsigned u
 template <int V> using tb_par = std::integral_constant<int, V>;
 
 // LDS: nine exchange planes (H1, E1, H2 x 3 components) per plane parity,
-// blockDim.x entries each, passed as the launch's dynamic shared memory.
+// MAXT entries each (compile-time offsets, so that one address register
+// serves most slots), after a pad of txw + 1 entries that the out-of-tile
+// neighbours of halo lanes may read; passed as dynamic shared memory.
 // PM: CPML axes the launch's tiles can meet (bit 0 x, bit 1 y, bit 2 z; 0:
 // none), so that the instantiations for the CPML-free interior and for the
 // slabs of the CPML shell carry only the psi of their own axes.
@@ -406,8 +408,8 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
 {
     constexpr bool PML = PM != 0;
     extern __shared__ __align__(16) unsigned char tb_lds[];
-    T *const L = reinterpret_cast<T *>(tb_lds);
-    const int NTP = blockDim.x;
+    constexpr int NTP = MAXT;
+    T *const L = reinterpret_cast<T *>(tb_lds) + tt.txw + 1;
     // XCD bands: workgroup b runs on XCD b % 8, which takes the contiguous
     // tile range [c per, (c+1) per), so that halo rows stay in its L2
     const int lb = (blockIdx.x & 7) * tt.per + (blockIdx.x >> 3);
@@ -431,11 +433,9 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     // stored: the box's cells of the update box and of its ghost layer n+1
     const bool core = ing && tx >= 2 && tx <= tt.txw - 2 && ty >= 2 && ty <= tt.tyw - 2 && I < bx.i0 + bx.ni &&
                       J < bx.j0 + bx.nj;
-    // LDS neighbours (clamped: out-of-tile neighbours only feed invalid values)
-    const int tim = t >= 1 ? t - 1 : t;
-    const int tip = t + 1 < NTP ? t + 1 : t;
-    const int tjm = t >= tt.txw ? t - tt.txw : t;
-    const int tjp = t + tt.txw < NTP ? t + tt.txw : t;
+    // LDS neighbours (out-of-tile ones, in the pads or other slots, only feed
+    // halo lanes, whose values are invalid anyway)
+    const int tim = t - 1, tip = t + 1, tjm = t - tt.txw, tjp = t + tt.txw;
     const int K0 = bx.k0 + kc * tt.lz;
     const int K1 = min(K0 + tt.lz, bx.k0 + bx.nk);
     const int Ks = max(K0 - 2, 0);

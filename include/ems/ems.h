@@ -122,9 +122,10 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
  * cols, rows >= 5 and cols * rows <= 1024 (512 in FP64; a larger FP64 tile
  * makes ems_fdtd_run fail with EMS_ERR_INVALID_ARG), its stored core
  * (cols - 3) x (rows - 3), and planes per workgroup (>= 1).  0 for
- * cols = rows or planes selects the default (FP32 68 x 15, FP64 36 x 14,
- * 12 planes).  With a CPML the tile applies to the CPML-free interior and the
- * z-slabs of the shell; the x- and y-slabs take narrow tiles of their own.
+ * cols = rows or planes selects the default (FP32 64 x 16 and 12 planes,
+ * FP64 64 x 8 and 16 planes).  With a CPML the tile applies to the CPML-free
+ * interior, and to the z-slabs of the shell if it has at most 512 threads
+ * (36 x 14 otherwise); the x- and y-slabs take narrow tiles of their own.
  * Results do not depend on it.  No reference counterpart. */
 ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes);
 

@@ -130,7 +130,8 @@ def test_zmarch_tilings_agree_bitwise_with_split_fp32(rows, planes, waves):
     assert np.array_equal(a.traces(), b.traces())
 
 
-@pytest.mark.parametrize("cols,rows,planes", [(5, 5, 1), (68, 11, 32), (20, 7, 3), (40, 25, 1000), (13, 9, 2)])
+@pytest.mark.parametrize("cols,rows,planes", [(5, 5, 1), (68, 11, 32), (20, 7, 3), (40, 25, 1000), (13, 9, 2),
+                                                (64, 5, 3), (64, 16, 12)])
 @pytest.mark.parametrize("name", ["lossy_cpml_rect", "pec_rect_nonsquare"])
 def test_tb2_tilings_agree_bitwise_with_split_fp32(name, cols, rows, planes):
     """The two-step sweep's tiles and chunks change only its halo
