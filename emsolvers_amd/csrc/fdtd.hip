@@ -931,37 +931,31 @@ ems_status run_finish(ems_fdtd *s, const RunPlan &rp)
     return EMS_OK;
 }
 
-// Runs num_steps steps; in timed mode records HIP events around the Yee
-// kernels of every step and around the whole sequence.
-template <typename T>
-ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+// Checks of a run on the two-step kernel (tile and array size).
+template <typename T> ems_status tb2_check(const ems_fdtd *s)
 {
-    RunPlan rp;
-    EMS_TRY(run_prepare<T>(s, num_steps, rp, true));
-    if (num_steps <= 0) return EMS_OK;
+    if (kernel_kind(s) != EMS_KERNEL_TB2) return EMS_OK;
+    int txw, tyw, lz;
+    tb_tile<T>(s, txw, tyw, lz);
+    if (txw * tyw > tb_max_threads<T>()) return invalid("two-step tile: at most 512 threads in FP64");
+    if (!tb2_fits(s)) return invalid("two-step kernel: a field array reaches 4 GiB (32-bit offsets); use the split sweeps");
+    return EMS_OK;
+}
+
+// The launches of a prepared run, all on the handle's stream (the two-step
+// kernel's CPML shell also on its side streams, joined back).  ev: when not
+// null, 2 (num_steps + 1) events; [2 l, 2 l + 1] are recorded around the Yee
+// launch(es) of launch group l (one step, or two for the two-step kernel).
+template <typename T>
+ems_status run_loop(ems_fdtd *s, int num_steps, const RunPlan &rp, std::vector<hipEvent_t> *ev, int &ngroups)
+{
     const int tf = s->d.tf, nsrc = rp.nsrc, npts = rp.npts;
     const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
     const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
-    const bool timed = total_ms != nullptr;
     const bool tb2 = kernel_kind(s) == EMS_KERNEL_TB2;
-    if (tb2) {
-        int txw, tyw, lz;
-        tb_tile<T>(s, txw, tyw, lz);
-        if (txw * tyw > tb_max_threads<T>()) return invalid("two-step tile: at most 512 threads in FP64");
-        if (!tb2_fits(s))
-            return invalid("two-step kernel: a field array reaches 4 GiB (32-bit offsets); use the split sweeps");
-    }
-    // events: [2 l, 2 l + 1] around the Yee launch(es) of launch group l
-    // (one step, or two for the two-step kernel); the last pair brackets all
-    std::vector<hipEvent_t> ev;
-    int ngroups = 0;
-    if (timed) {
-        ev.resize(2 * (size_t)num_steps + 2, nullptr);
-        for (auto &e : ev) EMS_HIP(hipEventCreate(&e));
-        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps], s->stream));
-    }
+    ngroups = 0;
     for (int step = 1; step <= num_steps;) {
-        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups], s->stream));
+        if (ev) EMS_HIP(hipEventRecord((*ev)[2 * (size_t)ngroups], s->stream));
         const PmlView<T> pm = pml_view<T>(s);
         if (tb2 && step + 1 <= num_steps) {
             // steps `step` and `step + 1`, sources and receivers inside
@@ -983,7 +977,7 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
             if (npts && step % tf == 0) pp.rec1 = s->d_rec.as<double>() + (size_t)(step / tf - 1) * npts;
             if (npts && (step + 1) % tf == 0) pp.rec2 = s->d_rec.as<double>() + (size_t)((step + 1) / tf - 1) * npts;
             EMS_TRY(launch_tb2<T>(s, ch, ce, pm, pp));
-            if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups + 1], s->stream));
+            if (ev) EMS_HIP(hipEventRecord((*ev)[2 * (size_t)ngroups + 1], s->stream));
             EMS_HIP(hipGetLastError());
             ++ngroups;
             step += 2;
@@ -991,7 +985,7 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         }
         launch_step<T>(s, ch, ce, pm);
         const Fields<T> f = fields<T>(s);
-        if (timed) EMS_HIP(hipEventRecord(ev[2 * (size_t)ngroups + 1], s->stream));
+        if (ev) EMS_HIP(hipEventRecord((*ev)[2 * (size_t)ngroups + 1], s->stream));
         EMS_HIP(hipGetLastError());
         ++ngroups;
         double *rec = nullptr;
@@ -1004,6 +998,28 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
         }
         ++step;
     }
+    return EMS_OK;
+}
+
+// Runs num_steps steps; in timed mode records HIP events around the Yee
+// kernels of every step and around the whole sequence.
+template <typename T>
+ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel_ms)
+{
+    RunPlan rp;
+    EMS_TRY(run_prepare<T>(s, num_steps, rp, true));
+    if (num_steps <= 0) return EMS_OK;
+    EMS_TRY(tb2_check<T>(s));
+    const bool timed = total_ms != nullptr;
+    // events: [2 l, 2 l + 1] around launch group l; the last pair brackets all
+    std::vector<hipEvent_t> ev;
+    int ngroups = 0;
+    if (timed) {
+        ev.resize(2 * (size_t)num_steps + 2, nullptr);
+        for (auto &e : ev) EMS_HIP(hipEventCreate(&e));
+        EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps], s->stream));
+    }
+    EMS_TRY(run_loop<T>(s, num_steps, rp, timed ? &ev : nullptr, ngroups));
     if (timed) {
         EMS_HIP(hipEventRecord(ev[2 * (size_t)num_steps + 1], s->stream));
         EMS_HIP(hipEventSynchronize(ev[2 * (size_t)num_steps + 1]));
@@ -1879,85 +1895,133 @@ template <typename T> ems_status batch_modes(ems_batch *b, BatchModes &md)
     return EMS_OK;
 }
 
+// Restores a batch's member list when the batched sweeps of a subset end.
+struct MemberSubset {
+    ems_batch *b;
+    std::vector<ems_fdtd *> all;
+    ~MemberSubset() { b->mem = all; }
+};
+
+// Members whose single runs take the two-step sweep (grids with a large
+// CPML-free interior) keep it in a batch: each runs its own launch sequence
+// on its own stream, forked from and joined back into the batch stream,
+// concurrently with the others.
+bool batch_own(const ems_fdtd *s, int num_steps) { return num_steps >= 2 && kernel_kind(s) == EMS_KERNEL_TB2; }
+
+// A batch run: the members that keep the two-step sweep on their own streams,
+// the others through the batched sweeps (one launch per sweep for all of
+// them) on the batch stream.  Every member's fields and traces equal its
+// single run's bit for bit.
 template <typename T> ems_status batch_run_impl(ems_batch *b, int num_steps, double *total_ms)
 {
-    const int nm = (int)b->mem.size();
-    std::vector<RunPlan> rp(nm);
-    for (int q = 0; q < nm; ++q) EMS_TRY(run_prepare<T>(b->mem[q], num_steps, rp[q], false));
-    if (num_steps <= 0 || nm == 0) {
+    std::vector<ems_fdtd *> all = b->mem, own, shared;
+    for (auto *s : all) (batch_own(s, num_steps) ? own : shared).push_back(s);
+    std::vector<RunPlan> rp_own(own.size()), rp(shared.size());
+    for (size_t q = 0; q < own.size(); ++q) {
+        EMS_TRY(run_prepare<T>(own[q], num_steps, rp_own[q], true));
+        EMS_TRY(tb2_check<T>(own[q]));
+    }
+    for (size_t q = 0; q < shared.size(); ++q) EMS_TRY(run_prepare<T>(shared[q], num_steps, rp[q], false));
+    if (num_steps <= 0 || all.empty()) {
         if (total_ms) *total_ms = 0.0;
         return EMS_OK;
     }
+    // the shared members' descriptors (before any launch: these copies are
+    // synchronous)
+    const int nm = (int)shared.size();
     BatchModes md;
-    EMS_TRY(batch_modes<T>(b, md));
-    std::vector<BatchMember<T>> ms;
     long total = 0;
-    EMS_TRY(build_members<T>(b, md.pml, ms, total));
-    std::vector<BatchPost> post(nm);
-    for (int q = 0; q < nm; ++q) {
-        ems_fdtd *s = b->mem[q];
-        BatchPost &p = post[q];
-        p.nsrc = rp[q].nsrc;
-        p.src_comp = s->d_src_comp.as<int>();
-        p.src_u = s->d_src_u.as<long>();
-        p.src_val = s->d_src_val.as<double>();
-        p.npts = rp[q].npts;
-        p.pt_comp = s->d_pt_comp.as<int>();
-        p.pt_u = s->d_pt_u.as<long>();
-        p.rec = (rp[q].npts && rp[q].nrec) ? s->d_rec.as<double>() : nullptr;
-        p.tf = s->d.tf;
-        p.step0 = 0;
-    }
-    EMS_TRY(b->d_post.reserve(sizeof(BatchPost) * nm));
-    EMS_HIP(hipMemcpy(b->d_post.p, post.data(), sizeof(BatchPost) * nm, hipMemcpyHostToDevice));
-    EMS_TRY(b->d_step.reserve(sizeof(int)));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (total_ms) {
-        EMS_HIP(hipEventCreate(&e0));
-        EMS_HIP(hipEventCreate(&e1));
-        EMS_HIP(hipEventRecord(e0, b->stream));
-    }
-    const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
-    const BatchPost *dp = b->d_post.as<BatchPost>();
-    int done = 0;
-    const int S = b->graph_steps;
-    if (S > 0 && num_steps >= S) {
-        // one captured graph of S steps, replayed; the step index lives on the device
+    if (nm > 0) {
+        MemberSubset subset{b, all};
+        b->mem = shared;
+        EMS_TRY(batch_modes<T>(b, md));
+        std::vector<BatchMember<T>> ms;
+        EMS_TRY(build_members<T>(b, md.pml, ms, total));
+        std::vector<BatchPost> post(nm);
+        for (int q = 0; q < nm; ++q) {
+            ems_fdtd *s = shared[q];
+            BatchPost &p = post[q];
+            p.nsrc = rp[q].nsrc;
+            p.src_comp = s->d_src_comp.as<int>();
+            p.src_u = s->d_src_u.as<long>();
+            p.src_val = s->d_src_val.as<double>();
+            p.npts = rp[q].npts;
+            p.pt_comp = s->d_pt_comp.as<int>();
+            p.pt_u = s->d_pt_u.as<long>();
+            p.rec = (rp[q].npts && rp[q].nrec) ? s->d_rec.as<double>() : nullptr;
+            p.tf = s->d.tf;
+            p.step0 = 0;
+        }
+        EMS_TRY(b->d_post.reserve(sizeof(BatchPost) * nm));
+        EMS_HIP(hipMemcpy(b->d_post.p, post.data(), sizeof(BatchPost) * nm, hipMemcpyHostToDevice));
+        EMS_TRY(b->d_step.reserve(sizeof(int)));
         const int one = 1;
         EMS_HIP(hipMemcpy(b->d_step.p, &one, sizeof(int), hipMemcpyHostToDevice));
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        EMS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int q = 0; q < S; ++q) {
-            batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
-            k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, 0, b->d_step.as<int>());
-            k_step_advance<<<1, 64, 0, b->stream>>>(b->d_step.as<int>());
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> joined(own.size(), nullptr);
+    struct Events {
+        hipEvent_t &e0, &e1;
+        std::vector<hipEvent_t> &j;
+        ~Events()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            for (auto e : j)
+                if (e) (void)hipEventDestroy(e);
         }
-        EMS_HIP(hipStreamEndCapture(b->stream, &graph));
-        EMS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        const int reps = num_steps / S;
-        for (int r = 0; r < reps; ++r) EMS_HIP(hipGraphLaunch(exec, b->stream));
-        done = reps * S;
-        EMS_HIP(hipStreamSynchronize(b->stream));
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+    } events{e0, e1, joined};
+    EMS_HIP(hipEventCreate(&e0));
+    if (total_ms) EMS_HIP(hipEventCreate(&e1));
+    for (auto &e : joined) EMS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    EMS_HIP(hipEventRecord(e0, b->stream));
+    for (size_t q = 0; q < own.size(); ++q) {
+        EMS_HIP(hipStreamWaitEvent(own[q]->stream, e0, 0));
+        int ngroups = 0;
+        EMS_TRY(run_loop<T>(own[q], num_steps, rp_own[q], nullptr, ngroups));
+        EMS_HIP(hipEventRecord(joined[q], own[q]->stream));
     }
-    for (int step = done + 1; step <= num_steps; ++step) {
-        batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
-        k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, step, nullptr);
+    if (nm > 0) {
+        const BatchMember<T> *dm = b->d_mem.as<BatchMember<T>>();
+        const BatchPost *dp = b->d_post.as<BatchPost>();
+        int done = 0;
+        const int S = b->graph_steps;
+        if (S > 0 && num_steps >= S) {
+            // one captured graph of S steps, replayed; the step index lives on the device
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            EMS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < S; ++q) {
+                batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
+                k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, 0, b->d_step.as<int>());
+                k_step_advance<<<1, 64, 0, b->stream>>>(b->d_step.as<int>());
+            }
+            EMS_HIP(hipStreamEndCapture(b->stream, &graph));
+            EMS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            const int reps = num_steps / S;
+            for (int r = 0; r < reps; ++r) EMS_HIP(hipGraphLaunch(exec, b->stream));
+            done = reps * S;
+            EMS_HIP(hipStreamSynchronize(b->stream));
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+        }
+        for (int step = done + 1; step <= num_steps; ++step) {
+            batch_sweeps_any<T>(b, md.mat, md.pml, total, nm);
+            k_post_batch<T><<<nm, 256, 0, b->stream>>>(dm, dp, step, nullptr);
+        }
+        EMS_HIP(hipGetLastError());
     }
-    EMS_HIP(hipGetLastError());
+    for (auto e : joined) EMS_HIP(hipStreamWaitEvent(b->stream, e, 0));
     if (total_ms) {
         EMS_HIP(hipEventRecord(e1, b->stream));
         EMS_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
         EMS_HIP(hipEventElapsedTime(&ms, e0, e1));
         *total_ms = ms;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     }
     EMS_HIP(hipStreamSynchronize(b->stream));
-    for (int q = 0; q < nm; ++q) EMS_TRY(run_finish(b->mem[q], rp[q]));
+    for (size_t q = 0; q < own.size(); ++q) EMS_TRY(run_finish(own[q], rp_own[q]));
+    for (size_t q = 0; q < shared.size(); ++q) EMS_TRY(run_finish(shared[q], rp[q]));
     return EMS_OK;
 }
 
@@ -1966,8 +2030,19 @@ template <typename T> ems_status batch_run_impl(ems_batch *b, int num_steps, dou
 // zero inputs stay zero) and keeps the fastest.
 template <typename T> ems_status batch_autotune_impl(ems_batch *b, int trial, int *rows, int *planes)
 {
+    if (b->mem.empty()) return invalid("empty batch");
+    // only the members that run through the batched sweeps
+    MemberSubset subset{b, b->mem};
+    std::vector<ems_fdtd *> shared;
+    for (auto *s : subset.all)
+        if (!batch_own(s, 2)) shared.push_back(s);
+    b->mem = shared;
     const int nm = (int)b->mem.size();
-    if (nm == 0) return invalid("empty batch");
+    if (nm == 0) {
+        if (rows) *rows = b->by;
+        if (planes) *planes = b->lz;
+        return EMS_OK;
+    }
     BatchModes md;
     EMS_TRY(batch_modes<T>(b, md));
     size_t fmax = 0, pmax = 0;

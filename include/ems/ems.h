@@ -242,7 +242,11 @@ ems_status ems_fdtd_gradient_plan(ems_fdtd *s, int num_steps, int *interval1, in
  * No reference counterpart (the reference runs one shot per process).  A
  * batch advances many FDTD handles (shots and models of uneven size, one
  * precision) together: one launch per sweep covers every member, on the
- * batch's own stream.  Each member keeps its own sources, receivers,
+ * batch's own stream, except for members whose single runs take the
+ * two-step sweep (EMS_KERNEL_TB2 set, or chosen by EMS_KERNEL_AUTO on grids
+ * with a large CPML-free interior): these run their own launch sequences,
+ * forked from and joined back into the batch stream, concurrently with the
+ * batched sweeps.  Each member keeps its own sources, receivers,
  * material, CPML and wavelet, and after ems_batch_run(b, n) it is in exactly
  * the state ems_fdtd_run(member, n) would have left it in (same traces and
  * fields, bit for bit).  Members must outlive the batch. */

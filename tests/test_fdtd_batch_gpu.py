@@ -90,6 +90,31 @@ def test_batch_members_equal_single_runs(precision, graph):
                 assert np.array_equal(mem[q].field(w), o.field(w)), (q, w)
 
 
+@pytest.mark.parametrize("precision,graph", [("f64", 0), ("f32", 8)])
+def test_batch_members_with_their_own_two_step_runs(precision, graph):
+    """Members set to the two-step sweep run it on their own streams next to
+    the batched sweeps of the others: every member equals its single run with
+    the same settings bit for bit, over runs of odd and even length."""
+    from emsolvers_amd.fdtd import Batch, KERNEL_TB2
+    print("batch with two-step members", precision, "graph", graph, flush=True)
+    ref = build_all(precision)
+    mem = build_all(precision)
+    for s in (ref[0], ref[3], mem[0], mem[3]):
+        s.set_kernel(KERNEL_TB2)
+    for s in ref:
+        s.run(37)
+        s.run(23)
+    b = Batch(mem)
+    b.set_graph(graph)
+    b.run(37)
+    b.run(23)
+    for q, (s, r) in enumerate(zip(mem, ref)):
+        assert np.array_equal(s.traces(), r.traces()), q
+        for w in range(6):
+            assert np.array_equal(s.field(w), r.field(w)), (q, w)
+        assert s.current_time == r.current_time
+
+
 def test_autotune_keeps_results():
     from emsolvers_amd.fdtd import Batch
     ref = build_all("f32")
