@@ -99,15 +99,17 @@ struct ems_fdtd {
 
 namespace {
 
-GridView make_grid(int nx, int ny, int nz)
+// The unified grid of an nx x ny x nz handle whose cell I = 0 lies `shift`
+// elements into a 128-byte line (rows keep the line-aligned pitch).
+GridView make_grid(int nx, int ny, int nz, long shift = 0)
 {
     GridView g;
     g.nx = nx;
     g.ny = ny;
     g.nz = nz;
-    g.pitch = ((long)nx + 2 + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
+    g.pitch = ((long)nx + 2 + shift + kPitchAlign - 1) / kPitchAlign * kPitchAlign;
     g.slab = g.pitch * (ny + 2);
-    g.off = kFrontPad;
+    g.off = kFrontPad + shift;
     return g;
 }
 
@@ -263,6 +265,62 @@ ems_status ensure_alt(ems_fdtd *s)
         EMS_HIP(hipMemcpy(s->f2[w].p, s->f[w].p, s->nalloc * s->esz, hipMemcpyDeviceToDevice));
     }
     s->alt_ready = true;
+    return EMS_OK;
+}
+
+// Row shift for the two-step kernel's CPML x-slabs (FP32): their tiles read
+// the columns [0, P+5) and [nx-P-2, nx+2) of every row; the shift puts each
+// range into as few 128-byte lines as possible (at 256^3 with P = 10, one
+// line each instead of one and two: the x-slabs fetched 341 MB per two steps
+// unshifted).  FP64 rows (16 words per line) gain nothing from a shift.
+long row_shift(int nx, int P, size_t esz)
+{
+    if (P <= 0 || esz != 4) return 0;
+    auto lines = [](long a, long b) { return (b - 1) / kPitchAlign - a / kPitchAlign + 1; };
+    long best = 0, bl = 1L << 30;
+    for (long sh = 0; sh < kPitchAlign; ++sh) {
+        const long l = lines(sh, sh + P + 5) + lines(sh + nx - P - 2, sh + nx + 2);
+        if (l < bl) {
+            bl = l;
+            best = sh;
+        }
+    }
+    return best;
+}
+
+// Moves every unified array of the handle to the layout with row shift
+// `shift` (cells I in [0, nx+1] of every row; pads zeroed).  Cached unified
+// indices and the adjoint buffers are rebuilt.
+ems_status relayout(ems_fdtd *s, long shift)
+{
+    const GridView g0 = s->g;
+    if (g0.off - kFrontPad == shift) return EMS_OK;
+    const GridView g1 = make_grid(g0.nx, g0.ny, g0.nz, shift);
+    const size_t n1 = (size_t)(g1.off + g1.slab * (g0.nz + 2) + kFrontPad);
+    const size_t e = s->esz;
+    auto move = [&](DevBuf &b) -> ems_status {
+        if (!b.p) return EMS_OK;
+        DevBuf nb;
+        EMS_TRY(nb.reserve(n1 * e));
+        EMS_HIP(hipMemset(nb.p, 0, n1 * e));
+        EMS_HIP(hipMemcpy2D(static_cast<char *>(nb.p) + g1.off * e, g1.pitch * e,
+                            static_cast<const char *>(b.p) + g0.off * e, g0.pitch * e, (size_t)(g0.nx + 2) * e,
+                            (size_t)(g0.ny + 2) * (g0.nz + 2), hipMemcpyDeviceToDevice));
+        b = std::move(nb);
+        return EMS_OK;
+    };
+    for (int w = 0; w < 6; ++w) {
+        EMS_TRY(move(s->f[w]));
+        EMS_TRY(move(s->f2[w]));
+    }
+    EMS_TRY(move(s->ca));
+    EMS_TRY(move(s->cb));
+    s->g = g1;
+    s->nalloc = n1;
+    for (auto &q : s->src) q.u = ref_to_unified(s->g, q.comp, q.i, q.j, q.k);
+    s->tiles_ok = false;
+    delete s->adj;
+    s->adj = nullptr;
     return EMS_OK;
 }
 
@@ -2631,8 +2689,8 @@ ems_status ems_fdtd_create(const ems_fdtd_desc *desc, ems_fdtd **out)
     if (!(desc->h > 0) || !(desc->dt > 0) || !(desc->eps > 0) || !(desc->mu > 0))
         return invalid("h, dt, eps and mu must be positive");
     if (desc->precision != EMS_F64 && desc->precision != EMS_F32) return invalid("unknown precision");
-    {  // the split sweeps address a plane with 32-bit byte offsets
-        const GridView g = make_grid(desc->nx, desc->ny, desc->nz);
+    {  // the split sweeps address a plane with 32-bit byte offsets (any row shift)
+        const GridView g = make_grid(desc->nx, desc->ny, desc->nz, kPitchAlign - 1);
         if ((double)g.slab * (desc->precision == EMS_F32 ? 4 : 8) >= 4294967296.0)
             return invalid("grid plane too large (one z-plane must be under 4 GiB)");
     }
@@ -2737,7 +2795,8 @@ ems_status ems_fdtd_set_cpml(ems_fdtd *s, int ncells, double order, double r0, d
     s->pml_kmax = kappa_max;
     s->pml_amax = alpha_max;
     s->dirty_pml = true;
-    return EMS_OK;
+    // the layout the two-step kernel's CPML slabs prefer, when it runs
+    return relayout(s, kernel_kind(s) == EMS_KERNEL_TB2 ? row_shift(s->g.nx, ncells, s->esz) : 0);
 }
 
 ems_status ems_fdtd_set_wavelet(ems_fdtd *s, int kind, double f0, double t0, double amp)

@@ -196,6 +196,40 @@ def test_tb2_ghost_cells_and_sources_there(precision, npml):
     assert np.array_equal(sims[0].traces(), sims[1].traces())
 
 
+def test_cpml_row_shift_moves_state_set_before():
+    """An FP32 handle on the two-step sweep moves its arrays to the row shift
+    of its CPML x-slabs when the CPML is set (34 columns and 4 cells: cell
+    I = 0 four words into a line); material, fields and sources set before
+    move with them.  Read-back fields equal those of a handle that got its
+    CPML first, and so do the fields and traces of the run."""
+    grid = (34, 30, 28)
+    nx, ny, nz = grid
+    h, dt = 1e-3, fc.cfl_dt(1e-3, 0.9)
+    rng = np.random.default_rng(21)
+    nm = (nx + 1) * (ny + 1) * (nz + 1)
+    er, sg = 1 + 3 * rng.uniform(0, 1, nm), rng.uniform(0, 0.5, nm)
+    sims = [gpu_sim(grid, h, dt, fc.EPS0, fc.MU0, 1, "f32", 4) for _ in range(2)]
+    vals = [rng.standard_normal(sims[0].field(w).size) for w in range(6)]
+    for q, s in enumerate(sims):
+        if q == 1:
+            s.set_cpml(4, 3.0, 1e-6, 1.5, 0.02)
+        s.set_material(er, sg)
+        for w in range(6):
+            s.set_field(w, vals[w])
+        s.setSource_z((17, 15, 14))
+        s.setSource_x((nx, 3, 4))
+        s.setObservationPoint([[(5, 5, 5)], [(30, 20, 10)], [(17, 15, 14)]])
+        if q == 0:
+            s.set_cpml(4, 3.0, 1e-6, 1.5, 0.02)
+    for w in range(6):
+        assert np.array_equal(sims[0].field(w), sims[1].field(w)), NAMES[w]
+    for s in sims:
+        s.run(9)
+    for w in range(6):
+        assert np.array_equal(sims[0].field(w), sims[1].field(w)), NAMES[w]
+    assert np.array_equal(sims[0].traces(), sims[1].traces())
+
+
 @pytest.mark.parametrize("rows,planes,flags", [(8, 1, 0), (8, 7, 1), (8, 8, 2), (16, 3, 3), (16, 1000, 1)])
 def test_fused_tilings_are_bitwise_identical_to_oracle(oracle_runs, rows, planes, flags):
     """Tile shape and K-chunking change only the halo recomputation, never the
