@@ -619,6 +619,10 @@ template <typename T> void tb_tile(const ems_fdtd *s, int &txw, int &tyw, int &l
 //     x-slabs  the other planes, I < P+3 or I > n-P-1, all rows   (x, y)
 //     y-slabs  the other planes and columns, J < P+3 or J > n-P-1 (y)
 // Every cell of the update box is stored by exactly one launch.
+// x-halo of a tile of width txw in the interior or z-slab launches: 4 for
+// one wavefront per row (I+1 neighbours by DPP), else 3
+inline int tb_hx(int txw) { return txw == 64 ? 4 : 3; }
+
 struct TbLaunch {
     int pm;  // CPML axes (bit 0 x, 1 y, 2 z)
     TbTiles tt;
@@ -632,7 +636,9 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
     // shell launches use workgroups of at most 512 threads (so that their
     // CPML instantiations get 256 registers)
     const int maxt = 512;
-    auto box = [&](int i0, int i1, int j0, int j1, int k0, int k1, int bw, int bh, int bz) {
+    // a tile row of one wavefront (64 columns, the interior and z-slabs) has
+    // a two-column high halo in x
+    auto box = [&](int i0, int i1, int j0, int j1, int k0, int k1, int bw, int bh, int bz, int hx = 3) {
         TbBox b{};
         b.i0 = i0;
         b.j0 = j0;
@@ -640,7 +646,7 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
         b.ni = i1 - i0;
         b.nj = j1 - j0;
         b.nk = k1 - k0;
-        b.ntx = (b.ni + bw - 4) / (bw - 3);
+        b.ntx = (b.ni + bw - hx - 1) / (bw - hx);
         b.nty = (b.nj + bh - 4) / (bh - 3);
         b.nkc = (b.nk + bz - 1) / bz;
         b.count = b.ni > 0 && b.nj > 0 && b.nk > 0 ? b.ntx * b.nty * b.nkc : 0;
@@ -665,7 +671,7 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
     const int n1x = g.nx + 2, n1y = g.ny + 2, n1z = g.nz + 2;
     const TbBox none{};
     if (P == 0) {
-        v.push_back(launch(0, txw, tyw, lz, box(0, n1x, 0, n1y, 0, n1z, txw, tyw, lz), none));
+        v.push_back(launch(0, txw, tyw, lz, box(0, n1x, 0, n1y, 0, n1z, txw, tyw, lz, tb_hx(txw)), none));
         return v;
     }
     // interior cores [P+3, n-P-1] on each axis
@@ -673,14 +679,15 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
     const int ex = g.nx - P, ey = g.ny - P, ez = g.nz - P;  // one past the interior cores
     if (ex <= m || ey <= m || ez <= m) {
         const int zw = txw * tyw <= maxt ? txw : 36, zh = txw * tyw <= maxt ? tyw : 14;
-        v.push_back(launch(7, zw, zh, lz, box(0, n1x, 0, n1y, 0, n1z, zw, zh, lz), none));
+        v.push_back(launch(7, zw, zh, lz, box(0, n1x, 0, n1y, 0, n1z, zw, zh, lz, tb_hx(zw)), none));
         return v;
     }
-    v.push_back(launch(0, txw, tyw, lz, box(m, ex, m, ey, m, ez, txw, tyw, lz), none));
+    v.push_back(launch(0, txw, tyw, lz, box(m, ex, m, ey, m, ez, txw, tyw, lz, tb_hx(txw)), none));
     // z-slabs: one chunk per slab
     const int zl = std::max(m, n1z - ez);
     const int zw = txw * tyw <= maxt ? txw : 36, zh = txw * tyw <= maxt ? tyw : 14;
-    v.push_back(launch(7, zw, zh, zl, box(0, n1x, 0, n1y, 0, m, zw, zh, zl), box(0, n1x, 0, n1y, ez, n1z, zw, zh, zl)));
+    v.push_back(launch(7, zw, zh, zl, box(0, n1x, 0, n1y, 0, m, zw, zh, zl, tb_hx(zw)),
+                       box(0, n1x, 0, n1y, ez, n1z, zw, zh, zl, tb_hx(zw))));
     // x-slabs: narrow tiles across I
     const int xw = std::max(m, n1x - ex) + 3;
     const int xh = std::min(64, maxt / xw);
@@ -695,7 +702,7 @@ template <typename T> std::vector<TbLaunch> tb_plan(const ems_fdtd *s)
 // Two steps in one advance (EMS_KERNEL_TB2): every launch reads the current
 // sets and writes the other field set and the other psi set (all twelve
 // slabs), which become current.
-template <typename T, int MAT, int PM, int MAXT>
+template <typename T, int MAT, int PM, int MAXT, bool XL>
 void launch_tb2_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &pp, const TbTiles &tt,
                   hipStream_t st)
 {
@@ -733,11 +740,11 @@ void launch_tb2_t(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &p
     const size_t lds = ((size_t)18 * MAXT + 2 * (tt.txw + 1)) * sizeof(T);
     static std::once_flag attr;  // per instantiation: allow the full exchange size
     std::call_once(attr, [] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2<T, MAT, PM, MAXT>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tb2<T, MAT, PM, MAXT, XL>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (18 * MAXT + 2 * (MAXT + 1)) * (int)sizeof(T));
     });
-    k_tb2<T, MAT, PM, MAXT><<<tt.per * 8, threads, lds, st>>>(g, ch, ce, ar, pm, ps, tt, pp);
+    k_tb2<T, MAT, PM, MAXT, XL><<<tt.per * 8, threads, lds, st>>>(g, ch, ce, ar, pm, ps, tt, pp);
 }
 
 template <typename T, int MAT>
@@ -746,11 +753,18 @@ void launch_tb2_m(ems_fdtd *s, T ch, T ce, const PmlView<T> &pm, const TbPost &p
     // the CPML-free interior of an FP32 grid may use workgroups of 1024
     // threads; everything else at most 512
     constexpr int M0 = sizeof(T) == 4 ? 1024 : 512;
+    const bool xl = tb_hx(l.tt.txw) == 4;
     switch (l.pm) {
-    case 0: launch_tb2_t<T, MAT, 0, M0>(s, ch, ce, pm, pp, l.tt, st); break;
-    case 2: launch_tb2_t<T, MAT, 2, 512>(s, ch, ce, pm, pp, l.tt, st); break;
-    case 3: launch_tb2_t<T, MAT, 3, 512>(s, ch, ce, pm, pp, l.tt, st); break;
-    default: launch_tb2_t<T, MAT, 7, 512>(s, ch, ce, pm, pp, l.tt, st); break;
+    case 0:
+        if (xl) launch_tb2_t<T, MAT, 0, M0, true>(s, ch, ce, pm, pp, l.tt, st);
+        else launch_tb2_t<T, MAT, 0, M0, false>(s, ch, ce, pm, pp, l.tt, st);
+        break;
+    case 2: launch_tb2_t<T, MAT, 2, 512, false>(s, ch, ce, pm, pp, l.tt, st); break;
+    case 3: launch_tb2_t<T, MAT, 3, 512, false>(s, ch, ce, pm, pp, l.tt, st); break;
+    default:
+        if (xl) launch_tb2_t<T, MAT, 7, 512, true>(s, ch, ce, pm, pp, l.tt, st);
+        else launch_tb2_t<T, MAT, 7, 512, false>(s, ch, ce, pm, pp, l.tt, st);
+        break;
     }
 }
 

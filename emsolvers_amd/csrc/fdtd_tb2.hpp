@@ -17,7 +17,10 @@
 // they never store.  Validity shrinks level by level:
 //     H1 everywhere (E^n at I+1, J+1 comes from global memory),
 //     E1 for tx, ty >= 1,  H2 for 1 <= tx, ty <= max-1,  E2 for 2 <= tx, ty <= max-1,
-// and invalid values are never stored nor fed into valid ones.
+// and invalid values are never stored nor fed into valid ones.  With XL (64
+// columns: one wavefront per tile row) E^n at I+1 is the next lane's value
+// (DPP), two loads fewer per plane; lane 63 then has no valid H1, so the
+// high halo is two columns: core tx in [2, 61], tiles step by 60.
 //
 // K march.  At plane K a thread computes H1(K), E1(K) and, lagging behind so
 // that one barrier per plane suffices, H2(K-1) and E2(K-2).  The LDS holds,
@@ -357,7 +360,7 @@ template <typename T> struct TbArrays {
 // for it where it is issued, which would serialise the march.
 // ui, ui_i1, ui_j1: the lane's cell and its I+1 / J+1 neighbours in a plane,
 // clamped to the allocated layer n+1.
-template <typename T, int MAT>
+template <typename T, int MAT, bool XL>
 __device__ __forceinline__ void tb_load(const TbArrays<T> &a, int nz, unsigned ui, unsigned ui_i1, unsigned ui_j1,
                                         int K, TbSet<T> &v)
 {
@@ -373,8 +376,12 @@ __device__ __forceinline__ void tb_load(const TbArrays<T> &a, int nz, unsigned u
     v.ez = lane_at(a.ez, u0);
     v.ex1 = lane_at(a.ex, u1);
     v.ey1 = lane_at(a.ey, u1);
-    v.ez_i1 = lane_at(a.ez, ui_i1 + pk);
-    v.ey_i1 = lane_at(a.ey, ui_i1 + pk);
+    if (XL) {  // from the next lane's ez, ey
+        v.ez_i1 = v.ey_i1 = (T)0;
+    } else {
+        v.ez_i1 = lane_at(a.ez, ui_i1 + pk);
+        v.ey_i1 = lane_at(a.ey, ui_i1 + pk);
+    }
     v.ez_j1 = lane_at(a.ez, ui_j1 + pk);
     v.ex_j1 = lane_at(a.ex, ui_j1 + pk);
     v.hx = lane_at(a.hx, u0);
@@ -402,7 +409,7 @@ template <int V> using tb_par = std::integral_constant<int, V>;
 // slabs of the CPML shell carry only the psi of their own axes.
 // MAXT: the largest workgroup the launch uses (1024 or 512), which sets the
 // register budget (128 or 256 VGPRs).
-template <typename T, int MAT, int PM, int MAXT>
+template <typename T, int MAT, int PM, int MAXT, bool XL>
 __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T> a, PmlView<T> pm, PsiSet<T> ps,
                                               TbTiles tt, TbPost pp)
 {
@@ -424,14 +431,17 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     const int NT = tt.txw * tt.tyw;
     const bool tv = t < NT;
     const int tx = tv ? t % tt.txw : 0, ty = tv ? t / tt.txw : 0;
-    const int I = bx.i0 + ti * (tt.txw - 3) - 2 + tx;
+    // XL (a tile row is one wavefront): E^n at I+1 comes from the next lane,
+    // so lane 63 has no valid H1 and the high halo is two columns
+    constexpr int HX = XL ? 4 : 3;
+    const int I = bx.i0 + ti * (tt.txw - HX) - 2 + tx;
     const int J = bx.j0 + tj * (tt.tyw - 3) - 2 + ty;
     const bool inb = tv && I >= 0 && I <= g.nx && J >= 0 && J <= g.ny;
     // the ghost layer I = nx+1 / J = ny+1 holds values (never updated) that
     // the cells I = nx / J = ny read: its threads pass their old values on
     const bool ing = tv && I >= 0 && I <= g.nx + 1 && J >= 0 && J <= g.ny + 1;
     // stored: the box's cells of the update box and of its ghost layer n+1
-    const bool core = ing && tx >= 2 && tx <= tt.txw - 2 && ty >= 2 && ty <= tt.tyw - 2 && I < bx.i0 + bx.ni &&
+    const bool core = ing && tx >= 2 && tx <= tt.txw - (HX - 1) && ty >= 2 && ty <= tt.tyw - 2 && I < bx.i0 + bx.ni &&
                       J < bx.j0 + bx.nj;
     // LDS neighbours (out-of-tile ones, in the pads or other slots, only feed
     // halo lanes, whose values are invalid anyway)
@@ -498,7 +508,8 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
             Prof<T> hpz{};
             if (PML && pz) hpz = pm.hp[2][K];
             if (PML) tb_psi_load(g, pm.P, pm.W, ps.oh, I, J, K, px, py, pz, psh);
-            tb_h<T, PML>(hx1, hy1, hz1, v.ez_j1, v.ez, v.ey1, v.ey, v.ex1, v.ex, v.ez_i1, v.ey_i1, v.ex_j1, ch, cI, cJ,
+            const T ez_i1 = XL ? lane_next(v.ez) : v.ez_i1, ey_i1 = XL ? lane_next(v.ey) : v.ey_i1;
+            tb_h<T, PML>(hx1, hy1, hz1, v.ez_j1, v.ez, v.ey1, v.ey, v.ex1, v.ex, ez_i1, ey_i1, v.ex_j1, ch, cI, cJ,
                          c0 && K >= 1, px, py, pz, hpx, hpy, hpz, psh);
         }
         slot(0, par, 0)[t] = ing ? hx1 : (T)0;
@@ -604,17 +615,17 @@ __global__ __launch_bounds__(MAXT) void k_tb2(GridView g, T ch, T ce, TbArrays<T
     TbSet<T> A, B;
     int K = Ks;
     if (K & 1) {
-        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K, B);
-        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 1, A);
+        tb_load<T, MAT, XL>(a, g.nz, ui, ui_i1, ui_j1, K, B);
+        tb_load<T, MAT, XL>(a, g.nz, ui, ui_i1, ui_j1, K + 1, A);
         plane(tb_par<1>{}, K, B);
         ++K;
     } else {
-        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K, A);
+        tb_load<T, MAT, XL>(a, g.nz, ui, ui_i1, ui_j1, K, A);
     }
     for (; K + 1 <= Ke; K += 2) {
-        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 1, B);
+        tb_load<T, MAT, XL>(a, g.nz, ui, ui_i1, ui_j1, K + 1, B);
         plane(tb_par<0>{}, K, A);
-        tb_load<T, MAT>(a, g.nz, ui, ui_i1, ui_j1, K + 2, A);
+        tb_load<T, MAT, XL>(a, g.nz, ui, ui_i1, ui_j1, K + 2, A);
         plane(tb_par<1>{}, K + 1, B);
     }
     if (K <= Ke) plane(tb_par<0>{}, K, A);
