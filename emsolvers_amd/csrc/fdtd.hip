@@ -298,23 +298,25 @@ ems_status relayout(ems_fdtd *s, long shift)
     const GridView g1 = make_grid(g0.nx, g0.ny, g0.nz, shift);
     const size_t n1 = (size_t)(g1.off + g1.slab * (g0.nz + 2) + kFrontPad);
     const size_t e = s->esz;
-    auto move = [&](DevBuf &b) -> ems_status {
-        if (!b.p) return EMS_OK;
-        DevBuf nb;
-        EMS_TRY(nb.reserve(n1 * e));
-        EMS_HIP(hipMemset(nb.p, 0, n1 * e));
-        EMS_HIP(hipMemcpy2D(static_cast<char *>(nb.p) + g1.off * e, g1.pitch * e,
+    // every moved array is built completely before any is replaced, so that
+    // an error leaves the handle as it was
+    DevBuf *arrays[14] = {&s->ca, &s->cb};
+    for (int w = 0; w < 6; ++w) {
+        arrays[2 + w] = &s->f[w];
+        arrays[8 + w] = &s->f2[w];
+    }
+    DevBuf moved[14];
+    for (int q = 0; q < 14; ++q) {
+        const DevBuf &b = *arrays[q];
+        if (!b.p) continue;
+        EMS_TRY(moved[q].reserve(n1 * e));
+        EMS_HIP(hipMemset(moved[q].p, 0, n1 * e));
+        EMS_HIP(hipMemcpy2D(static_cast<char *>(moved[q].p) + g1.off * e, g1.pitch * e,
                             static_cast<const char *>(b.p) + g0.off * e, g0.pitch * e, (size_t)(g0.nx + 2) * e,
                             (size_t)(g0.ny + 2) * (g0.nz + 2), hipMemcpyDeviceToDevice));
-        b = std::move(nb);
-        return EMS_OK;
-    };
-    for (int w = 0; w < 6; ++w) {
-        EMS_TRY(move(s->f[w]));
-        EMS_TRY(move(s->f2[w]));
     }
-    EMS_TRY(move(s->ca));
-    EMS_TRY(move(s->cb));
+    for (int q = 0; q < 14; ++q)
+        if (arrays[q]->p) *arrays[q] = std::move(moved[q]);
     s->g = g1;
     s->nalloc = n1;
     for (auto &q : s->src) q.u = ref_to_unified(s->g, q.comp, q.i, q.j, q.k);
