@@ -62,6 +62,7 @@ SIGNATURES = {
     "ems_fdtd_set_tiling": (_I, [_P, _I, _I, _I]),
     "ems_fdtd_set_zmarch": (_I, [_P, _I, _I, _I]),
     "ems_fdtd_set_tb2": (_I, [_P, _I, _I, _I]),
+    "ems_fdtd_autotune_tb2": (_I, [_P, _I, _IP, _IP, _IP]),
     "ems_fdtd_run": (_I, [_P, _I]),
     "ems_fdtd_run_timed": (_I, [_P, _I, _DP, _DP]),
     "ems_fdtd_run_static": (_I, [_P, _I]),

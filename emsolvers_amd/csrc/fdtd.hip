@@ -1113,6 +1113,130 @@ ems_status run_impl(ems_fdtd *s, int num_steps, double *total_ms, double *kernel
     return run_finish(s, rp);
 }
 
+// Autotuner of the two-step kernel: times candidate tiles (the tile shape
+// sets how many workgroups share a CU: 64 x 16 FP32 threads run one per CU,
+// 64 x 12 and 64 x 8 two or three) and planes per chunk on zeroed scratch
+// field and psi sets swapped in for the trials, without sources or
+// receivers; the handle's state, traces and time are untouched.  Keeps and
+// reports the fastest.
+template <typename T> ems_status autotune_tb2_impl(ems_fdtd *s, int trial, int *cols, int *rows, int *planes)
+{
+    const int kernel0 = s->kernel;
+    s->kernel = EMS_KERNEL_TB2;
+    RunPlan rp;
+    ems_status st0 = run_prepare<T>(s, 0, rp, true);
+    if (st0 == EMS_OK && !tb2_fits(s))
+        st0 = invalid("two-step kernel: a field array reaches 4 GiB (32-bit offsets)");
+    if (st0 != EMS_OK) {
+        s->kernel = kernel0;
+        return st0;
+    }
+    // the handle's sets, restored on every exit
+    struct Held {
+        ems_fdtd *s;
+        int kernel0;
+        DevBuf f[6], f2[6], psi[12], psi2[12];
+        bool held = false;
+        ~Held()
+        {
+            if (held) {
+                for (int w = 0; w < 6; ++w) {
+                    s->f[w] = std::move(f[w]);
+                    s->f2[w] = std::move(f2[w]);
+                }
+                for (int q = 0; q < 12; ++q) {
+                    s->psi[q] = std::move(psi[q]);
+                    s->psi2[q] = std::move(psi2[q]);
+                }
+            }
+            s->kernel = kernel0;
+        }
+    } h{s, kernel0};
+    DevBuf sf[6], sf2[6], sp[12], sp2[12];
+    const size_t fb = s->nalloc * s->esz;
+    for (int w = 0; w < 6; ++w) {
+        EMS_TRY(sf[w].reserve(fb));
+        EMS_TRY(sf2[w].reserve(fb));
+        EMS_HIP(hipMemset(sf[w].p, 0, fb));
+        EMS_HIP(hipMemset(sf2[w].p, 0, fb));
+    }
+    for (int q = 0; q < 12; ++q) {
+        if (s->psi[q].bytes) {
+            EMS_TRY(sp[q].reserve(s->psi[q].bytes));
+            EMS_HIP(hipMemset(sp[q].p, 0, s->psi[q].bytes));
+        }
+        if (s->psi2[q].bytes) {
+            EMS_TRY(sp2[q].reserve(s->psi2[q].bytes));
+            EMS_HIP(hipMemset(sp2[q].p, 0, s->psi2[q].bytes));
+        }
+    }
+    for (int w = 0; w < 6; ++w) {
+        h.f[w] = std::move(s->f[w]);
+        h.f2[w] = std::move(s->f2[w]);
+        s->f[w] = std::move(sf[w]);
+        s->f2[w] = std::move(sf2[w]);
+    }
+    for (int q = 0; q < 12; ++q) {
+        h.psi[q] = std::move(s->psi[q]);
+        h.psi2[q] = std::move(s->psi2[q]);
+        s->psi[q] = std::move(sp[q]);
+        s->psi2[q] = std::move(sp2[q]);
+    }
+    h.held = true;
+    const T ch = (T)(s->d.dt / s->d.mu / s->d.h);
+    const T ce = (T)(s->d.dt / s->d.eps / s->d.h);
+    TbPost pp{};
+    pp.s_k0 = pp.p_k0 = 0;
+    pp.s_k1 = pp.p_k1 = -1;  // no sources, no receivers
+    const int c4[][2] = {{64, 16}, {64, 12}, {64, 8}, {68, 15}};
+    const int c8[][2] = {{64, 8}, {64, 6}, {36, 14}};
+    const int lzs[] = {8, 12, 16};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EMS_HIP(hipEventCreate(&e0));
+    EMS_HIP(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{e0, e1};
+    const int nc = sizeof(T) == 4 ? 4 : 3;
+    const int pairs = std::max(1, trial / 2);
+    float best = 1e30f;
+    int bc = 0, br = 0, bl = 0;
+    for (int c = 0; c < nc; ++c)
+        for (int lz : lzs) {
+            s->tb_txw = sizeof(T) == 4 ? c4[c][0] : c8[c][0];
+            s->tb_tyw = sizeof(T) == 4 ? c4[c][1] : c8[c][1];
+            s->tb_lz = lz;
+            const PmlView<T> pm = pml_view<T>(s);
+            EMS_TRY(launch_tb2<T>(s, ch, ce, pm, pp));  // warm-up
+            EMS_HIP(hipEventRecord(e0, s->stream));
+            for (int q = 0; q < pairs; ++q) EMS_TRY(launch_tb2<T>(s, ch, ce, pm, pp));
+            EMS_HIP(hipEventRecord(e1, s->stream));
+            EMS_HIP(hipEventSynchronize(e1));
+            EMS_HIP(hipGetLastError());
+            float ms = 0.f;
+            EMS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bc = s->tb_txw;
+                br = s->tb_tyw;
+                bl = lz;
+            }
+        }
+    EMS_HIP(hipStreamSynchronize(s->stream));
+    s->tb_txw = bc;
+    s->tb_tyw = br;
+    s->tb_lz = bl;
+    if (cols) *cols = bc;
+    if (rows) *rows = br;
+    if (planes) *planes = bl;
+    return EMS_OK;
+}
+
 ems_status set_material_impl(ems_fdtd *s)
 {
     const GridView &g = s->g;
@@ -2848,6 +2972,15 @@ ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes)
     s->tb_tyw = rows;
     s->tb_lz = planes;
     return EMS_OK;
+}
+
+ems_status ems_fdtd_autotune_tb2(ems_fdtd *s, int trial_steps, int *cols, int *rows, int *planes)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (trial_steps < 1) return invalid("trial_steps must be >= 1");
+    return s->d.precision == EMS_F32 ? autotune_tb2_impl<float>(s, trial_steps, cols, rows, planes)
+                                     : autotune_tb2_impl<double>(s, trial_steps, cols, rows, planes);
 }
 
 ems_status ems_fdtd_set_zmarch(ems_fdtd *s, int rows, int planes, int waves)

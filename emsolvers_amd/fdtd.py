@@ -135,6 +135,14 @@ class FemSimulation:
         workgroup; 0 = default."""
         _lib.check(_lib.load().ems_fdtd_set_tb2(self._h, int(cols), int(rows), int(planes)))
 
+    def autotune_tb2(self, trial_steps=10):
+        """Times candidate two-step tilings on scratch fields (the state is
+        untouched), keeps the fastest and returns it as (cols, rows, planes)."""
+        c, r, p = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().ems_fdtd_autotune_tb2(self._h, int(trial_steps), ctypes.byref(c), ctypes.byref(r),
+                                                     ctypes.byref(p)))
+        return c.value, r.value, p.value
+
     def run_timed(self, num_steps):
         """Returns (total_ms, kernel_ms) measured with HIP events."""
         tot, ker = ctypes.c_double(), ctypes.c_double()

@@ -129,6 +129,16 @@ ems_status ems_fdtd_set_kernel(ems_fdtd *s, int variant);
  * Results do not depend on it.  No reference counterpart. */
 ems_status ems_fdtd_set_tb2(ems_fdtd *s, int cols, int rows, int planes);
 
+/* Autotuner of EMS_KERNEL_TB2: times candidate tiles (FP32 64 x 16, 64 x 12,
+ * 64 x 8 and 68 x 15 threads; FP64 64 x 8, 64 x 6 and 36 x 14; the tile sets
+ * how many workgroups share a CU) with 8, 12 and 16 planes per chunk, for
+ * about trial_steps steps each, on zeroed scratch fields swapped in for the
+ * trials (no sources or receivers; the handle's fields, traces and time are
+ * untouched), keeps the fastest as by ems_fdtd_set_tb2 and reports it
+ * (outputs may be NULL).  Results do not depend on it.  No reference
+ * counterpart. */
+ems_status ems_fdtd_autotune_tb2(ems_fdtd *s, int trial_steps, int *cols, int *rows, int *planes);
+
 /* Tiling of EMS_KERNEL_ZMARCH: rows per wavefront strip (2 or 4), planes
  * per chunk (>= 1), wavefronts per workgroup (1 or 4).  Defaults 4, 8, 1.
  * No reference counterpart. */

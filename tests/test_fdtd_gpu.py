@@ -150,6 +150,28 @@ def test_tb2_tilings_agree_bitwise_with_split_fp32(name, cols, rows, planes):
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_tb2_autotune_keeps_state_and_results(precision):
+    """The two-step autotuner times its candidates on scratch fields: 9
+    steps, the autotuner, then 8 more steps give the fields, traces and time
+    of the same runs without it (every tiling agrees bit for bit)."""
+    case = fc.PARITY_CASES["lossy_cpml_rect"]
+    print("two-step autotune", precision, flush=True)
+    a = build_gpu(case, precision, 4)
+    b = build_gpu(case, precision, 4)
+    a.run(9)
+    b.run(9)
+    cols, rows, planes = b.autotune_tb2(4)
+    tiles = {(64, 16), (64, 12), (64, 8), (68, 15)} if precision == "f32" else {(64, 8), (64, 6), (36, 14)}
+    assert (cols, rows) in tiles and planes in (8, 12, 16)
+    a.run(8)
+    b.run(8)
+    for w in range(6):
+        assert np.array_equal(a.field(w), b.field(w)), NAMES[w]
+    assert np.array_equal(a.traces(), b.traces(), equal_nan=True)
+    assert a.current_time == b.current_time
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
 def test_tb2_odd_steps_and_split_runs(precision):
     """Odd step counts end with one split step: runs of 3, 1, 2, 4 and 3
     steps in a row give the fields, traces (slots restart per run, as the
