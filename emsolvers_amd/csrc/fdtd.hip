@@ -858,6 +858,52 @@ struct RunPlan {
     double t_end = 0.0;
 };
 
+// The two-step kernel finds a plane's sources and receiver points through
+// CSR lists over K (stable: sources of one cell keep their order).  sc, su:
+// the sources in value-row order; pc, pu: the receiver points in raw-sample
+// column order.
+ems_status tb_lists(ems_fdtd *s, const std::vector<int> &sc, const std::vector<long> &su, const std::vector<int> &pc,
+                    const std::vector<long> &pu)
+{
+    const GridView &g = s->g;
+    auto plane_of = [&](long u) { return (int)((u - g.off) / g.slab); };
+    auto csr = [&](int n, const std::vector<int> &comp, const std::vector<long> &uu, DevBuf &kptr, DevBuf &dc,
+                   DevBuf &du, DevBuf &di, int (&krange)[2]) -> ems_status {
+        krange[0] = 0;
+        krange[1] = -1;
+        for (int q = 0; q < n; ++q) {
+            const int K = plane_of(uu[q]);
+            krange[0] = q == 0 ? K : std::min(krange[0], K);
+            krange[1] = q == 0 ? K : std::max(krange[1], K);
+        }
+        std::vector<int> order(n);
+        for (int q = 0; q < n; ++q) order[q] = q;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plane_of(uu[a]) < plane_of(uu[b]); });
+        std::vector<int> kp(g.nz + 3, 0), c2(std::max(n, 1)), i2(std::max(n, 1));
+        std::vector<long> u2(std::max(n, 1));
+        for (int q = 0; q < n; ++q) {
+            const int o = order[q];
+            c2[q] = comp[o];
+            u2[q] = uu[o];
+            i2[q] = o;
+            kp[plane_of(uu[o]) + 1]++;
+        }
+        for (int K = 0; K < g.nz + 2; ++K) kp[K + 1] += kp[K];
+        EMS_TRY(kptr.reserve(sizeof(int) * kp.size()));
+        EMS_TRY(dc.reserve(sizeof(int) * c2.size()));
+        EMS_TRY(du.reserve(sizeof(long) * u2.size()));
+        EMS_TRY(di.reserve(sizeof(int) * i2.size()));
+        EMS_HIP(hipMemcpy(kptr.p, kp.data(), sizeof(int) * kp.size(), hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(dc.p, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(du.p, u2.data(), sizeof(long) * u2.size(), hipMemcpyHostToDevice));
+        EMS_HIP(hipMemcpy(di.p, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice));
+        return EMS_OK;
+    };
+    EMS_TRY(csr((int)sc.size(), sc, su, s->d_tb_skptr, s->d_tb_scomp, s->d_tb_su, s->d_tb_sidx, s->tb_sk));
+    EMS_TRY(csr((int)pc.size(), pc, pu, s->d_tb_pkptr, s->d_tb_pcomp, s->d_tb_pu, s->d_tb_pidx, s->tb_pk));
+    return EMS_OK;
+}
+
 template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan &rp, bool fused_setup)
 {
     const int tf = s->d.tf;
@@ -932,47 +978,7 @@ template <typename T> ems_status run_prepare(ems_fdtd *s, int num_steps, RunPlan
         EMS_HIP(hipMemcpy(s->d_pt_comp.p, pc.data(), sizeof(int) * npts, hipMemcpyHostToDevice));
         EMS_HIP(hipMemcpy(s->d_pt_u.p, pu.data(), sizeof(long) * npts, hipMemcpyHostToDevice));
     }
-    if (kernel_kind(s) == EMS_KERNEL_TB2 && num_steps >= 2) {
-        // the two-step kernel finds a plane's sources and receiver points
-        // through CSR lists over K (stable: sources of one cell keep their order)
-        const GridView &g = s->g;
-        auto plane_of = [&](long u) { return (int)((u - g.off) / g.slab); };
-        auto csr = [&](int n, const std::vector<int> &comp, const std::vector<long> &uu, DevBuf &kptr, DevBuf &dc,
-                       DevBuf &du, DevBuf &di, int (&krange)[2]) -> ems_status {
-            krange[0] = 0;
-            krange[1] = -1;
-            for (int q = 0; q < n; ++q) {
-                const int K = plane_of(uu[q]);
-                krange[0] = q == 0 ? K : std::min(krange[0], K);
-                krange[1] = q == 0 ? K : std::max(krange[1], K);
-            }
-            std::vector<int> order(n);
-            for (int q = 0; q < n; ++q) order[q] = q;
-            std::stable_sort(order.begin(), order.end(),
-                             [&](int a, int b) { return plane_of(uu[a]) < plane_of(uu[b]); });
-            std::vector<int> kp(g.nz + 3, 0), c2(std::max(n, 1)), i2(std::max(n, 1));
-            std::vector<long> u2(std::max(n, 1));
-            for (int q = 0; q < n; ++q) {
-                const int o = order[q];
-                c2[q] = comp[o];
-                u2[q] = uu[o];
-                i2[q] = o;
-                kp[plane_of(uu[o]) + 1]++;
-            }
-            for (int K = 0; K < g.nz + 2; ++K) kp[K + 1] += kp[K];
-            EMS_TRY(kptr.reserve(sizeof(int) * kp.size()));
-            EMS_TRY(dc.reserve(sizeof(int) * c2.size()));
-            EMS_TRY(du.reserve(sizeof(long) * u2.size()));
-            EMS_TRY(di.reserve(sizeof(int) * i2.size()));
-            EMS_HIP(hipMemcpy(kptr.p, kp.data(), sizeof(int) * kp.size(), hipMemcpyHostToDevice));
-            EMS_HIP(hipMemcpy(dc.p, c2.data(), sizeof(int) * c2.size(), hipMemcpyHostToDevice));
-            EMS_HIP(hipMemcpy(du.p, u2.data(), sizeof(long) * u2.size(), hipMemcpyHostToDevice));
-            EMS_HIP(hipMemcpy(di.p, i2.data(), sizeof(int) * i2.size(), hipMemcpyHostToDevice));
-            return EMS_OK;
-        };
-        EMS_TRY(csr(nsrc, sc, su, s->d_tb_skptr, s->d_tb_scomp, s->d_tb_su, s->d_tb_sidx, s->tb_sk));
-        EMS_TRY(csr(npts, pc, pu, s->d_tb_pkptr, s->d_tb_pcomp, s->d_tb_pu, s->d_tb_pidx, s->tb_pk));
-    }
+    if (kernel_kind(s) == EMS_KERNEL_TB2 && num_steps >= 2) EMS_TRY(tb_lists(s, sc, su, pc, pu));
     rp.nsrc = nsrc;
     rp.npts = npts;
     rp.nrec = nrec;
@@ -1569,11 +1575,23 @@ ems_status grad_prepare(GradStore &gs, const std::vector<ems_fdtd *> &mem, int N
 //   snap(slot): the current E of every shot into snapshot slot `slot`
 //   save(idx) / load(idx): the state of every shot to / from checkpoint idx
 //   adjoint(n, slot): adjoint step n, E^n in slot `slot`, E^{n+1} in slot + 1
+// adv2 of schedules without paired steps
+struct NoPairs {};
+
 template <typename Ops> struct GradSchedule {
     Ops &op;
     GradPlan p;
     int N;
     int l2_base() const { return std::max(0, ceil_div(N, p.K1) - 1); }
+    // steps [a, b) without snapshots: in pairs where the ops have them
+    ems_status plain(int a, int b, bool record)
+    {
+        int n = a;
+        if constexpr (!std::is_same<decltype(op.adv2), NoPairs>::value)
+            for (; n + 1 < b; n += 2) EMS_TRY(op.adv2(n, record));
+        for (; n < b; ++n) EMS_TRY(op.adv(n, record, -1));
+        return EMS_OK;
+    }
     // a level-1 segment [a, b) from its start state: level-2 checkpoints and
     // the last sub-segment's snapshots
     ems_status level2_forward(int a, int b, bool record)
@@ -1582,9 +1600,13 @@ template <typename Ops> struct GradSchedule {
         for (int q = 0; q < n2; ++q) {
             const int c = a + q * p.Ks, d = std::min(b, c + p.Ks);
             const bool last = q == n2 - 1;
-            if (!last) EMS_TRY(op.save(l2_base() + q));
-            if (last) EMS_TRY(op.snap(0));
-            for (int n = c; n < d; ++n) EMS_TRY(op.adv(n, record, last ? n + 1 - c : -1));
+            if (!last) {
+                EMS_TRY(op.save(l2_base() + q));
+                EMS_TRY(plain(c, d, record));
+                continue;
+            }
+            EMS_TRY(op.snap(0));
+            for (int n = c; n < d; ++n) EMS_TRY(op.adv(n, record, n + 1 - c));
         }
         return EMS_OK;
     }
@@ -1595,7 +1617,7 @@ template <typename Ops> struct GradSchedule {
             const int a = q * p.K1, b = std::min(N, a + p.K1);
             if (q < n1 - 1) {
                 EMS_TRY(op.save(q));
-                for (int n = a; n < b; ++n) EMS_TRY(op.adv(n, true, -1));
+                EMS_TRY(plain(a, b, true));
             } else {
                 EMS_TRY(level2_forward(a, b, true));
             }
@@ -1762,17 +1784,23 @@ std::vector<double> material_factors(const ems_fdtd *s)
     return f;
 }
 
-template <typename A, typename B, typename C, typename D, typename E> struct GradOps {
+template <typename A, typename B, typename C, typename D, typename E, typename P = NoPairs> struct GradOps {
     A adv;
     B snap;
     C save;
     D load;
     E adjoint;
+    P adv2;  // steps n, n+1 without snapshots (the two-step kernel), or NoPairs
 };
 template <typename A, typename B, typename C, typename D, typename E>
 GradOps<A, B, C, D, E> grad_ops(A a, B b, C c, D d, E e)
 {
-    return GradOps<A, B, C, D, E>{a, b, c, d, e};
+    return GradOps<A, B, C, D, E>{a, b, c, d, e, NoPairs{}};
+}
+template <typename A, typename B, typename C, typename D, typename E, typename P>
+GradOps<A, B, C, D, E, P> grad_ops(A a, B b, C c, D d, E e, P p)
+{
+    return GradOps<A, B, C, D, E, P>{a, b, c, d, e, p};
 }
 
 template <typename T>
@@ -1854,6 +1882,40 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
         EMS_HIP(hipGetLastError());
         return EMS_OK;
     };
+    // steps n, n+1 by the two-step kernel where a single run would take it
+    // (the same bits as two single steps; sources and receivers inside)
+    const bool tb2 = kernel_kind(s) == EMS_KERNEL_TB2 && N >= 2;
+    if (tb2) {
+        EMS_TRY(tb2_check<T>(s));
+        EMS_TRY(tb_streams(s));
+        EMS_TRY(tb_lists(s, sc, su, pc, pu));
+    }
+    auto fpair = [&](int n, bool record) -> ems_status {
+        if (!tb2) {
+            EMS_TRY(fstep(n, record));
+            return fstep(n + 1, record);
+        }
+        TbPost pp{};
+        pp.s_kptr = s->d_tb_skptr.as<int>();
+        pp.s_comp = s->d_tb_scomp.as<int>();
+        pp.s_u = s->d_tb_su.as<long>();
+        pp.s_idx = s->d_tb_sidx.as<int>();
+        pp.v1 = s->d_src_val.as<double>() + (size_t)n * nsrc;
+        pp.v2 = s->d_src_val.as<double>() + (size_t)(n + 1) * nsrc;
+        pp.p_kptr = s->d_tb_pkptr.as<int>();
+        pp.p_comp = s->d_tb_pcomp.as<int>();
+        pp.p_u = s->d_tb_pu.as<long>();
+        pp.p_idx = s->d_tb_pidx.as<int>();
+        pp.s_k0 = s->tb_sk[0];
+        pp.s_k1 = s->tb_sk[1];
+        pp.p_k0 = s->tb_pk[0];
+        pp.p_k1 = s->tb_pk[1];
+        if (record && npts && (n + 1) % tf == 0) pp.rec1 = s->d_rec.as<double>() + (size_t)((n + 1) / tf - 1) * npts;
+        if (record && npts && (n + 2) % tf == 0) pp.rec2 = s->d_rec.as<double>() + (size_t)((n + 2) / tf - 1) * npts;
+        EMS_TRY(launch_tb2<T>(s, ch, ce, pml_view<T>(s), pp));
+        EMS_HIP(hipGetLastError());
+        return EMS_OK;
+    };
     // gradient buffers outside the store first: the store takes its share of
     // the memory that is left
     const size_t sb = state_bytes(s);
@@ -1893,7 +1955,8 @@ ems_status gradient_impl(ems_fdtd *s, int N, const double *d_obs, double *misfit
             EMS_HIP(hipGetLastError());
             cur = 1 - cur;
             return EMS_OK;
-        });
+        },
+        fpair);
     GradSchedule<decltype(ops)> sched{ops, plan, N};
     EMS_TRY(sched.first_pass());
     EMS_TRY(copy_state(s, A->fin.p, true));

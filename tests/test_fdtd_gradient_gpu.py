@@ -149,6 +149,41 @@ def test_memory_caps_do_not_change_results():
         c.gradient(N, dobs)
 
 
+@pytest.mark.parametrize("name", ["cpml-lossy", "eps-only-tf2"])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_gradient_two_step_kernel_bitwise(name, precision):
+    """With the two-step kernel the gradient's forward stretches without
+    snapshots advance in pairs of steps (the same bits as single steps):
+    misfit, gradients, traces and final state equal those of the split
+    sweeps, for the default schedule and for capped one- and two-level ones."""
+    case = CASES[name]
+    er, sg = _material(case)
+    dobs = _observed(case, er, sg)
+    N = case["steps"]
+    print("two-step gradient", name, precision, flush=True)
+    full = _gpu(case, er, sg, precision).gradient_plan(N)[2]
+    for frac in (None, 0.6, 0.35):
+        outs = []
+        for k in (1, 4):
+            s = _gpu(case, er, sg, precision)
+            s.set_kernel(k)
+            if frac is not None:
+                s.set_gradient_memory(int(full * frac))
+                try:
+                    s.gradient_plan(N)
+                except Exception:
+                    break  # nothing fits this cap
+            outs.append((s.gradient(N, dobs), s.traces(), [s.field(w) for w in range(6)]))
+        if len(outs) < 2:
+            continue
+        (a, ta, fa), (b, tb, fb) = outs
+        assert a[0] == b[0], frac
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), frac
+        np.testing.assert_array_equal(ta, tb)
+        for w in range(6):
+            assert np.array_equal(fa[w], fb[w]), (frac, w)
+
+
 def test_fp32_gradient_against_finite_differences():
     case = CASES["cpml-lossy"]
     er, sg = _material(case)
