@@ -25,6 +25,7 @@ struct ems_fem {
     std::vector<double> kscaled;  // n x n, FP64
     DevBuf K, off;
     DevBuf Kp;          // orders 2, 3: K in the MFMA A-operand layout (k_fem_elem_mfma)
+    DevBuf Kst;         // order 1: the merged stencil coefficients [3][33] of k_fem1z
     int elem_kind = EMS_FEM_ELEM_AUTO;  // orders 2, 3: element products (per element thread or matrix cores)
     // leapfrog: slot[s][comp]; symplectic: U = slot 0, P = slot 1, and the
     // ping-pong partners slot 2 (U), slot 3 (P)
@@ -41,8 +42,9 @@ struct ems_fem {
     DevBuf d_src, d_sval, d_obs, d_rec;
     // first order: z-marching all-component kernel with BY rows (4 or 8) and
     // lz planes per workgroup; 0 rows: one gather kernel per component;
-    // -1: per stepper, as measured fastest at 128^3 on MI355X (leapfrog 8 x 8,
-    // symplectic stages 4 x 4)
+    // -1: as measured fastest at 128^3 on MI355X: merged stencil 8 x 16
+    // (FP32 leapfrog 8 x 8), element row dots leapfrog 8 x 8, symplectic
+    // stages 4 x 4
     int z1_by = -1, z1_lz = 8;
 };
 
@@ -90,16 +92,25 @@ void launch_z1(ems_fem *s, CComp3<T> u, CComp3<T> pm, CComp3<T> other, Comp3<T> 
                T mm, const double *sval)
 {
     const FemGrid &g = s->g;
-    const int by = s->z1_by > 0 ? s->z1_by : DRIFT ? 4 : 8;
-    const int lz = s->z1_by > 0 ? s->z1_lz : DRIFT ? 4 : 8;
+    const bool stn = s->elem_kind == EMS_FEM_ELEM_STENCIL || s->elem_kind == EMS_FEM_ELEM_AUTO;
+    const int by = s->z1_by > 0 ? s->z1_by : (DRIFT && !stn) ? 4 : 8;
+    const int lz = s->z1_by > 0 ? s->z1_lz : !stn ? (DRIFT ? 4 : 8) : (!DRIFT && sizeof(T) == 4) ? 8 : 16;
     const int ns = (int)s->src_z.size();
     const dim3 grd((g.Nx + 63) / 64, (g.Ny + by - 1) / by, (g.Nz + lz - 1) / lz);
-    if (by == 4)
-        k_fem1z<T, 4, DRIFT><<<grd, dim3(64, 4, 1), 0, s->stream>>>(g, s->K.as<T>(), u, pm, other, o1, o2, lz, deltat,
-                                                                   hd, mm, ns, s->d_src.as<long>(), sval);
+    const T *K = s->K.as<T>(), *stc = s->Kst.as<T>();
+    const long *si = s->d_src.as<long>();
+    if (by == 4 && stn)
+        k_fem1z<T, 4, DRIFT, true><<<grd, dim3(64, 4, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
+                                                                         deltat, hd, mm, ns, si, sval);
+    else if (by == 4)
+        k_fem1z<T, 4, DRIFT, false><<<grd, dim3(64, 4, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
+                                                                          deltat, hd, mm, ns, si, sval);
+    else if (stn)
+        k_fem1z<T, 8, DRIFT, true><<<grd, dim3(64, 8, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
+                                                                         deltat, hd, mm, ns, si, sval);
     else
-        k_fem1z<T, 8, DRIFT><<<grd, dim3(64, 8, 1), 0, s->stream>>>(g, s->K.as<T>(), u, pm, other, o1, o2, lz, deltat,
-                                                                   hd, mm, ns, s->d_src.as<long>(), sval);
+        k_fem1z<T, 8, DRIFT, false><<<grd, dim3(64, 8, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
+                                                                          deltat, hd, mm, ns, si, sval);
 }
 
 // Row dots of every element (orders 2, 3) into etmp: per-element threads
@@ -349,6 +360,36 @@ template <typename T> ems_status upload_kmat(ems_fem *s)
                 }
         EMS_TRY(s->Kp.reserve(sizeof(T) * kp.size()));
         EMS_HIP(hipMemcpy(s->Kp.p, kp.data(), sizeof(T) * kp.size(), hipMemcpyHostToDevice));
+    } else {
+        // order 1: each component's 4 element rows summed into the terms of
+        // z1_term (elements at -1 / 0 along the transverse axes a1, a2; local
+        // entry m = 4 q + a + 2 b at offsets a, b along q's transverse axes)
+        std::vector<double> c(3 * kZ1Terms, 0.0);
+        for (int C = 0; C < 3; ++C) {
+            const int a1 = C == 0 ? 1 : 0, a2 = C == 2 ? 1 : 2;
+            for (int code = 0; code < 4; ++code) {
+                int e[3] = {0, 0, 0};
+                if (code & 1) e[a1] = -1;
+                if (code & 2) e[a2] = -1;
+                const int r = 4 * C + (code & 1) + (code & 2);
+                for (int m = 0; m < 12; ++m) {
+                    const int q = m / 4, b1 = q == 0 ? 1 : 0, b2 = q == 2 ? 1 : 2;
+                    int d[3] = {e[0], e[1], e[2]};
+                    d[b1] += m & 1;
+                    d[b2] += (m >> 1) & 1;
+                    int t = 0;
+                    for (; t < kZ1Terms; ++t) {
+                        const Z1Term z = z1_term(C, t);
+                        if (z.q == q && z.d[0] == d[0] && z.d[1] == d[1] && z.d[2] == d[2]) break;
+                    }
+                    if (t == kZ1Terms) return not_supported("order-1 stencil term missing");
+                    c[C * kZ1Terms + t] += s->kscaled[(size_t)r * 12 + m];
+                }
+            }
+        }
+        std::vector<T> ct(c.begin(), c.end());
+        EMS_TRY(s->Kst.reserve(sizeof(T) * ct.size()));
+        EMS_HIP(hipMemcpy(s->Kst.p, ct.data(), sizeof(T) * ct.size(), hipMemcpyHostToDevice));
     }
     return EMS_OK;
 }
@@ -671,8 +712,10 @@ ems_status ems_fem_set_element_product(ems_fem *s, int kind)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (kind != EMS_FEM_ELEM_AUTO && kind != EMS_FEM_ELEM_ROWS && kind != EMS_FEM_ELEM_MFMA)
+    if (kind != EMS_FEM_ELEM_AUTO && kind != EMS_FEM_ELEM_ROWS && kind != EMS_FEM_ELEM_MFMA &&
+        kind != EMS_FEM_ELEM_STENCIL)
         return invalid("unknown element product");
+    if (kind == EMS_FEM_ELEM_STENCIL && s->g.p != 1) return invalid("the merged stencil is first order only");
     s->elem_kind = kind;
     return EMS_OK;
 }

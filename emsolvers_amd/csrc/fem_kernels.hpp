@@ -615,6 +615,82 @@ __device__ __forceinline__ T z1_row_dot(const T *sK, const T *win, int r, int i,
     return temp;
 }
 
+// Merged first-order stencil.  Off the PEC faces a DOF lies in exactly 4
+// elements, and the sum of their 4 row dots is a fixed 33-point stencil per
+// component C over the window: 9 entries of array C in the plane across the
+// edge and 12 of each other array.  Term t of component C (a1 < a2 the axes
+// transverse to C, which are also the other components):
+//   t <  9: array C at offsets t % 3 - 1 along a1, t / 3 - 1 along a2;
+//   t >= 9: array q = a1 (t < 21) or a2, u = (t - 9) % 12: u & 1 along C,
+//           (u >> 1 & 1) - 1 along q, (u >> 2) - 1 along the remaining axis.
+// The host sums the scaled element rows into these coefficients once.
+struct Z1Term {
+    int q, d[3];
+};
+
+__host__ __device__ constexpr Z1Term z1_term(int C, int t)
+{
+    const int a1 = C == 0 ? 1 : 0, a2 = C == 2 ? 1 : 2;
+    Z1Term r{C, {0, 0, 0}};
+    if (t < 9) {
+        r.d[a1] = t % 3 - 1;
+        r.d[a2] = t / 3 - 1;
+        return r;
+    }
+    const int q = t < 21 ? a1 : a2, third = q == a1 ? a2 : a1;
+    const int u = (t - 9) % 12;
+    r.q = q;
+    r.d[C] = u & 1;
+    r.d[q] = ((u >> 1) & 1) - 1;
+    r.d[third] = (u >> 2) - 1;
+    return r;
+}
+
+constexpr int kZ1Terms = 33;
+
+// Stencil sum of component C at the window position lo of plane Z: one
+// partial sum per array, coefficients at uniform addresses (scalar loads).
+template <typename T, int BY, int C>
+__device__ __forceinline__ T z1_stencil(const T *__restrict__ cf, const T *win, int Z, int lo)
+{
+    using W = Z1Win<BY>;
+    const T *pz[3] = {win + ((Z + 3) & 3) * W::SLOT + lo, win + (Z & 3) * W::SLOT + lo,
+                      win + ((Z + 1) & 3) * W::SLOT + lo};
+    T s[3] = {0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < kZ1Terms; ++t) {
+        const Z1Term m = z1_term(C, t);
+        s[m.q] += cf[t] * pz[m.d[2] + 1][m.q * W::PLANE + m.d[0] + W::EX * m.d[1]];
+    }
+    return (s[0] + s[1]) + s[2];
+}
+
+// The step of one DOF from the merged stencil: the update of z1_dof below,
+// with the sources added after the operator sum.
+template <typename T, int BY, bool DRIFT>
+__device__ __forceinline__ void z1_dof_stn(const T *__restrict__ cf, const T *win, int comp, int Z, int lo, long idx,
+                                           T own, const T *__restrict__ other, T *o1, T *o2, T deltat, T hd, T mm,
+                                           int nsrc, const long *__restrict__ src_idx, T sv)
+{
+    const T sum = comp == 0   ? z1_stencil<T, BY, 0>(cf, win, Z, lo)
+                  : comp == 1 ? z1_stencil<T, BY, 1>(cf + kZ1Terms, win, Z, lo)
+                              : z1_stencil<T, BY, 2>(cf + 2 * kZ1Terms, win, Z, lo);
+    int nsrc_here = 0;
+    if (comp == 2)
+        for (int s = 0; s < nsrc; ++s) nsrc_here += src_idx[s] == idx;
+    if (!DRIFT) {
+        T v = (0 - sum) + ((T)2 * own - other[idx]);
+        for (int s = 0; s < nsrc_here; ++s) v -= sv;
+        o1[idx] = v;
+    } else {
+        T tmp = sum;
+        for (int s = 0; s < nsrc_here; ++s) tmp += sv;
+        const T p1 = other[idx] - deltat * tmp;
+        o1[idx] = own + hd * p1 / mm;
+        o2[idx] = p1;
+    }
+}
+
 // Global index of window entry e of plane P, or -1 outside the array.
 template <int BY>
 __device__ __forceinline__ long z1_entry(const FemGrid &g, int e, int ox, int oy, int P, int &q)
@@ -730,18 +806,20 @@ __device__ __forceinline__ void z1_zero(T *o1, T *o2, long idx)
 }
 
 // Leapfrog: u = E1, other = E0, out1 = E2.  Symplectic stage: u = U, pm = P,
-// other = P (the DOF's own momentum), out1 = U', out2 = P'.
-template <typename T, int BY, bool DRIFT>
-__global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restrict__ K, CComp3<T> u, CComp3<T> pm,
-                                                    CComp3<T> other, Comp3<T> out1, Comp3<T> out2, int lz, T deltat,
-                                                    T hd, T mm, int nsrc, const long *__restrict__ src_idx,
+// other = P (the DOF's own momentum), out1 = U', out2 = P'.  STN: the merged
+// stencil (coefficients stc[3][33]) instead of the 4 element row dots.
+template <typename T, int BY, bool DRIFT, bool STN>
+__global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restrict__ K, const T *__restrict__ stc,
+                                                    CComp3<T> u, CComp3<T> pm, CComp3<T> other, Comp3<T> out1,
+                                                    Comp3<T> out2, int lz, T deltat, T hd, T mm, int nsrc,
+                                                    const long *__restrict__ src_idx,
                                                     const double *__restrict__ src_val)
 {
     using W = Z1Win<BY>;
-    __shared__ T sK[144];
+    __shared__ T sK[STN ? 1 : 144];
     __shared__ T win[4 * W::SLOT];
     const int tid = threadIdx.y * 64 + threadIdx.x;
-    if (tid < 144) sK[tid] = K[tid];
+    if (!STN && tid < 144) sK[tid] = K[tid];
     const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * BY;
     const int ox = X0 - 1, oy = Y0 - 1;
     const int Z0 = blockIdx.z * lz, Z1 = min(Z0 + lz, g.Nz);
@@ -777,6 +855,9 @@ __global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restric
                 const T *ot = comp == 0 ? other.x : comp == 1 ? other.y : other.z;
                 const bool pec = comp == 0 ? (Y == 0 || Z == 0) : comp == 1 ? (X == 0 || Z == 0) : (X == 0 || Y == 0);
                 if (pec) z1_zero<T, DRIFT>(a1, a2, idx);
+                else if (STN)
+                    z1_dof_stn<T, BY, DRIFT>(stc, win, comp, Z, lo, idx, cur[comp * W::PLANE + lo], ot, a1, a2,
+                                             deltat, hd, mm, nsrc, src_idx, sv);
                 else z1_dof<T, BY, DRIFT>(g, sK, win, comp, X, Y, Z, ox, oy, idx, cur[comp * W::PLANE + lo], ot, a1,
                                           a2, deltat, hd, mm, nsrc, src_idx, sv);
             }

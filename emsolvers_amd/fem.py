@@ -129,14 +129,18 @@ class FemSimulation:
     def set_tiling(self, rows, planes=8):
         """First-order kernels: rows 4 or 8 = one z-marching pass over all
         components (64 x rows workgroups, `planes` planes each), 0 = one
-        gather kernel per component.  Results do not depend on it."""
+        gather kernel per component.  Results do not depend on it, except
+        that rows 0 always takes the element row dots (not the stencil)."""
         _lib.check(_lib.load().ems_fem_set_tiling(self._h, int(rows), int(planes)))
 
     def set_element_product(self, kind):
         """Orders 2 and 3: 'rows' (one thread per element, the reference's
         sums), 'mfma' (matrix cores; equal to rounding) or 'auto' (the
-        default: mfma at order 3, rows at order 2)."""
-        k = {"auto": 0, "rows": 1, "mfma": 2}[kind] if isinstance(kind, str) else int(kind)
+        default: mfma at order 3, rows at order 2).  Order 1: 'stencil' (one
+        33-point stencil per component in the z-march; equal to rounding;
+        also 'auto') or 'rows' (the element row dots in the reference's
+        order, bit for bit)."""
+        k = {"auto": 0, "rows": 1, "mfma": 2, "stencil": 3}[kind] if isinstance(kind, str) else int(kind)
         _lib.check(_lib.load().ems_fem_set_element_product(self._h, k))
 
     def ndof(self):

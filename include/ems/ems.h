@@ -369,8 +369,9 @@ ems_status ems_fem_dof_count(const ems_fem *s, size_t *n);
 /* First-order kernels (no reference counterpart): rows = 4 or 8 selects the
  * z-marching kernel that updates all three components in one pass with a
  * 64 x rows workgroup and `planes` planes per workgroup; rows = 0 one gather
- * kernel per component; rows = -1 (default) the measured choice per stepper.
- * Results do not depend on it. */
+ * kernel per component; rows = -1 (default) the measured choice per stepper
+ * and element product.  Results do not depend on it, except that rows = 0
+ * always takes the element row dots (see ems_fem_set_element_product). */
 ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes);
 
 /* Element products of orders 2 and 3 (no reference counterpart): ROWS one
@@ -380,8 +381,13 @@ ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes);
  * multiply-adds: the same results to rounding, not bit for bit); AUTO (the
  * default) MFMA at order 3 and ROWS at order 2, as measured fastest (order 3
  * 4.9-5.3x faster in FP64, 2.9x in FP32; order 2 within +-5 %).  Order 1
- * ignores it. */
-enum { EMS_FEM_ELEM_AUTO = 0, EMS_FEM_ELEM_ROWS = 1, EMS_FEM_ELEM_MFMA = 2 };
+ * (z-march tiling, rows 4 or 8): STENCIL (order 1 only, EMS_ERR_INVALID_ARG
+ * otherwise) sums each DOF's 4 element rows into one 33-point stencil per
+ * component (coefficients formed once from the scaled operator: the same
+ * results to rounding; 1.9-3.2x faster at 128^3), AUTO is STENCIL, ROWS and
+ * MFMA take the 4 element row dots in the reference's order (bit for bit).
+ * The per-component kernels (rows 0) always take the row dots. */
+enum { EMS_FEM_ELEM_AUTO = 0, EMS_FEM_ELEM_ROWS = 1, EMS_FEM_ELEM_MFMA = 2, EMS_FEM_ELEM_STENCIL = 3 };
 ems_status ems_fem_set_element_product(ems_fem *s, int kind);
 
 /* ------------------------------------------ first-order stabilized hex FEM

@@ -1,9 +1,11 @@
 """GPU parity of the structured-hex FEM (fem_v2 leapfrog, symplectic_2,
 symplectic_4; orders 1-3) against the FP64 oracle on the same inputs.
 
-Bar: FP64 bit-exact (the owner-gather kernels visit each DOF's elements in
-the serial reference's order and sum every element row from 0 in the same
-order); FP32 traces within 1e-5 relative L2 of the FP64 oracle.
+Bar: FP64 bit-exact with the element products in the reference's order
+('rows': the owner-gather kernels visit each DOF's elements in the serial
+reference's order and sum every element row from 0 in the same order);
+1e-10 relative L2 for the order-1 stencil and the order-2/3 matrix-core
+products; FP32 traces within 1e-5 relative L2 of the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -73,6 +75,7 @@ def test_first_order_tilings_bitwise(oracle_cache, stepper, rows, planes):
     o = oracle_cache(key)
     g = build("gpu", 1, stepper, case, "f64")
     g.set_tiling(rows, planes)
+    g.set_element_product("rows")  # the reference's sums (the default at order 1 is the stencil)
     g.run(case["steps"])
     assert np.array_equal(g.traces(), o.traces())
     for c in range(3):
@@ -91,6 +94,7 @@ def test_first_order_wide_grid_bitwise(stepper):
     for rows in (4, 8):
         g = build("gpu", 1, stepper, case, "f64")
         g.set_tiling(rows, 4)
+        g.set_element_product("rows")
         g.run(case["steps"])
         assert np.array_equal(g.traces(), o.traces())
         for c in range(3):
@@ -104,8 +108,7 @@ def test_fp64_bitwise_with_oracle(oracle_cache, key):
     print("case order", order, "stepper", stepper, flush=True)
     o = oracle_cache(key)
     g = build("gpu", order, stepper, case, "f64")
-    if order >= 2:
-        g.set_element_product("rows")  # the reference's sums (the default at order 3 is mfma)
+    g.set_element_product("rows")  # the reference's sums (the defaults at orders 1 and 3 are not)
     g.run(case["steps"])
     to, tg = o.traces(), g.traces()
     assert to.shape == tg.shape
@@ -144,6 +147,52 @@ def test_mfma_element_products_against_oracle(oracle_cache, key):
                                                                                          e32, e32r), flush=True)
     assert err <= 1e-10 and ferr <= 1e-10
     assert e32 <= 1e-5 and e32r <= 1e-5
+
+
+@pytest.mark.parametrize("rows,planes", [(4, 3), (8, 5)])
+@pytest.mark.parametrize("stepper", [0, 1, 2])
+def test_first_order_stencil_against_oracle(oracle_cache, stepper, rows, planes):
+    """First order with each component's 4 element rows merged into one
+    33-point stencil: FP64 traces and fields within 1e-10 relative L2 of the
+    oracle (the stencil sums in another order), FP32 traces within 1e-5 of
+    the FP64 oracle; also on a grid wider than one tile."""
+    key = (1, stepper)
+    case = CASES[key]
+    print("stencil", key, rows, planes, flush=True)
+    o = oracle_cache(key)
+    g = build("gpu", 1, stepper, case, "f64")
+    g.set_tiling(rows, planes)
+    g.set_element_product("stencil")
+    g.run(case["steps"])
+    err = rel_l2(g.traces()[..., 1:], o.traces()[..., 1:])
+    ferr = max(rel_l2(g.get(c, slot), o.get(c, slot)) for c in range(3) for slot in (0, 1))
+    f32 = build("gpu", 1, stepper, case, "f32")
+    f32.set_tiling(rows, planes)
+    f32.set_element_product("stencil")
+    f32.run(case["steps"])
+    e32 = rel_l2(f32.traces()[..., 1:], o.traces()[..., 1:])
+    print("stencil s%d: fp64 traces %.2e fields %.2e; fp32 %.2e" % (stepper, err, ferr, e32), flush=True)
+    assert np.abs(o.traces()[..., 1:]).max() > 0
+    assert err <= 1e-10 and ferr <= 1e-10
+    assert e32 <= 1e-5
+    if rows == 8:
+        wide = dict(grid=(70, 11, 9), steps=25, src=(64, 8, 4), obs=[(63, 8, 4), (65, 3, 4)], tf=1)
+        ow = build("oracle", 1, stepper, wide)
+        ow.run(wide["steps"])
+        gw = build("gpu", 1, stepper, wide, "f64")
+        gw.set_tiling(rows, 4)
+        gw.set_element_product("stencil")
+        gw.run(wide["steps"])
+        assert rel_l2(gw.traces()[..., 1:], ow.traces()[..., 1:]) <= 1e-10
+        for c in range(3):
+            assert rel_l2(gw.get(c, 1), ow.get(c, 1)) <= 1e-10, c
+
+
+def test_stencil_is_first_order_only():
+    from emsolvers_amd import _lib
+    g = build("gpu", 2, 0, CASES[(2, 0)], "f64")
+    with pytest.raises(_lib.EmsError):
+        g.set_element_product("stencil")
 
 
 @pytest.mark.parametrize("key", list(CASES), ids=lambda k: "p%d-s%d" % k)
