@@ -73,7 +73,7 @@ def gpu_fem(order, stepper, n, steps, precision="f64", elem="auto"):
                 grid=[n] * 3, dtype="fp64" if precision == "f64" else "fp32", element_product=elem)
 
 
-def gpu_fem1(stepper, n, steps, precision="f64"):
+def gpu_fem1(stepper, n, steps, precision="f64", elem="auto"):
     """First-order stabilized hex FEM (src/fem leapfrog, src/fem_abm) on n^3
     cells with a Hertz dipole; DOF-updates/s of the step kernels."""
     from emsolvers_amd.fem1 import FemSimulation as Fem, AbmSimulation as Abm
@@ -82,6 +82,7 @@ def gpu_fem1(stepper, n, steps, precision="f64"):
     dt = 0.3 * h / C0
     s = Fem((n, n, n), h, dt, EPS0, MU0, 1, 0.3, 0, precision=precision) if stepper == "fem" else \
         Abm((n, n, n), h, dt, EPS0, MU0, 1, precision=precision)
+    s.set_element_product(elem)
     s.setSourceDipole_x((n // 2, n // 2, n // 2))
     p = (n // 2 + 5, n // 2, n // 2)
     s.setObservationPoint([[p], [p], [p]])
@@ -90,7 +91,7 @@ def gpu_fem1(stepper, n, steps, precision="f64"):
     nd = s.ndof()
     return dict(value=round(nd * steps / (kernel_ms / 1e3) / 1e6, 1), unit="MDOF-updates/s", dofs=nd,
                 ms_per_step=round(kernel_ms / steps, 4), order=1, stepper=stepper, grid=[n] * 3,
-                dtype="fp64" if precision == "f64" else "fp32")
+                dtype="fp64" if precision == "f64" else "fp32", element_product=elem)
 
 
 def gpu_tet(n=40, stepper="symplectic_2", steps=2000, precision="f64", order=1, seed=5, sigma_fraction=0.0,

@@ -106,6 +106,7 @@ SIGNATURES = {
     "ems_fem_save_results": (_I, [_P, _I, ctypes.c_char_p]),
     "ems_fem_dof_count": (_I, [_P, ctypes.POINTER(_SZ)]),
     "ems_fem_set_element_product": (_I, [_P, _I]),
+    "ems_fem1_set_element_product": (_I, [_P, _I]),
     "ems_fem_set_tiling": (_I, [_P, _I, _I]),
     # first-order stabilized hex FEM (fem, fem_abm)
     "ems_fem1_element_matrix": (_I, [ctypes.POINTER(Fem1Desc), _DP]),

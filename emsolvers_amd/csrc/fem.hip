@@ -362,31 +362,10 @@ template <typename T> ems_status upload_kmat(ems_fem *s)
         EMS_HIP(hipMemcpy(s->Kp.p, kp.data(), sizeof(T) * kp.size(), hipMemcpyHostToDevice));
     } else {
         // order 1: each component's 4 element rows summed into the terms of
-        // z1_term (elements at -1 / 0 along the transverse axes a1, a2; local
-        // entry m = 4 q + a + 2 b at offsets a, b along q's transverse axes)
-        std::vector<double> c(3 * kZ1Terms, 0.0);
-        for (int C = 0; C < 3; ++C) {
-            const int a1 = C == 0 ? 1 : 0, a2 = C == 2 ? 1 : 2;
-            for (int code = 0; code < 4; ++code) {
-                int e[3] = {0, 0, 0};
-                if (code & 1) e[a1] = -1;
-                if (code & 2) e[a2] = -1;
-                const int r = 4 * C + (code & 1) + (code & 2);
-                for (int m = 0; m < 12; ++m) {
-                    const int q = m / 4, b1 = q == 0 ? 1 : 0, b2 = q == 2 ? 1 : 2;
-                    int d[3] = {e[0], e[1], e[2]};
-                    d[b1] += m & 1;
-                    d[b2] += (m >> 1) & 1;
-                    int t = 0;
-                    for (; t < kZ1Terms; ++t) {
-                        const Z1Term z = z1_term(C, t);
-                        if (z.q == q && z.d[0] == d[0] && z.d[1] == d[1] && z.d[2] == d[2]) break;
-                    }
-                    if (t == kZ1Terms) return not_supported("order-1 stencil term missing");
-                    c[C * kZ1Terms + t] += s->kscaled[(size_t)r * 12 + m];
-                }
-            }
-        }
+        // the merged stencil
+        std::vector<double> c(3 * kZ1Terms);
+        if (!z1_stencil_coefs(s->kscaled.data(), false, c.data()))
+            return not_supported("order-1 stencil term missing");
         std::vector<T> ct(c.begin(), c.end());
         EMS_TRY(s->Kst.reserve(sizeof(T) * ct.size()));
         EMS_HIP(hipMemcpy(s->Kst.p, ct.data(), sizeof(T) * ct.size(), hipMemcpyHostToDevice));

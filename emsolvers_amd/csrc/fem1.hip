@@ -9,6 +9,7 @@
 
 #include "common.hpp"
 #include "fem1_kernels.hpp"
+#include "fem_kernels.hpp"
 
 using namespace ems;
 using namespace ems::fem1;
@@ -23,6 +24,8 @@ struct ems_fem1 {
     size_t esz = 8;
     double K[144] = {};
     DevBuf dK, dKt;
+    DevBuf dKs, dKts;  // merged stencils [3][33] of K and K^T
+    int elem_kind = EMS_FEM_ELEM_AUTO;  // STENCIL (AUTO) or the element row dots (ROWS)
     // E/H: 3 rotating slots; Ep/Hp: predicted fields; E3/H3: scratch
     DevBuf E[3][3], H[3][3], Ep[3], Hp[3], E3[3], H3[3];
     int t0 = 0, t1 = 1, t2 = 2;
@@ -171,8 +174,37 @@ template <int COMP> dim3 grid_of(const Hex1 &g)
     return dim3((dx + 63) / 64, (dy + 3) / 4, dz);
 }
 
+bool stencil(const ems_fem1 *s) { return s->elem_kind != EMS_FEM_ELEM_ROWS; }
+
+// The leapfrog update E2 = (0 - K E1) + (2 E1 - E0) with PEC zeros is the
+// fem_v2 one on the same layouts: with the merged stencil it runs on that
+// family's z-marching kernel (no sources there: k_fem1_post adds them), 64 x 8
+// points per workgroup, 16 planes (FP32 8), as measured fastest at 128^3.
+template <typename T> void launch_leap_stencil(ems_fem1 *s)
+{
+    fem::FemGrid g{};
+    g.p = 1;
+    g.Nx = g.nex = s->g.Nx;
+    g.Ny = g.ney = s->g.Ny;
+    g.Nz = g.nez = s->g.Nz;
+    g.NX = s->g.n[0];
+    g.NY = s->g.n[1];
+    g.NZ = s->g.n[2];
+    g.D = 4;
+    g.n = 12;
+    const int lz = sizeof(T) == 4 ? 8 : 16;
+    auto c3 = [](DevBuf *b) { return fem::CComp3<T>{b[0].as<T>(), b[1].as<T>(), b[2].as<T>()}; };
+    const fem::CComp3<T> e0 = c3(s->E[s->t0]), e1 = c3(s->E[s->t1]);
+    DevBuf *o = s->E[s->t2];
+    const fem::Comp3<T> e2{o[0].as<T>(), o[1].as<T>(), o[2].as<T>()};
+    const dim3 grd((g.Nx + 63) / 64, (g.Ny + 7) / 8, (g.Nz + lz - 1) / lz);
+    fem::k_fem1z<T, 8, false, true><<<grd, dim3(64, 8, 1), 0, s->stream>>>(
+        g, s->dK.as<T>(), s->dKs.as<T>(), e1, e1, e0, e2, e2, lz, (T)0, (T)0, (T)1, 0, nullptr, nullptr);
+}
+
 template <typename T> void launch_leap(ems_fem1 *s)
 {
+    if (stencil(s)) return launch_leap_stencil<T>(s);
     const dim3 blk(64, 4, 1);
     const CF3<T> e0 = cf<T>(s->E[s->t0]), e1 = cf<T>(s->E[s->t1]);
     const F3<T> e2 = mf<T>(s->E[s->t2]);
@@ -194,10 +226,22 @@ template <typename T, bool CORR> void launch_abm(ems_fem1 *s)
     const CF3<T> e3 = cf<T>(s->E3), h3 = cf<T>(s->H3), e1 = cf<T>(s->E[s->t1]), h1 = cf<T>(s->H[s->t1]);
     const F3<T> eo = CORR ? mf<T>(s->E[s->t2]) : mf<T>(s->Ep);
     const F3<T> ho = CORR ? mf<T>(s->H[s->t2]) : mf<T>(s->Hp);
-    const T *K = s->dK.as<T>(), *Kt = s->dKt.as<T>();
-    k_fem1_abm<T, 0, CORR><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
-    k_fem1_abm<T, 1, CORR><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
-    k_fem1_abm<T, 2, CORR><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
+    const T *K = s->dK.as<T>(), *Kt = s->dKt.as<T>(), *cK = s->dKs.as<T>(), *cKt = s->dKts.as<T>();
+    if (stencil(s)) {
+        k_fem1_abm<T, 0, CORR, true><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
+                                                                             eo, ho);
+        k_fem1_abm<T, 1, CORR, true><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
+                                                                             eo, ho);
+        k_fem1_abm<T, 2, CORR, true><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
+                                                                             eo, ho);
+        return;
+    }
+    k_fem1_abm<T, 0, CORR, false><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
+                                                                          ho);
+    k_fem1_abm<T, 1, CORR, false><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
+                                                                          ho);
+    k_fem1_abm<T, 2, CORR, false><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
+                                                                          ho);
 }
 
 template <typename T> void launch_post(ems_fem1 *s, F3<T> e, const double *val, int skip_pec, int row)
@@ -409,6 +453,18 @@ template <typename T> ems_status upload_k(ems_fem1 *s)
     EMS_TRY(s->dKt.reserve(sizeof(T) * 144));
     EMS_HIP(hipMemcpy(s->dK.p, k.data(), sizeof(T) * 144, hipMemcpyHostToDevice));
     EMS_HIP(hipMemcpy(s->dKt.p, kt.data(), sizeof(T) * 144, hipMemcpyHostToDevice));
+    // merged stencils (src/fem numbering: y edges 4 + 2a + c), from FP64
+    double ktd[144], c[3 * kZ1Terms];
+    for (int r = 0; r < 12; ++r)
+        for (int q = 0; q < 12; ++q) ktd[r * 12 + q] = s->K[q * 12 + r];
+    DevBuf *dst[2] = {&s->dKs, &s->dKts};
+    const double *src[2] = {s->K, ktd};
+    for (int w = 0; w < 2; ++w) {
+        if (!z1_stencil_coefs(src[w], true, c)) return not_supported("first-order stencil term missing");
+        std::vector<T> ct(c, c + 3 * kZ1Terms);
+        EMS_TRY(dst[w]->reserve(sizeof(T) * ct.size()));
+        EMS_HIP(hipMemcpy(dst[w]->p, ct.data(), sizeof(T) * ct.size(), hipMemcpyHostToDevice));
+    }
     return EMS_OK;
 }
 
@@ -660,6 +716,16 @@ ems_status ems_fem1_save_results(ems_fem1 *s, int num_steps, const char *path)
             file << (size_t)j * s->d.tf * s->d.dt << "," << v[0] << "," << v[1] << "," << v[2] << std::endl;
         }
     }
+    return EMS_OK;
+}
+
+ems_status ems_fem1_set_element_product(ems_fem1 *s, int kind)
+{
+    clear_error();
+    if (!s) return invalid("null handle");
+    if (kind != EMS_FEM_ELEM_AUTO && kind != EMS_FEM_ELEM_ROWS && kind != EMS_FEM_ELEM_STENCIL)
+        return invalid("element product must be AUTO, ROWS or STENCIL");
+    s->elem_kind = kind;
     return EMS_OK;
 }
 

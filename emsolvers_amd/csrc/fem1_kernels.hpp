@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "hex_stencil.hpp"
 #include "lane_addr.hpp"
 
 namespace ems {
@@ -178,29 +179,57 @@ __global__ __launch_bounds__(256) void k_fem1_scratch(Hex1 g, CF3<T> e0, CF3<T> 
     }
 }
 
+// Merged stencil (hex_stencil.hpp) of edge (COMP, X, Y, Z) read straight
+// from the arrays: one partial sum per array, coefficients cf[33] at uniform
+// addresses.  Valid where both transverse coordinates are strictly inside.
+template <typename T, int COMP>
+__device__ __forceinline__ T stencil_at(const Hex1 &g, const T *__restrict__ cf, CF3<T> x, int X, int Y, int Z)
+{
+    T s[3] = {0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < kZ1Terms; ++t) {
+        const Z1Term m = z1_term(COMP, t);
+        const long dx = g.Nx + (m.q != 0), dy = g.Ny + (m.q != 1);
+        s[m.q] += cf[t] * x.c[m.q][(X + m.d[0]) + dx * ((Y + m.d[1]) + dy * (long)(Z + m.d[2]))];
+    }
+    return (s[0] + s[1]) + s[2];
+}
+
+template <int COMP> __device__ __forceinline__ bool interior1(const Hex1 &g, int X, int Y, int Z)
+{
+    if (COMP == 0) return Y > 0 && Y < g.Ny && Z > 0 && Z < g.Nz;
+    if (COMP == 1) return X > 0 && X < g.Nx && Z > 0 && Z < g.Nz;
+    return X > 0 && X < g.Nx && Y > 0 && Y < g.Ny;
+}
+
 // src/fem_abm element passes (fem_simulation.cpp:319-369, :521-558, :619-710)
 // followed by the "+= slot 1" pass:
 //   Eo = (0 + sum_e K_e^T E3) + E1,   Ho = (0 + sum_e K_e H3) + H1
 // Kt is K transposed (row-major), so both sides read rows.  PEC: when set,
 // PEC-tangential E edges are written as 0 (the corrector; the predictor
 // leaves them, as the reference applies the boundary condition only at the
-// end of the step).
-template <typename T, int COMP, bool PEC>
+// end of the step).  STN: interior edges take the merged stencils of K (cK)
+// and Kt (cKt) instead of the element row dots; edges on the faces keep them.
+template <typename T, int COMP, bool PEC, bool STN>
 __global__ __launch_bounds__(256) void k_fem1_abm(Hex1 g, const T *__restrict__ K, const T *__restrict__ Kt,
-                                                  CF3<T> e3, CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo, F3<T> ho)
+                                                  const T *__restrict__ cK, const T *__restrict__ cKt, CF3<T> e3,
+                                                  CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo, F3<T> ho)
 {
     int X, Y, Z;
     long idx;
     if (!coords<COMP>(g, X, Y, Z, idx)) return;
+    const bool stn = STN && interior1<COMP>(g, X, Y, Z);
     T ah = 0;
-    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ah += elem_dot<T>(g, K, r, h3, i, j, k); });
+    if (stn) ah = stencil_at<T, COMP>(g, cK + COMP * kZ1Terms, h3, X, Y, Z);
+    else for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ah += elem_dot<T>(g, K, r, h3, i, j, k); });
     ho.c[COMP][idx] = ah + h1.c[COMP][idx];
     if (PEC && pec1(g, COMP, X, Y, Z)) {
         eo.c[COMP][idx] = (T)0;
         return;
     }
     T ae = 0;
-    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ae += elem_dot<T>(g, Kt, r, e3, i, j, k); });
+    if (stn) ae = stencil_at<T, COMP>(g, cKt + COMP * kZ1Terms, e3, X, Y, Z);
+    else for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ae += elem_dot<T>(g, Kt, r, e3, i, j, k); });
     eo.c[COMP][idx] = ae + e1.c[COMP][idx];
 }
 

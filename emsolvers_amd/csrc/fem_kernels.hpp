@@ -20,6 +20,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "hex_stencil.hpp"
 #include "mfma.hpp"
 
 namespace ems {
@@ -615,39 +616,10 @@ __device__ __forceinline__ T z1_row_dot(const T *sK, const T *win, int r, int i,
     return temp;
 }
 
-// Merged first-order stencil.  Off the PEC faces a DOF lies in exactly 4
-// elements, and the sum of their 4 row dots is a fixed 33-point stencil per
-// component C over the window: 9 entries of array C in the plane across the
-// edge and 12 of each other array.  Term t of component C (a1 < a2 the axes
-// transverse to C, which are also the other components):
-//   t <  9: array C at offsets t % 3 - 1 along a1, t / 3 - 1 along a2;
-//   t >= 9: array q = a1 (t < 21) or a2, u = (t - 9) % 12: u & 1 along C,
-//           (u >> 1 & 1) - 1 along q, (u >> 2) - 1 along the remaining axis.
-// The host sums the scaled element rows into these coefficients once.
-struct Z1Term {
-    int q, d[3];
-};
-
-__host__ __device__ constexpr Z1Term z1_term(int C, int t)
-{
-    const int a1 = C == 0 ? 1 : 0, a2 = C == 2 ? 1 : 2;
-    Z1Term r{C, {0, 0, 0}};
-    if (t < 9) {
-        r.d[a1] = t % 3 - 1;
-        r.d[a2] = t / 3 - 1;
-        return r;
-    }
-    const int q = t < 21 ? a1 : a2, third = q == a1 ? a2 : a1;
-    const int u = (t - 9) % 12;
-    r.q = q;
-    r.d[C] = u & 1;
-    r.d[q] = ((u >> 1) & 1) - 1;
-    r.d[third] = (u >> 2) - 1;
-    return r;
-}
-
-constexpr int kZ1Terms = 33;
-
+// Merged first-order stencil (hex_stencil.hpp): off the PEC faces every DOF
+// lies in exactly 4 elements, and the sum of their row dots is one 33-point
+// stencil per component; the host sums the scaled element rows into its
+// coefficients once.
 // Stencil sum of component C at the window position lo of plane Z: one
 // partial sum per array, coefficients at uniform addresses (scalar loads).
 template <typename T, int BY, int C>
@@ -825,7 +797,7 @@ __global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restric
     const int Z0 = blockIdx.z * lz, Z1 = min(Z0 + lz, g.Nz);
     const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
     const bool act = X < g.Nx && Y < g.Ny;
-    const T sv = (T)src_val[0];
+    const T sv = nsrc > 0 ? (T)src_val[0] : (T)0;
     // warm-up: planes Z0-1 .. Z0+1 (symplectic stages stage the drifted
     // value; a separate drift pass measured slower at first order)
     for (int P = Z0 - 1; P <= Z0 + 1; ++P) {

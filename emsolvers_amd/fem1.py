@@ -110,6 +110,13 @@ class _Fem1:
         _lib.check(_lib.load().ems_fem1_matrix(self._h, _lib.dptr(out)))
         return out.reshape(12, 12)
 
+    def set_element_product(self, kind):
+        """'stencil' (the default, also 'auto': one 33-point stencil per
+        component on interior edges; equal to rounding) or 'rows' (the
+        element row dots in the reference's order, bit for bit)."""
+        k = {"auto": 0, "rows": 1, "stencil": 3}[kind] if isinstance(kind, str) else int(kind)
+        _lib.check(_lib.load().ems_fem1_set_element_product(self._h, k))
+
     def ndof(self):
         n = ctypes.c_size_t()
         _lib.check(_lib.load().ems_fem1_dof_count(self._h, ctypes.byref(n)))

@@ -462,6 +462,16 @@ ems_status ems_fem1_save_results(ems_fem1 *s, int num_steps, const char *path);
 /* Total edge DOFs per field set. */
 ems_status ems_fem1_dof_count(const ems_fem1 *s, size_t *n);
 
+/* Element operator application (no reference counterpart): STENCIL (and AUTO,
+ * the default) sums each edge's 4 element rows into one 33-point stencil per
+ * component, formed once from the FP64 element matrix (and its transpose for
+ * the fem_abm E update), on edges whose transverse coordinates are inside
+ * the grid; edges on the faces keep the element row dots.  The same results
+ * as the reference to rounding.  ROWS: the element row dots in the
+ * reference's order everywhere (bit for bit).  EMS_FEM_ELEM_MFMA is
+ * EMS_ERR_INVALID_ARG. */
+ems_status ems_fem1_set_element_product(ems_fem1 *s, int kind);
+
 /* --------------------------------- tetrahedral Nedelec (Whitney) elements
  * No reference counterpart: the unstructured-mesh extension of the edge-FEM
  * time steppers (src/symplectic_2, src/symplectic_4, src/fem_abm) on the

@@ -39,9 +39,17 @@ def read_csv(path):
     return np.array([[float(x) for x in l.split(",")] for l in lines[1:]])
 
 
-def close6(a, b):
-    """Equal at 6 significant digits (the default stream precision)."""
-    return np.all(np.abs(a - b) <= 5.01e-6 * np.abs(b) + 1e-300)
+def close6(a, b, floor=1e-300):
+    """Equal at 6 significant digits (the default stream precision), entries
+    below `floor` compared absolutely."""
+    return np.all(np.abs(a - b) <= 5.01e-6 * np.abs(b) + floor)
+
+
+def noise_floor(tr):
+    """The first-order FEM's default merged stencil sums in another order than
+    the reference: entries that vanish in exact arithmetic come out as
+    rounding noise of either sign (~1e-19 of a 1e9 trace)."""
+    return 1e-12 * np.abs(tr).max()
 
 
 def test_fdtd_driver_matches_reference_golden(tmp_path, golden):
@@ -88,7 +96,7 @@ def test_fem_driver_matches_oracle(tmp_path, n, stab, use_ofem, base, name):
     v = read_csv(tmp_path / name)
     assert v.shape == (tr.shape[0], 4)
     assert np.abs(tr).max() > 0
-    assert close6(v[:, 1:], tr)
+    assert close6(v[:, 1:], tr, noise_floor(tr))
     assert close6(v[:, 0], np.arange(tr.shape[0]) * n * dt)
 
 
@@ -102,7 +110,7 @@ def test_fem_abm_driver_matches_oracle(tmp_path):
     v = read_csv(tmp_path / "results_0_0.000000_0.250000_62.csv")
     assert v.shape == (63, 4)
     assert np.abs(tr).max() > 0
-    assert close6(v[:, 1:], tr)
+    assert close6(v[:, 1:], tr, noise_floor(tr))
 
 
 HEX = [

@@ -1,11 +1,12 @@
 """GPU parity of the first-order stabilized hex solvers (src/fem leapfrog,
 src/fem_abm Adams-Bashforth-Moulton) against the FP64 oracle.
 
-Bar: FP64 bit-exact (one thread per edge gathers its elements in the serial
-reference's order, each element row summed from 0 as the reference does; the
-element matrix is the same bits on both sides, pinned separately against its
-exact rational value); FP32 receiver traces within 1e-5 relative L2 of the
-FP64 oracle.
+Bar: FP64 bit-exact with the element row dots ('rows': one thread per edge
+gathers its elements in the serial reference's order, each element row
+summed from 0 as the reference does; the element matrix is the same bits on
+both sides, pinned separately against its exact rational value); 1e-10
+relative L2 with the merged stencil (the default); FP32 receiver traces
+within 1e-5 relative L2 of the FP64 oracle.
 """
 import os
 
@@ -39,7 +40,7 @@ OBS = [
 ]
 
 
-def build(kind, case, precision="f64"):
+def build(kind, case, precision="f64", elem="rows"):
     st, g = case["stepper"], case["grid"]
     if kind == "oracle":
         from oracle.fem1 import OracleFem1
@@ -52,6 +53,7 @@ def build(kind, case, precision="f64"):
         else:
             s = AbmSimulation(g, case["h"], case["dt"], EPS0, MU0, case["tf"], precision=precision,
                               stabilization_factor=case["stab"])
+        s.set_element_product(elem)
     for comp, p in case["pts"]:
         [s.setSource_x, s.setSource_y, s.setSource_z][comp](p)
     for d in case["dip"]:
@@ -95,6 +97,26 @@ def test_fp64_bitwise_with_oracle(oracle_runs, name):
         for slot in (0, 1):
             assert np.array_equal(g.field(f, slot), o.field(f, slot)), (f, slot)
     assert g.current_time == o.time
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_stencil_against_oracle(oracle_runs, name):
+    """The merged stencil (the library default): FP64 traces and final fields
+    within 1e-10 relative L2 of the oracle, FP32 traces within 1e-5."""
+    case = CASES[name]
+    print("stencil case", name, flush=True)
+    o = oracle_runs(name)
+    g = build("gpu", case, "f64", "stencil")
+    g.run(case["steps"])
+    err = rel_l2(g.traces(), o.traces())
+    ferr = max(rel_l2(g.field(f, slot), o.field(f, slot)) for f in range(nfields(case)) for slot in (0, 1))
+    f32 = build("gpu", case, "f32", "auto")
+    f32.run(case["steps"])
+    e32 = rel_l2(f32.traces(), o.traces())
+    print("stencil %s: fp64 traces %.2e fields %.2e; fp32 %.2e" % (name, err, ferr, e32), flush=True)
+    assert np.abs(o.traces()).max() > 0
+    assert err <= 1e-10 and ferr <= 1e-10
+    assert e32 <= 1e-5
 
 
 @pytest.mark.parametrize("name", ["fem-ofem0", "abm"])
@@ -169,5 +191,7 @@ def test_invalid_arguments():
         s.setObservationPoint([[], [], [(0, 0, 4)]])  # z-edges have k < nz
     with pytest.raises(EmsError):
         s.field(3)  # no H in src/fem
+    with pytest.raises(EmsError):
+        s.set_element_product(2)  # no matrix-core path at first order
     a = AbmSimulation((4, 4, 4), 0.02, 1e-11, EPS0, MU0, 1)
     assert a.field(3).shape == (4 * 5 * 5,)
