@@ -180,7 +180,7 @@ bool stencil(const ems_fem1 *s) { return s->elem_kind != EMS_FEM_ELEM_ROWS; }
 // fem_v2 one on the same layouts: with the merged stencil it runs on that
 // family's z-marching kernel (no sources there: k_fem1_post adds them), 64 x 8
 // points per workgroup, 16 planes (FP32 8), as measured fastest at 128^3.
-template <typename T> void launch_leap_stencil(ems_fem1 *s)
+fem::FemGrid z_grid(const ems_fem1 *s)
 {
     fem::FemGrid g{};
     g.p = 1;
@@ -192,6 +192,24 @@ template <typename T> void launch_leap_stencil(ems_fem1 *s)
     g.NZ = s->g.n[2];
     g.D = 4;
     g.n = 12;
+    return g;
+}
+
+// Largest per-component count of face edges (k_fem1_abm_faces).
+long face_edges(const Hex1 &g)
+{
+    const long N[3] = {g.Nx, g.Ny, g.Nz};
+    long m = 0;
+    for (int c = 0; c < 3; ++c) {
+        const int a1 = c == 0 ? 1 : 0, a2 = c == 2 ? 1 : 2;
+        m = std::max(m, N[c] * (2 * (N[a1] + 1) + 2 * (N[a2] - 1)));
+    }
+    return m;
+}
+
+template <typename T> void launch_leap_stencil(ems_fem1 *s)
+{
+    const fem::FemGrid g = z_grid(s);
     const int lz = sizeof(T) == 4 ? 8 : 16;
     auto c3 = [](DevBuf *b) { return fem::CComp3<T>{b[0].as<T>(), b[1].as<T>(), b[2].as<T>()}; };
     const fem::CComp3<T> e0 = c3(s->E[s->t0]), e1 = c3(s->E[s->t1]);
@@ -228,20 +246,23 @@ template <typename T, bool CORR> void launch_abm(ems_fem1 *s)
     const F3<T> ho = CORR ? mf<T>(s->H[s->t2]) : mf<T>(s->Hp);
     const T *K = s->dK.as<T>(), *Kt = s->dKt.as<T>(), *cK = s->dKs.as<T>(), *cKt = s->dKts.as<T>();
     if (stencil(s)) {
-        k_fem1_abm<T, 0, CORR, true><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
-                                                                             eo, ho);
-        k_fem1_abm<T, 1, CORR, true><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
-                                                                             eo, ho);
-        k_fem1_abm<T, 2, CORR, true><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1,
-                                                                             eo, ho);
+        // interior edges: the z-marching stencil pass per side; face edges:
+        // the element row dots
+        const fem::FemGrid fg = z_grid(s);
+        const int lz = sizeof(T) == 4 ? 8 : 16;
+        const dim3 grd((fg.Nx + 63) / 64, (fg.Ny + 7) / 8, (fg.Nz + lz - 1) / lz);
+        auto c3 = [](const CF3<T> &f) { return fem::CComp3<T>{f.c[0], f.c[1], f.c[2]}; };
+        auto m3 = [](const F3<T> &f) { return fem::Comp3<T>{f.c[0], f.c[1], f.c[2]}; };
+        fem::k_fem1z_sum<T, 8><<<grd, dim3(64, 8, 1), 0, s->stream>>>(fg, cK, c3(h3), c3(h1), m3(ho), lz);
+        fem::k_fem1z_sum<T, 8><<<grd, dim3(64, 8, 1), 0, s->stream>>>(fg, cKt, c3(e3), c3(e1), m3(eo), lz);
+        const long nf = face_edges(s->g);
+        k_fem1_abm_faces<T, CORR><<<dim3((unsigned)((nf + 255) / 256), 3, 1), 256, 0, s->stream>>>(
+            s->g, K, Kt, e3, h3, e1, h1, eo, ho);
         return;
     }
-    k_fem1_abm<T, 0, CORR, false><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
-                                                                          ho);
-    k_fem1_abm<T, 1, CORR, false><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
-                                                                          ho);
-    k_fem1_abm<T, 2, CORR, false><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, cK, cKt, e3, h3, e1, h1, eo,
-                                                                          ho);
+    k_fem1_abm<T, 0, CORR><<<grid_of<0>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
+    k_fem1_abm<T, 1, CORR><<<grid_of<1>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
+    k_fem1_abm<T, 2, CORR><<<grid_of<2>(s->g), blk, 0, s->stream>>>(s->g, K, Kt, e3, h3, e1, h1, eo, ho);
 }
 
 template <typename T> void launch_post(ems_fem1 *s, F3<T> e, const double *val, int skip_pec, int row)

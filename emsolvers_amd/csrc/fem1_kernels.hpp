@@ -36,11 +36,12 @@ template <typename T> struct CF3 {
 };
 
 // Row r of the 12 x 12 matrix M dotted with the 12 edge values of element
-// (i, j, k), m = 0..11 in order, summed from 0.  The callers' j and k are
-// wave-uniform (only i varies over the lanes): in FP32 the addresses are
-// taken as uniform bases plus i through lane_at (11 % faster at 128^3); in FP64 that form
-// measured 35 % slower and the plain 64-bit addresses are kept.
-template <typename T>
+// (i, j, k), m = 0..11 in order, summed from 0.  UNI: the callers' j and k
+// are wave-uniform (only i varies over the lanes), and in FP32 the addresses
+// are taken as uniform bases plus i through lane_at (11 % faster at 128^3);
+// in FP64 that form measured 35 % slower and the plain 64-bit addresses are
+// kept.  Callers whose j or k vary over a wave pass UNI = false.
+template <typename T, bool UNI = true>
 __device__ __forceinline__ T elem_dot(const Hex1 &g, const T *__restrict__ M, int r, CF3<T> x, int i, int j, int k)
 {
     const long Nx = g.Nx, Ny = g.Ny;
@@ -51,7 +52,7 @@ __device__ __forceinline__ T elem_dot(const Hex1 &g, const T *__restrict__ M, in
     const T *__restrict__ Y = x.c[1];
     const T *__restrict__ Z = x.c[2];
     T t = 0;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && UNI) {
         const long bx = Nx * (j + (Ny + 1) * (long)k);
         const long by = (Nx + 1) * (j + Ny * (long)k);
         const long bz = (Nx + 1) * (j + (Ny + 1) * (long)k);
@@ -179,58 +180,100 @@ __global__ __launch_bounds__(256) void k_fem1_scratch(Hex1 g, CF3<T> e0, CF3<T> 
     }
 }
 
-// Merged stencil (hex_stencil.hpp) of edge (COMP, X, Y, Z) read straight
-// from the arrays: one partial sum per array, coefficients cf[33] at uniform
-// addresses.  Valid where both transverse coordinates are strictly inside.
-template <typename T, int COMP>
-__device__ __forceinline__ T stencil_at(const Hex1 &g, const T *__restrict__ cf, CF3<T> x, int X, int Y, int Z)
-{
-    T s[3] = {0, 0, 0};
-#pragma unroll
-    for (int t = 0; t < kZ1Terms; ++t) {
-        const Z1Term m = z1_term(COMP, t);
-        const long dx = g.Nx + (m.q != 0), dy = g.Ny + (m.q != 1);
-        s[m.q] += cf[t] * x.c[m.q][(X + m.d[0]) + dx * ((Y + m.d[1]) + dy * (long)(Z + m.d[2]))];
-    }
-    return (s[0] + s[1]) + s[2];
-}
-
-template <int COMP> __device__ __forceinline__ bool interior1(const Hex1 &g, int X, int Y, int Z)
-{
-    if (COMP == 0) return Y > 0 && Y < g.Ny && Z > 0 && Z < g.Nz;
-    if (COMP == 1) return X > 0 && X < g.Nx && Z > 0 && Z < g.Nz;
-    return X > 0 && X < g.Nx && Y > 0 && Y < g.Ny;
-}
-
 // src/fem_abm element passes (fem_simulation.cpp:319-369, :521-558, :619-710)
 // followed by the "+= slot 1" pass:
 //   Eo = (0 + sum_e K_e^T E3) + E1,   Ho = (0 + sum_e K_e H3) + H1
 // Kt is K transposed (row-major), so both sides read rows.  PEC: when set,
 // PEC-tangential E edges are written as 0 (the corrector; the predictor
 // leaves them, as the reference applies the boundary condition only at the
-// end of the step).  STN: interior edges take the merged stencils of K (cK)
-// and Kt (cKt) instead of the element row dots; edges on the faces keep them.
-template <typename T, int COMP, bool PEC, bool STN>
-__global__ __launch_bounds__(256) void k_fem1_abm(Hex1 g, const T *__restrict__ K, const T *__restrict__ Kt,
-                                                  const T *__restrict__ cK, const T *__restrict__ cKt, CF3<T> e3,
-                                                  CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo, F3<T> ho)
+// end of the step).
+template <typename T, int COMP, bool PEC, bool UNI>
+__device__ __forceinline__ void abm_edge(const Hex1 &g, const T *__restrict__ K, const T *__restrict__ Kt,
+                                         CF3<T> e3, CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo, F3<T> ho, int X,
+                                         int Y, int Z, long idx)
 {
-    int X, Y, Z;
-    long idx;
-    if (!coords<COMP>(g, X, Y, Z, idx)) return;
-    const bool stn = STN && interior1<COMP>(g, X, Y, Z);
     T ah = 0;
-    if (stn) ah = stencil_at<T, COMP>(g, cK + COMP * kZ1Terms, h3, X, Y, Z);
-    else for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ah += elem_dot<T>(g, K, r, h3, i, j, k); });
+    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ah += elem_dot<T, UNI>(g, K, r, h3, i, j, k); });
     ho.c[COMP][idx] = ah + h1.c[COMP][idx];
     if (PEC && pec1(g, COMP, X, Y, Z)) {
         eo.c[COMP][idx] = (T)0;
         return;
     }
     T ae = 0;
-    if (stn) ae = stencil_at<T, COMP>(g, cKt + COMP * kZ1Terms, e3, X, Y, Z);
-    else for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ae += elem_dot<T>(g, Kt, r, e3, i, j, k); });
+    for_cands<COMP>(g, X, Y, Z, [&](int i, int j, int k, int r) { ae += elem_dot<T, UNI>(g, Kt, r, e3, i, j, k); });
     eo.c[COMP][idx] = ae + e1.c[COMP][idx];
+}
+
+template <typename T, int COMP, bool PEC>
+__global__ __launch_bounds__(256) void k_fem1_abm(Hex1 g, const T *__restrict__ K, const T *__restrict__ Kt,
+                                                  CF3<T> e3, CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo, F3<T> ho)
+{
+    int X, Y, Z;
+    long idx;
+    if (!coords<COMP>(g, X, Y, Z, idx)) return;
+    abm_edge<T, COMP, PEC, true>(g, K, Kt, e3, h3, e1, h1, eo, ho, X, Y, Z, idx);
+}
+
+// Edges of component COMP on the grid's faces, i.e. with a transverse
+// coordinate at 0 or N (the merged stencil covers the others): entry i of
+// Nf x P, Nf = the extent along COMP (fastest), P = 2 (N_a1 + 1) + 2 (N_a2 - 1)
+// the points on the border of the transverse [0, N_a1] x [0, N_a2]
+// rectangle: the rows a2 = 0 and a2 = N_a2, then the columns a1 = 0 and
+// a1 = N_a1 between them.
+template <int COMP> __device__ __forceinline__ int face_count(const Hex1 &g, int &nf)
+{
+    const int N[3] = {g.Nx, g.Ny, g.Nz};
+    const int a1 = COMP == 0 ? 1 : 0, a2 = COMP == 2 ? 1 : 2;
+    nf = N[COMP];
+    return 2 * (N[a1] + 1) + 2 * (N[a2] - 1);
+}
+
+template <int COMP>
+__device__ __forceinline__ bool face_edge(const Hex1 &g, long i, int &X, int &Y, int &Z, long &idx)
+{
+    const int N[3] = {g.Nx, g.Ny, g.Nz};
+    const int a1 = COMP == 0 ? 1 : 0, a2 = COMP == 2 ? 1 : 2;
+    int nf;
+    const int np = face_count<COMP>(g, nf);
+    if (i >= (long)nf * np) return false;
+    const int p = (int)(i / nf);
+    int c[3];
+    c[COMP] = (int)(i - (long)p * nf);
+    const int n1 = N[a1] + 1, n2 = N[a2] - 1;
+    if (p < 2 * n1) {
+        c[a1] = p < n1 ? p : p - n1;
+        c[a2] = p < n1 ? 0 : N[a2];
+    } else {
+        const int q = p - 2 * n1;
+        c[a2] = 1 + (q < n2 ? q : q - n2);
+        c[a1] = q < n2 ? 0 : N[a1];
+    }
+    X = c[0];
+    Y = c[1];
+    Z = c[2];
+    const int dx = g.Nx + (COMP != 0), dy = g.Ny + (COMP != 1);
+    idx = X + (long)dx * (Y + (long)dy * Z);
+    return true;
+}
+
+// The element passes above on the face edges only (blockIdx.y = component),
+// next to the z-marching stencil pass of the interior edges; j and k vary
+// over a wave here, so the FP32 row dots take plain addresses.
+template <typename T, bool PEC>
+__global__ __launch_bounds__(256) void k_fem1_abm_faces(Hex1 g, const T *__restrict__ K, const T *__restrict__ Kt,
+                                                        CF3<T> e3, CF3<T> h3, CF3<T> e1, CF3<T> h1, F3<T> eo,
+                                                        F3<T> ho)
+{
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int X, Y, Z;
+    long idx;
+    if (blockIdx.y == 0) {
+        if (face_edge<0>(g, i, X, Y, Z, idx)) abm_edge<T, 0, PEC, false>(g, K, Kt, e3, h3, e1, h1, eo, ho, X, Y, Z, idx);
+    } else if (blockIdx.y == 1) {
+        if (face_edge<1>(g, i, X, Y, Z, idx)) abm_edge<T, 1, PEC, false>(g, K, Kt, e3, h3, e1, h1, eo, ho, X, Y, Z, idx);
+    } else {
+        if (face_edge<2>(g, i, X, Y, Z, idx)) abm_edge<T, 2, PEC, false>(g, K, Kt, e3, h3, e1, h1, eo, ho, X, Y, Z, idx);
+    }
 }
 
 // One source update in the reference's order: field[comp][idx] (+|-)= value,

@@ -856,6 +856,54 @@ __global__ __launch_bounds__(64 * BY) void k_fem1z(FemGrid g, const T *__restric
     }
 }
 
+// Sum pass on the interior edges (both transverse coordinates strictly
+// inside): out = (merged stencil of x, coefficients stc[3][33]) + base, the
+// src/fem_abm element passes with their "+= slot 1"
+// (src/fem_abm/fem_simulation.cpp:319-369, :619-710).  The window and plane
+// march of k_fem1z over x; the face edges are another launch's.
+template <typename T, int BY>
+__global__ __launch_bounds__(64 * BY) void k_fem1z_sum(FemGrid g, const T *__restrict__ stc, CComp3<T> x,
+                                                       CComp3<T> base, Comp3<T> out, int lz)
+{
+    using W = Z1Win<BY>;
+    __shared__ T win[4 * W::SLOT];
+    const int X0 = blockIdx.x * 64, Y0 = blockIdx.y * BY;
+    const int ox = X0 - 1, oy = Y0 - 1;
+    const int Z0 = blockIdx.z * lz, Z1 = min(Z0 + lz, g.Nz);
+    const int X = X0 + threadIdx.x, Y = Y0 + threadIdx.y;
+    const bool act = X < g.Nx && Y < g.Ny;
+    for (int P = Z0 - 1; P <= Z0 + 1; ++P) {
+        Z1Regs<T, BY, false> r;
+        z1_fetch<T, BY, false>(g, x, x, ox, oy, P, r);
+        z1_store<T, BY, false>(win, P, r, (T)0, (T)1);
+    }
+    __syncthreads();
+    const long sx1 = g.Nx, sx2 = (long)g.Nx * (g.Ny + 1);
+    const long sy1 = g.Nx + 1, sy2 = (long)(g.Nx + 1) * g.Ny;
+    const long sz1 = g.Nx + 1, sz2 = (long)(g.Nx + 1) * (g.Ny + 1);
+    const int lo = ((int)threadIdx.x + 1) + W::EX * ((int)threadIdx.y + 1);
+    for (int Z = Z0; Z < Z1; ++Z) {
+        Z1Regs<T, BY, false> r;
+        z1_fetch<T, BY, false>(g, x, x, ox, oy, Z + 2, r);
+        if (act) {
+            if (Y > 0 && Z > 0) {
+                const long i = X + sx1 * Y + sx2 * Z;
+                out.x[i] = z1_stencil<T, BY, 0>(stc, win, Z, lo) + base.x[i];
+            }
+            if (X > 0 && Z > 0) {
+                const long i = X + sy1 * Y + sy2 * Z;
+                out.y[i] = z1_stencil<T, BY, 1>(stc + kZ1Terms, win, Z, lo) + base.y[i];
+            }
+            if (X > 0 && Y > 0) {
+                const long i = X + sz1 * Y + sz2 * Z;
+                out.z[i] = z1_stencil<T, BY, 2>(stc + 2 * kZ1Terms, win, Z, lo) + base.z[i];
+            }
+        }
+        z1_store<T, BY, false>(win, Z + 2, r, (T)0, (T)1);
+        __syncthreads();
+    }
+}
+
 // Receivers (src/fem_v2/fem_simulation.cpp:1275-1310): the four x edges and
 // four y edges around the point and one z edge, raw, into rec[obs][9].
 template <typename T>
