@@ -215,38 +215,39 @@ __global__ __launch_bounds__(256) void k_fem1_abm(Hex1 g, const T *__restrict__ 
 }
 
 // Edges of component COMP on the grid's faces, i.e. with a transverse
-// coordinate at 0 or N (the merged stencil covers the others): entry i of
-// Nf x P, Nf = the extent along COMP (fastest), P = 2 (N_a1 + 1) + 2 (N_a2 - 1)
-// the points on the border of the transverse [0, N_a1] x [0, N_a2]
-// rectangle: the rows a2 = 0 and a2 = N_a2, then the columns a1 = 0 and
-// a1 = N_a1 between them.
-template <int COMP> __device__ __forceinline__ int face_count(const Hex1 &g, int &nf)
-{
-    const int N[3] = {g.Nx, g.Ny, g.Nz};
-    const int a1 = COMP == 0 ? 1 : 0, a2 = COMP == 2 ? 1 : 2;
-    nf = N[COMP];
-    return 2 * (N[a1] + 1) + 2 * (N[a2] - 1);
-}
-
+// coordinate at 0 or N (the merged stencil covers the others), as two pairs
+// of sheets: A, a2 = 0 or N_a2 with a1 in [0, N_a1]; B, a1 = 0 or N_a1 with
+// a2 in [1, N_a2 - 1]; COMP runs over [0, N_COMP) on both.  Within a sheet
+// the lower-numbered free axis is the fastest, so that x runs along the
+// lanes wherever it is free.  Entry i of the N_COMP (2 (N_a1 + 1) +
+// 2 (N_a2 - 1)) face edges.
 template <int COMP>
 __device__ __forceinline__ bool face_edge(const Hex1 &g, long i, int &X, int &Y, int &Z, long &idx)
 {
     const int N[3] = {g.Nx, g.Ny, g.Nz};
-    const int a1 = COMP == 0 ? 1 : 0, a2 = COMP == 2 ? 1 : 2;
-    int nf;
-    const int np = face_count<COMP>(g, nf);
-    if (i >= (long)nf * np) return false;
-    const int p = (int)(i / nf);
+    constexpr int a1 = COMP == 0 ? 1 : 0, a2 = COMP == 2 ? 1 : 2;
+    const long nA = (long)N[COMP] * (N[a1] + 1), nB = (long)N[COMP] * (N[a2] - 1);
     int c[3];
-    c[COMP] = (int)(i - (long)p * nf);
-    const int n1 = N[a1] + 1, n2 = N[a2] - 1;
-    if (p < 2 * n1) {
-        c[a1] = p < n1 ? p : p - n1;
-        c[a2] = p < n1 ? 0 : N[a2];
+    if (i < 2 * nA) {
+        const int side = i >= nA;
+        const long j = i - side * nA;
+        constexpr int u = COMP < a1 ? COMP : a1, v = COMP < a1 ? a1 : COMP;
+        const int nu = u == COMP ? N[COMP] : N[a1] + 1;
+        c[u] = (int)(j % nu);
+        c[v] = (int)(j / nu);
+        c[a2] = side ? N[a2] : 0;
+    } else if (i < 2 * nA + 2 * nB) {
+        const long k = i - 2 * nA;
+        const int side = k >= nB;
+        const long j = k - side * nB;
+        constexpr int u = COMP < a2 ? COMP : a2, v = COMP < a2 ? a2 : COMP;
+        const int nu = u == COMP ? N[COMP] : N[a2] - 1;
+        c[u] = (int)(j % nu);
+        c[v] = (int)(j / nu);
+        c[a2] += 1;
+        c[a1] = side ? N[a1] : 0;
     } else {
-        const int q = p - 2 * n1;
-        c[a2] = 1 + (q < n2 ? q : q - n2);
-        c[a1] = q < n2 ? 0 : N[a1];
+        return false;
     }
     X = c[0];
     Y = c[1];
