@@ -29,6 +29,10 @@ public:
     void setSource_y(const std::array<int, 3> &p) { check(ems_fem1_add_source(h_, 1, p[0], p[1], p[2])); }
     void setSource_z(const std::array<int, 3> &p) { check(ems_fem1_add_source(h_, 2, p[0], p[1], p[2])); }
     void setSourceDipole_x(const std::array<int, 3> &p) { check(ems_fem1_add_dipole_x(h_, p[0], p[1], p[2])); }
+    // Not in the reference: EMS_FEM_ELEM_ROWS gives the reference's sums bit
+    // for bit (the default merged stencil sums in another order; see
+    // ems_fem1_set_element_product).
+    void setElementProduct(int kind) { check(ems_fem1_set_element_product(h_, kind)); }
     void setObservationPoint(const std::array<std::vector<std::array<int, 3>>, 3> &position)
     {
         const int counts[3] = {(int)position[0].size(), (int)position[1].size(), (int)position[2].size()};

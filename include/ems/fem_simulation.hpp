@@ -55,6 +55,10 @@ public:
     void setSource_y(const std::array<int, 3> &p) { check(ems_fem_add_source(h_, 1, p[0], p[1], p[2])); }
     void setSource_z(const std::array<int, 3> &p) { check(ems_fem_add_source(h_, 2, p[0], p[1], p[2])); }
     void setObservationPoint(const std::array<int, 3> &p) { check(ems_fem_add_observation(h_, p[0], p[1], p[2])); }
+    // Not in the reference: EMS_FEM_ELEM_ROWS gives the reference's sums bit
+    // for bit at every order (the defaults at orders 1 and 3 sum in another
+    // order; see ems_fem_set_element_product).
+    void setElementProduct(int kind) { check(ems_fem_set_element_product(h_, kind)); }
     void saveResults(int num_steps, const std::string &filename)
     {
         if (ems_fem_save_results(h_, num_steps, filename.c_str()) != EMS_OK)

@@ -166,6 +166,15 @@ double use_fdtd()
     J += b.gradient(10, {obs}, &mis, &ge, nullptr);
     return J + plan[0] + s.savedElectricField().size();
 }
+void use_fem()
+{
+    ems::fem_v2::FemSimulation v(1, {8, 8, 8}, 0.08, 1e-12, 8.85e-12, 1.26e-6, 1, 1);
+    v.setElementProduct(EMS_FEM_ELEM_ROWS);
+    ems::fem::FemSimulation f({8, 8, 8}, 0.08, 1e-12, 8.85e-12, 1.26e-6, 1, 0.3, 0);
+    f.setElementProduct(EMS_FEM_ELEM_STENCIL);
+    ems::fem_abm::FemSimulation a({8, 8, 8}, 0.08, 1e-12, 8.85e-12, 1.26e-6, 1);
+    a.setElementProduct(EMS_FEM_ELEM_AUTO);
+}
 double use_tet()
 {
     ems::tet::TetSimulation t({2, 2, 2}, 0.5, 1e-3, 1.0, 1.0, 1, EMS_TET_SYMPLECTIC4, 2, EMS_F64);
