@@ -42,9 +42,10 @@ struct ems_fem {
     DevBuf d_src, d_sval, d_obs, d_rec;
     // first order: z-marching all-component kernel with BY rows (4 or 8) and
     // lz planes per workgroup; 0 rows: one gather kernel per component;
-    // -1: as measured fastest at 128^3 on MI355X: merged stencil 8 x 16
-    // (FP32 leapfrog 8 x 8), element row dots leapfrog 8 x 8, symplectic
-    // stages 4 x 4
+    // 16 rows: the FP32 stencil only (8 otherwise); -1: as measured fastest
+    // at 128^3 on MI355X: merged stencil FP64 8 x 16, FP32 leapfrog 8 x 8,
+    // FP32 symplectic stages 16 x 8; element row dots leapfrog 8 x 8,
+    // symplectic stages 4 x 4
     int z1_by = -1, z1_lz = 8;
 };
 
@@ -93,12 +94,20 @@ void launch_z1(ems_fem *s, CComp3<T> u, CComp3<T> pm, CComp3<T> other, Comp3<T> 
 {
     const FemGrid &g = s->g;
     const bool stn = s->elem_kind == EMS_FEM_ELEM_STENCIL || s->elem_kind == EMS_FEM_ELEM_AUTO;
-    const int by = s->z1_by > 0 ? s->z1_by : (DRIFT && !stn) ? 4 : 8;
-    const int lz = s->z1_by > 0 ? s->z1_lz : !stn ? (DRIFT ? 4 : 8) : (!DRIFT && sizeof(T) == 4) ? 8 : 16;
+    int by = s->z1_by > 0 ? s->z1_by : !stn ? (DRIFT ? 4 : 8) : (DRIFT && sizeof(T) == 4) ? 16 : 8;
+    if (by == 16 && !(stn && sizeof(T) == 4)) by = 8;  // 16 rows: FP32 stencil only (1024-thread register budget)
+    const int lz = s->z1_by > 0 ? s->z1_lz : !stn ? (DRIFT ? 4 : 8) : sizeof(T) == 4 ? 8 : 16;
     const int ns = (int)s->src_z.size();
     const dim3 grd((g.Nx + 63) / 64, (g.Ny + by - 1) / by, (g.Nz + lz - 1) / lz);
     const T *K = s->K.as<T>(), *stc = s->Kst.as<T>();
     const long *si = s->d_src.as<long>();
+    if constexpr (sizeof(T) == 4) {
+        if (by == 16) {
+            k_fem1z<T, 16, DRIFT, true><<<grd, dim3(64, 16, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
+                                                                               deltat, hd, mm, ns, si, sval);
+            return;
+        }
+    }
     if (by == 4 && stn)
         k_fem1z<T, 4, DRIFT, true><<<grd, dim3(64, 4, 1), 0, s->stream>>>(g, K, stc, u, pm, other, o1, o2, lz,
                                                                          deltat, hd, mm, ns, si, sval);
@@ -680,7 +689,8 @@ ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes)
 {
     clear_error();
     if (!s) return invalid("null handle");
-    if (rows != -1 && rows != 0 && rows != 4 && rows != 8) return invalid("tile rows must be -1, 0, 4 or 8");
+    if (rows != -1 && rows != 0 && rows != 4 && rows != 8 && rows != 16)
+        return invalid("tile rows must be -1, 0, 4, 8 or 16");
     if (planes < 1) return invalid("tile planes must be >= 1");
     s->z1_by = rows;
     s->z1_lz = planes;

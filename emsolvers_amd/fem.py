@@ -127,9 +127,9 @@ class FemSimulation:
         self.set(2, slot, v[n0 + n1:])
 
     def set_tiling(self, rows, planes=8):
-        """First-order kernels: rows 4 or 8 = one z-marching pass over all
-        components (64 x rows workgroups, `planes` planes each), 0 = one
-        gather kernel per component.  Results do not depend on it, except
+        """First-order kernels: rows 4, 8 or 16 = one z-marching pass over all
+        components (64 x rows workgroups, `planes` planes each; 16 for the
+        FP32 stencil only, else 8), 0 = one gather kernel per component.  Results do not depend on it, except
         that rows 0 always takes the element row dots (not the stencil)."""
         _lib.check(_lib.load().ems_fem_set_tiling(self._h, int(rows), int(planes)))
 

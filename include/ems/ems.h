@@ -366,10 +366,11 @@ ems_status ems_fem_save_results(ems_fem *s, int num_steps, const char *filename)
 /* Total edge DOFs (NX + NY + NZ). */
 ems_status ems_fem_dof_count(const ems_fem *s, size_t *n);
 
-/* First-order kernels (no reference counterpart): rows = 4 or 8 selects the
- * z-marching kernel that updates all three components in one pass with a
- * 64 x rows workgroup and `planes` planes per workgroup; rows = 0 one gather
- * kernel per component; rows = -1 (default) the measured choice per stepper
+/* First-order kernels (no reference counterpart): rows = 4, 8 or 16 selects
+ * the z-marching kernel that updates all three components in one pass with a
+ * 64 x rows workgroup and `planes` planes per workgroup (16 rows for the FP32
+ * merged stencil only, 8 otherwise); rows = 0 one gather kernel per
+ * component; rows = -1 (default) the measured choice per stepper, precision
  * and element product.  Results do not depend on it, except that rows = 0
  * always takes the element row dots (see ems_fem_set_element_product). */
 ems_status ems_fem_set_tiling(ems_fem *s, int rows, int planes);

@@ -63,12 +63,13 @@ def oracle_cache():
     return get
 
 
-@pytest.mark.parametrize("rows,planes", [(0, 8), (4, 1), (4, 3), (8, 5), (8, 8)])
+@pytest.mark.parametrize("rows,planes", [(0, 8), (4, 1), (4, 3), (8, 5), (8, 8), (16, 3)])
 @pytest.mark.parametrize("stepper", [0, 1, 2])
 def test_first_order_tilings_bitwise(oracle_cache, stepper, rows, planes):
     """First order: the z-marching all-component kernel (64 x rows workgroups,
-    any number of planes per workgroup) and the per-component gather kernels
-    (rows = 0) give the oracle's bits."""
+    any number of planes per workgroup; 16 rows falls back to 8 for the row
+    dots) and the per-component gather kernels (rows = 0) give the oracle's
+    bits."""
     key = (1, stepper)
     case = CASES[key]
     print("tiling", key, rows, planes, flush=True)
@@ -149,7 +150,7 @@ def test_mfma_element_products_against_oracle(oracle_cache, key):
     assert e32 <= 1e-5 and e32r <= 1e-5
 
 
-@pytest.mark.parametrize("rows,planes", [(4, 3), (8, 5)])
+@pytest.mark.parametrize("rows,planes", [(4, 3), (8, 5), (16, 3)])
 @pytest.mark.parametrize("stepper", [0, 1, 2])
 def test_first_order_stencil_against_oracle(oracle_cache, stepper, rows, planes):
     """First order with each component's 4 element rows merged into one
@@ -175,7 +176,7 @@ def test_first_order_stencil_against_oracle(oracle_cache, stepper, rows, planes)
     assert np.abs(o.traces()[..., 1:]).max() > 0
     assert err <= 1e-10 and ferr <= 1e-10
     assert e32 <= 1e-5
-    if rows == 8:
+    if rows >= 8:
         wide = dict(grid=(70, 11, 9), steps=25, src=(64, 8, 4), obs=[(63, 8, 4), (65, 3, 4)], tf=1)
         ow = build("oracle", 1, stepper, wide)
         ow.run(wide["steps"])
@@ -186,6 +187,10 @@ def test_first_order_stencil_against_oracle(oracle_cache, stepper, rows, planes)
         assert rel_l2(gw.traces()[..., 1:], ow.traces()[..., 1:]) <= 1e-10
         for c in range(3):
             assert rel_l2(gw.get(c, 1), ow.get(c, 1)) <= 1e-10, c
+        fw = build("gpu", 1, stepper, wide, "f32")  # 16 rows run in FP32 (one partial row block here)
+        fw.set_tiling(rows, 4)
+        fw.run(wide["steps"])
+        assert rel_l2(fw.traces()[..., 1:], ow.traces()[..., 1:]) <= 1e-5
 
 
 def test_stencil_is_first_order_only():
