@@ -12,7 +12,10 @@
 // owns one edge and gathers the (at most 4) element contributions in the
 // order the reference's serial element loop (k, j, i) visits them, each row
 // dot product summed from 0 over m in order.  With contraction off the FP64
-// results equal a serial run of the reference bit for bit.
+// results equal a serial run of the reference bit for bit.  These gathers are
+// the EMS_FEM_ELEM_ROWS path; by default the interior edges take the merged
+// stencil of hex_stencil.hpp (fem1.hip: the fem leapfrog on k_fem1z, the
+// fem_abm passes on k_fem1z_sum), equal to the reference to rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
